@@ -1,0 +1,219 @@
+/*
+ * restirpt.h — C ABI of the MI355X (gfx950) ReSTIR PT core.
+ *
+ * This is the boundary the reference's C++ pass surface calls into. Each entry
+ * point names the reference interface it replaces (paths relative to the
+ * HummaWhite/Vulkan-ReSTIR-PT source tree). All device pointers are caller
+ * owned HIP device allocations; all calls are stream ordered on the given
+ * hipStream_t (0 = the null stream) and return an rt_status. On failure
+ * rt_last_error() returns a message for the calling thread.
+ *
+ * Data layouts follow the reference's std430 GLSL structs
+ * (src/shader/layouts.glsl:6-164) and G-buffer formats
+ * (src/GBufferPass.h:23-26):
+ *   depthNormal  RGBA32F  {euclidean depth, normal.xyz}, depth 0 = background
+ *   albedoMatId  RG32UI   {packUnorm4x8(albedo), matIndex << 16 | instance}
+ *   motionVector RG16F    {prevUV - curUV} as IEEE half
+ *   outputs      RGBA32F  (the reference's uDirectOutput / uIndirectOutput)
+ * Images are row-major, pixel (x, y) at index y * width + x, row 0 at the top.
+ * GRIS reservoir buffers are opaque device buffers of
+ * rt_gris_reservoir_bytes(width, height) bytes (planar, 80 B per pixel).
+ */
+#ifndef RESTIRPT_H
+#define RESTIRPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t rt_status;
+enum {
+    RT_OK = 0,
+    RT_ERR_INVALID_ARGUMENT = 1,
+    RT_ERR_HIP = 2,
+    RT_ERR_UNSUPPORTED = 3,
+    RT_ERR_CAPACITY = 4
+};
+
+/* ---- scene data model: src/shader/layouts.glsl:6-94, src/Scene.h:15-38 ---- */
+typedef struct rt_material {          /* layouts.glsl:6-25, 32 B */
+    float baseColor[3];
+    uint32_t type;                    /* 1 Lambert, 2 MetallicWorkflow, 3 Metal, 4 Dielectric, 6 Fake */
+    uint32_t textureIdx;              /* must be 0xffffffff: textures are not supported */
+    float metallic, roughness, ior;
+} rt_material;
+
+typedef struct rt_mesh_vertex {       /* layouts.glsl:53-58, 32 B */
+    float pos[3]; float uvx;
+    float norm[3]; float uvy;
+} rt_mesh_vertex;
+
+typedef struct rt_object_instance {   /* layouts.glsl:60-72, 224 B; matrices column-major */
+    float transform[16];
+    float transformInv[16];
+    float transformInvT[16];
+    float radiance[3]; float pad0;
+    uint32_t indexOffset, indexCount, matIndex; float pad2;
+} rt_object_instance;
+
+typedef struct rt_triangle_light {    /* layouts.glsl:74-83, 64 B */
+    float v0[3]; float nx;
+    float v1[3]; float ny;
+    float v2[3]; float nz;
+    float radiance[3]; float area;
+} rt_triangle_light;
+
+typedef struct rt_light_table_entry { /* layouts.glsl:85-88; entry 0 = {sumPower, numLights} */
+    float prob; uint32_t failId;
+} rt_light_table_entry;
+
+typedef struct rt_intersection {      /* layouts.glsl:90-94 */
+    float bary[2]; uint32_t instanceIdx, triangleIdx;
+} rt_intersection;
+
+typedef struct rt_camera {            /* layouts.glsl:27-51, 352 B (src/Camera.h:47-69) */
+    float view[16], proj[16], projView[16], lastProjView[16];
+    float pos[3]; float FOV;
+    float angle[3]; float nearZ;
+    float front[3]; float farZ;
+    float right[3]; float lensRadius;
+    float up[3]; float focalDist;
+    uint32_t filmSize[2]; uint32_t frameIndex; uint32_t seed;
+} rt_camera;
+
+typedef struct rt_gris_reservoir {    /* layouts.glsl:132-156, 96 B (host view only) */
+    rt_intersection rcIsec;
+    float rcLi[3]; uint32_t rcRng;
+    float rcWi[3]; uint32_t flags;
+    float pad[2]; float rcPrevSamplePdf; float rcJacobian;
+    float F[3]; uint32_t primaryRng;
+    float sampleCount, resampleWeight, contribWeight, pad0;
+} rt_gris_reservoir;
+
+/* Host arrays of a scene, as the reference binds them in descriptor set 1
+ * (layouts.glsl:171-178; filled by Scene::load, src/Scene.cpp:104-322). */
+typedef struct rt_scene_arrays {
+    const rt_material* materials;             uint32_t numMaterials;
+    const int32_t* materialIndices;           uint32_t numMaterialIndices;
+    const rt_mesh_vertex* vertices;           uint32_t numVertices;
+    const uint32_t* indices;                  uint32_t numIndices;
+    const rt_object_instance* instances;      uint32_t numInstances;
+    const rt_triangle_light* lights;          uint32_t numLights;
+    const rt_light_table_entry* lightTable;   uint32_t numLightTable;
+} rt_scene_arrays;
+
+/* GRISReSTIR::Settings (src/GRISReSTIR.h:11-17, push constant
+ * gris_reservoir.glsl:21-31) followed by the kernel constants the reference
+ * hard-codes. */
+typedef struct rt_gris_settings {
+    uint32_t shiftType;        /* 0 Reconnection, 1 Replay, 2 Hybrid */
+    float rrScale;
+    uint32_t temporalReuse;
+    uint32_t spatialReuse;
+    uint32_t cap;
+    uint32_t maxDepth;         /* MaxTracingDepth, gris_path_trace.glsl:46 (15) */
+    uint32_t spatialNeighbors; /* ResampleNum, gris_resample_spatial.glsl:65 (3) */
+    float spatialRadius;       /* ResampleRadius in pixels, gris_resample_spatial.glsl:66 (20) */
+} rt_gris_settings;
+
+/* Descriptor set 2 (RayImageDescSet) of the reference: layouts.glsl:180-195,
+ * bound per frame parity by Renderer::initDescriptor (src/Renderer.cpp:319-353). */
+typedef struct rt_frame {
+    uint32_t width, height;
+    float* directOutput;            /* binding 0, RGBA32F */
+    float* indirectOutput;          /* binding 1, RGBA32F */
+    const float* depthNormal;       /* binding 2 */
+    const float* depthNormalPrev;   /* binding 3 */
+    const uint32_t* albedoMatId;    /* binding 4 */
+    const uint32_t* albedoMatIdPrev;/* binding 5 */
+    const uint16_t* motionVector;   /* binding 6 */
+    void* grisReservoir;            /* binding 12 (this frame) */
+    void* grisReservoirPrev;        /* binding 13 */
+    void* grisReservoirTemp;        /* binding 14 */
+} rt_frame;
+
+typedef struct rt_scene_info {
+    uint32_t numPrimitives;    /* light triangles + instanced object triangles */
+    uint32_t numLights;
+    uint32_t bvhNodes;         /* internal nodes */
+    uint32_t bvhDepth;         /* max root-to-leaf internal-node count */
+    float sceneLo[3], sceneHi[3];
+    size_t deviceBytes;
+} rt_scene_info;
+
+typedef struct rt_device_scene rt_device_scene;
+
+const char* rt_last_error(void);
+
+/* Replaces DeviceScene::DeviceScene (src/Scene.cpp:324-330): uploads the
+ * buffers of createBufferAndImages (:368-446) and builds the acceleration
+ * structure of createAccelerationStructure (:448-547) as a GPU LBVH over all
+ * light and instanced object triangles. Synchronises the stream. */
+rt_status rt_device_scene_create(const rt_scene_arrays* arrays, hipStream_t stream, rt_device_scene** out);
+/* Replaces DeviceScene::destroy (src/Scene.cpp:332-349). */
+rt_status rt_device_scene_destroy(rt_device_scene* scene);
+rt_status rt_device_scene_info(const rt_device_scene* scene, rt_scene_info* out);
+/* Copies the LBVH to host memory (for parity checks): sorted Morton codes
+ * [N], leaf->primitive order [N], internal nodes [N-1] as 14 words each
+ * (left box min/max, right box min/max as float bits, left, right child;
+ * bit 31 marks a leaf). Any pointer may be null. */
+rt_status rt_bvh_export(const rt_device_scene* scene, uint64_t* codes, uint32_t* order, uint32_t* nodes14);
+
+/* Ray queries of src/shader/ray_query.glsl:6-70 (traceClosestHit /
+ * traceShadow) over device arrays: rays8 = {ox, oy, oz, dx, dy, dz, tmin, tmax}.
+ * mode 0 = closest hit (t, bary, instance custom index, primitive index;
+ * instance 0xffffffff on a miss), mode 1 = any hit (inst = 1 if occluded).
+ * skipLights excludes the light triangles (instance 0). */
+rt_status rt_trace_rays(const rt_device_scene* scene, const float* rays8, uint32_t n, int mode, int skipLights,
+                        float* t, float* bary2, uint32_t* inst, uint32_t* prim, hipStream_t stream);
+
+/* Replaces GBufferPass::render (src/GBufferPass.cpp:22-56) with a primary-ray
+ * kernel at pixel centres producing the same three images for the frame
+ * parity being rendered. cam->filmSize gives the image size. */
+rt_status rt_gbuffer_trace(const rt_device_scene* scene, const rt_camera* cam, float* depthNormal,
+                           uint32_t* albedoMatId, uint16_t* motionVector, hipStream_t stream);
+
+/* Plain path tracer with NEE+MIS: RayTracing::execute of di_naive.comp and
+ * gi_naive.comp (src/Renderer.cpp:453-469; src/shader/di_naive.glsl:79-167,
+ * gi_naive.glsl:28-149). which: bit 0 direct, bit 1 indirect. cams[0] is
+ * uCamera, cams[1] uPrevCamera. */
+rt_status rt_pt_plain(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
+                      uint32_t maxDepth, uint32_t which, hipStream_t stream);
+
+/* The three passes of GRISReSTIR::render (src/GRISReSTIR.cpp:9-53). */
+rt_status rt_gris_path_trace(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
+                             const rt_gris_settings* settings, hipStream_t stream);
+rt_status rt_gris_temporal(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
+                           const rt_gris_settings* settings, hipStream_t stream);
+rt_status rt_gris_spatial(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
+                          const rt_gris_settings* settings, hipStream_t stream);
+/* GRISReSTIR::render: path trace, temporal reuse, spatial reuse, in order. */
+rt_status rt_gris_render(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
+                         const rt_gris_settings* settings, hipStream_t stream);
+
+/* Reservoir buffers (the GRISReservoir storage buffers allocated by
+ * Renderer::createRayImage, src/Renderer.cpp:236-250). */
+size_t rt_gris_reservoir_bytes(uint32_t width, uint32_t height);
+/* Sets every reservoir to the reset state (GRISReservoirReset,
+ * gris_reservoir.glsl:75-79; other fields zero). */
+rt_status rt_gris_reservoir_reset(void* reservoirs, uint32_t width, uint32_t height, hipStream_t stream);
+/* Converts between the device layout and the reference's 96-B struct on the host. */
+rt_status rt_gris_reservoir_download(const void* reservoirs, uint32_t width, uint32_t height,
+                                     rt_gris_reservoir* host, hipStream_t stream);
+rt_status rt_gris_reservoir_upload(const rt_gris_reservoir* host, uint32_t width, uint32_t height,
+                                   void* reservoirs, hipStream_t stream);
+
+/* Traversal work counters (internal nodes visited, triangles tested, rays)
+ * accumulated by every kernel launched while enabled; off by default. The
+ * counting build of each kernel is only used while enabled. */
+rt_status rt_counters_enable(int enable);
+rt_status rt_counters_read(uint64_t out3[3], int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RESTIRPT_H */

@@ -1,0 +1,63 @@
+/*
+ * restirpt_host.h — C entry points of the C++ host layer (librestirpt_host.so).
+ *
+ * The host layer is the reference's C++ surface rebuilt over the C ABI of
+ * restirpt.h: Scene + light table (src/Scene.cpp:192-322,
+ * src/util/AliasTable.h), Camera (src/Camera.cpp), GBufferPass, RayTracing,
+ * GRISReSTIR and a headless frame driver standing in for Renderer
+ * (src/Renderer.cpp:193-257, 319-368, 400-503, 539-577). These functions expose
+ * it to callers that are not C++ (tests, benchmark drivers).
+ */
+#ifndef RESTIRPT_HOST_H
+#define RESTIRPT_HOST_H
+
+#include "restirpt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rt_host_scene rt_host_scene;
+typedef struct rt_renderer rt_renderer;
+
+/* Procedural scenes (the workloads this build is measured on):
+ *   0  closed box room of 12 quads + 64 icospheres (subdivision 2), Lambert,
+ *      one ceiling quad light of radiance 10 (~20.5k triangles)
+ *   1  512x512-cell fBm heightfield (6 octaves, 4 triangles per cell) + 200
+ *      instanced icospheres, MetallicWorkflow GGX, 16 emissive triangles of
+ *      log-uniform radiance in [1, 100] (~1.1M triangles)
+ *   2  a small test scene: one floor quad under one quad light
+ * Everything is drawn from PCG32 streams seeded by `seed`. */
+rt_status rt_scenegen(uint32_t config, uint32_t seed, rt_host_scene** out);
+rt_status rt_host_scene_destroy(rt_host_scene* scene);
+/* Borrowed views of the scene arrays (valid until the scene is destroyed). */
+rt_status rt_host_scene_arrays(const rt_host_scene* scene, rt_scene_arrays* out);
+/* Camera of frame `frame` of the scene's seeded camera path at width x height:
+ * out[0] = uCamera (frame's seed, lastProjView of the previous frame,
+ * frameIndex 0, clear flag on frame 0), out[1] = uPrevCamera. */
+rt_status rt_host_scene_camera(const rt_host_scene* scene, uint32_t frame, uint32_t width, uint32_t height,
+                               rt_camera out[2]);
+
+/* DiscreteSampler1D<float>::build (src/util/AliasTable.h:26-71): n + 1 entries. */
+rt_status rt_light_table_build(const float* distribution, uint32_t n, rt_light_table_entry* out);
+
+/* Headless frame driver: owns the frame images (two G-buffer parities,
+ * motion vectors, three reservoir buffers, direct/indirect outputs) and runs
+ * GBufferPass + GRISReSTIR (method 0) or GBufferPass + naive DI + naive GI
+ * (method 1) per frame, flipping the frame parity afterwards
+ * (Renderer::drawFrame, src/Renderer.cpp:539-569). */
+rt_status rt_renderer_create(rt_device_scene* scene, uint32_t width, uint32_t height, rt_renderer** out);
+rt_status rt_renderer_destroy(rt_renderer* r);
+rt_status rt_renderer_set_settings(rt_renderer* r, const rt_gris_settings* settings);
+rt_status rt_renderer_frame(rt_renderer* r, const rt_camera cams[2], uint32_t method, hipStream_t stream);
+/* Current frame's bindings (device pointers) for callers that launch passes themselves. */
+rt_status rt_renderer_bindings(const rt_renderer* r, rt_frame* out);
+/* Flips the frame parity (mCurFrame ^= 1). rt_renderer_frame does this itself. */
+rt_status rt_renderer_flip(rt_renderer* r);
+/* Copies an RGBA32F output (0 direct, 1 indirect) to host memory. */
+rt_status rt_renderer_read_output(const rt_renderer* r, uint32_t which, float* host, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RESTIRPT_HOST_H */

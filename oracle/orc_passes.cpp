@@ -1,0 +1,658 @@
+// ORACLE — test infrastructure only. Nothing in the product links or calls this.
+// Transliteration of the per-pixel GLSL passes; see orc_passes.h for sources.
+#include "orc_passes.h"
+
+namespace orc {
+
+// ---------------------------------------------------------------------------------
+// G-buffer (GBuffer.vert:19-29, GBuffer.frag:20-53) evaluated at the pixel centre.
+// Light triangles are not drawn by the reference's raster pass, so the primary
+// ray ignores them. Clip planes are applied as ray distances [near, far].
+void gbufferPixel(const Ctx& c, const Camera& cam, uint32_t x, uint32_t y, float* dn, uint32_t* am, uint16_t* mv) {
+    const uint32_t W = cam.filmSize[0], H = cam.filmSize[1];
+    vec2 uv = V2(((float)x + 0.5f) / (float)W, ((float)y + 0.5f) / (float)H);
+    Ray ray = pinholeCameraSampleRay(cam, V2(uv.x, 1.0f - uv.y), V2(0, 0));
+    Hit h = traceClosest(*c.B, makeRayIn(ray.ori, ray.dir, cam.nearZ, cam.farZ), true, c.st);
+    if (!h.hit) {
+        dn[0] = dn[1] = dn[2] = dn[3] = 0.0f; am[0] = am[1] = 0; mv[0] = mv[1] = 0;
+        return;
+    }
+    const SceneArrays& S = *c.S;
+    uint32_t meshIdx = h.isec.instanceIdx - 1;
+    const ObjectInstance& I = S.instances[meshIdx];
+    uint32_t tri = h.isec.triangleIdx;
+    float b1 = h.isec.bary[0], b2 = h.isec.bary[1], b0 = 1.0f - b1 - b2;
+    vec3 P[3], N[3];
+    for (int k = 0; k < 3; k++) {
+        const MeshVertex& v = S.vertices[S.indices[I.indexOffset + tri * 3 + k]];
+        P[k] = mat4MulPoint(I.transform, L3(v.pos), 1.0f);  // vsOut.pos
+        const float* m = I.transformInvT;                    // mat3(transformInvT) * aNorm
+        vec3 nn = V3(m[0], m[1], m[2]) * v.norm[0] + V3(m[4], m[5], m[6]) * v.norm[1] + V3(m[8], m[9], m[10]) * v.norm[2];
+        N[k] = normalize(nn);
+    }
+    vec3 pos = P[0] * b0 + P[1] * b1 + P[2] * b2;
+    vec3 nrm = normalize(N[0] * b0 + N[1] * b1 + N[2] * b2);
+    uint32_t matIndex = (uint32_t)S.materialIndices[I.indexOffset / 3 + tri];
+    vec3 albedo = L3(S.materials[matIndex].baseColor);
+    vec4 last = mat4MulVec4(cam.lastProjView, {pos.x, pos.y, pos.z, 1.0f});
+    float w = last.w;
+    last = {last.x / w, last.y / w, last.z / w, last.w / w};
+    vec2 lastC = V2(last.x * 0.5f + 0.5f, last.y * 0.5f + 0.5f);
+    vec2 thisC = V2(((float)x + 0.5f) / (float)W, ((float)y + 0.5f) / (float)H);
+    dn[0] = length(L3(cam.pos) - pos);
+    dn[1] = nrm.x; dn[2] = nrm.y; dn[3] = nrm.z;
+    am[0] = packAlbedo(albedo);
+    am[1] = (matIndex << 16) | meshIdx;
+    mv[0] = f32_to_f16(lastC.x - thisC.x);
+    mv[1] = f32_to_f16(lastC.y - thisC.y);
+}
+
+// ---------------------------------------------------------------------------------
+// di_naive.glsl:79-167
+vec3 directIllumination2(const Ctx& c, const Camera& cam, const GBufView& g, uint32_t x, uint32_t y) {
+    const SceneArrays& S = *c.S;
+    vec2 uv = V2(((float)x + 0.5f) / (float)g.W, ((float)y + 0.5f) / (float)g.H);
+    float depth; vec3 norm, albedo; int matMeshId;
+    uint32_t am[2]; fetchAlbedoMatId(g, (int)x, (int)y, am);
+    if (!unpackGBuffer(texDepthNormal(g, uv), am, depth, norm, albedo, matMeshId)) return V3s(0.0f);
+    int matId = matMeshId >> 16;
+    Ray ray = pinholeCameraSampleRay(cam, V2(uv.x, 1.0f - uv.y), V2(0, 0));
+    uint32_t rng = makeSeedPix(cam.seed, x, y);
+    vec3 radiance = V3s(0.0f);
+    vec3 pos = ray.ori + ray.dir * (depth - 1e-4f);
+    vec3 wo = -ray.dir;
+    const Material& mat = S.materials[matId];
+    // StreamSampler (di_naive.glsl:54-77)
+    vec3 sLi = V3s(0.0f); float sWeight = 0.0f, sSum = 0.0f;
+    auto addStream = [&](vec3 Li, float weight, float r) {
+        sSum += weight;
+        if (r * sSum < weight) { sWeight = weight; sLi = Li; }
+    };
+    if (!isBSDFDelta(mat)) {
+        vec3 lightDir; float lightDist, lightPdf, jac; vec2 bary; uint32_t id;
+        vec3 lightRadiance = sampleLightByPower(S, pos, lightDir, lightDist, lightPdf, jac, bary, id, sample4f(rng));
+        bool shadowed = traceShadow(c, pos, MinRayDistance, lightDir, lightDist - 1e-4f);
+        if (!shadowed && lightPdf > 1e-6f) {
+            float bsdfPdf = evalPdf(mat, norm, -ray.dir, lightDir);
+            float weight = MISWeight(lightPdf, bsdfPdf);
+            vec3 contrib = lightRadiance * evalBSDF(mat, albedo, norm, wo, lightDir) * satDot(norm, lightDir) / lightPdf * weight;
+            addStream(contrib, 100.0f, sample1f(rng));
+            radiance = radiance + contrib;
+        }
+    }
+    BSDFSample s{};
+    if (sampleBSDF(mat, albedo, norm, wo, sample3f(rng), s) && s.pdf > 1e-6f) {
+        Intersection isec = traceClosestHit(c, pos, MinRayDistance, s.wi, MaxRayDistance);
+        if (isec.instanceIdx != InvalidHitIndex) {
+            SurfaceInfo surf; loadSurfaceInfo(S, isec, surf);
+            float cosTheta = -dot(s.wi, surf.norm);
+            if (surf.isLight && cosTheta > 0.0f) {
+                float dist = length(surf.pos - pos);
+                float sumPower = S.lightTable[0].prob;
+                float lightPdf = luminance(surf.albedo) / sumPower * dist * dist / std::fabs(cosTheta);
+                float weight = isSampleTypeDelta(s.type) ? 1.0f : MISWeight(s.pdf, lightPdf);
+                float cosTerm = isSampleTypeDelta(s.type) ? 1.0f : satDot(norm, s.wi);
+                vec3 contrib = surf.albedo * s.bsdf * cosTerm / s.pdf * weight;
+                addStream(contrib, 1.0f, sample1f(rng));
+                radiance = radiance + contrib;
+            }
+        }
+    }
+    if (sWeight > 0.0f && sSum > 0.0f) radiance = sLi * sSum / sWeight;
+    else radiance = V3s(0.0f);
+    return clampColor(radiance);
+}
+
+// ---------------------------------------------------------------------------------
+// gi_naive.glsl:28-149
+vec3 indirectIllumination(const Ctx& c, const Camera& cam, const GBufView& g, uint32_t x, uint32_t y, uint32_t maxDepth) {
+    const SceneArrays& S = *c.S;
+    vec2 uv = V2(((float)x + 0.5f) / (float)g.W, ((float)y + 0.5f) / (float)g.H);
+    float depth; vec3 norm, albedo; int matMeshId;
+    uint32_t am[2]; fetchAlbedoMatId(g, (int)x, (int)y, am);
+    if (!unpackGBuffer(texDepthNormal(g, uv), am, depth, norm, albedo, matMeshId)) return V3s(0.0f);
+    int matId = matMeshId >> 16;
+    Ray ray = pinholeCameraSampleRay(cam, V2(uv.x, 1.0f - uv.y), V2(0, 0));
+    uint32_t rng = makeSeedPix(cam.seed, x, y);
+    vec3 radiance = V3s(0.0f), throughput = V3s(1.0f), lastPos = V3s(0.0f);
+    vec3 wo = -ray.dir;
+    SurfaceInfo surf;
+    surf.pos = ray.ori + ray.dir * (depth - 1e-4f);
+    surf.norm = norm; surf.albedo = albedo; surf.isLight = false; surf.matIndex = (uint32_t)matId;
+    Material mat = S.materials[matId];
+    BSDFSample s{};
+    for (uint32_t bounce = 0; bounce < maxDepth; bounce++) {
+        if (bounce > 0) {
+            Intersection isec = traceClosestHit(c, ray.ori, MinRayDistance, ray.dir, MaxRayDistance);
+            if (isec.instanceIdx == InvalidHitIndex) break;
+            loadSurfaceInfo(S, isec, surf);
+            mat = S.materials[surf.matIndex];
+        }
+        if (surf.isLight) {
+            float cosTheta = -dot(ray.dir, surf.norm);
+            if (bounce > 1 && cosTheta > 0.0f) {
+                float weight = 1.0f;
+                if (!isSampleTypeDelta(s.type)) {
+                    float dist = length(surf.pos - lastPos);
+                    float sumPower = S.lightTable[0].prob;
+                    float lightPdf = luminance(surf.albedo) / sumPower * dist * dist / std::fabs(cosTheta);
+                    weight = MISWeight(s.pdf, lightPdf);
+                }
+                radiance = radiance + surf.albedo * weight * throughput;
+            }
+            break;
+        }
+        if (bounce > 0 && !isBSDFDelta(mat)) {
+            vec3 lightDir; float lightDist, lightPdf, jac; vec2 bary; uint32_t id;
+            vec3 lightRadiance = sampleLightByPower(S, surf.pos, lightDir, lightDist, lightPdf, jac, bary, id, sample4f(rng));
+            bool shadowed = traceShadow(c, surf.pos, MinRayDistance, lightDir, lightDist - MinRayDistance);
+            if (!shadowed && lightPdf > 1e-6f) {
+                // gi_naive.glsl:116-118: the MIS weight is computed and then forced to 1.
+                float weight = 1.0f;
+                vec3 contrib = lightRadiance * evalBSDF(mat, surf.albedo, surf.norm, wo, lightDir) *
+                               satDot(surf.norm, lightDir) / lightPdf * weight * throughput;
+                radiance = radiance + contrib;
+            }
+        }
+        if (bounce > 4) {
+            float pdfTerminate = gmax(1.0f - luminance(throughput), 0.0f);
+            if (sample1f(rng) < pdfTerminate) break;
+            throughput = throughput / (1.0f - pdfTerminate);
+        }
+        if (!sampleBSDF(mat, surf.albedo, surf.norm, wo, sample3f(rng), s) || s.pdf < 1e-6f) break;
+        float cosTheta = isSampleTypeDelta(s.type) ? 1.0f : absDot(surf.norm, s.wi);
+        throughput = throughput * (s.bsdf * cosTheta / s.pdf);
+        lastPos = surf.pos;
+        wo = -s.wi;
+        ray.dir = s.wi;
+        ray.ori = surf.pos + ray.dir * 1e-4f;
+    }
+    return clampColor(radiance);
+}
+
+// ---------------------------------------------------------------------------------
+// gris_reservoir.glsl:37-136
+static inline uint32_t flagsRcVertexId(uint32_t f) { return f & 0xffu; }
+static inline void flagsSetRcVertexId(uint32_t& f, uint32_t id) { f = (f & 0xffffff00u) | (id & 0xffu); }
+static inline void flagsSetPathLength(uint32_t& f, uint32_t id) { f = (f & 0xffff00ffu) | ((id & 0xffu) << 8); }
+static inline uint32_t flagsRcVertexType(uint32_t f) { return (f >> 16) & 0xffu; }
+static inline void flagsSetRcVertexType(uint32_t& f, uint32_t t) { f = (f & 0xff00ffffu) | ((t & 0xffu) << 16); }
+static inline void setV(float* d, vec3 v) { d[0] = v.x; d[1] = v.y; d[2] = v.z; }
+static inline bool pathSampleIsValid(const GRISPathSample& p) { return p.rcIsec.instanceIdx != InvalidHitIndex; }
+static inline void reservoirReset(GRISReservoir& r) {
+    r.pathSample.rcIsec.instanceIdx = InvalidHitIndex;
+    r.sampleCount = 0.0f;
+    r.resampleWeight = 0.0f;
+}
+static inline bool reservoirIsValid(const GRISReservoir& r) {
+    return !std::isnan(r.resampleWeight) && r.resampleWeight >= 0.0f;
+}
+static inline bool reservoirMerge(GRISReservoir& r, const GRISReservoir& rhs, float rnd) {
+    r.sampleCount += rhs.sampleCount;
+    r.resampleWeight += rhs.resampleWeight;
+    if (rnd * r.resampleWeight < rhs.resampleWeight) { r.pathSample = rhs.pathSample; return true; }
+    return false;
+}
+static inline void reservoirCap(GRISReservoir& r, float cap) {
+    if (r.sampleCount > cap) { r.resampleWeight *= cap / r.sampleCount; r.sampleCount = cap; }
+}
+
+// gris_path_trace.glsl:35-43
+static inline uint32_t nextRcVertexSampleState(uint32_t state, bool connectible) {
+    if (state == 2) return 2;
+    if (!connectible) return 0;
+    return state + (connectible ? 1u : 0u);
+}
+
+// gris_path_trace.glsl:45-306
+vec3 tracePath(const Ctx& c, const Camera& cam, const GBufView& g, const GRISSettings& set, uint32_t x, uint32_t y,
+               GRISReservoir& out, bool& written) {
+    written = false;
+    const SceneArrays& S = *c.S;
+    GRISReservoir resv; std::memset(&resv, 0, sizeof(resv));
+    vec2 uv = V2(((float)x + 0.5f) / (float)g.W, ((float)y + 0.5f) / (float)g.H);
+    float depth; vec3 norm, albedo; int matMeshId;
+    uint32_t am[2]; fetchAlbedoMatId(g, (int)x, (int)y, am);
+    if (!unpackGBuffer(texDepthNormal(g, uv), am, depth, norm, albedo, matMeshId)) {
+        // the reference returns before writing the reservoir; the buffer keeps
+        // its previous contents
+        return V3s(0.0f);
+    }
+    int matId = matMeshId >> 16;
+    Ray ray = pinholeCameraSampleRay(cam, V2(uv.x, 1.0f - uv.y), V2(0, 0));
+    uint32_t rng = makeSeedPix(cam.seed, x, y);
+
+    vec3 radiance = V3s(0.0f), throughput = V3s(1.0f), rcThroughput = V3s(0.0f);
+    vec3 wo = -ray.dir;
+    vec3 lastPos = V3s(0.0f);
+    bool isLastVertexConnectible = false;
+    uint32_t sampleState = 0, lastSampleState = 0;
+    SurfaceInfo surf;
+    surf.pos = ray.ori + ray.dir * (depth - 1e-4f);
+    surf.norm = norm; surf.albedo = albedo; surf.isLight = false; surf.matIndex = (uint32_t)matId;
+    Material mat = S.materials[matId];
+    BSDFSample s{};
+    Intersection isec{};
+
+    GRISPathSample pathSample; std::memset(&pathSample, 0, sizeof(pathSample));
+    pathSample.rcIsec.instanceIdx = InvalidHitIndex;
+    pathSample.primaryRng = rng;
+
+    reservoirReset(resv);
+    // StreamSampler (gris_path_trace.glsl:10-33)
+    GRISPathSample streamSample; std::memset(&streamSample, 0, sizeof(streamSample));
+    float streamWeight = 0.0f, streamSum = 0.0f;
+    auto addStream = [&](const GRISPathSample& ps, float weight, float r) {
+        streamSum += weight;
+        if (r * streamSum < weight) { streamWeight = weight; streamSample = ps; }
+    };
+
+    for (uint32_t bounce = 0; bounce < set.maxDepth; bounce++) {
+        if (bounce > 0) {
+            isec = traceClosestHit(c, ray.ori, MinRayDistance, ray.dir, MaxRayDistance);
+            if (isec.instanceIdx == InvalidHitIndex) break;
+            loadSurfaceInfo(S, isec, surf);
+            mat = S.materials[surf.matIndex];
+        }
+        flagsSetPathLength(pathSample.flags, bounce + 1);
+        float cosPrevWi = dot(ray.dir, surf.norm);
+        float distToPrev = distance(lastPos, surf.pos);
+        float geometryJacobian = std::fabs(cosPrevWi) / square(distToPrev);
+        bool isThisVertexConnectible = surf.isLight || isBSDFConnectible(mat);
+        lastSampleState = sampleState;
+        sampleState = nextRcVertexSampleState(sampleState, isThisVertexConnectible);
+        if (set.shiftMode == ShiftReconnection && bounce == 1 && !surf.isLight) {
+            sampleState = 2; lastSampleState = 1;
+        }
+        float resvRandSample = sample1f(rng);
+
+        if (surf.isLight) {
+            if (bounce > 1 && cosPrevWi < 0.0f) {
+                float weight = 1.0f;
+                float sumPower = S.lightTable[0].prob;
+                float lightPdf = luminance(surf.albedo) / sumPower / geometryJacobian;
+                if (!isSampleTypeDelta(s.type)) weight = MISWeight(s.pdf, lightPdf);
+                vec3 weightedLi = surf.albedo * weight;
+                if (sampleState == 2 && lastSampleState == 2) {
+                    setV(pathSample.rcLi, L3(pathSample.rcLi) + weightedLi * rcThroughput);
+                    setV(pathSample.F, L3(pathSample.F) + weightedLi * throughput);
+                } else if ((sampleState == 2 && lastSampleState == 1) && isLastVertexConnectible &&
+                           distToPrev > GRISDistanceThreshold) {
+                    pathSample.rcIsec = isec;
+                    pathSample.rcRng = rng;
+                    pathSample.rcPrevSamplePdf = s.pdf;
+                    pathSample.rcJacobian = geometryJacobian;
+                    setV(pathSample.rcLi, weightedLi);
+                    setV(pathSample.rcWi, V3s(0.0f));
+                    setV(pathSample.F, weightedLi * throughput);
+                    flagsSetRcVertexId(pathSample.flags, bounce);
+                    flagsSetRcVertexType(pathSample.flags, RcVertexTypeLightScattered);
+                    addStream(pathSample, luminance(L3(pathSample.F)), resvRandSample);
+                }
+                radiance = radiance + weightedLi * throughput;
+            }
+            break;
+        }
+
+        bool connectible = isThisVertexConnectible && isLastVertexConnectible && distToPrev > GRISDistanceThreshold;
+        if ((sampleState == 2 && lastSampleState == 1) && (connectible || set.shiftMode == ShiftReconnection)) {
+            pathSample.rcIsec = isec;
+            pathSample.rcRng = rng;
+            pathSample.rcPrevSamplePdf = s.pdf;
+            pathSample.rcJacobian = geometryJacobian;
+            flagsSetRcVertexId(pathSample.flags, bounce);
+            flagsSetRcVertexType(pathSample.flags, RcVertexTypeSurface);
+            rcThroughput = V3s(1.0f);
+        }
+
+        vec4 lightRandSample = sample4f(rng);
+        resvRandSample = sample1f(rng);
+
+        if (bounce > 0 && !isBSDFDelta(mat)) {
+            vec3 lightDir; float lightDist, lightPdf, lightJacobian; vec2 lightBary; uint32_t lightId;
+            vec3 lightRadiance = sampleLightByPower(S, surf.pos, lightDir, lightDist, lightPdf, lightJacobian, lightBary,
+                                                    lightId, lightRandSample);
+            bool shadowed = traceShadow(c, surf.pos, MinRayDistance, lightDir, lightDist - MinRayDistance);
+            if (!shadowed && lightPdf > 1e-6f) {
+                float bsdfPdf = absDot(surf.norm, lightDir) * PiInv;
+                float weight = MISWeight(lightPdf, bsdfPdf);
+                vec3 scatterTerm = evalBSDF(mat, surf.albedo, surf.norm, wo, lightDir) * satDot(surf.norm, lightDir);
+                vec3 weightedLi = lightRadiance / lightPdf * weight;
+                if (sampleState == 2 && lastSampleState == 2) {
+                    setV(pathSample.rcLi, L3(pathSample.rcLi) + weightedLi * scatterTerm * rcThroughput);
+                    setV(pathSample.F, L3(pathSample.F) + weightedLi * scatterTerm * throughput);
+                } else if (sampleState == 2 && lastSampleState == 1) {
+                    setV(pathSample.rcLi, weightedLi);
+                    setV(pathSample.rcWi, lightDir);
+                    setV(pathSample.F, weightedLi * scatterTerm * throughput);
+                    addStream(pathSample, luminance(L3(pathSample.F)), resvRandSample);
+                } else if (sampleState == 1 && isThisVertexConnectible && lightDist > GRISDistanceThreshold) {
+                    pathSample.rcIsec.bary[0] = lightBary.x;
+                    pathSample.rcIsec.bary[1] = lightBary.y;
+                    pathSample.rcIsec.instanceIdx = 0;
+                    pathSample.rcIsec.triangleIdx = lightId;
+                    pathSample.rcRng = rng;
+                    pathSample.rcPrevSamplePdf = lightPdf;
+                    pathSample.rcJacobian = lightJacobian;
+                    setV(pathSample.rcLi, lightRadiance * weight);
+                    setV(pathSample.rcWi, V3s(0.0f));
+                    setV(pathSample.F, weightedLi * scatterTerm * throughput);
+                    flagsSetRcVertexId(pathSample.flags, bounce + 1);
+                    flagsSetRcVertexType(pathSample.flags, RcVertexTypeLightSampled);
+                    addStream(pathSample, luminance(L3(pathSample.F)), resvRandSample);
+                }
+                radiance = radiance + weightedLi * scatterTerm * throughput;
+            }
+        }
+
+        if (bounce > 4) {
+            float pdfTerminate = gmax(1.0f - luminance(throughput) * set.rrScale, 0.0f);
+            if (sample1f(rng) < pdfTerminate) break;
+            throughput = throughput / (1.0f - pdfTerminate);
+            rcThroughput = rcThroughput / (1.0f - pdfTerminate);
+        }
+
+        if (!sampleBSDF(mat, surf.albedo, surf.norm, wo, sample3f(rng), s) || s.pdf < 1e-6f) break;
+        float cosTheta = isSampleTypeDelta(s.type) ? 1.0f : absDot(surf.norm, s.wi);
+        vec3 scatterTerm = s.bsdf * cosTheta / s.pdf;
+        throughput = throughput * scatterTerm;
+        if (sampleState == 2 && lastSampleState == 2) {
+            rcThroughput = rcThroughput * scatterTerm;
+        } else if (sampleState == 2 && lastSampleState == 1) {
+            setV(pathSample.rcLi, V3s(0.0f));
+            setV(pathSample.rcWi, s.wi);
+            setV(pathSample.F, V3s(0.0f));
+            rcThroughput = rcThroughput / s.pdf;
+        }
+        lastPos = surf.pos;
+        wo = -s.wi;
+        ray.dir = s.wi;
+        ray.ori = surf.pos + ray.dir * 1e-4f;
+        isLastVertexConnectible = isThisVertexConnectible;
+    }
+
+    if (sampleState == 2 && lastSampleState == 2) {
+        addStream(pathSample, luminance(L3(pathSample.F)), sample1f(rng));
+    }
+    pathSample = streamSample;
+    if (streamSum > 0.0f && streamWeight > 0.0f) {
+        float k = streamSum / streamWeight;
+        setV(pathSample.F, L3(pathSample.F) * k);
+        setV(pathSample.rcLi, L3(pathSample.rcLi) * k);
+        resv.resampleWeight = luminance(L3(pathSample.F));
+    } else {
+        pathSample.rcIsec.instanceIdx = InvalidHitIndex;
+        setV(pathSample.F, V3s(0.0f));
+    }
+    resv.pathSample = pathSample;
+    resv.sampleCount = 1.0f;
+    out = resv;
+    written = true;
+    if (!pathSampleIsValid(pathSample)) return V3s(0.0f);
+    return radiance;
+}
+
+// ---------------------------------------------------------------------------------
+// gris_retrace.glsl:42-136
+struct RcData { Intersection rcPrevIsec; vec3 rcPrevWo; vec3 rcPrevThroughput; };
+
+static void traceReplayPathForHybridShift(const Ctx& c, const GRISSettings& set, Intersection isec, SurfaceInfo surf,
+                                          Ray ray, uint32_t targetFlags, uint32_t rng, RcData& rc) {
+    const SceneArrays& S = *c.S;
+    vec3 throughput = V3s(1.0f);
+    vec3 wo = -ray.dir;
+    Material mat = S.materials[surf.matIndex];
+    BSDFSample s{};
+    rc.rcPrevThroughput = V3s(0.0f);
+    rc.rcPrevWo = V3s(0.0f);
+    rc.rcPrevIsec = Intersection{};
+    rc.rcPrevIsec.instanceIdx = InvalidHitIndex;
+    uint32_t targetId = flagsRcVertexId(targetFlags);
+    if (targetId == 1) {
+        rc.rcPrevIsec = isec; rc.rcPrevWo = wo; rc.rcPrevThroughput = throughput;
+        return;
+    }
+    for (uint32_t bounce = 0; bounce < set.maxDepth; bounce++) {
+        if (bounce > 0) {
+            isec = traceClosestHit(c, ray.ori, MinRayDistance, ray.dir, MaxRayDistance);
+            if (isec.instanceIdx == InvalidHitIndex) break;
+            loadSurfaceInfo(S, isec, surf);
+            mat = S.materials[surf.matIndex];
+        }
+        bool isThisVertexConnectible = isBSDFConnectible(mat);
+        (void)sample1f(rng);
+        if (surf.isLight) break;
+        if (bounce == targetId - 1u) {
+            if (isThisVertexConnectible) {
+                rc.rcPrevIsec = isec; rc.rcPrevWo = wo; rc.rcPrevThroughput = throughput;
+            }
+            break;
+        }
+        (void)sample4f(rng);
+        (void)sample1f(rng);
+        if (bounce > 4) {
+            float pdfTerminate = gmax(1.0f - luminance(throughput) * set.rrScale, 0.0f);
+            if (sample1f(rng) < pdfTerminate) break;
+            throughput = throughput / (1.0f - pdfTerminate);
+        }
+        if (!sampleBSDF(mat, surf.albedo, surf.norm, wo, sample3f(rng), s) || s.pdf < 1e-6f) break;
+        float cosTheta = isSampleTypeDelta(s.type) ? 1.0f : absDot(surf.norm, s.wi);
+        throughput = throughput * (s.bsdf * cosTheta / s.pdf);
+        wo = -s.wi;
+        ray.dir = s.wi;
+        ray.ori = surf.pos + ray.dir * 1e-4f;
+    }
+}
+
+// gris_retrace.glsl:138-236
+static void reuseAndMerge(const Ctx& c, const GRISSettings& set, GRISReservoir& dstResv, const SurfaceInfo& dstPrimarySurf,
+                          const Intersection& dstPrimaryIsec, const Ray& primaryRay, GRISReservoir srcResv, uint32_t& rng) {
+    const SceneArrays& S = *c.S;
+    GRISPathSample srcSample = srcResv.pathSample;
+    RcData dstRc;
+    SurfaceInfo rcPrevSurf{}, rcSurf{};
+    Material rcMat{}, rcPrevMat{};
+    vec3 wi = V3s(0.0f);
+    vec3 Li = V3s(0.0f);
+    bool srcSampleValid = false;
+    float dstJacobian = 0.0f, jacobian = 0.0f, dist = 0.0f, dstPHat = 0.0f, dstSamplePdf = 0.0f;
+
+    if (pathSampleIsValid(srcSample)) {
+        traceReplayPathForHybridShift(c, set, dstPrimaryIsec, dstPrimarySurf, primaryRay, srcSample.flags,
+                                      srcSample.primaryRng, dstRc);
+        if (dstRc.rcPrevIsec.instanceIdx != InvalidHitIndex) {
+            if (dstRc.rcPrevIsec.instanceIdx == SpecialHitIndex) rcPrevSurf = dstPrimarySurf;
+            else loadSurfaceInfo(S, dstRc.rcPrevIsec, rcPrevSurf);
+            loadSurfaceInfo(S, srcSample.rcIsec, rcSurf);
+            rcMat = S.materials[rcSurf.matIndex];
+            rcPrevMat = S.materials[rcPrevSurf.matIndex];
+            dist = distance(rcPrevSurf.pos, rcSurf.pos);
+            wi = normalize(rcSurf.pos - rcPrevSurf.pos);
+            float cosTheta = -dot(rcSurf.norm, wi);
+            dstJacobian = std::fabs(cosTheta) / square(dist);
+            jacobian = dstJacobian / srcSample.rcJacobian;
+            if (dist > GRISDistanceThreshold && cosTheta > 0.0f && !std::isnan(jacobian) && srcSample.rcJacobian > 0.0f &&
+                isBSDFConnectible(rcPrevMat)) {
+                if (traceVisibility(c, rcPrevSurf.pos, rcSurf.pos)) srcSampleValid = true;
+            }
+        }
+    }
+    if (srcSampleValid) {
+        uint32_t rcType = flagsRcVertexType(srcSample.flags);
+        if (!std::isnan(srcSample.rcPrevSamplePdf) && srcSample.rcPrevSamplePdf > 1e-6f) {
+            Li = L3(srcSample.rcLi);
+            if (rcType == RcVertexTypeSurface && length(L3(srcSample.rcWi)) > 0.5f) {
+                Li = Li * (evalBSDF(rcMat, rcSurf.albedo, rcSurf.norm, -wi, L3(srcSample.rcWi)) *
+                           satDot(rcSurf.norm, L3(srcSample.rcWi)));
+            }
+            Li = Li * (evalBSDF(rcPrevMat, rcPrevSurf.albedo, rcPrevSurf.norm, dstRc.rcPrevWo, wi) * satDot(rcPrevSurf.norm, wi));
+            Li = Li * dstRc.rcPrevThroughput;
+            Li = Li / srcSample.rcPrevSamplePdf;
+            if (!isBlack(Li) && !hasNan(Li)) dstPHat = luminance(Li * jacobian);
+            if (rcType == RcVertexTypeLightSampled) {
+                float sumPower = S.lightTable[0].prob;
+                dstSamplePdf = luminance(rcSurf.albedo) / sumPower / dstJacobian;
+            } else {
+                dstSamplePdf = evalPdf(rcPrevMat, rcPrevSurf.norm, dstRc.rcPrevWo, wi);
+            }
+        }
+        float srcPHat = luminance(L3(srcSample.F));
+        srcSample.rcJacobian = dstJacobian;
+        srcSample.rcPrevSamplePdf = dstSamplePdf;
+        setV(srcSample.F, Li);
+        if (srcSample.rcPrevSamplePdf < 1e-6f || std::isnan(srcSample.rcPrevSamplePdf)) srcSample.rcPrevSamplePdf = 0.0f;
+        srcResv.pathSample = srcSample;
+        srcResv.resampleWeight *= dstPHat / srcPHat;
+    } else {
+        srcResv.resampleWeight = 0.0f;
+    }
+    if (reservoirIsValid(srcResv)) reservoirMerge(dstResv, srcResv, sample1f(rng));
+    reservoirCap(dstResv, (float)set.cap);
+}
+
+static inline GRISReservoir resvAt(const GRISReservoir* buf, uint32_t W, uint32_t H, int x, int y) {
+    // buffer index1D(pixel) as in the reference; out-of-range reads are zero
+    uint64_t idx = (uint64_t)W * (uint32_t)y + (uint32_t)x;
+    GRISReservoir r;
+    if (x < 0 || y < 0 || idx >= (uint64_t)W * H) { std::memset(&r, 0, sizeof(r)); return r; }
+    return buf[idx];
+}
+
+// gris_resample_temporal.glsl:11-34
+static bool findPreviousReservoir(const Camera& cam, const Camera& prevCam, const GBufView& gPrev,
+                                  const GRISReservoir* resvPrev, vec2 uv, vec3 pos, vec3 normal, int matMeshId,
+                                  GRISReservoir& resv) {
+    if (uv.x < 0.0f || uv.y < 0.0f || uv.x > 1.0f || uv.y > 1.0f) return false;
+    // GLSL leaves ivec2(NaN) undefined: a non-finite reprojection is rejected
+    if (std::isnan(uv.x) || std::isnan(uv.y)) return false;
+    int px = (int)(uv.x * (float)cam.filmSize[0]);
+    int py = (int)(uv.y * (float)cam.filmSize[1]);
+    float depthPrev; vec3 normalPrev, albedoPrev; int matMeshIdPrev;
+    uint32_t am[2]; fetchAlbedoMatId(gPrev, px, py, am);
+    if (!unpackGBuffer(texDepthNormal(gPrev, uv), am, depthPrev, normalPrev, albedoPrev, matMeshIdPrev)) return false;
+    Ray ray = pinholeCameraSampleRay(prevCam, V2(uv.x, 1.0f - uv.y), V2(0, 0));
+    vec3 posPrev = ray.ori + ray.dir * (depthPrev - 1e-4f);
+    if (matMeshIdPrev != matMeshId || dot(normalPrev, normal) < 0.95f || distance(pos, posPrev) > 0.5f) return false;
+    resv = resvAt(resvPrev, cam.filmSize[0], cam.filmSize[1], px, py);
+    return true;
+}
+
+// gris_resample_temporal.glsl:36-83
+bool temporalReuse(const Ctx& c, const Camera& cam, const Camera& prevCam, const GBufView& g, const GBufView& gPrev,
+                   const GRISSettings& set, const GRISReservoir* resvThis, const GRISReservoir* resvPrev,
+                   uint32_t x, uint32_t y, GRISReservoir& outTemp) {
+    const SceneArrays& S = *c.S;
+    vec2 uv = V2(((float)x + 0.5f) / (float)g.W, ((float)y + 0.5f) / (float)g.H);
+    float depth; vec3 norm, albedo; int matMeshId;
+    uint32_t am[2]; fetchAlbedoMatId(g, (int)x, (int)y, am);
+    if (!unpackGBuffer(texDepthNormal(g, uv), am, depth, norm, albedo, matMeshId)) return false;  // Temp not written
+    vec2 motion = fetchMotion(g, (int)x, (int)y);
+    int matId = matMeshId >> 16;
+    Ray ray = pinholeCameraSampleRay(cam, V2(uv.x, 1.0f - uv.y), V2(0, 0));
+    uint32_t rng = makeSeedPix(cam.seed, x, y) ^ 1u;
+    uint32_t resvRng = ~rng;
+    vec3 pos = ray.ori + ray.dir * (depth - 1e-4f);
+    (void)S;
+    GRISReservoir resv = resvThis[index1D(g.W, x, y)];
+    Intersection dstPrimaryIsec{};
+    dstPrimaryIsec.instanceIdx = SpecialHitIndex;
+    dstPrimaryIsec.bary[0] = uv.x; dstPrimaryIsec.bary[1] = uv.y;
+    if (set.temporalReuse) {
+        GRISReservoir temporalResv; std::memset(&temporalResv, 0, sizeof(temporalResv));
+        reservoirReset(temporalResv);
+        SurfaceInfo dstPrimarySurf{pos, norm, albedo, (uint32_t)matId, false};
+        if (((cam.frameIndex & CameraClearFlag) == 0) &&
+            findPreviousReservoir(cam, prevCam, gPrev, resvPrev, uv + motion, pos, norm, matMeshId, temporalResv)) {
+            if (reservoirIsValid(temporalResv)) {
+                reuseAndMerge(c, set, resv, dstPrimarySurf, dstPrimaryIsec, ray, temporalResv, resvRng);
+            }
+        }
+    }
+    if (!reservoirIsValid(resv)) reservoirReset(resv);
+    outTemp = resv;
+    return true;
+}
+
+// gris_resample_spatial.glsl:11-34
+static bool findNeighborReservoir(const Camera& cam, const GBufView& g, const GRISReservoir* resvTemp, vec2 uv, vec3 pos,
+                                  vec3 normal, GRISReservoir& resv) {
+    if (uv.x < 0.0f || uv.y < 0.0f || uv.x > 1.0f || uv.y > 1.0f) return false;
+    int px = (int)(uv.x * (float)cam.filmSize[0]);
+    int py = (int)(uv.y * (float)cam.filmSize[1]);
+    float depthPrev; vec3 normalPrev, albedoPrev; int matMeshIdPrev;
+    uint32_t am[2]; fetchAlbedoMatId(g, px, py, am);
+    if (!unpackGBuffer(texDepthNormal(g, uv), am, depthPrev, normalPrev, albedoPrev, matMeshIdPrev)) return false;
+    Ray ray = pinholeCameraSampleRay(cam, V2(uv.x, 1.0f - uv.y), V2(0, 0));
+    vec3 posPrev = ray.ori + ray.dir * (depthPrev - 1e-4f);
+    if (dot(normalPrev, normal) < 0.9f || distance(pos, posPrev) > 0.4f) return false;
+    resv = resvAt(resvTemp, cam.filmSize[0], cam.filmSize[1], px, py);
+    return true;
+}
+
+// gris_resample_spatial.glsl:36-135
+vec3 spatialReuse(const Ctx& c, const Camera& cam, const GBufView& g, const GRISSettings& set,
+                  const GRISReservoir* resvTemp, uint32_t x, uint32_t y, GRISReservoir& outThis, bool& written) {
+    written = false;
+    const SceneArrays& S = *c.S;
+    vec2 uv = V2(((float)x + 0.5f) / (float)g.W, ((float)y + 0.5f) / (float)g.H);
+    float depth; vec3 norm, albedo; int matMeshId;
+    uint32_t am[2]; fetchAlbedoMatId(g, (int)x, (int)y, am);
+    if (!unpackGBuffer(texDepthNormal(g, uv), am, depth, norm, albedo, matMeshId)) return V3s(0.0f);  // not written
+    uint32_t rng = makeSeedPix(cam.seed, x, y) ^ 2u;
+    Ray ray = pinholeCameraSampleRay(cam, V2(uv.x, 1.0f - uv.y), V2(0, 0));
+    vec3 pos = ray.ori + ray.dir * (depth - 1e-4f);
+    int matId = matMeshId >> 16;
+    vec3 radiance = V3s(0.0f);
+    GRISReservoir resv = resvTemp[index1D(g.W, x, y)];
+    Intersection dstPrimaryIsec{};
+    dstPrimaryIsec.instanceIdx = SpecialHitIndex;
+    dstPrimaryIsec.bary[0] = uv.x; dstPrimaryIsec.bary[1] = uv.y;
+    SurfaceInfo dstPrimarySurf{pos, norm, albedo, (uint32_t)matId, false};
+    if (set.spatialReuse) {
+        vec2 texelSize = V2(1.0f / (float)g.W, 1.0f / (float)g.H);
+        for (uint32_t i = 0; i < set.spatialNeighbors; i++) {
+            vec2 d = toConcentricDisk(sample2f(rng));
+            vec2 neighbor = uv + (d * set.spatialRadius) * texelSize;
+            GRISReservoir neighborResv; std::memset(&neighborResv, 0, sizeof(neighborResv));
+            reservoirReset(neighborResv);
+            if (findNeighborReservoir(cam, g, resvTemp, neighbor, pos, norm, neighborResv)) {
+                if (reservoirIsValid(neighborResv)) {
+                    reuseAndMerge(c, set, resv, dstPrimarySurf, dstPrimaryIsec, ray, neighborResv, rng);
+                }
+            }
+        }
+    }
+    if (!reservoirIsValid(resv)) reservoirReset(resv);
+    outThis = resv;
+    written = true;
+    if (reservoirIsValid(resv) && resv.sampleCount > 0.0f) {
+        GRISPathSample ps = resv.pathSample;
+        if (pathSampleIsValid(ps)) {
+            RcData rc;
+            traceReplayPathForHybridShift(c, set, dstPrimaryIsec, dstPrimarySurf, ray, ps.flags, ps.primaryRng, rc);
+            if (rc.rcPrevIsec.instanceIdx != InvalidHitIndex) {
+                SurfaceInfo rcPrevSurf{}, rcSurf{};
+                if (rc.rcPrevIsec.instanceIdx == SpecialHitIndex) rcPrevSurf = dstPrimarySurf;
+                else loadSurfaceInfo(S, rc.rcPrevIsec, rcPrevSurf);
+                loadSurfaceInfo(S, ps.rcIsec, rcSurf);
+                const Material& rcMat = S.materials[rcSurf.matIndex];
+                const Material& rcPrevMat = S.materials[rcPrevSurf.matIndex];
+                vec3 Li = L3(ps.rcLi);
+                uint32_t rcType = flagsRcVertexType(ps.flags);
+                vec3 wi = normalize(rcSurf.pos - rcPrevSurf.pos);
+                if (rcType == RcVertexTypeSurface && length(L3(ps.rcWi)) > 0.5f) {
+                    Li = Li * (evalBSDF(rcMat, rcSurf.albedo, rcSurf.norm, -wi, L3(ps.rcWi)) * satDot(rcSurf.norm, L3(ps.rcWi)));
+                }
+                Li = Li * (evalBSDF(rcPrevMat, rcPrevSurf.albedo, rcPrevSurf.norm, rc.rcPrevWo, wi) * satDot(rcPrevSurf.norm, wi));
+                Li = Li * rc.rcPrevThroughput;
+                Li = Li / ps.rcPrevSamplePdf;
+                if (!isBlack(Li) && !hasNan(Li)) {
+                    radiance = Li / luminance(Li) * resv.resampleWeight / resv.sampleCount;
+                }
+            }
+        }
+    }
+    return clampColor(radiance);
+}
+
+}  // namespace orc

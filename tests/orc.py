@@ -1,0 +1,162 @@
+"""ctypes binding of the oracle (oracle/liboracle.so) for the tests.
+
+The oracle is the scalar C++ restatement of the reference GLSL; it is only
+ever used as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+import rtpkg
+
+rt = rtpkg.rt
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_LIB):
+            subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+        L = C.CDLL(ORACLE_LIB)
+        P, u32 = C.c_void_p, C.c_uint32
+        L.orc_scene_create.argtypes = [C.POINTER(rt.SceneArrays)]
+        L.orc_scene_create.restype = P
+        L.orc_scene_destroy.argtypes = [P]
+        L.orc_scene_num_prims.argtypes = [P]
+        L.orc_scene_num_prims.restype = u32
+        L.orc_scene_world_tris.argtypes = [P, P, P]
+        L.orc_bvh_export.argtypes = [P, P, P, P, P]
+        L.orc_bvh_num_nodes.argtypes = [P]
+        L.orc_bvh_num_nodes.restype = u32
+        L.orc_trace_rays.argtypes = [P, P, u32, C.c_int, C.c_int, P, P, P, P, P, C.c_int]
+        L.orc_gbuffer.argtypes = [P, C.POINTER(rt.Camera), P, P, P, P, C.c_int]
+        L.orc_pt_plain.argtypes = [P, C.POINTER(rt.Camera), P, P, u32, P, P, P, C.c_int]
+        L.orc_gris_path_trace.argtypes = [P, C.POINTER(rt.Camera), P, P, C.POINTER(rt.GrisSettings), P, P, C.c_int]
+        L.orc_gris_temporal.argtypes = [P, C.POINTER(rt.Camera), P, P, P, P, P, C.POINTER(rt.GrisSettings), P, P, P,
+                                        P, C.c_int]
+        L.orc_gris_spatial.argtypes = [P, C.POINTER(rt.Camera), P, P, C.POINTER(rt.GrisSettings), P, P, P, P, C.c_int]
+        L.orc_hash2.argtypes = [u32]
+        L.orc_hash2.restype = u32
+        L.orc_make_seed.argtypes = [u32, u32, u32]
+        L.orc_make_seed.restype = u32
+        L.orc_sample1f.argtypes = [u32, u32, P]
+        L.orc_sincos.argtypes = [P, u32, P, P]
+        L.orc_concentric.argtypes = [P, u32, P]
+        L.orc_camera_rays.argtypes = [C.POINTER(rt.Camera), P]
+        L.orc_eval_bsdf.argtypes = [C.POINTER(rt.Material), P, P, u32, P, P]
+        L.orc_sample_bsdf.argtypes = [C.POINTER(rt.Material), P, P, u32, P, P, P, P, P]
+        L.orc_gtr2_sample.argtypes = [P, u32, C.c_float, P]
+        L.orc_sample_light.argtypes = [C.POINTER(rt.SceneArrays), P, u32, P, P]
+        L.orc_alias_build.argtypes = [P, u32, P]
+        L.orc_f32_to_f16.argtypes = [C.c_float]
+        L.orc_f32_to_f16.restype = C.c_uint16
+        _lib = L
+    return _lib
+
+
+def nthreads():
+    return max(1, min(8, os.cpu_count() or 1))
+
+
+def p(a):
+    return None if a is None else a.ctypes.data
+
+
+class OracleScene:
+    def __init__(self, arrays):
+        self.h = lib().orc_scene_create(C.byref(arrays))
+        self.n = lib().orc_scene_num_prims(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def bvh(self):
+        n = self.n
+        nn = lib().orc_bvh_num_nodes(self.h)
+        codes = np.zeros(n, np.uint64)
+        order = np.zeros(n, np.uint32)
+        nodes = np.zeros((nn, 14), np.uint32)
+        box = np.zeros(6, np.float32)
+        lib().orc_bvh_export(self.h, p(codes), p(order), p(nodes), p(box))
+        return codes, order, nodes, box
+
+    def world_tris(self):
+        pos = np.zeros((self.n, 9), np.float32)
+        ref = np.zeros((self.n, 2), np.uint32)
+        lib().orc_scene_world_tris(self.h, p(pos), p(ref))
+        return pos, ref
+
+    def trace(self, rays, mode=0, skip_lights=False):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = rays.shape[0]
+        t = np.zeros(n, np.float32)
+        bary = np.zeros((n, 2), np.float32)
+        inst = np.zeros(n, np.uint32)
+        prim = np.zeros(n, np.uint32)
+        stats = np.zeros(3, np.uint64)
+        lib().orc_trace_rays(self.h, p(rays), n, mode, int(skip_lights), p(t), p(bary), p(inst), p(prim), p(stats),
+                             nthreads())
+        return t, bary, inst, prim, stats
+
+    def gbuffer(self, cam):
+        W, H = cam.filmSize[0], cam.filmSize[1]
+        dn = np.zeros((H, W, 4), np.float32)
+        am = np.zeros((H, W, 2), np.uint32)
+        mv = np.zeros((H, W, 2), np.uint16)
+        st = np.zeros(3, np.uint64)
+        lib().orc_gbuffer(self.h, C.byref(cam), p(dn), p(am), p(mv), p(st), nthreads())
+        return dn, am, mv, st
+
+    def pt_plain(self, cam, dn, am, max_depth, direct=None, indirect=None):
+        W, H = cam.filmSize[0], cam.filmSize[1]
+        direct = np.zeros((H, W, 4), np.float32) if direct is None else direct
+        indirect = np.zeros((H, W, 4), np.float32) if indirect is None else indirect
+        st = np.zeros(3, np.uint64)
+        lib().orc_pt_plain(self.h, C.byref(cam), p(dn), p(am), max_depth, p(direct), p(indirect), p(st), nthreads())
+        return direct, indirect, st
+
+    def gris_path_trace(self, cams, dn, am, settings, resv):
+        st = np.zeros(3, np.uint64)
+        lib().orc_gris_path_trace(self.h, cams, p(dn), p(am), C.byref(settings), p(resv), p(st), nthreads())
+        return st
+
+    def gris_temporal(self, cams, dn, am, mv, dn_prev, am_prev, settings, resv_this, resv_prev, resv_temp):
+        st = np.zeros(3, np.uint64)
+        lib().orc_gris_temporal(self.h, cams, p(dn), p(am), p(mv), p(dn_prev), p(am_prev), C.byref(settings),
+                                p(resv_this), p(resv_prev), p(resv_temp), p(st), nthreads())
+        return st
+
+    def gris_spatial(self, cams, dn, am, settings, resv_temp, resv_this, indirect):
+        st = np.zeros(3, np.uint64)
+        lib().orc_gris_spatial(self.h, cams, p(dn), p(am), C.byref(settings), p(resv_temp), p(resv_this),
+                               p(indirect), p(st), nthreads())
+        return st
+
+
+RESV_DTYPE = np.dtype([("bary", np.float32, 2), ("instanceIdx", np.uint32), ("triangleIdx", np.uint32),
+                       ("rcLi", np.float32, 3), ("rcRng", np.uint32), ("rcWi", np.float32, 3), ("flags", np.uint32),
+                       ("pad", np.float32, 2), ("rcPrevSamplePdf", np.float32), ("rcJacobian", np.float32),
+                       ("F", np.float32, 3), ("primaryRng", np.uint32), ("sampleCount", np.float32),
+                       ("resampleWeight", np.float32), ("contribWeight", np.float32), ("pad0", np.float32)])
+assert RESV_DTYPE.itemsize == 96
+
+
+def reset_reservoirs(n):
+    r = np.zeros(n, RESV_DTYPE)
+    r["instanceIdx"] = 0xFFFFFFFF
+    return r
+
+
+RESV_FIELDS = ["bary", "instanceIdx", "triangleIdx", "rcLi", "rcRng", "rcWi", "flags", "rcPrevSamplePdf",
+               "rcJacobian", "F", "primaryRng", "sampleCount", "resampleWeight"]

@@ -1,0 +1,150 @@
+"""GPU parity: every kernel of the hot path against the oracle, through the C ABI.
+
+Integer/index work (LBVH codes, order, topology, hit ids) must be bit-exact;
+floating-point per-pixel results are compared at 1e-4 relative (north-star
+tolerance for the plain form) and are expected to be identical because both
+sides follow one FP contract (no contraction, correctly rounded div/sqrt,
+fixed sin/cos definition).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import gpuhelp  # noqa: E402
+import orc  # noqa: E402
+
+rt = gpuhelp.rt
+REL_TOL = 1e-4
+
+
+def close(a, b):
+    """Per-pixel agreement within REL_TOL relative (absolute below 1)."""
+    d = np.abs(a.astype(np.float64) - b.astype(np.float64))
+    tol = REL_TOL * np.maximum(np.abs(b.astype(np.float64)), 1.0)
+    return (d <= tol) | (np.isnan(a) & np.isnan(b))
+
+
+@pytest.fixture(scope="module", params=[0, 2], ids=["boxroom", "floor"])
+def scene(request):
+    print(f"[gpu] scene config {request.param}", flush=True)
+    hs = rt.HostScene(request.param, 0)
+    ds = rt.DeviceScene(hs.arrays)
+    osc = orc.OracleScene(hs.arrays)
+    return hs, ds, osc
+
+
+def test_lbvh_bitexact(scene):
+    hs, ds, osc = scene
+    codes, order, nodes, info = gpuhelp.gpu_bvh(ds)
+    ocodes, oorder, onodes, obox = osc.bvh()
+    assert info.numPrimitives == osc.n
+    assert np.array_equal(codes, ocodes), "Morton codes differ"
+    assert np.array_equal(order, oorder), "radix sort order differs"
+    assert np.array_equal(nodes, onodes), "LBVH nodes differ"
+    assert np.array_equal(np.array(info.sceneLo, np.float32), obox[:3])
+    assert info.bvhDepth <= 112
+
+
+def test_trace_rays_bitexact(scene):
+    hs, ds, osc = scene
+    lo, hi = (-1.0, 2.0) if hs.arrays.numInstances > 2 else (-3.0, 3.0)
+    rays = gpuhelp.random_rays(200000, lo, hi, seed=1)
+    for mode in (0, 1):
+        t, bary, inst, prim = gpuhelp.gpu_trace(ds, rays, mode)
+        ot, obary, oinst, oprim, _ = osc.trace(rays, mode)
+        assert np.array_equal(inst, oinst), f"mode {mode}: hit instance differs"
+        if mode == 0:
+            assert np.array_equal(prim, oprim)
+            assert np.array_equal(t.view(np.uint32), ot.view(np.uint32))
+            assert np.array_equal(bary.view(np.uint32), obary.view(np.uint32))
+
+
+def test_trace_counters_match_oracle(scene):
+    hs, ds, osc = scene
+    rays = gpuhelp.random_rays(20000, -1.0, 2.0, seed=2)
+    rt.check(rt.core().rt_counters_enable(1), "counters")
+    try:
+        out = np.zeros(3, np.uint64)
+        rt.check(rt.core().rt_counters_read(out.ctypes.data, 1), "read")
+        gpuhelp.gpu_trace(ds, rays, 0)
+        rt.check(rt.core().rt_counters_read(out.ctypes.data, 1), "read")
+    finally:
+        rt.core().rt_counters_enable(0)
+    _, _, _, _, st = osc.trace(rays, 0)
+    assert out[0] == st[1] and out[1] == st[2] and out[2] == st[0]
+
+
+def test_gbuffer_bitexact(scene):
+    hs, ds, osc = scene
+    W, H = 96, 64
+    run = gpuhelp.GrisRun(ds, W, H)
+    for frame in (0, 3):
+        cams = hs.camera(frame, W, H)
+        run.gbuffer(cams[0])
+        dn, am, mv = run.gbuffer_host()
+        odn, oam, omv, _ = osc.gbuffer(cams[0])
+        assert np.array_equal(dn.view(np.uint32), odn.view(np.uint32)), "depthNormal differs"
+        assert np.array_equal(am, oam), "albedoMatId differs"
+        assert np.array_equal(mv, omv), "motion vectors differ"
+
+
+def test_pt_plain_parity(scene):
+    hs, ds, osc = scene
+    W, H = 96, 64
+    run = gpuhelp.GrisRun(ds, W, H)
+    cams = hs.camera(0, W, H)
+    run.gbuffer(cams[0])
+    run.plain(cams, 4)
+    direct, indirect = run.output(0), run.output(1)
+    dn, am, _ = run.gbuffer_host()
+    od, oi, _ = osc.pt_plain(cams[0], dn, am, 4)
+    for name, g, o in (("direct", direct, od), ("indirect", indirect, oi)):
+        ok = close(g, o).all(axis=-1)
+        assert ok.mean() == 1.0, f"{name}: {int((~ok).sum())} pixels outside {REL_TOL}"
+        assert np.isfinite(g).all()
+
+
+@pytest.mark.parametrize("shift", [2, 0], ids=["hybrid", "reconnection"])
+def test_gris_sequence_parity(scene, shift):
+    """Three frames of the full GRIS pipeline (temporal + spatial), pass by pass."""
+    hs, ds, osc = scene
+    W, H = 80, 48
+    s = rt.settings(shift=shift, temporal=True, spatial=True, max_depth=6, cap=20, radius=20.0)
+    run = gpuhelp.GrisRun(ds, W, H)
+    n = W * H
+    o_dn = [None, None]
+    o_am = [None, None]
+    o_resv = [orc.reset_reservoirs(n), orc.reset_reservoirs(n)]
+    o_temp = orc.reset_reservoirs(n)
+    o_out = np.zeros((H, W, 4), np.float32)
+    cur = 0
+    for frame in range(3):
+        cams = hs.camera(frame, W, H)
+        run.gbuffer(cams[0])
+        dn, am, mv, _ = osc.gbuffer(cams[0])
+        o_dn[cur], o_am[cur] = dn, am
+        prev = cur ^ 1
+        pdn = o_dn[prev] if o_dn[prev] is not None else np.zeros_like(dn)
+        pam = o_am[prev] if o_am[prev] is not None else np.zeros_like(am)
+        if o_dn[prev] is None:  # first frame: the GPU's previous parity is zero-filled too
+            pass
+        run.path_trace(cams, s)
+        osc.gris_path_trace(cams, dn, am, s, o_resv[cur])
+        bad = gpuhelp.compare_reservoirs(run.reservoirs(run.resv[run.cur]), o_resv[cur])
+        assert not bad.any(), f"frame {frame} path trace: {int(bad.sum())} reservoirs differ"
+        run.temporal(cams, s)
+        osc.gris_temporal(cams, dn, am, mv, pdn, pam, s, o_resv[cur], o_resv[prev], o_temp)
+        bad = gpuhelp.compare_reservoirs(run.reservoirs(run.temp), o_temp)
+        assert not bad.any(), f"frame {frame} temporal: {int(bad.sum())} reservoirs differ"
+        run.spatial(cams, s)
+        osc.gris_spatial(cams, dn, am, s, o_temp, o_resv[cur], o_out)
+        bad = gpuhelp.compare_reservoirs(run.reservoirs(run.resv[run.cur]), o_resv[cur])
+        assert not bad.any(), f"frame {frame} spatial: {int(bad.sum())} reservoirs differ"
+        out = run.output(1)
+        ok = close(out, o_out).all(axis=-1)
+        assert ok.all(), f"frame {frame}: {int((~ok).sum())} output pixels outside {REL_TOL}"
+        run.flip()
+        cur ^= 1

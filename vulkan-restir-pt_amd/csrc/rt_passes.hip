@@ -1,0 +1,917 @@
+// Per-pixel kernels for gfx950: G-buffer producer, plain NEE+MIS path tracer
+// and the three GRIS (ReSTIR PT) passes.
+//
+// Dispatch mirrors the reference's 8x8 workgroups (src/shader/HostDevice.h:36-37,
+// zvk/core/ComputePipeline.cpp:11-25): one 64-lane wave per 8x8 pixel tile.
+// Each lane keeps its path state in registers and traverses the LBVH with an
+// LDS short stack. GRIS reservoirs live in HBM as five float4 planes
+// (80 B/pixel) so that every plane access of a wave is one contiguous 1-KiB
+// transaction:
+//   P0 rcIsec (bary.xy, instance, primitive)   P1 rcLi.xyz, rcRng
+//   P2 rcWi.xyz, flags                         P3 F.xyz, primaryRng
+//   P4 rcPrevSamplePdf, rcJacobian, sampleCount, resampleWeight
+// Arithmetic follows the reference GLSL expression by expression (see the
+// citations at each function) under the FP contract of rt_dmath.h.
+#include "rt_internal.h"
+#include <hip/hip_fp16.h>
+
+using namespace rtd;
+
+namespace {
+
+constexpr int kTileX = 8, kTileY = 8, kTileThreads = 64;
+
+struct GB {   // one G-buffer parity (descriptor bindings 2/4/6 or 3/5)
+    const float4* dn;
+    const uint2* am;
+    const uint32_t* mv;
+    uint32_t W, H;
+};
+
+// ---- camera.glsl:28-42 ------------------------------------------------------------------
+struct CamRay { f3 ori, dir; };
+RT_DEV CamRay pinholeRay(const rt_camera& cam, f2 uv) {
+    f2 texel = mk2(1.0f / (float)cam.filmSize[0], 1.0f / (float)cam.filmSize[1]);
+    f2 biased = uv + texel * mk2(0.0f, 0.0f);
+    f2 ndc = mk2(biased.x * 2.0f - 1.0f, biased.y * 2.0f - 1.0f);
+    float aspect = (float)cam.filmSize[0] / (float)cam.filmSize[1];
+    float tanFOV = rt_tan(cam.FOV * 0.5f * 0.017453292519943295f);
+    f3 pf = mk3(ndc.x * aspect * tanFOV, ndc.y * 1.0f * tanFOV, 1.0f);
+    f3 dir = normalize(pf);
+    dir = normalize(ld3(cam.right) * dir.x + ld3(cam.up) * dir.y + ld3(cam.front) * dir.z);
+    CamRay r; r.ori = ld3(cam.pos); r.dir = dir;
+    return r;
+}
+RT_DEV f2 pixelUV(uint32_t x, uint32_t y, uint32_t W, uint32_t H) {
+    return mk2(((float)x + 0.5f) / (float)W, ((float)y + 0.5f) / (float)H);
+}
+
+// ---- G-buffer access ----------------------------------------------------------------------
+// texture() with linear filtering and Repeat addressing, 8-bit subtexel weights.
+RT_DEV void bilinearSetup(float u, uint32_t W, int& i0, int& i1, float& a) {
+    float x = u * (float)W - 0.5f;
+    float fl = floorf(x);
+    float fr = x - fl;
+    int q = (int)(fr * 256.0f + 0.5f);
+    int ii = (int)fl;
+    if (q >= 256) { ii += 1; q = 0; }
+    int w = (int)W;
+    i0 = ((ii % w) + w) % w;
+    i1 = (((ii + 1) % w) + w) % w;
+    a = (float)q * (1.0f / 256.0f);
+}
+RT_DEV f4 texDN(const GB& g, f2 uv) {
+    int x0, x1, y0, y1; float a, b;
+    bilinearSetup(uv.x, g.W, x0, x1, a);
+    bilinearSetup(uv.y, g.H, y0, y1, b);
+    float4 t00 = g.dn[(size_t)g.W * y0 + x0], t10 = g.dn[(size_t)g.W * y0 + x1];
+    float4 t01 = g.dn[(size_t)g.W * y1 + x0], t11 = g.dn[(size_t)g.W * y1 + x1];
+    float oa = 1.0f - a, ob = 1.0f - b;
+    f4 r;
+    r.x = (t00.x * oa + t10.x * a) * ob + (t01.x * oa + t11.x * a) * b;
+    r.y = (t00.y * oa + t10.y * a) * ob + (t01.y * oa + t11.y * a) * b;
+    r.z = (t00.z * oa + t10.z * a) * ob + (t01.z * oa + t11.z * a) * b;
+    r.w = (t00.w * oa + t10.w * a) * ob + (t01.w * oa + t11.w * a) * b;
+    return r;
+}
+// At a pixel centre the 8-bit weights round to (0, 0) or (1, 1), so texture()
+// returns exactly the texel (valid for images narrower than 32768 pixels).
+RT_DEV f4 texDNCentre(const GB& g, uint32_t x, uint32_t y) {
+    float4 t = g.dn[(size_t)g.W * y + x];
+    f4 r; r.x = t.x; r.y = t.y; r.z = t.z; r.w = t.w;
+    return r;
+}
+RT_DEV uint2 fetchAM(const GB& g, int x, int y) {
+    if (x < 0 || y < 0 || x >= (int)g.W || y >= (int)g.H) return make_uint2(0u, 0u);
+    return g.am[(size_t)g.W * y + x];
+}
+RT_DEV f2 fetchMotion(const GB& g, int x, int y) {
+    uint32_t v = g.mv[(size_t)g.W * y + x];
+    return mk2(__half2float(__ushort_as_half((unsigned short)(v & 0xffffu))),
+               __half2float(__ushort_as_half((unsigned short)(v >> 16))));
+}
+RT_DEV f3 unpackAlbedo(uint32_t p) {
+    return mk3((float)(p & 0xffu) / 255.0f, (float)((p >> 8) & 0xffu) / 255.0f, (float)((p >> 16) & 0xffu) / 255.0f);
+}
+RT_DEV uint32_t packUnorm8(float c) { return (uint32_t)(gclamp(c, 0.0f, 1.0f) * 255.0f + 0.5f); }
+
+struct GPix { float depth; f3 norm; f3 albedo; int matMeshId; };
+// gbuffer_util.glsl:24-35
+RT_DEV bool unpackGB(f4 dn, uint2 am, GPix& p) {
+    p.depth = dn.x;
+    if (p.depth == 0.0f) return false;
+    p.norm = mk3(dn.y, dn.z, dn.w);
+    p.albedo = unpackAlbedo(am.x);
+    p.matMeshId = (int)am.y;
+    return true;
+}
+
+// ---- traversal wrappers (ray_query.glsl:6-70) ------------------------------------------------
+template <bool kCount>
+RT_DEV Isec traceClosestHit(const DevScene& S, TStack st, f3 o, float tmin, f3 d, float tmax, TraceCounters* ctr,
+                            bool skipLights = false, float* tOut = nullptr) {
+    RayT r = makeRay(o, d, tmin, tmax);
+    Isec h; h.b1 = 0.0f; h.b2 = 0.0f; h.inst = RT_INVALID_HIT; h.prim = 0;
+    float tBest = tmax;
+    bool hit = traverse<false, kCount>(S, r, skipLights, st, tBest, h, ctr);
+    if (!hit) { h.inst = RT_INVALID_HIT; h.b1 = 0.0f; h.b2 = 0.0f; h.prim = 0; }
+    if (tOut) *tOut = hit ? tBest : 0.0f;
+    return h;
+}
+template <bool kCount>
+RT_DEV bool traceShadow(const DevScene& S, TStack st, f3 o, float tmin, f3 d, float tmax, TraceCounters* ctr) {
+    RayT r = makeRay(o, d, tmin, tmax);
+    Isec h; float tBest = tmax;
+    return traverse<true, kCount>(S, r, false, st, tBest, h, ctr);
+}
+template <bool kCount>
+RT_DEV bool traceVisibility(const DevScene& S, TStack st, f3 from, f3 to, TraceCounters* ctr) {
+    return !traceShadow<kCount>(S, st, from, RT_MIN_RAY, normalize(to - from), distance(to, from) - RT_MIN_RAY, ctr);
+}
+
+RT_DEV void accumulate(float* out, size_t p, f3 c, uint32_t frameIndex) {
+    float k = (float)(frameIndex & 0x7fffffffu);
+    float4 o = reinterpret_cast<float4*>(out)[p];
+    f3 acc = mk3(o.x, o.y, o.z);
+    acc = (acc * k + c) / (k + 1.0f);
+    reinterpret_cast<float4*>(out)[p] = make_float4(acc.x, acc.y, acc.z, 1.0f);
+}
+
+#define PIXEL_INDEX                                                              \
+    const uint32_t x = blockIdx.x * kTileX + (threadIdx.x & (kTileX - 1));      \
+    const uint32_t y = blockIdx.y * kTileY + (threadIdx.x / kTileX);
+
+// ================================================================================================
+// Ray query test entry
+template <bool kCount>
+__global__ __launch_bounds__(kTileThreads) void k_trace_rays(DevScene S, const float* __restrict__ rays8, uint32_t n,
+                                                             int mode, int skipLights, float* t, float* bary2,
+                                                             uint32_t* inst, uint32_t* prim, TraceCounters* ctr) {
+    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    uint32_t i = blockIdx.x * kTileThreads + threadIdx.x;
+    if (i >= n) return;
+    TStack st{stk + threadIdx.x, kTileThreads};
+    const float* q = rays8 + 8 * (size_t)i;
+    f3 o = mk3(q[0], q[1], q[2]), d = mk3(q[3], q[4], q[5]);
+    if (mode == 1) {
+        bool occ = traceShadow<kCount>(S, st, o, q[6], d, q[7], ctr);
+        inst[i] = occ ? 1u : 0u; prim[i] = 0; t[i] = 0.0f; bary2[2 * i] = 0.0f; bary2[2 * i + 1] = 0.0f;
+        return;
+    }
+    float th;
+    Isec h = traceClosestHit<kCount>(S, st, o, q[6], d, q[7], ctr, skipLights != 0, &th);
+    t[i] = th; bary2[2 * i] = h.b1; bary2[2 * i + 1] = h.b2; inst[i] = h.inst; prim[i] = h.prim;
+}
+
+// ================================================================================================
+// G-buffer: GBuffer.vert:19-29 + GBuffer.frag:20-53 evaluated at pixel centres.
+// Light triangles are not drawn by the reference's raster pass; the clip
+// planes act as ray distances [near, far].
+template <bool kCount>
+__global__ __launch_bounds__(kTileThreads) void k_gbuffer(DevScene S, rt_camera cam, float4* __restrict__ dnOut,
+                                                          uint2* __restrict__ amOut, uint32_t* __restrict__ mvOut,
+                                                          TraceCounters* ctr) {
+    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    PIXEL_INDEX
+    const uint32_t W = cam.filmSize[0], H = cam.filmSize[1];
+    if (x >= W || y >= H) return;
+    TStack st{stk + threadIdx.x, kTileThreads};
+    size_t p = (size_t)y * W + x;
+    f2 uv = pixelUV(x, y, W, H);
+    CamRay ray = pinholeRay(cam, mk2(uv.x, 1.0f - uv.y));
+    Isec h = traceClosestHit<kCount>(S, st, ray.ori, cam.nearZ, ray.dir, cam.farZ, ctr, true);
+    if (h.inst == RT_INVALID_HIT) {
+        dnOut[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        amOut[p] = make_uint2(0u, 0u);
+        mvOut[p] = 0u;
+        return;
+    }
+    uint32_t meshIdx = h.inst - 1;
+    const DevInstance& I = S.instances[meshIdx];
+    uint32_t tri = h.prim;
+    float b1 = h.b1, b2 = h.b2, b0 = 1.0f - b1 - b2;
+    f3 P[3], N[3];
+    for (int k = 0; k < 3; k++) {
+        const rt_mesh_vertex& v = S.vertices[S.indices[I.indexOffset + tri * 3 + k]];
+        P[k] = colMul4(I.m, ld3(v.pos), 1.0f);
+        f3 nn = mk3(I.n[0], I.n[1], I.n[2]) * v.norm[0] + mk3(I.n[3], I.n[4], I.n[5]) * v.norm[1] +
+                mk3(I.n[6], I.n[7], I.n[8]) * v.norm[2];
+        N[k] = normalize(nn);
+    }
+    f3 pos = P[0] * b0 + P[1] * b1 + P[2] * b2;
+    f3 nrm = normalize(N[0] * b0 + N[1] * b1 + N[2] * b2);
+    uint32_t matIndex = (uint32_t)S.materialIndices[I.indexOffset / 3 + tri];
+    const rt_material& m = S.materials[matIndex];
+    const float* L = cam.lastProjView;
+    float lx = L[0] * pos.x + L[4] * pos.y + L[8] * pos.z + L[12] * 1.0f;
+    float ly = L[1] * pos.x + L[5] * pos.y + L[9] * pos.z + L[13] * 1.0f;
+    float lw = L[3] * pos.x + L[7] * pos.y + L[11] * pos.z + L[15] * 1.0f;
+    lx = lx / lw; ly = ly / lw;
+    f2 lastC = mk2(lx * 0.5f + 0.5f, ly * 0.5f + 0.5f);
+    dnOut[p] = make_float4(length(ld3(cam.pos) - pos), nrm.x, nrm.y, nrm.z);
+    uint32_t alb = packUnorm8(m.baseColor[0]) | (packUnorm8(m.baseColor[1]) << 8) |
+                   (packUnorm8(m.baseColor[2]) << 16) | (packUnorm8(1.0f) << 24);
+    amOut[p] = make_uint2(alb, (matIndex << 16) | meshIdx);
+    uint32_t mx = __half_as_ushort(__float2half_rn(lastC.x - uv.x));
+    uint32_t my = __half_as_ushort(__float2half_rn(lastC.y - uv.y));
+    mvOut[p] = mx | (my << 16);
+}
+
+// ================================================================================================
+// Plain PT: di_naive.glsl:79-167 (bit 0) and gi_naive.glsl:28-149 (bit 1) with
+// the accumulation of di_naive.comp / gi_naive.comp.
+template <bool kCount>
+RT_DEV f3 directIllumination2(const DevScene& S, TStack st, const rt_camera& cam, const GB& g, uint32_t x, uint32_t y,
+                              TraceCounters* ctr) {
+    f2 uv = pixelUV(x, y, g.W, g.H);
+    GPix gp;
+    if (!unpackGB(texDNCentre(g, x, y), fetchAM(g, (int)x, (int)y), gp)) return s3(0.0f);
+    int matId = gp.matMeshId >> 16;
+    CamRay ray = pinholeRay(cam, mk2(uv.x, 1.0f - uv.y));
+    uint32_t rng = makeSeedPix(cam.seed, x, y);
+    f3 radiance = s3(0.0f);
+    f3 pos = ray.ori + ray.dir * (gp.depth - 1e-4f);
+    f3 wo = -ray.dir;
+    Mat mat = loadMat(S, (uint32_t)matId);
+    f3 sLi = s3(0.0f); float sW = 0.0f, sSum = 0.0f;
+    if (!isBSDFDelta(mat)) {
+        f3 lightDir; float lightDist, lightPdf, jac; f2 bary; uint32_t id;
+        f3 Lr = sampleLightByPower(S, pos, lightDir, lightDist, lightPdf, jac, bary, id, sample4f(rng));
+        bool shadowed = traceShadow<kCount>(S, st, pos, RT_MIN_RAY, lightDir, lightDist - 1e-4f, ctr);
+        if (!shadowed && lightPdf > 1e-6f) {
+            float bsdfPdf = evalPdf(mat, gp.norm, -ray.dir, lightDir);
+            float w = MISWeight(lightPdf, bsdfPdf);
+            f3 contrib = Lr * evalBSDF(mat, gp.albedo, gp.norm, wo, lightDir) * satDot(gp.norm, lightDir) / lightPdf * w;
+            float r = sample1f(rng);
+            sSum += 100.0f;
+            if (r * sSum < 100.0f) { sW = 100.0f; sLi = contrib; }
+            radiance = radiance + contrib;
+        }
+    }
+    BSDFSample s; s.type = 0; s.pdf = 0.0f;
+    if (sampleBSDF(mat, gp.albedo, gp.norm, wo, sample3f(rng), s) && s.pdf > 1e-6f) {
+        Isec is = traceClosestHit<kCount>(S, st, pos, RT_MIN_RAY, s.wi, RT_MAX_RAY, ctr);
+        if (is.inst != RT_INVALID_HIT) {
+            Surf surf; loadSurfaceInfo(S, is, surf);
+            float cosTheta = -dot(s.wi, surf.norm);
+            if (surf.isLight && cosTheta > 0.0f) {
+                float dist = length(surf.pos - pos);
+                float sumPower = S.lightTable[0].prob;
+                float lightPdf = luminance(surf.albedo) / sumPower * dist * dist / fabsf(cosTheta);
+                float w = isSampleTypeDelta(s.type) ? 1.0f : MISWeight(s.pdf, lightPdf);
+                float cosTerm = isSampleTypeDelta(s.type) ? 1.0f : satDot(gp.norm, s.wi);
+                f3 contrib = surf.albedo * s.bsdf * cosTerm / s.pdf * w;
+                float r = sample1f(rng);
+                sSum += 1.0f;
+                if (r * sSum < 1.0f) { sW = 1.0f; sLi = contrib; }
+                radiance = radiance + contrib;
+            }
+        }
+    }
+    if (sW > 0.0f && sSum > 0.0f) radiance = sLi * sSum / sW;
+    else radiance = s3(0.0f);
+    return clampColor(radiance);
+}
+
+template <bool kCount>
+RT_DEV f3 indirectIllumination(const DevScene& S, TStack st, const rt_camera& cam, const GB& g, uint32_t x, uint32_t y,
+                               uint32_t maxDepth, TraceCounters* ctr) {
+    f2 uv = pixelUV(x, y, g.W, g.H);
+    GPix gp;
+    if (!unpackGB(texDNCentre(g, x, y), fetchAM(g, (int)x, (int)y), gp)) return s3(0.0f);
+    int matId = gp.matMeshId >> 16;
+    CamRay ray = pinholeRay(cam, mk2(uv.x, 1.0f - uv.y));
+    uint32_t rng = makeSeedPix(cam.seed, x, y);
+    f3 radiance = s3(0.0f), throughput = s3(1.0f), lastPos = s3(0.0f);
+    f3 wo = -ray.dir;
+    Surf surf;
+    surf.pos = ray.ori + ray.dir * (gp.depth - 1e-4f);
+    surf.norm = gp.norm; surf.albedo = gp.albedo; surf.isLight = false; surf.matIndex = (uint32_t)matId;
+    Mat mat = loadMat(S, (uint32_t)matId);
+    BSDFSample s; s.type = 0; s.pdf = 0.0f; s.wi = s3(0.0f); s.bsdf = s3(0.0f);
+    f3 rdir = ray.dir, rori = ray.ori;
+    for (uint32_t bounce = 0; bounce < maxDepth; bounce++) {
+        if (bounce > 0) {
+            Isec is = traceClosestHit<kCount>(S, st, rori, RT_MIN_RAY, rdir, RT_MAX_RAY, ctr);
+            if (is.inst == RT_INVALID_HIT) break;
+            loadSurfaceInfo(S, is, surf);
+            mat = loadMat(S, surf.matIndex);
+        }
+        if (surf.isLight) {
+            float cosTheta = -dot(rdir, surf.norm);
+            if (bounce > 1 && cosTheta > 0.0f) {
+                float w = 1.0f;
+                if (!isSampleTypeDelta(s.type)) {
+                    float dist = length(surf.pos - lastPos);
+                    float sumPower = S.lightTable[0].prob;
+                    float lightPdf = luminance(surf.albedo) / sumPower * dist * dist / fabsf(cosTheta);
+                    w = MISWeight(s.pdf, lightPdf);
+                }
+                radiance = radiance + surf.albedo * w * throughput;
+            }
+            break;
+        }
+        if (bounce > 0 && !isBSDFDelta(mat)) {
+            f3 lightDir; float lightDist, lightPdf, jac; f2 bary; uint32_t id;
+            f3 Lr = sampleLightByPower(S, surf.pos, lightDir, lightDist, lightPdf, jac, bary, id, sample4f(rng));
+            bool shadowed = traceShadow<kCount>(S, st, surf.pos, RT_MIN_RAY, lightDir, lightDist - RT_MIN_RAY, ctr);
+            if (!shadowed && lightPdf > 1e-6f) {
+                // gi_naive.glsl:116-118 forces the NEE MIS weight to 1
+                f3 contrib = Lr * evalBSDF(mat, surf.albedo, surf.norm, wo, lightDir) * satDot(surf.norm, lightDir) /
+                             lightPdf * 1.0f * throughput;
+                radiance = radiance + contrib;
+            }
+        }
+        if (bounce > 4) {
+            float pdfTerminate = gmax(1.0f - luminance(throughput), 0.0f);
+            if (sample1f(rng) < pdfTerminate) break;
+            throughput = throughput / (1.0f - pdfTerminate);
+        }
+        if (!sampleBSDF(mat, surf.albedo, surf.norm, wo, sample3f(rng), s) || s.pdf < 1e-6f) break;
+        float cosTheta = isSampleTypeDelta(s.type) ? 1.0f : absDot(surf.norm, s.wi);
+        throughput = throughput * (s.bsdf * cosTheta / s.pdf);
+        lastPos = surf.pos;
+        wo = -s.wi;
+        rdir = s.wi;
+        rori = surf.pos + rdir * 1e-4f;
+    }
+    return clampColor(radiance);
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(kTileThreads) void k_pt_plain(DevScene S, rt_camera cam, GB g, uint32_t maxDepth,
+                                                           uint32_t which, float* directOut, float* indirectOut,
+                                                           TraceCounters* ctr) {
+    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    PIXEL_INDEX
+    if (x >= g.W || y >= g.H) return;
+    TStack st{stk + threadIdx.x, kTileThreads};
+    size_t p = (size_t)y * g.W + x;
+    if (which & 1u) accumulate(directOut, p, directIllumination2<kCount>(S, st, cam, g, x, y, ctr), cam.frameIndex);
+    if (which & 2u) accumulate(indirectOut, p, indirectIllumination<kCount>(S, st, cam, g, x, y, maxDepth, ctr), cam.frameIndex);
+}
+
+// ================================================================================================
+// GRIS reservoir (layouts.glsl:132-156) in registers, planar in HBM.
+struct Resv {
+    Isec rcIsec;
+    f3 rcLi; uint32_t rcRng;
+    f3 rcWi; uint32_t flags;
+    float rcPrevSamplePdf, rcJacobian;
+    f3 F; uint32_t primaryRng;
+    float sampleCount, resampleWeight;
+};
+struct ResvPlanes { float4* p; size_t n; };
+
+RT_DEV Resv loadResv(const ResvPlanes& R, size_t i) {
+    float4 a = R.p[i], b = R.p[R.n + i], c = R.p[2 * R.n + i], d = R.p[3 * R.n + i], e = R.p[4 * R.n + i];
+    Resv r;
+    r.rcIsec.b1 = a.x; r.rcIsec.b2 = a.y; r.rcIsec.inst = __float_as_uint(a.z); r.rcIsec.prim = __float_as_uint(a.w);
+    r.rcLi = mk3(b.x, b.y, b.z); r.rcRng = __float_as_uint(b.w);
+    r.rcWi = mk3(c.x, c.y, c.z); r.flags = __float_as_uint(c.w);
+    r.F = mk3(d.x, d.y, d.z); r.primaryRng = __float_as_uint(d.w);
+    r.rcPrevSamplePdf = e.x; r.rcJacobian = e.y; r.sampleCount = e.z; r.resampleWeight = e.w;
+    return r;
+}
+RT_DEV void storeResv(const ResvPlanes& R, size_t i, const Resv& r) {
+    R.p[i] = make_float4(r.rcIsec.b1, r.rcIsec.b2, __uint_as_float(r.rcIsec.inst), __uint_as_float(r.rcIsec.prim));
+    R.p[R.n + i] = make_float4(r.rcLi.x, r.rcLi.y, r.rcLi.z, __uint_as_float(r.rcRng));
+    R.p[2 * R.n + i] = make_float4(r.rcWi.x, r.rcWi.y, r.rcWi.z, __uint_as_float(r.flags));
+    R.p[3 * R.n + i] = make_float4(r.F.x, r.F.y, r.F.z, __uint_as_float(r.primaryRng));
+    R.p[4 * R.n + i] = make_float4(r.rcPrevSamplePdf, r.rcJacobian, r.sampleCount, r.resampleWeight);
+}
+RT_DEV Resv zeroResv() {
+    Resv r;
+    r.rcIsec.b1 = 0.0f; r.rcIsec.b2 = 0.0f; r.rcIsec.inst = 0u; r.rcIsec.prim = 0u;
+    r.rcLi = s3(0.0f); r.rcRng = 0u; r.rcWi = s3(0.0f); r.flags = 0u;
+    r.rcPrevSamplePdf = 0.0f; r.rcJacobian = 0.0f; r.F = s3(0.0f); r.primaryRng = 0u;
+    r.sampleCount = 0.0f; r.resampleWeight = 0.0f;
+    return r;
+}
+// gris_reservoir.glsl:37-136
+RT_DEV uint32_t flagsRcVertexId(uint32_t f) { return f & 0xffu; }
+RT_DEV uint32_t flagsRcVertexType(uint32_t f) { return (f >> 16) & 0xffu; }
+RT_DEV void flagsSetRcVertexId(uint32_t& f, uint32_t id) { f = (f & 0xffffff00u) | (id & 0xffu); }
+RT_DEV void flagsSetPathLength(uint32_t& f, uint32_t v) { f = (f & 0xffff00ffu) | ((v & 0xffu) << 8); }
+RT_DEV void flagsSetRcVertexType(uint32_t& f, uint32_t t) { f = (f & 0xff00ffffu) | ((t & 0xffu) << 16); }
+RT_DEV void resvReset(Resv& r) { r.rcIsec.inst = RT_INVALID_HIT; r.sampleCount = 0.0f; r.resampleWeight = 0.0f; }
+RT_DEV bool resvIsValid(const Resv& r) { return !isnan(r.resampleWeight) && r.resampleWeight >= 0.0f; }
+RT_DEV void resvMerge(Resv& r, const Resv& rhs, float rnd) {
+    r.sampleCount += rhs.sampleCount;
+    r.resampleWeight += rhs.resampleWeight;
+    if (rnd * r.resampleWeight < rhs.resampleWeight) {
+        float sc = r.sampleCount, rw = r.resampleWeight;
+        r = rhs;
+        r.sampleCount = sc; r.resampleWeight = rw;
+    }
+}
+RT_DEV void resvCap(Resv& r, float cap) {
+    if (r.sampleCount > cap) { r.resampleWeight *= cap / r.sampleCount; r.sampleCount = cap; }
+}
+
+#define RC_LIGHT_SAMPLED 0u
+#define RC_LIGHT_SCATTERED 1u
+#define RC_SURFACE 2u
+#define GRIS_DIST_THRESHOLD 0.01f
+
+// gris_path_trace.glsl:45-306
+template <bool kCount>
+__global__ __launch_bounds__(kTileThreads) void k_gris_path_trace(DevScene S, rt_camera cam, GB g, rt_gris_settings set,
+                                                                  ResvPlanes outR, TraceCounters* ctr) {
+    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    PIXEL_INDEX
+    if (x >= g.W || y >= g.H) return;
+    TStack st{stk + threadIdx.x, kTileThreads};
+    f2 uv = pixelUV(x, y, g.W, g.H);
+    GPix gp;
+    if (!unpackGB(texDNCentre(g, x, y), fetchAM(g, (int)x, (int)y), gp)) return;  // reservoir not written
+    int matId = gp.matMeshId >> 16;
+    CamRay ray = pinholeRay(cam, mk2(uv.x, 1.0f - uv.y));
+    uint32_t rng = makeSeedPix(cam.seed, x, y);
+
+    f3 throughput = s3(1.0f), rcThroughput = s3(0.0f);
+    f3 wo = -ray.dir;
+    f3 lastPos = s3(0.0f);
+    bool isLastVertexConnectible = false;
+    uint32_t sampleState = 0, lastSampleState = 0;
+    Surf surf;
+    surf.pos = ray.ori + ray.dir * (gp.depth - 1e-4f);
+    surf.norm = gp.norm; surf.albedo = gp.albedo; surf.isLight = false; surf.matIndex = (uint32_t)matId;
+    Mat mat = loadMat(S, (uint32_t)matId);
+    BSDFSample s; s.type = 0; s.pdf = 0.0f; s.wi = s3(0.0f); s.bsdf = s3(0.0f);
+    Isec isec; isec.b1 = 0.0f; isec.b2 = 0.0f; isec.inst = 0u; isec.prim = 0u;
+    f3 rdir = ray.dir, rori = ray.ori;
+
+    Resv ps = zeroResv();  // the path sample (sampleCount/resampleWeight unused here)
+    ps.rcIsec.inst = RT_INVALID_HIT;
+    ps.primaryRng = rng;
+    Resv stream = zeroResv();
+    float streamWeight = 0.0f, streamSum = 0.0f;
+
+    for (uint32_t bounce = 0; bounce < set.maxDepth; bounce++) {
+        if (bounce > 0) {
+            isec = traceClosestHit<kCount>(S, st, rori, RT_MIN_RAY, rdir, RT_MAX_RAY, ctr);
+            if (isec.inst == RT_INVALID_HIT) break;
+            loadSurfaceInfo(S, isec, surf);
+            mat = loadMat(S, surf.matIndex);
+        }
+        flagsSetPathLength(ps.flags, bounce + 1);
+        float cosPrevWi = dot(rdir, surf.norm);
+        float distToPrev = distance(lastPos, surf.pos);
+        float geometryJacobian = fabsf(cosPrevWi) / square(distToPrev);
+        bool isThisVertexConnectible = surf.isLight || isBSDFConnectible(mat);
+        lastSampleState = sampleState;
+        sampleState = (sampleState == 2) ? 2u : (!isThisVertexConnectible ? 0u : sampleState + 1u);
+        if (set.shiftType == 0u && bounce == 1 && !surf.isLight) { sampleState = 2; lastSampleState = 1; }
+        float resvRandSample = sample1f(rng);
+
+        if (surf.isLight) {
+            if (bounce > 1 && cosPrevWi < 0.0f) {
+                float w = 1.0f;
+                float sumPower = S.lightTable[0].prob;
+                float lightPdf = luminance(surf.albedo) / sumPower / geometryJacobian;
+                if (!isSampleTypeDelta(s.type)) w = MISWeight(s.pdf, lightPdf);
+                f3 weightedLi = surf.albedo * w;
+                if (sampleState == 2 && lastSampleState == 2) {
+                    ps.rcLi = ps.rcLi + weightedLi * rcThroughput;
+                    ps.F = ps.F + weightedLi * throughput;
+                } else if ((sampleState == 2 && lastSampleState == 1) && isLastVertexConnectible &&
+                           distToPrev > GRIS_DIST_THRESHOLD) {
+                    ps.rcIsec = isec;
+                    ps.rcRng = rng;
+                    ps.rcPrevSamplePdf = s.pdf;
+                    ps.rcJacobian = geometryJacobian;
+                    ps.rcLi = weightedLi;
+                    ps.rcWi = s3(0.0f);
+                    ps.F = weightedLi * throughput;
+                    flagsSetRcVertexId(ps.flags, bounce);
+                    flagsSetRcVertexType(ps.flags, RC_LIGHT_SCATTERED);
+                    float wgt = luminance(ps.F);
+                    streamSum += wgt;
+                    if (resvRandSample * streamSum < wgt) { streamWeight = wgt; stream = ps; }
+                }
+            }
+            break;
+        }
+
+        bool connectible = isThisVertexConnectible && isLastVertexConnectible && distToPrev > GRIS_DIST_THRESHOLD;
+        if ((sampleState == 2 && lastSampleState == 1) && (connectible || set.shiftType == 0u)) {
+            ps.rcIsec = isec;
+            ps.rcRng = rng;
+            ps.rcPrevSamplePdf = s.pdf;
+            ps.rcJacobian = geometryJacobian;
+            flagsSetRcVertexId(ps.flags, bounce);
+            flagsSetRcVertexType(ps.flags, RC_SURFACE);
+            rcThroughput = s3(1.0f);
+        }
+
+        f4 lightRand = sample4f(rng);
+        resvRandSample = sample1f(rng);
+
+        if (bounce > 0 && !isBSDFDelta(mat)) {
+            f3 lightDir; float lightDist, lightPdf, lightJac; f2 lightBary; uint32_t lightId;
+            f3 Lr = sampleLightByPower(S, surf.pos, lightDir, lightDist, lightPdf, lightJac, lightBary, lightId, lightRand);
+            bool shadowed = traceShadow<kCount>(S, st, surf.pos, RT_MIN_RAY, lightDir, lightDist - RT_MIN_RAY, ctr);
+            if (!shadowed && lightPdf > 1e-6f) {
+                float bsdfPdf = absDot(surf.norm, lightDir) * RT_PI_INV;
+                float w = MISWeight(lightPdf, bsdfPdf);
+                f3 scatterTerm = evalBSDF(mat, surf.albedo, surf.norm, wo, lightDir) * satDot(surf.norm, lightDir);
+                f3 weightedLi = Lr / lightPdf * w;
+                if (sampleState == 2 && lastSampleState == 2) {
+                    ps.rcLi = ps.rcLi + weightedLi * scatterTerm * rcThroughput;
+                    ps.F = ps.F + weightedLi * scatterTerm * throughput;
+                } else if (sampleState == 2 && lastSampleState == 1) {
+                    ps.rcLi = weightedLi;
+                    ps.rcWi = lightDir;
+                    ps.F = weightedLi * scatterTerm * throughput;
+                    float wgt = luminance(ps.F);
+                    streamSum += wgt;
+                    if (resvRandSample * streamSum < wgt) { streamWeight = wgt; stream = ps; }
+                } else if (sampleState == 1 && isThisVertexConnectible && lightDist > GRIS_DIST_THRESHOLD) {
+                    ps.rcIsec.b1 = lightBary.x; ps.rcIsec.b2 = lightBary.y;
+                    ps.rcIsec.inst = 0u; ps.rcIsec.prim = lightId;
+                    ps.rcRng = rng;
+                    ps.rcPrevSamplePdf = lightPdf;
+                    ps.rcJacobian = lightJac;
+                    ps.rcLi = Lr * w;
+                    ps.rcWi = s3(0.0f);
+                    ps.F = weightedLi * scatterTerm * throughput;
+                    flagsSetRcVertexId(ps.flags, bounce + 1);
+                    flagsSetRcVertexType(ps.flags, RC_LIGHT_SAMPLED);
+                    float wgt = luminance(ps.F);
+                    streamSum += wgt;
+                    if (resvRandSample * streamSum < wgt) { streamWeight = wgt; stream = ps; }
+                }
+            }
+        }
+
+        if (bounce > 4) {
+            float pdfTerminate = gmax(1.0f - luminance(throughput) * set.rrScale, 0.0f);
+            if (sample1f(rng) < pdfTerminate) break;
+            throughput = throughput / (1.0f - pdfTerminate);
+            rcThroughput = rcThroughput / (1.0f - pdfTerminate);
+        }
+
+        if (!sampleBSDF(mat, surf.albedo, surf.norm, wo, sample3f(rng), s) || s.pdf < 1e-6f) break;
+        float cosTheta = isSampleTypeDelta(s.type) ? 1.0f : absDot(surf.norm, s.wi);
+        f3 scatterTerm = s.bsdf * cosTheta / s.pdf;
+        throughput = throughput * scatterTerm;
+        if (sampleState == 2 && lastSampleState == 2) {
+            rcThroughput = rcThroughput * scatterTerm;
+        } else if (sampleState == 2 && lastSampleState == 1) {
+            ps.rcLi = s3(0.0f);
+            ps.rcWi = s.wi;
+            ps.F = s3(0.0f);
+            rcThroughput = rcThroughput / s.pdf;
+        }
+        lastPos = surf.pos;
+        wo = -s.wi;
+        rdir = s.wi;
+        rori = surf.pos + rdir * 1e-4f;
+        isLastVertexConnectible = isThisVertexConnectible;
+    }
+
+    if (sampleState == 2 && lastSampleState == 2) {
+        float wgt = luminance(ps.F);
+        float r = sample1f(rng);
+        streamSum += wgt;
+        if (r * streamSum < wgt) { streamWeight = wgt; stream = ps; }
+    }
+    Resv out = stream;
+    out.resampleWeight = 0.0f;
+    if (streamSum > 0.0f && streamWeight > 0.0f) {
+        float k = streamSum / streamWeight;
+        out.F = out.F * k;
+        out.rcLi = out.rcLi * k;
+        out.resampleWeight = luminance(out.F);
+    } else {
+        out.rcIsec.inst = RT_INVALID_HIT;
+        out.F = s3(0.0f);
+    }
+    out.sampleCount = 1.0f;
+    storeResv(outR, (size_t)y * g.W + x, out);
+}
+
+// ---- gris_retrace.glsl:42-136 --------------------------------------------------------------------
+struct RcData { Isec rcPrevIsec; f3 rcPrevWo; f3 rcPrevThroughput; };
+
+template <bool kCount>
+RT_DEV void replayHybrid(const DevScene& S, TStack st, const rt_gris_settings& set, Isec isec, Surf surf, f3 rdir, f3 rori,
+                         uint32_t targetFlags, uint32_t rng, RcData& rc, TraceCounters* ctr) {
+    f3 throughput = s3(1.0f);
+    f3 wo = -rdir;
+    Mat mat = loadMat(S, surf.matIndex);
+    BSDFSample s;
+    rc.rcPrevThroughput = s3(0.0f);
+    rc.rcPrevWo = s3(0.0f);
+    rc.rcPrevIsec.b1 = 0.0f; rc.rcPrevIsec.b2 = 0.0f; rc.rcPrevIsec.prim = 0u;
+    rc.rcPrevIsec.inst = RT_INVALID_HIT;
+    uint32_t targetId = flagsRcVertexId(targetFlags);
+    if (targetId == 1) { rc.rcPrevIsec = isec; rc.rcPrevWo = wo; rc.rcPrevThroughput = throughput; return; }
+    for (uint32_t bounce = 0; bounce < set.maxDepth; bounce++) {
+        if (bounce > 0) {
+            isec = traceClosestHit<kCount>(S, st, rori, RT_MIN_RAY, rdir, RT_MAX_RAY, ctr);
+            if (isec.inst == RT_INVALID_HIT) break;
+            loadSurfaceInfo(S, isec, surf);
+            mat = loadMat(S, surf.matIndex);
+        }
+        bool conn = isBSDFConnectible(mat);
+        (void)sample1f(rng);
+        if (surf.isLight) break;
+        if (bounce == targetId - 1u) {
+            if (conn) { rc.rcPrevIsec = isec; rc.rcPrevWo = wo; rc.rcPrevThroughput = throughput; }
+            break;
+        }
+        (void)sample4f(rng);
+        (void)sample1f(rng);
+        if (bounce > 4) {
+            float pdfTerminate = gmax(1.0f - luminance(throughput) * set.rrScale, 0.0f);
+            if (sample1f(rng) < pdfTerminate) break;
+            throughput = throughput / (1.0f - pdfTerminate);
+        }
+        if (!sampleBSDF(mat, surf.albedo, surf.norm, wo, sample3f(rng), s) || s.pdf < 1e-6f) break;
+        float cosTheta = isSampleTypeDelta(s.type) ? 1.0f : absDot(surf.norm, s.wi);
+        throughput = throughput * (s.bsdf * cosTheta / s.pdf);
+        wo = -s.wi;
+        rdir = s.wi;
+        rori = surf.pos + rdir * 1e-4f;
+    }
+}
+
+// Reconnection through the rc vertex: the shading shared by
+// GRISReservoirReuseAndMerge (gris_retrace.glsl:188-213) and the final
+// shading of spatialReuse (gris_resample_spatial.glsl:114-125).
+RT_DEV f3 reconnectLi(const DevScene& S, const Resv& src, const Surf& rcPrevSurf, const Surf& rcSurf, f3 wi,
+                      const RcData& rc) {
+    Mat rcMat = loadMat(S, rcSurf.matIndex);
+    Mat rcPrevMat = loadMat(S, rcPrevSurf.matIndex);
+    f3 Li = src.rcLi;
+    uint32_t rcType = flagsRcVertexType(src.flags);
+    if (rcType == RC_SURFACE && length(src.rcWi) > 0.5f)
+        Li = Li * (evalBSDF(rcMat, rcSurf.albedo, rcSurf.norm, -wi, src.rcWi) * satDot(rcSurf.norm, src.rcWi));
+    Li = Li * (evalBSDF(rcPrevMat, rcPrevSurf.albedo, rcPrevSurf.norm, rc.rcPrevWo, wi) * satDot(rcPrevSurf.norm, wi));
+    Li = Li * rc.rcPrevThroughput;
+    Li = Li / src.rcPrevSamplePdf;
+    return Li;
+}
+
+// gris_retrace.glsl:138-236
+template <bool kCount>
+RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& set, Resv& dst, const Surf& dstPrimarySurf,
+                          const Isec& dstPrimaryIsec, f3 primDir, f3 primOri, Resv src, uint32_t& rng, TraceCounters* ctr) {
+    RcData rc;
+    Surf rcPrevSurf, rcSurf;
+    f3 wi = s3(0.0f), Li = s3(0.0f);
+    bool srcSampleValid = false;
+    float dstJacobian = 0.0f, jacobian = 0.0f, dstPHat = 0.0f, dstSamplePdf = 0.0f;
+    if (src.rcIsec.inst != RT_INVALID_HIT) {
+        replayHybrid<kCount>(S, st, set, dstPrimaryIsec, dstPrimarySurf, primDir, primOri, src.flags, src.primaryRng, rc, ctr);
+        if (rc.rcPrevIsec.inst != RT_INVALID_HIT) {
+            if (rc.rcPrevIsec.inst == RT_SPECIAL_HIT) rcPrevSurf = dstPrimarySurf;
+            else loadSurfaceInfo(S, rc.rcPrevIsec, rcPrevSurf);
+            loadSurfaceInfo(S, src.rcIsec, rcSurf);
+            Mat rcPrevMat = loadMat(S, rcPrevSurf.matIndex);
+            float dist = distance(rcPrevSurf.pos, rcSurf.pos);
+            wi = normalize(rcSurf.pos - rcPrevSurf.pos);
+            float cosTheta = -dot(rcSurf.norm, wi);
+            dstJacobian = fabsf(cosTheta) / square(dist);
+            jacobian = dstJacobian / src.rcJacobian;
+            if (dist > GRIS_DIST_THRESHOLD && cosTheta > 0.0f && !isnan(jacobian) && src.rcJacobian > 0.0f &&
+                isBSDFConnectible(rcPrevMat)) {
+                if (traceVisibility<kCount>(S, st, rcPrevSurf.pos, rcSurf.pos, ctr)) srcSampleValid = true;
+            }
+        }
+    }
+    if (srcSampleValid) {
+        uint32_t rcType = flagsRcVertexType(src.flags);
+        if (!isnan(src.rcPrevSamplePdf) && src.rcPrevSamplePdf > 1e-6f) {
+            Li = reconnectLi(S, src, rcPrevSurf, rcSurf, wi, rc);
+            if (!isBlack(Li) && !hasNan(Li)) dstPHat = luminance(Li * jacobian);
+            if (rcType == RC_LIGHT_SAMPLED) {
+                float sumPower = S.lightTable[0].prob;
+                dstSamplePdf = luminance(rcSurf.albedo) / sumPower / dstJacobian;
+            } else {
+                Mat rcPrevMat = loadMat(S, rcPrevSurf.matIndex);
+                dstSamplePdf = evalPdf(rcPrevMat, rcPrevSurf.norm, rc.rcPrevWo, wi);
+            }
+        }
+        float srcPHat = luminance(src.F);
+        src.rcJacobian = dstJacobian;
+        src.rcPrevSamplePdf = dstSamplePdf;
+        src.F = Li;
+        if (src.rcPrevSamplePdf < 1e-6f || isnan(src.rcPrevSamplePdf)) src.rcPrevSamplePdf = 0.0f;
+        src.resampleWeight *= dstPHat / srcPHat;
+    } else {
+        src.resampleWeight = 0.0f;
+    }
+    if (resvIsValid(src)) resvMerge(dst, src, sample1f(rng));
+    resvCap(dst, (float)set.cap);
+}
+
+RT_DEV Resv resvAt(const ResvPlanes& R, uint32_t W, uint32_t H, int x, int y) {
+    unsigned long long idx = (unsigned long long)W * (uint32_t)y + (uint32_t)x;
+    if (x < 0 || y < 0 || idx >= (unsigned long long)W * H) return zeroResv();
+    return loadResv(R, (size_t)idx);
+}
+
+// gris_resample_temporal.glsl:11-83
+template <bool kCount>
+__global__ __launch_bounds__(kTileThreads) void k_gris_temporal(DevScene S, rt_camera cam, rt_camera prevCam, GB g,
+                                                                GB gPrev, rt_gris_settings set, ResvPlanes thisR,
+                                                                ResvPlanes prevR, ResvPlanes tempR, TraceCounters* ctr) {
+    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    PIXEL_INDEX
+    if (x >= g.W || y >= g.H) return;
+    TStack st{stk + threadIdx.x, kTileThreads};
+    f2 uv = pixelUV(x, y, g.W, g.H);
+    GPix gp;
+    if (!unpackGB(texDNCentre(g, x, y), fetchAM(g, (int)x, (int)y), gp)) return;  // Temp not written
+    f2 motion = fetchMotion(g, (int)x, (int)y);
+    int matId = gp.matMeshId >> 16;
+    CamRay ray = pinholeRay(cam, mk2(uv.x, 1.0f - uv.y));
+    uint32_t rng = makeSeedPix(cam.seed, x, y) ^ 1u;
+    uint32_t resvRng = ~rng;
+    f3 pos = ray.ori + ray.dir * (gp.depth - 1e-4f);
+    size_t p = (size_t)y * g.W + x;
+    Resv resv = loadResv(thisR, p);
+    Isec dstPrimaryIsec; dstPrimaryIsec.inst = RT_SPECIAL_HIT; dstPrimaryIsec.b1 = uv.x; dstPrimaryIsec.b2 = uv.y;
+    dstPrimaryIsec.prim = 0u;
+    if (set.temporalReuse) {
+        Surf dstPrimarySurf; dstPrimarySurf.pos = pos; dstPrimarySurf.norm = gp.norm; dstPrimarySurf.albedo = gp.albedo;
+        dstPrimarySurf.matIndex = (uint32_t)matId; dstPrimarySurf.isLight = false;
+        if ((cam.frameIndex & 0x80000000u) == 0u) {
+            // findPreviousReservoir (gris_resample_temporal.glsl:11-34)
+            f2 puv = uv + motion;
+            // a non-finite reprojection (half overflow of the motion vector) is rejected
+            if (!(puv.x < 0.0f || puv.y < 0.0f || puv.x > 1.0f || puv.y > 1.0f || isnan(puv.x) || isnan(puv.y))) {
+                int px = (int)(puv.x * (float)cam.filmSize[0]);
+                int py = (int)(puv.y * (float)cam.filmSize[1]);
+                GPix pp;
+                if (unpackGB(texDN(gPrev, puv), fetchAM(gPrev, px, py), pp)) {
+                    CamRay pr = pinholeRay(prevCam, mk2(puv.x, 1.0f - puv.y));
+                    f3 posPrev = pr.ori + pr.dir * (pp.depth - 1e-4f);
+                    if (!(pp.matMeshId != gp.matMeshId || dot(pp.norm, gp.norm) < 0.95f || distance(pos, posPrev) > 0.5f)) {
+                        Resv tr = resvAt(prevR, cam.filmSize[0], cam.filmSize[1], px, py);
+                        if (resvIsValid(tr))
+                            reuseAndMerge<kCount>(S, st, set, resv, dstPrimarySurf, dstPrimaryIsec, ray.dir, ray.ori, tr,
+                                                  resvRng, ctr);
+                    }
+                }
+            }
+        }
+    }
+    if (!resvIsValid(resv)) resvReset(resv);
+    storeResv(tempR, p, resv);
+}
+
+// gris_resample_spatial.glsl:11-135 and gris_resample_spatial.comp accumulation
+template <bool kCount>
+__global__ __launch_bounds__(kTileThreads) void k_gris_spatial(DevScene S, rt_camera cam, GB g, rt_gris_settings set,
+                                                               ResvPlanes tempR, ResvPlanes thisR, float* indirectOut,
+                                                               TraceCounters* ctr) {
+    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    PIXEL_INDEX
+    if (x >= g.W || y >= g.H) return;
+    TStack st{stk + threadIdx.x, kTileThreads};
+    size_t p = (size_t)y * g.W + x;
+    f2 uv = pixelUV(x, y, g.W, g.H);
+    GPix gp;
+    f3 radiance = s3(0.0f);
+    if (unpackGB(texDNCentre(g, x, y), fetchAM(g, (int)x, (int)y), gp)) {
+        uint32_t rng = makeSeedPix(cam.seed, x, y) ^ 2u;
+        CamRay ray = pinholeRay(cam, mk2(uv.x, 1.0f - uv.y));
+        f3 pos = ray.ori + ray.dir * (gp.depth - 1e-4f);
+        int matId = gp.matMeshId >> 16;
+        Resv resv = loadResv(tempR, p);
+        Isec dstPrimaryIsec; dstPrimaryIsec.inst = RT_SPECIAL_HIT; dstPrimaryIsec.b1 = uv.x; dstPrimaryIsec.b2 = uv.y;
+        dstPrimaryIsec.prim = 0u;
+        Surf dstPrimarySurf; dstPrimarySurf.pos = pos; dstPrimarySurf.norm = gp.norm; dstPrimarySurf.albedo = gp.albedo;
+        dstPrimarySurf.matIndex = (uint32_t)matId; dstPrimarySurf.isLight = false;
+        if (set.spatialReuse) {
+            f2 texel = mk2(1.0f / (float)g.W, 1.0f / (float)g.H);
+            for (uint32_t i = 0; i < set.spatialNeighbors; i++) {
+                f2 dsk = toConcentricDisk(sample2f(rng));
+                f2 nb = uv + (dsk * set.spatialRadius) * texel;
+                // findNeighborReservoir (gris_resample_spatial.glsl:11-34)
+                if (nb.x < 0.0f || nb.y < 0.0f || nb.x > 1.0f || nb.y > 1.0f) continue;
+                int px = (int)(nb.x * (float)cam.filmSize[0]);
+                int py = (int)(nb.y * (float)cam.filmSize[1]);
+                GPix np;
+                if (!unpackGB(texDN(g, nb), fetchAM(g, px, py), np)) continue;
+                CamRay nr = pinholeRay(cam, mk2(nb.x, 1.0f - nb.y));
+                f3 posN = nr.ori + nr.dir * (np.depth - 1e-4f);
+                if (dot(np.norm, gp.norm) < 0.9f || distance(pos, posN) > 0.4f) continue;
+                Resv nres = resvAt(tempR, cam.filmSize[0], cam.filmSize[1], px, py);
+                if (resvIsValid(nres))
+                    reuseAndMerge<kCount>(S, st, set, resv, dstPrimarySurf, dstPrimaryIsec, ray.dir, ray.ori, nres, rng, ctr);
+            }
+        }
+        if (!resvIsValid(resv)) resvReset(resv);
+        storeResv(thisR, p, resv);
+        if (resvIsValid(resv) && resv.sampleCount > 0.0f && resv.rcIsec.inst != RT_INVALID_HIT) {
+            RcData rc;
+            replayHybrid<kCount>(S, st, set, dstPrimaryIsec, dstPrimarySurf, ray.dir, ray.ori, resv.flags, resv.primaryRng,
+                                 rc, ctr);
+            if (rc.rcPrevIsec.inst != RT_INVALID_HIT) {
+                Surf rcPrevSurf, rcSurf;
+                if (rc.rcPrevIsec.inst == RT_SPECIAL_HIT) rcPrevSurf = dstPrimarySurf;
+                else loadSurfaceInfo(S, rc.rcPrevIsec, rcPrevSurf);
+                loadSurfaceInfo(S, resv.rcIsec, rcSurf);
+                f3 wi = normalize(rcSurf.pos - rcPrevSurf.pos);
+                f3 Li = reconnectLi(S, resv, rcPrevSurf, rcSurf, wi, rc);
+                if (!isBlack(Li) && !hasNan(Li)) radiance = Li / luminance(Li) * resv.resampleWeight / resv.sampleCount;
+            }
+        }
+        radiance = clampColor(radiance);
+    }
+    accumulate(indirectOut, p, radiance, cam.frameIndex);
+}
+
+__global__ void k_resv_reset(ResvPlanes R) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R.n) return;
+    Resv r = zeroResv();
+    r.rcIsec.inst = RT_INVALID_HIT;
+    storeResv(R, i, r);
+}
+
+inline dim3 tileGrid(uint32_t W, uint32_t H) { return dim3((W + kTileX - 1) / kTileX, (H + kTileY - 1) / kTileY, 1); }
+inline GB makeGB(const float* dn, const uint32_t* am, const uint16_t* mv, uint32_t W, uint32_t H) {
+    GB g; g.dn = reinterpret_cast<const float4*>(dn); g.am = reinterpret_cast<const uint2*>(am);
+    g.mv = reinterpret_cast<const uint32_t*>(mv); g.W = W; g.H = H;
+    return g;
+}
+inline ResvPlanes planes(void* p, uint32_t W, uint32_t H) {
+    ResvPlanes r; r.p = reinterpret_cast<float4*>(p); r.n = (size_t)W * H; return r;
+}
+
+}  // namespace
+
+// ===================================================================================================
+// Launchers (called by the C ABI after argument checks)
+namespace rti {
+
+#define RT_DISPATCH(KERNEL, GRID, BLOCK, STREAM, ...)                                              \
+    do {                                                                                             \
+        if (gCountersEnabled) KERNEL<true><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__, counterBuffer()); \
+        else KERNEL<false><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__, nullptr);                        \
+    } while (0)
+
+hipError_t launchTraceRays(const rt_device_scene* s, const float* rays8, uint32_t n, int mode, int skipLights, float* t,
+                           float* bary2, uint32_t* inst, uint32_t* prim, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    dim3 grid((n + kTileThreads - 1) / kTileThreads);
+    RT_DISPATCH(k_trace_rays, grid, kTileThreads, stream, s->dev, rays8, n, mode, skipLights, t, bary2, inst, prim);
+    return hipGetLastError();
+}
+
+hipError_t launchGBuffer(const rt_device_scene* s, const rt_camera& cam, float* dn, uint32_t* am, uint16_t* mv,
+                         hipStream_t stream) {
+    RT_DISPATCH(k_gbuffer, tileGrid(cam.filmSize[0], cam.filmSize[1]), kTileThreads, stream, s->dev, cam,
+                reinterpret_cast<float4*>(dn), reinterpret_cast<uint2*>(am), reinterpret_cast<uint32_t*>(mv));
+    return hipGetLastError();
+}
+
+hipError_t launchPtPlain(const rt_device_scene* s, const rt_camera* cams, const rt_frame& f, uint32_t maxDepth,
+                         uint32_t which, hipStream_t stream) {
+    GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
+    RT_DISPATCH(k_pt_plain, tileGrid(f.width, f.height), kTileThreads, stream, s->dev, cams[0], g, maxDepth, which,
+                f.directOutput, f.indirectOutput);
+    return hipGetLastError();
+}
+
+hipError_t launchGrisPathTrace(const rt_device_scene* s, const rt_camera* cams, const rt_frame& f,
+                               const rt_gris_settings& set, hipStream_t stream) {
+    GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
+    RT_DISPATCH(k_gris_path_trace, tileGrid(f.width, f.height), kTileThreads, stream, s->dev, cams[0], g, set,
+                planes(f.grisReservoir, f.width, f.height));
+    return hipGetLastError();
+}
+
+hipError_t launchGrisTemporal(const rt_device_scene* s, const rt_camera* cams, const rt_frame& f,
+                              const rt_gris_settings& set, hipStream_t stream) {
+    GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
+    GB gp = makeGB(f.depthNormalPrev, f.albedoMatIdPrev, nullptr, f.width, f.height);
+    RT_DISPATCH(k_gris_temporal, tileGrid(f.width, f.height), kTileThreads, stream, s->dev, cams[0], cams[1], g, gp, set,
+                planes(f.grisReservoir, f.width, f.height), planes(f.grisReservoirPrev, f.width, f.height),
+                planes(f.grisReservoirTemp, f.width, f.height));
+    return hipGetLastError();
+}
+
+hipError_t launchGrisSpatial(const rt_device_scene* s, const rt_camera* cams, const rt_frame& f,
+                             const rt_gris_settings& set, hipStream_t stream) {
+    GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
+    RT_DISPATCH(k_gris_spatial, tileGrid(f.width, f.height), kTileThreads, stream, s->dev, cams[0], g, set,
+                planes(f.grisReservoirTemp, f.width, f.height), planes(f.grisReservoir, f.width, f.height),
+                f.indirectOutput);
+    return hipGetLastError();
+}
+
+hipError_t launchResvReset(void* resv, uint32_t W, uint32_t H, hipStream_t stream) {
+    ResvPlanes R = planes(resv, W, H);
+    if (R.n == 0) return hipSuccess;
+    k_resv_reset<<<(unsigned)((R.n + 255) / 256), 256, 0, stream>>>(R);
+    return hipGetLastError();
+}
+
+}  // namespace rti

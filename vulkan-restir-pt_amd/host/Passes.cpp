@@ -1,0 +1,109 @@
+// Render passes of the host layer over the C ABI.
+#include "Passes.h"
+
+namespace {
+template <class T>
+T* deviceAlloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+    if (hipMemset(p, 0, bytes) != hipSuccess) throw std::runtime_error("hipMemset failed");
+    return reinterpret_cast<T*>(p);
+}
+}  // namespace
+
+// ---- GBufferPass: src/GBufferPass.cpp:69-123 (createResource), :22-56 (render) ----
+GBufferPass::GBufferPass(uint32_t w, uint32_t h) : width(w), height(h) {
+    size_t n = (size_t)w * h;
+    for (int j = 0; j < 2; j++) {
+        depthNormal[j] = deviceAlloc<float>(n * DepthNormalBytes);
+        albedoMatId[j] = deviceAlloc<uint32_t>(n * AlbedoMatIdBytes);
+    }
+    motionVector = deviceAlloc<uint16_t>(n * MotionVectorBytes);
+}
+GBufferPass::~GBufferPass() {
+    for (int j = 0; j < 2; j++) { (void)hipFree(depthNormal[j]); (void)hipFree(albedoMatId[j]); }
+    (void)hipFree(motionVector);
+}
+void GBufferPass::render(hipStream_t stream, uint32_t curFrame, const DeviceScene& scene, const rt_camera& cam) {
+    rtThrow(rt_gbuffer_trace(scene.handle(), &cam, depthNormal[curFrame], albedoMatId[curFrame], motionVector, stream),
+            "GBufferPass::render");
+}
+
+// ---- RayTracing: src/RayTracing.cpp:137-147 ----
+void RayTracing::execute(hipStream_t stream, const DeviceScene& scene, const rt_camera cams[2], const rt_frame& frame,
+                         const void* pushConstant) {
+    if (mode != RayQuery || !mEntry) throw std::runtime_error("RayTracing::execute: pipeline not created");
+    rtThrow(mEntry(scene.handle(), cams, &frame, pushConstant, stream), "RayTracing::execute");
+}
+
+// ---- GRISReSTIR: src/GRISReSTIR.cpp:9-67 ----
+void GRISReSTIR::createPipeline() {
+    auto bind = [](rt_status (*fn)(const rt_device_scene*, const rt_camera*, const rt_frame*, const rt_gris_settings*,
+                                   hipStream_t)) {
+        return [fn](const rt_device_scene* s, const rt_camera* c, const rt_frame* f, const void* pc, hipStream_t st) {
+            return fn(s, c, f, static_cast<const rt_gris_settings*>(pc), st);
+        };
+    };
+    mPathTracePass.createPipeline(bind(&rt_gris_path_trace));
+    mTemporalReusePass.createPipeline(bind(&rt_gris_temporal));
+    mSpatialReusePass.createPipeline(bind(&rt_gris_spatial));
+}
+// Passes are stream ordered, which stands in for the reference's pipeline barriers.
+void GRISReSTIR::render(hipStream_t stream, const DeviceScene& scene, const rt_camera cams[2], const rt_frame& frame) {
+    mPathTracePass.execute(stream, scene, cams, frame, &settings);
+    mTemporalReusePass.execute(stream, scene, cams, frame, &settings);
+    mSpatialReusePass.execute(stream, scene, cams, frame, &settings);
+}
+
+// ---- Renderer: src/Renderer.cpp:193-257 (createRayImage), 319-353 (bindings), 400-569 ----
+Renderer::Renderer(const DeviceScene& scene, uint32_t w, uint32_t h)
+    : width(w), height(h), mScene(scene), mGBuffer(w, h) {
+    size_t n = (size_t)w * h;
+    directOutput = deviceAlloc<float>(n * 16);
+    indirectOutput = deviceAlloc<float>(n * 16);
+    size_t rb = rt_gris_reservoir_bytes(w, h);
+    for (int j = 0; j < 2; j++) {
+        grisReservoir[j] = deviceAlloc<uint8_t>(rb);
+        rtThrow(rt_gris_reservoir_reset(grisReservoir[j], w, h, 0), "rt_gris_reservoir_reset");
+    }
+    grisReservoirTemp = deviceAlloc<uint8_t>(rb);
+    rtThrow(rt_gris_reservoir_reset(grisReservoirTemp, w, h, 0), "rt_gris_reservoir_reset");
+    if (hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("hipDeviceSynchronize failed");
+    gris.createPipeline();
+    uint32_t* maxDepth = &maxDepthNaive;
+    mNaiveDI.createPipeline([maxDepth](const rt_device_scene* s, const rt_camera* c, const rt_frame* f, const void*,
+                                       hipStream_t st) { return rt_pt_plain(s, c, f, *maxDepth, 1u, st); });
+    mNaiveGI.createPipeline([maxDepth](const rt_device_scene* s, const rt_camera* c, const rt_frame* f, const void*,
+                                       hipStream_t st) { return rt_pt_plain(s, c, f, *maxDepth, 2u, st); });
+}
+Renderer::~Renderer() {
+    (void)hipFree(directOutput); (void)hipFree(indirectOutput);
+    (void)hipFree(grisReservoir[0]); (void)hipFree(grisReservoir[1]); (void)hipFree(grisReservoirTemp);
+}
+rt_frame Renderer::bindings() const {
+    rt_frame f{};
+    uint32_t thisFrame = mCurFrame, lastFrame = mCurFrame ^ 1u;
+    f.width = width; f.height = height;
+    f.directOutput = directOutput;
+    f.indirectOutput = indirectOutput;
+    f.depthNormal = mGBuffer.depthNormal[thisFrame];
+    f.depthNormalPrev = mGBuffer.depthNormal[lastFrame];
+    f.albedoMatId = mGBuffer.albedoMatId[thisFrame];
+    f.albedoMatIdPrev = mGBuffer.albedoMatId[lastFrame];
+    f.motionVector = mGBuffer.motionVector;
+    f.grisReservoir = grisReservoir[thisFrame];
+    f.grisReservoirPrev = grisReservoir[lastFrame];
+    f.grisReservoirTemp = grisReservoirTemp;
+    return f;
+}
+void Renderer::drawFrame(hipStream_t stream, const rt_camera cams[2], Method method) {
+    mGBuffer.render(stream, mCurFrame, mScene, cams[0]);
+    rt_frame f = bindings();
+    if (method == ResampledPT) {
+        gris.render(stream, mScene, cams, f);
+    } else {
+        mNaiveDI.execute(stream, mScene, cams, f);
+        mNaiveGI.execute(stream, mScene, cams, f);
+    }
+    flip();
+}

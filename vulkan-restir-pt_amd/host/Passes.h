@@ -1,0 +1,89 @@
+// Render passes of the host layer, keeping the reference's pass surface:
+//   GBufferPass  (src/GBufferPass.h:22-69)  images depthNormal[2],
+//                albedoMatId[2], motionVector; render() for a frame parity
+//   RayTracing   (src/RayTracing.h:27-48)   one per-pixel pass bound to a
+//                kernel entry of the C ABI; execute() with the frame bindings
+//                and a push-constant block
+//   GRISReSTIR   (src/GRISReSTIR.h:7-37)    Settings, createPipeline(),
+//                render() = path trace, temporal reuse, spatial reuse
+//   Renderer     headless stand-in for src/Renderer.cpp's frame loop: owns the
+//                ray images and reservoirs, binds the frame parity and flips it
+#pragma once
+#include "Scene.h"
+#include <functional>
+
+class GBufferPass {
+public:
+    static constexpr size_t DepthNormalBytes = 16, AlbedoMatIdBytes = 8, MotionVectorBytes = 4;
+
+    GBufferPass(uint32_t width, uint32_t height);
+    ~GBufferPass();
+    GBufferPass(const GBufferPass&) = delete;
+    GBufferPass& operator=(const GBufferPass&) = delete;
+
+    void render(hipStream_t stream, uint32_t curFrame, const DeviceScene& scene, const rt_camera& camera);
+
+    float* depthNormal[2] = {nullptr, nullptr};
+    uint32_t* albedoMatId[2] = {nullptr, nullptr};
+    uint16_t* motionVector = nullptr;
+    uint32_t width, height;
+};
+
+struct RayTracing {
+    enum Mode : uint32_t { RayQuery = 0, Undefined };
+    using Entry = std::function<rt_status(const rt_device_scene*, const rt_camera*, const rt_frame*, const void*, hipStream_t)>;
+
+    void createPipeline(Entry entry) { mEntry = std::move(entry); mode = mEntry ? RayQuery : Undefined; }
+    void execute(hipStream_t stream, const DeviceScene& scene, const rt_camera cams[2], const rt_frame& frame,
+                 const void* pushConstant = nullptr);
+
+    Mode mode = Undefined;
+
+private:
+    Entry mEntry;
+};
+
+class GRISReSTIR {
+public:
+    enum ShiftType : uint32_t { Reconnection = 0, Replay = 1, Hybrid = 2 };
+    using Settings = rt_gris_settings;
+
+    void createPipeline();
+    void render(hipStream_t stream, const DeviceScene& scene, const rt_camera cams[2], const rt_frame& frame);
+
+    // src/GRISReSTIR.h:29 {Hybrid, 1, false, true, 20}; then the kernel constants
+    // MaxTracingDepth = 15, ResampleNum = 3, ResampleRadius = 20
+    Settings settings = {Hybrid, 1.0f, 0u, 1u, 20u, 15u, 3u, 20.0f};
+
+private:
+    RayTracing mPathTracePass, mTemporalReusePass, mSpatialReusePass;
+};
+
+class Renderer {
+public:
+    enum Method : uint32_t { ResampledPT = 0, Naive = 1 };
+
+    Renderer(const DeviceScene& scene, uint32_t width, uint32_t height);
+    ~Renderer();
+    Renderer(const Renderer&) = delete;
+    Renderer& operator=(const Renderer&) = delete;
+
+    // G-buffer pass, then the indirect (and for Naive also direct) method; flips the parity.
+    void drawFrame(hipStream_t stream, const rt_camera cams[2], Method method);
+    rt_frame bindings() const;
+    void flip() { mCurFrame ^= 1u; }
+
+    GRISReSTIR gris;
+    uint32_t maxDepthNaive = 15;
+    float* directOutput = nullptr;
+    float* indirectOutput = nullptr;
+    void* grisReservoir[2] = {nullptr, nullptr};
+    void* grisReservoirTemp = nullptr;
+    uint32_t width, height;
+
+private:
+    const DeviceScene& mScene;
+    GBufferPass mGBuffer;
+    RayTracing mNaiveDI, mNaiveGI;
+    uint32_t mCurFrame = 0;
+};

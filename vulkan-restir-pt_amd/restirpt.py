@@ -1,0 +1,291 @@
+"""Python bindings (ctypes) of the MI355X ReSTIR PT core.
+
+Mirrors include/restirpt.h and include/restirpt_host.h. The structures below
+have the byte layouts of the reference's GLSL structs (src/shader/layouts.glsl)
+so numpy arrays can be handed to the C ABI without conversion.
+
+torch is imported before the HIP libraries are loaded: torch ships its own
+libamdhip64 with the same SONAME, and loading ours first would put two HIP
+runtimes in one process. Device memory for tests and benchmarks is torch
+tensors; the kernels themselves are in librestirpt.so and fail loudly when it
+is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+try:  # one HIP runtime per process: torch's, when torch is present
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    torch = None
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CORE_LIB = os.path.join(HERE, "librestirpt.so")
+HOST_LIB = os.path.join(HERE, "librestirpt_host.so")
+
+RT_OK = 0
+
+# ---------------------------------------------------------------------------------------
+# Structures (include/restirpt.h)
+
+
+class Material(C.Structure):
+    _fields_ = [("baseColor", C.c_float * 3), ("type", C.c_uint32), ("textureIdx", C.c_uint32),
+                ("metallic", C.c_float), ("roughness", C.c_float), ("ior", C.c_float)]
+
+
+class MeshVertex(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("uvx", C.c_float), ("norm", C.c_float * 3), ("uvy", C.c_float)]
+
+
+class ObjectInstance(C.Structure):
+    _fields_ = [("transform", C.c_float * 16), ("transformInv", C.c_float * 16), ("transformInvT", C.c_float * 16),
+                ("radiance", C.c_float * 3), ("pad0", C.c_float), ("indexOffset", C.c_uint32),
+                ("indexCount", C.c_uint32), ("matIndex", C.c_uint32), ("pad2", C.c_float)]
+
+
+class TriangleLight(C.Structure):
+    _fields_ = [("v0", C.c_float * 3), ("nx", C.c_float), ("v1", C.c_float * 3), ("ny", C.c_float),
+                ("v2", C.c_float * 3), ("nz", C.c_float), ("radiance", C.c_float * 3), ("area", C.c_float)]
+
+
+class LightTableEntry(C.Structure):
+    _fields_ = [("prob", C.c_float), ("failId", C.c_uint32)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("view", C.c_float * 16), ("proj", C.c_float * 16), ("projView", C.c_float * 16),
+                ("lastProjView", C.c_float * 16), ("pos", C.c_float * 3), ("FOV", C.c_float),
+                ("angle", C.c_float * 3), ("nearZ", C.c_float), ("front", C.c_float * 3), ("farZ", C.c_float),
+                ("right", C.c_float * 3), ("lensRadius", C.c_float), ("up", C.c_float * 3),
+                ("focalDist", C.c_float), ("filmSize", C.c_uint32 * 2), ("frameIndex", C.c_uint32),
+                ("seed", C.c_uint32)]
+
+
+class GrisReservoir(C.Structure):
+    _fields_ = [("bary", C.c_float * 2), ("instanceIdx", C.c_uint32), ("triangleIdx", C.c_uint32),
+                ("rcLi", C.c_float * 3), ("rcRng", C.c_uint32), ("rcWi", C.c_float * 3), ("flags", C.c_uint32),
+                ("pad", C.c_float * 2), ("rcPrevSamplePdf", C.c_float), ("rcJacobian", C.c_float),
+                ("F", C.c_float * 3), ("primaryRng", C.c_uint32), ("sampleCount", C.c_float),
+                ("resampleWeight", C.c_float), ("contribWeight", C.c_float), ("pad0", C.c_float)]
+
+
+class SceneArrays(C.Structure):
+    _fields_ = [("materials", C.c_void_p), ("numMaterials", C.c_uint32),
+                ("materialIndices", C.c_void_p), ("numMaterialIndices", C.c_uint32),
+                ("vertices", C.c_void_p), ("numVertices", C.c_uint32),
+                ("indices", C.c_void_p), ("numIndices", C.c_uint32),
+                ("instances", C.c_void_p), ("numInstances", C.c_uint32),
+                ("lights", C.c_void_p), ("numLights", C.c_uint32),
+                ("lightTable", C.c_void_p), ("numLightTable", C.c_uint32)]
+
+
+class GrisSettings(C.Structure):
+    _fields_ = [("shiftType", C.c_uint32), ("rrScale", C.c_float), ("temporalReuse", C.c_uint32),
+                ("spatialReuse", C.c_uint32), ("cap", C.c_uint32), ("maxDepth", C.c_uint32),
+                ("spatialNeighbors", C.c_uint32), ("spatialRadius", C.c_float)]
+
+
+class Frame(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("directOutput", C.c_void_p),
+                ("indirectOutput", C.c_void_p), ("depthNormal", C.c_void_p), ("depthNormalPrev", C.c_void_p),
+                ("albedoMatId", C.c_void_p), ("albedoMatIdPrev", C.c_void_p), ("motionVector", C.c_void_p),
+                ("grisReservoir", C.c_void_p), ("grisReservoirPrev", C.c_void_p), ("grisReservoirTemp", C.c_void_p)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("numPrimitives", C.c_uint32), ("numLights", C.c_uint32), ("bvhNodes", C.c_uint32),
+                ("bvhDepth", C.c_uint32), ("sceneLo", C.c_float * 3), ("sceneHi", C.c_float * 3),
+                ("deviceBytes", C.c_size_t)]
+
+
+assert C.sizeof(Material) == 32 and C.sizeof(MeshVertex) == 32 and C.sizeof(ObjectInstance) == 224
+assert C.sizeof(TriangleLight) == 64 and C.sizeof(Camera) == 352 and C.sizeof(GrisReservoir) == 96
+
+
+def settings(shift=2, rr_scale=1.0, temporal=False, spatial=True, cap=20, max_depth=15, neighbors=3, radius=20.0):
+    """GRISReSTIR::Settings (src/GRISReSTIR.h:29 defaults) + kernel constants."""
+    return GrisSettings(shift, rr_scale, int(temporal), int(spatial), cap, max_depth, neighbors, radius)
+
+
+# ---------------------------------------------------------------------------------------
+# Library loading
+
+CORE_SYMBOLS = [
+    "rt_last_error", "rt_device_scene_create", "rt_device_scene_destroy", "rt_device_scene_info", "rt_bvh_export",
+    "rt_trace_rays", "rt_gbuffer_trace", "rt_pt_plain", "rt_gris_path_trace", "rt_gris_temporal", "rt_gris_spatial",
+    "rt_gris_render", "rt_gris_reservoir_bytes", "rt_gris_reservoir_reset", "rt_gris_reservoir_download",
+    "rt_gris_reservoir_upload", "rt_counters_enable", "rt_counters_read",
+]
+HOST_SYMBOLS = [
+    "rt_scenegen", "rt_host_scene_destroy", "rt_host_scene_arrays", "rt_host_scene_camera", "rt_light_table_build",
+    "rt_renderer_create", "rt_renderer_destroy", "rt_renderer_set_settings", "rt_renderer_frame",
+    "rt_renderer_bindings", "rt_renderer_flip", "rt_renderer_read_output",
+]
+
+_core = None
+_host = None
+
+
+def _declare_core(L):
+    P, u32, i32 = C.c_void_p, C.c_uint32, C.c_int32
+    L.rt_last_error.restype = C.c_char_p
+    sigs = {
+        "rt_device_scene_create": [C.POINTER(SceneArrays), P, C.POINTER(P)],
+        "rt_device_scene_destroy": [P],
+        "rt_device_scene_info": [P, C.POINTER(SceneInfo)],
+        "rt_bvh_export": [P, P, P, P],
+        "rt_trace_rays": [P, P, u32, C.c_int, C.c_int, P, P, P, P, P],
+        "rt_gbuffer_trace": [P, C.POINTER(Camera), P, P, P, P],
+        "rt_pt_plain": [P, C.POINTER(Camera), C.POINTER(Frame), u32, u32, P],
+        "rt_gris_path_trace": [P, C.POINTER(Camera), C.POINTER(Frame), C.POINTER(GrisSettings), P],
+        "rt_gris_temporal": [P, C.POINTER(Camera), C.POINTER(Frame), C.POINTER(GrisSettings), P],
+        "rt_gris_spatial": [P, C.POINTER(Camera), C.POINTER(Frame), C.POINTER(GrisSettings), P],
+        "rt_gris_render": [P, C.POINTER(Camera), C.POINTER(Frame), C.POINTER(GrisSettings), P],
+        "rt_gris_reservoir_reset": [P, u32, u32, P],
+        "rt_gris_reservoir_download": [P, u32, u32, P, P],
+        "rt_gris_reservoir_upload": [P, u32, u32, P, P],
+        "rt_counters_enable": [C.c_int],
+        "rt_counters_read": [P, C.c_int],
+    }
+    for name, args in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = i32
+    L.rt_gris_reservoir_bytes.argtypes = [u32, u32]
+    L.rt_gris_reservoir_bytes.restype = C.c_size_t
+
+
+def _declare_host(L):
+    P, u32, i32 = C.c_void_p, C.c_uint32, C.c_int32
+    sigs = {
+        "rt_scenegen": [u32, u32, C.POINTER(P)],
+        "rt_host_scene_destroy": [P],
+        "rt_host_scene_arrays": [P, C.POINTER(SceneArrays)],
+        "rt_host_scene_camera": [P, u32, u32, u32, C.POINTER(Camera)],
+        "rt_light_table_build": [P, u32, P],
+        "rt_renderer_create": [P, u32, u32, C.POINTER(P)],
+        "rt_renderer_destroy": [P],
+        "rt_renderer_set_settings": [P, C.POINTER(GrisSettings)],
+        "rt_renderer_frame": [P, C.POINTER(Camera), u32, P],
+        "rt_renderer_bindings": [P, C.POINTER(Frame)],
+        "rt_renderer_flip": [P],
+        "rt_renderer_read_output": [P, u32, P, P],
+    }
+    for name, args in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = i32
+
+
+def core():
+    """librestirpt.so (HIP kernels + C ABI). Raises if it has not been built."""
+    global _core
+    if _core is None:
+        if not os.path.exists(CORE_LIB):
+            raise RuntimeError(f"{CORE_LIB} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        L = C.CDLL(CORE_LIB, mode=C.RTLD_GLOBAL)
+        _declare_core(L)
+        _core = L
+    return _core
+
+
+def host():
+    """librestirpt_host.so (C++ pass surface, scene generator, frame driver)."""
+    global _host
+    if _host is None:
+        core()
+        if not os.path.exists(HOST_LIB):
+            raise RuntimeError(f"{HOST_LIB} is missing: build it first")
+        L = C.CDLL(HOST_LIB, mode=C.RTLD_GLOBAL)
+        _declare_host(L)
+        _host = L
+    return _host
+
+
+def check(status, what="rt call"):
+    if status != RT_OK:
+        msg = core().rt_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed with status {status}: {msg}")
+
+
+# ---------------------------------------------------------------------------------------
+# Host scene (procedural workloads) with numpy views of its arrays
+
+
+def _view(ptr, count, ctype):
+    if count == 0:
+        return np.zeros(0, dtype=np.dtype(ctype))
+    arr = (ctype * count).from_address(ptr)
+    return np.ctypeslib.as_array(arr)
+
+
+class HostScene:
+    """Procedural scene from rt_scenegen (config 0 box room, 1 terrain, 2 floor + light)."""
+
+    def __init__(self, config: int, seed: int = 0):
+        H = host()
+        h = C.c_void_p()
+        check(H.rt_scenegen(config, seed, C.byref(h)), "rt_scenegen")
+        self.handle = h
+        self.arrays = SceneArrays()
+        check(H.rt_host_scene_arrays(h, C.byref(self.arrays)), "rt_host_scene_arrays")
+
+    def camera(self, frame: int, width: int, height: int):
+        cams = (Camera * 2)()
+        check(host().rt_host_scene_camera(self.handle, frame, width, height, cams), "rt_host_scene_camera")
+        return cams
+
+    def numpy(self):
+        a = self.arrays
+        return {
+            "materials": _view(a.materials, a.numMaterials, Material),
+            "materialIndices": _view(a.materialIndices, a.numMaterialIndices, C.c_int32),
+            "vertices": _view(a.vertices, a.numVertices, MeshVertex),
+            "indices": _view(a.indices, a.numIndices, C.c_uint32),
+            "instances": _view(a.instances, a.numInstances, ObjectInstance),
+            "lights": _view(a.lights, a.numLights, TriangleLight),
+            "lightTable": _view(a.lightTable, a.numLightTable, LightTableEntry),
+        }
+
+    def num_triangles(self):
+        n = self.arrays.numLights
+        inst = _view(self.arrays.instances, self.arrays.numInstances, ObjectInstance)
+        return n + int(sum(int(i["indexCount"]) // 3 for i in inst))
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _host is not None:
+            _host.rt_host_scene_destroy(self.handle)
+            self.handle = None
+
+
+class DeviceScene:
+    """rt_device_scene: buffers uploaded and LBVH built on the current device."""
+
+    def __init__(self, arrays: SceneArrays, stream=None):
+        L = core()
+        h = C.c_void_p()
+        check(L.rt_device_scene_create(C.byref(arrays), stream, C.byref(h)), "rt_device_scene_create")
+        self.handle = h
+
+    def info(self) -> SceneInfo:
+        i = SceneInfo()
+        check(core().rt_device_scene_info(self.handle, C.byref(i)), "rt_device_scene_info")
+        return i
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _core is not None:
+            _core.rt_device_scene_destroy(self.handle)
+            self.handle = None
+
+
+def ptr(t):
+    """Device (or host) pointer of a torch tensor / numpy array."""
+    if t is None:
+        return None
+    if isinstance(t, np.ndarray):
+        return t.ctypes.data
+    return t.data_ptr()
