@@ -1,0 +1,248 @@
+"""Benchmark: ReSTIR PT (GRIS) frames at 1080p on one MI355X.
+
+A step is one frame of the hot path over one 1920x1080 view: G-buffer
+(primary rays), GRIS initial path generation, temporal reuse and spatial reuse
+(hybrid shift, 3 neighbours in 30 px, M-cap 20, depth 6, 1 path/px) on the
+procedural ~1.11M-triangle terrain scene, following its seeded 16-frame camera
+walk. Scene, LBVH and frame buffers are resident in HBM before timing.
+
+Output: one JSON line with Mpaths/s (pixels x paths/px x frames / s), the
+roofline of the dominant kernel (algorithmic bytes per launch / measured
+launch time vs 8 TB/s HBM), and a CPU baseline (the oracle, timed on a bounded
+sample on the host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "vulkan-restir-pt_amd"))
+
+METRIC = "Mpaths/s @1080p ReSTIR PT; equal-time relMSE, learned vs plain"
+PEAK_HBM_GBS = 8000.0
+# Algorithmic HBM bytes per pixel per pass (per-pixel G-buffer / reservoir /
+# output traffic, 80-B reservoirs) and per traversal step.
+IO_BYTES = {"gbuffer": 28, "path_trace": 104, "temporal": 292, "spatial": 528}
+NODE_BYTES, TRI_BYTES = 64, 48
+PASSES = ["gbuffer", "path_trace", "temporal", "spatial"]
+KERNELS = {"gbuffer": "k_gbuffer", "path_trace": "k_gris_path_trace", "temporal": "k_gris_temporal",
+           "spatial": "k_gris_spatial"}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-frames", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-counters", action="store_true")
+    ap.add_argument("--only-kernel-loop", action="store_true", help="time frames only (for profilers)")
+    return ap.parse_args()
+
+
+def settings(rt, width):
+    return rt.settings(shift=2, rr_scale=1.0, temporal=True, spatial=True, cap=20, max_depth=6, neighbors=3,
+                       radius=30.0 * width / 1920.0)
+
+
+class FrameLoop:
+    """Runs frames through the C ABI on torch's current stream, with optional
+    events between the kernels of each frame."""
+
+    def __init__(self, rt, hs, ds, W, H, s):
+        import torch
+
+        self.rt, self.hs, self.W, self.H, self.s = rt, hs, W, H, s
+        self.ds = ds
+        H_ = rt.host()
+        r = C.c_void_p()
+        rt.check(H_.rt_renderer_create(ds.handle, W, H, C.byref(r)), "rt_renderer_create")
+        self.r = r
+        rt.check(H_.rt_renderer_set_settings(r, C.byref(s)), "settings")
+        self.stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        self.torch = torch
+
+    def frame(self, k, events=None):
+        rt, L, H_ = self.rt, self.rt.core(), self.rt.host()
+        cams = self.hs.camera(k % 16, self.W, self.H)
+        f = rt.Frame()
+        rt.check(H_.rt_renderer_bindings(self.r, C.byref(f)), "bindings")
+        ev = events
+        if ev: ev[0].record()
+        rt.check(L.rt_gbuffer_trace(self.ds.handle, C.byref(cams[0]), f.depthNormal, f.albedoMatId, f.motionVector,
+                                    self.stream), "gbuffer")
+        if ev: ev[1].record()
+        rt.check(L.rt_gris_path_trace(self.ds.handle, cams, C.byref(f), C.byref(self.s), self.stream), "path trace")
+        if ev: ev[2].record()
+        rt.check(L.rt_gris_temporal(self.ds.handle, cams, C.byref(f), C.byref(self.s), self.stream), "temporal")
+        if ev: ev[3].record()
+        rt.check(L.rt_gris_spatial(self.ds.handle, cams, C.byref(f), C.byref(self.s), self.stream), "spatial")
+        if ev: ev[4].record()
+        rt.check(H_.rt_renderer_flip(self.r), "flip")
+
+    def close(self):
+        self.rt.host().rt_renderer_destroy(self.r)
+
+
+def count_work(rt, hs, ds, W, H, s, warmup, steps):
+    """Traversal work (nodes, triangles, rays) per pass over the timed frames,
+    from the counting kernel builds on a fresh replay of the same frames."""
+    import numpy as np
+
+    L = rt.core()
+    loop = FrameLoop(rt, hs, ds, W, H, s)
+    rt.check(L.rt_counters_enable(1), "counters")
+    tot = {p: np.zeros(3, np.uint64) for p in PASSES}
+    buf = np.zeros(3, np.uint64)
+    try:
+        for k in range(warmup + steps):
+            cams = hs.camera(k % 16, W, H)
+            f = rt.Frame()
+            rt.check(rt.host().rt_renderer_bindings(loop.r, C.byref(f)), "bindings")
+            calls = [
+                lambda: L.rt_gbuffer_trace(ds.handle, C.byref(cams[0]), f.depthNormal, f.albedoMatId, f.motionVector,
+                                           loop.stream),
+                lambda: L.rt_gris_path_trace(ds.handle, cams, C.byref(f), C.byref(s), loop.stream),
+                lambda: L.rt_gris_temporal(ds.handle, cams, C.byref(f), C.byref(s), loop.stream),
+                lambda: L.rt_gris_spatial(ds.handle, cams, C.byref(f), C.byref(s), loop.stream),
+            ]
+            for p, call in zip(PASSES, calls):
+                rt.check(L.rt_counters_read(buf.ctypes.data, 1), "read")
+                rt.check(call(), p)
+                rt.check(L.rt_counters_read(buf.ctypes.data, 1), "read")
+                if k >= warmup:
+                    tot[p] += buf
+            rt.check(rt.host().rt_renderer_flip(loop.r), "flip")
+    finally:
+        L.rt_counters_enable(0)
+        loop.close()
+    return tot
+
+
+def cpu_baseline(rt, hs, args):
+    """The oracle (scalar C++ restatement) on the host cores over a bounded sample."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+
+    W, H = args.width, args.height
+    s = settings(rt, W)
+    osc = orc.OracleScene(hs.arrays)
+    orc.nthreads = lambda: args.cpu_threads
+    n = W * H
+    this = [orc.reset_reservoirs(n), orc.reset_reservoirs(n)]
+    temp = orc.reset_reservoirs(n)
+    out = np.zeros((H, W, 4), np.float32)
+    dns, ams = [None, None], [None, None]
+    cur = 0
+    t0 = time.perf_counter()
+    for k in range(args.cpu_frames):
+        cams = hs.camera(k % 16, W, H)
+        dn, am, mv, _ = osc.gbuffer(cams[0])
+        dns[cur], ams[cur] = dn, am
+        pdn = dns[cur ^ 1] if dns[cur ^ 1] is not None else np.zeros_like(dn)
+        pam = ams[cur ^ 1] if ams[cur ^ 1] is not None else np.zeros_like(am)
+        osc.gris_path_trace(cams, dn, am, s, this[cur])
+        osc.gris_temporal(cams, dn, am, mv, pdn, pam, s, this[cur], this[cur ^ 1], temp)
+        osc.gris_spatial(cams, dn, am, s, temp, this[cur], out)
+        cur ^= 1
+    dt = time.perf_counter() - t0
+    return {"value": round(n * args.cpu_frames / dt / 1e6, 4), "unit": "Mpaths/s", "cores": args.cpu_threads,
+            "kind": "port",
+            "sample": f"oracle (scalar C++ restatement of the reference GLSL) on {args.cpu_threads} host threads: "
+                      f"{args.cpu_frames} frames of the same pipeline (G-buffer + GRIS path trace + temporal + "
+                      f"spatial) at {W}x{H}, same scene, settings and camera walk (frames 0-"
+                      f"{args.cpu_frames - 1}), {dt:.1f} s"}
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_1080p_terrain_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].strip())
+            except ValueError:
+                return None
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs (--gpus > 1) are not part of it"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import restirpt as rt
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    W, H = args.width, args.height
+    hs = rt.HostScene(1, args.seed)
+    ds = rt.DeviceScene(hs.arrays)
+    info = ds.info()
+    s = settings(rt, W)
+    loop = FrameLoop(rt, hs, ds, W, H, s)
+    for k in range(args.warmup):
+        loop.frame(k)
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        loop.frame(args.warmup + k, None if args.only_kernel_loop else ev[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    loop.close()
+    ms_per_step = wall * 1e3 / args.steps
+    mpaths = W * H * args.steps / wall / 1e6
+    line = {"metric": METRIC, "value": round(mpaths, 3), "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "ReSTIR PT (GRIS) plain form, hybrid shift, temporal + spatial reuse "
+                                   "(3 neighbours, 30 px, M-cap 20), 1920x1080, 1 path/px, max depth 6, procedural "
+                                   "terrain (512x512-cell fBm heightfield, 4 tris/cell) + 200 instanced icospheres, "
+                                   "16 emissive triangles, seeded 16-frame camera walk",
+                       "width": W, "height": H, "triangles": int(info.numPrimitives), "emissive": 16,
+                       "max_depth": 6, "bvh_depth": int(info.bvhDepth)}}
+    if args.only_kernel_loop:
+        print(json.dumps(line))
+        return
+    pass_ms = {p: float(np.mean([ev[k][i].elapsed_time(ev[k][i + 1]) for k in range(args.steps)]))
+               for i, p in enumerate(PASSES)}
+    line["passes_ms"] = {p: round(v, 3) for p, v in pass_ms.items()}
+    if not args.no_counters:
+        work = count_work(rt, hs, ds, W, H, s, args.warmup, args.steps)
+        rays = sum(int(work[p][2]) for p in PASSES)
+        line["mrays_per_s"] = round(rays / wall / 1e6, 1)
+        dom = max(PASSES, key=lambda p: pass_ms[p])
+        nodes, tris = int(work[dom][0]), int(work[dom][1])
+        per_launch = (IO_BYTES[dom] * W * H * args.steps + NODE_BYTES * nodes + TRI_BYTES * tris) / args.steps
+        achieved = per_launch / (pass_ms[dom] * 1e-3) / 1e9
+        line["roofline"] = {"bound": "hbm", "kernel": KERNELS[dom], "achieved": round(achieved, 1),
+                            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                            "traffic": read_traffic(KERNELS[dom]),
+                            "algorithmic_bytes_per_launch": int(per_launch),
+                            "launch_ms": round(pass_ms[dom], 3)}
+        line["work_per_frame"] = {p: {"rays": int(work[p][2]) // args.steps, "nodes": int(work[p][0]) // args.steps,
+                                      "tris": int(work[p][1]) // args.steps} for p in PASSES}
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(rt, hs, args)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

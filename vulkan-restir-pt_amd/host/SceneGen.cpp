@@ -207,9 +207,9 @@ std::unique_ptr<Scene> terrain(uint32_t seed) {
     auto S = std::make_unique<Scene>();
     Pcg32 rng(seed, 1);
     const int N = 512;
-    const float extent = 100.0f, cell = extent / N, amp = 10.0f;
+    const float extent = 24.0f, cell = extent / N, amp = 2.4f;
     Noise noise{seed * 7919u + 13u};
-    auto height = [&](float x, float y) { return amp * noise.fbm(x * 0.03f + 100.0f, y * 0.03f + 100.0f, 6); };
+    auto height = [&](float x, float y) { return amp * noise.fbm(x * 0.125f + 100.0f, y * 0.125f + 100.0f, 6); };
     auto normalAt = [&](float x, float y) {
         float e = cell * 0.5f;
         float dx = (height(x + e, y) - height(x - e, y)) / (2 * e);
@@ -271,22 +271,22 @@ std::unique_ptr<Scene> terrain(uint32_t seed) {
         modelOff[k] = addModelSharedVerts(*S, vbase, ico.tri, mat);
     }
     for (int i = 0; i < 200; i++) {
-        float r = rng.uniform(0.5f, 2.5f);
-        float x = rng.uniform(-45.0f, 45.0f), y = rng.uniform(-45.0f, 45.0f);
+        float r = rng.uniform(0.12f, 0.6f);
+        float x = rng.uniform(-10.8f, 10.8f), y = rng.uniform(-10.8f, 10.8f);
         float z = height(x, y) + r * rng.uniform(0.2f, 1.0f);
         float m[16]; translateScale(m, {x, y, z}, r);
         addInstance(*S, modelOff[i % 8], (uint32_t)ico.tri.size(), m);
     }
     // 16 emissive triangles above the terrain, facing down, radiance log-uniform in [1, 100]
     for (int i = 0; i < 16; i++) {
-        float cx = rng.uniform(-40.0f, 40.0f), cy = rng.uniform(-40.0f, 40.0f);
-        float cz = height(cx, cy) + rng.uniform(8.0f, 20.0f);
-        float s = rng.uniform(1.0f, 3.0f), ang = rng.uniform(0.0f, 6.2831853f);
+        float cx = rng.uniform(-9.6f, 9.6f), cy = rng.uniform(-9.6f, 9.6f);
+        float cz = height(cx, cy) + rng.uniform(2.0f, 5.0f);
+        float s = rng.uniform(0.25f, 0.75f), ang = rng.uniform(0.0f, 6.2831853f);
         float L = std::exp(rng.uniform(0.0f, std::log(100.0f)));
         float p[3][3];
         for (int k = 0; k < 3; k++) {
             float a = ang + k * 2.0943951f;
-            p[k][0] = cx + s * std::cos(a); p[k][1] = cy + s * std::sin(a); p[k][2] = cz + rng.uniform(-0.3f, 0.3f);
+            p[k][0] = cx + s * std::cos(a); p[k][1] = cy + s * std::sin(a); p[k][2] = cz + rng.uniform(-0.07f, 0.07f);
         }
         float rad[3] = {L, L * rng.uniform(0.8f, 1.0f), L * rng.uniform(0.7f, 1.0f)};
         // counter-clockwise seen from above => normal up; swap to face down
@@ -296,15 +296,15 @@ std::unique_ptr<Scene> terrain(uint32_t seed) {
     S->fov = 60.0f;
     // 16-frame seeded small-rotation random walk
     Pcg32 cam(seed, 7);
-    Scene::PathKey k{{0.0f, -48.0f, 18.0f}, {0.0f, -14.0f, 0.0f}, 0};
+    Scene::PathKey k{{0.0f, -12.5f, 5.5f}, {0.0f, -22.0f, 0.0f}, 0};
     for (int f = 0; f < 64; f++) {
         k.seed = cam.next();
         S->cameraPath.push_back(k);
         k.angle.x += cam.uniform(-0.5f, 0.5f);
         k.angle.y += cam.uniform(-0.25f, 0.25f);
-        k.pos.x += cam.uniform(-0.05f, 0.05f);
-        k.pos.y += cam.uniform(-0.05f, 0.05f);
-        k.pos.z += cam.uniform(-0.05f, 0.05f);
+        k.pos.x += cam.uniform(-0.01f, 0.01f);
+        k.pos.y += cam.uniform(-0.01f, 0.01f);
+        k.pos.z += cam.uniform(-0.01f, 0.01f);
     }
     return S;
 }
