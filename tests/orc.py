@@ -77,9 +77,12 @@ class OracleScene:
         self.n = lib().orc_scene_num_prims(self.h)
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().orc_scene_destroy(self.h)
-            self.h = None
+        h, self.h = getattr(self, "h", None), None
+        if h and _lib is not None:
+            try:
+                _lib.orc_scene_destroy(h)
+            except Exception:
+                pass
 
     def bvh(self):
         n = self.n
