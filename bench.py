@@ -47,7 +47,23 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-counters", action="store_true")
     ap.add_argument("--only-kernel-loop", action="store_true", help="time frames only (for profilers)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1-GiB device copy (calibrates HBM byte counters)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """A plain device-to-device copy of exactly 1 GiB, three times: the reference
+    point for scaling FETCH_SIZE / WRITE_SIZE counter readings to bytes."""
+    import torch
+
+    n = 1 << 30
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n, "copies": 3}))
 
 
 def settings(rt, width):
@@ -191,6 +207,9 @@ def main():
     import restirpt as rt
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.copy_calibration:
+        copy_calibration()
+        return
     W, H = args.width, args.height
     hs = rt.HostScene(1, args.seed)
     ds = rt.DeviceScene(hs.arrays)
