@@ -107,6 +107,91 @@ def test_pt_plain_parity(scene):
         assert np.isfinite(g).all()
 
 
+def run_gris_frames(hs, ds, osc, W, H, s, frames, check_passes=True):
+    """Runs `frames` GRIS frames on the GPU and the oracle side by side; returns the
+    number of differing reservoirs / output pixels per pass and frame."""
+    run = gpuhelp.GrisRun(ds, W, H)
+    n = W * H
+    o_dn, o_am = [None, None], [None, None]
+    o_resv = [orc.reset_reservoirs(n), orc.reset_reservoirs(n)]
+    o_temp = orc.reset_reservoirs(n)
+    o_out = np.zeros((H, W, 4), np.float32)
+    cur = 0
+    report = []
+    for frame in range(frames):
+        print(f"[gpu] frame {frame} {W}x{H}", flush=True)
+        cams = hs.camera(frame, W, H)
+        run.gbuffer(cams[0])
+        dn, am, mv, _ = osc.gbuffer(cams[0])
+        g_dn, g_am, g_mv = run.gbuffer_host()
+        gb_bad = int((~gpuhelp.same_float(g_dn, dn)).any(-1).sum() + (g_am != am).any(-1).sum()
+                     + (g_mv != mv).any(-1).sum())
+        o_dn[cur], o_am[cur] = dn, am
+        prev = cur ^ 1
+        pdn = o_dn[prev] if o_dn[prev] is not None else np.zeros_like(dn)
+        pam = o_am[prev] if o_am[prev] is not None else np.zeros_like(am)
+        run.path_trace(cams, s)
+        osc.gris_path_trace(cams, dn, am, s, o_resv[cur])
+        pt_bad = int(gpuhelp.compare_reservoirs(run.reservoirs(run.resv[run.cur]), o_resv[cur]).sum())
+        run.temporal(cams, s)
+        osc.gris_temporal(cams, dn, am, mv, pdn, pam, s, o_resv[cur], o_resv[prev], o_temp)
+        tp_bad = int(gpuhelp.compare_reservoirs(run.reservoirs(run.temp), o_temp).sum())
+        run.spatial(cams, s)
+        osc.gris_spatial(cams, dn, am, s, o_temp, o_resv[cur], o_out)
+        sp_bad = int(gpuhelp.compare_reservoirs(run.reservoirs(run.resv[run.cur]), o_resv[cur]).sum())
+        out = run.output(1)
+        out_bad = int((~close(out, o_out).all(axis=-1)).sum())
+        report.append((gb_bad, pt_bad, tp_bad, sp_bad, out_bad))
+        run.flip()
+        cur ^= 1
+    return report
+
+
+def test_gris_terrain_1080p_parity():
+    """The benchmark workload itself: 1920x1080, 1.11M triangles, hybrid shift, temporal +
+    spatial (3 neighbours, 30 px, cap 20), depth 6 — two frames, every pass against the oracle."""
+    hs = rt.HostScene(1, 1)
+    ds = rt.DeviceScene(hs.arrays)
+    osc = orc.OracleScene(hs.arrays)
+    s = rt.settings(shift=2, temporal=True, spatial=True, cap=20, max_depth=6, neighbors=3, radius=30.0)
+    report = run_gris_frames(hs, ds, osc, 1920, 1080, s, 2)
+    print("[gpu] mismatches per frame (gbuffer, path trace, temporal, spatial, output):", report, flush=True)
+    assert all(sum(r) == 0 for r in report), report
+
+
+def test_pt_plain_boxroom_256():
+    """Plain PT with NEE+MIS on the box room at 256x256, depth 4 (the plain-PT workload)."""
+    hs = rt.HostScene(0, 0)
+    ds = rt.DeviceScene(hs.arrays)
+    osc = orc.OracleScene(hs.arrays)
+    W = H = 256
+    run = gpuhelp.GrisRun(ds, W, H)
+    cams = hs.camera(0, W, H)
+    run.gbuffer(cams[0])
+    run.plain(cams, 4)
+    dn, am, _ = run.gbuffer_host()
+    od, oi, _ = osc.pt_plain(cams[0], dn, am, 4)
+    for g, o in ((run.output(0), od), (run.output(1), oi)):
+        assert close(g, o).all()
+
+
+def test_gpu_frames_are_deterministic():
+    hs = rt.HostScene(1, 1)
+    ds = rt.DeviceScene(hs.arrays)
+    W, H = 640, 360
+    s = rt.settings(temporal=True, spatial=True, max_depth=6, radius=10.0)
+    outs = []
+    for _ in range(2):
+        run = gpuhelp.GrisRun(ds, W, H)
+        for frame in range(3):
+            out = run.frame(hs.camera(frame, W, H), s)
+            run.flip()
+        outs.append((out, run.reservoirs(run.resv[run.cur ^ 1])))
+    assert np.array_equal(outs[0][0].view(np.uint32), outs[1][0].view(np.uint32))
+    assert not gpuhelp.compare_reservoirs(outs[0][1], outs[1][1]).any()
+    assert np.isfinite(outs[0][0]).all() and (outs[0][0][..., :3] >= 0).all()
+
+
 @pytest.mark.parametrize("shift", [2, 0], ids=["hybrid", "reconnection"])
 def test_gris_sequence_parity(scene, shift):
     """Three frames of the full GRIS pipeline (temporal + spatial), pass by pass."""
