@@ -28,7 +28,9 @@ PEAK_HBM_GBS = 8000.0
 # Algorithmic HBM bytes per pixel per pass (per-pixel G-buffer / reservoir /
 # output traffic, 80-B reservoirs) and per traversal step.
 IO_BYTES = {"gbuffer": 28, "path_trace": 104, "temporal": 292, "spatial": 528}
-NODE_BYTES, TRI_BYTES = 64, 48
+# a 4-wide node visit reads 112 B (4 child boxes + 4 child references); a
+# triangle test reads 48 B
+NODE_BYTES, TRI_BYTES = 112, 48
 PASSES = ["gbuffer", "path_trace", "temporal", "spatial"]
 KERNELS = {"gbuffer": "k_gbuffer", "path_trace": "k_gris_path_trace", "temporal": "k_gris_temporal",
            "spatial": "k_gris_spatial"}

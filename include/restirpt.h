@@ -143,6 +143,8 @@ typedef struct rt_scene_info {
     uint32_t bvhDepth;         /* max root-to-leaf internal-node count */
     float sceneLo[3], sceneHi[3];
     size_t deviceBytes;
+    uint32_t bvh4Nodes;        /* nodes of the collapsed 4-wide tree that is traversed */
+    uint32_t bvh4Depth;
 } rt_scene_info;
 
 typedef struct rt_device_scene rt_device_scene;
@@ -162,6 +164,11 @@ rt_status rt_device_scene_info(const rt_device_scene* scene, rt_scene_info* out)
  * (left box min/max, right box min/max as float bits, left, right child;
  * bit 31 marks a leaf). Any pointer may be null. */
 rt_status rt_bvh_export(const rt_device_scene* scene, uint64_t* codes, uint32_t* order, uint32_t* nodes14);
+/* Copies the collapsed 4-wide tree that the ray queries traverse: 28 words per
+ * node (child boxes as lo[axis][slot], hi[axis][slot] float bits, then 4 child
+ * references; 0xffffffff = empty slot, bit 31 = leaf). rt_scene_info.bvh4Nodes
+ * gives the count. */
+rt_status rt_bvh4_export(const rt_device_scene* scene, uint32_t* nodes28);
 
 /* Ray queries of src/shader/ray_query.glsl:6-70 (traceClosestHit /
  * traceShadow) over device arrays: rays8 = {ox, oy, oz, dx, dy, dz, tmin, tmax}.

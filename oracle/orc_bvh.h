@@ -38,11 +38,18 @@ struct BVHNode {
 };
 static const uint32_t LeafFlag = 0x80000000u;
 
+struct BVH4Node {
+    float lo[3][4], hi[3][4];  // [axis][slot]
+    uint32_t child[4];
+};
+static const uint32_t EmptyRef = 0xffffffffu;
+
 struct BVH {
     std::vector<WorldTri> tris;           // primitive order
     std::vector<uint64_t> codes;          // sorted Morton codes
     std::vector<uint32_t> order;          // leaf i -> primitive id
     std::vector<BVHNode> nodes;           // N-1 internal nodes (root 0)
+    std::vector<BVH4Node> nodes4;         // collapsed 4-wide tree (root 0)
     float sceneLo[3], sceneHi[3];
 };
 
@@ -200,6 +207,82 @@ static inline void buildBVH(BVH& B) {
     }
 }
 
+// ---- 4-wide collapse ------------------------------------------------------------------
+// Every binary internal node n defines a candidate 4-wide node: start from its two
+// children and, while fewer than four, replace the internal child with the largest
+// box surface area 2(dx dy + dy dz + dz dx) (first such child on ties) by its two
+// children, in place (left child at its position, right child right after). The
+// 4-wide tree keeps the root and every internal child of a kept node; kept nodes are
+// numbered in increasing binary index. Empty slots hold the ref 0xffffffff and an
+// empty box (+inf, -inf).
+static inline float boxArea(const float* lo, const float* hi) {
+    float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+
+struct Slot { uint32_t ref; float lo[3], hi[3]; };
+
+static inline int collapseSlots(const BVH& B, uint32_t n, Slot out[4]) {
+    const BVHNode& nd = B.nodes[n];
+    int cnt = 2;
+    out[0].ref = nd.left; out[1].ref = nd.right;
+    for (int a = 0; a < 3; a++) {
+        out[0].lo[a] = nd.lmin[a]; out[0].hi[a] = nd.lmax[a];
+        out[1].lo[a] = nd.rmin[a]; out[1].hi[a] = nd.rmax[a];
+    }
+    while (cnt < 4) {
+        int best = -1; float bestA = -1.0f;
+        for (int k = 0; k < cnt; k++) {
+            if (out[k].ref & LeafFlag) continue;
+            float A = boxArea(out[k].lo, out[k].hi);
+            if (best < 0 || A > bestA) { best = k; bestA = A; }
+        }
+        if (best < 0) break;
+        const BVHNode& c = B.nodes[out[best].ref];
+        for (int k = cnt; k > best + 1; k--) out[k] = out[k - 1];
+        out[best].ref = c.left; out[best + 1].ref = c.right;
+        for (int a = 0; a < 3; a++) {
+            out[best].lo[a] = c.lmin[a]; out[best].hi[a] = c.lmax[a];
+            out[best + 1].lo[a] = c.rmin[a]; out[best + 1].hi[a] = c.rmax[a];
+        }
+        cnt++;
+    }
+    return cnt;
+}
+
+static inline void collapseBVH4(const BVH& B, std::vector<BVH4Node>& out) {
+    out.clear();
+    size_t nb = B.nodes.size();
+    if (nb == 0) return;
+    std::vector<uint8_t> keep(nb, 0);
+    std::vector<uint32_t> stack = {0};
+    keep[0] = 1;
+    while (!stack.empty()) {
+        uint32_t n = stack.back(); stack.pop_back();
+        Slot s[4]; int cnt = collapseSlots(B, n, s);
+        for (int k = 0; k < cnt; k++)
+            if (!(s[k].ref & LeafFlag) && !keep[s[k].ref]) { keep[s[k].ref] = 1; stack.push_back(s[k].ref); }
+    }
+    std::vector<uint32_t> newId(nb, EmptyRef);
+    uint32_t m = 0;
+    for (size_t i = 0; i < nb; i++) if (keep[i]) newId[i] = m++;
+    out.resize(m);
+    for (size_t i = 0; i < nb; i++) {
+        if (!keep[i]) continue;
+        Slot s[4]; int cnt = collapseSlots(B, (uint32_t)i, s);
+        BVH4Node& o = out[newId[i]];
+        for (int k = 0; k < 4; k++) {
+            if (k < cnt) {
+                o.child[k] = (s[k].ref & LeafFlag) ? s[k].ref : newId[s[k].ref];
+                for (int a = 0; a < 3; a++) { o.lo[a][k] = s[k].lo[a]; o.hi[a][k] = s[k].hi[a]; }
+            } else {
+                o.child[k] = EmptyRef;
+                for (int a = 0; a < 3; a++) { o.lo[a][k] = INFINITY; o.hi[a][k] = -INFINITY; }
+            }
+        }
+    }
+}
+
 // ---- ray / box, ray / triangle -------------------------------------------------
 struct RayIn { vec3 o, d, invd; float tmin, tmax; };
 static inline RayIn makeRayIn(vec3 o, vec3 d, float tmin, float tmax) {
@@ -342,6 +425,77 @@ static inline bool traceAny(const BVH& B, const RayIn& r, TraceStats* st) {
         node = next;
     }
     return false;
+}
+
+// 4-wide traversal. At each node the hit children are ordered by entry distance
+// (slot order on ties); leaves are tested in that order as they come, the nearest
+// internal child is visited next and the other internal children are pushed
+// farthest first.
+template <bool kAny>
+static inline bool traverse4(const BVH& B, const RayIn& r, bool skipLights, float& tBest, Hit& h, TraceStats* st) {
+    if (B.nodes4.empty()) return false;
+    uint32_t stack[512]; int sp = 0;
+    uint32_t node = 0;
+    bool found = false;
+    for (;;) {
+        const BVH4Node& nd = B.nodes4[node];
+        if (st) st->nodes++;
+        int ord[4]; float tk[4]; int nh = 0;
+        for (int k = 0; k < 4; k++) {
+            if (nd.child[k] == EmptyRef) continue;
+            float lo[3] = {nd.lo[0][k], nd.lo[1][k], nd.lo[2][k]}, hi[3] = {nd.hi[0][k], nd.hi[1][k], nd.hi[2][k]};
+            float te;
+            if (!boxHit(r, lo, hi, tBest, te)) continue;
+            int j = nh++;
+            while (j > 0 && tk[j - 1] > te) { tk[j] = tk[j - 1]; ord[j] = ord[j - 1]; j--; }
+            tk[j] = te; ord[j] = k;
+        }
+        uint32_t next = EmptyRef;
+        uint32_t pend[3]; int np = 0;
+        for (int q = 0; q < nh; q++) {
+            uint32_t ref = nd.child[ord[q]];
+            if (ref & LeafFlag) {
+                const WorldTri& tri = B.tris[B.order[ref & ~LeafFlag]];
+                if (skipLights && tri.inst == 0) continue;
+                if (st) st->tris++;
+                float t, b1, b2;
+                if (triHit(r, tri, tBest, t, b1, b2)) {
+                    found = true;
+                    if (kAny) return true;
+                    tBest = t; h.t = t;
+                    h.isec.bary[0] = b1; h.isec.bary[1] = b2; h.isec.instanceIdx = tri.inst; h.isec.triangleIdx = tri.prim;
+                }
+            } else if (next == EmptyRef) {
+                next = ref;
+            } else {
+                pend[np++] = ref;
+            }
+        }
+        while (np > 0) stack[sp++] = pend[--np];
+        if (next == EmptyRef) {
+            if (sp == 0) break;
+            next = stack[--sp];
+        }
+        node = next;
+    }
+    return found;
+}
+
+static inline Hit traceClosest4(const BVH& B, const RayIn& r, bool skipLights, TraceStats* st) {
+    Hit h; h.hit = false; h.t = r.tmax;
+    h.isec.instanceIdx = InvalidHitIndex; h.isec.triangleIdx = 0; h.isec.bary[0] = h.isec.bary[1] = 0.0f;
+    float tBest = r.tmax;
+    bool hit = traverse4<false>(B, r, skipLights, tBest, h, st);
+    if (st) st->rays++;
+    h.hit = hit;
+    if (!hit) { h.isec.instanceIdx = InvalidHitIndex; h.isec.triangleIdx = 0; h.isec.bary[0] = h.isec.bary[1] = 0.0f; }
+    return h;
+}
+static inline bool traceAny4(const BVH& B, const RayIn& r, TraceStats* st) {
+    Hit h; float tBest = r.tmax;
+    bool hit = traverse4<true>(B, r, false, tBest, h, st);
+    if (st) st->rays++;
+    return hit;
 }
 
 // Brute force over all primitives in primitive order (parity reference for

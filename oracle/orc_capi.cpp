@@ -75,7 +75,15 @@ void* orc_scene_create(const SceneArrays* in) {
             s->lights.data(), (uint32_t)s->lights.size(), s->table.data(), (uint32_t)s->table.size()};
     buildWorldTris(s->S, s->bvh.tris);
     buildBVH(s->bvh);
+    collapseBVH4(s->bvh, s->bvh.nodes4);
     return s;
+}
+
+// 4-wide nodes as 28 words each: lo[3][4], hi[3][4] (float bits), child[4].
+uint32_t orc_bvh4_num_nodes(void* h) { return (uint32_t)((OScene*)h)->bvh.nodes4.size(); }
+void orc_bvh4_export(void* h, uint32_t* nodes28) {
+    const BVH& B = ((OScene*)h)->bvh;
+    for (size_t i = 0; i < B.nodes4.size(); i++) std::memcpy(nodes28 + 28 * i, &B.nodes4[i], 112);
 }
 void orc_scene_destroy(void* h) { delete (OScene*)h; }
 uint32_t orc_scene_num_prims(void* h) { return (uint32_t)((OScene*)h)->bvh.tris.size(); }
@@ -103,7 +111,8 @@ void orc_bvh_export(void* h, uint64_t* codes, uint32_t* order, uint32_t* nodes14
 }
 uint32_t orc_bvh_num_nodes(void* h) { return (uint32_t)((OScene*)h)->bvh.nodes.size(); }
 
-// rays8: ox oy oz dx dy dz tmin tmax. mode 0 closest, 1 any, 2 brute force closest.
+// rays8: ox oy oz dx dy dz tmin tmax. mode 0 closest, 1 any (4-wide tree), 2 brute force
+// closest, 3 closest and 4 any over the binary tree.
 // out: t, bary(2), inst, prim per ray (inst = 0xffffffff on miss; any-hit: inst = hit?1:0)
 void orc_trace_rays(void* h, const float* rays8, uint32_t n, int mode, int skipLights, float* outT, float* outBary,
                     uint32_t* outInst, uint32_t* outPrim, uint64_t* stats3, int nthreads) {
@@ -115,12 +124,14 @@ void orc_trace_rays(void* h, const float* rays8, uint32_t n, int mode, int skipL
         for (uint32_t i = r * rowsPer; i < n && i < (r + 1) * rowsPer; i++) {
             const float* q = rays8 + 8 * (size_t)i;
             RayIn ray = makeRayIn(V3(q[0], q[1], q[2]), V3(q[3], q[4], q[5]), q[6], q[7]);
-            if (mode == 1) {
-                bool hit = traceAny(s->bvh, ray, &st[t]);
+            if (mode == 1 || mode == 4) {
+                bool hit = mode == 1 ? traceAny4(s->bvh, ray, &st[t]) : traceAny(s->bvh, ray, &st[t]);
                 outInst[i] = hit ? 1u : 0u; outT[i] = 0; outBary[2 * i] = outBary[2 * i + 1] = 0; outPrim[i] = 0;
                 continue;
             }
-            Hit hh = (mode == 2) ? traceBrute(s->bvh, ray, skipLights != 0) : traceClosest(s->bvh, ray, skipLights != 0, &st[t]);
+            Hit hh = (mode == 2)   ? traceBrute(s->bvh, ray, skipLights != 0)
+                     : (mode == 3) ? traceClosest(s->bvh, ray, skipLights != 0, &st[t])
+                                   : traceClosest4(s->bvh, ray, skipLights != 0, &st[t]);
             outT[i] = hh.hit ? hh.t : 0.0f;
             outBary[2 * i] = hh.isec.bary[0]; outBary[2 * i + 1] = hh.isec.bary[1];
             outInst[i] = hh.isec.instanceIdx; outPrim[i] = hh.isec.triangleIdx;

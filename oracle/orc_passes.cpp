@@ -12,7 +12,7 @@ void gbufferPixel(const Ctx& c, const Camera& cam, uint32_t x, uint32_t y, float
     const uint32_t W = cam.filmSize[0], H = cam.filmSize[1];
     vec2 uv = V2(((float)x + 0.5f) / (float)W, ((float)y + 0.5f) / (float)H);
     Ray ray = pinholeCameraSampleRay(cam, V2(uv.x, 1.0f - uv.y), V2(0, 0));
-    Hit h = traceClosest(*c.B, makeRayIn(ray.ori, ray.dir, cam.nearZ, cam.farZ), true, c.st);
+    Hit h = traceClosest4(*c.B, makeRayIn(ray.ori, ray.dir, cam.nearZ, cam.farZ), true, c.st);
     if (!h.hit) {
         dn[0] = dn[1] = dn[2] = dn[3] = 0.0f; am[0] = am[1] = 0; mv[0] = mv[1] = 0;
         return;

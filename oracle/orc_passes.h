@@ -123,10 +123,10 @@ static inline bool unpackGBuffer(vec4 dn, const uint32_t am[2], float& depth, ve
 
 // ---- traversal wrappers (ray_query.glsl:6-70) ----------------------------------------
 static inline Intersection traceClosestHit(const Ctx& c, vec3 o, float tmin, vec3 d, float tmax) {
-    return traceClosest(*c.B, makeRayIn(o, d, tmin, tmax), false, c.st).isec;
+    return traceClosest4(*c.B, makeRayIn(o, d, tmin, tmax), false, c.st).isec;
 }
 static inline bool traceShadow(const Ctx& c, vec3 o, float tmin, vec3 d, float tmax) {
-    return traceAny(*c.B, makeRayIn(o, d, tmin, tmax), c.st);
+    return traceAny4(*c.B, makeRayIn(o, d, tmin, tmax), c.st);
 }
 static inline bool traceVisibility(const Ctx& c, vec3 from, vec3 to) {
     return !traceShadow(c, from, MinRayDistance, normalize(to - from), distance(to, from) - MinRayDistance);

@@ -146,6 +146,13 @@ def gpu_bvh(ds):
     return codes, order, nodes, info
 
 
+def gpu_bvh4(ds):
+    info = ds.info()
+    nodes = np.zeros((max(1, info.bvh4Nodes), 28), np.uint32)
+    rt.check(rt.core().rt_bvh4_export(ds.handle, nodes.ctypes.data), "export4")
+    return nodes[: info.bvh4Nodes]
+
+
 def compare_reservoirs(a, b):
     """Mask of pixels whose reservoir fields differ (floats compared by value, NaN == NaN)."""
     bad = np.zeros(a.shape[0], bool)

@@ -37,6 +37,9 @@ def lib():
         L.orc_bvh_export.argtypes = [P, P, P, P, P]
         L.orc_bvh_num_nodes.argtypes = [P]
         L.orc_bvh_num_nodes.restype = u32
+        L.orc_bvh4_num_nodes.argtypes = [P]
+        L.orc_bvh4_num_nodes.restype = u32
+        L.orc_bvh4_export.argtypes = [P, P]
         L.orc_trace_rays.argtypes = [P, P, u32, C.c_int, C.c_int, P, P, P, P, P, C.c_int]
         L.orc_gbuffer.argtypes = [P, C.POINTER(rt.Camera), P, P, P, P, C.c_int]
         L.orc_pt_plain.argtypes = [P, C.POINTER(rt.Camera), P, P, u32, P, P, P, C.c_int]
@@ -93,6 +96,12 @@ class OracleScene:
         box = np.zeros(6, np.float32)
         lib().orc_bvh_export(self.h, p(codes), p(order), p(nodes), p(box))
         return codes, order, nodes, box
+
+    def bvh4(self):
+        nn = lib().orc_bvh4_num_nodes(self.h)
+        nodes = np.zeros((nn, 28), np.uint32)
+        lib().orc_bvh4_export(self.h, p(nodes))
+        return nodes
 
     def world_tris(self):
         pos = np.zeros((self.n, 9), np.float32)

@@ -45,7 +45,8 @@ def test_lbvh_bitexact(scene):
     assert np.array_equal(order, oorder), "radix sort order differs"
     assert np.array_equal(nodes, onodes), "LBVH nodes differ"
     assert np.array_equal(np.array(info.sceneLo, np.float32), obox[:3])
-    assert info.bvhDepth <= 112
+    assert np.array_equal(gpuhelp.gpu_bvh4(ds), osc.bvh4()), "4-wide nodes differ"
+    assert 3 * info.bvh4Depth <= 256
 
 
 def test_trace_rays_bitexact(scene):

@@ -77,6 +77,55 @@ def test_lbvh_structure(boxroom):
     assert (seen_node == 1).all(), "every internal node has exactly one parent"
 
 
+def test_bvh4_collapse_structure(boxroom):
+    """The traversed 4-wide tree covers every leaf once, its child boxes are the exact
+    unions of their contents, and its closest hits equal the binary tree's."""
+    hs, osc = boxroom
+    codes, order, nodes, box = osc.bvh()
+    n4 = osc.bvh4()
+    assert 0 < len(n4) < len(nodes)
+    pos, _ = osc.world_tris()
+    tri_lo = pos.reshape(-1, 3, 3).min(1)
+    tri_hi = pos.reshape(-1, 3, 3).max(1)
+    f = n4[:, :24].view(np.float32).reshape(-1, 2, 3, 4)  # [node][lo/hi][axis][slot]
+    child = n4[:, 24:28]
+    leaves = np.zeros(len(order), int)
+    parents = np.zeros(len(n4), int)
+    parents[0] = 1
+
+    def contents(ref):
+        if int(ref) & LEAF:
+            p = order[int(ref) & 0x7FFFFFFF]
+            return tri_lo[p], tri_hi[p]
+        lo = np.min(np.where(child[ref] != 0xFFFFFFFF, f[ref, 0], np.inf), axis=1)
+        hi = np.max(np.where(child[ref] != 0xFFFFFFFF, f[ref, 1], -np.inf), axis=1)
+        return lo, hi
+
+    for i in range(len(n4)):
+        for k in range(4):
+            ref = child[i, k]
+            if ref == 0xFFFFFFFF:
+                assert np.isinf(f[i, :, :, k]).all()
+                continue
+            if int(ref) & LEAF:
+                leaves[int(ref) & 0x7FFFFFFF] += 1
+            else:
+                parents[ref] += 1
+            lo, hi = contents(ref)
+            assert np.array_equal(f[i, 0, :, k], lo) and np.array_equal(f[i, 1, :, k], hi)
+    assert (leaves == 1).all() and (parents == 1).all()
+    g = np.random.default_rng(9)
+    rays = np.zeros((20000, 8), np.float32)
+    rays[:, :3] = g.uniform([-0.9, -0.9, 0.1], [0.9, 0.9, 1.9], (20000, 3))
+    d = g.normal(size=(20000, 3))
+    rays[:, 3:6] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 6], rays[:, 7] = 1e-4, 1e7
+    t4, _, i4, _, s4 = osc.trace(rays, 0)
+    t2, _, i2, _, s2 = osc.trace(rays, 3)
+    assert np.array_equal(t4, t2)
+    assert s4[1] < 0.75 * s2[1], "the 4-wide tree should visit far fewer nodes"
+
+
 def test_closest_hit_matches_brute_force(boxroom):
     hs, osc = boxroom
     g = np.random.default_rng(5)

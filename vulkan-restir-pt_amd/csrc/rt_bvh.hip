@@ -311,9 +311,137 @@ __global__ void k_depth(const uint32_t* __restrict__ parentI, const uint32_t* __
     atomicMax(maxDepth, d);
 }
 
+// 7. collapse to a 4-wide tree. A binary node's candidate slots: its two
+// children, then twice (while fewer than four) the internal slot with the
+// largest surface area is replaced in place by its two children.
+struct Slot4 { uint32_t ref; float lo[3], hi[3]; };
+
+__device__ __forceinline__ float boxArea(const float* lo, const float* hi) {
+    float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+__device__ __forceinline__ void readSlots(const float4* nodes, uint32_t n, Slot4& a, Slot4& b) {
+    const float* f = reinterpret_cast<const float*>(nodes + 4 * (size_t)n);
+    a.ref = __float_as_uint(f[12]); b.ref = __float_as_uint(f[13]);
+    for (int k = 0; k < 3; k++) { a.lo[k] = f[k]; a.hi[k] = f[3 + k]; b.lo[k] = f[6 + k]; b.hi[k] = f[9 + k]; }
+}
+__device__ int collapseSlots(const float4* nodes, uint32_t n, Slot4 s[4]) {
+    readSlots(nodes, n, s[0], s[1]);
+    int cnt = 2;
+#pragma unroll
+    for (int step = 0; step < 2; step++) {
+        int best = -1; float bestA = -1.0f;
+        for (int k = 0; k < cnt; k++) {
+            if (s[k].ref & RT_LEAF) continue;
+            float A = boxArea(s[k].lo, s[k].hi);
+            if (best < 0 || A > bestA) { best = k; bestA = A; }
+        }
+        if (best < 0) break;
+        Slot4 l, r;
+        readSlots(nodes, s[best].ref, l, r);
+        for (int k = cnt; k > best + 1; k--) s[k] = s[k - 1];
+        s[best] = l; s[best + 1] = r;
+        cnt++;
+    }
+    return cnt;
+}
+
+// frontier expansion: marks the internal slots of every frontier node
+__global__ void k_collapse_mark(const float4* __restrict__ nodes, const uint32_t* __restrict__ frontier, uint32_t nf,
+                                uint32_t* __restrict__ keep, uint32_t* __restrict__ next, uint32_t* __restrict__ nNext) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nf) return;
+    Slot4 s[4];
+    int cnt = collapseSlots(nodes, frontier[i], s);
+    for (int k = 0; k < cnt; k++) {
+        if (s[k].ref & RT_LEAF) continue;
+        keep[s[k].ref] = 1u;
+        next[atomicAdd(nNext, 1u)] = s[k].ref;
+    }
+}
+
+__global__ void k_collapse_emit(const float4* __restrict__ nodes, uint32_t nb, const uint32_t* __restrict__ keep,
+                                const uint32_t* __restrict__ newId, float4* __restrict__ nodes4) {
+    uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nb || !keep[n]) return;
+    Slot4 s[4];
+    int cnt = collapseSlots(nodes, n, s);
+    float lo[3][4], hi[3][4];
+    uint32_t ch[4];
+    for (int k = 0; k < 4; k++) {
+        if (k < cnt) {
+            ch[k] = (s[k].ref & RT_LEAF) ? s[k].ref : newId[s[k].ref];
+            for (int a = 0; a < 3; a++) { lo[a][k] = s[k].lo[a]; hi[a][k] = s[k].hi[a]; }
+        } else {
+            ch[k] = 0xffffffffu;
+            for (int a = 0; a < 3; a++) { lo[a][k] = INFINITY; hi[a][k] = -INFINITY; }
+        }
+    }
+    float4* o = nodes4 + 8 * (size_t)newId[n];
+    for (int a = 0; a < 3; a++) {
+        o[a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+        o[3 + a] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+    }
+    o[6] = make_float4(__uint_as_float(ch[0]), __uint_as_float(ch[1]), __uint_as_float(ch[2]), __uint_as_float(ch[3]));
+    o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 }  // namespace
+
+namespace rti {
+
+// Collapses the binary LBVH (s->nodes) into the 4-wide tree s->nodes4.
+static bool collapseBVH4(rt_device_scene* s, uint32_t nb, hipStream_t stream) {
+    uint32_t *keep = nullptr, *newId = nullptr, *fa = nullptr, *fb = nullptr, *cnt = nullptr;
+    bool ok = hipCheck(hipMalloc(&keep, sizeof(uint32_t) * nb), "hipMalloc") &&
+              hipCheck(hipMalloc(&newId, sizeof(uint32_t) * nb), "hipMalloc") &&
+              hipCheck(hipMalloc(&fa, sizeof(uint32_t) * nb), "hipMalloc") &&
+              hipCheck(hipMalloc(&fb, sizeof(uint32_t) * nb), "hipMalloc") &&
+              hipCheck(hipMalloc(&cnt, sizeof(uint32_t)), "hipMalloc");
+    uint32_t depth4 = 0, total = 0;
+    if (ok) {
+        uint32_t one = 1u, zero = 0u;
+        ok = hipCheck(hipMemsetAsync(keep, 0, sizeof(uint32_t) * nb, stream), "memset") &&
+             hipCheck(hipMemcpyAsync(keep, &one, 4, hipMemcpyHostToDevice, stream), "copy") &&
+             hipCheck(hipMemcpyAsync(fa, &zero, 4, hipMemcpyHostToDevice, stream), "copy");
+        uint32_t nf = 1;
+        while (ok && nf > 0) {
+            depth4++;
+            ok = hipCheck(hipMemsetAsync(cnt, 0, 4, stream), "memset");
+            if (!ok) break;
+            k_collapse_mark<<<cdiv(nf, kBlock), kBlock, 0, stream>>>(s->nodes, fa, nf, keep, fb, cnt);
+            ok = hipCheck(hipGetLastError(), "collapse mark") &&
+                 hipCheck(hipMemcpyAsync(&nf, cnt, 4, hipMemcpyDeviceToHost, stream), "copy") &&
+                 hipCheck(hipStreamSynchronize(stream), "sync");
+            std::swap(fa, fb);
+        }
+    }
+    if (ok) {
+        ok = hipCheck(hipMemcpyAsync(newId, keep, sizeof(uint32_t) * nb, hipMemcpyDeviceToDevice, stream), "copy");
+        k_scan_single<<<1, 1024, 0, stream>>>(newId, nb);
+        uint32_t lastKeep = 0, lastId = 0;
+        ok = ok && hipCheck(hipMemcpyAsync(&lastKeep, keep + nb - 1, 4, hipMemcpyDeviceToHost, stream), "copy") &&
+             hipCheck(hipMemcpyAsync(&lastId, newId + nb - 1, 4, hipMemcpyDeviceToHost, stream), "copy") &&
+             hipCheck(hipStreamSynchronize(stream), "sync");
+        total = lastId + lastKeep;
+    }
+    if (ok) {
+        ok = hipCheck(hipMalloc(&s->nodes4, sizeof(float4) * 8 * (size_t)total), "hipMalloc");
+        if (ok) s->deviceBytes += sizeof(float4) * 8 * (size_t)total;
+    }
+    if (ok) {
+        k_collapse_emit<<<cdiv(nb, kBlock), kBlock, 0, stream>>>(s->nodes, nb, keep, newId, s->nodes4);
+        ok = hipCheck(hipGetLastError(), "collapse emit") && hipCheck(hipStreamSynchronize(stream), "sync");
+    }
+    s->numNodes4 = total;
+    s->bvh4Depth = depth4;
+    (void)hipFree(keep); (void)hipFree(newId); (void)hipFree(fa); (void)hipFree(fb); (void)hipFree(cnt);
+    return ok;
+}
+
+}  // namespace rti
 
 namespace rti {
 
@@ -412,8 +540,11 @@ bool buildLBVH(rt_device_scene* s, const std::vector<uint32_t>& instPrimOffset, 
     }
     (void)hipFree(dOffsets); (void)hipFree(keysB); (void)hipFree(valsB); (void)hipFree(blockHist); (void)hipFree(partial); (void)hipFree(box);
     (void)hipFree(parentI); (void)hipFree(parentL); (void)hipFree(counters); (void)hipFree(dDepth);
-    if (ok && s->bvhDepth > RT_LDS_STACK + RT_SPILL_STACK) {
-        setError("LBVH depth " + std::to_string(s->bvhDepth) + " exceeds the traversal stack");
+    if (ok) ok = collapseBVH4(s, nNodes, stream);
+    // a 4-wide node pushes at most three children: the stack never holds more
+    // than 3 (depth - 1) entries
+    if (ok && 3 * (uint64_t)s->bvh4Depth > RT_LDS_STACK + RT_SPILL_STACK) {
+        setError("BVH depth " + std::to_string(s->bvh4Depth) + " exceeds the traversal stack");
         return false;
     }
     return ok;

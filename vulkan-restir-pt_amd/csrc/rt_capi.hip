@@ -50,7 +50,7 @@ bool upload(rt_device_scene* s, T** dst, const T* src, size_t count, hipStream_t
 }
 
 void freeScene(rt_device_scene* s) {
-    (void)hipFree(s->nodes); (void)hipFree(s->leafTris); (void)hipFree(s->primTris); (void)hipFree(s->codes); (void)hipFree(s->order);
+    (void)hipFree(s->nodes); (void)hipFree(s->nodes4); (void)hipFree(s->leafTris); (void)hipFree(s->primTris); (void)hipFree(s->codes); (void)hipFree(s->order);
     (void)hipFree(s->materials); (void)hipFree(s->materialIndices); (void)hipFree(s->vertices); (void)hipFree(s->indices);
     (void)hipFree(s->instances); (void)hipFree(s->lights); (void)hipFree(s->lightTable);
     delete s;
@@ -151,7 +151,7 @@ rt_status rt_device_scene_create(const rt_scene_arrays* a, hipStream_t stream, r
         d.numPrims = s->numPrims; d.numLights = s->numLights; d.numLightTable = a->numLightTable;
         d.numMaterials = a->numMaterials;
         ok = rti::buildLBVH(s, offs, stream);
-        d.nodes = s->nodes; d.leafTris = s->leafTris;
+        d.nodes = s->nodes4; d.leafTris = s->leafTris;
     }
     if (ok) ok = rti::hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
     if (!ok) { freeScene(s); return RT_ERR_HIP; }
@@ -171,6 +171,8 @@ rt_status rt_device_scene_info(const rt_device_scene* s, rt_scene_info* out) {
     out->numLights = s->numLights;
     out->bvhNodes = s->numPrims > 1 ? s->numPrims - 1 : (s->numPrims ? 1 : 0);
     out->bvhDepth = s->bvhDepth;
+    out->bvh4Nodes = s->numNodes4;
+    out->bvh4Depth = s->bvh4Depth;
     for (int a = 0; a < 3; a++) { out->sceneLo[a] = s->sceneLo[a]; out->sceneHi[a] = s->sceneHi[a]; }
     out->deviceBytes = s->deviceBytes;
     return RT_OK;
@@ -188,6 +190,15 @@ rt_status rt_bvh_export(const rt_device_scene* s, uint64_t* codes, uint32_t* ord
         RT_HIP(hipMemcpy(h.data(), s->nodes, sizeof(float4) * 4 * nn, hipMemcpyDeviceToHost));
         for (uint32_t i = 0; i < nn; i++) std::memcpy(nodes14 + 14 * (size_t)i, &h[4 * (size_t)i], 56);
     }
+    return RT_OK;
+}
+
+rt_status rt_bvh4_export(const rt_device_scene* s, uint32_t* nodes28) {
+    if (!s || !nodes28) return invalid("null argument");
+    if (s->numNodes4 == 0) return RT_OK;
+    std::vector<float4> h((size_t)s->numNodes4 * 8);
+    RT_HIP(hipMemcpy(h.data(), s->nodes4, sizeof(float4) * h.size(), hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < s->numNodes4; i++) std::memcpy(nodes28 + 28 * (size_t)i, &h[8 * (size_t)i], 112);
     return RT_OK;
 }
 

@@ -1,23 +1,29 @@
-// Software ray traversal over the LBVH (replaces the driver traversal behind
+// Software ray traversal (replaces the driver traversal behind
 // rayQueryProceedEXT, src/shader/ray_query.glsl:6-70; CDNA4 has no ray
 // tracing units).
 //
-// Per lane: a short stack in LDS (kLdsStack entries, strided by the block size
-// so a wave's pushes and pops hit distinct banks) that spills to a private
-// array for deep trees. Both child boxes of a node arrive in one 64-B load;
-// the nearer child is visited first (left on ties), the other is pushed.
+// The traversed tree is the 4-wide collapse of the binary LBVH: one 128-B node
+// (8 x float4: lo.x/lo.y/lo.z/hi.x/hi.y/hi.z of the four children as float4s,
+// the four child references, padding) per visit. The hit children are ordered
+// by entry distance (slot order on ties) with a five-comparator network;
+// leaves are tested in that order, the nearest internal child is visited next
+// and the other internal children are pushed farthest first.
+//
+// Per lane: a short stack in LDS (strided by the block size so a wave's pushes
+// and pops hit distinct banks) that spills to a private array for deep trees.
 // Leaves hold one triangle, tested with the watertight algorithm of Woop,
-// Benthin and Wald (JCGT 2013) including its double-precision fallback.
-// Box tests use (bound - o) * (1 / d) with IEEE minNum/maxNum so that 0 * inf
-// slabs are dropped. Closest hit accepts tmin <= t < tBest.
+// Benthin and Wald (JCGT 2013) including its double-precision fallback. Box
+// tests use (bound - o) * (1 / d) with IEEE minNum/maxNum so that 0 * inf slabs
+// are dropped. Closest hit accepts tmin <= t < tBest.
 #pragma once
 #include "rt_dscene.h"
 
 #ifndef RT_LDS_STACK
 #define RT_LDS_STACK 16
 #endif
-#define RT_SPILL_STACK 96
+#define RT_SPILL_STACK 240
 #define RT_LEAF 0x80000000u
+#define RT_EMPTY 0xffffffffu
 
 namespace rtd {
 
@@ -91,6 +97,14 @@ struct TStack {
     uint32_t stride;
 };
 
+// (t, slot) key order; a missed child carries t = +inf and sorts last.
+RT_DEV void cswap(float& ta, uint32_t& ra, float& tb, uint32_t& rb) {
+    bool sw = tb < ta;
+    float t = sw ? tb : ta; uint32_t r = sw ? rb : ra;
+    tb = sw ? ta : tb; rb = sw ? ra : rb;
+    ta = t; ra = r;
+}
+
 template <bool kAny, bool kCount>
 RT_DEV bool traverse(const DevScene& S, const RayT& r, bool skipLights, TStack st, float& tBest, Isec& hit,
                      TraceCounters* ctr) {
@@ -101,50 +115,65 @@ RT_DEV bool traverse(const DevScene& S, const RayT& r, bool skipLights, TStack s
     if (S.numPrims == 0) return false;
     uint32_t node = 0;
     for (;;) {
-        const float4* np = S.nodes + 4 * (size_t)node;
-        float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+        const float4* np = S.nodes + 8 * (size_t)node;
+        float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5], cr = np[6];
         if (kCount) cn++;
-        float tl, tr;
-        bool hl = boxHit(r, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, tBest, tl);
-        bool hr = boxHit(r, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, tBest, tr);
-        uint32_t cl = __float_as_uint(n3.x), cr = __float_as_uint(n3.y);
-        uint32_t c[2]; int nc = 0;
-        if (hl && hr) {
-            bool lf = tl <= tr;
-            c[0] = lf ? cl : cr; c[1] = lf ? cr : cl; nc = 2;
-        } else if (hl) { c[0] = cl; nc = 1; }
-        else if (hr) { c[0] = cr; nc = 1; }
-        uint32_t next = 0xffffffffu;
-        for (int k = 0; k < nc; k++) {
-            uint32_t ch = c[k];
-            if (ch & RT_LEAF) {
-                uint32_t leaf = ch & ~RT_LEAF;
-                const float4* tp = S.leafTris + 3 * (size_t)leaf;
-                float4 a = tp[0], b = tp[1];
-                uint32_t inst = __float_as_uint(a.w);
-                if (!(skipLights && inst == 0)) {
-                    float4 cc = tp[2];
-                    if (kCount) ct++;
-                    float t, b1, b2;
-                    if (triHit(r, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), tBest, t, b1, b2)) {
-                        found = true;
-                        if (kAny) {
-                            if (kCount) { atomicAdd(&ctr->nodes, cn); atomicAdd(&ctr->tris, ct); atomicAdd(&ctr->rays, 1ull); }
-                            return true;
-                        }
-                        tBest = t;
-                        hit.b1 = b1; hit.b2 = b2; hit.inst = inst; hit.prim = __float_as_uint(b.w);
-                    }
+        uint32_t c0 = __float_as_uint(cr.x), c1 = __float_as_uint(cr.y), c2 = __float_as_uint(cr.z),
+                 c3 = __float_as_uint(cr.w);
+        float t0, t1, t2, t3;
+        bool h0 = c0 != RT_EMPTY && boxHit(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, tBest, t0);
+        bool h1 = c1 != RT_EMPTY && boxHit(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, tBest, t1);
+        bool h2 = c2 != RT_EMPTY && boxHit(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, tBest, t2);
+        bool h3 = c3 != RT_EMPTY && boxHit(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, tBest, t3);
+        // keys: (t, slot); strict < keeps slot order on equal t
+        float k0 = h0 ? t0 : INFINITY, k1 = h1 ? t1 : INFINITY, k2 = h2 ? t2 : INFINITY, k3 = h3 ? t3 : INFINITY;
+        uint32_t r0 = h0 ? c0 : RT_EMPTY, r1 = h1 ? c1 : RT_EMPTY, r2 = h2 ? c2 : RT_EMPTY, r3 = h3 ? c3 : RT_EMPTY;
+        // stable 4-sort network of adjacent comparators (insertion order)
+        cswap(k0, r0, k1, r1);
+        cswap(k1, r1, k2, r2); cswap(k0, r0, k1, r1);
+        cswap(k2, r2, k3, r3); cswap(k1, r1, k2, r2); cswap(k0, r0, k1, r1);
+        uint32_t rs[4] = {r0, r1, r2, r3};
+        // leaves in order
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            uint32_t ref = rs[q];
+            if (ref == RT_EMPTY || !(ref & RT_LEAF)) continue;
+            uint32_t leaf = ref & ~RT_LEAF;
+            const float4* tp = S.leafTris + 3 * (size_t)leaf;
+            float4 a = tp[0], b = tp[1];
+            uint32_t inst = __float_as_uint(a.w);
+            if (skipLights && inst == 0) continue;
+            float4 cc = tp[2];
+            if (kCount) ct++;
+            float t, b1, b2;
+            if (triHit(r, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), tBest, t, b1, b2)) {
+                found = true;
+                if (kAny) {
+                    if (kCount) { atomicAdd(&ctr->nodes, cn); atomicAdd(&ctr->tris, ct); atomicAdd(&ctr->rays, 1ull); }
+                    return true;
                 }
-            } else if (next == 0xffffffffu) {
-                next = ch;
-            } else if (sp < RT_LDS_STACK + RT_SPILL_STACK) {  // the build rejects deeper trees
-                if (sp < RT_LDS_STACK) st.lds[sp * st.stride] = ch;
-                else spill[sp - RT_LDS_STACK] = ch;
+                tBest = t;
+                hit.b1 = b1; hit.b2 = b2; hit.inst = inst; hit.prim = __float_as_uint(b.w);
+            }
+        }
+        // nearest internal child next, the others pushed farthest first
+        uint32_t next = RT_EMPTY;
+        int first = -1;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (first < 0 && rs[q] != RT_EMPTY && !(rs[q] & RT_LEAF)) first = q;
+#pragma unroll
+        for (int q = 3; q >= 0; q--) {
+            uint32_t ref = rs[q];
+            if (ref == RT_EMPTY || (ref & RT_LEAF)) continue;
+            if (q == first) { next = ref; continue; }
+            if (sp < RT_LDS_STACK + RT_SPILL_STACK) {  // the build bounds the stack depth
+                if (sp < RT_LDS_STACK) st.lds[sp * st.stride] = ref;
+                else spill[sp - RT_LDS_STACK] = ref;
                 sp++;
             }
         }
-        if (next == 0xffffffffu) {
+        if (next == RT_EMPTY) {
             if (sp == 0) break;
             sp--;
             next = (sp < RT_LDS_STACK) ? st.lds[sp * st.stride] : spill[sp - RT_LDS_STACK];

@@ -8,7 +8,9 @@
 struct rt_device_scene {
     rtd::DevScene dev;            // device view passed to kernels by value
     // owned device allocations
-    float4* nodes = nullptr;      // 4 x float4 per internal node
+    float4* nodes = nullptr;      // binary LBVH: 4 x float4 per internal node
+    float4* nodes4 = nullptr;     // collapsed 4-wide tree: 8 x float4 per node (traversed)
+    uint32_t numNodes4 = 0, bvh4Depth = 0;
     float4* leafTris = nullptr;   // 3 x float4 per leaf
     float4* primTris = nullptr;   // 3 x float4 per primitive (primitive order)
     unsigned long long* codes = nullptr;  // sorted Morton codes

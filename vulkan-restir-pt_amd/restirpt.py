@@ -99,7 +99,7 @@ class Frame(C.Structure):
 class SceneInfo(C.Structure):
     _fields_ = [("numPrimitives", C.c_uint32), ("numLights", C.c_uint32), ("bvhNodes", C.c_uint32),
                 ("bvhDepth", C.c_uint32), ("sceneLo", C.c_float * 3), ("sceneHi", C.c_float * 3),
-                ("deviceBytes", C.c_size_t)]
+                ("deviceBytes", C.c_size_t), ("bvh4Nodes", C.c_uint32), ("bvh4Depth", C.c_uint32)]
 
 
 assert C.sizeof(Material) == 32 and C.sizeof(MeshVertex) == 32 and C.sizeof(ObjectInstance) == 224
@@ -115,7 +115,7 @@ def settings(shift=2, rr_scale=1.0, temporal=False, spatial=True, cap=20, max_de
 # Library loading
 
 CORE_SYMBOLS = [
-    "rt_last_error", "rt_device_scene_create", "rt_device_scene_destroy", "rt_device_scene_info", "rt_bvh_export",
+    "rt_last_error", "rt_device_scene_create", "rt_device_scene_destroy", "rt_device_scene_info", "rt_bvh_export", "rt_bvh4_export",
     "rt_trace_rays", "rt_gbuffer_trace", "rt_pt_plain", "rt_gris_path_trace", "rt_gris_temporal", "rt_gris_spatial",
     "rt_gris_render", "rt_gris_reservoir_bytes", "rt_gris_reservoir_reset", "rt_gris_reservoir_download",
     "rt_gris_reservoir_upload", "rt_counters_enable", "rt_counters_read",
@@ -138,6 +138,7 @@ def _declare_core(L):
         "rt_device_scene_destroy": [P],
         "rt_device_scene_info": [P, C.POINTER(SceneInfo)],
         "rt_bvh_export": [P, P, P, P],
+        "rt_bvh4_export": [P, P],
         "rt_trace_rays": [P, P, u32, C.c_int, C.c_int, P, P, P, P, P],
         "rt_gbuffer_trace": [P, C.POINTER(Camera), P, P, P, P],
         "rt_pt_plain": [P, C.POINTER(Camera), C.POINTER(Frame), u32, u32, P],
