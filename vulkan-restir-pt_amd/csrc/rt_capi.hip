@@ -52,7 +52,7 @@ bool upload(rt_device_scene* s, T** dst, const T* src, size_t count, hipStream_t
 void freeScene(rt_device_scene* s) {
     (void)hipFree(s->nodes); (void)hipFree(s->nodes4); (void)hipFree(s->leafTris); (void)hipFree(s->primTris); (void)hipFree(s->codes); (void)hipFree(s->order);
     (void)hipFree(s->materials); (void)hipFree(s->materialIndices); (void)hipFree(s->vertices); (void)hipFree(s->indices);
-    (void)hipFree(s->instances); (void)hipFree(s->lights); (void)hipFree(s->lightTable);
+    (void)hipFree(s->instances); (void)hipFree(s->lights); (void)hipFree(s->lightTable); (void)hipFree(s->queues);
     delete s;
 }
 
@@ -144,6 +144,12 @@ rt_status rt_device_scene_create(const rt_scene_arrays* a, hipStream_t stream, r
               upload(s, &s->instances, inst.data(), inst.size(), stream) &&
               upload(s, &s->lights, a->lights, a->numLights, stream) &&
               upload(s, &s->lightTable, a->lightTable, a->numLightTable, stream);
+    if (ok) {
+        const size_t qb = sizeof(rtd::WorkQueue) * rt_device_scene::kQueueSlots;
+        ok = rti::hipCheck(hipMalloc(&s->queues, qb), "hipMalloc") &&
+             rti::hipCheck(hipMemsetAsync(s->queues, 0, qb, stream), "hipMemsetAsync");
+        s->deviceBytes += qb;
+    }
     if (ok) {
         rtd::DevScene& d = s->dev;
         d.materials = s->materials; d.materialIndices = s->materialIndices; d.vertices = s->vertices;

@@ -30,6 +30,11 @@ namespace rtd {
 // Work counters of the counting kernel builds (nodes, triangles, rays).
 struct TraceCounters { unsigned long long nodes, tris, rays; };
 
+// Pixel queue of a persistent pass: `next` is the next unclaimed work item,
+// `done` counts finished workgroups. Both are zero between launches (the
+// last workgroup of a launch clears them). One 128-B line per queue.
+struct WorkQueue { uint32_t next, done, pad[30]; };
+
 struct RayT {
     f3 o, d, invd;
     float tmin, tmax;

@@ -2,6 +2,7 @@
 // launchers (not part of the public interface).
 #pragma once
 #include "rt_dtrace.h"
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -26,6 +27,12 @@ struct rt_device_scene {
     uint32_t bvhDepth = 0;
     float sceneLo[3] = {0, 0, 0}, sceneHi[3] = {0, 0, 0};
     size_t deviceBytes = 0;
+    // Pixel queues of the persistent passes, handed out round robin: up to
+    // kQueueSlots such launches on this scene may be in flight at once.
+    static constexpr uint32_t kQueueSlots = 64;
+    rtd::WorkQueue* queues = nullptr;
+    mutable std::atomic<uint32_t> queueTick{0};
+    rtd::WorkQueue* nextQueue() const { return queues + (queueTick.fetch_add(1u) % kQueueSlots); }
 };
 
 namespace rti {

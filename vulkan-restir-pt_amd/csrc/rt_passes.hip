@@ -21,6 +21,30 @@ namespace {
 
 constexpr int kTileX = 8, kTileY = 8, kTileThreads = 64;
 
+// Occupancy that each kernel's register allocation is held to, in waves per
+// SIMD (a lane then has at most 512 / waves VGPRs). Chosen by measurement on
+// the 1080p terrain frame: the GRIS kernels run fastest at 4-5 waves even with
+// some spilling, ~25% faster than at the 2-3 waves their unconstrained
+// allocation (150-170 VGPRs) gives. Each can be overridden at build time.
+#ifndef RT_WAVES_TRACE
+#define RT_WAVES_TRACE 4
+#endif
+#ifndef RT_WAVES_GBUFFER
+#define RT_WAVES_GBUFFER 4
+#endif
+#ifndef RT_WAVES_PT_PLAIN
+#define RT_WAVES_PT_PLAIN 4
+#endif
+#ifndef RT_WAVES_PATH_TRACE
+#define RT_WAVES_PATH_TRACE 4
+#endif
+#ifndef RT_WAVES_TEMPORAL
+#define RT_WAVES_TEMPORAL 4
+#endif
+#ifndef RT_WAVES_SPATIAL
+#define RT_WAVES_SPATIAL 5
+#endif
+
 struct GB {   // one G-buffer parity (descriptor bindings 2/4/6 or 3/5)
     const float4* dn;
     const uint2* am;
@@ -144,7 +168,7 @@ RT_DEV void accumulate(float* out, size_t p, f3 c, uint32_t frameIndex) {
 // ================================================================================================
 // Ray query test entry
 template <bool kCount>
-__global__ __launch_bounds__(kTileThreads) void k_trace_rays(DevScene S, const float* __restrict__ rays8, uint32_t n,
+__global__ __launch_bounds__(kTileThreads, RT_WAVES_TRACE) void k_trace_rays(DevScene S, const float* __restrict__ rays8, uint32_t n,
                                                              int mode, int skipLights, float* t, float* bary2,
                                                              uint32_t* inst, uint32_t* prim, TraceCounters* ctr) {
     __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
@@ -168,7 +192,7 @@ __global__ __launch_bounds__(kTileThreads) void k_trace_rays(DevScene S, const f
 // Light triangles are not drawn by the reference's raster pass; the clip
 // planes act as ray distances [near, far].
 template <bool kCount>
-__global__ __launch_bounds__(kTileThreads) void k_gbuffer(DevScene S, rt_camera cam, float4* __restrict__ dnOut,
+__global__ __launch_bounds__(kTileThreads, RT_WAVES_GBUFFER) void k_gbuffer(DevScene S, rt_camera cam, float4* __restrict__ dnOut,
                                                           uint2* __restrict__ amOut, uint32_t* __restrict__ mvOut,
                                                           TraceCounters* ctr) {
     __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
@@ -339,7 +363,7 @@ RT_DEV f3 indirectIllumination(const DevScene& S, TStack st, const rt_camera& ca
 }
 
 template <bool kCount>
-__global__ __launch_bounds__(kTileThreads) void k_pt_plain(DevScene S, rt_camera cam, GB g, uint32_t maxDepth,
+__global__ __launch_bounds__(kTileThreads, RT_WAVES_PT_PLAIN) void k_pt_plain(DevScene S, rt_camera cam, GB g, uint32_t maxDepth,
                                                            uint32_t which, float* directOut, float* indirectOut,
                                                            TraceCounters* ctr) {
     __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
@@ -414,47 +438,76 @@ RT_DEV void resvCap(Resv& r, float cap) {
 #define RC_SURFACE 2u
 #define GRIS_DIST_THRESHOLD 0.01f
 
-// gris_path_trace.glsl:45-306
-template <bool kCount>
-__global__ __launch_bounds__(kTileThreads) void k_gris_path_trace(DevScene S, rt_camera cam, GB g, rt_gris_settings set,
-                                                                  ResvPlanes outR, TraceCounters* ctr) {
-    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
-    PIXEL_INDEX
-    if (x >= g.W || y >= g.H) return;
-    TStack st{stk + threadIdx.x, kTileThreads};
+// gris_path_trace.glsl:45-306, run as a persistent kernel with path
+// regeneration: a lane whose path ends takes the next pixel of a tile-ordered
+// work queue at once, so a wave keeps tracing while any pixel is left. Each
+// pixel's arithmetic and random-number stream are those of the reference's
+// one-invocation-per-pixel loop; only the assignment of pixels to lanes moves.
+
+// Locals of the reference's bounce loop that live from one vertex to the next.
+struct PTPath {
+    uint32_t bounce, rng, sampleState, lastSampleState;
+    bool isLastVertexConnectible;
+    f3 throughput, rcThroughput, wo, lastPos, rdir, rori;
+    BSDFSample s;
+    Isec isec;
+    Resv ps, stream;  // the path sample and the stream-RIS pick
+    float streamWeight, streamSum;
+};
+
+// Start of the invocation (gris_path_trace.glsl:45-100): primary vertex from
+// the G-buffer. Returns false for background pixels, whose reservoir is left
+// untouched as in the reference.
+RT_DEV bool ptBegin(const DevScene& S, const rt_camera& cam, const GB& g, uint32_t x, uint32_t y, PTPath& P, Surf& surf,
+                    Mat& mat) {
     f2 uv = pixelUV(x, y, g.W, g.H);
     GPix gp;
-    if (!unpackGB(texDNCentre(g, x, y), fetchAM(g, (int)x, (int)y), gp)) return;  // reservoir not written
+    if (!unpackGB(texDNCentre(g, x, y), fetchAM(g, (int)x, (int)y), gp)) return false;
     int matId = gp.matMeshId >> 16;
     CamRay ray = pinholeRay(cam, mk2(uv.x, 1.0f - uv.y));
-    uint32_t rng = makeSeedPix(cam.seed, x, y);
-
-    f3 throughput = s3(1.0f), rcThroughput = s3(0.0f);
-    f3 wo = -ray.dir;
-    f3 lastPos = s3(0.0f);
-    bool isLastVertexConnectible = false;
-    uint32_t sampleState = 0, lastSampleState = 0;
-    Surf surf;
+    P.rng = makeSeedPix(cam.seed, x, y);
+    P.bounce = 0;
+    P.throughput = s3(1.0f); P.rcThroughput = s3(0.0f);
+    P.wo = -ray.dir;
+    P.lastPos = s3(0.0f);
+    P.isLastVertexConnectible = false;
+    P.sampleState = 0; P.lastSampleState = 0;
     surf.pos = ray.ori + ray.dir * (gp.depth - 1e-4f);
     surf.norm = gp.norm; surf.albedo = gp.albedo; surf.isLight = false; surf.matIndex = (uint32_t)matId;
-    Mat mat = loadMat(S, (uint32_t)matId);
-    BSDFSample s; s.type = 0; s.pdf = 0.0f; s.wi = s3(0.0f); s.bsdf = s3(0.0f);
-    Isec isec; isec.b1 = 0.0f; isec.b2 = 0.0f; isec.inst = 0u; isec.prim = 0u;
-    f3 rdir = ray.dir, rori = ray.ori;
+    mat = loadMat(S, (uint32_t)matId);
+    P.s.type = 0; P.s.pdf = 0.0f; P.s.wi = s3(0.0f); P.s.bsdf = s3(0.0f);
+    P.isec.b1 = 0.0f; P.isec.b2 = 0.0f; P.isec.inst = 0u; P.isec.prim = 0u;
+    P.rdir = ray.dir; P.rori = ray.ori;
+    P.ps = zeroResv();  // the path sample (sampleCount/resampleWeight unused here)
+    P.ps.rcIsec.inst = RT_INVALID_HIT;
+    P.ps.primaryRng = P.rng;
+    P.stream = zeroResv();
+    P.streamWeight = 0.0f; P.streamSum = 0.0f;
+    return true;
+}
 
-    Resv ps = zeroResv();  // the path sample (sampleCount/resampleWeight unused here)
-    ps.rcIsec.inst = RT_INVALID_HIT;
-    ps.primaryRng = rng;
-    Resv stream = zeroResv();
-    float streamWeight = 0.0f, streamSum = 0.0f;
-
-    for (uint32_t bounce = 0; bounce < set.maxDepth; bounce++) {
-        if (bounce > 0) {
-            isec = traceClosestHit<kCount>(S, st, rori, RT_MIN_RAY, rdir, RT_MAX_RAY, ctr);
-            if (isec.inst == RT_INVALID_HIT) break;
-            loadSurfaceInfo(S, isec, surf);
-            mat = loadMat(S, surf.matIndex);
-        }
+// One vertex of the bounce loop, after its closest-hit query. Returns false
+// where the reference leaves the loop; otherwise the next ray is in rori/rdir.
+template <bool kCount>
+RT_DEV bool ptVertex(const DevScene& S, TStack st, const rt_gris_settings& set, PTPath& P, const Surf& surf,
+                     const Mat& mat, const uint32_t bounce, TraceCounters* ctr) {
+    uint32_t& rng = P.rng;
+    uint32_t& sampleState = P.sampleState;
+    uint32_t& lastSampleState = P.lastSampleState;
+    bool& isLastVertexConnectible = P.isLastVertexConnectible;
+    f3& throughput = P.throughput;
+    f3& rcThroughput = P.rcThroughput;
+    f3& wo = P.wo;
+    f3& lastPos = P.lastPos;
+    f3& rdir = P.rdir;
+    f3& rori = P.rori;
+    BSDFSample& s = P.s;
+    const Isec& isec = P.isec;
+    Resv& ps = P.ps;
+    Resv& stream = P.stream;
+    float& streamWeight = P.streamWeight;
+    float& streamSum = P.streamSum;
+    {
         flagsSetPathLength(ps.flags, bounce + 1);
         float cosPrevWi = dot(rdir, surf.norm);
         float distToPrev = distance(lastPos, surf.pos);
@@ -491,7 +544,7 @@ __global__ __launch_bounds__(kTileThreads) void k_gris_path_trace(DevScene S, rt
                     if (resvRandSample * streamSum < wgt) { streamWeight = wgt; stream = ps; }
                 }
             }
-            break;
+            return false;
         }
 
         bool connectible = isThisVertexConnectible && isLastVertexConnectible && distToPrev > GRIS_DIST_THRESHOLD;
@@ -547,12 +600,12 @@ __global__ __launch_bounds__(kTileThreads) void k_gris_path_trace(DevScene S, rt
 
         if (bounce > 4) {
             float pdfTerminate = gmax(1.0f - luminance(throughput) * set.rrScale, 0.0f);
-            if (sample1f(rng) < pdfTerminate) break;
+            if (sample1f(rng) < pdfTerminate) return false;
             throughput = throughput / (1.0f - pdfTerminate);
             rcThroughput = rcThroughput / (1.0f - pdfTerminate);
         }
 
-        if (!sampleBSDF(mat, surf.albedo, surf.norm, wo, sample3f(rng), s) || s.pdf < 1e-6f) break;
+        if (!sampleBSDF(mat, surf.albedo, surf.norm, wo, sample3f(rng), s) || s.pdf < 1e-6f) return false;
         float cosTheta = isSampleTypeDelta(s.type) ? 1.0f : absDot(surf.norm, s.wi);
         f3 scatterTerm = s.bsdf * cosTheta / s.pdf;
         throughput = throughput * scatterTerm;
@@ -570,17 +623,22 @@ __global__ __launch_bounds__(kTileThreads) void k_gris_path_trace(DevScene S, rt
         rori = surf.pos + rdir * 1e-4f;
         isLastVertexConnectible = isThisVertexConnectible;
     }
+    return true;
+}
 
-    if (sampleState == 2 && lastSampleState == 2) {
-        float wgt = luminance(ps.F);
-        float r = sample1f(rng);
-        streamSum += wgt;
-        if (r * streamSum < wgt) { streamWeight = wgt; stream = ps; }
+// After the bounce loop (gris_path_trace.glsl:280-306): last stream-RIS
+// candidate and the reservoir written for the pixel.
+RT_DEV Resv ptFinish(PTPath& P) {
+    if (P.sampleState == 2 && P.lastSampleState == 2) {
+        float wgt = luminance(P.ps.F);
+        float r = sample1f(P.rng);
+        P.streamSum += wgt;
+        if (r * P.streamSum < wgt) { P.streamWeight = wgt; P.stream = P.ps; }
     }
-    Resv out = stream;
+    Resv out = P.stream;
     out.resampleWeight = 0.0f;
-    if (streamSum > 0.0f && streamWeight > 0.0f) {
-        float k = streamSum / streamWeight;
+    if (P.streamSum > 0.0f && P.streamWeight > 0.0f) {
+        float k = P.streamSum / P.streamWeight;
         out.F = out.F * k;
         out.rcLi = out.rcLi * k;
         out.resampleWeight = luminance(out.F);
@@ -589,7 +647,65 @@ __global__ __launch_bounds__(kTileThreads) void k_gris_path_trace(DevScene S, rt
         out.F = s3(0.0f);
     }
     out.sampleCount = 1.0f;
-    storeResv(outR, (size_t)y * g.W + x, out);
+    return out;
+}
+
+// Pixels are handed out in 8x8-tile order, so the lanes refilled together by
+// one wave take neighbouring pixels. Every lane leaves the loop once the queue
+// is drained; the last wave to finish re-arms the queue for the next launch.
+template <bool kCount>
+__global__ __launch_bounds__(kTileThreads, RT_WAVES_PATH_TRACE) void k_gris_path_trace(
+    DevScene S, rt_camera cam, GB g, rt_gris_settings set, ResvPlanes outR, WorkQueue* q, TraceCounters* ctr) {
+    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    TStack st{stk + threadIdx.x, kTileThreads};
+    const uint32_t lane = threadIdx.x;
+    const uint32_t tilesX = (g.W + kTileX - 1) / kTileX;
+    const uint32_t total = tilesX * ((g.H + kTileY - 1) / kTileY) * kTileThreads;
+    PTPath P;
+    uint32_t pix = 0;
+    bool live = false, drained = false;
+    while (true) {
+        // lanes without a path take the next pixels; a background pixel or a
+        // path that ends at its primary vertex sends the lane back for more
+        while (true) {
+            const bool want = !live && !drained;
+            const unsigned long long mask = __ballot(want);
+            if (mask == 0ull) break;
+            const int leader = __ffsll((long long)mask) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(&q->next, (uint32_t)__popcll(mask));
+            base = (uint32_t)__shfl((int)base, leader);
+            if (!want) continue;
+            const uint32_t w = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+            if (w >= total) { drained = true; continue; }
+            const uint32_t tile = w / kTileThreads, t = w % kTileThreads;
+            const uint32_t x = (tile % tilesX) * kTileX + t % kTileX, y = (tile / tilesX) * kTileY + t / kTileX;
+            Surf surf; Mat mat;
+            if (x >= g.W || y >= g.H || !ptBegin(S, cam, g, x, y, P, surf, mat)) continue;
+            pix = y * g.W + x;
+            live = ptVertex<kCount>(S, st, set, P, surf, mat, 0u, ctr) && ++P.bounce < set.maxDepth;
+            if (!live) storeResv(outR, pix, ptFinish(P));
+        }
+        if (__ballot(live) == 0ull) break;
+        if (live) {
+            P.isec = traceClosestHit<kCount>(S, st, P.rori, RT_MIN_RAY, P.rdir, RT_MAX_RAY, ctr);
+            live = P.isec.inst != RT_INVALID_HIT;
+            if (live) {
+                Surf surf;
+                loadSurfaceInfo(S, P.isec, surf);
+                Mat mat = loadMat(S, surf.matIndex);
+                live = ptVertex<kCount>(S, st, set, P, surf, mat, P.bounce, ctr) && ++P.bounce < set.maxDepth;
+            }
+            if (!live) storeResv(outR, pix, ptFinish(P));
+        }
+    }
+    if (lane == 0) {
+        __threadfence();
+        if (atomicAdd(&q->done, 1u) == gridDim.x - 1u) {
+            atomicExch(&q->next, 0u);
+            atomicExch(&q->done, 0u);
+        }
+    }
 }
 
 // ---- gris_retrace.glsl:42-136 --------------------------------------------------------------------
@@ -716,7 +832,7 @@ RT_DEV Resv resvAt(const ResvPlanes& R, uint32_t W, uint32_t H, int x, int y) {
 
 // gris_resample_temporal.glsl:11-83
 template <bool kCount>
-__global__ __launch_bounds__(kTileThreads) void k_gris_temporal(DevScene S, rt_camera cam, rt_camera prevCam, GB g,
+__global__ __launch_bounds__(kTileThreads, RT_WAVES_TEMPORAL) void k_gris_temporal(DevScene S, rt_camera cam, rt_camera prevCam, GB g,
                                                                 GB gPrev, rt_gris_settings set, ResvPlanes thisR,
                                                                 ResvPlanes prevR, ResvPlanes tempR, TraceCounters* ctr) {
     __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
@@ -766,7 +882,7 @@ __global__ __launch_bounds__(kTileThreads) void k_gris_temporal(DevScene S, rt_c
 
 // gris_resample_spatial.glsl:11-135 and gris_resample_spatial.comp accumulation
 template <bool kCount>
-__global__ __launch_bounds__(kTileThreads) void k_gris_spatial(DevScene S, rt_camera cam, GB g, rt_gris_settings set,
+__global__ __launch_bounds__(kTileThreads, RT_WAVES_SPATIAL) void k_gris_spatial(DevScene S, rt_camera cam, GB g, rt_gris_settings set,
                                                                ResvPlanes tempR, ResvPlanes thisR, float* indirectOut,
                                                                TraceCounters* ctr) {
     __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
@@ -880,11 +996,30 @@ hipError_t launchPtPlain(const rt_device_scene* s, const rt_camera* cams, const 
     return hipGetLastError();
 }
 
+// Workgroups of a persistent pass: as many as the device holds at once, never
+// more than there are 8x8 tiles.
+template <class Kernel>
+static hipError_t persistentGrid(Kernel kernel, uint32_t W, uint32_t H, uint32_t& grid) {
+    int dev = 0, cus = 0, perCU = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(kernel), kTileThreads, 0);
+    if (e != hipSuccess) return e;
+    const uint64_t tiles = (uint64_t)((W + kTileX - 1) / kTileX) * ((H + kTileY - 1) / kTileY);
+    grid = (uint32_t)std::min<uint64_t>(tiles, (uint64_t)std::max(1, cus) * (uint64_t)std::max(1, perCU));
+    return hipSuccess;
+}
+
 hipError_t launchGrisPathTrace(const rt_device_scene* s, const rt_camera* cams, const rt_frame& f,
                                const rt_gris_settings& set, hipStream_t stream) {
     GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
-    RT_DISPATCH(k_gris_path_trace, tileGrid(f.width, f.height), kTileThreads, stream, s->dev, cams[0], g, set,
-                planes(f.grisReservoir, f.width, f.height));
+    uint32_t grid = 0;
+    hipError_t e = gCountersEnabled ? persistentGrid(k_gris_path_trace<true>, f.width, f.height, grid)
+                                    : persistentGrid(k_gris_path_trace<false>, f.width, f.height, grid);
+    if (e != hipSuccess) return e;
+    RT_DISPATCH(k_gris_path_trace, dim3(grid), kTileThreads, stream, s->dev, cams[0], g, set,
+                planes(f.grisReservoir, f.width, f.height), s->nextQueue());
     return hipGetLastError();
 }
 
