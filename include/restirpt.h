@@ -118,6 +118,13 @@ typedef struct rt_gris_settings {
     uint32_t maxDepth;         /* MaxTracingDepth, gris_path_trace.glsl:46 (15) */
     uint32_t spatialNeighbors; /* ResampleNum, gris_resample_spatial.glsl:65 (3) */
     float spatialRadius;       /* ResampleRadius in pixels, gris_resample_spatial.glsl:66 (20) */
+    /* Target function of the temporal and spatial resampling (GRISToScalar,
+     * gris_reservoir.glsl:33-35, at the sites gris_retrace.glsl:204,215 and
+     * gris_resample_spatial.glsl:128): 0 = luminance (the reference),
+     * 1 = learned (luminance of the contribution with the reconnection
+     * radiance replaced by an rt_mlp prediction; needs rt_gris_learned_refresh
+     * after each path-trace pass). */
+    uint32_t targetFunction;
 } rt_gris_settings;
 
 /* Descriptor set 2 (RayImageDescSet) of the reference: layouts.glsl:180-195,
@@ -203,7 +210,11 @@ rt_status rt_gris_render(const rt_device_scene* scene, const rt_camera cams[2], 
                          const rt_gris_settings* settings, hipStream_t stream);
 
 /* Reservoir buffers (the GRISReservoir storage buffers allocated by
- * Renderer::createRayImage, src/Renderer.cpp:236-250). */
+ * Renderer::createRayImage, src/Renderer.cpp:236-250): 96 B per pixel, the
+ * reference's reservoir size. Five 16-B planes hold the reference fields; the
+ * sixth holds the learned target data (predicted reconnection radiance and
+ * the target value the resample weight refers to) and is only used when
+ * targetFunction = 1. */
 size_t rt_gris_reservoir_bytes(uint32_t width, uint32_t height);
 /* Sets every reservoir to the reset state (GRISReservoirReset,
  * gris_reservoir.glsl:75-79; other fields zero). */
@@ -213,6 +224,69 @@ rt_status rt_gris_reservoir_download(const void* reservoirs, uint32_t width, uin
                                      rt_gris_reservoir* host, hipStream_t stream);
 rt_status rt_gris_reservoir_upload(const rt_gris_reservoir* host, uint32_t width, uint32_t height,
                                    void* reservoirs, hipStream_t stream);
+/* Host copy of the learned-target plane: 4 floats per pixel (predicted
+ * reconnection radiance RGB, target value of the resample weight). */
+rt_status rt_gris_learned_download(const void* reservoirs, uint32_t width, uint32_t height, float* host4,
+                                   hipStream_t stream);
+rt_status rt_gris_learned_upload(const float* host4, uint32_t width, uint32_t height, void* reservoirs,
+                                 hipStream_t stream);
+
+/* ---- Learned target function (no counterpart in the reference) ----------
+ * A multiresolution hash-grid encoding of world position (levels x 2 FP16
+ * features, 2^log2TableSize entries per level) feeding an MLP with two hidden
+ * layers of 64 (FP16 weights on the matrix cores, FP32 accumulation, ReLU,
+ * exp output) that predicts RGB radiance. Trained online with Adam on a
+ * relative L2 loss mean((y^ - y)^2 / (sg(y^)^2 + 0.01)). */
+typedef struct rt_mlp rt_mlp;
+typedef struct rt_mlp_config {
+    uint32_t levels;            /* 16 (only value supported) */
+    uint32_t featuresPerLevel;  /* 2 (only value supported) */
+    uint32_t log2TableSize;     /* 19; 10..24 */
+    uint32_t baseResolution;    /* 16 */
+    uint32_t maxResolution;     /* 2048 */
+    uint32_t hiddenWidth;       /* 64 (only value supported) */
+    uint32_t hiddenLayers;      /* 2 (only value supported) */
+    uint32_t seed;              /* parameter initialisation */
+    float learningRate;         /* 1e-2 */
+    float beta1, beta2, epsilon;/* 0.9, 0.99, 1e-15 */
+    float boundsLo[3], boundsHi[3];  /* world box mapped to the unit cube */
+} rt_mlp_config;
+
+rt_status rt_mlp_create(const rt_mlp_config* config, hipStream_t stream, rt_mlp** out);
+rt_status rt_mlp_destroy(rt_mlp* mlp);
+/* Parameter counts: grid = levels * 2^log2TableSize * 2, weights = 32*64 + 64*64 + 64*3. */
+rt_status rt_mlp_param_counts(const rt_mlp* mlp, uint64_t* gridParams, uint64_t* weightParams);
+/* FP32 master parameters on the host: grid [level][entry][feature], weights
+ * W1 [32][64], W2 [64][64], W3 [64][3] (input-major). Setting also refreshes
+ * the FP16 copies the kernels read and clears the Adam state. */
+rt_status rt_mlp_get_params(const rt_mlp* mlp, float* grid, float* weights);
+rt_status rt_mlp_set_params(rt_mlp* mlp, const float* grid, const float* weights);
+/* Inference on device arrays: pos [n][3] world positions -> rgb [n][3]. */
+rt_status rt_mlp_infer(const rt_mlp* mlp, const float* pos, uint32_t n, float* rgb, hipStream_t stream);
+/* Gradients of the loss over a batch (device arrays; a sample whose target has
+ * a negative first component is masked out). loss (device, may be null)
+ * receives the mean loss. The gradients stay on the device for the next
+ * rt_mlp_adam_step; rt_mlp_read_gradients copies them to the host. */
+rt_status rt_mlp_gradients(rt_mlp* mlp, const float* pos, const float* target, uint32_t n, float* loss,
+                           hipStream_t stream);
+rt_status rt_mlp_read_gradients(const rt_mlp* mlp, float* gridGrad, float* weightGrad);
+rt_status rt_mlp_adam_step(rt_mlp* mlp, hipStream_t stream);
+/* rt_mlp_gradients + rt_mlp_adam_step. */
+rt_status rt_mlp_train_step(rt_mlp* mlp, const float* pos, const float* target, uint32_t n, float* loss,
+                            hipStream_t stream);
+/* One training step on the samples gathered by the last rt_gris_learned_refresh. */
+rt_status rt_mlp_train_frame(rt_mlp* mlp, float* loss, hipStream_t stream);
+/* Host copy of that batch: *count samples of position [3] and target [3]
+ * (arrays of at least 65536 x 3 floats; either may be null). */
+rt_status rt_mlp_batch_download(const rt_mlp* mlp, float* pos, float* target, uint32_t* count);
+
+/* After rt_gris_path_trace: predicts the reconnection radiance of every
+ * pixel's reservoir with the network, rescales the resample weight to the
+ * learned target and fills the learned-target plane; also gathers the frame's
+ * training batch (reconnection vertex position, reconnection radiance) from a
+ * fixed stride of pixels. */
+rt_status rt_gris_learned_refresh(const rt_device_scene* scene, rt_mlp* mlp, const rt_camera cams[2],
+                                  const rt_frame* frame, hipStream_t stream);
 
 /* Traversal work counters (internal nodes visited, triangles tested, rays)
  * accumulated by every kernel launched while enabled; off by default. The

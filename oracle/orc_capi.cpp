@@ -194,10 +194,13 @@ void orc_gris_path_trace(void* h, const Camera* cams, const float* depthNormal, 
     sumStats(st, stats3);
 }
 
+// learn* (float4 per pixel, may be null): learned-target planes of the three
+// buffers, used when set->targetFunction == 1.
 void orc_gris_temporal(void* h, const Camera* cams, const float* depthNormal, const uint32_t* albedoMatId,
                        const uint16_t* motion, const float* depthNormalPrev, const uint32_t* albedoMatIdPrev,
                        const GRISSettings* set, const GRISReservoir* resvThis, const GRISReservoir* resvPrev,
-                       GRISReservoir* resvTemp, uint64_t* stats3, int nthreads) {
+                       GRISReservoir* resvTemp, uint64_t* stats3, int nthreads, const float* learnThis,
+                       const float* learnPrev, float* learnTemp) {
     OScene* s = (OScene*)h;
     const Camera& cam = cams[0];
     const uint32_t W = cam.filmSize[0], H = cam.filmSize[1];
@@ -208,7 +211,12 @@ void orc_gris_temporal(void* h, const Camera* cams, const float* depthNormal, co
         Ctx c{&s->S, &s->bvh, &st[t]};
         for (uint32_t x = 0; x < W; x++) {
             GRISReservoir r;
-            if (temporalReuse(c, cam, cams[1], g, gp, *set, resvThis, resvPrev, x, y, r)) resvTemp[(size_t)y * W + x] = r;
+            LearnedData l{};
+            if (temporalReuse(c, cam, cams[1], g, gp, *set, resvThis, resvPrev, x, y, r, (const LearnedData*)learnThis,
+                              (const LearnedData*)learnPrev, &l)) {
+                resvTemp[(size_t)y * W + x] = r;
+                if (learnTemp) ((LearnedData*)learnTemp)[(size_t)y * W + x] = l;
+            }
         }
     });
     sumStats(st, stats3);
@@ -216,7 +224,7 @@ void orc_gris_temporal(void* h, const Camera* cams, const float* depthNormal, co
 
 void orc_gris_spatial(void* h, const Camera* cams, const float* depthNormal, const uint32_t* albedoMatId,
                       const GRISSettings* set, const GRISReservoir* resvTemp, GRISReservoir* resvThis, float* indirectOut,
-                      uint64_t* stats3, int nthreads) {
+                      uint64_t* stats3, int nthreads, const float* learnTemp, float* learnThis) {
     OScene* s = (OScene*)h;
     const Camera& cam = cams[0];
     const uint32_t W = cam.filmSize[0], H = cam.filmSize[1];
@@ -227,12 +235,45 @@ void orc_gris_spatial(void* h, const Camera* cams, const float* depthNormal, con
         for (uint32_t x = 0; x < W; x++) {
             size_t p = (size_t)y * W + x;
             GRISReservoir r; bool written;
-            vec3 col = spatialReuse(c, cam, g, *set, resvTemp, x, y, r, written);
-            if (written) resvThis[p] = r;
+            LearnedData l{};
+            vec3 col = spatialReuse(c, cam, g, *set, resvTemp, x, y, r, written, (const LearnedData*)learnTemp, &l);
+            if (written) {
+                resvThis[p] = r;
+                if (learnThis) ((LearnedData*)learnThis)[p] = l;
+            }
             accumulate(indirectOut + 4 * p, col, cam.frameIndex);
         }
     });
     sumStats(st, stats3);
+}
+
+// Learned-target refresh over a frame, given the network's predictions lhat4
+// (float4 per pixel; xyz used). Writes the refreshed reservoirs in place, the
+// learned planes to learnOut4 and the training batch (p % stride == offset).
+void orc_gris_learned_refresh(void* h, uint32_t W, uint32_t H, const float* depthNormal, GRISReservoir* resv,
+                              const float* lhat4, float* learnOut4, float* batchPos, float* batchTarget, uint32_t stride,
+                              uint32_t offset, int nthreads) {
+    OScene* s = (OScene*)h;
+    GBufView g{depthNormal, nullptr, nullptr, W, H};
+    std::vector<TraceStats> st(nthreads > 1 ? nthreads : 1);
+    parallelRows(H, nthreads, [&](uint32_t y, int t) {
+        Ctx c{&s->S, &s->bvh, &st[t]};
+        for (uint32_t x = 0; x < W; x++) {
+            const size_t p = (size_t)y * W + x;
+            LearnedData l{};
+            bool need = false;
+            vec3 pos = V3s(0.0f);
+            if (learnedRefresh(c, g, x, y, resv[p], lhat4 + 4 * p, l, need, pos)) ((LearnedData*)learnOut4)[p] = l;
+            if (p % stride == offset) {
+                const size_t slot = p / stride;
+                const GRISPathSample& ps = resv[p].pathSample;
+                batchPos[3 * slot] = pos.x; batchPos[3 * slot + 1] = pos.y; batchPos[3 * slot + 2] = pos.z;
+                batchTarget[3 * slot] = need ? ps.rcLi[0] : -1.0f;
+                batchTarget[3 * slot + 1] = need ? ps.rcLi[1] : 0.0f;
+                batchTarget[3 * slot + 2] = need ? ps.rcLi[2] : 0.0f;
+            }
+        }
+    });
 }
 
 // ---- component entry points for pinning tests ------------------------------------------------

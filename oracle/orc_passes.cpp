@@ -445,8 +445,32 @@ static void traceReplayPathForHybridShift(const Ctx& c, const GRISSettings& set,
 }
 
 // gris_retrace.glsl:138-236
-static void reuseAndMerge(const Ctx& c, const GRISSettings& set, GRISReservoir& dstResv, const SurfaceInfo& dstPrimarySurf,
-                          const Intersection& dstPrimaryIsec, const Ray& primaryRay, GRISReservoir srcResv, uint32_t& rng) {
+// Reconnection shading of gris_retrace.glsl:188-203 with the reconnection
+// radiance rcLi given (the sample's own, or the learned prediction).
+static vec3 reconnectLi(const GRISPathSample& srcSample, vec3 rcLi, const Material& rcMat, const SurfaceInfo& rcSurf,
+                        const Material& rcPrevMat, const SurfaceInfo& rcPrevSurf, vec3 wi, const RcData& rc) {
+    vec3 Li = rcLi;
+    uint32_t rcType = flagsRcVertexType(srcSample.flags);
+    if (rcType == RcVertexTypeSurface && length(L3(srcSample.rcWi)) > 0.5f) {
+        Li = Li * (evalBSDF(rcMat, rcSurf.albedo, rcSurf.norm, -wi, L3(srcSample.rcWi)) *
+                   satDot(rcSurf.norm, L3(srcSample.rcWi)));
+    }
+    Li = Li * (evalBSDF(rcPrevMat, rcPrevSurf.albedo, rcPrevSurf.norm, rc.rcPrevWo, wi) * satDot(rcPrevSurf.norm, wi));
+    Li = Li * rc.rcPrevThroughput;
+    Li = Li / srcSample.rcPrevSamplePdf;
+    return Li;
+}
+// Learned target: a surface rc vertex's radiance is replaced by the prediction.
+static vec3 targetRcLi(const GRISPathSample& ps, const LearnedData& L) {
+    return flagsRcVertexType(ps.flags) == RcVertexTypeSurface ? V3(L.L[0], L.L[1], L.L[2]) : L3(ps.rcLi);
+}
+
+// dstL/srcL are null for the luminance target; with the learned target the
+// sites gris_retrace.glsl:204 (dstPHat) and :215 (srcPHat) use the learned
+// values and the picked sample's learned data moves with it.
+static void reuseAndMerge(const Ctx& c, const GRISSettings& set, GRISReservoir& dstResv, LearnedData* dstL,
+                          const SurfaceInfo& dstPrimarySurf, const Intersection& dstPrimaryIsec, const Ray& primaryRay,
+                          GRISReservoir srcResv, LearnedData srcL, bool learned, uint32_t& rng) {
     const SceneArrays& S = *c.S;
     GRISPathSample srcSample = srcResv.pathSample;
     RcData dstRc;
@@ -479,16 +503,11 @@ static void reuseAndMerge(const Ctx& c, const GRISSettings& set, GRISReservoir& 
     }
     if (srcSampleValid) {
         uint32_t rcType = flagsRcVertexType(srcSample.flags);
+        vec3 LiL = V3s(0.0f);
         if (!std::isnan(srcSample.rcPrevSamplePdf) && srcSample.rcPrevSamplePdf > 1e-6f) {
-            Li = L3(srcSample.rcLi);
-            if (rcType == RcVertexTypeSurface && length(L3(srcSample.rcWi)) > 0.5f) {
-                Li = Li * (evalBSDF(rcMat, rcSurf.albedo, rcSurf.norm, -wi, L3(srcSample.rcWi)) *
-                           satDot(rcSurf.norm, L3(srcSample.rcWi)));
-            }
-            Li = Li * (evalBSDF(rcPrevMat, rcPrevSurf.albedo, rcPrevSurf.norm, dstRc.rcPrevWo, wi) * satDot(rcPrevSurf.norm, wi));
-            Li = Li * dstRc.rcPrevThroughput;
-            Li = Li / srcSample.rcPrevSamplePdf;
-            if (!isBlack(Li) && !hasNan(Li)) dstPHat = luminance(Li * jacobian);
+            Li = reconnectLi(srcSample, L3(srcSample.rcLi), rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, dstRc);
+            if (learned) LiL = reconnectLi(srcSample, targetRcLi(srcSample, srcL), rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, dstRc);
+            if (!isBlack(Li) && !hasNan(Li)) dstPHat = luminance((learned ? LiL : Li) * jacobian);
             if (rcType == RcVertexTypeLightSampled) {
                 float sumPower = S.lightTable[0].prob;
                 dstSamplePdf = luminance(rcSurf.albedo) / sumPower / dstJacobian;
@@ -496,7 +515,8 @@ static void reuseAndMerge(const Ctx& c, const GRISSettings& set, GRISReservoir& 
                 dstSamplePdf = evalPdf(rcPrevMat, rcPrevSurf.norm, dstRc.rcPrevWo, wi);
             }
         }
-        float srcPHat = luminance(L3(srcSample.F));
+        float srcPHat = learned ? srcL.pRef : luminance(L3(srcSample.F));
+        if (learned) srcL.pRef = luminance(LiL);
         srcSample.rcJacobian = dstJacobian;
         srcSample.rcPrevSamplePdf = dstSamplePdf;
         setV(srcSample.F, Li);
@@ -506,8 +526,19 @@ static void reuseAndMerge(const Ctx& c, const GRISSettings& set, GRISReservoir& 
     } else {
         srcResv.resampleWeight = 0.0f;
     }
-    if (reservoirIsValid(srcResv)) reservoirMerge(dstResv, srcResv, sample1f(rng));
+    if (reservoirIsValid(srcResv) && reservoirMerge(dstResv, srcResv, sample1f(rng)) && dstL) *dstL = srcL;
     reservoirCap(dstResv, (float)set.cap);
+}
+
+// Index of a reservoir read, or -1 where the read is out of range (reads zero).
+static inline int64_t resvIndex(uint32_t W, uint32_t H, int x, int y) {
+    uint64_t idx = (uint64_t)W * (uint32_t)y + (uint32_t)x;
+    return (x < 0 || y < 0 || idx >= (uint64_t)W * H) ? -1 : (int64_t)idx;
+}
+static inline LearnedData learnedAt(const LearnedData* buf, int64_t idx) {
+    LearnedData l{{0.0f, 0.0f, 0.0f}, 0.0f};
+    if (buf && idx >= 0) l = buf[idx];
+    return l;
 }
 
 static inline GRISReservoir resvAt(const GRISReservoir* buf, uint32_t W, uint32_t H, int x, int y) {
@@ -521,7 +552,7 @@ static inline GRISReservoir resvAt(const GRISReservoir* buf, uint32_t W, uint32_
 // gris_resample_temporal.glsl:11-34
 static bool findPreviousReservoir(const Camera& cam, const Camera& prevCam, const GBufView& gPrev,
                                   const GRISReservoir* resvPrev, vec2 uv, vec3 pos, vec3 normal, int matMeshId,
-                                  GRISReservoir& resv) {
+                                  GRISReservoir& resv, int64_t& idx) {
     if (uv.x < 0.0f || uv.y < 0.0f || uv.x > 1.0f || uv.y > 1.0f) return false;
     // GLSL leaves ivec2(NaN) undefined: a non-finite reprojection is rejected
     if (std::isnan(uv.x) || std::isnan(uv.y)) return false;
@@ -534,13 +565,16 @@ static bool findPreviousReservoir(const Camera& cam, const Camera& prevCam, cons
     vec3 posPrev = ray.ori + ray.dir * (depthPrev - 1e-4f);
     if (matMeshIdPrev != matMeshId || dot(normalPrev, normal) < 0.95f || distance(pos, posPrev) > 0.5f) return false;
     resv = resvAt(resvPrev, cam.filmSize[0], cam.filmSize[1], px, py);
+    idx = resvIndex(cam.filmSize[0], cam.filmSize[1], px, py);
     return true;
 }
 
 // gris_resample_temporal.glsl:36-83
 bool temporalReuse(const Ctx& c, const Camera& cam, const Camera& prevCam, const GBufView& g, const GBufView& gPrev,
                    const GRISSettings& set, const GRISReservoir* resvThis, const GRISReservoir* resvPrev,
-                   uint32_t x, uint32_t y, GRISReservoir& outTemp) {
+                   uint32_t x, uint32_t y, GRISReservoir& outTemp, const LearnedData* learnThis,
+                   const LearnedData* learnPrev, LearnedData* outLearn) {
+    const bool learned = set.targetFunction == 1u;
     const SceneArrays& S = *c.S;
     vec2 uv = V2(((float)x + 0.5f) / (float)g.W, ((float)y + 0.5f) / (float)g.H);
     float depth; vec3 norm, albedo; int matMeshId;
@@ -554,6 +588,7 @@ bool temporalReuse(const Ctx& c, const Camera& cam, const Camera& prevCam, const
     vec3 pos = ray.ori + ray.dir * (depth - 1e-4f);
     (void)S;
     GRISReservoir resv = resvThis[index1D(g.W, x, y)];
+    LearnedData L = learnedAt(learned ? learnThis : nullptr, (int64_t)index1D(g.W, x, y));
     Intersection dstPrimaryIsec{};
     dstPrimaryIsec.instanceIdx = SpecialHitIndex;
     dstPrimaryIsec.bary[0] = uv.x; dstPrimaryIsec.bary[1] = uv.y;
@@ -561,21 +596,24 @@ bool temporalReuse(const Ctx& c, const Camera& cam, const Camera& prevCam, const
         GRISReservoir temporalResv; std::memset(&temporalResv, 0, sizeof(temporalResv));
         reservoirReset(temporalResv);
         SurfaceInfo dstPrimarySurf{pos, norm, albedo, (uint32_t)matId, false};
+        int64_t pidx = -1;
         if (((cam.frameIndex & CameraClearFlag) == 0) &&
-            findPreviousReservoir(cam, prevCam, gPrev, resvPrev, uv + motion, pos, norm, matMeshId, temporalResv)) {
+            findPreviousReservoir(cam, prevCam, gPrev, resvPrev, uv + motion, pos, norm, matMeshId, temporalResv, pidx)) {
             if (reservoirIsValid(temporalResv)) {
-                reuseAndMerge(c, set, resv, dstPrimarySurf, dstPrimaryIsec, ray, temporalResv, resvRng);
+                reuseAndMerge(c, set, resv, learned ? &L : nullptr, dstPrimarySurf, dstPrimaryIsec, ray, temporalResv,
+                              learnedAt(learned ? learnPrev : nullptr, pidx), learned, resvRng);
             }
         }
     }
     if (!reservoirIsValid(resv)) reservoirReset(resv);
     outTemp = resv;
+    if (outLearn) *outLearn = L;
     return true;
 }
 
 // gris_resample_spatial.glsl:11-34
 static bool findNeighborReservoir(const Camera& cam, const GBufView& g, const GRISReservoir* resvTemp, vec2 uv, vec3 pos,
-                                  vec3 normal, GRISReservoir& resv) {
+                                  vec3 normal, GRISReservoir& resv, int64_t& idx) {
     if (uv.x < 0.0f || uv.y < 0.0f || uv.x > 1.0f || uv.y > 1.0f) return false;
     int px = (int)(uv.x * (float)cam.filmSize[0]);
     int py = (int)(uv.y * (float)cam.filmSize[1]);
@@ -586,13 +624,16 @@ static bool findNeighborReservoir(const Camera& cam, const GBufView& g, const GR
     vec3 posPrev = ray.ori + ray.dir * (depthPrev - 1e-4f);
     if (dot(normalPrev, normal) < 0.9f || distance(pos, posPrev) > 0.4f) return false;
     resv = resvAt(resvTemp, cam.filmSize[0], cam.filmSize[1], px, py);
+    idx = resvIndex(cam.filmSize[0], cam.filmSize[1], px, py);
     return true;
 }
 
 // gris_resample_spatial.glsl:36-135
 vec3 spatialReuse(const Ctx& c, const Camera& cam, const GBufView& g, const GRISSettings& set,
-                  const GRISReservoir* resvTemp, uint32_t x, uint32_t y, GRISReservoir& outThis, bool& written) {
+                  const GRISReservoir* resvTemp, uint32_t x, uint32_t y, GRISReservoir& outThis, bool& written,
+                  const LearnedData* learnTemp, LearnedData* outLearn) {
     written = false;
+    const bool learned = set.targetFunction == 1u;
     const SceneArrays& S = *c.S;
     vec2 uv = V2(((float)x + 0.5f) / (float)g.W, ((float)y + 0.5f) / (float)g.H);
     float depth; vec3 norm, albedo; int matMeshId;
@@ -604,6 +645,7 @@ vec3 spatialReuse(const Ctx& c, const Camera& cam, const GBufView& g, const GRIS
     int matId = matMeshId >> 16;
     vec3 radiance = V3s(0.0f);
     GRISReservoir resv = resvTemp[index1D(g.W, x, y)];
+    LearnedData L = learnedAt(learned ? learnTemp : nullptr, (int64_t)index1D(g.W, x, y));
     Intersection dstPrimaryIsec{};
     dstPrimaryIsec.instanceIdx = SpecialHitIndex;
     dstPrimaryIsec.bary[0] = uv.x; dstPrimaryIsec.bary[1] = uv.y;
@@ -615,15 +657,18 @@ vec3 spatialReuse(const Ctx& c, const Camera& cam, const GBufView& g, const GRIS
             vec2 neighbor = uv + (d * set.spatialRadius) * texelSize;
             GRISReservoir neighborResv; std::memset(&neighborResv, 0, sizeof(neighborResv));
             reservoirReset(neighborResv);
-            if (findNeighborReservoir(cam, g, resvTemp, neighbor, pos, norm, neighborResv)) {
+            int64_t nidx = -1;
+            if (findNeighborReservoir(cam, g, resvTemp, neighbor, pos, norm, neighborResv, nidx)) {
                 if (reservoirIsValid(neighborResv)) {
-                    reuseAndMerge(c, set, resv, dstPrimarySurf, dstPrimaryIsec, ray, neighborResv, rng);
+                    reuseAndMerge(c, set, resv, learned ? &L : nullptr, dstPrimarySurf, dstPrimaryIsec, ray, neighborResv,
+                                  learnedAt(learned ? learnTemp : nullptr, nidx), learned, rng);
                 }
             }
         }
     }
     if (!reservoirIsValid(resv)) reservoirReset(resv);
     outThis = resv;
+    if (outLearn) *outLearn = L;
     written = true;
     if (reservoirIsValid(resv) && resv.sampleCount > 0.0f) {
         GRISPathSample ps = resv.pathSample;
@@ -637,22 +682,51 @@ vec3 spatialReuse(const Ctx& c, const Camera& cam, const GBufView& g, const GRIS
                 loadSurfaceInfo(S, ps.rcIsec, rcSurf);
                 const Material& rcMat = S.materials[rcSurf.matIndex];
                 const Material& rcPrevMat = S.materials[rcPrevSurf.matIndex];
-                vec3 Li = L3(ps.rcLi);
-                uint32_t rcType = flagsRcVertexType(ps.flags);
                 vec3 wi = normalize(rcSurf.pos - rcPrevSurf.pos);
-                if (rcType == RcVertexTypeSurface && length(L3(ps.rcWi)) > 0.5f) {
-                    Li = Li * (evalBSDF(rcMat, rcSurf.albedo, rcSurf.norm, -wi, L3(ps.rcWi)) * satDot(rcSurf.norm, L3(ps.rcWi)));
-                }
-                Li = Li * (evalBSDF(rcPrevMat, rcPrevSurf.albedo, rcPrevSurf.norm, rc.rcPrevWo, wi) * satDot(rcPrevSurf.norm, wi));
-                Li = Li * rc.rcPrevThroughput;
-                Li = Li / ps.rcPrevSamplePdf;
+                vec3 Li = reconnectLi(ps, L3(ps.rcLi), rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, rc);
+                float pHat = learned
+                                 ? luminance(reconnectLi(ps, targetRcLi(ps, L), rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, rc))
+                                 : luminance(Li);
                 if (!isBlack(Li) && !hasNan(Li)) {
-                    radiance = Li / luminance(Li) * resv.resampleWeight / resv.sampleCount;
+                    radiance = Li / pHat * resv.resampleWeight / resv.sampleCount;
                 }
             }
         }
     }
     return clampColor(radiance);
+}
+
+// Learned-target refresh (no reference counterpart): for a surface rc vertex
+// the contribution with the predicted radiance is F_L = F * Lhat / rcLi per
+// channel (F is linear in rcLi); resampleWeight, which after stream RIS refers
+// to luminance(F), is rescaled to refer to luminance(F_L).
+bool learnedRefresh(const Ctx& c, const GBufView& g, uint32_t x, uint32_t y, GRISReservoir& resv, const float Lhat[3],
+                    LearnedData& out, bool& need, vec3& pos) {
+    const size_t p = index1D(g.W, x, y);
+    need = false;
+    pos = V3s(0.0f);
+    if (g.depthNormal[4 * p] == 0.0f) return false;
+    const GRISPathSample& ps = resv.pathSample;
+    need = reservoirIsValid(resv) && ps.rcIsec.instanceIdx != InvalidHitIndex &&
+           flagsRcVertexType(ps.flags) == RcVertexTypeSurface;
+    if (need) {
+        SurfaceInfo sf{};
+        loadSurfaceInfo(*c.S, ps.rcIsec, sf);
+        pos = sf.pos;
+    }
+    vec3 L = V3s(0.0f), F = L3(ps.F), FL = F;
+    const float pOld = luminance(F);
+    if (need) {
+        L = V3(Lhat[0], Lhat[1], Lhat[2]);
+        FL.x = ps.rcLi[0] > 0.0f ? F.x * (L.x / ps.rcLi[0]) : F.x;
+        FL.y = ps.rcLi[1] > 0.0f ? F.y * (L.y / ps.rcLi[1]) : F.y;
+        FL.z = ps.rcLi[2] > 0.0f ? F.z * (L.z / ps.rcLi[2]) : F.z;
+    }
+    const float pNew = luminance(FL);
+    if (need && pOld > 0.0f) resv.resampleWeight = resv.resampleWeight * (pNew / pOld);
+    out.L[0] = L.x; out.L[1] = L.y; out.L[2] = L.z;
+    out.pRef = need ? pNew : pOld;
+    return true;
 }
 
 }  // namespace orc

@@ -22,7 +22,11 @@ struct GRISSettings {
     uint32_t maxDepth;          // MaxTracingDepth (gris_path_trace.glsl:46, 15)
     uint32_t spatialNeighbors;  // ResampleNum (gris_resample_spatial.glsl:65, 3)
     float spatialRadius;        // ResampleRadius (gris_resample_spatial.glsl:66, 20)
+    uint32_t targetFunction;    // 0 luminance (GRISToScalar), 1 learned
 };
+// Learned-target data of a reservoir: predicted reconnection radiance and the
+// target value of the sample that resampleWeight refers to.
+struct LearnedData { float L[3]; float pRef; };
 static const uint32_t ShiftReconnection = 0, ShiftReplay = 1, ShiftHybrid = 2;
 static const uint32_t RcVertexTypeLightSampled = 0, RcVertexTypeLightScattered = 1, RcVertexTypeSurface = 2;
 static const float GRISDistanceThreshold = 0.01f;
@@ -142,8 +146,16 @@ vec3 tracePath(const Ctx& c, const Camera& cam, const GBufView& g, const GRISSet
                GRISReservoir& out, bool& written);
 bool temporalReuse(const Ctx& c, const Camera& cam, const Camera& prevCam, const GBufView& g, const GBufView& gPrev,
                    const GRISSettings& set, const GRISReservoir* resvThis, const GRISReservoir* resvPrev,
-                   uint32_t x, uint32_t y, GRISReservoir& outTemp);
+                   uint32_t x, uint32_t y, GRISReservoir& outTemp, const LearnedData* learnThis = nullptr,
+                   const LearnedData* learnPrev = nullptr, LearnedData* outLearn = nullptr);
 vec3 spatialReuse(const Ctx& c, const Camera& cam, const GBufView& g, const GRISSettings& set,
-                  const GRISReservoir* resvTemp, uint32_t x, uint32_t y, GRISReservoir& outThis, bool& written);
+                  const GRISReservoir* resvTemp, uint32_t x, uint32_t y, GRISReservoir& outThis, bool& written,
+                  const LearnedData* learnTemp = nullptr, LearnedData* outLearn = nullptr);
+// Learned-target refresh of one pixel's reservoir after the path-trace pass,
+// given the network's prediction Lhat at its rc vertex. Returns false for
+// background pixels (nothing written). need: the sample has a surface rc
+// vertex (pos = its position).
+bool learnedRefresh(const Ctx& c, const GBufView& g, uint32_t x, uint32_t y, GRISReservoir& resv, const float Lhat[3],
+                    LearnedData& out, bool& need, vec3& pos);
 
 }  // namespace orc

@@ -93,6 +93,18 @@ class GrisRun:
         f = self.frame_struct()
         rt.check(rt.core().rt_pt_plain(self.ds.handle, cams, C.byref(f), max_depth, which, stream()), "pt plain")
 
+    def learned_refresh(self, mlp, cams):
+        f = self.frame_struct()
+        rt.check(rt.core().rt_gris_learned_refresh(self.ds.handle, mlp.handle, cams, C.byref(f), stream()), "refresh")
+
+    def learned(self, buf):
+        """Learned-target plane of a reservoir buffer: (W*H, 4) float32."""
+        out = np.zeros((self.W * self.H, 4), np.float32)
+        torch.cuda.synchronize()
+        rt.check(rt.core().rt_gris_learned_download(buf.data_ptr(), self.W, self.H, out.ctypes.data, stream()),
+                 "learned download")
+        return out
+
     def output(self, which=1):
         t = self.indirect if which == 1 else self.direct
         return to_host(t, np.float32, (self.H, self.W, 4))

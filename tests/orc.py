@@ -45,8 +45,17 @@ def lib():
         L.orc_pt_plain.argtypes = [P, C.POINTER(rt.Camera), P, P, u32, P, P, P, C.c_int]
         L.orc_gris_path_trace.argtypes = [P, C.POINTER(rt.Camera), P, P, C.POINTER(rt.GrisSettings), P, P, C.c_int]
         L.orc_gris_temporal.argtypes = [P, C.POINTER(rt.Camera), P, P, P, P, P, C.POINTER(rt.GrisSettings), P, P, P,
-                                        P, C.c_int]
-        L.orc_gris_spatial.argtypes = [P, C.POINTER(rt.Camera), P, P, C.POINTER(rt.GrisSettings), P, P, P, P, C.c_int]
+                                        P, C.c_int, P, P, P]
+        L.orc_gris_spatial.argtypes = [P, C.POINTER(rt.Camera), P, P, C.POINTER(rt.GrisSettings), P, P, P, P, C.c_int,
+                                       P, P]
+        L.orc_gris_learned_refresh.argtypes = [P, u32, u32, P, P, P, P, P, P, u32, u32, C.c_int]
+        L.orc_mlp_forward.argtypes = [u32, u32, u32, P, P, P, P, P, u32, P, C.c_int]
+        L.orc_mlp_gradients.argtypes = [u32, u32, u32, P, P, P, P, P, P, u32, P, P, C.c_int]
+        L.orc_mlp_gradients.restype = C.c_double
+        L.orc_mlp_loss.argtypes = [u32, u32, u32, P, P, P, P, P, P, u32, P, P]
+        L.orc_mlp_loss.restype = C.c_double
+        L.orc_mlp_levels.argtypes = [u32, u32, u32, P]
+        L.orc_mlp_levels.restype = u32
         L.orc_hash2.argtypes = [u32]
         L.orc_hash2.restype = u32
         L.orc_make_seed.argtypes = [u32, u32, u32]
@@ -143,17 +152,31 @@ class OracleScene:
         lib().orc_gris_path_trace(self.h, cams, p(dn), p(am), C.byref(settings), p(resv), p(st), nthreads())
         return st
 
-    def gris_temporal(self, cams, dn, am, mv, dn_prev, am_prev, settings, resv_this, resv_prev, resv_temp):
+    def gris_temporal(self, cams, dn, am, mv, dn_prev, am_prev, settings, resv_this, resv_prev, resv_temp,
+                      learn_this=None, learn_prev=None, learn_temp=None):
         st = np.zeros(3, np.uint64)
         lib().orc_gris_temporal(self.h, cams, p(dn), p(am), p(mv), p(dn_prev), p(am_prev), C.byref(settings),
-                                p(resv_this), p(resv_prev), p(resv_temp), p(st), nthreads())
+                                p(resv_this), p(resv_prev), p(resv_temp), p(st), nthreads(), p(learn_this),
+                                p(learn_prev), p(learn_temp))
         return st
 
-    def gris_spatial(self, cams, dn, am, settings, resv_temp, resv_this, indirect):
+    def gris_spatial(self, cams, dn, am, settings, resv_temp, resv_this, indirect, learn_temp=None, learn_this=None):
         st = np.zeros(3, np.uint64)
         lib().orc_gris_spatial(self.h, cams, p(dn), p(am), C.byref(settings), p(resv_temp), p(resv_this),
-                               p(indirect), p(st), nthreads())
+                               p(indirect), p(st), nthreads(), p(learn_temp), p(learn_this))
         return st
+
+    def learned_refresh(self, W, H, dn, resv, lhat4, stride, offset):
+        """Refreshes resv in place from the predictions lhat4 (H*W x 4); returns (learned planes, batch pos,
+        batch target) with the batch over pixels p % stride == offset."""
+        n = W * H
+        learn = np.zeros((n, 4), np.float32)
+        slots = (n - offset + stride - 1) // stride
+        bpos = np.zeros((slots, 3), np.float32)
+        btgt = np.zeros((slots, 3), np.float32)
+        lib().orc_gris_learned_refresh(self.h, W, H, p(dn), p(resv), p(np.ascontiguousarray(lhat4, np.float32)),
+                                       p(learn), p(bpos), p(btgt), stride, offset, nthreads())
+        return learn, bpos, btgt
 
 
 RESV_DTYPE = np.dtype([("bary", np.float32, 2), ("instanceIdx", np.uint32), ("triangleIdx", np.uint32),
@@ -162,6 +185,42 @@ RESV_DTYPE = np.dtype([("bary", np.float32, 2), ("instanceIdx", np.uint32), ("tr
                        ("F", np.float32, 3), ("primaryRng", np.uint32), ("sampleCount", np.float32),
                        ("resampleWeight", np.float32), ("contribWeight", np.float32), ("pad0", np.float32)])
 assert RESV_DTYPE.itemsize == 96
+
+
+class OracleMlp:
+    """Scalar restatement of the hash-grid MLP (mode 1: FP16 rounding as the kernels; mode 0: exact)."""
+
+    def __init__(self, log2_table, base_res, max_res, lo, hi):
+        self.log2, self.base, self.max = log2_table, base_res, max_res
+        self.lo = np.asarray(lo, np.float32)
+        self.hi = np.asarray(hi, np.float32)
+
+    def levels(self):
+        res = np.zeros(16, np.uint32)
+        dense = lib().orc_mlp_levels(self.log2, self.base, self.max, p(res))
+        return res, dense
+
+    def forward(self, grid, weights, pos, mode=1):
+        pos = np.ascontiguousarray(pos, np.float32)
+        out = np.zeros((len(pos), 3), np.float32)
+        lib().orc_mlp_forward(self.log2, self.base, self.max, p(self.lo), p(self.hi), p(grid), p(weights), p(pos),
+                              len(pos), p(out), mode)
+        return out
+
+    def gradients(self, grid, weights, pos, target, mode=1):
+        pos = np.ascontiguousarray(pos, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        gg = np.zeros(len(grid), np.float64)
+        gw = np.zeros(len(weights), np.float64)
+        loss = lib().orc_mlp_gradients(self.log2, self.base, self.max, p(self.lo), p(self.hi), p(grid), p(weights),
+                                       p(pos), p(target), len(pos), p(gg), p(gw), mode)
+        return loss, gg, gw
+
+    def loss(self, grid, weights, pos, target, den_in=None, den_out=None):
+        pos = np.ascontiguousarray(pos, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        return lib().orc_mlp_loss(self.log2, self.base, self.max, p(self.lo), p(self.hi), p(grid), p(weights), p(pos),
+                                  p(target), len(pos), p(den_in), p(den_out))
 
 
 def reset_reservoirs(n):

@@ -38,8 +38,10 @@ def test_struct_sizes_match_reference_layouts():
     assert C.sizeof(rt.ObjectInstance) == 224
     assert C.sizeof(rt.TriangleLight) == 64
     assert C.sizeof(rt.Material) == 32 and C.sizeof(rt.MeshVertex) == 32
-    assert C.sizeof(rt.GrisSettings) == 32     # 20-B push constant + 3 kernel constants
-    assert rt.core().rt_gris_reservoir_bytes(1920, 1080) == 1920 * 1080 * 80
+    assert C.sizeof(rt.GrisSettings) == 36     # 20-B push constant + 3 kernel constants + target function
+    assert C.sizeof(rt.MlpConfig) == 72
+    # five 16-B planes of reference fields + the learned-target plane = the reference's 96-B reservoir
+    assert rt.core().rt_gris_reservoir_bytes(1920, 1080) == 1920 * 1080 * 96
 
 
 def test_argument_checks_fail_loudly():

@@ -30,6 +30,7 @@ hipError_t launchGrisTemporal(const rt_device_scene*, const rt_camera*, const rt
 hipError_t launchGrisSpatial(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_gris_settings&,
                              hipStream_t);
 hipError_t launchResvReset(void*, uint32_t, uint32_t, hipStream_t);
+hipError_t launchLearnedRefresh(const rt_device_scene*, rt_mlp*, const rt_camera*, const rt_frame&, hipStream_t);
 
 }  // namespace rti
 
@@ -104,6 +105,7 @@ rt_status checkSettings(const rt_gris_settings* st) {
     if (st->maxDepth == 0 || st->maxDepth > 255) return invalid("maxDepth must be in [1, 255]");
     if (st->shiftType > 2) return invalid("shiftType must be 0, 1 or 2");
     if (st->spatialNeighbors > 64) return invalid("spatialNeighbors must be <= 64");
+    if (st->targetFunction > 1) return invalid("targetFunction must be 0 (luminance) or 1 (learned)");
     return RT_OK;
 }
 
@@ -282,7 +284,7 @@ rt_status rt_gris_render(const rt_device_scene* s, const rt_camera cams[2], cons
     return st;
 }
 
-size_t rt_gris_reservoir_bytes(uint32_t w, uint32_t h) { return (size_t)w * h * 80; }
+size_t rt_gris_reservoir_bytes(uint32_t w, uint32_t h) { return (size_t)w * h * 96; }
 
 rt_status rt_gris_reservoir_reset(void* r, uint32_t w, uint32_t h, hipStream_t stream) {
     if (!r) return invalid("reservoir buffer is null");
@@ -330,6 +332,32 @@ rt_status rt_gris_reservoir_upload(const rt_gris_reservoir* host, uint32_t w, ui
     }
     RT_HIP(hipMemcpyAsync(r, pl.data(), sizeof(float4) * 5 * n, hipMemcpyHostToDevice, stream));
     RT_HIP(hipStreamSynchronize(stream));
+    return RT_OK;
+}
+
+rt_status rt_gris_learned_download(const void* r, uint32_t w, uint32_t h, float* host4, hipStream_t stream) {
+    if (!r || !host4) return invalid("null argument");
+    const size_t n = (size_t)w * h;
+    RT_HIP(hipMemcpyAsync(host4, static_cast<const float4*>(r) + 5 * n, sizeof(float4) * n, hipMemcpyDeviceToHost, stream));
+    RT_HIP(hipStreamSynchronize(stream));
+    return RT_OK;
+}
+
+rt_status rt_gris_learned_upload(const float* host4, uint32_t w, uint32_t h, void* r, hipStream_t stream) {
+    if (!r || !host4) return invalid("null argument");
+    const size_t n = (size_t)w * h;
+    RT_HIP(hipMemcpyAsync(static_cast<float4*>(r) + 5 * n, host4, sizeof(float4) * n, hipMemcpyHostToDevice, stream));
+    RT_HIP(hipStreamSynchronize(stream));
+    return RT_OK;
+}
+
+rt_status rt_gris_learned_refresh(const rt_device_scene* s, rt_mlp* mlp, const rt_camera cams[2], const rt_frame* f,
+                                  hipStream_t stream) {
+    rt_status st = checkFrame(s, cams, f);
+    if (st != RT_OK) return st;
+    if (!mlp) return invalid("mlp is null");
+    if (!f->grisReservoir) return invalid("frame reservoir is null");
+    RT_LAUNCH(rti::launchLearnedRefresh(s, mlp, cams, *f, stream));
     return RT_OK;
 }
 

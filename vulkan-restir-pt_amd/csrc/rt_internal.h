@@ -2,6 +2,7 @@
 // launchers (not part of the public interface).
 #pragma once
 #include "rt_dtrace.h"
+#include "rt_mlp.h"
 #include <atomic>
 #include <string>
 #include <vector>
@@ -33,6 +34,30 @@ struct rt_device_scene {
     rtd::WorkQueue* queues = nullptr;
     mutable std::atomic<uint32_t> queueTick{0};
     rtd::WorkQueue* nextQueue() const { return queues + (queueTick.fetch_add(1u) % kQueueSlots); }
+};
+
+// Learned target function: parameters, optimiser state and the frame's
+// training batch, all on the device.
+struct rt_mlp {
+    rt_mlp_config cfg{};
+    rtm::MlpView view{};          // kernels' view of the FP16 copies
+    uint64_t gridParams = 0;      // levels * T * 2
+    float* gridF32 = nullptr;     // FP32 master grid
+    _Float16* gridF16 = nullptr;  // FP16 copy read by the kernels
+    float* gridGrad = nullptr;    // summed per-sample gradients (unscaled)
+    float* gridM = nullptr;
+    float* gridV = nullptr;
+    float* wF32 = nullptr;
+    _Float16* wF16 = nullptr;
+    float* wGrad = nullptr;       // reduced weight gradients (unscaled), then loss sum, sample count
+    float* wM = nullptr;
+    float* wV = nullptr;
+    float* partial = nullptr;     // per-wave weight-gradient partial sums
+    uint32_t partialWaves = 0;
+    float* batchPos = nullptr;    // training batch gathered from the frame
+    float* batchTarget = nullptr;
+    uint32_t batchSize = 0;       // samples gathered by the last refresh
+    uint32_t step = 0;            // Adam steps taken
 };
 
 namespace rti {

@@ -1,0 +1,495 @@
+// Learned target function: parameter set-up, inference, fused forward +
+// backward over a training batch, deterministic weight-gradient reduction and
+// Adam, behind the rt_mlp_* entry points of include/restirpt.h.
+//
+// Training data flow per step: one wave per 32-sample tile evaluates the
+// network (rt_mlp.h), forms the relative-L2 output gradient, back-propagates
+// through the three layers on the matrix cores, scatters the encoding
+// gradient into the FP32 grid-gradient table with atomics, and accumulates
+// the weight gradients of all its tiles in registers. Per-wave partial sums
+// are reduced in a fixed order, so the weight gradients are deterministic;
+// the grid gradients are sums of atomics (order-dependent in the last bits).
+#include "rt_internal.h"
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+using namespace rtm;
+
+namespace {
+
+constexpr int kStats = 2;                 // loss sum, valid sample count (after the weight gradients)
+constexpr int kPartialStride = kWeights + kStats;
+constexpr uint32_t kMaxTrainWaves = 512;
+constexpr int kInferWaves = 4;
+
+// ---- initialisation -------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t splitmix(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__device__ __forceinline__ float uniformPM(uint64_t seed, uint64_t i) {  // [-1, 1)
+    return (float)(splitmix(seed * 0x100000001B3ull + i) >> 40) * (2.0f / 16777216.0f) - 1.0f;
+}
+
+__global__ void k_mlp_init_grid(float* g32, _Float16* g16, uint64_t n, uint32_t seed) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v = uniformPM(seed, i) * 1e-4f;
+    g32[i] = v;
+    g16[i] = (_Float16)v;
+}
+__global__ void k_mlp_init_weights(float* w32, _Float16* w16, uint32_t seed) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kWeights) return;
+    float lim;  // Glorot uniform per layer
+    if (i < kW2) lim = sqrtf(6.0f / (float)(kEnc + kHidden));
+    else if (i < kW3) lim = sqrtf(6.0f / (float)(kHidden + kHidden));
+    else lim = sqrtf(6.0f / (float)(kHidden + kOut));
+    float v = uniformPM((uint64_t)seed ^ 0xA5A5A5A5ull, (uint64_t)i) * lim;
+    w32[i] = v;
+    w16[i] = (_Float16)v;
+}
+__global__ void k_fill_f32(float* p, uint64_t n, float v) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_to_f16(const float* src, _Float16* dst, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (_Float16)src[i];
+}
+
+__device__ __forceinline__ void loadWeightsLDS(_Float16* sW, const _Float16* w) {
+    for (int i = threadIdx.x; i < kWeights; i += blockDim.x) sW[i] = w[i];
+    __syncthreads();
+}
+
+// ---- inference ------------------------------------------------------------------------------
+__global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(MlpView M, const float* __restrict__ pos, uint32_t n,
+                                                                float* __restrict__ rgb) {
+    __shared__ _Float16 sW[kWeights];
+    loadWeightsLDS(sW, M.w);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t s = (blockIdx.x * kInferWaves + wave) * kTileSamples + (lane & 31u);
+    float u = 0.0f, v = 0.0f, w = 0.0f;
+    if (s < n) normalizePos(M, pos[3 * (size_t)s], pos[3 * (size_t)s + 1], pos[3 * (size_t)s + 2], u, v, w);
+    h4 enc[4];
+    encodeLane(M, lane >> 5, u, v, w, enc);
+    Fwd F;
+    forwardTile(sW, lane, enc, F);
+    if (lane < 32u && s < n) {
+        rgb[3 * (size_t)s] = outAct(F.z3[0]);
+        rgb[3 * (size_t)s + 1] = outAct(F.z3[1]);
+        rgb[3 * (size_t)s + 2] = outAct(F.z3[2]);
+    }
+}
+
+// ---- fused forward + backward -----------------------------------------------------------------
+struct TrainLDS {
+    _Float16 w[kWeights];
+    _Float16 enc[kTileSamples][kEnc];     // encoding, [sample][feature]
+    _Float16 h1[kTileSamples][kHidden];   // relu(z1)
+    _Float16 h2[kTileSamples][kHidden];   // relu(z2)
+    _Float16 d1[kTileSamples][kHidden];   // dL/dz1
+    _Float16 d2[kTileSamples][kHidden];   // dL/dz2
+    _Float16 d3[kTileSamples][4];         // dL/dz3
+};
+
+// Row block g of an accumulator tile t of this lane: neuron 32t + 8g + 4h + e.
+__device__ __forceinline__ int neuron(int t, int g, int h, int e) { return 32 * t + 8 * g + 4 * h + e; }
+
+__global__ __launch_bounds__(64) void k_mlp_train(MlpView M, const float* __restrict__ pos,
+                                                  const float* __restrict__ target, uint32_t n,
+                                                  float* __restrict__ gridGrad, float* __restrict__ partial) {
+    __shared__ TrainLDS L;
+    loadWeightsLDS(L.w, M.w);
+    const uint32_t lane = threadIdx.x;
+    const int j = (int)(lane & 31u), h = (int)(lane >> 5);
+    const uint32_t tiles = (n + kTileSamples - 1) / kTileSamples;
+    f16v gW1[2], gW2[2][2], gW3[2];
+    for (int t = 0; t < 2; t++) { gW1[t] = zero16(); gW3[t] = zero16(); gW2[t][0] = zero16(); gW2[t][1] = zero16(); }
+    float lossSum = 0.0f, count = 0.0f;
+
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const uint32_t s = tile * kTileSamples + (uint32_t)j;
+        float u = 0.0f, v = 0.0f, w = 0.0f;
+        float y[3] = {-1.0f, 0.0f, 0.0f};
+        if (s < n) {
+            normalizePos(M, pos[3 * (size_t)s], pos[3 * (size_t)s + 1], pos[3 * (size_t)s + 2], u, v, w);
+            y[0] = target[3 * (size_t)s]; y[1] = target[3 * (size_t)s + 1]; y[2] = target[3 * (size_t)s + 2];
+        }
+        const bool valid = s < n && y[0] >= 0.0f && y[1] >= 0.0f && y[2] >= 0.0f && isfinite(y[0]) &&
+                           isfinite(y[1]) && isfinite(y[2]);
+        h4 enc[4];
+        encodeLane(M, (uint32_t)h, u, v, w, enc);
+        Fwd F;
+        forwardTile(L.w, lane, enc, F);
+
+        // output gradient (rows 0-2 live in registers 0-2 of lanes 0-31)
+        h4 d3b;
+        for (int e = 0; e < 4; e++) d3b[e] = (_Float16)0.0f;
+        if (h == 0) {
+            for (int c = 0; c < 3; c++) {
+                float z = F.z3[c];
+                float yh = outAct(z);
+                float den = yh * yh + kLossEps;
+                float diff = yh - y[c];
+                float g = 0.0f;
+                if (valid) {
+                    lossSum += diff * diff / den;
+                    // d/dz of (yh - y)^2 / sg(yh^2 + eps) with yh = exp(z)
+                    g = (fabsf(z) < kLogClamp) ? 2.0f * diff / den * yh : 0.0f;
+                    g = fminf(fmaxf(g, -kGradClip), kGradClip);
+                }
+                d3b[c] = (_Float16)g;
+            }
+            if (valid) count += 1.0f;
+        }
+        // dh2 = W3 dz3 (one k-step: outputs 0-7, of which 0-2 are real)
+        f16v dz2[2];
+        for (int t = 0; t < 2; t++) {
+            f16v acc = mfma(bwdA(L.w + kW3, kOut, 32 * t + j, 4 * h, kOut), d3b, zero16());
+            for (int r = 0; r < 16; r++) acc[r] = F.z2[t][r] > 0.0f ? acc[r] : 0.0f;
+            dz2[t] = acc;
+        }
+        // dh1 = W2 dz2
+        f16v dz1[2];
+        for (int t = 0; t < 2; t++) {
+            f16v acc = zero16();
+            for (int s8 = 0; s8 < 8; s8++) {
+                h4 b;
+                for (int e = 0; e < 4; e++) b[e] = (_Float16)dz2[s8 >> 2][4 * (s8 & 3) + e];
+                acc = mfma(bwdA(L.w + kW2, kHidden, 32 * t + j, 8 * s8 + 4 * h, kHidden), b, acc);
+            }
+            for (int r = 0; r < 16; r++) acc[r] = F.z1[t][r] > 0.0f ? acc[r] : 0.0f;
+            dz1[t] = acc;
+        }
+        // denc = W1 dz1: rows = the 32 encoding features, laid out like enc[]
+        f16v denc = zero16();
+        for (int s8 = 0; s8 < 8; s8++) {
+            h4 b;
+            for (int e = 0; e < 4; e++) b[e] = (_Float16)dz1[s8 >> 2][4 * (s8 & 3) + e];
+            denc = mfma(bwdA(L.w + kW1, kHidden, j, 8 * s8 + 4 * h, kHidden), b, denc);
+        }
+        // scatter the encoding gradient: register group g holds levels 4g + 2h (+1)
+        if (s < n && valid) {
+            for (int g = 0; g < 4; g++) {
+                for (int q = 0; q < 2; q++) {
+                    const int lev = 4 * g + 2 * h + q;
+                    const float d0 = denc[4 * g + 2 * q], d1 = denc[4 * g + 2 * q + 1];
+                    uint32_t ci[3]; float fr[3];
+                    levelCell(M, lev, u, v, w, ci, fr);
+                    for (int c = 0; c < 8; c++) {
+                        Corner k = corner(M, lev, ci, fr, c);
+                        atomicAdd(gridGrad + 2 * (size_t)k.slot, k.wt * d0);
+                        atomicAdd(gridGrad + 2 * (size_t)k.slot + 1, k.wt * d1);
+                    }
+                }
+            }
+        }
+        // stage activations and gradients as [sample][neuron] for the weight-gradient products
+        for (int s4 = 0; s4 < 4; s4++)
+            for (int e = 0; e < 4; e++) L.enc[j][8 * s4 + 4 * h + e] = enc[s4][e];
+        for (int t = 0; t < 2; t++)
+            for (int g = 0; g < 4; g++)
+                for (int e = 0; e < 4; e++) {
+                    const int nn = neuron(t, g, h, e);
+                    L.h1[j][nn] = (_Float16)fmaxf(F.z1[t][4 * g + e], 0.0f);
+                    L.h2[j][nn] = (_Float16)fmaxf(F.z2[t][4 * g + e], 0.0f);
+                    L.d1[j][nn] = (_Float16)dz1[t][4 * g + e];
+                    L.d2[j][nn] = (_Float16)dz2[t][4 * g + e];
+                }
+        if (h == 0)
+            for (int e = 0; e < 4; e++) L.d3[j][e] = d3b[e];
+        __syncthreads();
+        // dW^T[o][i] += sum over samples dz[o][j] * act[i][j]; k-step s4 covers samples 8 s4 + 4h + e
+        for (int s4 = 0; s4 < 4; s4++) {
+            const int jb = 8 * s4 + 4 * h;
+            h4 aD1[2], aD2[2], aD3, bEnc, bH1[2], bH2[2];
+            for (int e = 0; e < 4; e++) {
+                for (int t = 0; t < 2; t++) {
+                    aD1[t][e] = L.d1[jb + e][32 * t + j];
+                    aD2[t][e] = L.d2[jb + e][32 * t + j];
+                    bH1[t][e] = L.h1[jb + e][32 * t + j];
+                    bH2[t][e] = L.h2[jb + e][32 * t + j];
+                }
+                aD3[e] = (j < kOut) ? L.d3[jb + e][j] : (_Float16)0.0f;
+                bEnc[e] = L.enc[jb + e][j];
+            }
+            for (int t = 0; t < 2; t++) {
+                gW1[t] = mfma(aD1[t], bEnc, gW1[t]);
+                for (int u2 = 0; u2 < 2; u2++) gW2[t][u2] = mfma(aD2[t], bH1[u2], gW2[t][u2]);
+                gW3[t] = mfma(aD3, bH2[t], gW3[t]);
+            }
+        }
+        __syncthreads();
+    }
+
+    // per-wave partials in the weight layout (input-major): C row = output neuron, column = input
+    float* P = partial + (size_t)blockIdx.x * kPartialStride;
+    for (int t = 0; t < 2; t++)
+        for (int g = 0; g < 4; g++)
+            for (int e = 0; e < 4; e++) {
+                const int o = neuron(t, g, h, e), r = 4 * g + e;
+                P[kW1 + j * kHidden + o] = gW1[t][r];
+                for (int u2 = 0; u2 < 2; u2++) P[kW2 + (32 * u2 + j) * kHidden + o] = gW2[t][u2][r];
+            }
+    for (int t = 0; t < 2; t++)
+        if (h == 0)
+            for (int e = 0; e < kOut; e++) P[kW3 + (32 * t + j) * kOut + e] = gW3[t][e];
+    // loss and count: sum over lanes 0-31
+    for (int off = 16; off > 0; off >>= 1) {
+        lossSum += __shfl_xor(lossSum, off);
+        count += __shfl_xor(count, off);
+    }
+    if (lane == 0) { P[kWeights] = lossSum; P[kWeights + 1] = count; }
+}
+
+// Weight gradients and stats: fixed-order sum over the waves' partials.
+__global__ void k_mlp_reduce(const float* __restrict__ partial, uint32_t waves, float* __restrict__ out,
+                             float* __restrict__ lossOut) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= kPartialStride) return;
+    float acc = 0.0f;
+    for (uint32_t w = 0; w < waves; w++) acc += partial[(size_t)w * kPartialStride + k];
+    out[k] = acc;
+    if (k == 0 && lossOut) {
+        float cnt = 0.0f, sum = 0.0f;
+        for (uint32_t w = 0; w < waves; w++) {
+            sum += partial[(size_t)w * kPartialStride + kWeights];
+            cnt += partial[(size_t)w * kPartialStride + kWeights + 1];
+        }
+        *lossOut = cnt > 0.0f ? sum / (3.0f * cnt) : 0.0f;
+    }
+}
+
+// Adam on one parameter array; the gradient of the mean loss is grad / (3 count).
+// Grid entries no sample touched (gradient exactly zero) are left unchanged.
+__global__ void k_mlp_adam(float* __restrict__ p32, _Float16* __restrict__ p16, const float* __restrict__ grad,
+                           float* __restrict__ m, float* __restrict__ v, uint64_t n, const float* __restrict__ stats,
+                           float lr, float b1, float b2, float eps, float c1, float c2, int skipZero) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gr = grad[i];
+    if (skipZero && gr == 0.0f) return;
+    float cnt = stats[1];
+    float g = cnt > 0.0f ? gr / (3.0f * cnt) : 0.0f;
+    float mi = b1 * m[i] + (1.0f - b1) * g;
+    float vi = b2 * v[i] + (1.0f - b2) * g * g;
+    m[i] = mi; v[i] = vi;
+    float p = p32[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p32[i] = p;
+    p16[i] = (_Float16)p;
+}
+
+inline unsigned blocks(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+rt_status invalid(const char* m) { rti::setError(m); return RT_ERR_INVALID_ARGUMENT; }
+
+void freeMlp(rt_mlp* m) {
+    (void)hipFree(m->gridF32); (void)hipFree(m->gridF16); (void)hipFree(m->gridGrad); (void)hipFree(m->gridM);
+    (void)hipFree(m->gridV); (void)hipFree(m->wF32); (void)hipFree(m->wF16); (void)hipFree(m->wGrad);
+    (void)hipFree(m->wM); (void)hipFree(m->wV); (void)hipFree(m->partial); (void)hipFree(m->batchPos);
+    (void)hipFree(m->batchTarget);
+    delete m;
+}
+
+hipError_t zeroAdam(rt_mlp* m, hipStream_t st) {
+    k_fill_f32<<<blocks(m->gridParams, 256), 256, 0, st>>>(m->gridM, m->gridParams, 0.0f);
+    k_fill_f32<<<blocks(m->gridParams, 256), 256, 0, st>>>(m->gridV, m->gridParams, 0.0f);
+    k_fill_f32<<<blocks(kWeights, 256), 256, 0, st>>>(m->wM, kWeights, 0.0f);
+    k_fill_f32<<<blocks(kWeights, 256), 256, 0, st>>>(m->wV, kWeights, 0.0f);
+    m->step = 0;
+    return hipGetLastError();
+}
+
+}  // namespace
+
+namespace rti {
+// Gradients of a batch into m->gridGrad / m->wGrad (used by the ABI and the frame trainer).
+hipError_t mlpGradients(rt_mlp* m, const float* pos, const float* target, uint32_t n, float* loss, hipStream_t st) {
+    k_fill_f32<<<blocks(m->gridParams, 256), 256, 0, st>>>(m->gridGrad, m->gridParams, 0.0f);
+    const uint32_t tiles = (n + kTileSamples - 1) / kTileSamples;
+    const uint32_t waves = std::max(1u, std::min(tiles, m->partialWaves));
+    if (n) k_mlp_train<<<waves, 64, 0, st>>>(m->view, pos, target, n, m->gridGrad, m->partial);
+    else k_fill_f32<<<blocks(kPartialStride, 256), 256, 0, st>>>(m->partial, kPartialStride, 0.0f);
+    k_mlp_reduce<<<blocks(kPartialStride, 256), 256, 0, st>>>(m->partial, n ? waves : 1u, m->wGrad, loss);
+    return hipGetLastError();
+}
+hipError_t mlpAdam(rt_mlp* m, hipStream_t st) {
+    m->step++;
+    const rt_mlp_config& c = m->cfg;
+    const float c1 = 1.0f - (float)std::pow((double)c.beta1, (double)m->step);
+    const float c2 = 1.0f - (float)std::pow((double)c.beta2, (double)m->step);
+    const float* stats = m->wGrad + kWeights;
+    k_mlp_adam<<<blocks(m->gridParams, 256), 256, 0, st>>>(m->gridF32, m->gridF16, m->gridGrad, m->gridM, m->gridV,
+                                                            m->gridParams, stats, c.learningRate, c.beta1, c.beta2,
+                                                            c.epsilon, c1, c2, 1);
+    k_mlp_adam<<<blocks(kWeights, 256), 256, 0, st>>>(m->wF32, m->wF16, m->wGrad, m->wM, m->wV, kWeights, stats,
+                                                       c.learningRate, c.beta1, c.beta2, c.epsilon, c1, c2, 0);
+    return hipGetLastError();
+}
+}  // namespace rti
+
+extern "C" {
+
+rt_status rt_mlp_create(const rt_mlp_config* c, hipStream_t stream, rt_mlp** out) {
+    if (!out) return invalid("out is null");
+    *out = nullptr;
+    if (!c) return invalid("config is null");
+    if (c->levels != (uint32_t)kLevels || c->featuresPerLevel != (uint32_t)kFeatures || c->hiddenWidth != (uint32_t)kHidden ||
+        c->hiddenLayers != 2u) {
+        rti::setError("only 16 levels x 2 features and 2 hidden layers of 64 are supported");
+        return RT_ERR_UNSUPPORTED;
+    }
+    if (c->log2TableSize < 10 || c->log2TableSize > 24) return invalid("log2TableSize must be in [10, 24]");
+    if (c->baseResolution < 1 || c->maxResolution < c->baseResolution || c->maxResolution > 65535)
+        return invalid("resolutions must satisfy 1 <= base <= max <= 65535");
+    for (int a = 0; a < 3; a++)
+        if (!(c->boundsHi[a] > c->boundsLo[a])) return invalid("bounds must have positive extent");
+    if (!(c->learningRate > 0.0f) || !(c->beta1 >= 0.0f && c->beta1 < 1.0f) || !(c->beta2 >= 0.0f && c->beta2 < 1.0f) ||
+        !(c->epsilon >= 0.0f))
+        return invalid("invalid optimiser settings");
+    rt_mlp* m = new rt_mlp();
+    m->cfg = *c;
+    const uint64_t T = 1ull << c->log2TableSize;
+    m->gridParams = (uint64_t)kLevels * T * kFeatures;
+    MlpView& V = m->view;
+    V.tableMask = (uint32_t)(T - 1);
+    V.denseMask = 0;
+    const double b = std::exp((std::log((double)c->maxResolution) - std::log((double)c->baseResolution)) / (kLevels - 1));
+    for (int l = 0; l < kLevels; l++) {
+        uint32_t N = (uint32_t)std::floor((double)c->baseResolution * std::pow(b, (double)l) + 1e-9);
+        V.res[l] = N;
+        if ((uint64_t)(N + 1) * (N + 1) * (N + 1) <= T) V.denseMask |= 1u << l;
+    }
+    for (int a = 0; a < 3; a++) { V.lo[a] = c->boundsLo[a]; V.invExt[a] = 1.0f / (c->boundsHi[a] - c->boundsLo[a]); }
+    m->partialWaves = kMaxTrainWaves;
+    const size_t G = m->gridParams;
+    bool ok = rti::hipCheck(hipMalloc(&m->gridF32, 4 * G), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->gridF16, 2 * G), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->gridGrad, 4 * G), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->gridM, 4 * G), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->gridV, 4 * G), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->wF32, 4 * kWeights), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->wF16, 2 * kWeights), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->wGrad, 4 * kPartialStride), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->wM, 4 * kWeights), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->wV, 4 * kWeights), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->partial, 4 * (size_t)kPartialStride * kMaxTrainWaves), "hipMalloc");
+    if (ok) {
+        V.grid = reinterpret_cast<const uint32_t*>(m->gridF16);
+        V.w = m->wF16;
+        k_mlp_init_grid<<<blocks(G, 256), 256, 0, stream>>>(m->gridF32, m->gridF16, G, c->seed);
+        k_mlp_init_weights<<<blocks(kWeights, 256), 256, 0, stream>>>(m->wF32, m->wF16, c->seed);
+        k_fill_f32<<<blocks(G, 256), 256, 0, stream>>>(m->gridGrad, G, 0.0f);
+        k_fill_f32<<<blocks(kPartialStride, 256), 256, 0, stream>>>(m->wGrad, kPartialStride, 0.0f);
+        ok = rti::hipCheck(zeroAdam(m, stream), "init kernels") &&
+             rti::hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+    if (!ok) { freeMlp(m); return RT_ERR_HIP; }
+    *out = m;
+    return RT_OK;
+}
+
+rt_status rt_mlp_destroy(rt_mlp* m) {
+    if (!m) return invalid("mlp is null");
+    freeMlp(m);
+    return RT_OK;
+}
+
+rt_status rt_mlp_param_counts(const rt_mlp* m, uint64_t* gridParams, uint64_t* weightParams) {
+    if (!m) return invalid("mlp is null");
+    if (gridParams) *gridParams = m->gridParams;
+    if (weightParams) *weightParams = kWeights;
+    return RT_OK;
+}
+
+rt_status rt_mlp_get_params(const rt_mlp* m, float* grid, float* weights) {
+    if (!m) return invalid("mlp is null");
+    if (grid) RT_HIP(hipMemcpy(grid, m->gridF32, 4 * m->gridParams, hipMemcpyDeviceToHost));
+    if (weights) RT_HIP(hipMemcpy(weights, m->wF32, 4 * kWeights, hipMemcpyDeviceToHost));
+    return RT_OK;
+}
+
+rt_status rt_mlp_set_params(rt_mlp* m, const float* grid, const float* weights) {
+    if (!m) return invalid("mlp is null");
+    if (grid) {
+        RT_HIP(hipMemcpy(m->gridF32, grid, 4 * m->gridParams, hipMemcpyHostToDevice));
+        k_to_f16<<<blocks(m->gridParams, 256), 256>>>(m->gridF32, m->gridF16, m->gridParams);
+    }
+    if (weights) {
+        RT_HIP(hipMemcpy(m->wF32, weights, 4 * kWeights, hipMemcpyHostToDevice));
+        k_to_f16<<<blocks(kWeights, 256), 256>>>(m->wF32, m->wF16, kWeights);
+    }
+    RT_HIP(zeroAdam(m, nullptr));
+    RT_HIP(hipDeviceSynchronize());
+    return RT_OK;
+}
+
+rt_status rt_mlp_infer(const rt_mlp* m, const float* pos, uint32_t n, float* rgb, hipStream_t stream) {
+    if (!m) return invalid("mlp is null");
+    if (n == 0) return RT_OK;
+    if (!pos || !rgb) return invalid("null array");
+    const uint32_t tiles = (n + kTileSamples - 1) / kTileSamples;
+    k_mlp_infer<<<(tiles + kInferWaves - 1) / kInferWaves, 64 * kInferWaves, 0, stream>>>(m->view, pos, n, rgb);
+    RT_HIP(hipGetLastError());
+    return RT_OK;
+}
+
+rt_status rt_mlp_gradients(rt_mlp* m, const float* pos, const float* target, uint32_t n, float* loss,
+                           hipStream_t stream) {
+    if (!m) return invalid("mlp is null");
+    if (n && (!pos || !target)) return invalid("null array");
+    RT_HIP(rti::mlpGradients(m, pos, target, n, loss, stream));
+    return RT_OK;
+}
+
+rt_status rt_mlp_read_gradients(const rt_mlp* m, float* gridGrad, float* weightGrad) {
+    if (!m) return invalid("mlp is null");
+    RT_HIP(hipDeviceSynchronize());
+    std::vector<float> w(kPartialStride);
+    RT_HIP(hipMemcpy(w.data(), m->wGrad, 4 * kPartialStride, hipMemcpyDeviceToHost));
+    const float cnt = w[kWeights + 1];
+    const float scale = cnt > 0.0f ? 1.0f / (3.0f * cnt) : 0.0f;
+    if (weightGrad)
+        for (int i = 0; i < kWeights; i++) weightGrad[i] = w[i] * scale;
+    if (gridGrad) {
+        RT_HIP(hipMemcpy(gridGrad, m->gridGrad, 4 * m->gridParams, hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < m->gridParams; i++) gridGrad[i] *= scale;
+    }
+    return RT_OK;
+}
+
+rt_status rt_mlp_adam_step(rt_mlp* m, hipStream_t stream) {
+    if (!m) return invalid("mlp is null");
+    RT_HIP(rti::mlpAdam(m, stream));
+    return RT_OK;
+}
+
+rt_status rt_mlp_train_step(rt_mlp* m, const float* pos, const float* target, uint32_t n, float* loss,
+                            hipStream_t stream) {
+    rt_status s = rt_mlp_gradients(m, pos, target, n, loss, stream);
+    if (s != RT_OK) return s;
+    return rt_mlp_adam_step(m, stream);
+}
+
+rt_status rt_mlp_batch_download(const rt_mlp* m, float* pos, float* target, uint32_t* count) {
+    if (!m) return invalid("mlp is null");
+    const uint32_t n = m->batchPos ? m->batchSize : 0u;
+    if (count) *count = n;
+    RT_HIP(hipDeviceSynchronize());
+    if (n && pos) RT_HIP(hipMemcpy(pos, m->batchPos, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
+    if (n && target) RT_HIP(hipMemcpy(target, m->batchTarget, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
+    return RT_OK;
+}
+
+rt_status rt_mlp_train_frame(rt_mlp* m, float* loss, hipStream_t stream) {
+    if (!m) return invalid("mlp is null");
+    if (!m->batchPos || m->batchSize == 0) return invalid("no training batch: call rt_gris_learned_refresh first");
+    return rt_mlp_train_step(m, m->batchPos, m->batchTarget, m->batchSize, loss, stream);
+}
+
+}  // extern "C"

@@ -384,9 +384,13 @@ struct Resv {
     float rcPrevSamplePdf, rcJacobian;
     f3 F; uint32_t primaryRng;
     float sampleCount, resampleWeight;
+    // learned target only (plane P5): predicted reconnection radiance and the
+    // target value of the sample that resampleWeight refers to
+    f3 Lhat; float pRef;
 };
 struct ResvPlanes { float4* p; size_t n; };
 
+template <bool kLearned = false>
 RT_DEV Resv loadResv(const ResvPlanes& R, size_t i) {
     float4 a = R.p[i], b = R.p[R.n + i], c = R.p[2 * R.n + i], d = R.p[3 * R.n + i], e = R.p[4 * R.n + i];
     Resv r;
@@ -395,14 +399,22 @@ RT_DEV Resv loadResv(const ResvPlanes& R, size_t i) {
     r.rcWi = mk3(c.x, c.y, c.z); r.flags = __float_as_uint(c.w);
     r.F = mk3(d.x, d.y, d.z); r.primaryRng = __float_as_uint(d.w);
     r.rcPrevSamplePdf = e.x; r.rcJacobian = e.y; r.sampleCount = e.z; r.resampleWeight = e.w;
+    if (kLearned) {
+        float4 l = R.p[5 * R.n + i];
+        r.Lhat = mk3(l.x, l.y, l.z); r.pRef = l.w;
+    } else {
+        r.Lhat = s3(0.0f); r.pRef = 0.0f;
+    }
     return r;
 }
+template <bool kLearned = false>
 RT_DEV void storeResv(const ResvPlanes& R, size_t i, const Resv& r) {
     R.p[i] = make_float4(r.rcIsec.b1, r.rcIsec.b2, __uint_as_float(r.rcIsec.inst), __uint_as_float(r.rcIsec.prim));
     R.p[R.n + i] = make_float4(r.rcLi.x, r.rcLi.y, r.rcLi.z, __uint_as_float(r.rcRng));
     R.p[2 * R.n + i] = make_float4(r.rcWi.x, r.rcWi.y, r.rcWi.z, __uint_as_float(r.flags));
     R.p[3 * R.n + i] = make_float4(r.F.x, r.F.y, r.F.z, __uint_as_float(r.primaryRng));
     R.p[4 * R.n + i] = make_float4(r.rcPrevSamplePdf, r.rcJacobian, r.sampleCount, r.resampleWeight);
+    if (kLearned) R.p[5 * R.n + i] = make_float4(r.Lhat.x, r.Lhat.y, r.Lhat.z, r.pRef);
 }
 RT_DEV Resv zeroResv() {
     Resv r;
@@ -410,6 +422,7 @@ RT_DEV Resv zeroResv() {
     r.rcLi = s3(0.0f); r.rcRng = 0u; r.rcWi = s3(0.0f); r.flags = 0u;
     r.rcPrevSamplePdf = 0.0f; r.rcJacobian = 0.0f; r.F = s3(0.0f); r.primaryRng = 0u;
     r.sampleCount = 0.0f; r.resampleWeight = 0.0f;
+    r.Lhat = s3(0.0f); r.pRef = 0.0f;
     return r;
 }
 // gris_reservoir.glsl:37-136
@@ -756,12 +769,13 @@ RT_DEV void replayHybrid(const DevScene& S, TStack st, const rt_gris_settings& s
 
 // Reconnection through the rc vertex: the shading shared by
 // GRISReservoirReuseAndMerge (gris_retrace.glsl:188-213) and the final
-// shading of spatialReuse (gris_resample_spatial.glsl:114-125).
-RT_DEV f3 reconnectLi(const DevScene& S, const Resv& src, const Surf& rcPrevSurf, const Surf& rcSurf, f3 wi,
+// shading of spatialReuse (gris_resample_spatial.glsl:114-125). rcLi is the
+// sample's reconnection radiance (or, for the learned target, its prediction).
+RT_DEV f3 reconnectLi(const DevScene& S, const Resv& src, f3 rcLi, const Surf& rcPrevSurf, const Surf& rcSurf, f3 wi,
                       const RcData& rc) {
     Mat rcMat = loadMat(S, rcSurf.matIndex);
     Mat rcPrevMat = loadMat(S, rcPrevSurf.matIndex);
-    f3 Li = src.rcLi;
+    f3 Li = rcLi;
     uint32_t rcType = flagsRcVertexType(src.flags);
     if (rcType == RC_SURFACE && length(src.rcWi) > 0.5f)
         Li = Li * (evalBSDF(rcMat, rcSurf.albedo, rcSurf.norm, -wi, src.rcWi) * satDot(rcSurf.norm, src.rcWi));
@@ -771,8 +785,15 @@ RT_DEV f3 reconnectLi(const DevScene& S, const Resv& src, const Surf& rcPrevSurf
     return Li;
 }
 
-// gris_retrace.glsl:138-236
-template <bool kCount>
+// Learned target: the reconnection radiance of a surface rc vertex is replaced
+// by the network's prediction at that vertex; light rc vertices keep their
+// exact emitted radiance.
+RT_DEV f3 targetRcLi(const Resv& r) { return flagsRcVertexType(r.flags) == RC_SURFACE ? r.Lhat : r.rcLi; }
+
+// gris_retrace.glsl:138-236. With kLearned the target function at the sites
+// :204 (dstPHat) and :215 (srcPHat) is the learned one; the sample's target
+// value in its new pixel travels with it as pRef.
+template <bool kCount, bool kLearned = false>
 RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& set, Resv& dst, const Surf& dstPrimarySurf,
                           const Isec& dstPrimaryIsec, f3 primDir, f3 primOri, Resv src, uint32_t& rng, TraceCounters* ctr) {
     RcData rc;
@@ -800,9 +821,11 @@ RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& 
     }
     if (srcSampleValid) {
         uint32_t rcType = flagsRcVertexType(src.flags);
+        f3 LiL = s3(0.0f);
         if (!isnan(src.rcPrevSamplePdf) && src.rcPrevSamplePdf > 1e-6f) {
-            Li = reconnectLi(S, src, rcPrevSurf, rcSurf, wi, rc);
-            if (!isBlack(Li) && !hasNan(Li)) dstPHat = luminance(Li * jacobian);
+            Li = reconnectLi(S, src, src.rcLi, rcPrevSurf, rcSurf, wi, rc);
+            if (kLearned) LiL = reconnectLi(S, src, targetRcLi(src), rcPrevSurf, rcSurf, wi, rc);
+            if (!isBlack(Li) && !hasNan(Li)) dstPHat = luminance((kLearned ? LiL : Li) * jacobian);
             if (rcType == RC_LIGHT_SAMPLED) {
                 float sumPower = S.lightTable[0].prob;
                 dstSamplePdf = luminance(rcSurf.albedo) / sumPower / dstJacobian;
@@ -811,7 +834,8 @@ RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& 
                 dstSamplePdf = evalPdf(rcPrevMat, rcPrevSurf.norm, rc.rcPrevWo, wi);
             }
         }
-        float srcPHat = luminance(src.F);
+        float srcPHat = kLearned ? src.pRef : luminance(src.F);
+        if (kLearned) src.pRef = luminance(LiL);
         src.rcJacobian = dstJacobian;
         src.rcPrevSamplePdf = dstSamplePdf;
         src.F = Li;
@@ -824,14 +848,15 @@ RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& 
     resvCap(dst, (float)set.cap);
 }
 
+template <bool kLearned>
 RT_DEV Resv resvAt(const ResvPlanes& R, uint32_t W, uint32_t H, int x, int y) {
     unsigned long long idx = (unsigned long long)W * (uint32_t)y + (uint32_t)x;
     if (x < 0 || y < 0 || idx >= (unsigned long long)W * H) return zeroResv();
-    return loadResv(R, (size_t)idx);
+    return loadResv<kLearned>(R, (size_t)idx);
 }
 
 // gris_resample_temporal.glsl:11-83
-template <bool kCount>
+template <bool kCount, bool kLearned>
 __global__ __launch_bounds__(kTileThreads, RT_WAVES_TEMPORAL) void k_gris_temporal(DevScene S, rt_camera cam, rt_camera prevCam, GB g,
                                                                 GB gPrev, rt_gris_settings set, ResvPlanes thisR,
                                                                 ResvPlanes prevR, ResvPlanes tempR, TraceCounters* ctr) {
@@ -849,7 +874,7 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_TEMPORAL) void k_gris_tempor
     uint32_t resvRng = ~rng;
     f3 pos = ray.ori + ray.dir * (gp.depth - 1e-4f);
     size_t p = (size_t)y * g.W + x;
-    Resv resv = loadResv(thisR, p);
+    Resv resv = loadResv<kLearned>(thisR, p);
     Isec dstPrimaryIsec; dstPrimaryIsec.inst = RT_SPECIAL_HIT; dstPrimaryIsec.b1 = uv.x; dstPrimaryIsec.b2 = uv.y;
     dstPrimaryIsec.prim = 0u;
     if (set.temporalReuse) {
@@ -867,9 +892,9 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_TEMPORAL) void k_gris_tempor
                     CamRay pr = pinholeRay(prevCam, mk2(puv.x, 1.0f - puv.y));
                     f3 posPrev = pr.ori + pr.dir * (pp.depth - 1e-4f);
                     if (!(pp.matMeshId != gp.matMeshId || dot(pp.norm, gp.norm) < 0.95f || distance(pos, posPrev) > 0.5f)) {
-                        Resv tr = resvAt(prevR, cam.filmSize[0], cam.filmSize[1], px, py);
+                        Resv tr = resvAt<kLearned>(prevR, cam.filmSize[0], cam.filmSize[1], px, py);
                         if (resvIsValid(tr))
-                            reuseAndMerge<kCount>(S, st, set, resv, dstPrimarySurf, dstPrimaryIsec, ray.dir, ray.ori, tr,
+                            reuseAndMerge<kCount, kLearned>(S, st, set, resv, dstPrimarySurf, dstPrimaryIsec, ray.dir, ray.ori, tr,
                                                   resvRng, ctr);
                     }
                 }
@@ -877,11 +902,11 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_TEMPORAL) void k_gris_tempor
         }
     }
     if (!resvIsValid(resv)) resvReset(resv);
-    storeResv(tempR, p, resv);
+    storeResv<kLearned>(tempR, p, resv);
 }
 
 // gris_resample_spatial.glsl:11-135 and gris_resample_spatial.comp accumulation
-template <bool kCount>
+template <bool kCount, bool kLearned>
 __global__ __launch_bounds__(kTileThreads, RT_WAVES_SPATIAL) void k_gris_spatial(DevScene S, rt_camera cam, GB g, rt_gris_settings set,
                                                                ResvPlanes tempR, ResvPlanes thisR, float* indirectOut,
                                                                TraceCounters* ctr) {
@@ -898,7 +923,7 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_SPATIAL) void k_gris_spatial
         CamRay ray = pinholeRay(cam, mk2(uv.x, 1.0f - uv.y));
         f3 pos = ray.ori + ray.dir * (gp.depth - 1e-4f);
         int matId = gp.matMeshId >> 16;
-        Resv resv = loadResv(tempR, p);
+        Resv resv = loadResv<kLearned>(tempR, p);
         Isec dstPrimaryIsec; dstPrimaryIsec.inst = RT_SPECIAL_HIT; dstPrimaryIsec.b1 = uv.x; dstPrimaryIsec.b2 = uv.y;
         dstPrimaryIsec.prim = 0u;
         Surf dstPrimarySurf; dstPrimarySurf.pos = pos; dstPrimarySurf.norm = gp.norm; dstPrimarySurf.albedo = gp.albedo;
@@ -917,13 +942,13 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_SPATIAL) void k_gris_spatial
                 CamRay nr = pinholeRay(cam, mk2(nb.x, 1.0f - nb.y));
                 f3 posN = nr.ori + nr.dir * (np.depth - 1e-4f);
                 if (dot(np.norm, gp.norm) < 0.9f || distance(pos, posN) > 0.4f) continue;
-                Resv nres = resvAt(tempR, cam.filmSize[0], cam.filmSize[1], px, py);
+                Resv nres = resvAt<kLearned>(tempR, cam.filmSize[0], cam.filmSize[1], px, py);
                 if (resvIsValid(nres))
-                    reuseAndMerge<kCount>(S, st, set, resv, dstPrimarySurf, dstPrimaryIsec, ray.dir, ray.ori, nres, rng, ctr);
+                    reuseAndMerge<kCount, kLearned>(S, st, set, resv, dstPrimarySurf, dstPrimaryIsec, ray.dir, ray.ori, nres, rng, ctr);
             }
         }
         if (!resvIsValid(resv)) resvReset(resv);
-        storeResv(thisR, p, resv);
+        storeResv<kLearned>(thisR, p, resv);
         if (resvIsValid(resv) && resv.sampleCount > 0.0f && resv.rcIsec.inst != RT_INVALID_HIT) {
             RcData rc;
             replayHybrid<kCount>(S, st, set, dstPrimaryIsec, dstPrimarySurf, ray.dir, ray.ori, resv.flags, resv.primaryRng,
@@ -934,8 +959,10 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_SPATIAL) void k_gris_spatial
                 else loadSurfaceInfo(S, rc.rcPrevIsec, rcPrevSurf);
                 loadSurfaceInfo(S, resv.rcIsec, rcSurf);
                 f3 wi = normalize(rcSurf.pos - rcPrevSurf.pos);
-                f3 Li = reconnectLi(S, resv, rcPrevSurf, rcSurf, wi, rc);
-                if (!isBlack(Li) && !hasNan(Li)) radiance = Li / luminance(Li) * resv.resampleWeight / resv.sampleCount;
+                f3 Li = reconnectLi(S, resv, resv.rcLi, rcPrevSurf, rcSurf, wi, rc);
+                float pHat = kLearned ? luminance(reconnectLi(S, resv, targetRcLi(resv), rcPrevSurf, rcSurf, wi, rc))
+                                      : luminance(Li);
+                if (!isBlack(Li) && !hasNan(Li)) radiance = Li / pHat * resv.resampleWeight / resv.sampleCount;
             }
         }
         radiance = clampColor(radiance);
@@ -948,7 +975,82 @@ __global__ void k_resv_reset(ResvPlanes R) {
     if (i >= R.n) return;
     Resv r = zeroResv();
     r.rcIsec.inst = RT_INVALID_HIT;
-    storeResv(R, i, r);
+    storeResv<true>(R, i, r);
+}
+
+// Learned-target refresh after the path-trace pass. Each wave takes 32
+// pixels: lanes 0-31 own them, all 64 lanes encode the reconnection vertex
+// (each half of the grid levels) and the network runs on the matrix cores.
+// For a surface rc vertex the contribution with the predicted radiance,
+// F_L = F * Lhat / rcLi per channel, gives the learned target value; the
+// resample weight, which refers to luminance(F) after stream RIS, is rescaled
+// to refer to it. Every pixel with p % stride == offset also writes training
+// slot p / stride (position, reconnection radiance; target -1 if unusable).
+constexpr int kRefreshWaves = 4;
+__global__ __launch_bounds__(64 * kRefreshWaves) void k_gris_learned_refresh(DevScene S, rtm::MlpView M, GB g, ResvPlanes R,
+                                                                           float* __restrict__ batchPos,
+                                                                           float* __restrict__ batchTarget,
+                                                                           uint32_t stride, uint32_t offset) {
+    __shared__ _Float16 sW[rtm::kWeights];
+    for (int i = threadIdx.x; i < rtm::kWeights; i += blockDim.x) sW[i] = M.w[i];
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t n = g.W * g.H;
+    const uint32_t tiles = (n + 31u) / 32u;
+    for (uint32_t tile = blockIdx.x * kRefreshWaves + wave; tile < tiles; tile += gridDim.x * kRefreshWaves) {
+        const uint32_t p = tile * 32u + (lane & 31u);
+        const bool own = lane < 32u && p < n;
+        bool written = false, need = false;
+        Resv r = zeroResv();
+        f3 y = s3(0.0f);
+        if (own) {
+            written = g.dn[p].x != 0.0f;
+            if (written) {
+                r = loadResv(R, p);
+                need = resvIsValid(r) && r.rcIsec.inst != RT_INVALID_HIT && flagsRcVertexType(r.flags) == RC_SURFACE;
+                if (need) {
+                    Surf sf;
+                    loadSurfaceInfo(S, r.rcIsec, sf);
+                    y = sf.pos;
+                }
+            }
+        }
+        float u, v, w;
+        rtm::normalizePos(M, y.x, y.y, y.z, u, v, w);
+        u = __shfl(u, (int)(lane & 31u));
+        v = __shfl(v, (int)(lane & 31u));
+        w = __shfl(w, (int)(lane & 31u));
+        rtm::h4 enc[4];
+        rtm::encodeLane(M, lane >> 5, u, v, w, enc);
+        rtm::Fwd F;
+        rtm::forwardTile(sW, lane, enc, F);
+        if (!own) continue;
+        if (written) {
+            f3 L = s3(0.0f);
+            const float pOld = luminance(r.F);
+            f3 FL = r.F;
+            if (need) {
+                L = mk3(rtm::outAct(F.z3[0]), rtm::outAct(F.z3[1]), rtm::outAct(F.z3[2]));
+                FL.x = r.rcLi.x > 0.0f ? r.F.x * (L.x / r.rcLi.x) : r.F.x;
+                FL.y = r.rcLi.y > 0.0f ? r.F.y * (L.y / r.rcLi.y) : r.F.y;
+                FL.z = r.rcLi.z > 0.0f ? r.F.z * (L.z / r.rcLi.z) : r.F.z;
+            }
+            const float pNew = luminance(FL);
+            float rw = r.resampleWeight;
+            if (need && pOld > 0.0f) rw = rw * (pNew / pOld);
+            R.p[4 * R.n + p].w = rw;
+            R.p[5 * R.n + p] = make_float4(L.x, L.y, L.z, need ? pNew : pOld);
+        }
+        if (p % stride == offset) {
+            const uint32_t slot = p / stride;
+            batchPos[3 * (size_t)slot] = y.x;
+            batchPos[3 * (size_t)slot + 1] = y.y;
+            batchPos[3 * (size_t)slot + 2] = y.z;
+            batchTarget[3 * (size_t)slot] = need ? r.rcLi.x : -1.0f;
+            batchTarget[3 * (size_t)slot + 1] = need ? r.rcLi.y : 0.0f;
+            batchTarget[3 * (size_t)slot + 2] = need ? r.rcLi.z : 0.0f;
+        }
+    }
 }
 
 inline dim3 tileGrid(uint32_t W, uint32_t H) { return dim3((W + kTileX - 1) / kTileX, (H + kTileY - 1) / kTileY, 1); }
@@ -971,6 +1073,18 @@ namespace rti {
     do {                                                                                             \
         if (gCountersEnabled) KERNEL<true><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__, counterBuffer()); \
         else KERNEL<false><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__, nullptr);                        \
+    } while (0)
+
+// Same, for kernels that also take the target function as a template argument.
+#define RT_DISPATCH_L(KERNEL, LEARNED, GRID, BLOCK, STREAM, ...)                                                  \
+    do {                                                                                                            \
+        if (LEARNED) {                                                                                              \
+            if (gCountersEnabled) KERNEL<true, true><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__, counterBuffer());     \
+            else KERNEL<false, true><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__, nullptr);                             \
+        } else {                                                                                                    \
+            if (gCountersEnabled) KERNEL<true, false><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__, counterBuffer());    \
+            else KERNEL<false, false><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__, nullptr);                            \
+        }                                                                                                           \
     } while (0)
 
 hipError_t launchTraceRays(const rt_device_scene* s, const float* rays8, uint32_t n, int mode, int skipLights, float* t,
@@ -1027,18 +1141,39 @@ hipError_t launchGrisTemporal(const rt_device_scene* s, const rt_camera* cams, c
                               const rt_gris_settings& set, hipStream_t stream) {
     GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
     GB gp = makeGB(f.depthNormalPrev, f.albedoMatIdPrev, nullptr, f.width, f.height);
-    RT_DISPATCH(k_gris_temporal, tileGrid(f.width, f.height), kTileThreads, stream, s->dev, cams[0], cams[1], g, gp, set,
-                planes(f.grisReservoir, f.width, f.height), planes(f.grisReservoirPrev, f.width, f.height),
-                planes(f.grisReservoirTemp, f.width, f.height));
+    RT_DISPATCH_L(k_gris_temporal, set.targetFunction == 1u, tileGrid(f.width, f.height), kTileThreads, stream, s->dev,
+                  cams[0], cams[1], g, gp, set, planes(f.grisReservoir, f.width, f.height),
+                  planes(f.grisReservoirPrev, f.width, f.height), planes(f.grisReservoirTemp, f.width, f.height));
     return hipGetLastError();
 }
 
 hipError_t launchGrisSpatial(const rt_device_scene* s, const rt_camera* cams, const rt_frame& f,
                              const rt_gris_settings& set, hipStream_t stream) {
     GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
-    RT_DISPATCH(k_gris_spatial, tileGrid(f.width, f.height), kTileThreads, stream, s->dev, cams[0], g, set,
-                planes(f.grisReservoirTemp, f.width, f.height), planes(f.grisReservoir, f.width, f.height),
-                f.indirectOutput);
+    RT_DISPATCH_L(k_gris_spatial, set.targetFunction == 1u, tileGrid(f.width, f.height), kTileThreads, stream, s->dev,
+                  cams[0], g, set, planes(f.grisReservoirTemp, f.width, f.height),
+                  planes(f.grisReservoir, f.width, f.height), f.indirectOutput);
+    return hipGetLastError();
+}
+
+hipError_t launchLearnedRefresh(const rt_device_scene* s, rt_mlp* m, const rt_camera* cams, const rt_frame& f,
+                                hipStream_t stream) {
+    constexpr uint32_t kBatchCap = 1u << 16;
+    const uint32_t n = f.width * f.height;
+    if (!m->batchPos) {
+        hipError_t e = hipMalloc(&m->batchPos, sizeof(float) * 3 * kBatchCap);
+        if (e == hipSuccess) e = hipMalloc(&m->batchTarget, sizeof(float) * 3 * kBatchCap);
+        if (e != hipSuccess) return e;
+    }
+    const uint32_t stride = std::max(1u, (n + kBatchCap - 1) / kBatchCap);
+    const uint32_t offset = (cams[0].frameIndex & 0x7fffffffu) % stride;
+    m->batchSize = offset < n ? (n - offset + stride - 1) / stride : 0u;
+    GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
+    const uint32_t tiles = (n + 31u) / 32u;
+    const uint32_t blocks = std::min<uint32_t>((tiles + kRefreshWaves - 1) / kRefreshWaves, 4096u);
+    k_gris_learned_refresh<<<blocks, 64 * kRefreshWaves, 0, stream>>>(s->dev, m->view, g,
+                                                                      planes(f.grisReservoir, f.width, f.height),
+                                                                      m->batchPos, m->batchTarget, stride, offset);
     return hipGetLastError();
 }
 

@@ -86,7 +86,15 @@ class SceneArrays(C.Structure):
 class GrisSettings(C.Structure):
     _fields_ = [("shiftType", C.c_uint32), ("rrScale", C.c_float), ("temporalReuse", C.c_uint32),
                 ("spatialReuse", C.c_uint32), ("cap", C.c_uint32), ("maxDepth", C.c_uint32),
-                ("spatialNeighbors", C.c_uint32), ("spatialRadius", C.c_float)]
+                ("spatialNeighbors", C.c_uint32), ("spatialRadius", C.c_float), ("targetFunction", C.c_uint32)]
+
+
+class MlpConfig(C.Structure):
+    _fields_ = [("levels", C.c_uint32), ("featuresPerLevel", C.c_uint32), ("log2TableSize", C.c_uint32),
+                ("baseResolution", C.c_uint32), ("maxResolution", C.c_uint32), ("hiddenWidth", C.c_uint32),
+                ("hiddenLayers", C.c_uint32), ("seed", C.c_uint32), ("learningRate", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float), ("boundsLo", C.c_float * 3),
+                ("boundsHi", C.c_float * 3)]
 
 
 class Frame(C.Structure):
@@ -106,9 +114,16 @@ assert C.sizeof(Material) == 32 and C.sizeof(MeshVertex) == 32 and C.sizeof(Obje
 assert C.sizeof(TriangleLight) == 64 and C.sizeof(Camera) == 352 and C.sizeof(GrisReservoir) == 96
 
 
-def settings(shift=2, rr_scale=1.0, temporal=False, spatial=True, cap=20, max_depth=15, neighbors=3, radius=20.0):
+def settings(shift=2, rr_scale=1.0, temporal=False, spatial=True, cap=20, max_depth=15, neighbors=3, radius=20.0,
+             learned=False):
     """GRISReSTIR::Settings (src/GRISReSTIR.h:29 defaults) + kernel constants."""
-    return GrisSettings(shift, rr_scale, int(temporal), int(spatial), cap, max_depth, neighbors, radius)
+    return GrisSettings(shift, rr_scale, int(temporal), int(spatial), cap, max_depth, neighbors, radius, int(learned))
+
+
+def mlp_config(lo, hi, log2_table=19, base_res=16, max_res=2048, seed=3, lr=1e-2):
+    """Hash grid 16 x 2, 2^log2_table entries per level; MLP 2 x 64; Adam (0.9, 0.99, 1e-15)."""
+    return MlpConfig(16, 2, log2_table, base_res, max_res, 64, 2, seed, lr, 0.9, 0.99, 1e-15,
+                     (C.c_float * 3)(*lo), (C.c_float * 3)(*hi))
 
 
 # ---------------------------------------------------------------------------------------
@@ -118,7 +133,10 @@ CORE_SYMBOLS = [
     "rt_last_error", "rt_device_scene_create", "rt_device_scene_destroy", "rt_device_scene_info", "rt_bvh_export", "rt_bvh4_export",
     "rt_trace_rays", "rt_gbuffer_trace", "rt_pt_plain", "rt_gris_path_trace", "rt_gris_temporal", "rt_gris_spatial",
     "rt_gris_render", "rt_gris_reservoir_bytes", "rt_gris_reservoir_reset", "rt_gris_reservoir_download",
-    "rt_gris_reservoir_upload", "rt_counters_enable", "rt_counters_read",
+    "rt_gris_reservoir_upload", "rt_gris_learned_download", "rt_gris_learned_upload", "rt_counters_enable",
+    "rt_counters_read", "rt_mlp_create", "rt_mlp_destroy", "rt_mlp_param_counts", "rt_mlp_get_params",
+    "rt_mlp_set_params", "rt_mlp_infer", "rt_mlp_gradients", "rt_mlp_read_gradients", "rt_mlp_adam_step",
+    "rt_mlp_train_step", "rt_mlp_train_frame", "rt_mlp_batch_download", "rt_gris_learned_refresh",
 ]
 HOST_SYMBOLS = [
     "rt_scenegen", "rt_host_scene_destroy", "rt_host_scene_arrays", "rt_host_scene_camera", "rt_light_table_build",
@@ -149,8 +167,23 @@ def _declare_core(L):
         "rt_gris_reservoir_reset": [P, u32, u32, P],
         "rt_gris_reservoir_download": [P, u32, u32, P, P],
         "rt_gris_reservoir_upload": [P, u32, u32, P, P],
+        "rt_gris_learned_download": [P, u32, u32, P, P],
+        "rt_gris_learned_upload": [P, u32, u32, P, P],
         "rt_counters_enable": [C.c_int],
         "rt_counters_read": [P, C.c_int],
+        "rt_mlp_create": [C.POINTER(MlpConfig), P, C.POINTER(P)],
+        "rt_mlp_destroy": [P],
+        "rt_mlp_param_counts": [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
+        "rt_mlp_get_params": [P, P, P],
+        "rt_mlp_set_params": [P, P, P],
+        "rt_mlp_infer": [P, P, u32, P, P],
+        "rt_mlp_gradients": [P, P, P, u32, P, P],
+        "rt_mlp_read_gradients": [P, P, P],
+        "rt_mlp_adam_step": [P, P],
+        "rt_mlp_train_step": [P, P, P, u32, P, P],
+        "rt_mlp_train_frame": [P, P, P],
+        "rt_mlp_batch_download": [P, P, P, C.POINTER(C.c_uint32)],
+        "rt_gris_learned_refresh": [P, P, C.POINTER(Camera), C.POINTER(Frame), P],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -280,6 +313,50 @@ class DeviceScene:
     def __del__(self):
         if getattr(self, "handle", None) and _core is not None:
             _core.rt_device_scene_destroy(self.handle)
+            self.handle = None
+
+
+class Mlp:
+    """rt_mlp: the learned target function (hash grid + MLP) on the current device."""
+
+    def __init__(self, config: MlpConfig, stream=None):
+        L = core()
+        h = C.c_void_p()
+        check(L.rt_mlp_create(C.byref(config), stream, C.byref(h)), "rt_mlp_create")
+        self.handle = h
+        self.config = config
+        g, w = C.c_uint64(), C.c_uint64()
+        check(L.rt_mlp_param_counts(h, C.byref(g), C.byref(w)), "rt_mlp_param_counts")
+        self.grid_params, self.weight_params = g.value, w.value
+
+    def params(self):
+        grid = np.zeros(self.grid_params, np.float32)
+        w = np.zeros(self.weight_params, np.float32)
+        check(core().rt_mlp_get_params(self.handle, grid.ctypes.data, w.ctypes.data), "rt_mlp_get_params")
+        return grid, w
+
+    def set_params(self, grid=None, weights=None):
+        g = None if grid is None else np.ascontiguousarray(grid, np.float32)
+        w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+        check(core().rt_mlp_set_params(self.handle, ptr(g), ptr(w)), "rt_mlp_set_params")
+
+    def read_gradients(self):
+        grid = np.zeros(self.grid_params, np.float32)
+        w = np.zeros(self.weight_params, np.float32)
+        check(core().rt_mlp_read_gradients(self.handle, grid.ctypes.data, w.ctypes.data), "rt_mlp_read_gradients")
+        return grid, w
+
+    def batch(self):
+        """Training batch gathered by the last rt_gris_learned_refresh: (pos [n,3], target [n,3])."""
+        pos = np.zeros((1 << 16, 3), np.float32)
+        tgt = np.zeros((1 << 16, 3), np.float32)
+        n = C.c_uint32()
+        check(core().rt_mlp_batch_download(self.handle, pos.ctypes.data, tgt.ctypes.data, C.byref(n)), "batch")
+        return pos[: n.value], tgt[: n.value]
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _core is not None:
+            _core.rt_mlp_destroy(self.handle)
             self.handle = None
 
 
