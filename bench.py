@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--only-kernel-loop", action="store_true", help="time frames only (for profilers)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1-GiB device copy (calibrates HBM byte counters)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the equal-time relMSE comparison")
+    ap.add_argument("--lvp-frames", type=int, default=64, help="frames of the plain form in the equal-time run")
+    ap.add_argument("--ref-frames", type=int, default=4096, help="frames of the converged reference image")
     return ap.parse_args()
 
 
@@ -184,6 +187,139 @@ def cpu_baseline(rt, hs, args):
                       f"{args.cpu_frames - 1}), {dt:.1f} s"}
 
 
+class TorchFrames:
+    """Frame images and reservoirs of one view as torch tensors, with the passes
+    launched through the C ABI (so that outputs can be accumulated in FP64)."""
+
+    def __init__(self, rt, ds, W, H):
+        import torch
+
+        self.rt, self.ds, self.W, self.H, self.torch = rt, ds, W, H, torch
+        n = W * H
+
+        def z(nbytes):
+            return torch.zeros(nbytes, dtype=torch.uint8, device="cuda")
+
+        self.dn, self.am, self.mv = [z(16 * n), z(16 * n)], [z(8 * n), z(8 * n)], z(4 * n)
+        rb = rt.core().rt_gris_reservoir_bytes(W, H)
+        self.resv, self.temp = [z(rb), z(rb)], z(rb)
+        self.direct = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+        self.out = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+        self.stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        self.cur = 0
+        self.reset()
+
+    def reset(self):
+        for b in self.resv + [self.temp]:
+            self.rt.check(self.rt.core().rt_gris_reservoir_reset(b.data_ptr(), self.W, self.H, self.stream), "reset")
+        self.cur = 0
+
+    def frame_struct(self):
+        f = self.rt.Frame()
+        c, l = self.cur, self.cur ^ 1
+        f.width, f.height = self.W, self.H
+        f.directOutput, f.indirectOutput = self.direct.data_ptr(), self.out.data_ptr()
+        f.depthNormal, f.depthNormalPrev = self.dn[c].data_ptr(), self.dn[l].data_ptr()
+        f.albedoMatId, f.albedoMatIdPrev = self.am[c].data_ptr(), self.am[l].data_ptr()
+        f.motionVector = self.mv.data_ptr()
+        f.grisReservoir, f.grisReservoirPrev = self.resv[c].data_ptr(), self.resv[l].data_ptr()
+        f.grisReservoirTemp = self.temp.data_ptr()
+        return f
+
+    def frame(self, cams, s, mlp=None, events=None):
+        """G-buffer, GRIS path trace, [learned refresh], temporal, spatial, [one training step]."""
+        rt, L, st = self.rt, self.rt.core(), self.stream
+        f = self.frame_struct()
+        if events: events[0].record()
+        rt.check(L.rt_gbuffer_trace(self.ds.handle, C.byref(cams[0]), f.depthNormal, f.albedoMatId, f.motionVector,
+                                    st), "gbuffer")
+        rt.check(L.rt_gris_path_trace(self.ds.handle, cams, C.byref(f), C.byref(s), st), "path trace")
+        if mlp is not None:
+            rt.check(L.rt_gris_learned_refresh(self.ds.handle, mlp.handle, cams, C.byref(f), st), "refresh")
+        rt.check(L.rt_gris_temporal(self.ds.handle, cams, C.byref(f), C.byref(s), st), "temporal")
+        rt.check(L.rt_gris_spatial(self.ds.handle, cams, C.byref(f), C.byref(s), st), "spatial")
+        if mlp is not None:
+            rt.check(L.rt_mlp_train_frame(mlp.handle, None, st), "train")
+        if events: events[1].record()
+        self.cur ^= 1
+
+
+def static_cams(rt, base, i, seed):
+    """Camera pair for frame i of a still view: the first frame's camera with a fresh
+    per-frame seed, frameIndex 0 (each frame's output is that frame's estimate; the
+    clear flag on frame 0), previous camera = itself."""
+    c = (rt.Camera * 2)()
+    C.memmove(C.addressof(c), C.addressof(base), C.sizeof(rt.Camera))
+    h = (seed * 0x9E3779B1 + i * 0x85EBCA77 + 0x165667B1) & 0xFFFFFFFF
+    h ^= h >> 15
+    h = (h * 0x2C1B3C6D) & 0xFFFFFFFF
+    c[0].seed = h ^ (h >> 12)
+    c[0].frameIndex = 0x80000000 if i == 0 else 0
+    for k in range(16):
+        c[0].lastProjView[k] = c[0].projView[k]
+    C.memmove(C.addressof(c) + C.sizeof(rt.Camera), C.addressof(c), C.sizeof(rt.Camera))
+    return c
+
+
+def learned_vs_plain(rt, hs, ds, args):
+    """Equal-time relMSE of the learned-target form against the plain (luminance) form
+    on a still 1080p view, both judged against a converged reference: the same GRIS
+    estimator without reuse (unbiased per frame), args.ref_frames frames, FP64 mean.
+    relMSE = mean over pixels of sum_rgb (I - R)^2 / (R^2 + 1e-2)."""
+    import torch
+
+    W, H = args.width, args.height
+    base = hs.camera(0, W, H)
+    fr = TorchFrames(rt, ds, W, H)
+    radius = 30.0 * W / 1920.0
+    s_plain = settings(rt, W)
+    s_ref = rt.settings(shift=2, temporal=False, spatial=False, cap=20, max_depth=6, neighbors=3, radius=radius)
+    s_learn = rt.settings(shift=2, temporal=True, spatial=True, cap=20, max_depth=6, neighbors=3, radius=radius,
+                          learned=True)
+    info = ds.info()
+    lo, hi = [float(v) for v in info.sceneLo], [float(v) for v in info.sceneHi]
+    ext = [max(h - l, 1e-3) for l, h in zip(lo, hi)]
+    bounds = ([l - 0.1 * e for l, e in zip(lo, ext)], [h + 0.1 * e for h, e in zip(hi, ext)])
+
+    def new_mlp():
+        return rt.Mlp(rt.mlp_config(bounds[0], bounds[1], log2_table=19, seed=3))
+
+    def run(s, frames, seed, mlp=None):
+        fr.reset()
+        acc = torch.zeros((H, W, 3), dtype=torch.float64, device="cuda")
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(frames)]
+        for i in range(frames):
+            fr.frame(static_cams(rt, base, i, seed), s, mlp, ev[i])
+            acc += fr.out[..., :3].double()
+        torch.cuda.synchronize()
+        ms = sum(e[0].elapsed_time(e[1]) for e in ev) / frames
+        return acc / frames, ms
+
+    t0 = time.perf_counter()
+    ref, ref_ms = run(s_ref, args.ref_frames, 7)
+    ref_s = time.perf_counter() - t0
+    _, plain_ms = run(s_plain, 16, 11)
+    _, learn_ms = run(s_learn, 16, 13, new_mlp())
+    n_plain = args.lvp_frames
+    budget = n_plain * plain_ms
+    n_learn = max(1, int(budget // learn_ms))
+    img_p, plain_ms = run(s_plain, n_plain, 17)
+    img_l, learn_ms = run(s_learn, n_learn, 17, new_mlp())
+
+    def relmse(img):
+        return float((((img - ref) ** 2) / (ref ** 2 + 1e-2)).sum(-1).mean())
+
+    rp, rl = relmse(img_p), relmse(img_l)
+    return {"relmse_plain": round(rp, 6), "relmse_learned": round(rl, 6),
+            "plain_over_learned": round(rp / rl, 4) if rl > 0 else None,
+            "frames": {"plain": n_plain, "learned": n_learn}, "ms_per_frame": {"plain": round(plain_ms, 3),
+                                                                                "learned": round(learn_ms, 3)},
+            "equal_time_ms": round(budget, 1),
+            "reference": f"GRIS without reuse, {args.ref_frames} frames ({ref_ms:.2f} ms each, {ref_s:.1f} s), FP64 mean",
+            "view": f"terrain camera frame 0 held still, {W}x{H}, per-frame seeds; learned: hash grid 16x2x2^19 FP16 "
+                    f"+ MLP 2x64 FP16 (MFMA), trained online each frame on 2^16 reconnection samples, Adam lr 1e-2"}
+
+
 def read_traffic(kernel):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_1080p_terrain_traffic.md")
@@ -260,6 +396,8 @@ def main():
                             "launch_ms": round(pass_ms[dom], 3)}
         line["work_per_frame"] = {p: {"rays": int(work[p][2]) // args.steps, "nodes": int(work[p][0]) // args.steps,
                                       "tris": int(work[p][1]) // args.steps} for p in PASSES}
+    if not args.no_learned:
+        line["learned_vs_plain"] = learned_vs_plain(rt, hs, ds, args)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(rt, hs, args)
     print(json.dumps(line))

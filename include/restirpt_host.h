@@ -56,6 +56,15 @@ rt_status rt_renderer_bindings(const rt_renderer* r, rt_frame* out);
 rt_status rt_renderer_flip(rt_renderer* r);
 /* Copies an RGBA32F output (0 direct, 1 indirect) to host memory. */
 rt_status rt_renderer_read_output(const rt_renderer* r, uint32_t which, float* host, hipStream_t stream);
+/* Resets the three reservoir buffers (the GUI's clearReservoir, src/Renderer.cpp:567). */
+rt_status rt_renderer_reset_reservoirs(rt_renderer* r, hipStream_t stream);
+/* Creates the renderer's learned target function (hash grid 16 x 2 x
+ * 2^log2TableSize, MLP 2 x 64, bounds = scene box + 10%) and binds it to the
+ * GRIS passes; frames with settings.targetFunction = 1 then run the learned
+ * refresh after path tracing and one training step after spatial reuse.
+ * *mlp (may be null) receives the network, owned by the renderer. */
+rt_status rt_renderer_enable_learned(rt_renderer* r, uint32_t seed, uint32_t log2TableSize, hipStream_t stream,
+                                     rt_mlp** mlp);
 
 #ifdef __cplusplus
 }

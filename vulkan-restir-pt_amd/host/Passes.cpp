@@ -50,9 +50,37 @@ void GRISReSTIR::createPipeline() {
 }
 // Passes are stream ordered, which stands in for the reference's pipeline barriers.
 void GRISReSTIR::render(hipStream_t stream, const DeviceScene& scene, const rt_camera cams[2], const rt_frame& frame) {
+    const bool learnedTarget = settings.targetFunction == 1u;
+    if (learnedTarget && !learned) throw std::runtime_error("GRISReSTIR: the learned target needs a network");
     mPathTracePass.execute(stream, scene, cams, frame, &settings);
+    if (learnedTarget)
+        rtThrow(rt_gris_learned_refresh(scene.handle(), learned, cams, &frame, stream), "rt_gris_learned_refresh");
     mTemporalReusePass.execute(stream, scene, cams, frame, &settings);
     mSpatialReusePass.execute(stream, scene, cams, frame, &settings);
+    if (learnedTarget) rtThrow(rt_mlp_train_frame(learned, nullptr, stream), "rt_mlp_train_frame");
+}
+
+LearnedTarget::LearnedTarget(const DeviceScene& scene, uint32_t seed, hipStream_t stream, uint32_t log2TableSize) {
+    rt_scene_info info{};
+    rtThrow(rt_device_scene_info(scene.handle(), &info), "rt_device_scene_info");
+    rt_mlp_config c{};
+    c.levels = 16; c.featuresPerLevel = 2; c.log2TableSize = log2TableSize; c.baseResolution = 16;
+    c.maxResolution = 2048; c.hiddenWidth = 64; c.hiddenLayers = 2; c.seed = seed;
+    c.learningRate = 1e-2f; c.beta1 = 0.9f; c.beta2 = 0.99f; c.epsilon = 1e-15f;
+    for (int a = 0; a < 3; a++) {
+        float ext = std::max(info.sceneHi[a] - info.sceneLo[a], 1e-3f);
+        c.boundsLo[a] = info.sceneLo[a] - 0.1f * ext;
+        c.boundsHi[a] = info.sceneHi[a] + 0.1f * ext;
+    }
+    rtThrow(rt_mlp_create(&c, stream, &mlp), "rt_mlp_create");
+}
+LearnedTarget::~LearnedTarget() {
+    if (mlp) rt_mlp_destroy(mlp);
+}
+
+void Renderer::resetReservoirs(hipStream_t stream) {
+    for (int j = 0; j < 2; j++) rtThrow(rt_gris_reservoir_reset(grisReservoir[j], width, height, stream), "reset");
+    rtThrow(rt_gris_reservoir_reset(grisReservoirTemp, width, height, stream), "reset");
 }
 
 // ---- Renderer: src/Renderer.cpp:193-257 (createRayImage), 319-353 (bindings), 400-569 ----

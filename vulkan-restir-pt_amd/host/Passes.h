@@ -49,14 +49,28 @@ public:
     using Settings = rt_gris_settings;
 
     void createPipeline();
+    // With settings.targetFunction = 1 the frame is: path trace, learned refresh,
+    // temporal reuse, spatial reuse, one training step of the network.
     void render(hipStream_t stream, const DeviceScene& scene, const rt_camera cams[2], const rt_frame& frame);
 
     // src/GRISReSTIR.h:29 {Hybrid, 1, false, true, 20}; then the kernel constants
-    // MaxTracingDepth = 15, ResampleNum = 3, ResampleRadius = 20
-    Settings settings = {Hybrid, 1.0f, 0u, 1u, 20u, 15u, 3u, 20.0f};
+    // MaxTracingDepth = 15, ResampleNum = 3, ResampleRadius = 20; luminance target
+    Settings settings = {Hybrid, 1.0f, 0u, 1u, 20u, 15u, 3u, 20.0f, 0u};
+    rt_mlp* learned = nullptr;  // borrowed; required when settings.targetFunction = 1
 
 private:
     RayTracing mPathTracePass, mTemporalReusePass, mSpatialReusePass;
+};
+
+// Owns the learned target function of a scene: network bounds are the scene
+// box padded by 10%.
+class LearnedTarget {
+public:
+    LearnedTarget(const DeviceScene& scene, uint32_t seed, hipStream_t stream, uint32_t log2TableSize = 19);
+    ~LearnedTarget();
+    LearnedTarget(const LearnedTarget&) = delete;
+    LearnedTarget& operator=(const LearnedTarget&) = delete;
+    rt_mlp* mlp = nullptr;
 };
 
 class Renderer {
@@ -72,6 +86,8 @@ public:
     void drawFrame(hipStream_t stream, const rt_camera cams[2], Method method);
     rt_frame bindings() const;
     void flip() { mCurFrame ^= 1u; }
+    // Resets the three reservoir buffers (the GUI's clearReservoir, src/Renderer.cpp:567).
+    void resetReservoirs(hipStream_t stream);
 
     GRISReSTIR gris;
     uint32_t maxDepthNaive = 15;

@@ -5,7 +5,11 @@
 #include <exception>
 
 struct rt_host_scene { std::unique_ptr<Scene> scene; };
-struct rt_renderer { std::unique_ptr<DeviceScene> scene; std::unique_ptr<Renderer> r; };
+struct rt_renderer {
+    std::unique_ptr<DeviceScene> scene;
+    std::unique_ptr<Renderer> r;
+    std::unique_ptr<LearnedTarget> learned;
+};
 
 namespace {
 thread_local std::string hostError;
@@ -94,6 +98,7 @@ rt_status rt_renderer_create(rt_device_scene* scene, uint32_t width, uint32_t he
 rt_status rt_renderer_destroy(rt_renderer* r) {
     if (!r) return RT_ERR_INVALID_ARGUMENT;
     r->r.reset();
+    r->learned.reset();
     delete r;
     return RT_OK;
 }
@@ -134,6 +139,30 @@ rt_status rt_renderer_read_output(const rt_renderer* r, uint32_t which, float* h
     if (hipMemcpyAsync(host, src, bytes, hipMemcpyDeviceToHost, stream) != hipSuccess) return RT_ERR_HIP;
     if (hipStreamSynchronize(stream) != hipSuccess) return RT_ERR_HIP;
     return RT_OK;
+}
+
+rt_status rt_renderer_reset_reservoirs(rt_renderer* r, hipStream_t stream) {
+    if (!r) return RT_ERR_INVALID_ARGUMENT;
+    try {
+        r->r->resetReservoirs(stream);
+        return RT_OK;
+    } catch (const std::exception& e) {
+        return fail(RT_ERR_HIP, e.what());
+    }
+}
+
+rt_status rt_renderer_enable_learned(rt_renderer* r, uint32_t seed, uint32_t log2TableSize, hipStream_t stream,
+                                     rt_mlp** mlp) {
+    if (!r || log2TableSize < 10 || log2TableSize > 24) return RT_ERR_INVALID_ARGUMENT;
+    try {
+        r->r->gris.learned = nullptr;
+        r->learned = std::make_unique<LearnedTarget>(*r->scene, seed, stream, log2TableSize);
+        r->r->gris.learned = r->learned->mlp;
+        if (mlp) *mlp = r->learned->mlp;
+        return RT_OK;
+    } catch (const std::exception& e) {
+        return fail(RT_ERR_HIP, e.what());
+    }
 }
 
 }  // extern "C"

@@ -141,7 +141,8 @@ CORE_SYMBOLS = [
 HOST_SYMBOLS = [
     "rt_scenegen", "rt_host_scene_destroy", "rt_host_scene_arrays", "rt_host_scene_camera", "rt_light_table_build",
     "rt_renderer_create", "rt_renderer_destroy", "rt_renderer_set_settings", "rt_renderer_frame",
-    "rt_renderer_bindings", "rt_renderer_flip", "rt_renderer_read_output",
+    "rt_renderer_bindings", "rt_renderer_flip", "rt_renderer_read_output", "rt_renderer_reset_reservoirs",
+    "rt_renderer_enable_learned",
 ]
 
 _core = None
@@ -208,6 +209,8 @@ def _declare_host(L):
         "rt_renderer_bindings": [P, C.POINTER(Frame)],
         "rt_renderer_flip": [P],
         "rt_renderer_read_output": [P, u32, P, P],
+        "rt_renderer_reset_reservoirs": [P, P],
+        "rt_renderer_enable_learned": [P, u32, u32, P, C.POINTER(P)],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
