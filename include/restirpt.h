@@ -17,7 +17,7 @@
  *   outputs      RGBA32F  (the reference's uDirectOutput / uIndirectOutput)
  * Images are row-major, pixel (x, y) at index y * width + x, row 0 at the top.
  * GRIS reservoir buffers are opaque device buffers of
- * rt_gris_reservoir_bytes(width, height) bytes (planar, 80 B per pixel).
+ * rt_gris_reservoir_bytes(width, height) bytes (planar, 96 B per pixel).
  */
 #ifndef RESTIRPT_H
 #define RESTIRPT_H
@@ -287,6 +287,20 @@ rt_status rt_mlp_batch_download(const rt_mlp* mlp, float* pos, float* target, ui
  * fixed stride of pixels. */
 rt_status rt_gris_learned_refresh(const rt_device_scene* scene, rt_mlp* mlp, const rt_camera cams[2],
                                   const rt_frame* frame, hipStream_t stream);
+
+/* ---- Frame graphs -------------------------------------------------------
+ * The per-frame command recording of Renderer::recordRenderCommand
+ * (src/Renderer.cpp:400-503, one command buffer per frame) as HIP graphs: one
+ * GRIS frame of a view (G-buffer, path trace, [learned refresh], temporal,
+ * spatial, [one training step]) per frame parity, built on the first launch
+ * of that parity; later launches update the kernel arguments (cameras, Adam
+ * step) and replay. frames[j] are the bindings of parity j (same size); the
+ * graph owns its pixel queue. mlp is required when settings->targetFunction = 1. */
+typedef struct rt_frame_graph rt_frame_graph;
+rt_status rt_frame_graph_create(const rt_device_scene* scene, const rt_frame frames[2],
+                                const rt_gris_settings* settings, rt_mlp* mlp, rt_frame_graph** out);
+rt_status rt_frame_graph_launch(rt_frame_graph* graph, uint32_t parity, const rt_camera cams[2], hipStream_t stream);
+rt_status rt_frame_graph_destroy(rt_frame_graph* graph);
 
 /* Traversal work counters (internal nodes visited, triangles tested, rays)
  * accumulated by every kernel launched while enabled; off by default. The

@@ -8,6 +8,7 @@
 namespace rti {
 
 thread_local std::string tlsError;
+thread_local std::vector<KernelLaunch>* tlsRecorder = nullptr;
 bool gCountersEnabled = false;
 static rtd::TraceCounters* gCounters = nullptr;
 
@@ -24,7 +25,7 @@ hipError_t launchTraceRays(const rt_device_scene*, const float*, uint32_t, int, 
 hipError_t launchGBuffer(const rt_device_scene*, const rt_camera&, float*, uint32_t*, uint16_t*, hipStream_t);
 hipError_t launchPtPlain(const rt_device_scene*, const rt_camera*, const rt_frame&, uint32_t, uint32_t, hipStream_t);
 hipError_t launchGrisPathTrace(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_gris_settings&,
-                               hipStream_t);
+                               hipStream_t, rtd::WorkQueue* queue = nullptr);
 hipError_t launchGrisTemporal(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_gris_settings&,
                               hipStream_t);
 hipError_t launchGrisSpatial(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_gris_settings&,
@@ -278,6 +279,9 @@ rt_status rt_gris_spatial(const rt_device_scene* s, const rt_camera cams[2], con
 
 rt_status rt_gris_render(const rt_device_scene* s, const rt_camera cams[2], const rt_frame* f,
                          const rt_gris_settings* set, hipStream_t stream) {
+    if (set && set->targetFunction == 1u)
+        return invalid("rt_gris_render runs the luminance target only; the learned target needs "
+                       "rt_gris_learned_refresh between the path-trace and temporal passes");
     rt_status st = rt_gris_path_trace(s, cams, f, set, stream);
     if (st == RT_OK) st = rt_gris_temporal(s, cams, f, set, stream);
     if (st == RT_OK) st = rt_gris_spatial(s, cams, f, set, stream);

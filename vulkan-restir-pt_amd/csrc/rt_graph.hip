@@ -1,0 +1,168 @@
+// Frame graphs: one GRIS frame of a view as a HIP graph per frame parity.
+//
+// The frame's launches are recorded through rti::launch (same launchers as
+// the eager entry points, so the kernels and arguments are identical), turned
+// into a chain of kernel nodes on the first launch of a parity, and on later
+// launches re-recorded with the new cameras and written into the instantiated
+// graph with hipGraphExecKernelNodeSetParams before it is replayed. The graph
+// holds no memset or copy nodes: counters that a frame needs cleared are
+// cleared by kernels (the pixel queue re-arms itself, the gradient tables are
+// zeroed by fill kernels).
+#include "rt_internal.h"
+
+namespace rti {
+hipError_t launchGBuffer(const rt_device_scene*, const rt_camera&, float*, uint32_t*, uint16_t*, hipStream_t);
+hipError_t launchGrisPathTrace(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_gris_settings&,
+                               hipStream_t, rtd::WorkQueue* queue);
+hipError_t launchGrisTemporal(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_gris_settings&,
+                              hipStream_t);
+hipError_t launchGrisSpatial(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_gris_settings&,
+                             hipStream_t);
+hipError_t launchLearnedRefresh(const rt_device_scene*, rt_mlp*, const rt_camera*, const rt_frame&, hipStream_t);
+hipError_t mlpGradients(rt_mlp* m, const float* pos, const float* target, uint32_t n, float* loss, hipStream_t st);
+hipError_t mlpAdam(rt_mlp* m, hipStream_t st);
+}  // namespace rti
+
+struct rt_frame_graph {
+    const rt_device_scene* scene = nullptr;
+    rt_frame frames[2]{};
+    rt_gris_settings settings{};
+    rt_mlp* mlp = nullptr;
+    rtd::WorkQueue* queue = nullptr;  // this graph's own pixel queue
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    std::vector<hipGraphNode_t> nodes[2];
+    std::vector<const void*> funcs[2];
+};
+
+namespace {
+
+rt_status invalid(const char* m) { rti::setError(m); return RT_ERR_INVALID_ARGUMENT; }
+
+// The frame's launches for one parity, recorded (nothing is launched).
+hipError_t recordFrame(rt_frame_graph* g, uint32_t parity, const rt_camera* cams, std::vector<rti::KernelLaunch>& out) {
+    out.clear();
+    rti::tlsRecorder = &out;
+    const rt_frame& f = g->frames[parity];
+    const bool learned = g->settings.targetFunction == 1u;
+    hipError_t e = rti::launchGBuffer(g->scene, cams[0], const_cast<float*>(f.depthNormal),
+                                      const_cast<uint32_t*>(f.albedoMatId), const_cast<uint16_t*>(f.motionVector),
+                                      nullptr);
+    if (e == hipSuccess) e = rti::launchGrisPathTrace(g->scene, cams, f, g->settings, nullptr, g->queue);
+    if (e == hipSuccess && learned) e = rti::launchLearnedRefresh(g->scene, g->mlp, cams, f, nullptr);
+    if (e == hipSuccess) e = rti::launchGrisTemporal(g->scene, cams, f, g->settings, nullptr);
+    if (e == hipSuccess) e = rti::launchGrisSpatial(g->scene, cams, f, g->settings, nullptr);
+    if (e == hipSuccess && learned) {
+        e = rti::mlpGradients(g->mlp, g->mlp->batchPos, g->mlp->batchTarget, g->mlp->batchSize, nullptr, nullptr);
+        if (e == hipSuccess) e = rti::mlpAdam(g->mlp, nullptr);
+    }
+    rti::tlsRecorder = nullptr;
+    return e;
+}
+
+hipKernelNodeParams nodeParams(rti::KernelLaunch& k) {
+    hipKernelNodeParams p{};
+    p.func = const_cast<void*>(k.func);
+    p.gridDim = k.grid;
+    p.blockDim = k.block;
+    p.sharedMemBytes = 0;
+    p.kernelParams = k.args.data();
+    p.extra = nullptr;
+    return p;
+}
+
+void dropGraph(rt_frame_graph* g, uint32_t j) {
+    if (g->exec[j]) (void)hipGraphExecDestroy(g->exec[j]);
+    if (g->graph[j]) (void)hipGraphDestroy(g->graph[j]);
+    g->exec[j] = nullptr;
+    g->graph[j] = nullptr;
+    g->nodes[j].clear();
+    g->funcs[j].clear();
+}
+
+hipError_t buildGraph(rt_frame_graph* g, uint32_t j, std::vector<rti::KernelLaunch>& L) {
+    dropGraph(g, j);
+    hipError_t e = hipGraphCreate(&g->graph[j], 0);
+    hipGraphNode_t prev = nullptr;
+    for (size_t i = 0; e == hipSuccess && i < L.size(); i++) {
+        hipKernelNodeParams p = nodeParams(L[i]);
+        hipGraphNode_t node = nullptr;
+        e = hipGraphAddKernelNode(&node, g->graph[j], prev ? &prev : nullptr, prev ? 1 : 0, &p);
+        g->nodes[j].push_back(node);
+        g->funcs[j].push_back(L[i].func);
+        prev = node;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec[j], g->graph[j], nullptr, nullptr, 0);
+    if (e != hipSuccess) dropGraph(g, j);
+    return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+rt_status rt_frame_graph_create(const rt_device_scene* scene, const rt_frame frames[2], const rt_gris_settings* settings,
+                                rt_mlp* mlp, rt_frame_graph** out) {
+    if (!out) return invalid("out is null");
+    *out = nullptr;
+    if (!scene || !frames || !settings) return invalid("null argument");
+    if (settings->maxDepth == 0 || settings->maxDepth > 255 || settings->shiftType > 2 ||
+        settings->spatialNeighbors > 64 || settings->targetFunction > 1)
+        return invalid("invalid settings");
+    if (settings->targetFunction == 1u && !mlp) return invalid("the learned target needs a network");
+    for (int j = 0; j < 2; j++) {
+        const rt_frame& f = frames[j];
+        if (f.width == 0 || f.height == 0 || f.width >= 32768 || f.height >= 32768 || f.width != frames[0].width ||
+            f.height != frames[0].height)
+            return invalid("both parities need the same frame size in [1, 32767]");
+        if (!f.depthNormal || !f.albedoMatId || !f.motionVector || !f.grisReservoir || !f.grisReservoirPrev ||
+            !f.grisReservoirTemp || !f.indirectOutput)
+            return invalid("frame bindings are incomplete");
+    }
+    rt_frame_graph* g = new rt_frame_graph();
+    g->scene = scene;
+    g->frames[0] = frames[0];
+    g->frames[1] = frames[1];
+    g->settings = *settings;
+    g->mlp = mlp;
+    if (!rti::hipCheck(hipMalloc(&g->queue, sizeof(rtd::WorkQueue)), "hipMalloc") ||
+        !rti::hipCheck(hipMemset(g->queue, 0, sizeof(rtd::WorkQueue)), "hipMemset")) {
+        (void)hipFree(g->queue);
+        delete g;
+        return RT_ERR_HIP;
+    }
+    *out = g;
+    return RT_OK;
+}
+
+rt_status rt_frame_graph_launch(rt_frame_graph* g, uint32_t parity, const rt_camera cams[2], hipStream_t stream) {
+    if (!g || !cams) return invalid("null argument");
+    if (parity > 1) return invalid("parity must be 0 or 1");
+    if (cams[0].filmSize[0] != g->frames[0].width || cams[0].filmSize[1] != g->frames[0].height)
+        return invalid("camera filmSize differs from the frame size");
+    std::vector<rti::KernelLaunch> L;
+    RT_HIP(recordFrame(g, parity, cams, L));
+    bool same = g->exec[parity] && L.size() == g->nodes[parity].size();
+    for (size_t i = 0; same && i < L.size(); i++) same = L[i].func == g->funcs[parity][i];
+    if (same) {
+        for (size_t i = 0; i < L.size(); i++) {
+            hipKernelNodeParams p = nodeParams(L[i]);
+            RT_HIP(hipGraphExecKernelNodeSetParams(g->exec[parity], g->nodes[parity][i], &p));
+        }
+    } else {
+        RT_HIP(buildGraph(g, parity, L));
+    }
+    RT_HIP(hipGraphLaunch(g->exec[parity], stream));
+    return RT_OK;
+}
+
+rt_status rt_frame_graph_destroy(rt_frame_graph* g) {
+    if (!g) return invalid("graph is null");
+    dropGraph(g, 0);
+    dropGraph(g, 1);
+    (void)hipFree(g->queue);
+    delete g;
+    return RT_OK;
+}
+
+}  // extern "C"

@@ -4,7 +4,9 @@
 #include "rt_dtrace.h"
 #include "rt_mlp.h"
 #include <atomic>
+#include <memory>
 #include <string>
+#include <tuple>
 #include <vector>
 
 struct rt_device_scene {
@@ -44,7 +46,7 @@ struct rt_mlp {
     uint64_t gridParams = 0;      // levels * T * 2
     float* gridF32 = nullptr;     // FP32 master grid
     _Float16* gridF16 = nullptr;  // FP16 copy read by the kernels
-    float* gridGrad = nullptr;    // summed per-sample gradients (unscaled)
+    long long* gridGrad = nullptr;  // summed per-sample gradients, 32.32 fixed point (order-independent sums)
     float* gridM = nullptr;
     float* gridV = nullptr;
     float* wF32 = nullptr;
@@ -71,6 +73,38 @@ bool buildLBVH(rt_device_scene* s, const std::vector<uint32_t>& instPrimOffset, 
 // Counters of the counting kernel builds.
 extern bool gCountersEnabled;
 rtd::TraceCounters* counterBuffer();
+
+// A kernel launch with its arguments held by value: launched at once, or
+// recorded while a recorder is installed on this thread (frame graphs build
+// and update their kernel nodes from recorded launches).
+struct KernelLaunch {
+    const void* func = nullptr;
+    dim3 grid, block;
+    std::shared_ptr<void> store;   // owns the argument values
+    std::vector<void*> args;       // pointers into store, in parameter order
+};
+extern thread_local std::vector<KernelLaunch>* tlsRecorder;
+
+template <class... P, class... A>
+hipError_t launch(void (*kernel)(P...), dim3 grid, dim3 block, hipStream_t stream, A&&... a) {
+    using Args = std::tuple<std::decay_t<P>...>;
+    if (tlsRecorder) {
+        auto store = std::make_shared<Args>(std::forward<A>(a)...);
+        KernelLaunch k;
+        k.func = reinterpret_cast<const void*>(kernel);
+        k.grid = grid;
+        k.block = block;
+        std::apply([&](auto&... x) { (k.args.push_back(static_cast<void*>(&x)), ...); }, *store);
+        k.store = store;
+        tlsRecorder->push_back(std::move(k));
+        return hipSuccess;
+    }
+    Args values(std::forward<A>(a)...);
+    void* ptrs[sizeof...(P) > 0 ? sizeof...(P) : 1];
+    size_t i = 0;
+    std::apply([&](auto&... x) { ((ptrs[i++] = static_cast<void*>(&x)), ...); }, values);
+    return hipLaunchKernel(reinterpret_cast<const void*>(kernel), grid, block, ptrs, 0, stream);
+}
 
 }  // namespace rti
 

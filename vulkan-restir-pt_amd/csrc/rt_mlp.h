@@ -25,6 +25,7 @@ constexpr int kOut = 3;
 constexpr int kW1 = 0, kW2 = kEnc * kHidden, kW3 = kW2 + kHidden * kHidden;
 constexpr int kWeights = kW3 + kHidden * kOut;  // 6336
 constexpr int kTileSamples = 32;
+constexpr uint32_t kBatchCap = 1u << 16;  // training samples gathered per frame
 constexpr float kLossEps = 0.01f;    // relative L2: (y^ - y)^2 / (sg(y^)^2 + eps)
 constexpr float kGradClip = 64.0f;   // per-sample output gradient bound (FP16 range)
 constexpr float kLogClamp = 20.0f;   // exp output activation input bound

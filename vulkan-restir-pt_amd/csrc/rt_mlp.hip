@@ -5,16 +5,18 @@
 // Training data flow per step: one wave per 32-sample tile evaluates the
 // network (rt_mlp.h), forms the relative-L2 output gradient, back-propagates
 // through the three layers on the matrix cores, scatters the encoding
-// gradient into the FP32 grid-gradient table with atomics, and accumulates
-// the weight gradients of all its tiles in registers. Per-wave partial sums
-// are reduced in a fixed order, so the weight gradients are deterministic;
-// the grid gradients are sums of atomics (order-dependent in the last bits).
+// gradient into the grid-gradient table with 64-bit integer atomics on 32.32
+// fixed-point values (integer sums do not depend on the order of the atomics),
+// and accumulates the weight gradients of all its tiles in registers. Per-wave
+// partial sums are reduced in a fixed order. Training is therefore
+// deterministic: the same batch gives bit-identical parameters.
 #include "rt_internal.h"
 #include <cmath>
 #include <cstring>
 #include <vector>
 
 using namespace rtm;
+using rti::launch;
 
 namespace {
 
@@ -22,6 +24,8 @@ constexpr int kStats = 2;                 // loss sum, valid sample count (after
 constexpr int kPartialStride = kWeights + kStats;
 constexpr uint32_t kMaxTrainWaves = 512;
 constexpr int kInferWaves = 4;
+constexpr float kFix = 4294967296.0f;        // 2^32: grid-gradient fixed-point scale
+constexpr double kInvFix = 1.0 / 4294967296.0;
 
 // ---- initialisation -------------------------------------------------------------------------
 __device__ __forceinline__ uint64_t splitmix(uint64_t x) {
@@ -55,6 +59,10 @@ __global__ void k_mlp_init_weights(float* w32, _Float16* w16, uint32_t seed) {
 __global__ void k_fill_f32(float* p, uint64_t n, float v) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+__global__ void k_fill_i64(long long* p, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 __global__ void k_to_f16(const float* src, _Float16* dst, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -102,7 +110,7 @@ __device__ __forceinline__ int neuron(int t, int g, int h, int e) { return 32 * 
 
 __global__ __launch_bounds__(64) void k_mlp_train(MlpView M, const float* __restrict__ pos,
                                                   const float* __restrict__ target, uint32_t n,
-                                                  float* __restrict__ gridGrad, float* __restrict__ partial) {
+                                                  long long* __restrict__ gridGrad, float* __restrict__ partial) {
     __shared__ TrainLDS L;
     loadWeightsLDS(L.w, M.w);
     const uint32_t lane = threadIdx.x;
@@ -183,8 +191,9 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpView M, const float* __rest
                     levelCell(M, lev, u, v, w, ci, fr);
                     for (int c = 0; c < 8; c++) {
                         Corner k = corner(M, lev, ci, fr, c);
-                        atomicAdd(gridGrad + 2 * (size_t)k.slot, k.wt * d0);
-                        atomicAdd(gridGrad + 2 * (size_t)k.slot + 1, k.wt * d1);
+                        unsigned long long* gp = reinterpret_cast<unsigned long long*>(gridGrad + 2 * (size_t)k.slot);
+                        atomicAdd(gp, (unsigned long long)__float2ll_rn(k.wt * d0 * kFix));
+                        atomicAdd(gp + 1, (unsigned long long)__float2ll_rn(k.wt * d1 * kFix));
                     }
                 }
             }
@@ -265,15 +274,19 @@ __global__ void k_mlp_reduce(const float* __restrict__ partial, uint32_t waves, 
     }
 }
 
+__device__ __forceinline__ float gradValue(float g) { return g; }
+__device__ __forceinline__ float gradValue(long long g) { return (float)((double)g * kInvFix); }
+
 // Adam on one parameter array; the gradient of the mean loss is grad / (3 count).
 // Grid entries no sample touched (gradient exactly zero) are left unchanged.
-__global__ void k_mlp_adam(float* __restrict__ p32, _Float16* __restrict__ p16, const float* __restrict__ grad,
+template <class G>
+__global__ void k_mlp_adam(float* __restrict__ p32, _Float16* __restrict__ p16, const G* __restrict__ grad,
                            float* __restrict__ m, float* __restrict__ v, uint64_t n, const float* __restrict__ stats,
                            float lr, float b1, float b2, float eps, float c1, float c2, int skipZero) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float gr = grad[i];
-    if (skipZero && gr == 0.0f) return;
+    if (skipZero && grad[i] == G(0)) return;
+    float gr = gradValue(grad[i]);
     float cnt = stats[1];
     float g = cnt > 0.0f ? gr / (3.0f * cnt) : 0.0f;
     float mi = b1 * m[i] + (1.0f - b1) * g;
@@ -310,12 +323,13 @@ hipError_t zeroAdam(rt_mlp* m, hipStream_t st) {
 namespace rti {
 // Gradients of a batch into m->gridGrad / m->wGrad (used by the ABI and the frame trainer).
 hipError_t mlpGradients(rt_mlp* m, const float* pos, const float* target, uint32_t n, float* loss, hipStream_t st) {
-    k_fill_f32<<<blocks(m->gridParams, 256), 256, 0, st>>>(m->gridGrad, m->gridParams, 0.0f);
+    (void)launch(k_fill_i64, dim3(blocks(m->gridParams, 256)), dim3(256), st, m->gridGrad, m->gridParams);
     const uint32_t tiles = (n + kTileSamples - 1) / kTileSamples;
     const uint32_t waves = std::max(1u, std::min(tiles, m->partialWaves));
-    if (n) k_mlp_train<<<waves, 64, 0, st>>>(m->view, pos, target, n, m->gridGrad, m->partial);
-    else k_fill_f32<<<blocks(kPartialStride, 256), 256, 0, st>>>(m->partial, kPartialStride, 0.0f);
-    k_mlp_reduce<<<blocks(kPartialStride, 256), 256, 0, st>>>(m->partial, n ? waves : 1u, m->wGrad, loss);
+    if (n) (void)launch(k_mlp_train, dim3(waves), dim3(64), st, m->view, pos, target, n, m->gridGrad, m->partial);
+    else (void)launch(k_fill_f32, dim3(blocks(kPartialStride, 256)), dim3(256), st, m->partial, (uint64_t)kPartialStride, 0.0f);
+    (void)launch(k_mlp_reduce, dim3(blocks(kPartialStride, 256)), dim3(256), st, (const float*)m->partial,
+                 n ? waves : 1u, m->wGrad, loss);
     return hipGetLastError();
 }
 hipError_t mlpAdam(rt_mlp* m, hipStream_t st) {
@@ -324,11 +338,12 @@ hipError_t mlpAdam(rt_mlp* m, hipStream_t st) {
     const float c1 = 1.0f - (float)std::pow((double)c.beta1, (double)m->step);
     const float c2 = 1.0f - (float)std::pow((double)c.beta2, (double)m->step);
     const float* stats = m->wGrad + kWeights;
-    k_mlp_adam<<<blocks(m->gridParams, 256), 256, 0, st>>>(m->gridF32, m->gridF16, m->gridGrad, m->gridM, m->gridV,
-                                                            m->gridParams, stats, c.learningRate, c.beta1, c.beta2,
-                                                            c.epsilon, c1, c2, 1);
-    k_mlp_adam<<<blocks(kWeights, 256), 256, 0, st>>>(m->wF32, m->wF16, m->wGrad, m->wM, m->wV, kWeights, stats,
-                                                       c.learningRate, c.beta1, c.beta2, c.epsilon, c1, c2, 0);
+    (void)launch(k_mlp_adam<long long>, dim3(blocks(m->gridParams, 256)), dim3(256), st, m->gridF32, m->gridF16,
+                 (const long long*)m->gridGrad, m->gridM, m->gridV, m->gridParams, stats, c.learningRate, c.beta1,
+                 c.beta2, c.epsilon, c1, c2, 1);
+    (void)launch(k_mlp_adam<float>, dim3(blocks(kWeights, 256)), dim3(256), st, m->wF32, m->wF16,
+                 (const float*)m->wGrad, m->wM, m->wV, (uint64_t)kWeights, stats, c.learningRate, c.beta1, c.beta2,
+                 c.epsilon, c1, c2, 0);
     return hipGetLastError();
 }
 }  // namespace rti
@@ -370,7 +385,7 @@ rt_status rt_mlp_create(const rt_mlp_config* c, hipStream_t stream, rt_mlp** out
     const size_t G = m->gridParams;
     bool ok = rti::hipCheck(hipMalloc(&m->gridF32, 4 * G), "hipMalloc") &&
               rti::hipCheck(hipMalloc(&m->gridF16, 2 * G), "hipMalloc") &&
-              rti::hipCheck(hipMalloc(&m->gridGrad, 4 * G), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->gridGrad, 8 * G), "hipMalloc") &&
               rti::hipCheck(hipMalloc(&m->gridM, 4 * G), "hipMalloc") &&
               rti::hipCheck(hipMalloc(&m->gridV, 4 * G), "hipMalloc") &&
               rti::hipCheck(hipMalloc(&m->wF32, 4 * kWeights), "hipMalloc") &&
@@ -378,13 +393,15 @@ rt_status rt_mlp_create(const rt_mlp_config* c, hipStream_t stream, rt_mlp** out
               rti::hipCheck(hipMalloc(&m->wGrad, 4 * kPartialStride), "hipMalloc") &&
               rti::hipCheck(hipMalloc(&m->wM, 4 * kWeights), "hipMalloc") &&
               rti::hipCheck(hipMalloc(&m->wV, 4 * kWeights), "hipMalloc") &&
-              rti::hipCheck(hipMalloc(&m->partial, 4 * (size_t)kPartialStride * kMaxTrainWaves), "hipMalloc");
+              rti::hipCheck(hipMalloc(&m->partial, 4 * (size_t)kPartialStride * kMaxTrainWaves), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->batchPos, sizeof(float) * 3 * kBatchCap), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->batchTarget, sizeof(float) * 3 * kBatchCap), "hipMalloc");
     if (ok) {
         V.grid = reinterpret_cast<const uint32_t*>(m->gridF16);
         V.w = m->wF16;
         k_mlp_init_grid<<<blocks(G, 256), 256, 0, stream>>>(m->gridF32, m->gridF16, G, c->seed);
         k_mlp_init_weights<<<blocks(kWeights, 256), 256, 0, stream>>>(m->wF32, m->wF16, c->seed);
-        k_fill_f32<<<blocks(G, 256), 256, 0, stream>>>(m->gridGrad, G, 0.0f);
+        k_fill_i64<<<blocks(G, 256), 256, 0, stream>>>(m->gridGrad, G);
         k_fill_f32<<<blocks(kPartialStride, 256), 256, 0, stream>>>(m->wGrad, kPartialStride, 0.0f);
         ok = rti::hipCheck(zeroAdam(m, stream), "init kernels") &&
              rti::hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
@@ -457,8 +474,9 @@ rt_status rt_mlp_read_gradients(const rt_mlp* m, float* gridGrad, float* weightG
     if (weightGrad)
         for (int i = 0; i < kWeights; i++) weightGrad[i] = w[i] * scale;
     if (gridGrad) {
-        RT_HIP(hipMemcpy(gridGrad, m->gridGrad, 4 * m->gridParams, hipMemcpyDeviceToHost));
-        for (uint64_t i = 0; i < m->gridParams; i++) gridGrad[i] *= scale;
+        std::vector<long long> g(m->gridParams);
+        RT_HIP(hipMemcpy(g.data(), m->gridGrad, 8 * m->gridParams, hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < m->gridParams; i++) gridGrad[i] = (float)((double)g[i] * kInvFix) * scale;
     }
     return RT_OK;
 }
@@ -478,7 +496,7 @@ rt_status rt_mlp_train_step(rt_mlp* m, const float* pos, const float* target, ui
 
 rt_status rt_mlp_batch_download(const rt_mlp* m, float* pos, float* target, uint32_t* count) {
     if (!m) return invalid("mlp is null");
-    const uint32_t n = m->batchPos ? m->batchSize : 0u;
+    const uint32_t n = m->batchSize;
     if (count) *count = n;
     RT_HIP(hipDeviceSynchronize());
     if (n && pos) RT_HIP(hipMemcpy(pos, m->batchPos, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
@@ -488,7 +506,7 @@ rt_status rt_mlp_batch_download(const rt_mlp* m, float* pos, float* target, uint
 
 rt_status rt_mlp_train_frame(rt_mlp* m, float* loss, hipStream_t stream) {
     if (!m) return invalid("mlp is null");
-    if (!m->batchPos || m->batchSize == 0) return invalid("no training batch: call rt_gris_learned_refresh first");
+    if (m->batchSize == 0) return invalid("no training batch: call rt_gris_learned_refresh first");
     return rt_mlp_train_step(m, m->batchPos, m->batchTarget, m->batchSize, loss, stream);
 }
 

@@ -1069,22 +1069,23 @@ inline ResvPlanes planes(void* p, uint32_t W, uint32_t H) {
 // Launchers (called by the C ABI after argument checks)
 namespace rti {
 
-#define RT_DISPATCH(KERNEL, GRID, BLOCK, STREAM, ...)                                              \
-    do {                                                                                             \
-        if (gCountersEnabled) KERNEL<true><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__, counterBuffer()); \
-        else KERNEL<false><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__, nullptr);                        \
+// Counting or plain build of a kernel, launched (or recorded) through rti::launch.
+#define RT_DISPATCH(KERNEL, GRID, BLOCK, STREAM, ...)                                                  \
+    do {                                                                                                 \
+        if (gCountersEnabled) (void)launch(KERNEL<true>, GRID, BLOCK, STREAM, __VA_ARGS__, counterBuffer()); \
+        else (void)launch(KERNEL<false>, GRID, BLOCK, STREAM, __VA_ARGS__, nullptr);                        \
     } while (0)
 
 // Same, for kernels that also take the target function as a template argument.
-#define RT_DISPATCH_L(KERNEL, LEARNED, GRID, BLOCK, STREAM, ...)                                                  \
-    do {                                                                                                            \
-        if (LEARNED) {                                                                                              \
-            if (gCountersEnabled) KERNEL<true, true><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__, counterBuffer());     \
-            else KERNEL<false, true><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__, nullptr);                             \
-        } else {                                                                                                    \
-            if (gCountersEnabled) KERNEL<true, false><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__, counterBuffer());    \
-            else KERNEL<false, false><<<GRID, BLOCK, 0, STREAM>>>(__VA_ARGS__, nullptr);                            \
-        }                                                                                                           \
+#define RT_DISPATCH_L(KERNEL, LEARNED, GRID, BLOCK, STREAM, ...)                                                \
+    do {                                                                                                          \
+        if (LEARNED) {                                                                                            \
+            if (gCountersEnabled) (void)launch(KERNEL<true, true>, GRID, BLOCK, STREAM, __VA_ARGS__, counterBuffer()); \
+            else (void)launch(KERNEL<false, true>, GRID, BLOCK, STREAM, __VA_ARGS__, nullptr);                  \
+        } else {                                                                                                  \
+            if (gCountersEnabled) (void)launch(KERNEL<true, false>, GRID, BLOCK, STREAM, __VA_ARGS__, counterBuffer()); \
+            else (void)launch(KERNEL<false, false>, GRID, BLOCK, STREAM, __VA_ARGS__, nullptr);                 \
+        }                                                                                                         \
     } while (0)
 
 hipError_t launchTraceRays(const rt_device_scene* s, const float* rays8, uint32_t n, int mode, int skipLights, float* t,
@@ -1125,15 +1126,16 @@ static hipError_t persistentGrid(Kernel kernel, uint32_t W, uint32_t H, uint32_t
     return hipSuccess;
 }
 
+// queue: the pixel queue of this launch (null: the scene's next round-robin slot).
 hipError_t launchGrisPathTrace(const rt_device_scene* s, const rt_camera* cams, const rt_frame& f,
-                               const rt_gris_settings& set, hipStream_t stream) {
+                               const rt_gris_settings& set, hipStream_t stream, WorkQueue* queue) {
     GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
     uint32_t grid = 0;
     hipError_t e = gCountersEnabled ? persistentGrid(k_gris_path_trace<true>, f.width, f.height, grid)
                                     : persistentGrid(k_gris_path_trace<false>, f.width, f.height, grid);
     if (e != hipSuccess) return e;
     RT_DISPATCH(k_gris_path_trace, dim3(grid), kTileThreads, stream, s->dev, cams[0], g, set,
-                planes(f.grisReservoir, f.width, f.height), s->nextQueue());
+                planes(f.grisReservoir, f.width, f.height), queue ? queue : s->nextQueue());
     return hipGetLastError();
 }
 
@@ -1158,22 +1160,15 @@ hipError_t launchGrisSpatial(const rt_device_scene* s, const rt_camera* cams, co
 
 hipError_t launchLearnedRefresh(const rt_device_scene* s, rt_mlp* m, const rt_camera* cams, const rt_frame& f,
                                 hipStream_t stream) {
-    constexpr uint32_t kBatchCap = 1u << 16;
     const uint32_t n = f.width * f.height;
-    if (!m->batchPos) {
-        hipError_t e = hipMalloc(&m->batchPos, sizeof(float) * 3 * kBatchCap);
-        if (e == hipSuccess) e = hipMalloc(&m->batchTarget, sizeof(float) * 3 * kBatchCap);
-        if (e != hipSuccess) return e;
-    }
-    const uint32_t stride = std::max(1u, (n + kBatchCap - 1) / kBatchCap);
+    const uint32_t stride = std::max(1u, (n + rtm::kBatchCap - 1) / rtm::kBatchCap);
     const uint32_t offset = (cams[0].frameIndex & 0x7fffffffu) % stride;
     m->batchSize = offset < n ? (n - offset + stride - 1) / stride : 0u;
     GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
     const uint32_t tiles = (n + 31u) / 32u;
     const uint32_t blocks = std::min<uint32_t>((tiles + kRefreshWaves - 1) / kRefreshWaves, 4096u);
-    k_gris_learned_refresh<<<blocks, 64 * kRefreshWaves, 0, stream>>>(s->dev, m->view, g,
-                                                                      planes(f.grisReservoir, f.width, f.height),
-                                                                      m->batchPos, m->batchTarget, stride, offset);
+    (void)launch(k_gris_learned_refresh, dim3(blocks), dim3(64 * kRefreshWaves), stream, s->dev, m->view, g,
+                 planes(f.grisReservoir, f.width, f.height), m->batchPos, m->batchTarget, stride, offset);
     return hipGetLastError();
 }
 

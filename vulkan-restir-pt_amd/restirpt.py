@@ -137,6 +137,7 @@ CORE_SYMBOLS = [
     "rt_counters_read", "rt_mlp_create", "rt_mlp_destroy", "rt_mlp_param_counts", "rt_mlp_get_params",
     "rt_mlp_set_params", "rt_mlp_infer", "rt_mlp_gradients", "rt_mlp_read_gradients", "rt_mlp_adam_step",
     "rt_mlp_train_step", "rt_mlp_train_frame", "rt_mlp_batch_download", "rt_gris_learned_refresh",
+    "rt_frame_graph_create", "rt_frame_graph_launch", "rt_frame_graph_destroy",
 ]
 HOST_SYMBOLS = [
     "rt_scenegen", "rt_host_scene_destroy", "rt_host_scene_arrays", "rt_host_scene_camera", "rt_light_table_build",
@@ -185,6 +186,9 @@ def _declare_core(L):
         "rt_mlp_train_frame": [P, P, P],
         "rt_mlp_batch_download": [P, P, P, C.POINTER(C.c_uint32)],
         "rt_gris_learned_refresh": [P, P, C.POINTER(Camera), C.POINTER(Frame), P],
+        "rt_frame_graph_create": [P, C.POINTER(Frame), C.POINTER(GrisSettings), P, C.POINTER(P)],
+        "rt_frame_graph_launch": [P, u32, C.POINTER(Camera), P],
+        "rt_frame_graph_destroy": [P],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
