@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1-GiB device copy (calibrates HBM byte counters)")
     ap.add_argument("--no-learned", action="store_true", help="skip the equal-time relMSE comparison")
+    ap.add_argument("--learned-kernel-loop", action="store_true",
+                    help="time learned-target frames only (for profilers)")
     ap.add_argument("--lvp-frames", type=int, default=64, help="frames of the plain form in the equal-time run")
     ap.add_argument("--ref-frames", type=int, default=4096, help="frames of the converged reference image")
     return ap.parse_args()
@@ -320,6 +322,34 @@ def learned_vs_plain(rt, hs, ds, args):
                     f"+ MLP 2x64 FP16 (MFMA), trained online each frame on 2^16 reconnection samples, Adam lr 1e-2"}
 
 
+def learned_kernel_loop(rt, hs, ds, args):
+    """Learned-target frames of the benchmark camera walk (path trace, refresh, temporal,
+    spatial, one training step) with per-pass HIP events, for profiling."""
+    import torch
+
+    W, H = args.width, args.height
+    fr = TorchFrames(rt, ds, W, H)
+    s = rt.settings(shift=2, temporal=True, spatial=True, cap=20, max_depth=6, neighbors=3,
+                    radius=30.0 * W / 1920.0, learned=True)
+    info = ds.info()
+    lo, hi = [float(v) for v in info.sceneLo], [float(v) for v in info.sceneHi]
+    ext = [max(h - l, 1e-3) for l, h in zip(lo, hi)]
+    mlp = rt.Mlp(rt.mlp_config([l - 0.1 * e for l, e in zip(lo, ext)], [h + 0.1 * e for h, e in zip(hi, ext)],
+                               log2_table=19, seed=3))
+    for k in range(args.warmup):
+        fr.frame(hs.camera(k % 16, W, H), s, mlp)
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        fr.frame(hs.camera((args.warmup + k) % 16, W, H), s, mlp, ev[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return {"learned_frames": args.steps, "ms_per_frame": round(wall * 1e3 / args.steps, 3),
+            "gpu_ms_per_frame": round(sum(e[0].elapsed_time(e[1]) for e in ev) / args.steps, 3),
+            "mpaths_per_s": round(W * H * args.steps / wall / 1e6, 3)}
+
+
 def read_traffic(kernel):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_1080p_terrain_traffic.md")
@@ -353,6 +383,9 @@ def main():
     ds = rt.DeviceScene(hs.arrays)
     info = ds.info()
     s = settings(rt, W)
+    if args.learned_kernel_loop:
+        print(json.dumps(learned_kernel_loop(rt, hs, ds, args)))
+        return
     loop = FrameLoop(rt, hs, ds, W, H, s)
     for k in range(args.warmup):
         loop.frame(k)
