@@ -257,20 +257,27 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpView M, const float* __rest
 }
 
 // Weight gradients and stats: fixed-order sum over the waves' partials.
+// Sum of column k over the waves' partials in a fixed order (eight
+// interleaved accumulators for memory-level parallelism, combined pairwise).
+__device__ __forceinline__ float columnSum(const float* __restrict__ partial, uint32_t waves, int k) {
+    float a[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t w = 0;
+    for (; w + 8u <= waves; w += 8u)
+#pragma unroll
+        for (int i = 0; i < 8; i++) a[i] += partial[(size_t)(w + i) * kPartialStride + k];
+    for (; w < waves; w++) a[0] += partial[(size_t)w * kPartialStride + k];
+    return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+}
+
 __global__ void k_mlp_reduce(const float* __restrict__ partial, uint32_t waves, float* __restrict__ out,
                              float* __restrict__ lossOut) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= kPartialStride) return;
-    float acc = 0.0f;
-    for (uint32_t w = 0; w < waves; w++) acc += partial[(size_t)w * kPartialStride + k];
+    const float acc = columnSum(partial, waves, k);
     out[k] = acc;
-    if (k == 0 && lossOut) {
-        float cnt = 0.0f, sum = 0.0f;
-        for (uint32_t w = 0; w < waves; w++) {
-            sum += partial[(size_t)w * kPartialStride + kWeights];
-            cnt += partial[(size_t)w * kPartialStride + kWeights + 1];
-        }
-        *lossOut = cnt > 0.0f ? sum / (3.0f * cnt) : 0.0f;
+    if (k == kWeights && lossOut) {
+        const float cnt = columnSum(partial, waves, kWeights + 1);
+        *lossOut = cnt > 0.0f ? acc / (3.0f * cnt) : 0.0f;
     }
 }
 

@@ -461,12 +461,38 @@ RT_DEV void resvCap(Resv& r, float cap) {
 struct PTPath {
     uint32_t bounce, rng, sampleState, lastSampleState;
     bool isLastVertexConnectible;
-    f3 throughput, rcThroughput, wo, lastPos, rdir, rori;
+    bool picked;  // a stream-RIS candidate has been picked (it is kept in LDS)
+    f3 throughput, rcThroughput, lastPos, rdir, rori;  // wo = -rdir
     BSDFSample s;
     Isec isec;
-    Resv ps, stream;  // the path sample and the stream-RIS pick
+    Resv ps;  // the path sample
     float streamWeight, streamSum;
 };
+
+// The stream-RIS pick of a lane lives in LDS (18 words, stride = lanes per
+// workgroup): it is written only when a candidate is picked and read once at
+// the end of the path, so keeping it out of the registers buys occupancy.
+constexpr int kStreamWords = 18;
+RT_DEV void storeStream(float* slot, const Resv& r) {
+    const float v[kStreamWords] = {r.rcIsec.b1, r.rcIsec.b2, __uint_as_float(r.rcIsec.inst), __uint_as_float(r.rcIsec.prim),
+                                   r.rcLi.x, r.rcLi.y, r.rcLi.z, __uint_as_float(r.rcRng),
+                                   r.rcWi.x, r.rcWi.y, r.rcWi.z, __uint_as_float(r.flags),
+                                   r.rcPrevSamplePdf, r.rcJacobian, r.F.x, r.F.y, r.F.z, __uint_as_float(r.primaryRng)};
+#pragma unroll
+    for (int i = 0; i < kStreamWords; i++) slot[i * kTileThreads] = v[i];
+}
+RT_DEV Resv loadStream(const float* slot) {
+    float v[kStreamWords];
+#pragma unroll
+    for (int i = 0; i < kStreamWords; i++) v[i] = slot[i * kTileThreads];
+    Resv r = zeroResv();
+    r.rcIsec.b1 = v[0]; r.rcIsec.b2 = v[1]; r.rcIsec.inst = __float_as_uint(v[2]); r.rcIsec.prim = __float_as_uint(v[3]);
+    r.rcLi = mk3(v[4], v[5], v[6]); r.rcRng = __float_as_uint(v[7]);
+    r.rcWi = mk3(v[8], v[9], v[10]); r.flags = __float_as_uint(v[11]);
+    r.rcPrevSamplePdf = v[12]; r.rcJacobian = v[13];
+    r.F = mk3(v[14], v[15], v[16]); r.primaryRng = __float_as_uint(v[17]);
+    return r;
+}
 
 // Start of the invocation (gris_path_trace.glsl:45-100): primary vertex from
 // the G-buffer. Returns false for background pixels, whose reservoir is left
@@ -481,7 +507,6 @@ RT_DEV bool ptBegin(const DevScene& S, const rt_camera& cam, const GB& g, uint32
     P.rng = makeSeedPix(cam.seed, x, y);
     P.bounce = 0;
     P.throughput = s3(1.0f); P.rcThroughput = s3(0.0f);
-    P.wo = -ray.dir;
     P.lastPos = s3(0.0f);
     P.isLastVertexConnectible = false;
     P.sampleState = 0; P.lastSampleState = 0;
@@ -494,7 +519,7 @@ RT_DEV bool ptBegin(const DevScene& S, const rt_camera& cam, const GB& g, uint32
     P.ps = zeroResv();  // the path sample (sampleCount/resampleWeight unused here)
     P.ps.rcIsec.inst = RT_INVALID_HIT;
     P.ps.primaryRng = P.rng;
-    P.stream = zeroResv();
+    P.picked = false;
     P.streamWeight = 0.0f; P.streamSum = 0.0f;
     return true;
 }
@@ -502,7 +527,7 @@ RT_DEV bool ptBegin(const DevScene& S, const rt_camera& cam, const GB& g, uint32
 // One vertex of the bounce loop, after its closest-hit query. Returns false
 // where the reference leaves the loop; otherwise the next ray is in rori/rdir.
 template <bool kCount>
-RT_DEV bool ptVertex(const DevScene& S, TStack st, const rt_gris_settings& set, PTPath& P, const Surf& surf,
+RT_DEV bool ptVertex(const DevScene& S, TStack st, float* slot, const rt_gris_settings& set, PTPath& P, const Surf& surf,
                      const Mat& mat, const uint32_t bounce, TraceCounters* ctr) {
     uint32_t& rng = P.rng;
     uint32_t& sampleState = P.sampleState;
@@ -510,14 +535,13 @@ RT_DEV bool ptVertex(const DevScene& S, TStack st, const rt_gris_settings& set, 
     bool& isLastVertexConnectible = P.isLastVertexConnectible;
     f3& throughput = P.throughput;
     f3& rcThroughput = P.rcThroughput;
-    f3& wo = P.wo;
+    const f3 wo = -P.rdir;
     f3& lastPos = P.lastPos;
     f3& rdir = P.rdir;
     f3& rori = P.rori;
     BSDFSample& s = P.s;
     const Isec& isec = P.isec;
     Resv& ps = P.ps;
-    Resv& stream = P.stream;
     float& streamWeight = P.streamWeight;
     float& streamSum = P.streamSum;
     {
@@ -554,7 +578,7 @@ RT_DEV bool ptVertex(const DevScene& S, TStack st, const rt_gris_settings& set, 
                     flagsSetRcVertexType(ps.flags, RC_LIGHT_SCATTERED);
                     float wgt = luminance(ps.F);
                     streamSum += wgt;
-                    if (resvRandSample * streamSum < wgt) { streamWeight = wgt; stream = ps; }
+                    if (resvRandSample * streamSum < wgt) { streamWeight = wgt; storeStream(slot, ps); P.picked = true; }
                 }
             }
             return false;
@@ -592,7 +616,7 @@ RT_DEV bool ptVertex(const DevScene& S, TStack st, const rt_gris_settings& set, 
                     ps.F = weightedLi * scatterTerm * throughput;
                     float wgt = luminance(ps.F);
                     streamSum += wgt;
-                    if (resvRandSample * streamSum < wgt) { streamWeight = wgt; stream = ps; }
+                    if (resvRandSample * streamSum < wgt) { streamWeight = wgt; storeStream(slot, ps); P.picked = true; }
                 } else if (sampleState == 1 && isThisVertexConnectible && lightDist > GRIS_DIST_THRESHOLD) {
                     ps.rcIsec.b1 = lightBary.x; ps.rcIsec.b2 = lightBary.y;
                     ps.rcIsec.inst = 0u; ps.rcIsec.prim = lightId;
@@ -606,7 +630,7 @@ RT_DEV bool ptVertex(const DevScene& S, TStack st, const rt_gris_settings& set, 
                     flagsSetRcVertexType(ps.flags, RC_LIGHT_SAMPLED);
                     float wgt = luminance(ps.F);
                     streamSum += wgt;
-                    if (resvRandSample * streamSum < wgt) { streamWeight = wgt; stream = ps; }
+                    if (resvRandSample * streamSum < wgt) { streamWeight = wgt; storeStream(slot, ps); P.picked = true; }
                 }
             }
         }
@@ -631,7 +655,6 @@ RT_DEV bool ptVertex(const DevScene& S, TStack st, const rt_gris_settings& set, 
             rcThroughput = rcThroughput / s.pdf;
         }
         lastPos = surf.pos;
-        wo = -s.wi;
         rdir = s.wi;
         rori = surf.pos + rdir * 1e-4f;
         isLastVertexConnectible = isThisVertexConnectible;
@@ -641,14 +664,20 @@ RT_DEV bool ptVertex(const DevScene& S, TStack st, const rt_gris_settings& set, 
 
 // After the bounce loop (gris_path_trace.glsl:280-306): last stream-RIS
 // candidate and the reservoir written for the pixel.
-RT_DEV Resv ptFinish(PTPath& P) {
+RT_DEV Resv ptFinish(PTPath& P, const float* slot) {
+    Resv out;
+    bool lastPicked = false;
     if (P.sampleState == 2 && P.lastSampleState == 2) {
         float wgt = luminance(P.ps.F);
         float r = sample1f(P.rng);
         P.streamSum += wgt;
-        if (r * P.streamSum < wgt) { P.streamWeight = wgt; P.stream = P.ps; }
+        if (r * P.streamSum < wgt) {
+            P.streamWeight = wgt;
+            out = P.ps;
+            lastPicked = true;
+        }
     }
-    Resv out = P.stream;
+    if (!lastPicked) out = P.picked ? loadStream(slot) : zeroResv();
     out.resampleWeight = 0.0f;
     if (P.streamSum > 0.0f && P.streamWeight > 0.0f) {
         float k = P.streamSum / P.streamWeight;
@@ -670,7 +699,9 @@ template <bool kCount>
 __global__ __launch_bounds__(kTileThreads, RT_WAVES_PATH_TRACE) void k_gris_path_trace(
     DevScene S, rt_camera cam, GB g, rt_gris_settings set, ResvPlanes outR, WorkQueue* q, TraceCounters* ctr) {
     __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    __shared__ float picks[kStreamWords * kTileThreads];
     TStack st{stk + threadIdx.x, kTileThreads};
+    float* slot = picks + threadIdx.x;
     const uint32_t lane = threadIdx.x;
     const uint32_t tilesX = (g.W + kTileX - 1) / kTileX;
     const uint32_t total = tilesX * ((g.H + kTileY - 1) / kTileY) * kTileThreads;
@@ -696,8 +727,8 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_PATH_TRACE) void k_gris_path
             Surf surf; Mat mat;
             if (x >= g.W || y >= g.H || !ptBegin(S, cam, g, x, y, P, surf, mat)) continue;
             pix = y * g.W + x;
-            live = ptVertex<kCount>(S, st, set, P, surf, mat, 0u, ctr) && ++P.bounce < set.maxDepth;
-            if (!live) storeResv(outR, pix, ptFinish(P));
+            live = ptVertex<kCount>(S, st, slot, set, P, surf, mat, 0u, ctr) && ++P.bounce < set.maxDepth;
+            if (!live) storeResv(outR, pix, ptFinish(P, slot));
         }
         if (__ballot(live) == 0ull) break;
         if (live) {
@@ -707,9 +738,9 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_PATH_TRACE) void k_gris_path
                 Surf surf;
                 loadSurfaceInfo(S, P.isec, surf);
                 Mat mat = loadMat(S, surf.matIndex);
-                live = ptVertex<kCount>(S, st, set, P, surf, mat, P.bounce, ctr) && ++P.bounce < set.maxDepth;
+                live = ptVertex<kCount>(S, st, slot, set, P, surf, mat, P.bounce, ctr) && ++P.bounce < set.maxDepth;
             }
-            if (!live) storeResv(outR, pix, ptFinish(P));
+            if (!live) storeResv(outR, pix, ptFinish(P, slot));
         }
     }
     if (lane == 0) {
@@ -978,70 +1009,83 @@ __global__ void k_resv_reset(ResvPlanes R) {
     storeResv<true>(R, i, r);
 }
 
-// Learned-target refresh after the path-trace pass. Each wave takes 32
-// pixels: lanes 0-31 own them, all 64 lanes encode the reconnection vertex
-// (each half of the grid levels) and the network runs on the matrix cores.
-// For a surface rc vertex the contribution with the predicted radiance,
-// F_L = F * Lhat / rcLi per channel, gives the learned target value; the
-// resample weight, which refers to luminance(F) after stream RIS, is rescaled
-// to refer to it. Every pixel with p % stride == offset also writes training
-// slot p / stride (position, reconnection radiance; target -1 if unusable).
-constexpr int kRefreshWaves = 4;
-__global__ __launch_bounds__(64 * kRefreshWaves) void k_gris_learned_refresh(DevScene S, rtm::MlpView M, GB g, ResvPlanes R,
-                                                                           float* __restrict__ batchPos,
-                                                                           float* __restrict__ batchTarget,
-                                                                           uint32_t stride, uint32_t offset) {
+// Learned-target refresh after the path-trace pass. A one-wave workgroup walks
+// 64-pixel chunks; each lane reads its pixel's reservoir and writes the
+// training slot at once, while the pixels whose sample has a surface rc vertex
+// (the only ones the network serves, often ~10 %) are compacted into an LDS
+// queue; whenever 32 are queued the wave encodes them (lanes 0-31 and 32-63
+// each gathering half the grid levels of the same sample) and runs the network
+// on the matrix cores. For such a sample F_L = F * Lhat / rcLi per channel
+// gives the learned target value and the resample weight, which refers to
+// luminance(F) after stream RIS, is rescaled to refer to it; other pixels keep
+// their weight and record luminance(F). Every pixel with p % stride == offset
+// also writes training slot p / stride (position, reconnection radiance;
+// target -1 if unusable).
+struct RefreshItem { uint32_t p; float y[3], F[3], rcLi[3], rw; };
+constexpr int kRefreshQueue = 96;  // 32 in flight + up to 63 more from one chunk
+__global__ __launch_bounds__(64) void k_gris_learned_refresh(DevScene S, rtm::MlpView M, GB g, ResvPlanes R,
+                                                             float* __restrict__ batchPos,
+                                                             float* __restrict__ batchTarget, uint32_t stride,
+                                                             uint32_t offset) {
     __shared__ _Float16 sW[rtm::kWeights];
+    __shared__ RefreshItem q[kRefreshQueue];
     for (int i = threadIdx.x; i < rtm::kWeights; i += blockDim.x) sW[i] = M.w[i];
     __syncthreads();
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x;
     const uint32_t n = g.W * g.H;
-    const uint32_t tiles = (n + 31u) / 32u;
-    for (uint32_t tile = blockIdx.x * kRefreshWaves + wave; tile < tiles; tile += gridDim.x * kRefreshWaves) {
-        const uint32_t p = tile * 32u + (lane & 31u);
-        const bool own = lane < 32u && p < n;
-        bool written = false, need = false;
-        Resv r = zeroResv();
-        f3 y = s3(0.0f);
-        if (own) {
-            written = g.dn[p].x != 0.0f;
-            if (written) {
-                r = loadResv(R, p);
-                need = resvIsValid(r) && r.rcIsec.inst != RT_INVALID_HIT && flagsRcVertexType(r.flags) == RC_SURFACE;
-                if (need) {
-                    Surf sf;
-                    loadSurfaceInfo(S, r.rcIsec, sf);
-                    y = sf.pos;
-                }
-            }
-        }
-        float u, v, w;
-        rtm::normalizePos(M, y.x, y.y, y.z, u, v, w);
-        u = __shfl(u, (int)(lane & 31u));
-        v = __shfl(v, (int)(lane & 31u));
-        w = __shfl(w, (int)(lane & 31u));
+    const uint32_t chunks = (n + 63u) / 64u;
+    uint32_t queued = 0;  // wave-uniform
+    // runs the network on queue entries 0-31 (fewer when flushing) and retires them
+    auto drain = [&](uint32_t count) {
+        const uint32_t j = lane & 31u;
+        const bool has = j < count;
+        float u = 0.0f, v = 0.0f, w = 0.0f;
+        if (has) rtm::normalizePos(M, q[j].y[0], q[j].y[1], q[j].y[2], u, v, w);
         rtm::h4 enc[4];
         rtm::encodeLane(M, lane >> 5, u, v, w, enc);
         rtm::Fwd F;
         rtm::forwardTile(sW, lane, enc, F);
-        if (!own) continue;
-        if (written) {
-            f3 L = s3(0.0f);
-            const float pOld = luminance(r.F);
-            f3 FL = r.F;
-            if (need) {
-                L = mk3(rtm::outAct(F.z3[0]), rtm::outAct(F.z3[1]), rtm::outAct(F.z3[2]));
-                FL.x = r.rcLi.x > 0.0f ? r.F.x * (L.x / r.rcLi.x) : r.F.x;
-                FL.y = r.rcLi.y > 0.0f ? r.F.y * (L.y / r.rcLi.y) : r.F.y;
-                FL.z = r.rcLi.z > 0.0f ? r.F.z * (L.z / r.rcLi.z) : r.F.z;
-            }
-            const float pNew = luminance(FL);
-            float rw = r.resampleWeight;
-            if (need && pOld > 0.0f) rw = rw * (pNew / pOld);
-            R.p[4 * R.n + p].w = rw;
-            R.p[5 * R.n + p] = make_float4(L.x, L.y, L.z, need ? pNew : pOld);
+        if (lane < 32u && has) {
+            const RefreshItem& it = q[j];
+            f3 L = mk3(rtm::outAct(F.z3[0]), rtm::outAct(F.z3[1]), rtm::outAct(F.z3[2]));
+            f3 Fv = mk3(it.F[0], it.F[1], it.F[2]);
+            f3 FL;
+            FL.x = it.rcLi[0] > 0.0f ? Fv.x * (L.x / it.rcLi[0]) : Fv.x;
+            FL.y = it.rcLi[1] > 0.0f ? Fv.y * (L.y / it.rcLi[1]) : Fv.y;
+            FL.z = it.rcLi[2] > 0.0f ? Fv.z * (L.z / it.rcLi[2]) : Fv.z;
+            const float pOld = luminance(Fv), pNew = luminance(FL);
+            float rw = it.rw;
+            if (pOld > 0.0f) rw = rw * (pNew / pOld);
+            R.p[4 * R.n + it.p].w = rw;
+            R.p[5 * R.n + it.p] = make_float4(L.x, L.y, L.z, pNew);
         }
-        if (p % stride == offset) {
+        __syncthreads();
+        // move the entries behind the drained ones to the front
+        const uint32_t rest = queued - count;
+        RefreshItem keep;
+        if (lane < rest) keep = q[count + lane];
+        __syncthreads();
+        if (lane < rest) q[lane] = keep;
+        __syncthreads();
+        queued = rest;
+    };
+    for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        const uint32_t p = c * 64u + lane;
+        bool need = false;
+        Resv r = zeroResv();
+        f3 y = s3(0.0f);
+        if (p < n && g.dn[p].x != 0.0f) {
+            r = loadResv(R, p);
+            need = resvIsValid(r) && r.rcIsec.inst != RT_INVALID_HIT && flagsRcVertexType(r.flags) == RC_SURFACE;
+            if (need) {
+                Surf sf;
+                loadSurfaceInfo(S, r.rcIsec, sf);
+                y = sf.pos;
+            } else {
+                R.p[5 * R.n + p] = make_float4(0.0f, 0.0f, 0.0f, luminance(r.F));
+            }
+        }
+        if (p < n && p % stride == offset) {
             const uint32_t slot = p / stride;
             batchPos[3 * (size_t)slot] = y.x;
             batchPos[3 * (size_t)slot + 1] = y.y;
@@ -1050,7 +1094,20 @@ __global__ __launch_bounds__(64 * kRefreshWaves) void k_gris_learned_refresh(Dev
             batchTarget[3 * (size_t)slot + 1] = need ? r.rcLi.y : 0.0f;
             batchTarget[3 * (size_t)slot + 2] = need ? r.rcLi.z : 0.0f;
         }
+        const unsigned long long mask = __ballot(need);
+        if (need) {
+            RefreshItem& it = q[queued + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))];
+            it.p = p;
+            it.y[0] = y.x; it.y[1] = y.y; it.y[2] = y.z;
+            it.F[0] = r.F.x; it.F[1] = r.F.y; it.F[2] = r.F.z;
+            it.rcLi[0] = r.rcLi.x; it.rcLi[1] = r.rcLi.y; it.rcLi[2] = r.rcLi.z;
+            it.rw = r.resampleWeight;
+        }
+        queued += (uint32_t)__popcll(mask);
+        __syncthreads();
+        while (queued >= 32u) drain(32u);
     }
+    if (queued > 0u) drain(queued);
 }
 
 inline dim3 tileGrid(uint32_t W, uint32_t H) { return dim3((W + kTileX - 1) / kTileX, (H + kTileY - 1) / kTileY, 1); }
@@ -1165,9 +1222,9 @@ hipError_t launchLearnedRefresh(const rt_device_scene* s, rt_mlp* m, const rt_ca
     const uint32_t offset = (cams[0].frameIndex & 0x7fffffffu) % stride;
     m->batchSize = offset < n ? (n - offset + stride - 1) / stride : 0u;
     GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
-    const uint32_t tiles = (n + 31u) / 32u;
-    const uint32_t blocks = std::min<uint32_t>((tiles + kRefreshWaves - 1) / kRefreshWaves, 4096u);
-    (void)launch(k_gris_learned_refresh, dim3(blocks), dim3(64 * kRefreshWaves), stream, s->dev, m->view, g,
+    const uint32_t chunks = (n + 63u) / 64u;
+    const uint32_t blocks = std::min<uint32_t>(chunks, 2048u);
+    (void)launch(k_gris_learned_refresh, dim3(blocks), dim3(64), stream, s->dev, m->view, g,
                  planes(f.grisReservoir, f.width, f.height), m->batchPos, m->batchTarget, stride, offset);
     return hipGetLastError();
 }
