@@ -27,6 +27,10 @@ typedef struct rt_renderer rt_renderer;
  *      instanced icospheres, MetallicWorkflow GGX, 16 emissive triangles of
  *      log-uniform radiance in [1, 100] (~1.1M triangles)
  *   2  a small test scene: one floor quad under one quad light
+ *   3  as 1 with a 200x200-cell heightfield (~225k triangles)
+ *   4  as 1 with an 860x860-cell heightfield (~3.0M triangles)
+ *   5  1024x1024-cell heightfield + 2000 instanced icospheres, 64 emissive
+ *      triangles, GGX roughness in [0.02, 1] (~4.8M triangles)
  * Everything is drawn from PCG32 streams seeded by `seed`. */
 rt_status rt_scenegen(uint32_t config, uint32_t seed, rt_host_scene** out);
 rt_status rt_host_scene_destroy(rt_host_scene* scene);

@@ -40,6 +40,9 @@ def test_struct_sizes_match_reference_layouts():
     assert C.sizeof(rt.Material) == 32 and C.sizeof(rt.MeshVertex) == 32
     assert C.sizeof(rt.GrisSettings) == 36     # 20-B push constant + 3 kernel constants + target function
     assert C.sizeof(rt.MlpConfig) == 72
+    # rt_view: scene pointer + 2 cameras + rt_frame (88 B) + settings (36 B), padded to 8
+    assert C.sizeof(rt.Frame) == 88 and C.sizeof(rt.View) == 8 + 2 * 352 + 88 + 36 + 4
+    assert C.sizeof(rt.BatchTuning) == 8
     # five 16-B planes of reference fields + the learned-target plane = the reference's 96-B reservoir
     assert rt.core().rt_gris_reservoir_bytes(1920, 1080) == 1920 * 1080 * 96
 

@@ -14,6 +14,8 @@
 // citations at each function) under the FP contract of rt_dmath.h.
 #include "rt_internal.h"
 #include <hip/hip_fp16.h>
+#include <algorithm>
+#include <cstring>
 
 using namespace rtd;
 
@@ -192,14 +194,10 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_TRACE) void k_trace_rays(Dev
 // Light triangles are not drawn by the reference's raster pass; the clip
 // planes act as ray distances [near, far].
 template <bool kCount>
-__global__ __launch_bounds__(kTileThreads, RT_WAVES_GBUFFER) void k_gbuffer(DevScene S, rt_camera cam, float4* __restrict__ dnOut,
-                                                          uint2* __restrict__ amOut, uint32_t* __restrict__ mvOut,
-                                                          TraceCounters* ctr) {
-    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
-    PIXEL_INDEX
+RT_DEV void gbufferPixel(const DevScene& S, const rt_camera& cam, float4* __restrict__ dnOut, uint2* __restrict__ amOut,
+                         uint32_t* __restrict__ mvOut, uint32_t x, uint32_t y, TStack st, TraceCounters* ctr) {
     const uint32_t W = cam.filmSize[0], H = cam.filmSize[1];
     if (x >= W || y >= H) return;
-    TStack st{stk + threadIdx.x, kTileThreads};
     size_t p = (size_t)y * W + x;
     f2 uv = pixelUV(x, y, W, H);
     CamRay ray = pinholeRay(cam, mk2(uv.x, 1.0f - uv.y));
@@ -239,6 +237,14 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_GBUFFER) void k_gbuffer(DevS
     uint32_t mx = __half_as_ushort(__float2half_rn(lastC.x - uv.x));
     uint32_t my = __half_as_ushort(__float2half_rn(lastC.y - uv.y));
     mvOut[p] = mx | (my << 16);
+}
+template <bool kCount>
+__global__ __launch_bounds__(kTileThreads, RT_WAVES_GBUFFER) void k_gbuffer(DevScene S, rt_camera cam, float4* __restrict__ dnOut,
+                                                          uint2* __restrict__ amOut, uint32_t* __restrict__ mvOut,
+                                                          TraceCounters* ctr) {
+    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    PIXEL_INDEX
+    gbufferPixel<kCount>(S, cam, dnOut, amOut, mvOut, x, y, TStack{stk + threadIdx.x, kTileThreads}, ctr);
 }
 
 // ================================================================================================
@@ -695,14 +701,12 @@ RT_DEV Resv ptFinish(PTPath& P, const float* slot) {
 // Pixels are handed out in 8x8-tile order, so the lanes refilled together by
 // one wave take neighbouring pixels. Every lane leaves the loop once the queue
 // is drained; the last wave to finish re-arms the queue for the next launch.
+// Drains one view's pixel queue (the body of the persistent kernels): returns
+// when every lane of the wave is out of work.
 template <bool kCount>
-__global__ __launch_bounds__(kTileThreads, RT_WAVES_PATH_TRACE) void k_gris_path_trace(
-    DevScene S, rt_camera cam, GB g, rt_gris_settings set, ResvPlanes outR, WorkQueue* q, TraceCounters* ctr) {
-    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
-    __shared__ float picks[kStreamWords * kTileThreads];
-    TStack st{stk + threadIdx.x, kTileThreads};
-    float* slot = picks + threadIdx.x;
-    const uint32_t lane = threadIdx.x;
+RT_DEV void ptRun(const DevScene& S, const rt_camera& cam, const GB& g, const rt_gris_settings& set, const ResvPlanes& outR,
+                  WorkQueue* q, TStack st, float* slot, TraceCounters* ctr) {
+    const uint32_t lane = threadIdx.x & 63u;
     const uint32_t tilesX = (g.W + kTileX - 1) / kTileX;
     const uint32_t total = tilesX * ((g.H + kTileY - 1) / kTileY) * kTileThreads;
     PTPath P;
@@ -743,13 +747,26 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_PATH_TRACE) void k_gris_path
             if (!live) storeResv(outR, pix, ptFinish(P, slot));
         }
     }
-    if (lane == 0) {
+}
+
+// Counts this workgroup as finished with queue q; the last one re-arms it.
+RT_DEV void queueDone(WorkQueue* q) {
+    if ((threadIdx.x & 63u) == 0u) {
         __threadfence();
         if (atomicAdd(&q->done, 1u) == gridDim.x - 1u) {
             atomicExch(&q->next, 0u);
             atomicExch(&q->done, 0u);
         }
     }
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(kTileThreads, RT_WAVES_PATH_TRACE) void k_gris_path_trace(
+    DevScene S, rt_camera cam, GB g, rt_gris_settings set, ResvPlanes outR, WorkQueue* q, TraceCounters* ctr) {
+    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    __shared__ float picks[kStreamWords * kTileThreads];
+    ptRun<kCount>(S, cam, g, set, outR, q, TStack{stk + threadIdx.x, kTileThreads}, picks + threadIdx.x, ctr);
+    queueDone(q);
 }
 
 // ---- gris_retrace.glsl:42-136 --------------------------------------------------------------------
@@ -888,13 +905,10 @@ RT_DEV Resv resvAt(const ResvPlanes& R, uint32_t W, uint32_t H, int x, int y) {
 
 // gris_resample_temporal.glsl:11-83
 template <bool kCount, bool kLearned>
-__global__ __launch_bounds__(kTileThreads, RT_WAVES_TEMPORAL) void k_gris_temporal(DevScene S, rt_camera cam, rt_camera prevCam, GB g,
-                                                                GB gPrev, rt_gris_settings set, ResvPlanes thisR,
-                                                                ResvPlanes prevR, ResvPlanes tempR, TraceCounters* ctr) {
-    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
-    PIXEL_INDEX
+RT_DEV void temporalPixel(const DevScene& S, const rt_camera& cam, const rt_camera& prevCam, const GB& g, const GB& gPrev,
+                          const rt_gris_settings& set, const ResvPlanes& thisR, const ResvPlanes& prevR,
+                          const ResvPlanes& tempR, uint32_t x, uint32_t y, TStack st, TraceCounters* ctr) {
     if (x >= g.W || y >= g.H) return;
-    TStack st{stk + threadIdx.x, kTileThreads};
     f2 uv = pixelUV(x, y, g.W, g.H);
     GPix gp;
     if (!unpackGB(texDNCentre(g, x, y), fetchAM(g, (int)x, (int)y), gp)) return;  // Temp not written
@@ -935,16 +949,22 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_TEMPORAL) void k_gris_tempor
     if (!resvIsValid(resv)) resvReset(resv);
     storeResv<kLearned>(tempR, p, resv);
 }
+template <bool kCount, bool kLearned>
+__global__ __launch_bounds__(kTileThreads, RT_WAVES_TEMPORAL) void k_gris_temporal(DevScene S, rt_camera cam, rt_camera prevCam, GB g,
+                                                                GB gPrev, rt_gris_settings set, ResvPlanes thisR,
+                                                                ResvPlanes prevR, ResvPlanes tempR, TraceCounters* ctr) {
+    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    PIXEL_INDEX
+    temporalPixel<kCount, kLearned>(S, cam, prevCam, g, gPrev, set, thisR, prevR, tempR, x, y,
+                                    TStack{stk + threadIdx.x, kTileThreads}, ctr);
+}
 
 // gris_resample_spatial.glsl:11-135 and gris_resample_spatial.comp accumulation
 template <bool kCount, bool kLearned>
-__global__ __launch_bounds__(kTileThreads, RT_WAVES_SPATIAL) void k_gris_spatial(DevScene S, rt_camera cam, GB g, rt_gris_settings set,
-                                                               ResvPlanes tempR, ResvPlanes thisR, float* indirectOut,
-                                                               TraceCounters* ctr) {
-    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
-    PIXEL_INDEX
+RT_DEV void spatialPixel(const DevScene& S, const rt_camera& cam, const GB& g, const rt_gris_settings& set,
+                         const ResvPlanes& tempR, const ResvPlanes& thisR, float* indirectOut, uint32_t x, uint32_t y,
+                         TStack st, TraceCounters* ctr) {
     if (x >= g.W || y >= g.H) return;
-    TStack st{stk + threadIdx.x, kTileThreads};
     size_t p = (size_t)y * g.W + x;
     f2 uv = pixelUV(x, y, g.W, g.H);
     GPix gp;
@@ -999,6 +1019,116 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_SPATIAL) void k_gris_spatial
         radiance = clampColor(radiance);
     }
     accumulate(indirectOut, p, radiance, cam.frameIndex);
+}
+template <bool kCount, bool kLearned>
+__global__ __launch_bounds__(kTileThreads, RT_WAVES_SPATIAL) void k_gris_spatial(DevScene S, rt_camera cam, GB g, rt_gris_settings set,
+                                                               ResvPlanes tempR, ResvPlanes thisR, float* indirectOut,
+                                                               TraceCounters* ctr) {
+    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    PIXEL_INDEX
+    spatialPixel<kCount, kLearned>(S, cam, g, set, tempR, thisR, indirectOut, x, y, TStack{stk + threadIdx.x, kTileThreads},
+                                   ctr);
+}
+
+// ---- Batched views ---------------------------------------------------------------------------
+// Everything one view's passes read, in one descriptor; the batch kernels take
+// a device array of them. The view of a workgroup is made wave-uniform, so its
+// descriptor fields are read with scalar loads.
+struct ViewDesc {
+    DevScene S;
+    rt_camera cam, prevCam;
+    GB g, gPrev;
+    ResvPlanes cur, prev, temp;
+    float* indirectOut;
+    float4* dnOut;
+    uint2* amOut;
+    uint32_t* mvOut;
+    rt_gris_settings set;
+    uint32_t tileBase, tilesX, tiles, pad;
+    WorkQueue* queue;
+};
+
+// View of a linear tile index: the last view whose tileBase <= tile.
+RT_DEV uint32_t viewOfTile(const ViewDesc* __restrict__ V, uint32_t numViews, uint32_t tile) {
+    uint32_t lo = 0, hi = numViews;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (V[mid].tileBase <= tile) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+// Workgroup -> tile. Consecutive workgroups are dealt round robin to the 8
+// XCDs; with the swizzle, XCD k walks the k-th contiguous eighth of the tile
+// list, so the rays of neighbouring tiles share that XCD's L2.
+RT_DEV uint32_t batchTile(uint32_t b, uint32_t totalTiles, uint32_t swz) {
+    if (!swz) return b;
+    const uint32_t per = (totalTiles + 7u) / 8u;
+    return (b % 8u) * per + b / 8u;
+}
+RT_DEV bool batchPixel(const ViewDesc* __restrict__ V, uint32_t numViews, uint32_t totalTiles, uint32_t swz,
+                       uint32_t& v, uint32_t& x, uint32_t& y) {
+    const uint32_t tile = batchTile(blockIdx.x, totalTiles, swz);
+    if (tile >= totalTiles) return false;
+    v = __builtin_amdgcn_readfirstlane(viewOfTile(V, numViews, tile));
+    const uint32_t t = tile - V[v].tileBase, tx = V[v].tilesX;
+    x = (t % tx) * kTileX + (threadIdx.x & (kTileX - 1));
+    y = (t / tx) * kTileY + threadIdx.x / kTileX;
+    return true;
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(kTileThreads, RT_WAVES_GBUFFER) void k_gbuffer_batch(const ViewDesc* __restrict__ V,
+                                                                                  uint32_t numViews, uint32_t totalTiles,
+                                                                                  uint32_t swz, TraceCounters* ctr) {
+    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    uint32_t v, x, y;
+    if (!batchPixel(V, numViews, totalTiles, swz, v, x, y)) return;
+    const ViewDesc& D = V[v];
+    gbufferPixel<kCount>(D.S, D.cam, D.dnOut, D.amOut, D.mvOut, x, y, TStack{stk + threadIdx.x, kTileThreads}, ctr);
+}
+
+// All views' GRIS path tracing in one persistent launch: workgroup b starts at
+// view b % numViews and drains the views' pixel queues in turn, so waves move
+// to the views that still have pixels.
+template <bool kCount>
+__global__ __launch_bounds__(kTileThreads, RT_WAVES_PATH_TRACE) void k_gris_path_trace_batch(const ViewDesc* __restrict__ V,
+                                                                                            uint32_t numViews,
+                                                                                            TraceCounters* ctr) {
+    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    __shared__ float picks[kStreamWords * kTileThreads];
+    for (uint32_t i = 0; i < numViews; i++) {
+        const uint32_t v = __builtin_amdgcn_readfirstlane((blockIdx.x + i) % numViews);
+        const ViewDesc& D = V[v];
+        ptRun<kCount>(D.S, D.cam, D.g, D.set, D.cur, D.queue, TStack{stk + threadIdx.x, kTileThreads},
+                      picks + threadIdx.x, ctr);
+        queueDone(D.queue);
+    }
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(kTileThreads, RT_WAVES_TEMPORAL) void k_gris_temporal_batch(const ViewDesc* __restrict__ V,
+                                                                                        uint32_t numViews,
+                                                                                        uint32_t totalTiles, uint32_t swz,
+                                                                                        TraceCounters* ctr) {
+    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    uint32_t v, x, y;
+    if (!batchPixel(V, numViews, totalTiles, swz, v, x, y)) return;
+    const ViewDesc& D = V[v];
+    temporalPixel<kCount, false>(D.S, D.cam, D.prevCam, D.g, D.gPrev, D.set, D.cur, D.prev, D.temp, x, y,
+                                 TStack{stk + threadIdx.x, kTileThreads}, ctr);
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(kTileThreads, RT_WAVES_SPATIAL) void k_gris_spatial_batch(const ViewDesc* __restrict__ V,
+                                                                                      uint32_t numViews, uint32_t totalTiles,
+                                                                                      uint32_t swz, TraceCounters* ctr) {
+    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    uint32_t v, x, y;
+    if (!batchPixel(V, numViews, totalTiles, swz, v, x, y)) return;
+    const ViewDesc& D = V[v];
+    spatialPixel<kCount, false>(D.S, D.cam, D.g, D.set, D.temp, D.cur, D.indirectOut, x, y,
+                                TStack{stk + threadIdx.x, kTileThreads}, ctr);
 }
 
 __global__ void k_resv_reset(ResvPlanes R) {
@@ -1226,6 +1356,65 @@ hipError_t launchLearnedRefresh(const rt_device_scene* s, rt_mlp* m, const rt_ca
     const uint32_t blocks = std::min<uint32_t>(chunks, 2048u);
     (void)launch(k_gris_learned_refresh, dim3(blocks), dim3(64), stream, s->dev, m->view, g,
                  planes(f.grisReservoir, f.width, f.height), m->batchPos, m->batchTarget, stride, offset);
+    return hipGetLastError();
+}
+
+// ---- batched views ----
+size_t batchDescBytes() { return sizeof(ViewDesc); }
+
+uint32_t batchViewTiles(const rt_frame& f) {
+    return ((f.width + kTileX - 1) / kTileX) * ((f.height + kTileY - 1) / kTileY);
+}
+
+// Writes the descriptor of one view into host memory at dst.
+void batchFillDesc(void* dst, const rt_device_scene* scene, const rt_camera* cams, const rt_frame& f,
+                   const rt_gris_settings& set, uint32_t tileBase, WorkQueue* queue) {
+    ViewDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.S = scene->dev;
+    d.cam = cams[0];
+    d.prevCam = cams[1];
+    d.g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
+    d.gPrev = makeGB(f.depthNormalPrev, f.albedoMatIdPrev, nullptr, f.width, f.height);
+    d.cur = planes(f.grisReservoir, f.width, f.height);
+    d.prev = planes(f.grisReservoirPrev, f.width, f.height);
+    d.temp = planes(f.grisReservoirTemp, f.width, f.height);
+    d.indirectOut = f.indirectOutput;
+    d.dnOut = reinterpret_cast<float4*>(const_cast<float*>(f.depthNormal));
+    d.amOut = reinterpret_cast<uint2*>(const_cast<uint32_t*>(f.albedoMatId));
+    d.mvOut = reinterpret_cast<uint32_t*>(const_cast<uint16_t*>(f.motionVector));
+    d.set = set;
+    d.tileBase = tileBase;
+    d.tilesX = (f.width + kTileX - 1) / kTileX;
+    d.tiles = batchViewTiles(f);
+    d.queue = queue;
+    std::memcpy(dst, &d, sizeof(d));
+}
+
+// Resident workgroups of the batched persistent path trace per CU, and the CU count.
+hipError_t batchOccupancy(uint32_t& perCU, uint32_t& cus) {
+    int dev = 0, c = 0, o = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, reinterpret_cast<const void*>(k_gris_path_trace_batch<false>),
+                                                         kTileThreads, 0);
+    perCU = (uint32_t)std::max(1, o);
+    cus = (uint32_t)std::max(1, c);
+    return e;
+}
+
+// One frame of all views: G-buffer, path trace (persistent, across views),
+// temporal, spatial. descs = device array of numViews descriptors.
+hipError_t launchBatchFrame(const void* descs, uint32_t numViews, uint32_t totalTiles, uint32_t persistentGroups,
+                            uint32_t xcdSwizzle, hipStream_t stream) {
+    const ViewDesc* V = reinterpret_cast<const ViewDesc*>(descs);
+    const uint32_t swz = xcdSwizzle ? 1u : 0u;
+    const dim3 tileGridDim(swz ? 8u * ((totalTiles + 7u) / 8u) : totalTiles);
+    RT_DISPATCH(k_gbuffer_batch, tileGridDim, kTileThreads, stream, V, numViews, totalTiles, swz);
+    RT_DISPATCH(k_gris_path_trace_batch, dim3(std::max(1u, persistentGroups)), kTileThreads, stream, V, numViews);
+    RT_DISPATCH(k_gris_temporal_batch, tileGridDim, kTileThreads, stream, V, numViews, totalTiles, swz);
+    RT_DISPATCH(k_gris_spatial_batch, tileGridDim, kTileThreads, stream, V, numViews, totalTiles, swz);
     return hipGetLastError();
 }
 

@@ -203,10 +203,13 @@ std::unique_ptr<Scene> boxRoom(uint32_t seed) {
     return S;
 }
 
-std::unique_ptr<Scene> terrain(uint32_t seed) {
+// fBm heightfield of N x N cells (4 triangles per cell) over 24 units, numSpheres
+// instanced icospheres (8 unique models) and numLights emissive triangles;
+// GGX roughness uniform in [roughMin, 1], metallic with p = 0.3.
+std::unique_ptr<Scene> terrain(uint32_t seed, int N = 512, int numSpheres = 200, int numLights = 16,
+                               float roughMin = 0.05f) {
     auto S = std::make_unique<Scene>();
     Pcg32 rng(seed, 1);
-    const int N = 512;
     const float extent = 24.0f, cell = extent / N, amp = 2.4f;
     Noise noise{seed * 7919u + 13u};
     auto height = [&](float x, float y) { return amp * noise.fbm(x * 0.125f + 100.0f, y * 0.125f + 100.0f, 6); };
@@ -218,12 +221,12 @@ std::unique_ptr<Scene> terrain(uint32_t seed) {
         float l = std::sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
         return Vec3{n.x / l, n.y / l, n.z / l};
     };
-    // 64 patch materials (8 x 8 blocks of 64 x 64 cells)
+    // 64 patch materials (8 x 8 blocks of N/8 x N/8 cells)
     int32_t matBase = (int32_t)S->materials.size();
     for (int i = 0; i < 64; i++) {
         float metallic = rng.uniform() < 0.3f ? 1.0f : 0.0f;
         S->materials.push_back(ggx(rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f),
-                                   rng.uniform(0.05f, 1.0f), metallic));
+                                   rng.uniform(roughMin, 1.0f), metallic));
     }
     std::vector<rt_mesh_vertex> v;
     v.reserve((size_t)(N + 1) * (N + 1) + (size_t)N * N);
@@ -249,7 +252,7 @@ std::unique_ptr<Scene> terrain(uint32_t seed) {
             uint32_t m = centreBase + j * N + i;
             uint32_t q[12] = {a, b, m, b, c, m, c, d, m, d, a, m};
             tris.insert(tris.end(), q, q + 12);
-            int32_t mat = matBase + (j / 64) * 8 + (i / 64);
+            int32_t mat = matBase + (j * 8 / N) * 8 + (i * 8 / N);
             for (int t = 0; t < 4; t++) triMat.push_back(mat);
         }
     uint32_t ioff = addModel(*S, v, tris, triMat);
@@ -267,18 +270,18 @@ std::unique_ptr<Scene> terrain(uint32_t seed) {
         int32_t mat = (int32_t)S->materials.size();
         float metallic = rng.uniform() < 0.3f ? 1.0f : 0.0f;
         S->materials.push_back(ggx(rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f),
-                                   rng.uniform(0.05f, 1.0f), metallic));
+                                   rng.uniform(roughMin, 1.0f), metallic));
         modelOff[k] = addModelSharedVerts(*S, vbase, ico.tri, mat);
     }
-    for (int i = 0; i < 200; i++) {
+    for (int i = 0; i < numSpheres; i++) {
         float r = rng.uniform(0.12f, 0.6f);
         float x = rng.uniform(-10.8f, 10.8f), y = rng.uniform(-10.8f, 10.8f);
         float z = height(x, y) + r * rng.uniform(0.2f, 1.0f);
         float m[16]; translateScale(m, {x, y, z}, r);
         addInstance(*S, modelOff[i % 8], (uint32_t)ico.tri.size(), m);
     }
-    // 16 emissive triangles above the terrain, facing down, radiance log-uniform in [1, 100]
-    for (int i = 0; i < 16; i++) {
+    // emissive triangles above the terrain, facing down, radiance log-uniform in [1, 100]
+    for (int i = 0; i < numLights; i++) {
         float cx = rng.uniform(-9.6f, 9.6f), cy = rng.uniform(-9.6f, 9.6f);
         float cz = height(cx, cy) + rng.uniform(2.0f, 5.0f);
         float s = rng.uniform(0.25f, 0.75f), ang = rng.uniform(0.0f, 6.2831853f);
@@ -332,6 +335,9 @@ std::unique_ptr<Scene> generateScene(uint32_t config, uint32_t seed) {
     case 0: return boxRoom(seed);
     case 1: return terrain(seed);
     case 2: return floorUnderLight(seed);
+    case 3: return terrain(seed, 200);                        // ~225k triangles
+    case 4: return terrain(seed, 860);                        // ~3.0M triangles
+    case 5: return terrain(seed, 1024, 2000, 64, 0.02f);      // ~4.8M triangles, 64 lights
     default: return nullptr;
     }
 }

@@ -104,6 +104,14 @@ class Frame(C.Structure):
                 ("grisReservoir", C.c_void_p), ("grisReservoirPrev", C.c_void_p), ("grisReservoirTemp", C.c_void_p)]
 
 
+class View(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("cams", Camera * 2), ("frame", Frame), ("settings", GrisSettings)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("persistentGroups", C.c_uint32), ("xcdSwizzle", C.c_uint32)]
+
+
 class SceneInfo(C.Structure):
     _fields_ = [("numPrimitives", C.c_uint32), ("numLights", C.c_uint32), ("bvhNodes", C.c_uint32),
                 ("bvhDepth", C.c_uint32), ("sceneLo", C.c_float * 3), ("sceneHi", C.c_float * 3),
@@ -137,7 +145,8 @@ CORE_SYMBOLS = [
     "rt_counters_read", "rt_mlp_create", "rt_mlp_destroy", "rt_mlp_param_counts", "rt_mlp_get_params",
     "rt_mlp_set_params", "rt_mlp_infer", "rt_mlp_gradients", "rt_mlp_read_gradients", "rt_mlp_adam_step",
     "rt_mlp_train_step", "rt_mlp_train_frame", "rt_mlp_batch_download", "rt_gris_learned_refresh",
-    "rt_frame_graph_create", "rt_frame_graph_launch", "rt_frame_graph_destroy",
+    "rt_frame_graph_create", "rt_frame_graph_launch", "rt_frame_graph_destroy", "rt_batch_create",
+    "rt_batch_destroy", "rt_batch_frame", "rt_batch_set_tuning", "rt_batch_get_tuning", "rt_batch_autotune",
 ]
 HOST_SYMBOLS = [
     "rt_scenegen", "rt_host_scene_destroy", "rt_host_scene_arrays", "rt_host_scene_camera", "rt_light_table_build",
@@ -189,6 +198,12 @@ def _declare_core(L):
         "rt_frame_graph_create": [P, C.POINTER(Frame), C.POINTER(GrisSettings), P, C.POINTER(P)],
         "rt_frame_graph_launch": [P, u32, C.POINTER(Camera), P],
         "rt_frame_graph_destroy": [P],
+        "rt_batch_create": [u32, C.POINTER(P)],
+        "rt_batch_destroy": [P],
+        "rt_batch_frame": [P, C.POINTER(View), u32, P],
+        "rt_batch_set_tuning": [P, C.POINTER(BatchTuning)],
+        "rt_batch_get_tuning": [P, C.POINTER(BatchTuning)],
+        "rt_batch_autotune": [P, C.POINTER(View), u32, u32, P, C.POINTER(BatchTuning), C.POINTER(C.c_float)],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
