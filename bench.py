@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--no-learned", action="store_true", help="skip the equal-time relMSE comparison")
     ap.add_argument("--learned-kernel-loop", action="store_true",
                     help="time learned-target frames only (for profilers)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-view leg")
+    ap.add_argument("--batched-views", type=int, default=64)
     ap.add_argument("--lvp-frames", type=int, default=64, help="frames of the plain form in the equal-time run")
     ap.add_argument("--ref-frames", type=int, default=4096, help="frames of the converged reference image")
     return ap.parse_args()
@@ -350,6 +352,89 @@ def learned_kernel_loop(rt, hs, ds, args):
             "mpaths_per_s": round(W * H * args.steps / wall / 1e6, 3)}
 
 
+def batched_views(rt, args, hs1, ds1):
+    """Batched plain ReSTIR PT: args.batched_views views over four scenes (~20k, ~225k, ~1.1M and ~3.0M
+    triangles), square resolutions log-uniform in [128, 1024] (multiples of 8), max depth uniform in
+    {3..8}, each view on its own frame of its scene's camera walk; one rt_batch_frame per frame after
+    the autotuner has picked the persistent grid and tile order. Also times the same frames launched
+    view by view."""
+    import numpy as np
+    import torch
+
+    g = np.random.default_rng(2)
+    hosts = [rt.HostScene(0, 2), rt.HostScene(3, 2), hs1, rt.HostScene(4, 2)]
+    devs = [rt.DeviceScene(hosts[0].arrays), rt.DeviceScene(hosts[1].arrays), ds1, rt.DeviceScene(hosts[3].arrays)]
+    tris = [int(d.info().numPrimitives) for d in devs]
+    specs = []
+    for i in range(args.batched_views):
+        size = int(np.exp(g.uniform(np.log(128), np.log(1024)))) // 8 * 8
+        specs.append((i % 4, size, int(g.integers(3, 9)), int(g.integers(0, 64))))
+    frames = [TorchFrames(rt, devs[si], n, n) for si, n, _, _ in specs]
+    sets = [rt.settings(shift=2, temporal=True, spatial=True, cap=20, max_depth=d, neighbors=3, radius=20.0)
+            for _, _, d, _ in specs]
+    pixels = sum(n * n for _, n, _, _ in specs)
+    L = rt.core()
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def views_for(k):
+        vs = (rt.View * len(specs))()
+        for i, (si, n, _, f0) in enumerate(specs):
+            cams = hosts[si].camera((f0 + k) % 64, n, n)
+            v = vs[i]
+            v.scene = devs[si].handle.value
+            C.memmove(C.addressof(v.cams), C.addressof(cams), C.sizeof(rt.Camera) * 2)
+            v.frame = frames[i].frame_struct()
+            v.settings = sets[i]
+        return vs
+
+    def flip():
+        for fr in frames:
+            fr.cur ^= 1
+
+    b = C.c_void_p()
+    rt.check(L.rt_batch_create(len(specs), C.byref(b)), "batch")
+    try:
+        best, tune_ms = rt.BatchTuning(), C.c_float()
+        rt.check(L.rt_batch_autotune(b, views_for(0), len(specs), 2, stream, C.byref(best), C.byref(tune_ms)), "tune")
+        for fr in frames:
+            fr.reset()
+        for k in range(2):
+            rt.check(L.rt_batch_frame(b, views_for(k), len(specs), stream), "frame")
+            flip()
+        torch.cuda.synchronize()
+        steps = 8
+        t0 = time.perf_counter()
+        for k in range(steps):
+            rt.check(L.rt_batch_frame(b, views_for(2 + k), len(specs), stream), "frame")
+            flip()
+        torch.cuda.synchronize()
+        batched_ms = (time.perf_counter() - t0) * 1e3 / steps
+        # the same frames launched view by view (4 launches per view)
+        t0 = time.perf_counter()
+        for k in range(2):
+            vs = views_for(10 + k)
+            for i in range(len(specs)):
+                v = vs[i]
+                cams = v.cams
+                rt.check(L.rt_gbuffer_trace(v.scene, C.byref(cams[0]), v.frame.depthNormal, v.frame.albedoMatId,
+                                            v.frame.motionVector, stream), "gbuffer")
+                for fn in (L.rt_gris_path_trace, L.rt_gris_temporal, L.rt_gris_spatial):
+                    rt.check(fn(v.scene, cams, C.byref(v.frame), C.byref(v.settings), stream), "pass")
+            flip()
+        torch.cuda.synchronize()
+        single_ms = (time.perf_counter() - t0) * 1e3 / 2
+    finally:
+        L.rt_batch_destroy(b)
+    return {"views": len(specs), "pixels_per_frame": pixels, "scene_triangles": tris,
+            "ms_per_frame": round(batched_ms, 3), "mpaths_per_s": round(pixels / batched_ms / 1e3, 2),
+            "view_by_view_ms_per_frame": round(single_ms, 3),
+            "tuning": {"persistent_groups": int(best.persistentGroups), "xcd_swizzle": int(best.xcdSwizzle),
+                       "tuned_ms": round(float(tune_ms.value), 3)},
+            "workload": "batched plain ReSTIR PT (hybrid shift, temporal + spatial, 3 neighbours in 20 px, "
+                        "M-cap 20), square views log-uniform in [128, 1024] px, depth U{3..8}, scenes "
+                        "round robin over 4 procedural scenes"}
+
+
 def read_traffic(kernel):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_1080p_terrain_traffic.md")
@@ -429,6 +514,8 @@ def main():
                             "launch_ms": round(pass_ms[dom], 3)}
         line["work_per_frame"] = {p: {"rays": int(work[p][2]) // args.steps, "nodes": int(work[p][0]) // args.steps,
                                       "tris": int(work[p][1]) // args.steps} for p in PASSES}
+    if not args.no_batched:
+        line["batched"] = batched_views(rt, args, hs, ds)
     if not args.no_learned:
         line["learned_vs_plain"] = learned_vs_plain(rt, hs, ds, args)
     if not args.no_cpu_baseline:

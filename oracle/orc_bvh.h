@@ -288,6 +288,14 @@ struct RayIn { vec3 o, d, invd; float tmin, tmax; };
 static inline RayIn makeRayIn(vec3 o, vec3 d, float tmin, float tmax) {
     return {o, d, V3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z), tmin, tmax};
 }
+// A ray with a non-finite origin or direction hits nothing (no triangle test
+// can pass with NaN coordinates); it is rejected before traversal, because
+// the slab test below would accept every box for it (minNum/maxNum drop the
+// NaN operands) and walk the whole tree.
+static inline bool rayFinite(const RayIn& r) {
+    return std::isfinite(r.o.x) && std::isfinite(r.o.y) && std::isfinite(r.o.z) && std::isfinite(r.d.x) &&
+           std::isfinite(r.d.y) && std::isfinite(r.d.z);
+}
 // Slab test; IEEE minNum/maxNum drop NaN operands (0 * inf).
 static inline bool boxHit(const RayIn& r, const float* lo, const float* hi, float tBest, float& tEnter) {
     float tx0 = (lo[0] - r.o.x) * r.invd.x, tx1 = (hi[0] - r.o.x) * r.invd.x;
@@ -347,7 +355,7 @@ static inline Hit traceClosest(const BVH& B, const RayIn& r, bool skipLights, Tr
     Hit h; h.hit = false; h.t = r.tmax;
     h.isec.instanceIdx = InvalidHitIndex; h.isec.triangleIdx = 0; h.isec.bary[0] = h.isec.bary[1] = 0.0f;
     if (st) st->rays++;
-    if (B.nodes.empty()) return h;
+    if (B.nodes.empty() || !rayFinite(r)) return h;
     uint32_t stack[128]; int sp = 0;
     uint32_t node = 0;
     float tBest = r.tmax;
@@ -391,7 +399,7 @@ static inline Hit traceClosest(const BVH& B, const RayIn& r, bool skipLights, Tr
 
 static inline bool traceAny(const BVH& B, const RayIn& r, TraceStats* st) {
     if (st) st->rays++;
-    if (B.nodes.empty()) return false;
+    if (B.nodes.empty() || !rayFinite(r)) return false;
     uint32_t stack[128]; int sp = 0;
     uint32_t node = 0;
     const float tmax = r.tmax;
@@ -433,7 +441,7 @@ static inline bool traceAny(const BVH& B, const RayIn& r, TraceStats* st) {
 // farthest first.
 template <bool kAny>
 static inline bool traverse4(const BVH& B, const RayIn& r, bool skipLights, float& tBest, Hit& h, TraceStats* st) {
-    if (B.nodes4.empty()) return false;
+    if (B.nodes4.empty() || !rayFinite(r)) return false;
     uint32_t stack[512]; int sp = 0;
     uint32_t node = 0;
     bool found = false;

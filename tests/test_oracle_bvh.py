@@ -209,3 +209,25 @@ def test_watertight_shared_edges():
     rays[:, :3], rays[:, 3:6], rays[:, 6], rays[:, 7] = o, d, 0.0, 1e7
     t, bary, inst, prim, _ = osc.trace(rays, 0, skip_lights=True)
     assert (inst != 0xFFFFFFFF).all(), f"{int((inst == 0xFFFFFFFF).sum())} rays slipped through shared edges"
+
+
+def test_nonfinite_rays_miss_without_traversal(boxroom):
+    """A ray with a NaN or infinite origin or direction hits nothing and visits no node (the
+    slab test would otherwise accept every box for it and walk the whole tree)."""
+    hs, osc = boxroom
+    rays = np.zeros((6, 8), np.float32)
+    rays[:, :3] = [0.1, 0.2, 0.5]
+    rays[:, 3:6] = [0.0, 0.0, 1.0]
+    rays[:, 6], rays[:, 7] = 1e-4, 1e7
+    rays[0, 3] = np.nan
+    rays[1, 0] = np.nan
+    rays[2, 4] = np.inf
+    rays[3, 2] = -np.inf
+    rays[4, 3:6] = np.nan
+    for mode in (0, 1, 3, 4):
+        t, bary, inst, prim, st = osc.trace(rays[:5], mode)
+        assert (inst[:5] == (0 if mode in (1, 4) else 0xFFFFFFFF)).all()
+        assert st[1] == 0 and st[2] == 0 and st[0] == 5
+    # the finite ray still hits the room
+    _, _, inst, _, st = osc.trace(rays[5:], 0)
+    assert inst[0] != 0xFFFFFFFF and st[1] > 0

@@ -118,6 +118,14 @@ RT_DEV bool traverse(const DevScene& S, const RayT& r, bool skipLights, TStack s
     bool found = false;
     unsigned long long cn = 0, ct = 0;
     if (S.numPrims == 0) return false;
+    // A non-finite ray hits nothing; reject it here, because the slab test
+    // (fminf/fmaxf drop NaN operands) would accept every box and the ray would
+    // walk the whole tree.
+    if (!(isfinite(r.o.x) && isfinite(r.o.y) && isfinite(r.o.z) && isfinite(r.d.x) && isfinite(r.d.y) &&
+          isfinite(r.d.z))) {
+        if (kCount) atomicAdd(&ctr->rays, 1ull);
+        return false;
+    }
     uint32_t node = 0;
     for (;;) {
         const float4* np = S.nodes + 8 * (size_t)node;
