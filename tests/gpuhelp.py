@@ -181,3 +181,19 @@ def compare_reservoirs(a, b):
 
 def same_float(a, b):
     return (a == b) | (np.isnan(a) & np.isnan(b))
+
+
+def still_cams(base, i, seed):
+    """Camera pair for frame i of a still view: base[0] with a per-frame seed, frameIndex i (clear
+    flag on frame 0), previous camera = itself (no motion)."""
+    c = (rt.Camera * 2)()
+    C.memmove(C.addressof(c), C.addressof(base), C.sizeof(rt.Camera))
+    h = (seed * 0x9E3779B1 + i * 0x85EBCA77 + 0x165667B1) & 0xFFFFFFFF
+    h ^= h >> 15
+    h = (h * 0x2C1B3C6D) & 0xFFFFFFFF
+    c[0].seed = h ^ (h >> 12)
+    c[0].frameIndex = 0x80000000 if i == 0 else i
+    for k in range(16):
+        c[0].lastProjView[k] = c[0].projView[k]
+    C.memmove(C.addressof(c) + C.sizeof(rt.Camera), C.addressof(c), C.sizeof(rt.Camera))
+    return c
