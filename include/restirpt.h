@@ -231,6 +231,61 @@ rt_status rt_gris_learned_download(const void* reservoirs, uint32_t width, uint3
 rt_status rt_gris_learned_upload(const float* host4, uint32_t width, uint32_t height, void* reservoirs,
                                  hipStream_t stream);
 
+/* ---- ReSTIR DI ------------------------------------------------------------
+ * The direct-lighting method "ReSTIR DI" of the reference (TestReSTIR,
+ * src/TestReSTIR.h:7-38, selected by Renderer's directMethod = ResampledDI,
+ * src/Renderer.cpp:457-459): path generation (one light sample and one BSDF
+ * sample per pixel streamed into a reservoir), temporal reuse and spatial
+ * reuse (10 neighbours in 20 px, M-cap 40) with the reconnection or random
+ * replay shift, the spatial pass shading the pixel into directOutput. */
+typedef struct rt_di_settings {   /* TestReSTIR::Settings, src/TestReSTIR.h:12-17 (push constant) */
+    uint32_t shiftType;           /* 0 Reconnection, 1 Replay, 2 Hybrid (merges nothing in the reference) */
+    uint32_t sampleType;          /* 0 Light, 1 BSDF, 2 Both */
+    uint32_t temporalReuse;
+    uint32_t spatialReuse;
+} rt_di_settings;
+
+typedef struct rt_di_reservoir {  /* DIReservoir, layouts.glsl:97-115, 64 B (host view only) */
+    rt_intersection isec;
+    float Li[3]; float pad0;
+    float jacobian; float samplePdf; uint32_t rng; uint32_t isLightSample;
+    uint32_t sampleCount; float resampleWeight; float contribWeight; float weight;
+} rt_di_reservoir;
+
+/* Descriptor set 2 bindings 7/8/9 (layouts.glsl:187-189): this frame's,
+ * the previous frame's and the temporal reservoirs, each
+ * rt_di_reservoir_bytes(width, height) of device memory; swapped per frame
+ * parity like the reference's mDIReservoir[.][2] (src/Renderer.cpp:339-341). */
+typedef struct rt_di_reservoirs {
+    void* reservoir;
+    void* reservoirPrev;
+    void* reservoirTemp;
+} rt_di_reservoirs;
+
+/* 64 B per pixel, the reference's DIReservoirData (src/Renderer.cpp:33). */
+size_t rt_di_reservoir_bytes(uint32_t width, uint32_t height);
+/* Sets every reservoir to the reset state with an invalid sample. */
+rt_status rt_di_reservoir_reset(void* reservoirs, uint32_t width, uint32_t height, hipStream_t stream);
+/* Converts between the device layout and the reference's 64-B struct on the host. */
+rt_status rt_di_reservoir_download(const void* reservoirs, uint32_t width, uint32_t height, rt_di_reservoir* host,
+                                   hipStream_t stream);
+rt_status rt_di_reservoir_upload(const rt_di_reservoir* host, uint32_t width, uint32_t height, void* reservoirs,
+                                 hipStream_t stream);
+/* The three passes of TestReSTIR::render (src/TestReSTIR.cpp:9-36): di_path_gen,
+ * di_temporal and di_spatial (.comp/.glsl). Path generation writes
+ * reservoirs->reservoir; temporal reads it and reservoirPrev and writes
+ * reservoirTemp; spatial reads reservoirTemp, writes reservoir and accumulates
+ * frame->directOutput. The frame's GRIS reservoir bindings are not used. */
+rt_status rt_di_path_gen(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
+                         const rt_di_reservoirs* reservoirs, const rt_di_settings* settings, hipStream_t stream);
+rt_status rt_di_temporal(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
+                         const rt_di_reservoirs* reservoirs, const rt_di_settings* settings, hipStream_t stream);
+rt_status rt_di_spatial(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
+                        const rt_di_reservoirs* reservoirs, const rt_di_settings* settings, hipStream_t stream);
+/* TestReSTIR::render: path generation, temporal reuse, spatial reuse, in order. */
+rt_status rt_di_render(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
+                       const rt_di_reservoirs* reservoirs, const rt_di_settings* settings, hipStream_t stream);
+
 /* ---- Learned target function (no counterpart in the reference) ----------
  * A multiresolution hash-grid encoding of world position (levels x 2 FP16
  * features, 2^log2TableSize entries per level) feeding an MLP with two hidden

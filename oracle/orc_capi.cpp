@@ -174,6 +174,66 @@ void orc_pt_plain(void* h, const Camera* cam, const float* depthNormal, const ui
     sumStats(st, stats3);
 }
 
+// ---- ReSTIR DI passes ------------------------------------------------------------------------
+// cams[0] = uCamera, cams[1] = uPrevCamera; reservoirs are the reference's 64-B DIReservoir.
+void orc_di_path_gen(void* h, const Camera* cams, const float* depthNormal, const uint32_t* albedoMatId,
+                     const DISettings* set, DIReservoir* resvThis, uint64_t* stats3, int nthreads) {
+    OScene* s = (OScene*)h;
+    const Camera& cam = cams[0];
+    const uint32_t W = cam.filmSize[0], H = cam.filmSize[1];
+    GBufView g{depthNormal, albedoMatId, nullptr, W, H};
+    std::vector<TraceStats> st(nthreads > 1 ? nthreads : 1);
+    parallelRows(H, nthreads, [&](uint32_t y, int t) {
+        Ctx c{&s->S, &s->bvh, &st[t]};
+        for (uint32_t x = 0; x < W; x++) {
+            DIReservoir r;
+            if (diPathGen(c, cam, g, *set, x, y, r)) resvThis[(size_t)y * W + x] = r;
+        }
+    });
+    sumStats(st, stats3);
+}
+
+void orc_di_temporal(void* h, const Camera* cams, const float* depthNormal, const uint32_t* albedoMatId,
+                     const uint16_t* motion, const float* depthNormalPrev, const uint32_t* albedoMatIdPrev,
+                     const DISettings* set, const DIReservoir* resvThis, const DIReservoir* resvPrev,
+                     DIReservoir* resvTemp, uint64_t* stats3, int nthreads) {
+    OScene* s = (OScene*)h;
+    const Camera& cam = cams[0];
+    const uint32_t W = cam.filmSize[0], H = cam.filmSize[1];
+    GBufView g{depthNormal, albedoMatId, motion, W, H};
+    GBufView gp{depthNormalPrev, albedoMatIdPrev, nullptr, W, H};
+    std::vector<TraceStats> st(nthreads > 1 ? nthreads : 1);
+    parallelRows(H, nthreads, [&](uint32_t y, int t) {
+        Ctx c{&s->S, &s->bvh, &st[t]};
+        for (uint32_t x = 0; x < W; x++) {
+            DIReservoir r;
+            if (diTemporal(c, cam, cams[1], g, gp, *set, resvThis, resvPrev, x, y, r)) resvTemp[(size_t)y * W + x] = r;
+        }
+    });
+    sumStats(st, stats3);
+}
+
+void orc_di_spatial(void* h, const Camera* cams, const float* depthNormal, const uint32_t* albedoMatId,
+                    const DISettings* set, const DIReservoir* resvTemp, DIReservoir* resvThis, float* directOut,
+                    uint64_t* stats3, int nthreads) {
+    OScene* s = (OScene*)h;
+    const Camera& cam = cams[0];
+    const uint32_t W = cam.filmSize[0], H = cam.filmSize[1];
+    GBufView g{depthNormal, albedoMatId, nullptr, W, H};
+    std::vector<TraceStats> st(nthreads > 1 ? nthreads : 1);
+    parallelRows(H, nthreads, [&](uint32_t y, int t) {
+        Ctx c{&s->S, &s->bvh, &st[t]};
+        for (uint32_t x = 0; x < W; x++) {
+            const size_t p = (size_t)y * W + x;
+            DIReservoir r; bool written;
+            vec3 col = diSpatial(c, cam, g, *set, resvTemp, x, y, r, written);
+            if (written) resvThis[p] = r;
+            accumulate(directOut + 4 * p, col, cam.frameIndex);
+        }
+    });
+    sumStats(st, stats3);
+}
+
 // ---- GRIS passes ----------------------------------------------------------------------------
 // cams[0] = uCamera, cams[1] = uPrevCamera
 void orc_gris_path_trace(void* h, const Camera* cams, const float* depthNormal, const uint32_t* albedoMatId,

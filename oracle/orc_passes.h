@@ -6,6 +6,8 @@
 //   di / gi      : src/shader/di_naive.glsl:79-167, gi_naive.glsl:28-149
 //   GRIS         : src/shader/gris_path_trace.glsl:45-306, gris_retrace.glsl:42-236,
 //                  gris_resample_temporal.glsl:11-83, gris_resample_spatial.glsl:11-135
+//   ReSTIR DI    : src/shader/di_path_gen.glsl:9-36, di_temporal.glsl:9-88,
+//                  di_spatial.glsl:9-119, di_reservoir.glsl:28-290 (orc_di.cpp)
 //   wrappers     : *.comp main() accumulation (frameIndex & mask)
 // Uninitialised GLSL locals are taken as zero.
 #pragma once
@@ -157,5 +159,33 @@ vec3 spatialReuse(const Ctx& c, const Camera& cam, const GBufView& g, const GRIS
 // vertex (pos = its position).
 bool learnedRefresh(const Ctx& c, const GBufView& g, uint32_t x, uint32_t y, GRISReservoir& resv, const float Lhat[3],
                     LearnedData& out, bool& need, vec3& pos);
+
+// ---- ReSTIR DI (TestReSTIR) ---------------------------------------------------------
+struct DIPathSample {        // layouts.glsl:97-106, 48 B
+    Intersection isec;
+    float Li[3]; float pad0;
+    float jacobian; float samplePdf; uint32_t rng; uint32_t isLightSample;
+};
+struct DIReservoir {         // layouts.glsl:108-115, 64 B
+    DIPathSample pathSample;
+    uint32_t sampleCount; float resampleWeight; float contribWeight; float weight;
+};
+static_assert(sizeof(DIReservoir) == 64, "DI reservoir layout");
+struct DISettings {          // TestReSTIR::Settings, src/TestReSTIR.h:12-17
+    uint32_t shiftType;      // 0 Reconnection, 1 Replay, 2 Hybrid (merges nothing)
+    uint32_t sampleType;     // 0 Light, 1 BSDF, 2 Both
+    uint32_t temporalReuse;
+    uint32_t spatialReuse;
+};
+static const uint32_t DISampleLight = 0, DISampleBSDF = 1, DISampleBoth = 2;
+
+// Each returns false (nothing written) for a background pixel.
+bool diPathGen(const Ctx& c, const Camera& cam, const GBufView& g, const DISettings& set, uint32_t x, uint32_t y,
+               DIReservoir& out);
+bool diTemporal(const Ctx& c, const Camera& cam, const Camera& prevCam, const GBufView& g, const GBufView& gPrev,
+                const DISettings& set, const DIReservoir* resvThis, const DIReservoir* resvPrev, uint32_t x,
+                uint32_t y, DIReservoir& out);
+vec3 diSpatial(const Ctx& c, const Camera& cam, const GBufView& g, const DISettings& set,
+               const DIReservoir* resvTemp, uint32_t x, uint32_t y, DIReservoir& outThis, bool& written);
 
 }  // namespace orc

@@ -197,3 +197,60 @@ def still_cams(base, i, seed):
         c[0].lastProjView[k] = c[0].projView[k]
     C.memmove(C.addressof(c) + C.sizeof(rt.Camera), C.addressof(c), C.sizeof(rt.Camera))
     return c
+
+
+class DIRun(GrisRun):
+    """GrisRun plus the three DI reservoir buffers of descriptor bindings 7/8/9 (two parities + Temp)."""
+
+    def __init__(self, ds, W, H):
+        super().__init__(ds, W, H)
+        rb = rt.core().rt_di_reservoir_bytes(W, H)
+        self.di = [dev_empty(rb), dev_empty(rb)]
+        self.di_temp = dev_empty(rb)
+        for b in self.di + [self.di_temp]:
+            rt.check(rt.core().rt_di_reservoir_reset(b.data_ptr(), W, H, stream()), "di reset")
+
+    def di_struct(self):
+        r = rt.DIReservoirs()
+        r.reservoir, r.reservoirPrev = self.di[self.cur].data_ptr(), self.di[self.cur ^ 1].data_ptr()
+        r.reservoirTemp = self.di_temp.data_ptr()
+        return r
+
+    def _di(self, name, cams, s):
+        f = self.frame_struct()
+        r = self.di_struct()
+        rt.check(getattr(rt.core(), name)(self.ds.handle, cams, C.byref(f), C.byref(r), C.byref(s), stream()), name)
+
+    def di_path_gen(self, cams, s):
+        self._di("rt_di_path_gen", cams, s)
+
+    def di_temporal(self, cams, s):
+        self._di("rt_di_temporal", cams, s)
+
+    def di_spatial(self, cams, s):
+        self._di("rt_di_spatial", cams, s)
+
+    def di_render(self, cams, s):
+        self._di("rt_di_render", cams, s)
+
+    def di_reservoirs(self, buf):
+        out = np.zeros(self.W * self.H, orc.DI_DTYPE)
+        torch.cuda.synchronize()
+        rt.check(rt.core().rt_di_reservoir_download(buf.data_ptr(), self.W, self.H, out.ctypes.data, stream()),
+                 "di download")
+        return out
+
+
+DI_FLOAT_FIELDS = ["bary", "Li", "pad0", "jacobian", "samplePdf", "resampleWeight", "contribWeight", "weight"]
+DI_UINT_FIELDS = ["instanceIdx", "triangleIdx", "rng", "isLightSample", "sampleCount"]
+
+
+def compare_di(a, b):
+    """Mask of pixels whose DI reservoirs differ (floats by value with NaN == NaN, integers exactly)."""
+    bad = np.zeros(a.shape[0], bool)
+    for f in DI_FLOAT_FIELDS:
+        x, y = a[f].reshape(a.shape[0], -1), b[f].reshape(a.shape[0], -1)
+        bad |= ~((x == y) | (np.isnan(x) & np.isnan(y))).all(axis=1)
+    for f in DI_UINT_FIELDS:
+        bad |= a[f] != b[f]
+    return bad

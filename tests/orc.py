@@ -48,6 +48,10 @@ def lib():
                                         P, C.c_int, P, P, P]
         L.orc_gris_spatial.argtypes = [P, C.POINTER(rt.Camera), P, P, C.POINTER(rt.GrisSettings), P, P, P, P, C.c_int,
                                        P, P]
+        L.orc_di_path_gen.argtypes = [P, C.POINTER(rt.Camera), P, P, C.POINTER(rt.DISettings), P, P, C.c_int]
+        L.orc_di_temporal.argtypes = [P, C.POINTER(rt.Camera), P, P, P, P, P, C.POINTER(rt.DISettings), P, P, P, P,
+                                      C.c_int]
+        L.orc_di_spatial.argtypes = [P, C.POINTER(rt.Camera), P, P, C.POINTER(rt.DISettings), P, P, P, P, C.c_int]
         L.orc_gris_learned_refresh.argtypes = [P, u32, u32, P, P, P, P, P, P, u32, u32, C.c_int]
         L.orc_mlp_forward.argtypes = [u32, u32, u32, P, P, P, P, P, u32, P, C.c_int]
         L.orc_mlp_gradients.argtypes = [u32, u32, u32, P, P, P, P, P, P, u32, P, P, C.c_int]
@@ -166,6 +170,23 @@ class OracleScene:
                                p(indirect), p(st), nthreads(), p(learn_temp), p(learn_this))
         return st
 
+    def di_path_gen(self, cams, dn, am, settings, resv):
+        st = np.zeros(3, np.uint64)
+        lib().orc_di_path_gen(self.h, cams, p(dn), p(am), C.byref(settings), p(resv), p(st), nthreads())
+        return st
+
+    def di_temporal(self, cams, dn, am, mv, dn_prev, am_prev, settings, resv_this, resv_prev, resv_temp):
+        st = np.zeros(3, np.uint64)
+        lib().orc_di_temporal(self.h, cams, p(dn), p(am), p(mv), p(dn_prev), p(am_prev), C.byref(settings),
+                              p(resv_this), p(resv_prev), p(resv_temp), p(st), nthreads())
+        return st
+
+    def di_spatial(self, cams, dn, am, settings, resv_temp, resv_this, direct):
+        st = np.zeros(3, np.uint64)
+        lib().orc_di_spatial(self.h, cams, p(dn), p(am), C.byref(settings), p(resv_temp), p(resv_this), p(direct),
+                             p(st), nthreads())
+        return st
+
     def learned_refresh(self, W, H, dn, resv, lhat4, stride, offset):
         """Refreshes resv in place from the predictions lhat4 (H*W x 4); returns (learned planes, batch pos,
         batch target) with the batch over pixels p % stride == offset."""
@@ -185,6 +206,20 @@ RESV_DTYPE = np.dtype([("bary", np.float32, 2), ("instanceIdx", np.uint32), ("tr
                        ("F", np.float32, 3), ("primaryRng", np.uint32), ("sampleCount", np.float32),
                        ("resampleWeight", np.float32), ("contribWeight", np.float32), ("pad0", np.float32)])
 assert RESV_DTYPE.itemsize == 96
+
+
+DI_DTYPE = np.dtype([("bary", np.float32, 2), ("instanceIdx", np.uint32), ("triangleIdx", np.uint32),
+                     ("Li", np.float32, 3), ("pad0", np.float32), ("jacobian", np.float32), ("samplePdf", np.float32),
+                     ("rng", np.uint32), ("isLightSample", np.uint32), ("sampleCount", np.uint32),
+                     ("resampleWeight", np.float32), ("contribWeight", np.float32), ("weight", np.float32)])
+assert DI_DTYPE.itemsize == 64
+
+
+def reset_di_reservoirs(n):
+    """The state rt_di_reservoir_reset writes: zeros with an invalid sample."""
+    r = np.zeros(n, DI_DTYPE)
+    r["instanceIdx"] = 0xFFFFFFFF
+    return r
 
 
 class OracleMlp:

@@ -43,6 +43,10 @@ def test_struct_sizes_match_reference_layouts():
     # rt_view: scene pointer + 2 cameras + rt_frame (88 B) + settings (36 B), padded to 8
     assert C.sizeof(rt.Frame) == 88 and C.sizeof(rt.View) == 8 + 2 * 352 + 88 + 36 + 4
     assert C.sizeof(rt.BatchTuning) == 8
+    assert C.sizeof(rt.DIReservoir) == 64      # layouts.glsl DIReservoir (std430)
+    assert C.sizeof(rt.DISettings) == 16       # TestReSTIR::Settings push constant
+    assert C.sizeof(rt.DIReservoirs) == 24
+    assert rt.core().rt_di_reservoir_bytes(1920, 1080) == 1920 * 1080 * 64
     # five 16-B planes of reference fields + the learned-target plane = the reference's 96-B reservoir
     assert rt.core().rt_gris_reservoir_bytes(1920, 1080) == 1920 * 1080 * 96
 
@@ -78,6 +82,13 @@ def test_argument_checks_fail_loudly():
     f.depthNormal = f.albedoMatId = 8
     assert L.rt_gris_path_trace(fake, cams, C.byref(f), C.byref(bad), None) == 1
     assert b"maxDepth" in L.rt_last_error()
+    r = rt.DIReservoirs()
+    assert L.rt_di_render(fake, cams, C.byref(f), C.byref(r), C.byref(rt.di_settings(shift=3)), None) == 1
+    assert b"shiftType" in L.rt_last_error()
+    assert L.rt_di_render(fake, cams, C.byref(f), C.byref(r), C.byref(rt.di_settings(sample=3)), None) == 1
+    assert b"sampleType" in L.rt_last_error()
+    assert L.rt_di_path_gen(fake, cams, C.byref(f), None, C.byref(rt.di_settings()), None) == 1
+    assert b"reservoirs is null" in L.rt_last_error()
 
 
 def test_missing_library_is_an_error(monkeypatch):

@@ -32,6 +32,13 @@ hipError_t launchGrisSpatial(const rt_device_scene*, const rt_camera*, const rt_
                              hipStream_t);
 hipError_t launchResvReset(void*, uint32_t, uint32_t, hipStream_t);
 hipError_t launchLearnedRefresh(const rt_device_scene*, rt_mlp*, const rt_camera*, const rt_frame&, hipStream_t);
+hipError_t launchDiPathGen(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_di_reservoirs&,
+                           const rt_di_settings&, hipStream_t);
+hipError_t launchDiTemporal(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_di_reservoirs&,
+                            const rt_di_settings&, hipStream_t);
+hipError_t launchDiSpatial(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_di_reservoirs&,
+                           const rt_di_settings&, hipStream_t);
+hipError_t launchDiReset(void*, uint32_t, uint32_t, hipStream_t);
 
 }  // namespace rti
 
@@ -101,6 +108,18 @@ rt_status checkFrame(const rt_device_scene* s, const rt_camera* cams, const rt_f
     return RT_OK;
 }
 
+rt_status checkDI(const rt_device_scene* s, const rt_camera* cams, const rt_frame* f, const rt_di_reservoirs* r,
+                  const rt_di_settings* set) {
+    rt_status st = checkFrame(s, cams, f);
+    if (st != RT_OK) return st;
+    if (!r) return invalid("reservoirs is null");
+    if (!set) return invalid("settings is null");
+    if (set->shiftType > 2) return invalid("shiftType must be 0, 1 or 2");
+    if (set->sampleType > 2) return invalid("sampleType must be 0, 1 or 2");
+    if (s->numLights == 0) return invalid("the scene has no lights");
+    return RT_OK;
+}
+
 rt_status checkSettings(const rt_gris_settings* st) {
     if (!st) return invalid("settings is null");
     if (st->maxDepth == 0 || st->maxDepth > 255) return invalid("maxDepth must be in [1, 255]");
@@ -136,6 +155,7 @@ rt_status rt_device_scene_create(const rt_scene_arrays* a, hipStream_t stream, r
         for (int c = 0; c < 4; c++)
             for (int r = 0; r < 3; r++) { D.m[c * 3 + r] = I.transform[c * 4 + r]; D.n[c * 3 + r] = I.transformInvT[c * 4 + r]; }
         D.indexOffset = I.indexOffset;
+        D.triCount = I.indexCount / 3;
         offs[k] = prims;
         prims += I.indexCount / 3;
     }
@@ -159,6 +179,7 @@ rt_status rt_device_scene_create(const rt_scene_arrays* a, hipStream_t stream, r
         d.indices = s->indices; d.instances = s->instances; d.lights = s->lights; d.lightTable = s->lightTable;
         d.numPrims = s->numPrims; d.numLights = s->numLights; d.numLightTable = a->numLightTable;
         d.numMaterials = a->numMaterials;
+        d.numInstances = a->numInstances;
         ok = rti::buildLBVH(s, offs, stream);
         d.nodes = s->nodes4; d.leafTris = s->leafTris;
     }
@@ -353,6 +374,76 @@ rt_status rt_gris_learned_upload(const float* host4, uint32_t w, uint32_t h, voi
     RT_HIP(hipMemcpyAsync(static_cast<float4*>(r) + 5 * n, host4, sizeof(float4) * n, hipMemcpyHostToDevice, stream));
     RT_HIP(hipStreamSynchronize(stream));
     return RT_OK;
+}
+
+// ---- ReSTIR DI --------------------------------------------------------------------------------
+size_t rt_di_reservoir_bytes(uint32_t w, uint32_t h) { return (size_t)w * h * 64; }
+
+rt_status rt_di_reservoir_reset(void* r, uint32_t w, uint32_t h, hipStream_t stream) {
+    if (!r) return invalid("reservoir buffer is null");
+    RT_LAUNCH(rti::launchDiReset(r, w, h, stream));
+    return RT_OK;
+}
+
+// Device plane k holds the k-th 16 bytes of the reference struct.
+rt_status rt_di_reservoir_download(const void* r, uint32_t w, uint32_t h, rt_di_reservoir* host, hipStream_t stream) {
+    if (!r || !host) return invalid("null argument");
+    static_assert(sizeof(rt_di_reservoir) == 64, "DIReservoir is 64 B");
+    const size_t n = (size_t)w * h;
+    std::vector<float4> pl(n * 4);
+    RT_HIP(hipMemcpyAsync(pl.data(), r, sizeof(float4) * 4 * n, hipMemcpyDeviceToHost, stream));
+    RT_HIP(hipStreamSynchronize(stream));
+    for (size_t i = 0; i < n; i++)
+        for (int k = 0; k < 4; k++) std::memcpy(reinterpret_cast<char*>(&host[i]) + 16 * k, &pl[k * n + i], 16);
+    return RT_OK;
+}
+
+rt_status rt_di_reservoir_upload(const rt_di_reservoir* host, uint32_t w, uint32_t h, void* r, hipStream_t stream) {
+    if (!r || !host) return invalid("null argument");
+    const size_t n = (size_t)w * h;
+    std::vector<float4> pl(n * 4);
+    for (size_t i = 0; i < n; i++)
+        for (int k = 0; k < 4; k++) std::memcpy(&pl[k * n + i], reinterpret_cast<const char*>(&host[i]) + 16 * k, 16);
+    RT_HIP(hipMemcpyAsync(r, pl.data(), sizeof(float4) * 4 * n, hipMemcpyHostToDevice, stream));
+    RT_HIP(hipStreamSynchronize(stream));
+    return RT_OK;
+}
+
+rt_status rt_di_path_gen(const rt_device_scene* s, const rt_camera cams[2], const rt_frame* f,
+                         const rt_di_reservoirs* r, const rt_di_settings* set, hipStream_t stream) {
+    rt_status st = checkDI(s, cams, f, r, set);
+    if (st != RT_OK) return st;
+    if (!r->reservoir) return invalid("reservoir buffer is null");
+    RT_LAUNCH(rti::launchDiPathGen(s, cams, *f, *r, *set, stream));
+    return RT_OK;
+}
+
+rt_status rt_di_temporal(const rt_device_scene* s, const rt_camera cams[2], const rt_frame* f,
+                         const rt_di_reservoirs* r, const rt_di_settings* set, hipStream_t stream) {
+    rt_status st = checkDI(s, cams, f, r, set);
+    if (st != RT_OK) return st;
+    if (!r->reservoir || !r->reservoirPrev || !r->reservoirTemp) return invalid("reservoir buffer is null");
+    if (!f->depthNormalPrev || !f->albedoMatIdPrev || !f->motionVector) return invalid("previous G-buffer is null");
+    RT_LAUNCH(rti::launchDiTemporal(s, cams, *f, *r, *set, stream));
+    return RT_OK;
+}
+
+rt_status rt_di_spatial(const rt_device_scene* s, const rt_camera cams[2], const rt_frame* f,
+                        const rt_di_reservoirs* r, const rt_di_settings* set, hipStream_t stream) {
+    rt_status st = checkDI(s, cams, f, r, set);
+    if (st != RT_OK) return st;
+    if (!r->reservoir || !r->reservoirTemp) return invalid("reservoir buffer is null");
+    if (!f->directOutput) return invalid("directOutput is null");
+    RT_LAUNCH(rti::launchDiSpatial(s, cams, *f, *r, *set, stream));
+    return RT_OK;
+}
+
+rt_status rt_di_render(const rt_device_scene* s, const rt_camera cams[2], const rt_frame* f,
+                       const rt_di_reservoirs* r, const rt_di_settings* set, hipStream_t stream) {
+    rt_status st = rt_di_path_gen(s, cams, f, r, set, stream);
+    if (st == RT_OK) st = rt_di_temporal(s, cams, f, r, set, stream);
+    if (st == RT_OK) st = rt_di_spatial(s, cams, f, r, set, stream);
+    return st;
 }
 
 rt_status rt_gris_learned_refresh(const rt_device_scene* s, rt_mlp* mlp, const rt_camera cams[2], const rt_frame* f,

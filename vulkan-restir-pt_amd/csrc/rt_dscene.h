@@ -19,7 +19,8 @@ struct DevInstance {
     float m[12];     // transform columns c0..c3 (xyz each)
     float n[12];     // transformInvT columns c0..c3 (xyz each)
     uint32_t indexOffset;
-    uint32_t pad[3];
+    uint32_t triCount;   // triangles of the instance (bounds of sample references)
+    uint32_t pad[2];
 };
 
 struct DevScene {
@@ -36,6 +37,7 @@ struct DevScene {
     uint32_t numLights;
     uint32_t numLightTable;
     uint32_t numMaterials;
+    uint32_t numInstances;
 };
 
 // ---- material.glsl --------------------------------------------------------------------

@@ -73,6 +73,24 @@ class GrisReservoir(C.Structure):
                 ("resampleWeight", C.c_float), ("contribWeight", C.c_float), ("pad0", C.c_float)]
 
 
+class DIReservoir(C.Structure):
+    """DIReservoir (layouts.glsl:97-115), 64 B."""
+    _fields_ = [("bary", C.c_float * 2), ("instanceIdx", C.c_uint32), ("triangleIdx", C.c_uint32),
+                ("Li", C.c_float * 3), ("pad0", C.c_float), ("jacobian", C.c_float), ("samplePdf", C.c_float),
+                ("rng", C.c_uint32), ("isLightSample", C.c_uint32), ("sampleCount", C.c_uint32),
+                ("resampleWeight", C.c_float), ("contribWeight", C.c_float), ("weight", C.c_float)]
+
+
+class DISettings(C.Structure):
+    """TestReSTIR::Settings (src/TestReSTIR.h:12-17)."""
+    _fields_ = [("shiftType", C.c_uint32), ("sampleType", C.c_uint32), ("temporalReuse", C.c_uint32),
+                ("spatialReuse", C.c_uint32)]
+
+
+class DIReservoirs(C.Structure):
+    _fields_ = [("reservoir", C.c_void_p), ("reservoirPrev", C.c_void_p), ("reservoirTemp", C.c_void_p)]
+
+
 class SceneArrays(C.Structure):
     _fields_ = [("materials", C.c_void_p), ("numMaterials", C.c_uint32),
                 ("materialIndices", C.c_void_p), ("numMaterialIndices", C.c_uint32),
@@ -120,12 +138,18 @@ class SceneInfo(C.Structure):
 
 assert C.sizeof(Material) == 32 and C.sizeof(MeshVertex) == 32 and C.sizeof(ObjectInstance) == 224
 assert C.sizeof(TriangleLight) == 64 and C.sizeof(Camera) == 352 and C.sizeof(GrisReservoir) == 96
+assert C.sizeof(DIReservoir) == 64 and C.sizeof(DISettings) == 16
 
 
 def settings(shift=2, rr_scale=1.0, temporal=False, spatial=True, cap=20, max_depth=15, neighbors=3, radius=20.0,
              learned=False):
     """GRISReSTIR::Settings (src/GRISReSTIR.h:29 defaults) + kernel constants."""
     return GrisSettings(shift, rr_scale, int(temporal), int(spatial), cap, max_depth, neighbors, radius, int(learned))
+
+
+def di_settings(shift=0, sample=0, temporal=False, spatial=True):
+    """TestReSTIR::Settings; defaults are the reference's {Reconnection, Light, 0, 1} (src/TestReSTIR.h:31)."""
+    return DISettings(shift, sample, int(temporal), int(spatial))
 
 
 def mlp_config(lo, hi, log2_table=19, base_res=16, max_res=2048, seed=3, lr=1e-2):
@@ -147,6 +171,8 @@ CORE_SYMBOLS = [
     "rt_mlp_train_step", "rt_mlp_train_frame", "rt_mlp_batch_download", "rt_gris_learned_refresh",
     "rt_frame_graph_create", "rt_frame_graph_launch", "rt_frame_graph_destroy", "rt_batch_create",
     "rt_batch_destroy", "rt_batch_frame", "rt_batch_set_tuning", "rt_batch_get_tuning", "rt_batch_autotune",
+    "rt_di_reservoir_bytes", "rt_di_reservoir_reset", "rt_di_reservoir_download", "rt_di_reservoir_upload",
+    "rt_di_path_gen", "rt_di_temporal", "rt_di_spatial", "rt_di_render",
 ]
 HOST_SYMBOLS = [
     "rt_scenegen", "rt_host_scene_destroy", "rt_host_scene_arrays", "rt_host_scene_camera", "rt_light_table_build",
@@ -204,13 +230,20 @@ def _declare_core(L):
         "rt_batch_set_tuning": [P, C.POINTER(BatchTuning)],
         "rt_batch_get_tuning": [P, C.POINTER(BatchTuning)],
         "rt_batch_autotune": [P, C.POINTER(View), u32, u32, P, C.POINTER(BatchTuning), C.POINTER(C.c_float)],
+        "rt_di_reservoir_reset": [P, u32, u32, P],
+        "rt_di_reservoir_download": [P, u32, u32, P, P],
+        "rt_di_reservoir_upload": [P, u32, u32, P, P],
     }
+    di_pass = [P, C.POINTER(Camera), C.POINTER(Frame), C.POINTER(DIReservoirs), C.POINTER(DISettings), P]
+    for name in ("rt_di_path_gen", "rt_di_temporal", "rt_di_spatial", "rt_di_render"):
+        sigs[name] = di_pass
     for name, args in sigs.items():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = i32
-    L.rt_gris_reservoir_bytes.argtypes = [u32, u32]
-    L.rt_gris_reservoir_bytes.restype = C.c_size_t
+    for name in ("rt_gris_reservoir_bytes", "rt_di_reservoir_bytes"):
+        getattr(L, name).argtypes = [u32, u32]
+        getattr(L, name).restype = C.c_size_t
 
 
 def _declare_host(L):
