@@ -36,6 +36,14 @@ KERNELS = {"gbuffer": "k_gbuffer", "path_trace": "k_gris_path_trace", "temporal"
            "spatial": "k_gris_spatial"}
 
 
+# ReSTIR DI passes: per-pixel G-buffer, reservoir (64 B) and output bytes, every neighbour's
+# bilinear depth-normal (4 texels), albedo/material and reservoir counted as read.
+DI_PASSES = ["gbuffer", "di_path_gen", "di_temporal", "di_spatial"]
+DI_IO_BYTES = {"gbuffer": 28, "di_path_gen": 88, "di_temporal": 292, "di_spatial": 1544}
+DI_KERNELS = {"gbuffer": "k_gbuffer", "di_path_gen": "k_di_path_gen", "di_temporal": "k_di_temporal",
+              "di_spatial": "k_di_spatial"}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +66,8 @@ def parse():
     ap.add_argument("--batched-views", type=int, default=64)
     ap.add_argument("--lvp-frames", type=int, default=64, help="frames of the plain form in the equal-time run")
     ap.add_argument("--ref-frames", type=int, default=4096, help="frames of the converged reference image")
+    ap.add_argument("--no-di", action="store_true", help="skip the ReSTIR DI leg")
+    ap.add_argument("--di-kernel-loop", action="store_true", help="time ReSTIR DI frames only (for profilers)")
     return ap.parse_args()
 
 
@@ -470,6 +480,92 @@ def batched_views(rt, args, hs1, ds1):
                         "round robin over 4 procedural scenes"}
 
 
+def restir_di(rt, hs, ds, args, kernel_loop=False):
+    """ReSTIR DI (the reference's TestReSTIR: path generation, temporal reuse, spatial reuse with
+    10 neighbours in 20 px and M-cap 40) at the bench resolution on the same terrain and camera walk,
+    with the reference's default settings {Reconnection, Light, temporal off, spatial on} and with
+    temporal reuse on and both sample types. Per-pass times from HIP events on the launch stream;
+    traversal work from the counting kernel builds on a replay of the same frames."""
+    import numpy as np
+    import torch
+
+    W, H = args.width, args.height
+    L, H_ = rt.core(), rt.host()
+    r = C.c_void_p()
+    rt.check(H_.rt_renderer_create(ds.handle, W, H, C.byref(r)), "rt_renderer_create")
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    variants = {"reference_defaults": rt.di_settings(shift=0, sample=0, temporal=False, spatial=True),
+                "temporal_spatial_both": rt.di_settings(shift=0, sample=2, temporal=True, spatial=True)}
+    if kernel_loop:
+        variants = {"reference_defaults": variants["reference_defaults"]}
+    res = {}
+    cnt = np.zeros(3, np.uint64)
+
+    def frame(k, s, ev=None, work=None):
+        cams = hs.camera(k % 16, W, H)
+        f, d = rt.Frame(), rt.DIReservoirs()
+        rt.check(H_.rt_renderer_bindings(r, C.byref(f)), "bindings")
+        rt.check(H_.rt_renderer_di_bindings(r, C.byref(d)), "di bindings")
+        calls = [lambda: L.rt_gbuffer_trace(ds.handle, C.byref(cams[0]), f.depthNormal, f.albedoMatId,
+                                            f.motionVector, stream),
+                 lambda: L.rt_di_path_gen(ds.handle, cams, C.byref(f), C.byref(d), C.byref(s), stream),
+                 lambda: L.rt_di_temporal(ds.handle, cams, C.byref(f), C.byref(d), C.byref(s), stream),
+                 lambda: L.rt_di_spatial(ds.handle, cams, C.byref(f), C.byref(d), C.byref(s), stream)]
+        for i, (p, call) in enumerate(zip(DI_PASSES, calls)):
+            if ev: ev[i].record()
+            if work is not None: rt.check(L.rt_counters_read(cnt.ctypes.data, 1), "read")
+            rt.check(call(), p)
+            if work is not None:
+                rt.check(L.rt_counters_read(cnt.ctypes.data, 1), "read")
+                work[p] += cnt
+        if ev: ev[4].record()
+        rt.check(H_.rt_renderer_flip(r), "flip")
+
+    try:
+        for name, s in variants.items():
+            rt.check(H_.rt_renderer_reset_reservoirs(r, stream), "reset")
+            for k in range(args.warmup):
+                frame(k, s)
+            torch.cuda.synchronize()
+            ev = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
+            t0 = time.perf_counter()
+            for k in range(args.steps):
+                frame(args.warmup + k, s, None if kernel_loop else ev[k])
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            v = {"ms_per_frame": round(wall * 1e3 / args.steps, 3),
+                 "mpixels_per_s": round(W * H * args.steps / wall / 1e6, 2)}
+            if not kernel_loop:
+                pass_ms = {p: float(np.mean([ev[k][i].elapsed_time(ev[k][i + 1]) for k in range(args.steps)]))
+                           for i, p in enumerate(DI_PASSES)}
+                v["passes_ms"] = {p: round(x, 3) for p, x in pass_ms.items()}
+                work = {p: np.zeros(3, np.uint64) for p in DI_PASSES}
+                rt.check(L.rt_counters_enable(1), "counters")
+                try:
+                    for k in range(args.steps):
+                        frame(args.warmup + k, s, work=work)
+                finally:
+                    L.rt_counters_enable(0)
+                rays = sum(int(work[p][2]) for p in DI_PASSES)
+                v["mrays_per_s"] = round(rays / (sum(pass_ms.values()) * 1e-3 * args.steps) / 1e6, 1)
+                dom = max(DI_PASSES[1:], key=lambda p: pass_ms[p])
+                per_launch = (DI_IO_BYTES[dom] * W * H + (NODE_BYTES * int(work[dom][0]) +
+                                                         TRI_BYTES * int(work[dom][1])) / args.steps)
+                achieved = per_launch / (pass_ms[dom] * 1e-3) / 1e9
+                v["roofline"] = {"bound": "hbm", "kernel": DI_KERNELS[dom], "achieved": round(achieved, 1),
+                                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                                 "algorithmic_bytes_per_launch": int(per_launch), "launch_ms": round(pass_ms[dom], 3)}
+                v["work_per_frame"] = {p: {"rays": int(work[p][2]) // args.steps,
+                                           "nodes": int(work[p][0]) // args.steps,
+                                           "tris": int(work[p][1]) // args.steps} for p in DI_PASSES}
+            res[name] = v
+    finally:
+        H_.rt_renderer_destroy(r)
+    res["workload"] = (f"ReSTIR DI (TestReSTIR) frames: G-buffer + path generation + temporal + spatial "
+                       f"(10 neighbours, 20 px, M-cap 40, reconnection shift), {W}x{H}, same terrain and camera walk")
+    return res
+
+
 def read_traffic(kernel):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_1080p_terrain_traffic.md")
@@ -505,6 +601,9 @@ def main():
     s = settings(rt, W)
     if args.learned_kernel_loop:
         print(json.dumps(learned_kernel_loop(rt, hs, ds, args)))
+        return
+    if args.di_kernel_loop:
+        print(json.dumps(restir_di(rt, hs, ds, args, kernel_loop=True)))
         return
     loop = FrameLoop(rt, hs, ds, W, H, s)
     for k in range(args.warmup):
@@ -549,6 +648,8 @@ def main():
                             "launch_ms": round(pass_ms[dom], 3)}
         line["work_per_frame"] = {p: {"rays": int(work[p][2]) // args.steps, "nodes": int(work[p][0]) // args.steps,
                                       "tris": int(work[p][1]) // args.steps} for p in PASSES}
+    if not args.no_di:
+        line["restir_di"] = restir_di(rt, hs, ds, args)
     if not args.no_batched:
         line["batched"] = batched_views(rt, args, hs, ds)
     if not args.no_learned:

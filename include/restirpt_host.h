@@ -54,13 +54,25 @@ rt_status rt_renderer_create(rt_device_scene* scene, uint32_t width, uint32_t he
 rt_status rt_renderer_destroy(rt_renderer* r);
 rt_status rt_renderer_set_settings(rt_renderer* r, const rt_gris_settings* settings);
 rt_status rt_renderer_frame(rt_renderer* r, const rt_camera cams[2], uint32_t method, hipStream_t stream);
+/* One frame with the reference's separate method choices (Renderer::Settings
+ * directMethod / indirectMethod, src/Renderer.h:25-31, dispatched as in
+ * Renderer::recordRenderCommand, src/Renderer.cpp:440-470): G-buffer, then the
+ * direct method (0 None, 1 Naive, 2 ReSTIR DI), then the indirect method
+ * (0 None, 1 Naive, 3 ReSTIR PT; 2 = ReSTIR GI is not part of this build);
+ * flips the frame parity. */
+rt_status rt_renderer_frame_methods(rt_renderer* r, const rt_camera cams[2], uint32_t directMethod,
+                                    uint32_t indirectMethod, hipStream_t stream);
+/* TestReSTIR::settings of the renderer's ReSTIR DI pass. */
+rt_status rt_renderer_set_di_settings(rt_renderer* r, const rt_di_settings* settings);
+/* Current frame's DI reservoir bindings (descriptor bindings 7/8/9). */
+rt_status rt_renderer_di_bindings(const rt_renderer* r, rt_di_reservoirs* out);
 /* Current frame's bindings (device pointers) for callers that launch passes themselves. */
 rt_status rt_renderer_bindings(const rt_renderer* r, rt_frame* out);
 /* Flips the frame parity (mCurFrame ^= 1). rt_renderer_frame does this itself. */
 rt_status rt_renderer_flip(rt_renderer* r);
 /* Copies an RGBA32F output (0 direct, 1 indirect) to host memory. */
 rt_status rt_renderer_read_output(const rt_renderer* r, uint32_t which, float* host, hipStream_t stream);
-/* Resets the three reservoir buffers (the GUI's clearReservoir, src/Renderer.cpp:567). */
+/* Resets the GRIS and DI reservoir buffers (the GUI's clearReservoir, src/Renderer.cpp:567). */
 rt_status rt_renderer_reset_reservoirs(rt_renderer* r, hipStream_t stream);
 /* Creates the renderer's learned target function (hash grid 16 x 2 x
  * 2^log2TableSize, MLP 2 x 64, bounds = scene box + 10%) and binds it to the

@@ -161,3 +161,38 @@ def test_di_argument_checks():
     assert b"sampleType" in L.rt_last_error()
     r.reservoirTemp = None
     assert L.rt_di_temporal(ds.handle, cams, C.byref(f), C.byref(r), C.byref(s), None) == 1
+
+
+def test_renderer_methods_match_pass_calls():
+    """rt_renderer_frame_methods(direct = ReSTIR DI, indirect = ReSTIR PT) equals the G-buffer pass,
+    rt_di_render and rt_gris_render called in that order on separate buffers, frame after frame."""
+    hs = rt.HostScene(0, 0)
+    ds = rt.DeviceScene(hs.arrays)
+    W, H = 56, 40
+    Hh = rt.host()
+    r = C.c_void_p()
+    rt.check(Hh.rt_renderer_create(ds.handle, W, H, C.byref(r)), "renderer")
+    try:
+        gs = rt.settings(shift=2, temporal=True, spatial=True, max_depth=5)
+        rt.check(Hh.rt_renderer_set_settings(r, C.byref(gs)), "settings")
+        di = rt.di_settings(shift=1, sample=2, temporal=True, spatial=True)
+        rt.check(Hh.rt_renderer_set_di_settings(r, C.byref(di)), "di settings")
+        run = gpuhelp.DIRun(ds, W, H)
+        for frame in range(3):
+            print(f"[gpu] renderer methods frame {frame}", flush=True)
+            cams = hs.camera(frame, W, H)
+            rt.check(Hh.rt_renderer_frame_methods(r, cams, 2, 3, gpuhelp.stream()), "frame")
+            run.gbuffer(cams[0])
+            run.di_render(cams, di)
+            f = run.frame_struct()
+            rt.check(rt.core().rt_gris_render(ds.handle, cams, C.byref(f), C.byref(gs), gpuhelp.stream()), "gris")
+            torch.cuda.synchronize()
+            for which in (0, 1):
+                host = np.zeros((H, W, 4), np.float32)
+                rt.check(Hh.rt_renderer_read_output(r, which, host.ctypes.data, gpuhelp.stream()), "read")
+                assert np.array_equal(host.view(np.uint32), run.output(which).view(np.uint32)), (frame, which)
+            run.flip()
+        assert run.output(0)[..., :3].mean() > 0 and run.output(1)[..., :3].mean() > 0
+        assert Hh.rt_renderer_frame_methods(r, cams, 2, 2, gpuhelp.stream()) != 0  # ReSTIR GI is not built
+    finally:
+        Hh.rt_renderer_destroy(r)

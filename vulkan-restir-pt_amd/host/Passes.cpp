@@ -60,6 +60,26 @@ void GRISReSTIR::render(hipStream_t stream, const DeviceScene& scene, const rt_c
     if (learnedTarget) rtThrow(rt_mlp_train_frame(learned, nullptr, stream), "rt_mlp_train_frame");
 }
 
+// ---- TestReSTIR: src/TestReSTIR.cpp:9-46 ----
+void TestReSTIR::createPipeline() {
+    const rt_di_reservoirs* bound = &mBound;
+    auto bind = [bound](rt_status (*fn)(const rt_device_scene*, const rt_camera*, const rt_frame*,
+                                        const rt_di_reservoirs*, const rt_di_settings*, hipStream_t)) {
+        return [fn, bound](const rt_device_scene* s, const rt_camera* c, const rt_frame* f, const void* pc,
+                           hipStream_t st) { return fn(s, c, f, bound, static_cast<const rt_di_settings*>(pc), st); };
+    };
+    mPathTracePass.createPipeline(bind(&rt_di_path_gen));
+    mTemporalReusePass.createPipeline(bind(&rt_di_temporal));
+    mSpatialReusePass.createPipeline(bind(&rt_di_spatial));
+}
+void TestReSTIR::render(hipStream_t stream, const DeviceScene& scene, const rt_camera cams[2], const rt_frame& frame,
+                        const rt_di_reservoirs& reservoirs) {
+    mBound = reservoirs;
+    mPathTracePass.execute(stream, scene, cams, frame, &settings);
+    mTemporalReusePass.execute(stream, scene, cams, frame, &settings);
+    mSpatialReusePass.execute(stream, scene, cams, frame, &settings);
+}
+
 LearnedTarget::LearnedTarget(const DeviceScene& scene, uint32_t seed, hipStream_t stream, uint32_t log2TableSize) {
     rt_scene_info info{};
     rtThrow(rt_device_scene_info(scene.handle(), &info), "rt_device_scene_info");
@@ -81,6 +101,8 @@ LearnedTarget::~LearnedTarget() {
 void Renderer::resetReservoirs(hipStream_t stream) {
     for (int j = 0; j < 2; j++) rtThrow(rt_gris_reservoir_reset(grisReservoir[j], width, height, stream), "reset");
     rtThrow(rt_gris_reservoir_reset(grisReservoirTemp, width, height, stream), "reset");
+    for (int j = 0; j < 2; j++) rtThrow(rt_di_reservoir_reset(diReservoir[j], width, height, stream), "reset");
+    rtThrow(rt_di_reservoir_reset(diReservoirTemp, width, height, stream), "reset");
 }
 
 // ---- Renderer: src/Renderer.cpp:193-257 (createRayImage), 319-353 (bindings), 400-569 ----
@@ -96,8 +118,16 @@ Renderer::Renderer(const DeviceScene& scene, uint32_t w, uint32_t h)
     }
     grisReservoirTemp = deviceAlloc<uint8_t>(rb);
     rtThrow(rt_gris_reservoir_reset(grisReservoirTemp, w, h, 0), "rt_gris_reservoir_reset");
+    const size_t db = rt_di_reservoir_bytes(w, h);
+    for (int j = 0; j < 2; j++) {
+        diReservoir[j] = deviceAlloc<uint8_t>(db);
+        rtThrow(rt_di_reservoir_reset(diReservoir[j], w, h, 0), "rt_di_reservoir_reset");
+    }
+    diReservoirTemp = deviceAlloc<uint8_t>(db);
+    rtThrow(rt_di_reservoir_reset(diReservoirTemp, w, h, 0), "rt_di_reservoir_reset");
     if (hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("hipDeviceSynchronize failed");
     gris.createPipeline();
+    restirDI.createPipeline();
     uint32_t* maxDepth = &maxDepthNaive;
     mNaiveDI.createPipeline([maxDepth](const rt_device_scene* s, const rt_camera* c, const rt_frame* f, const void*,
                                        hipStream_t st) { return rt_pt_plain(s, c, f, *maxDepth, 1u, st); });
@@ -107,6 +137,14 @@ Renderer::Renderer(const DeviceScene& scene, uint32_t w, uint32_t h)
 Renderer::~Renderer() {
     (void)hipFree(directOutput); (void)hipFree(indirectOutput);
     (void)hipFree(grisReservoir[0]); (void)hipFree(grisReservoir[1]); (void)hipFree(grisReservoirTemp);
+    (void)hipFree(diReservoir[0]); (void)hipFree(diReservoir[1]); (void)hipFree(diReservoirTemp);
+}
+rt_di_reservoirs Renderer::diBindings() const {
+    rt_di_reservoirs r{};
+    r.reservoir = diReservoir[mCurFrame];
+    r.reservoirPrev = diReservoir[mCurFrame ^ 1u];
+    r.reservoirTemp = diReservoirTemp;
+    return r;
 }
 rt_frame Renderer::bindings() const {
     rt_frame f{};
@@ -133,5 +171,16 @@ void Renderer::drawFrame(hipStream_t stream, const rt_camera cams[2], Method met
         mNaiveDI.execute(stream, mScene, cams, f);
         mNaiveGI.execute(stream, mScene, cams, f);
     }
+    flip();
+}
+void Renderer::drawFrame(hipStream_t stream, const rt_camera cams[2], uint32_t directMethod, uint32_t indirectMethod) {
+    if (directMethod > MethodResampled || indirectMethod > MethodResampledPT || indirectMethod == MethodResampled)
+        throw std::invalid_argument("Renderer::drawFrame: unsupported direct or indirect method");
+    mGBuffer.render(stream, mCurFrame, mScene, cams[0]);
+    const rt_frame f = bindings();
+    if (directMethod == MethodNaive) mNaiveDI.execute(stream, mScene, cams, f);
+    else if (directMethod == MethodResampled) restirDI.render(stream, mScene, cams, f, diBindings());
+    if (indirectMethod == MethodNaive) mNaiveGI.execute(stream, mScene, cams, f);
+    else if (indirectMethod == MethodResampledPT) gris.render(stream, mScene, cams, f);
     flip();
 }

@@ -6,6 +6,9 @@
 //                and a push-constant block
 //   GRISReSTIR   (src/GRISReSTIR.h:7-37)    Settings, createPipeline(),
 //                render() = path trace, temporal reuse, spatial reuse
+//   TestReSTIR   (src/TestReSTIR.h:7-38)    the ReSTIR DI method: Settings,
+//                createPipeline(), render() = path generation, temporal
+//                reuse, spatial reuse
 //   Renderer     headless stand-in for src/Renderer.cpp's frame loop: owns the
 //                ray images and reservoirs, binds the frame parity and flips it
 #pragma once
@@ -62,6 +65,24 @@ private:
     RayTracing mPathTracePass, mTemporalReusePass, mSpatialReusePass;
 };
 
+class TestReSTIR {
+public:
+    enum ShiftType : uint32_t { Reconnection = 0, Replay = 1, Hybrid = 2 };
+    enum SampleType : uint32_t { Light = 0, BSDF = 1, Both = 2 };
+    using Settings = rt_di_settings;
+
+    void createPipeline();
+    // reservoirs: descriptor bindings 7/8/9 of the frame being rendered
+    void render(hipStream_t stream, const DeviceScene& scene, const rt_camera cams[2], const rt_frame& frame,
+                const rt_di_reservoirs& reservoirs);
+
+    Settings settings = {Reconnection, Light, 0u, 1u};  // src/TestReSTIR.h:31
+
+private:
+    RayTracing mPathTracePass, mTemporalReusePass, mSpatialReusePass;
+    rt_di_reservoirs mBound{};
+};
+
 // Owns the learned target function of a scene: network bounds are the scene
 // box padded by 10%.
 class LearnedTarget {
@@ -76,6 +97,9 @@ public:
 class Renderer {
 public:
     enum Method : uint32_t { ResampledPT = 0, Naive = 1 };
+    // Renderer::Settings directMethod / indirectMethod values (src/Renderer.h:25-31):
+    // direct None, Naive, ResampledDI; indirect None, Naive, ResampledGI (not built), ResampledPT
+    enum RayTracingMethod : uint32_t { MethodNone = 0, MethodNaive = 1, MethodResampled = 2, MethodResampledPT = 3 };
 
     Renderer(const DeviceScene& scene, uint32_t width, uint32_t height);
     ~Renderer();
@@ -84,17 +108,23 @@ public:
 
     // G-buffer pass, then the indirect (and for Naive also direct) method; flips the parity.
     void drawFrame(hipStream_t stream, const rt_camera cams[2], Method method);
+    // G-buffer pass, the direct method, the indirect method (src/Renderer.cpp:440-470); flips the parity.
+    void drawFrame(hipStream_t stream, const rt_camera cams[2], uint32_t directMethod, uint32_t indirectMethod);
     rt_frame bindings() const;
+    rt_di_reservoirs diBindings() const;
     void flip() { mCurFrame ^= 1u; }
-    // Resets the three reservoir buffers (the GUI's clearReservoir, src/Renderer.cpp:567).
+    // Resets the GRIS and DI reservoir buffers (the GUI's clearReservoir, src/Renderer.cpp:567).
     void resetReservoirs(hipStream_t stream);
 
     GRISReSTIR gris;
+    TestReSTIR restirDI;
     uint32_t maxDepthNaive = 15;
     float* directOutput = nullptr;
     float* indirectOutput = nullptr;
     void* grisReservoir[2] = {nullptr, nullptr};
     void* grisReservoirTemp = nullptr;
+    void* diReservoir[2] = {nullptr, nullptr};
+    void* diReservoirTemp = nullptr;
     uint32_t width, height;
 
 private:

@@ -120,6 +120,32 @@ rt_status rt_renderer_frame(rt_renderer* r, const rt_camera cams[2], uint32_t me
     }
 }
 
+rt_status rt_renderer_frame_methods(rt_renderer* r, const rt_camera cams[2], uint32_t directMethod,
+                                    uint32_t indirectMethod, hipStream_t stream) {
+    if (!r || !cams) return RT_ERR_INVALID_ARGUMENT;
+    if (directMethod > Renderer::MethodResampled || indirectMethod > Renderer::MethodResampledPT ||
+        indirectMethod == Renderer::MethodResampled)
+        return fail(RT_ERR_INVALID_ARGUMENT, "direct method must be 0..2 and indirect method 0, 1 or 3");
+    try {
+        r->r->drawFrame(stream, cams, directMethod, indirectMethod);
+        return RT_OK;
+    } catch (const std::exception& e) {
+        return fail(RT_ERR_HIP, e.what());
+    }
+}
+
+rt_status rt_renderer_set_di_settings(rt_renderer* r, const rt_di_settings* s) {
+    if (!r || !s || s->shiftType > 2 || s->sampleType > 2) return RT_ERR_INVALID_ARGUMENT;
+    r->r->restirDI.settings = *s;
+    return RT_OK;
+}
+
+rt_status rt_renderer_di_bindings(const rt_renderer* r, rt_di_reservoirs* out) {
+    if (!r || !out) return RT_ERR_INVALID_ARGUMENT;
+    *out = r->r->diBindings();
+    return RT_OK;
+}
+
 rt_status rt_renderer_bindings(const rt_renderer* r, rt_frame* out) {
     if (!r || !out) return RT_ERR_INVALID_ARGUMENT;
     *out = r->r->bindings();
