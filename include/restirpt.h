@@ -125,6 +125,14 @@ typedef struct rt_gris_settings {
      * radiance replaced by an rt_mlp prediction; needs rt_gris_learned_refresh
      * after each path-trace pass). */
     uint32_t targetFunction;
+    /* Resampling weights of the temporal and spatial merges: 0 = the reference's
+     * (each candidate counts its M, final weight divided by the total M,
+     * gris_resample_spatial.glsl:128; biased where a shift fails), 1 = unbiased
+     * (the total M is replaced by the sum of the M of the domains whose inverse
+     * shift of the picked sample is valid with a positive target -- the "1/Z"
+     * contribution weights; M-cap applied once per pass; at most 8 spatial
+     * neighbours). No counterpart in the reference. */
+    uint32_t unbiasedReuse;
 } rt_gris_settings;
 
 /* Descriptor set 2 (RayImageDescSet) of the reference: layouts.glsl:180-195,

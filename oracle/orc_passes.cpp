@@ -1,6 +1,7 @@
 // ORACLE — test infrastructure only. Nothing in the product links or calls this.
 // Transliteration of the per-pixel GLSL passes; see orc_passes.h for sources.
 #include "orc_passes.h"
+#include <vector>
 
 namespace orc {
 
@@ -282,7 +283,8 @@ vec3 tracePath(const Ctx& c, const Camera& cam, const GBufView& g, const GRISSet
                     pathSample.rcRng = rng;
                     pathSample.rcPrevSamplePdf = s.pdf;
                     pathSample.rcJacobian = geometryJacobian;
-                    setV(pathSample.rcLi, weightedLi);
+                    // unbiased reuse keeps the raw radiance (MIS re-evaluated at each reconnection)
+                    setV(pathSample.rcLi, set.unbiasedReuse ? surf.albedo : weightedLi);
                     setV(pathSample.rcWi, V3s(0.0f));
                     setV(pathSample.F, weightedLi * throughput);
                     flagsSetRcVertexId(pathSample.flags, bounce);
@@ -334,7 +336,7 @@ vec3 tracePath(const Ctx& c, const Camera& cam, const GBufView& g, const GRISSet
                     pathSample.rcRng = rng;
                     pathSample.rcPrevSamplePdf = lightPdf;
                     pathSample.rcJacobian = lightJacobian;
-                    setV(pathSample.rcLi, lightRadiance * weight);
+                    setV(pathSample.rcLi, set.unbiasedReuse ? lightRadiance : lightRadiance * weight);
                     setV(pathSample.rcWi, V3s(0.0f));
                     setV(pathSample.F, weightedLi * scatterTerm * throughput);
                     flagsSetRcVertexId(pathSample.flags, bounce + 1);
@@ -460,6 +462,16 @@ static vec3 reconnectLi(const GRISPathSample& srcSample, vec3 rcLi, const Materi
     Li = Li / srcSample.rcPrevSamplePdf;
     return Li;
 }
+// Unbiased reuse: MIS weight of a light reconnection vertex at the current
+// predecessor (light sampling vs the cosine pdf, gris_path_trace.glsl:203-204;
+// BSDF hit vs light sampling, :128-133); 1 for a surface vertex.
+static float rcLightMIS(const SceneArrays& S, uint32_t rcType, const SurfaceInfo& rcPrevSurf, vec3 rcPrevWo,
+                        const SurfaceInfo& rcSurf, vec3 wi, float geomJac) {
+    if (rcType == RcVertexTypeSurface) return 1.0f;
+    float lightPdf = luminance(rcSurf.albedo) / S.lightTable[0].prob / geomJac;
+    if (rcType == RcVertexTypeLightSampled) return MISWeight(lightPdf, absDot(rcPrevSurf.norm, wi) * PiInv);
+    return MISWeight(evalPdf(S.materials[rcPrevSurf.matIndex], rcPrevSurf.norm, rcPrevWo, wi), lightPdf);
+}
 // Learned target: a surface rc vertex's radiance is replaced by the prediction.
 static vec3 targetRcLi(const GRISPathSample& ps, const LearnedData& L) {
     return flagsRcVertexType(ps.flags) == RcVertexTypeSurface ? V3(L.L[0], L.L[1], L.L[2]) : L3(ps.rcLi);
@@ -470,7 +482,7 @@ static vec3 targetRcLi(const GRISPathSample& ps, const LearnedData& L) {
 // values and the picked sample's learned data moves with it.
 static void reuseAndMerge(const Ctx& c, const GRISSettings& set, GRISReservoir& dstResv, LearnedData* dstL,
                           const SurfaceInfo& dstPrimarySurf, const Intersection& dstPrimaryIsec, const Ray& primaryRay,
-                          GRISReservoir srcResv, LearnedData srcL, bool learned, uint32_t& rng) {
+                          GRISReservoir srcResv, LearnedData srcL, bool learned, uint32_t& rng, bool unbiased = false) {
     const SceneArrays& S = *c.S;
     GRISPathSample srcSample = srcResv.pathSample;
     RcData dstRc;
@@ -505,8 +517,12 @@ static void reuseAndMerge(const Ctx& c, const GRISSettings& set, GRISReservoir& 
         uint32_t rcType = flagsRcVertexType(srcSample.flags);
         vec3 LiL = V3s(0.0f);
         if (!std::isnan(srcSample.rcPrevSamplePdf) && srcSample.rcPrevSamplePdf > 1e-6f) {
-            Li = reconnectLi(srcSample, L3(srcSample.rcLi), rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, dstRc);
-            if (learned) LiL = reconnectLi(srcSample, targetRcLi(srcSample, srcL), rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, dstRc);
+            const float misW = unbiased ? rcLightMIS(S, rcType, rcPrevSurf, dstRc.rcPrevWo, rcSurf, wi, dstJacobian) : 1.0f;
+            Li = reconnectLi(srcSample, unbiased ? L3(srcSample.rcLi) * misW : L3(srcSample.rcLi), rcMat, rcSurf, rcPrevMat,
+                             rcPrevSurf, wi, dstRc);
+            if (learned)
+                LiL = reconnectLi(srcSample, unbiased ? targetRcLi(srcSample, srcL) * misW : targetRcLi(srcSample, srcL),
+                                  rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, dstRc);
             if (!isBlack(Li) && !hasNan(Li)) dstPHat = luminance((learned ? LiL : Li) * jacobian);
             if (rcType == RcVertexTypeLightSampled) {
                 float sumPower = S.lightTable[0].prob;
@@ -516,6 +532,13 @@ static void reuseAndMerge(const Ctx& c, const GRISSettings& set, GRISReservoir& 
             }
         }
         float srcPHat = learned ? srcL.pRef : luminance(L3(srcSample.F));
+        if (unbiased) {
+            // keep F consistent with the new reconnection pdf; reject a zero-pdf reconnection
+            const float r = (dstSamplePdf > 1e-6f) ? srcSample.rcPrevSamplePdf / dstSamplePdf : 0.0f;
+            if (r == 0.0f) dstPHat = 0.0f;
+            Li = Li * r;
+            LiL = LiL * r;
+        }
         if (learned) srcL.pRef = luminance(LiL);
         srcSample.rcJacobian = dstJacobian;
         srcSample.rcPrevSamplePdf = dstSamplePdf;
@@ -527,7 +550,69 @@ static void reuseAndMerge(const Ctx& c, const GRISSettings& set, GRISReservoir& 
         srcResv.resampleWeight = 0.0f;
     }
     if (reservoirIsValid(srcResv) && reservoirMerge(dstResv, srcResv, sample1f(rng)) && dstL) *dstL = srcL;
-    reservoirCap(dstResv, (float)set.cap);
+    if (!unbiased) reservoirCap(dstResv, (float)set.cap);
+}
+
+// ---- unbiased reuse mode (no reference counterpart) ------------------------------
+// The "1/Z" contribution weights of unbiased ReSTIR (Bitterli et al. 2020, Sec. 4) for
+// the shift-mapped GRIS merges (Lin et al. 2022): after the merges the final weight
+// divides by Z = sum of the M of the domains that could have produced the picked
+// sample y instead of the total M. Domain j supports y when the hybrid shift of y
+// onto j's primary vertex is valid under reuseAndMerge's conditions and the shifted
+// sample's own contribution there (with its pdf at the reconnection predecessor)
+// has a positive target value.
+static bool shiftSupports(const Ctx& c, const GRISSettings& set, const GRISPathSample& y, const LearnedData& yL,
+                          bool learned, const SurfaceInfo& primSurf, const Intersection& primIsec, const Ray& primRay) {
+    const SceneArrays& S = *c.S;
+    if (!pathSampleIsValid(y)) return false;
+    RcData rc;
+    traceReplayPathForHybridShift(c, set, primIsec, primSurf, primRay, y.flags, y.primaryRng, rc);
+    if (rc.rcPrevIsec.instanceIdx == InvalidHitIndex) return false;
+    SurfaceInfo rcPrevSurf{}, rcSurf{};
+    if (rc.rcPrevIsec.instanceIdx == SpecialHitIndex) rcPrevSurf = primSurf;
+    else loadSurfaceInfo(S, rc.rcPrevIsec, rcPrevSurf);
+    loadSurfaceInfo(S, y.rcIsec, rcSurf);
+    const Material& rcMat = S.materials[rcSurf.matIndex];
+    const Material& rcPrevMat = S.materials[rcPrevSurf.matIndex];
+    float dist = distance(rcPrevSurf.pos, rcSurf.pos);
+    vec3 wi = normalize(rcSurf.pos - rcPrevSurf.pos);
+    float cosTheta = -dot(rcSurf.norm, wi);
+    float jac = std::fabs(cosTheta) / square(dist);
+    if (!(dist > GRISDistanceThreshold && cosTheta > 0.0f && jac > 0.0f && !std::isnan(jac) && isBSDFConnectible(rcPrevMat)))
+        return false;
+    if (!traceVisibility(c, rcPrevSurf.pos, rcSurf.pos)) return false;
+    GRISPathSample x = y;
+    x.rcPrevSamplePdf = flagsRcVertexType(y.flags) == RcVertexTypeLightSampled
+                            ? luminance(rcSurf.albedo) / S.lightTable[0].prob / jac
+                            : evalPdf(rcPrevMat, rcPrevSurf.norm, rc.rcPrevWo, wi);
+    if (std::isnan(x.rcPrevSamplePdf) || !(x.rcPrevSamplePdf > 1e-6f)) return false;
+    const float misW = rcLightMIS(S, flagsRcVertexType(y.flags), rcPrevSurf, rc.rcPrevWo, rcSurf, wi, jac);
+    vec3 Li = reconnectLi(x, L3(x.rcLi) * misW, rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, rc);
+    // x is a candidate of that domain iff its resampling weight luminance(F) is positive
+    if (hasNan(Li) || !(luminance(Li) > 0.0f)) return false;
+    if (learned)
+        return luminance(reconnectLi(x, targetRcLi(x, yL) * misW, rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, rc)) > 0.0f;
+    return true;
+}
+
+// Primary vertex of pixel (x, y) as its own passes rebuilt it; false for background.
+static bool primaryOf(const Camera& cam, const GBufView& g, int x, int y, SurfaceInfo& surf, Intersection& isec,
+                      Ray& ray) {
+    if (x < 0 || y < 0 || x >= (int)g.W || y >= (int)g.H) return false;
+    vec2 uv = V2(((float)x + 0.5f) / (float)g.W, ((float)y + 0.5f) / (float)g.H);
+    float depth; vec3 norm, albedo; int matMeshId;
+    uint32_t am[2]; fetchAlbedoMatId(g, x, y, am);
+    if (!unpackGBuffer(texDepthNormal(g, uv), am, depth, norm, albedo, matMeshId)) return false;
+    ray = pinholeCameraSampleRay(cam, V2(uv.x, 1.0f - uv.y), V2(0, 0));
+    surf = SurfaceInfo{ray.ori + ray.dir * (depth - 1e-4f), norm, albedo, (uint32_t)(matMeshId >> 16), false};
+    isec = Intersection{};
+    isec.instanceIdx = SpecialHitIndex;
+    isec.bary[0] = uv.x; isec.bary[1] = uv.y;
+    return true;
+}
+
+static void normaliseZ(GRISReservoir& r, float z) {
+    if (z > 0.0f) r.resampleWeight = r.resampleWeight * (r.sampleCount / z);
 }
 
 // Index of a reservoir read, or -1 where the read is out of range (reads zero).
@@ -592,6 +677,10 @@ bool temporalReuse(const Ctx& c, const Camera& cam, const Camera& prevCam, const
     Intersection dstPrimaryIsec{};
     dstPrimaryIsec.instanceIdx = SpecialHitIndex;
     dstPrimaryIsec.bary[0] = uv.x; dstPrimaryIsec.bary[1] = uv.y;
+    const bool unbiased = set.unbiasedReuse != 0u;
+    const float mCanon = resv.sampleCount;
+    float mPrev = 0.0f;
+    int prevX = -1, prevY = -1;
     if (set.temporalReuse) {
         GRISReservoir temporalResv; std::memset(&temporalResv, 0, sizeof(temporalResv));
         reservoirReset(temporalResv);
@@ -601,9 +690,24 @@ bool temporalReuse(const Ctx& c, const Camera& cam, const Camera& prevCam, const
             findPreviousReservoir(cam, prevCam, gPrev, resvPrev, uv + motion, pos, norm, matMeshId, temporalResv, pidx)) {
             if (reservoirIsValid(temporalResv)) {
                 reuseAndMerge(c, set, resv, learned ? &L : nullptr, dstPrimarySurf, dstPrimaryIsec, ray, temporalResv,
-                              learnedAt(learned ? learnPrev : nullptr, pidx), learned, resvRng);
+                              learnedAt(learned ? learnPrev : nullptr, pidx), learned, resvRng, unbiased);
+                mPrev = temporalResv.sampleCount;
+                vec2 puv = uv + motion;
+                prevX = (int)(puv.x * (float)cam.filmSize[0]);
+                prevY = (int)(puv.y * (float)cam.filmSize[1]);
             }
         }
+    }
+    if (unbiased) {
+        if (mPrev > 0.0f && reservoirIsValid(resv) && pathSampleIsValid(resv.pathSample)) {
+            float z = mCanon;
+            SurfaceInfo ps{}; Intersection pi{}; Ray pr{};
+            if (primaryOf(prevCam, gPrev, prevX, prevY, ps, pi, pr) &&
+                shiftSupports(c, set, resv.pathSample, L, learned, ps, pi, pr))
+                z += mPrev;
+            normaliseZ(resv, z);
+        }
+        reservoirCap(resv, (float)set.cap);
     }
     if (!reservoirIsValid(resv)) reservoirReset(resv);
     outTemp = resv;
@@ -650,6 +754,10 @@ vec3 spatialReuse(const Ctx& c, const Camera& cam, const GBufView& g, const GRIS
     dstPrimaryIsec.instanceIdx = SpecialHitIndex;
     dstPrimaryIsec.bary[0] = uv.x; dstPrimaryIsec.bary[1] = uv.y;
     SurfaceInfo dstPrimarySurf{pos, norm, albedo, (uint32_t)matId, false};
+    const bool unbiased = set.unbiasedReuse != 0u;
+    const float mCanon = resv.sampleCount;
+    std::vector<std::pair<int, int>> nbPix;
+    std::vector<float> nbM;
     if (set.spatialReuse) {
         vec2 texelSize = V2(1.0f / (float)g.W, 1.0f / (float)g.H);
         for (uint32_t i = 0; i < set.spatialNeighbors; i++) {
@@ -661,10 +769,29 @@ vec3 spatialReuse(const Ctx& c, const Camera& cam, const GBufView& g, const GRIS
             if (findNeighborReservoir(cam, g, resvTemp, neighbor, pos, norm, neighborResv, nidx)) {
                 if (reservoirIsValid(neighborResv)) {
                     reuseAndMerge(c, set, resv, learned ? &L : nullptr, dstPrimarySurf, dstPrimaryIsec, ray, neighborResv,
-                                  learnedAt(learned ? learnTemp : nullptr, nidx), learned, rng);
+                                  learnedAt(learned ? learnTemp : nullptr, nidx), learned, rng, unbiased);
+                    if (unbiased && nbPix.size() < UnbiasedMaxNeighbors) {
+                        nbPix.emplace_back((int)(neighbor.x * (float)cam.filmSize[0]),
+                                           (int)(neighbor.y * (float)cam.filmSize[1]));
+                        nbM.push_back(neighborResv.sampleCount);
+                    }
                 }
             }
         }
+    }
+    if (unbiased) {
+        if (!nbPix.empty() && reservoirIsValid(resv) && pathSampleIsValid(resv.pathSample)) {
+            float z = mCanon;
+            for (size_t j = 0; j < nbPix.size(); j++) {
+                if (!(nbM[j] > 0.0f)) continue;
+                SurfaceInfo ns{}; Intersection ni{}; Ray nr{};
+                if (primaryOf(cam, g, nbPix[j].first, nbPix[j].second, ns, ni, nr) &&
+                    shiftSupports(c, set, resv.pathSample, L, learned, ns, ni, nr))
+                    z += nbM[j];
+            }
+            normaliseZ(resv, z);
+        }
+        reservoirCap(resv, (float)set.cap);
     }
     if (!reservoirIsValid(resv)) reservoirReset(resv);
     outThis = resv;
@@ -683,10 +810,16 @@ vec3 spatialReuse(const Ctx& c, const Camera& cam, const GBufView& g, const GRIS
                 const Material& rcMat = S.materials[rcSurf.matIndex];
                 const Material& rcPrevMat = S.materials[rcPrevSurf.matIndex];
                 vec3 wi = normalize(rcSurf.pos - rcPrevSurf.pos);
-                vec3 Li = reconnectLi(ps, L3(ps.rcLi), rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, rc);
-                float pHat = learned
-                                 ? luminance(reconnectLi(ps, targetRcLi(ps, L), rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, rc))
-                                 : luminance(Li);
+                float misW = 1.0f;
+                if (unbiased) {
+                    float gj = std::fabs(dot(rcSurf.norm, wi)) / square(distance(rcPrevSurf.pos, rcSurf.pos));
+                    misW = rcLightMIS(S, flagsRcVertexType(ps.flags), rcPrevSurf, rc.rcPrevWo, rcSurf, wi, gj);
+                }
+                vec3 Li = reconnectLi(ps, unbiased ? L3(ps.rcLi) * misW : L3(ps.rcLi), rcMat, rcSurf, rcPrevMat, rcPrevSurf,
+                                      wi, rc);
+                float pHat = learned ? luminance(reconnectLi(ps, unbiased ? targetRcLi(ps, L) * misW : targetRcLi(ps, L),
+                                                             rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, rc))
+                                     : luminance(Li);
                 if (!isBlack(Li) && !hasNan(Li)) {
                     radiance = Li / pHat * resv.resampleWeight / resv.sampleCount;
                 }

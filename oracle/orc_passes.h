@@ -25,7 +25,9 @@ struct GRISSettings {
     uint32_t spatialNeighbors;  // ResampleNum (gris_resample_spatial.glsl:65, 3)
     float spatialRadius;        // ResampleRadius (gris_resample_spatial.glsl:66, 20)
     uint32_t targetFunction;    // 0 luminance (GRISToScalar), 1 learned
+    uint32_t unbiasedReuse;     // 0 the reference's 1/M weights, 1 the "1/Z" contribution weights
 };
+static const uint32_t UnbiasedMaxNeighbors = 8;
 // Learned-target data of a reservoir: predicted reconnection radiance and the
 // target value of the sample that resampleWeight refers to.
 struct LearnedData { float L[3]; float pRef; };

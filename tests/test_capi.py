@@ -38,10 +38,11 @@ def test_struct_sizes_match_reference_layouts():
     assert C.sizeof(rt.ObjectInstance) == 224
     assert C.sizeof(rt.TriangleLight) == 64
     assert C.sizeof(rt.Material) == 32 and C.sizeof(rt.MeshVertex) == 32
-    assert C.sizeof(rt.GrisSettings) == 36     # 20-B push constant + 3 kernel constants + target function
+    # 20-B push constant + 3 kernel constants + target function + resampling-weight mode
+    assert C.sizeof(rt.GrisSettings) == 40
     assert C.sizeof(rt.MlpConfig) == 72
-    # rt_view: scene pointer + 2 cameras + rt_frame (88 B) + settings (36 B), padded to 8
-    assert C.sizeof(rt.Frame) == 88 and C.sizeof(rt.View) == 8 + 2 * 352 + 88 + 36 + 4
+    # rt_view: scene pointer + 2 cameras + rt_frame (88 B) + settings (40 B)
+    assert C.sizeof(rt.Frame) == 88 and C.sizeof(rt.View) == 8 + 2 * 352 + 88 + 40
     assert C.sizeof(rt.BatchTuning) == 8
     assert C.sizeof(rt.DIReservoir) == 64      # layouts.glsl DIReservoir (std430)
     assert C.sizeof(rt.DISettings) == 16       # TestReSTIR::Settings push constant

@@ -30,16 +30,18 @@ def bits_equal(a, b):
     return np.array_equal(np.ascontiguousarray(a).view(np.uint32), np.ascontiguousarray(b).view(np.uint32))
 
 
-@pytest.mark.parametrize("config,W,H", [(0, 96, 64), (1, 160, 96)], ids=["boxroom", "terrain"])
-def test_learned_gris_parity(config, W, H):
-    print(f"[gpu] learned parity config {config} {W}x{H}", flush=True)
+@pytest.mark.parametrize("config,W,H,unbiased", [(0, 96, 64, False), (1, 160, 96, False), (1, 160, 96, True)],
+                         ids=["boxroom", "terrain", "terrain-unbiased"])
+def test_learned_gris_parity(config, W, H, unbiased):
+    print(f"[gpu] learned parity config {config} {W}x{H} unbiased {unbiased}", flush=True)
     hs = rt.HostScene(config, 1)
     ds = rt.DeviceScene(hs.arrays)
     osc = orc.OracleScene(hs.arrays)
     lo, hi = scene_bounds(ds)
     mlp = rt.Mlp(rt.mlp_config(lo, hi, log2_table=16, seed=3))
     omlp = orc.OracleMlp(16, 16, 2048, lo, hi)
-    s = rt.settings(shift=2, temporal=True, spatial=True, max_depth=6, cap=20, radius=20.0, learned=True)
+    s = rt.settings(shift=2, temporal=True, spatial=True, max_depth=6, cap=20, radius=20.0, learned=True,
+                    unbiased=unbiased)
     run = gpuhelp.GrisRun(ds, W, H)
     n = W * H
     o_resv = [orc.reset_reservoirs(n), orc.reset_reservoirs(n)]

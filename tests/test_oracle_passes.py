@@ -124,3 +124,44 @@ def test_reservoir_cap_and_clear_flag():
         assert np.isfinite(out).all() and (out[..., :3] >= 0).all()
         prev_dn, prev_am = dn, am
         cur ^= 1
+
+
+@pytest.mark.parametrize("unbiased", [False, True])
+def test_self_reuse_weights(unbiased):
+    """Spatial reuse with a zero radius merges each pixel's own reservoir again (the identity shift).
+    With the unbiased weights the shaded image equals the no-reuse image pixel by pixel (the two
+    copies of the sample are averaged over Z = 2, or a copy whose shift fails is not counted); the
+    reference's 1/M weights halve the pixels whose identity shift fails (darkening)."""
+    hs = rt.HostScene(0, 1)
+    osc = orc.OracleScene(hs.arrays)
+    W, H = 64, 48
+    n = W * H
+    base = hs.camera(0, W, H)[0]
+    cams = (rt.Camera * 2)()
+    ctypes.memmove(ctypes.addressof(cams), ctypes.addressof(base), ctypes.sizeof(rt.Camera))
+    cams[0].frameIndex = 0x80000000
+    ctypes.memmove(ctypes.addressof(cams) + ctypes.sizeof(rt.Camera), ctypes.addressof(cams), ctypes.sizeof(rt.Camera))
+    dn, am, mv, _ = osc.gbuffer(cams[0])
+
+    def render(s):
+        r = orc.reset_reservoirs(n)
+        t = orc.reset_reservoirs(n)
+        out = np.zeros((H, W, 4), np.float32)
+        osc.gris_path_trace(cams, dn, am, s, r)
+        osc.gris_temporal(cams, dn, am, mv, dn, am, s, r, orc.reset_reservoirs(n), t)
+        osc.gris_spatial(cams, dn, am, s, t, r, out)
+        return out[..., :3].astype(np.float64), r
+
+    none, _ = render(rt.settings(shift=2, max_depth=4, temporal=False, spatial=False, unbiased=unbiased))
+    self_, r = render(rt.settings(shift=2, max_depth=4, temporal=False, spatial=True, radius=0.0, neighbors=3,
+                                  unbiased=unbiased))
+    lit = none.sum(-1) > 1e-3
+    assert lit.mean() > 0.3
+    ratio = self_.sum(-1)[lit] / none.sum(-1)[lit]
+    assert (r["sampleCount"][dn.reshape(n, 4)[:, 0] > 0] == 4).all()
+    if unbiased:
+        assert np.abs(ratio - 1.0).max() < 1e-3
+    else:
+        halved = np.abs(ratio - 0.25) < 1e-3  # all three self-shifts failed: 1/4 of the weight survives
+        assert np.abs(ratio[~halved] - 1.0).max() < 1e-3
+        assert halved.any()

@@ -57,6 +57,7 @@ rt_status checkView(const rt_view& v) {
     if (s.maxDepth == 0 || s.maxDepth > 255 || s.shiftType > 2 || s.spatialNeighbors > 64)
         return invalid("invalid view settings");
     if (s.targetFunction != 0u) return invalid("batched views use the luminance target");
+    if (s.unbiasedReuse != 0u) return invalid("batched views use the reference's resampling weights");
     return RT_OK;
 }
 

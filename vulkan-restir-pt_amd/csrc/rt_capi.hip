@@ -125,6 +125,9 @@ rt_status checkSettings(const rt_gris_settings* st) {
     if (st->maxDepth == 0 || st->maxDepth > 255) return invalid("maxDepth must be in [1, 255]");
     if (st->shiftType > 2) return invalid("shiftType must be 0, 1 or 2");
     if (st->spatialNeighbors > 64) return invalid("spatialNeighbors must be <= 64");
+    if (st->unbiasedReuse > 1) return invalid("unbiasedReuse must be 0 or 1");
+    if (st->unbiasedReuse && st->spatialNeighbors > 8)
+        return invalid("unbiasedReuse supports at most 8 spatialNeighbors");
     if (st->targetFunction > 1) return invalid("targetFunction must be 0 (luminance) or 1 (learned)");
     return RT_OK;
 }

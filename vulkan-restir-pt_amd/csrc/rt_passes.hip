@@ -455,7 +455,9 @@ RT_DEV bool ptVertex(const DevScene& S, TStack st, float* slot, const rt_gris_se
                     ps.rcRng = rng;
                     ps.rcPrevSamplePdf = s.pdf;
                     ps.rcJacobian = geometryJacobian;
-                    ps.rcLi = weightedLi;
+                    // unbiased reuse: the MIS weight depends on the predecessor and is re-evaluated
+                    // wherever the sample is reconnected, so the vertex keeps the raw radiance
+                    ps.rcLi = set.unbiasedReuse ? surf.albedo : weightedLi;
                     ps.rcWi = s3(0.0f);
                     ps.F = weightedLi * throughput;
                     flagsSetRcVertexId(ps.flags, bounce);
@@ -507,7 +509,7 @@ RT_DEV bool ptVertex(const DevScene& S, TStack st, float* slot, const rt_gris_se
                     ps.rcRng = rng;
                     ps.rcPrevSamplePdf = lightPdf;
                     ps.rcJacobian = lightJac;
-                    ps.rcLi = Lr * w;
+                    ps.rcLi = set.unbiasedReuse ? Lr : Lr * w;
                     ps.rcWi = s3(0.0f);
                     ps.F = weightedLi * scatterTerm * throughput;
                     flagsSetRcVertexId(ps.flags, bounce + 1);
@@ -716,10 +718,27 @@ RT_DEV f3 reconnectLi(const DevScene& S, const Resv& src, f3 rcLi, const Surf& r
 // exact emitted radiance.
 RT_DEV f3 targetRcLi(const Resv& r) { return flagsRcVertexType(r.flags) == RC_SURFACE ? r.Lhat : r.rcLi; }
 
+// Unbiased reuse: MIS weight of a light reconnection vertex at the current
+// predecessor (the path tracer then stores the vertex's raw radiance): light
+// sampling against the cosine pdf (gris_path_trace.glsl:203-204) or the BSDF hit
+// against light sampling (:128-133). 1 for a surface vertex.
+RT_DEV float rcLightMIS(const DevScene& S, uint32_t rcType, const Surf& rcPrevSurf, f3 rcPrevWo, const Surf& rcSurf,
+                        f3 wi, float geomJac) {
+    if (rcType == RC_SURFACE) return 1.0f;
+    const float lightPdf = luminance(rcSurf.albedo) / S.lightTable[0].prob / geomJac;
+    if (rcType == RC_LIGHT_SAMPLED) return MISWeight(lightPdf, absDot(rcPrevSurf.norm, wi) * RT_PI_INV);
+    return MISWeight(evalPdf(loadMat(S, rcPrevSurf.matIndex), rcPrevSurf.norm, rcPrevWo, wi), lightPdf);
+}
+
 // gris_retrace.glsl:138-236. With kLearned the target function at the sites
 // :204 (dstPHat) and :215 (srcPHat) is the learned one; the sample's target
 // value in its new pixel travels with it as pRef.
-template <bool kCount, bool kLearned = false>
+// kUnbiased (the unbiased reuse mode, no reference counterpart): the merged
+// sample keeps F consistent with its new reconnection pdf (F is rescaled by
+// pdf_src / pdf_dst, so the next shift's target ratio is exact, as it is for a
+// fresh sample), a shift to a zero-pdf reconnection is rejected, and the M-cap
+// is left to the caller (applied once per pass after the 1/Z normalisation).
+template <bool kCount, bool kLearned = false, bool kUnbiased = false>
 RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& set, Resv& dst, const Surf& dstPrimarySurf,
                           const Isec& dstPrimaryIsec, f3 primDir, f3 primOri, Resv src, uint32_t& rng, TraceCounters* ctr) {
     RcData rc;
@@ -749,8 +768,10 @@ RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& 
         uint32_t rcType = flagsRcVertexType(src.flags);
         f3 LiL = s3(0.0f);
         if (!isnan(src.rcPrevSamplePdf) && src.rcPrevSamplePdf > 1e-6f) {
-            Li = reconnectLi(S, src, src.rcLi, rcPrevSurf, rcSurf, wi, rc);
-            if (kLearned) LiL = reconnectLi(S, src, targetRcLi(src), rcPrevSurf, rcSurf, wi, rc);
+            const float misW = kUnbiased ? rcLightMIS(S, rcType, rcPrevSurf, rc.rcPrevWo, rcSurf, wi, dstJacobian) : 1.0f;
+            Li = reconnectLi(S, src, kUnbiased ? src.rcLi * misW : src.rcLi, rcPrevSurf, rcSurf, wi, rc);
+            if (kLearned)
+                LiL = reconnectLi(S, src, kUnbiased ? targetRcLi(src) * misW : targetRcLi(src), rcPrevSurf, rcSurf, wi, rc);
             if (!isBlack(Li) && !hasNan(Li)) dstPHat = luminance((kLearned ? LiL : Li) * jacobian);
             if (rcType == RC_LIGHT_SAMPLED) {
                 float sumPower = S.lightTable[0].prob;
@@ -761,6 +782,12 @@ RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& 
             }
         }
         float srcPHat = kLearned ? src.pRef : luminance(src.F);
+        if (kUnbiased) {
+            const float r = (dstSamplePdf > 1e-6f) ? src.rcPrevSamplePdf / dstSamplePdf : 0.0f;
+            if (r == 0.0f) dstPHat = 0.0f;
+            Li = Li * r;
+            LiL = LiL * r;
+        }
         if (kLearned) src.pRef = luminance(LiL);
         src.rcJacobian = dstJacobian;
         src.rcPrevSamplePdf = dstSamplePdf;
@@ -771,7 +798,65 @@ RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& 
         src.resampleWeight = 0.0f;
     }
     if (resvIsValid(src)) resvMerge(dst, src, sample1f(rng));
-    resvCap(dst, (float)set.cap);
+    if (!kUnbiased) resvCap(dst, (float)set.cap);
+}
+
+// Unbiased reuse: can the domain whose primary vertex is (surf, isec, ray) have
+// produced the picked sample y? True when the hybrid shift of y onto that
+// vertex is valid under the conditions of reuseAndMerge (replay to the
+// reconnection predecessor, connectible, distance threshold, front-facing,
+// visible) and the shifted sample's own contribution there -- with its pdf at
+// that predecessor -- has a positive target value.
+template <bool kCount, bool kLearned>
+RT_DEV bool shiftSupports(const DevScene& S, TStack st, const rt_gris_settings& set, const Resv& y, const Surf& primSurf,
+                          const Isec& primIsec, f3 primDir, f3 primOri, TraceCounters* ctr) {
+    if (y.rcIsec.inst == RT_INVALID_HIT) return false;
+    RcData rc;
+    replayHybrid<kCount>(S, st, set, primIsec, primSurf, primDir, primOri, y.flags, y.primaryRng, rc, ctr);
+    if (rc.rcPrevIsec.inst == RT_INVALID_HIT) return false;
+    Surf rcPrevSurf, rcSurf;
+    if (rc.rcPrevIsec.inst == RT_SPECIAL_HIT) rcPrevSurf = primSurf;
+    else loadSurfaceInfo(S, rc.rcPrevIsec, rcPrevSurf);
+    loadSurfaceInfo(S, y.rcIsec, rcSurf);
+    const Mat rcPrevMat = loadMat(S, rcPrevSurf.matIndex);
+    const float dist = distance(rcPrevSurf.pos, rcSurf.pos);
+    const f3 wi = normalize(rcSurf.pos - rcPrevSurf.pos);
+    const float cosTheta = -dot(rcSurf.norm, wi);
+    const float jac = fabsf(cosTheta) / square(dist);
+    if (!(dist > GRIS_DIST_THRESHOLD && cosTheta > 0.0f && jac > 0.0f && !isnan(jac) && isBSDFConnectible(rcPrevMat)))
+        return false;
+    if (!traceVisibility<kCount>(S, st, rcPrevSurf.pos, rcSurf.pos, ctr)) return false;
+    Resv x = y;
+    x.rcPrevSamplePdf = flagsRcVertexType(y.flags) == RC_LIGHT_SAMPLED
+                            ? luminance(rcSurf.albedo) / S.lightTable[0].prob / jac
+                            : evalPdf(rcPrevMat, rcPrevSurf.norm, rc.rcPrevWo, wi);
+    if (isnan(x.rcPrevSamplePdf) || !(x.rcPrevSamplePdf > 1e-6f)) return false;
+    const float misW = rcLightMIS(S, flagsRcVertexType(y.flags), rcPrevSurf, rc.rcPrevWo, rcSurf, wi, jac);
+    const f3 Li = reconnectLi(S, x, x.rcLi * misW, rcPrevSurf, rcSurf, wi, rc);
+    // x is a candidate of that domain iff its resampling weight luminance(F) is positive
+    if (hasNan(Li) || !(luminance(Li) > 0.0f)) return false;
+    if (kLearned) return luminance(reconnectLi(S, x, targetRcLi(x) * misW, rcPrevSurf, rcSurf, wi, rc)) > 0.0f;
+    return true;
+}
+
+// Primary vertex of pixel (x, y) as its own passes rebuilt it (texel-centre
+// G-buffer, camera ray through the pixel centre); false for background.
+RT_DEV bool primaryOf(const rt_camera& cam, const GB& g, int x, int y, Surf& surf, Isec& isec, CamRay& ray) {
+    if (x < 0 || y < 0 || x >= (int)g.W || y >= (int)g.H) return false;
+    GPix gp;
+    if (!unpackGB(texDNCentre(g, (uint32_t)x, (uint32_t)y), fetchAM(g, x, y), gp)) return false;
+    const f2 uv = pixelUV((uint32_t)x, (uint32_t)y, g.W, g.H);
+    ray = pinholeRay(cam, mk2(uv.x, 1.0f - uv.y));
+    surf.pos = ray.ori + ray.dir * (gp.depth - 1e-4f);
+    surf.norm = gp.norm; surf.albedo = gp.albedo; surf.matIndex = (uint32_t)(gp.matMeshId >> 16); surf.isLight = false;
+    isec.inst = RT_SPECIAL_HIT; isec.b1 = uv.x; isec.b2 = uv.y; isec.prim = 0u;
+    return true;
+}
+
+// 1/Z normalisation of a merged reservoir: resampleWeight * M / Z, so that the
+// final weight resampleWeight / M equals the unnormalised sum over Z.
+RT_DEV void normaliseZ(Resv& r, float z) {
+    if (z > 0.0f) r.resampleWeight = r.resampleWeight * (r.sampleCount / z);
 }
 
 template <bool kLearned>
@@ -782,7 +867,7 @@ RT_DEV Resv resvAt(const ResvPlanes& R, uint32_t W, uint32_t H, int x, int y) {
 }
 
 // gris_resample_temporal.glsl:11-83
-template <bool kCount, bool kLearned>
+template <bool kCount, bool kLearned, bool kUnbiased = false>
 RT_DEV void temporalPixel(const DevScene& S, const rt_camera& cam, const rt_camera& prevCam, const GB& g, const GB& gPrev,
                           const rt_gris_settings& set, const ResvPlanes& thisR, const ResvPlanes& prevR,
                           const ResvPlanes& tempR, uint32_t x, uint32_t y, TStack st, TraceCounters* ctr) {
@@ -800,6 +885,9 @@ RT_DEV void temporalPixel(const DevScene& S, const rt_camera& cam, const rt_came
     Resv resv = loadResv<kLearned>(thisR, p);
     Isec dstPrimaryIsec; dstPrimaryIsec.inst = RT_SPECIAL_HIT; dstPrimaryIsec.b1 = uv.x; dstPrimaryIsec.b2 = uv.y;
     dstPrimaryIsec.prim = 0u;
+    const float mCanon = resv.sampleCount;
+    float mPrev = 0.0f;
+    int prevX = -1, prevY = -1;
     if (set.temporalReuse) {
         Surf dstPrimarySurf; dstPrimarySurf.pos = pos; dstPrimarySurf.norm = gp.norm; dstPrimarySurf.albedo = gp.albedo;
         dstPrimarySurf.matIndex = (uint32_t)matId; dstPrimarySurf.isLight = false;
@@ -816,29 +904,42 @@ RT_DEV void temporalPixel(const DevScene& S, const rt_camera& cam, const rt_came
                     f3 posPrev = pr.ori + pr.dir * (pp.depth - 1e-4f);
                     if (!(pp.matMeshId != gp.matMeshId || dot(pp.norm, gp.norm) < 0.95f || distance(pos, posPrev) > 0.5f)) {
                         Resv tr = resvAt<kLearned>(prevR, cam.filmSize[0], cam.filmSize[1], px, py);
-                        if (resvIsValid(tr))
-                            reuseAndMerge<kCount, kLearned>(S, st, set, resv, dstPrimarySurf, dstPrimaryIsec, ray.dir, ray.ori, tr,
-                                                  resvRng, ctr);
+                        if (resvIsValid(tr)) {
+                            reuseAndMerge<kCount, kLearned, kUnbiased>(S, st, set, resv, dstPrimarySurf, dstPrimaryIsec,
+                                                                        ray.dir, ray.ori, tr, resvRng, ctr);
+                            mPrev = tr.sampleCount; prevX = px; prevY = py;
+                        }
                     }
                 }
             }
         }
     }
+    if (kUnbiased) {
+        if (mPrev > 0.0f && resvIsValid(resv) && resv.rcIsec.inst != RT_INVALID_HIT) {
+            float z = mCanon;
+            Surf ps; Isec pi; CamRay pr;
+            if (primaryOf(prevCam, gPrev, prevX, prevY, ps, pi, pr) &&
+                shiftSupports<kCount, kLearned>(S, st, set, resv, ps, pi, pr.dir, pr.ori, ctr))
+                z += mPrev;
+            normaliseZ(resv, z);
+        }
+        resvCap(resv, (float)set.cap);
+    }
     if (!resvIsValid(resv)) resvReset(resv);
     storeResv<kLearned>(tempR, p, resv);
 }
-template <bool kCount, bool kLearned>
+template <bool kCount, bool kLearned, bool kUnbiased>
 __global__ __launch_bounds__(kTileThreads, RT_WAVES_TEMPORAL) void k_gris_temporal(DevScene S, rt_camera cam, rt_camera prevCam, GB g,
                                                                 GB gPrev, rt_gris_settings set, ResvPlanes thisR,
                                                                 ResvPlanes prevR, ResvPlanes tempR, TraceCounters* ctr) {
     __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
     PIXEL_INDEX
-    temporalPixel<kCount, kLearned>(S, cam, prevCam, g, gPrev, set, thisR, prevR, tempR, x, y,
+    temporalPixel<kCount, kLearned, kUnbiased>(S, cam, prevCam, g, gPrev, set, thisR, prevR, tempR, x, y,
                                     TStack{stk + threadIdx.x, kTileThreads}, ctr);
 }
 
 // gris_resample_spatial.glsl:11-135 and gris_resample_spatial.comp accumulation
-template <bool kCount, bool kLearned>
+template <bool kCount, bool kLearned, bool kUnbiased = false>
 RT_DEV void spatialPixel(const DevScene& S, const rt_camera& cam, const GB& g, const rt_gris_settings& set,
                          const ResvPlanes& tempR, const ResvPlanes& thisR, float* indirectOut, uint32_t x, uint32_t y,
                          TStack st, TraceCounters* ctr) {
@@ -857,6 +958,11 @@ RT_DEV void spatialPixel(const DevScene& S, const rt_camera& cam, const GB& g, c
         dstPrimaryIsec.prim = 0u;
         Surf dstPrimarySurf; dstPrimarySurf.pos = pos; dstPrimarySurf.norm = gp.norm; dstPrimarySurf.albedo = gp.albedo;
         dstPrimarySurf.matIndex = (uint32_t)matId; dstPrimarySurf.isLight = false;
+        const float mCanon = resv.sampleCount;
+        // unbiased mode: the merged neighbours (pixel, M), at most 8 (the settings check)
+        uint32_t nbPix[kUnbiased ? 8 : 1];
+        float nbM[kUnbiased ? 8 : 1];
+        uint32_t nbCount = 0;
         if (set.spatialReuse) {
             f2 texel = mk2(1.0f / (float)g.W, 1.0f / (float)g.H);
             for (uint32_t i = 0; i < set.spatialNeighbors; i++) {
@@ -872,9 +978,30 @@ RT_DEV void spatialPixel(const DevScene& S, const rt_camera& cam, const GB& g, c
                 f3 posN = nr.ori + nr.dir * (np.depth - 1e-4f);
                 if (dot(np.norm, gp.norm) < 0.9f || distance(pos, posN) > 0.4f) continue;
                 Resv nres = resvAt<kLearned>(tempR, cam.filmSize[0], cam.filmSize[1], px, py);
-                if (resvIsValid(nres))
-                    reuseAndMerge<kCount, kLearned>(S, st, set, resv, dstPrimarySurf, dstPrimaryIsec, ray.dir, ray.ori, nres, rng, ctr);
+                if (resvIsValid(nres)) {
+                    reuseAndMerge<kCount, kLearned, kUnbiased>(S, st, set, resv, dstPrimarySurf, dstPrimaryIsec, ray.dir,
+                                                                ray.ori, nres, rng, ctr);
+                    if (kUnbiased && nbCount < 8u) {
+                        nbPix[nbCount] = ((uint32_t)px << 16) | (uint32_t)py;
+                        nbM[nbCount] = nres.sampleCount;
+                        nbCount++;
+                    }
+                }
             }
+        }
+        if (kUnbiased) {
+            if (nbCount > 0u && resvIsValid(resv) && resv.rcIsec.inst != RT_INVALID_HIT) {
+                float z = mCanon;
+                for (uint32_t j = 0; j < nbCount; j++) {
+                    if (!(nbM[j] > 0.0f)) continue;
+                    Surf ns; Isec ni; CamRay nr;
+                    if (primaryOf(cam, g, (int)(nbPix[j] >> 16), (int)(nbPix[j] & 0xffffu), ns, ni, nr) &&
+                        shiftSupports<kCount, kLearned>(S, st, set, resv, ns, ni, nr.dir, nr.ori, ctr))
+                        z += nbM[j];
+                }
+                normaliseZ(resv, z);
+            }
+            resvCap(resv, (float)set.cap);
         }
         if (!resvIsValid(resv)) resvReset(resv);
         storeResv<kLearned>(thisR, p, resv);
@@ -888,8 +1015,14 @@ RT_DEV void spatialPixel(const DevScene& S, const rt_camera& cam, const GB& g, c
                 else loadSurfaceInfo(S, rc.rcPrevIsec, rcPrevSurf);
                 loadSurfaceInfo(S, resv.rcIsec, rcSurf);
                 f3 wi = normalize(rcSurf.pos - rcPrevSurf.pos);
-                f3 Li = reconnectLi(S, resv, resv.rcLi, rcPrevSurf, rcSurf, wi, rc);
-                float pHat = kLearned ? luminance(reconnectLi(S, resv, targetRcLi(resv), rcPrevSurf, rcSurf, wi, rc))
+                float misW = 1.0f;
+                if (kUnbiased) {
+                    const float gj = fabsf(dot(rcSurf.norm, wi)) / square(distance(rcPrevSurf.pos, rcSurf.pos));
+                    misW = rcLightMIS(S, flagsRcVertexType(resv.flags), rcPrevSurf, rc.rcPrevWo, rcSurf, wi, gj);
+                }
+                f3 Li = reconnectLi(S, resv, kUnbiased ? resv.rcLi * misW : resv.rcLi, rcPrevSurf, rcSurf, wi, rc);
+                float pHat = kLearned ? luminance(reconnectLi(S, resv, kUnbiased ? targetRcLi(resv) * misW : targetRcLi(resv),
+                                                              rcPrevSurf, rcSurf, wi, rc))
                                       : luminance(Li);
                 if (!isBlack(Li) && !hasNan(Li)) radiance = Li / pHat * resv.resampleWeight / resv.sampleCount;
             }
@@ -898,13 +1031,14 @@ RT_DEV void spatialPixel(const DevScene& S, const rt_camera& cam, const GB& g, c
     }
     accumulate(indirectOut, p, radiance, cam.frameIndex);
 }
-template <bool kCount, bool kLearned>
+template <bool kCount, bool kLearned, bool kUnbiased>
 __global__ __launch_bounds__(kTileThreads, RT_WAVES_SPATIAL) void k_gris_spatial(DevScene S, rt_camera cam, GB g, rt_gris_settings set,
                                                                ResvPlanes tempR, ResvPlanes thisR, float* indirectOut,
                                                                TraceCounters* ctr) {
     __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
     PIXEL_INDEX
-    spatialPixel<kCount, kLearned>(S, cam, g, set, tempR, thisR, indirectOut, x, y, TStack{stk + threadIdx.x, kTileThreads},
+    spatialPixel<kCount, kLearned, kUnbiased>(S, cam, g, set, tempR, thisR, indirectOut, x, y,
+                                              TStack{stk + threadIdx.x, kTileThreads},
                                    ctr);
 }
 
@@ -1135,17 +1269,29 @@ namespace rti {
         else (void)launch(KERNEL<false>, GRID, BLOCK, STREAM, __VA_ARGS__, nullptr);                        \
     } while (0)
 
-// Same, for kernels that also take the target function as a template argument.
-#define RT_DISPATCH_L(KERNEL, LEARNED, GRID, BLOCK, STREAM, ...)                                                \
-    do {                                                                                                          \
-        if (LEARNED) {                                                                                            \
-            if (gCountersEnabled) (void)launch(KERNEL<true, true>, GRID, BLOCK, STREAM, __VA_ARGS__, counterBuffer()); \
-            else (void)launch(KERNEL<false, true>, GRID, BLOCK, STREAM, __VA_ARGS__, nullptr);                  \
-        } else {                                                                                                  \
-            if (gCountersEnabled) (void)launch(KERNEL<true, false>, GRID, BLOCK, STREAM, __VA_ARGS__, counterBuffer()); \
-            else (void)launch(KERNEL<false, false>, GRID, BLOCK, STREAM, __VA_ARGS__, nullptr);                 \
-        }                                                                                                         \
+// Same, for kernels that also take the target function and the resampling-weight mode.
+#define RT_DISPATCH_LU(KERNEL, LEARNED, UNBIASED, GRID, BLOCK, STREAM, ...)                                   \
+    do {                                                                                                      \
+        TraceCounters* ctr_ = gCountersEnabled ? counterBuffer() : nullptr;                                     \
+        if (gCountersEnabled) {                                                                               \
+            if (LEARNED) {                                                                                    \
+                if (UNBIASED) (void)launch(KERNEL<true, true, true>, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_);   \
+                else (void)launch(KERNEL<true, true, false>, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_);           \
+            } else {                                                                                          \
+                if (UNBIASED) (void)launch(KERNEL<true, false, true>, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_);  \
+                else (void)launch(KERNEL<true, false, false>, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_);          \
+            }                                                                                                 \
+        } else {                                                                                              \
+            if (LEARNED) {                                                                                    \
+                if (UNBIASED) (void)launch(KERNEL<false, true, true>, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_);  \
+                else (void)launch(KERNEL<false, true, false>, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_);          \
+            } else {                                                                                          \
+                if (UNBIASED) (void)launch(KERNEL<false, false, true>, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_); \
+                else (void)launch(KERNEL<false, false, false>, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_);         \
+            }                                                                                                 \
+        }                                                                                                     \
     } while (0)
+
 
 hipError_t launchTraceRays(const rt_device_scene* s, const float* rays8, uint32_t n, int mode, int skipLights, float* t,
                            float* bary2, uint32_t* inst, uint32_t* prim, hipStream_t stream) {
@@ -1202,7 +1348,8 @@ hipError_t launchGrisTemporal(const rt_device_scene* s, const rt_camera* cams, c
                               const rt_gris_settings& set, hipStream_t stream) {
     GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
     GB gp = makeGB(f.depthNormalPrev, f.albedoMatIdPrev, nullptr, f.width, f.height);
-    RT_DISPATCH_L(k_gris_temporal, set.targetFunction == 1u, tileGrid(f.width, f.height), kTileThreads, stream, s->dev,
+    RT_DISPATCH_LU(k_gris_temporal, set.targetFunction == 1u, set.unbiasedReuse == 1u, tileGrid(f.width, f.height),
+                   kTileThreads, stream, s->dev,
                   cams[0], cams[1], g, gp, set, planes(f.grisReservoir, f.width, f.height),
                   planes(f.grisReservoirPrev, f.width, f.height), planes(f.grisReservoirTemp, f.width, f.height));
     return hipGetLastError();
@@ -1211,7 +1358,8 @@ hipError_t launchGrisTemporal(const rt_device_scene* s, const rt_camera* cams, c
 hipError_t launchGrisSpatial(const rt_device_scene* s, const rt_camera* cams, const rt_frame& f,
                              const rt_gris_settings& set, hipStream_t stream) {
     GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
-    RT_DISPATCH_L(k_gris_spatial, set.targetFunction == 1u, tileGrid(f.width, f.height), kTileThreads, stream, s->dev,
+    RT_DISPATCH_LU(k_gris_spatial, set.targetFunction == 1u, set.unbiasedReuse == 1u, tileGrid(f.width, f.height),
+                   kTileThreads, stream, s->dev,
                   cams[0], g, set, planes(f.grisReservoirTemp, f.width, f.height),
                   planes(f.grisReservoir, f.width, f.height), f.indirectOutput);
     return hipGetLastError();

@@ -57,8 +57,8 @@ public:
     void render(hipStream_t stream, const DeviceScene& scene, const rt_camera cams[2], const rt_frame& frame);
 
     // src/GRISReSTIR.h:29 {Hybrid, 1, false, true, 20}; then the kernel constants
-    // MaxTracingDepth = 15, ResampleNum = 3, ResampleRadius = 20; luminance target
-    Settings settings = {Hybrid, 1.0f, 0u, 1u, 20u, 15u, 3u, 20.0f, 0u};
+    // MaxTracingDepth = 15, ResampleNum = 3, ResampleRadius = 20; luminance target; the reference's 1/M weights
+    Settings settings = {Hybrid, 1.0f, 0u, 1u, 20u, 15u, 3u, 20.0f, 0u, 0u};
     rt_mlp* learned = nullptr;  // borrowed; required when settings.targetFunction = 1
 
 private:

@@ -104,7 +104,8 @@ class SceneArrays(C.Structure):
 class GrisSettings(C.Structure):
     _fields_ = [("shiftType", C.c_uint32), ("rrScale", C.c_float), ("temporalReuse", C.c_uint32),
                 ("spatialReuse", C.c_uint32), ("cap", C.c_uint32), ("maxDepth", C.c_uint32),
-                ("spatialNeighbors", C.c_uint32), ("spatialRadius", C.c_float), ("targetFunction", C.c_uint32)]
+                ("spatialNeighbors", C.c_uint32), ("spatialRadius", C.c_float), ("targetFunction", C.c_uint32),
+                ("unbiasedReuse", C.c_uint32)]
 
 
 class MlpConfig(C.Structure):
@@ -142,9 +143,10 @@ assert C.sizeof(DIReservoir) == 64 and C.sizeof(DISettings) == 16
 
 
 def settings(shift=2, rr_scale=1.0, temporal=False, spatial=True, cap=20, max_depth=15, neighbors=3, radius=20.0,
-             learned=False):
+             learned=False, unbiased=False):
     """GRISReSTIR::Settings (src/GRISReSTIR.h:29 defaults) + kernel constants."""
-    return GrisSettings(shift, rr_scale, int(temporal), int(spatial), cap, max_depth, neighbors, radius, int(learned))
+    return GrisSettings(shift, rr_scale, int(temporal), int(spatial), cap, max_depth, neighbors, radius, int(learned),
+                        int(unbiased))
 
 
 def di_settings(shift=0, sample=0, temporal=False, spatial=True):
