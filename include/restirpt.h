@@ -294,6 +294,21 @@ rt_status rt_di_spatial(const rt_device_scene* scene, const rt_camera cams[2], c
 rt_status rt_di_render(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
                        const rt_di_reservoirs* reservoirs, const rt_di_settings* settings, hipStream_t stream);
 
+/* ---- Post-processing -------------------------------------------------------
+ * PostProcessFrag (src/PostProcessFrag.h:5-45, post_proc.frag:16-43): the
+ * direct and indirect outputs summed (each can be switched off), tone mapped
+ * (0 none, 1 Filmic, 2 ACES), gamma corrected (1/2.2) and stored as the
+ * 8-bit image the reference presents; bytes R, G, B, A per pixel, row 0 at the
+ * top (the screenshot layout of Renderer::writeScreenshot, src/Renderer.cpp:758-793).
+ * rgba8 is a device buffer of width * height * 4 bytes. */
+typedef struct rt_post_settings {  /* PostProcessFrag::PushConstant; Renderer defaults {1, 1, 0, 0} */
+    uint32_t toneMapping;
+    uint32_t correctGamma;
+    uint32_t noDirect;
+    uint32_t noIndirect;
+} rt_post_settings;
+rt_status rt_post_process(const rt_frame* frame, const rt_post_settings* settings, uint8_t* rgba8, hipStream_t stream);
+
 /* ---- Learned target function (no counterpart in the reference) ----------
  * A multiresolution hash-grid encoding of world position (levels x 2 FP16
  * features, 2^log2TableSize entries per level) feeding an MLP with two hidden

@@ -82,6 +82,14 @@ rt_status rt_renderer_reset_reservoirs(rt_renderer* r, hipStream_t stream);
 rt_status rt_renderer_enable_learned(rt_renderer* r, uint32_t seed, uint32_t log2TableSize, hipStream_t stream,
                                      rt_mlp** mlp);
 
+/* Writes an 8-bit RGBA image (host memory, row 0 at the top) as a PNG file:
+ * the stbi_write_png call of Renderer::writeScreenshot (src/Renderer.cpp:787-788).
+ * Image data is stored in uncompressed deflate blocks. */
+rt_status rt_write_png(const char* path, const uint8_t* rgba8, uint32_t width, uint32_t height);
+/* Renderer::writeScreenshot after PostProcessFrag: post-processes the current
+ * outputs (the last frame drawn) and writes them to path as a PNG. */
+rt_status rt_renderer_screenshot(rt_renderer* r, const rt_post_settings* settings, const char* path, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

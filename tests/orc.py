@@ -52,6 +52,9 @@ def lib():
         L.orc_di_temporal.argtypes = [P, C.POINTER(rt.Camera), P, P, P, P, P, C.POINTER(rt.DISettings), P, P, P, P,
                                       C.c_int]
         L.orc_di_spatial.argtypes = [P, C.POINTER(rt.Camera), P, P, C.POINTER(rt.DISettings), P, P, P, P, C.c_int]
+        L.orc_post_process.argtypes = [P, P, u32, u32, P, P]
+        L.orc_tonemap.argtypes = [C.c_float, u32]
+        L.orc_tonemap.restype = C.c_float
         L.orc_gris_learned_refresh.argtypes = [P, u32, u32, P, P, P, P, P, P, u32, u32, C.c_int]
         L.orc_mlp_forward.argtypes = [u32, u32, u32, P, P, P, P, P, u32, P, C.c_int]
         L.orc_mlp_gradients.argtypes = [u32, u32, u32, P, P, P, P, P, P, u32, P, P, C.c_int]
@@ -206,6 +209,16 @@ RESV_DTYPE = np.dtype([("bary", np.float32, 2), ("instanceIdx", np.uint32), ("tr
                        ("F", np.float32, 3), ("primaryRng", np.uint32), ("sampleCount", np.float32),
                        ("resampleWeight", np.float32), ("contribWeight", np.float32), ("pad0", np.float32)])
 assert RESV_DTYPE.itemsize == 96
+
+
+def post_process(direct, indirect, settings4):
+    """direct/indirect: (H, W, 4) float32; settings4: (toneMapping, correctGamma, noDirect, noIndirect)."""
+    H, W = direct.shape[:2]
+    out = np.zeros((H, W, 4), np.uint8)
+    st = np.asarray(settings4, np.uint32)
+    lib().orc_post_process(p(np.ascontiguousarray(direct, np.float32)), p(np.ascontiguousarray(indirect, np.float32)),
+                           W, H, p(st), p(out))
+    return out
 
 
 DI_DTYPE = np.dtype([("bary", np.float32, 2), ("instanceIdx", np.uint32), ("triangleIdx", np.uint32),

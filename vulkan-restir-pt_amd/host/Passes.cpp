@@ -1,5 +1,7 @@
 // Render passes of the host layer over the C ABI.
 #include "Passes.h"
+#include "restirpt_host.h"
+#include <vector>
 
 namespace {
 template <class T>
@@ -138,6 +140,19 @@ Renderer::~Renderer() {
     (void)hipFree(directOutput); (void)hipFree(indirectOutput);
     (void)hipFree(grisReservoir[0]); (void)hipFree(grisReservoir[1]); (void)hipFree(grisReservoirTemp);
     (void)hipFree(diReservoir[0]); (void)hipFree(diReservoir[1]); (void)hipFree(diReservoirTemp);
+}
+void Renderer::screenshot(hipStream_t stream, const rt_post_settings& post, const std::string& path) {
+    const size_t bytes = (size_t)width * height * 4;
+    uint8_t* dev = deviceAlloc<uint8_t>(bytes);
+    std::vector<uint8_t> host(bytes);
+    const rt_frame f = bindings();
+    rt_status st = rt_post_process(&f, &post, dev, stream);
+    hipError_t e = st == RT_OK ? hipMemcpyAsync(host.data(), dev, bytes, hipMemcpyDeviceToHost, stream) : hipSuccess;
+    if (e == hipSuccess && st == RT_OK) e = hipStreamSynchronize(stream);
+    (void)hipFree(dev);
+    rtThrow(st, "rt_post_process");
+    if (e != hipSuccess) throw std::runtime_error("Renderer::screenshot: copy failed");
+    rtThrow(rt_write_png(path.c_str(), host.data(), width, height), "rt_write_png");
 }
 rt_di_reservoirs Renderer::diBindings() const {
     rt_di_reservoirs r{};

@@ -115,6 +115,9 @@ public:
     void flip() { mCurFrame ^= 1u; }
     // Resets the GRIS and DI reservoir buffers (the GUI's clearReservoir, src/Renderer.cpp:567).
     void resetReservoirs(hipStream_t stream);
+    // PostProcessFrag + Renderer::writeScreenshot (src/Renderer.cpp:758-793): the current outputs
+    // post-processed and written to path as a PNG.
+    void screenshot(hipStream_t stream, const rt_post_settings& post, const std::string& path);
 
     GRISReSTIR gris;
     TestReSTIR restirDI;

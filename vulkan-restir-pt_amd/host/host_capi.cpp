@@ -167,6 +167,16 @@ rt_status rt_renderer_read_output(const rt_renderer* r, uint32_t which, float* h
     return RT_OK;
 }
 
+rt_status rt_renderer_screenshot(rt_renderer* r, const rt_post_settings* settings, const char* path, hipStream_t stream) {
+    if (!r || !settings || !path) return RT_ERR_INVALID_ARGUMENT;
+    try {
+        r->r->screenshot(stream, *settings, path);
+        return RT_OK;
+    } catch (const std::exception& e) {
+        return fail(RT_ERR_INVALID_ARGUMENT, e.what());
+    }
+}
+
 rt_status rt_renderer_reset_reservoirs(rt_renderer* r, hipStream_t stream) {
     if (!r) return RT_ERR_INVALID_ARGUMENT;
     try {

@@ -91,6 +91,12 @@ class DIReservoirs(C.Structure):
     _fields_ = [("reservoir", C.c_void_p), ("reservoirPrev", C.c_void_p), ("reservoirTemp", C.c_void_p)]
 
 
+class PostSettings(C.Structure):
+    """PostProcessFrag::PushConstant; the Renderer's defaults are Filmic + gamma."""
+    _fields_ = [("toneMapping", C.c_uint32), ("correctGamma", C.c_uint32), ("noDirect", C.c_uint32),
+                ("noIndirect", C.c_uint32)]
+
+
 class SceneArrays(C.Structure):
     _fields_ = [("materials", C.c_void_p), ("numMaterials", C.c_uint32),
                 ("materialIndices", C.c_void_p), ("numMaterialIndices", C.c_uint32),
@@ -174,14 +180,14 @@ CORE_SYMBOLS = [
     "rt_frame_graph_create", "rt_frame_graph_launch", "rt_frame_graph_destroy", "rt_batch_create",
     "rt_batch_destroy", "rt_batch_frame", "rt_batch_set_tuning", "rt_batch_get_tuning", "rt_batch_autotune",
     "rt_di_reservoir_bytes", "rt_di_reservoir_reset", "rt_di_reservoir_download", "rt_di_reservoir_upload",
-    "rt_di_path_gen", "rt_di_temporal", "rt_di_spatial", "rt_di_render",
+    "rt_di_path_gen", "rt_di_temporal", "rt_di_spatial", "rt_di_render", "rt_post_process",
 ]
 HOST_SYMBOLS = [
     "rt_scenegen", "rt_host_scene_destroy", "rt_host_scene_arrays", "rt_host_scene_camera", "rt_light_table_build",
     "rt_renderer_create", "rt_renderer_destroy", "rt_renderer_set_settings", "rt_renderer_frame",
     "rt_renderer_bindings", "rt_renderer_flip", "rt_renderer_read_output", "rt_renderer_reset_reservoirs",
     "rt_renderer_enable_learned", "rt_renderer_frame_methods", "rt_renderer_set_di_settings",
-    "rt_renderer_di_bindings",
+    "rt_renderer_di_bindings", "rt_write_png", "rt_renderer_screenshot",
 ]
 
 _core = None
@@ -236,6 +242,7 @@ def _declare_core(L):
         "rt_di_reservoir_reset": [P, u32, u32, P],
         "rt_di_reservoir_download": [P, u32, u32, P, P],
         "rt_di_reservoir_upload": [P, u32, u32, P, P],
+        "rt_post_process": [C.POINTER(Frame), C.POINTER(PostSettings), P, P],
     }
     di_pass = [P, C.POINTER(Camera), C.POINTER(Frame), C.POINTER(DIReservoirs), C.POINTER(DISettings), P]
     for name in ("rt_di_path_gen", "rt_di_temporal", "rt_di_spatial", "rt_di_render"):
@@ -269,6 +276,8 @@ def _declare_host(L):
         "rt_renderer_frame_methods": [P, C.POINTER(Camera), u32, u32, P],
         "rt_renderer_set_di_settings": [P, C.POINTER(DISettings)],
         "rt_renderer_di_bindings": [P, C.POINTER(DIReservoirs)],
+        "rt_write_png": [C.c_char_p, P, u32, u32],
+        "rt_renderer_screenshot": [P, C.POINTER(PostSettings), C.c_char_p, P],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
