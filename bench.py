@@ -566,6 +566,57 @@ def restir_di(rt, hs, ds, args, kernel_loop=False):
     return res
 
 
+def restir_gi(rt, hs, ds, args):
+    """ReSTIR GI temporal reuse (the reference's indirect method 2: one pass per frame tracing up to
+    15 bounces from the primary vertex, merging into the reprojected previous reservoir, M-cap 40)
+    at the bench resolution on the same terrain and camera walk; HIP-event times of its kernel."""
+    import numpy as np
+    import torch
+
+    W, H = args.width, args.height
+    L, H_ = rt.core(), rt.host()
+    r = C.c_void_p()
+    rt.check(H_.rt_renderer_create(ds.handle, W, H, C.byref(r)), "rt_renderer_create")
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    gi = [torch.zeros(L.rt_gi_reservoir_bytes(W, H), dtype=torch.uint8, device="cuda") for _ in range(2)]
+    for b in gi:
+        rt.check(L.rt_gi_reservoir_reset(b.data_ptr(), W, H, stream), "reset")
+    cur = [0]
+
+    def frame(k, ev=None):
+        cams = hs.camera(k % 16, W, H)
+        f = rt.Frame()
+        rt.check(H_.rt_renderer_bindings(r, C.byref(f)), "bindings")
+        g = rt.GIReservoirs()
+        g.reservoir, g.reservoirPrev = gi[cur[0]].data_ptr(), gi[cur[0] ^ 1].data_ptr()
+        cur[0] ^= 1
+        rt.check(L.rt_gbuffer_trace(ds.handle, C.byref(cams[0]), f.depthNormal, f.albedoMatId, f.motionVector,
+                                    stream), "gbuffer")
+        if ev: ev[0].record()
+        rt.check(L.rt_gi_temporal(ds.handle, cams, C.byref(f), C.byref(g), stream), "gi")
+        if ev: ev[1].record()
+        rt.check(H_.rt_renderer_flip(r), "flip")
+
+    try:
+        for k in range(args.warmup):
+            frame(k)
+        torch.cuda.synchronize()
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            frame(args.warmup + k, ev[k])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        gi_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+    finally:
+        H_.rt_renderer_destroy(r)
+    return {"ms_per_frame": round(wall * 1e3 / args.steps, 3), "gi_temporal_ms": round(gi_ms, 3),
+            "mpixels_per_s": round(W * H * args.steps / wall / 1e6, 2),
+            "workload": f"ReSTIR GI temporal (G-buffer + gi_resample_temporal, <= 15 bounces, M-cap 40), {W}x{H}, "
+                        f"same terrain and camera walk"}
+
+
 def read_traffic(kernel):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_1080p_terrain_traffic.md")
@@ -650,6 +701,7 @@ def main():
                                       "tris": int(work[p][1]) // args.steps} for p in PASSES}
     if not args.no_di:
         line["restir_di"] = restir_di(rt, hs, ds, args)
+        line["restir_gi"] = restir_gi(rt, hs, ds, args)
     if not args.no_batched:
         line["batched"] = batched_views(rt, args, hs, ds)
     if not args.no_learned:

@@ -294,6 +294,35 @@ rt_status rt_di_spatial(const rt_device_scene* scene, const rt_camera cams[2], c
 rt_status rt_di_render(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
                        const rt_di_reservoirs* reservoirs, const rt_di_settings* settings, hipStream_t stream);
 
+/* ---- ReSTIR GI (temporal) -------------------------------------------------
+ * The indirect method "ReSTIR GI" of the reference (gi_resample_temporal.comp /
+ * .glsl:34-213, run by Renderer::recordRenderCommand for indirectMethod =
+ * ResampledGI, src/Renderer.cpp:470-471): one pass that traces a path from the
+ * primary vertex (reconnection vertex = the first secondary hit, its outgoing
+ * radiance gathered with NEE + MIS, at most 15 bounces), merges it into the
+ * reprojected previous reservoir (M-cap 40), shades the reservoir's sample
+ * through a visibility ray and accumulates frame->indirectOutput. */
+typedef struct rt_gi_reservoir {  /* GIReservoir, layouts.glsl:117-129, 48 B (host view only) */
+    rt_intersection rcIsec;
+    float rcLo[3]; uint32_t rcPrevCoord;
+    uint32_t sampleCount; float resampleWeight; float contribWeight; float pad0;
+} rt_gi_reservoir;
+/* Descriptor set 2 bindings 10/11 (layouts.glsl:190-191), swapped per frame
+ * parity like mGIReservoir[.][2] (src/Renderer.cpp:342-343). */
+typedef struct rt_gi_reservoirs {
+    void* reservoir;
+    void* reservoirPrev;
+} rt_gi_reservoirs;
+/* 48 B per pixel, the reference's GIReservoirData (src/Renderer.cpp:34). */
+size_t rt_gi_reservoir_bytes(uint32_t width, uint32_t height);
+rt_status rt_gi_reservoir_reset(void* reservoirs, uint32_t width, uint32_t height, hipStream_t stream);
+rt_status rt_gi_reservoir_download(const void* reservoirs, uint32_t width, uint32_t height, rt_gi_reservoir* host,
+                                   hipStream_t stream);
+rt_status rt_gi_reservoir_upload(const rt_gi_reservoir* host, uint32_t width, uint32_t height, void* reservoirs,
+                                 hipStream_t stream);
+rt_status rt_gi_temporal(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
+                         const rt_gi_reservoirs* reservoirs, hipStream_t stream);
+
 /* ---- Post-processing -------------------------------------------------------
  * PostProcessFrag (src/PostProcessFrag.h:5-45, post_proc.frag:16-43): the
  * direct and indirect outputs summed (each can be switched off), tone mapped

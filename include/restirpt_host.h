@@ -58,8 +58,7 @@ rt_status rt_renderer_frame(rt_renderer* r, const rt_camera cams[2], uint32_t me
  * directMethod / indirectMethod, src/Renderer.h:25-31, dispatched as in
  * Renderer::recordRenderCommand, src/Renderer.cpp:440-470): G-buffer, then the
  * direct method (0 None, 1 Naive, 2 ReSTIR DI), then the indirect method
- * (0 None, 1 Naive, 3 ReSTIR PT; 2 = ReSTIR GI is not part of this build);
- * flips the frame parity. */
+ * (0 None, 1 Naive, 2 ReSTIR GI, 3 ReSTIR PT); flips the frame parity. */
 rt_status rt_renderer_frame_methods(rt_renderer* r, const rt_camera cams[2], uint32_t directMethod,
                                     uint32_t indirectMethod, hipStream_t stream);
 /* TestReSTIR::settings of the renderer's ReSTIR DI pass. */

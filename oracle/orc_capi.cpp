@@ -174,6 +174,27 @@ void orc_pt_plain(void* h, const Camera* cam, const float* depthNormal, const ui
     sumStats(st, stats3);
 }
 
+// ---- ReSTIR GI temporal -----------------------------------------------------------------------
+void orc_gi_temporal(void* h, const Camera* cams, const float* depthNormal, const uint32_t* albedoMatId,
+                     const uint16_t* motion, const float* depthNormalPrev, const uint32_t* albedoMatIdPrev,
+                     GIReservoir* resvThis, const GIReservoir* resvPrev, float* indirectOut, uint64_t* stats3,
+                     int nthreads) {
+    OScene* s = (OScene*)h;
+    const Camera& cam = cams[0];
+    const uint32_t W = cam.filmSize[0], H = cam.filmSize[1];
+    GBufView g{depthNormal, albedoMatId, motion, W, H};
+    GBufView gp{depthNormalPrev, albedoMatIdPrev, nullptr, W, H};
+    std::vector<TraceStats> st(nthreads > 1 ? nthreads : 1);
+    parallelRows(H, nthreads, [&](uint32_t y, int t) {
+        Ctx c{&s->S, &s->bvh, &st[t]};
+        for (uint32_t x = 0; x < W; x++) {
+            vec3 col = clampColor(giTemporal(c, cam, cams[1], g, gp, resvThis, resvPrev, x, y));
+            accumulate(indirectOut + 4 * ((size_t)y * W + x), col, cam.frameIndex);
+        }
+    });
+    sumStats(st, stats3);
+}
+
 // ---- ReSTIR DI passes ------------------------------------------------------------------------
 // cams[0] = uCamera, cams[1] = uPrevCamera; reservoirs are the reference's 64-B DIReservoir.
 void orc_di_path_gen(void* h, const Camera* cams, const float* depthNormal, const uint32_t* albedoMatId,

@@ -6,6 +6,7 @@
 //   di / gi      : src/shader/di_naive.glsl:79-167, gi_naive.glsl:28-149
 //   GRIS         : src/shader/gris_path_trace.glsl:45-306, gris_retrace.glsl:42-236,
 //                  gris_resample_temporal.glsl:11-83, gris_resample_spatial.glsl:11-135
+//   ReSTIR GI    : src/shader/gi_resample_temporal.glsl:9-213 (orc_gi.cpp)
 //   ReSTIR DI    : src/shader/di_path_gen.glsl:9-36, di_temporal.glsl:9-88,
 //                  di_spatial.glsl:9-119, di_reservoir.glsl:28-290 (orc_di.cpp)
 //   wrappers     : *.comp main() accumulation (frameIndex & mask)
@@ -180,6 +181,21 @@ struct DISettings {          // TestReSTIR::Settings, src/TestReSTIR.h:12-17
     uint32_t spatialReuse;
 };
 static const uint32_t DISampleLight = 0, DISampleBSDF = 1, DISampleBoth = 2;
+
+// ---- ReSTIR GI (temporal) -------------------------------------------------------------
+struct GIPathSample {        // layouts.glsl:117-121, 32 B
+    Intersection rcIsec;
+    float rcLo[3]; uint32_t rcPrevCoord;
+};
+struct GIReservoir {         // layouts.glsl:123-129, 48 B
+    GIPathSample pathSample;
+    uint32_t sampleCount; float resampleWeight; float contribWeight; float pad0;
+};
+static_assert(sizeof(GIReservoir) == 48, "GI reservoir layout");
+// Writes resvThis[pixel] (a background pixel's reservoir is reset in place);
+// returns the radiance before clampColor.
+vec3 giTemporal(const Ctx& c, const Camera& cam, const Camera& prevCam, const GBufView& g, const GBufView& gPrev,
+                GIReservoir* resvThis, const GIReservoir* resvPrev, uint32_t x, uint32_t y);
 
 // Each returns false (nothing written) for a background pixel.
 bool diPathGen(const Ctx& c, const Camera& cam, const GBufView& g, const DISettings& set, uint32_t x, uint32_t y,

@@ -52,6 +52,7 @@ def lib():
         L.orc_di_temporal.argtypes = [P, C.POINTER(rt.Camera), P, P, P, P, P, C.POINTER(rt.DISettings), P, P, P, P,
                                       C.c_int]
         L.orc_di_spatial.argtypes = [P, C.POINTER(rt.Camera), P, P, C.POINTER(rt.DISettings), P, P, P, P, C.c_int]
+        L.orc_gi_temporal.argtypes = [P, C.POINTER(rt.Camera), P, P, P, P, P, P, P, P, P, C.c_int]
         L.orc_post_process.argtypes = [P, P, u32, u32, P, P]
         L.orc_tonemap.argtypes = [C.c_float, u32]
         L.orc_tonemap.restype = C.c_float
@@ -190,6 +191,12 @@ class OracleScene:
                              p(st), nthreads())
         return st
 
+    def gi_temporal(self, cams, dn, am, mv, dn_prev, am_prev, resv_this, resv_prev, indirect):
+        st = np.zeros(3, np.uint64)
+        lib().orc_gi_temporal(self.h, cams, p(dn), p(am), p(mv), p(dn_prev), p(am_prev), p(resv_this), p(resv_prev),
+                              p(indirect), p(st), nthreads())
+        return st
+
     def learned_refresh(self, W, H, dn, resv, lhat4, stride, offset):
         """Refreshes resv in place from the predictions lhat4 (H*W x 4); returns (learned planes, batch pos,
         batch target) with the batch over pixels p % stride == offset."""
@@ -226,6 +233,19 @@ DI_DTYPE = np.dtype([("bary", np.float32, 2), ("instanceIdx", np.uint32), ("tria
                      ("rng", np.uint32), ("isLightSample", np.uint32), ("sampleCount", np.uint32),
                      ("resampleWeight", np.float32), ("contribWeight", np.float32), ("weight", np.float32)])
 assert DI_DTYPE.itemsize == 64
+
+
+GI_DTYPE = np.dtype([("bary", np.float32, 2), ("instanceIdx", np.uint32), ("triangleIdx", np.uint32),
+                     ("rcLo", np.float32, 3), ("rcPrevCoord", np.uint32), ("sampleCount", np.uint32),
+                     ("resampleWeight", np.float32), ("contribWeight", np.float32), ("pad0", np.float32)])
+assert GI_DTYPE.itemsize == 48
+
+
+def reset_gi_reservoirs(n):
+    """The state rt_gi_reservoir_reset writes: zeros with an invalid sample."""
+    r = np.zeros(n, GI_DTYPE)
+    r["instanceIdx"] = 0xFFFFFFFF
+    return r
 
 
 def reset_di_reservoirs(n):

@@ -48,6 +48,8 @@ def test_struct_sizes_match_reference_layouts():
     assert C.sizeof(rt.DISettings) == 16       # TestReSTIR::Settings push constant
     assert C.sizeof(rt.DIReservoirs) == 24
     assert C.sizeof(rt.PostSettings) == 16     # PostProcessFrag::PushConstant
+    assert C.sizeof(rt.GIReservoirs) == 16
+    assert rt.core().rt_gi_reservoir_bytes(1920, 1080) == 1920 * 1080 * 48
     assert rt.core().rt_di_reservoir_bytes(1920, 1080) == 1920 * 1080 * 64
     # five 16-B planes of reference fields + the learned-target plane = the reference's 96-B reservoir
     assert rt.core().rt_gris_reservoir_bytes(1920, 1080) == 1920 * 1080 * 96

@@ -193,6 +193,6 @@ def test_renderer_methods_match_pass_calls():
                 assert np.array_equal(host.view(np.uint32), run.output(which).view(np.uint32)), (frame, which)
             run.flip()
         assert run.output(0)[..., :3].mean() > 0 and run.output(1)[..., :3].mean() > 0
-        assert Hh.rt_renderer_frame_methods(r, cams, 2, 2, gpuhelp.stream()) != 0  # ReSTIR GI is not built
+        assert Hh.rt_renderer_frame_methods(r, cams, 3, 3, gpuhelp.stream()) != 0
     finally:
         Hh.rt_renderer_destroy(r)

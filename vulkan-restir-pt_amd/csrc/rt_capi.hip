@@ -39,6 +39,9 @@ hipError_t launchDiTemporal(const rt_device_scene*, const rt_camera*, const rt_f
 hipError_t launchDiSpatial(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_di_reservoirs&,
                            const rt_di_settings&, hipStream_t);
 hipError_t launchDiReset(void*, uint32_t, uint32_t, hipStream_t);
+hipError_t launchGiTemporal(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_gi_reservoirs&,
+                            hipStream_t);
+hipError_t launchGiReset(void*, uint32_t, uint32_t, hipStream_t);
 
 }  // namespace rti
 
@@ -447,6 +450,51 @@ rt_status rt_di_render(const rt_device_scene* s, const rt_camera cams[2], const 
     if (st == RT_OK) st = rt_di_temporal(s, cams, f, r, set, stream);
     if (st == RT_OK) st = rt_di_spatial(s, cams, f, r, set, stream);
     return st;
+}
+
+// ---- ReSTIR GI ---------------------------------------------------------------------------------
+size_t rt_gi_reservoir_bytes(uint32_t w, uint32_t h) { return (size_t)w * h * 48; }
+
+rt_status rt_gi_reservoir_reset(void* r, uint32_t w, uint32_t h, hipStream_t stream) {
+    if (!r) return invalid("reservoir buffer is null");
+    RT_LAUNCH(rti::launchGiReset(r, w, h, stream));
+    return RT_OK;
+}
+
+// Device plane k holds the k-th 16 bytes of the reference struct.
+rt_status rt_gi_reservoir_download(const void* r, uint32_t w, uint32_t h, rt_gi_reservoir* host, hipStream_t stream) {
+    if (!r || !host) return invalid("null argument");
+    static_assert(sizeof(rt_gi_reservoir) == 48, "GIReservoir is 48 B");
+    const size_t n = (size_t)w * h;
+    std::vector<float4> pl(n * 3);
+    RT_HIP(hipMemcpyAsync(pl.data(), r, sizeof(float4) * 3 * n, hipMemcpyDeviceToHost, stream));
+    RT_HIP(hipStreamSynchronize(stream));
+    for (size_t i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) std::memcpy(reinterpret_cast<char*>(&host[i]) + 16 * k, &pl[k * n + i], 16);
+    return RT_OK;
+}
+
+rt_status rt_gi_reservoir_upload(const rt_gi_reservoir* host, uint32_t w, uint32_t h, void* r, hipStream_t stream) {
+    if (!r || !host) return invalid("null argument");
+    const size_t n = (size_t)w * h;
+    std::vector<float4> pl(n * 3);
+    for (size_t i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) std::memcpy(&pl[k * n + i], reinterpret_cast<const char*>(&host[i]) + 16 * k, 16);
+    RT_HIP(hipMemcpyAsync(r, pl.data(), sizeof(float4) * 3 * n, hipMemcpyHostToDevice, stream));
+    RT_HIP(hipStreamSynchronize(stream));
+    return RT_OK;
+}
+
+rt_status rt_gi_temporal(const rt_device_scene* s, const rt_camera cams[2], const rt_frame* f,
+                         const rt_gi_reservoirs* r, hipStream_t stream) {
+    rt_status st = checkFrame(s, cams, f);
+    if (st != RT_OK) return st;
+    if (!r || !r->reservoir || !r->reservoirPrev) return invalid("reservoir buffer is null");
+    if (!f->depthNormalPrev || !f->albedoMatIdPrev || !f->motionVector) return invalid("previous G-buffer is null");
+    if (!f->indirectOutput) return invalid("indirectOutput is null");
+    if (s->numLights == 0) return invalid("the scene has no lights");
+    RT_LAUNCH(rti::launchGiTemporal(s, cams, *f, *r, stream));
+    return RT_OK;
 }
 
 rt_status rt_gris_learned_refresh(const rt_device_scene* s, rt_mlp* mlp, const rt_camera cams[2], const rt_frame* f,

@@ -105,6 +105,7 @@ void Renderer::resetReservoirs(hipStream_t stream) {
     rtThrow(rt_gris_reservoir_reset(grisReservoirTemp, width, height, stream), "reset");
     for (int j = 0; j < 2; j++) rtThrow(rt_di_reservoir_reset(diReservoir[j], width, height, stream), "reset");
     rtThrow(rt_di_reservoir_reset(diReservoirTemp, width, height, stream), "reset");
+    for (int j = 0; j < 2; j++) rtThrow(rt_gi_reservoir_reset(giReservoir[j], width, height, stream), "reset");
 }
 
 // ---- Renderer: src/Renderer.cpp:193-257 (createRayImage), 319-353 (bindings), 400-569 ----
@@ -127,6 +128,10 @@ Renderer::Renderer(const DeviceScene& scene, uint32_t w, uint32_t h)
     }
     diReservoirTemp = deviceAlloc<uint8_t>(db);
     rtThrow(rt_di_reservoir_reset(diReservoirTemp, w, h, 0), "rt_di_reservoir_reset");
+    for (int j = 0; j < 2; j++) {
+        giReservoir[j] = deviceAlloc<uint8_t>(rt_gi_reservoir_bytes(w, h));
+        rtThrow(rt_gi_reservoir_reset(giReservoir[j], w, h, 0), "rt_gi_reservoir_reset");
+    }
     if (hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("hipDeviceSynchronize failed");
     gris.createPipeline();
     restirDI.createPipeline();
@@ -140,6 +145,7 @@ Renderer::~Renderer() {
     (void)hipFree(directOutput); (void)hipFree(indirectOutput);
     (void)hipFree(grisReservoir[0]); (void)hipFree(grisReservoir[1]); (void)hipFree(grisReservoirTemp);
     (void)hipFree(diReservoir[0]); (void)hipFree(diReservoir[1]); (void)hipFree(diReservoirTemp);
+    (void)hipFree(giReservoir[0]); (void)hipFree(giReservoir[1]);
 }
 void Renderer::screenshot(hipStream_t stream, const rt_post_settings& post, const std::string& path) {
     const size_t bytes = (size_t)width * height * 4;
@@ -153,6 +159,12 @@ void Renderer::screenshot(hipStream_t stream, const rt_post_settings& post, cons
     rtThrow(st, "rt_post_process");
     if (e != hipSuccess) throw std::runtime_error("Renderer::screenshot: copy failed");
     rtThrow(rt_write_png(path.c_str(), host.data(), width, height), "rt_write_png");
+}
+rt_gi_reservoirs Renderer::giBindings() const {
+    rt_gi_reservoirs r{};
+    r.reservoir = giReservoir[mCurFrame];
+    r.reservoirPrev = giReservoir[mCurFrame ^ 1u];
+    return r;
 }
 rt_di_reservoirs Renderer::diBindings() const {
     rt_di_reservoirs r{};
@@ -189,13 +201,16 @@ void Renderer::drawFrame(hipStream_t stream, const rt_camera cams[2], Method met
     flip();
 }
 void Renderer::drawFrame(hipStream_t stream, const rt_camera cams[2], uint32_t directMethod, uint32_t indirectMethod) {
-    if (directMethod > MethodResampled || indirectMethod > MethodResampledPT || indirectMethod == MethodResampled)
+    if (directMethod > MethodResampled || indirectMethod > MethodResampledPT)
         throw std::invalid_argument("Renderer::drawFrame: unsupported direct or indirect method");
     mGBuffer.render(stream, mCurFrame, mScene, cams[0]);
     const rt_frame f = bindings();
     if (directMethod == MethodNaive) mNaiveDI.execute(stream, mScene, cams, f);
     else if (directMethod == MethodResampled) restirDI.render(stream, mScene, cams, f, diBindings());
     if (indirectMethod == MethodNaive) mNaiveGI.execute(stream, mScene, cams, f);
-    else if (indirectMethod == MethodResampledPT) gris.render(stream, mScene, cams, f);
+    else if (indirectMethod == MethodResampled) {
+        const rt_gi_reservoirs gi = giBindings();
+        rtThrow(rt_gi_temporal(mScene.handle(), cams, &f, &gi, stream), "rt_gi_temporal");
+    } else if (indirectMethod == MethodResampledPT) gris.render(stream, mScene, cams, f);
     flip();
 }

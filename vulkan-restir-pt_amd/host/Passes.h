@@ -98,7 +98,7 @@ class Renderer {
 public:
     enum Method : uint32_t { ResampledPT = 0, Naive = 1 };
     // Renderer::Settings directMethod / indirectMethod values (src/Renderer.h:25-31):
-    // direct None, Naive, ResampledDI; indirect None, Naive, ResampledGI (not built), ResampledPT
+    // direct None, Naive, ResampledDI; indirect None, Naive, ResampledGI, ResampledPT
     enum RayTracingMethod : uint32_t { MethodNone = 0, MethodNaive = 1, MethodResampled = 2, MethodResampledPT = 3 };
 
     Renderer(const DeviceScene& scene, uint32_t width, uint32_t height);
@@ -112,6 +112,7 @@ public:
     void drawFrame(hipStream_t stream, const rt_camera cams[2], uint32_t directMethod, uint32_t indirectMethod);
     rt_frame bindings() const;
     rt_di_reservoirs diBindings() const;
+    rt_gi_reservoirs giBindings() const;
     void flip() { mCurFrame ^= 1u; }
     // Resets the GRIS and DI reservoir buffers (the GUI's clearReservoir, src/Renderer.cpp:567).
     void resetReservoirs(hipStream_t stream);
@@ -128,6 +129,7 @@ public:
     void* grisReservoirTemp = nullptr;
     void* diReservoir[2] = {nullptr, nullptr};
     void* diReservoirTemp = nullptr;
+    void* giReservoir[2] = {nullptr, nullptr};
     uint32_t width, height;
 
 private:

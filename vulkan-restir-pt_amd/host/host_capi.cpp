@@ -123,9 +123,8 @@ rt_status rt_renderer_frame(rt_renderer* r, const rt_camera cams[2], uint32_t me
 rt_status rt_renderer_frame_methods(rt_renderer* r, const rt_camera cams[2], uint32_t directMethod,
                                     uint32_t indirectMethod, hipStream_t stream) {
     if (!r || !cams) return RT_ERR_INVALID_ARGUMENT;
-    if (directMethod > Renderer::MethodResampled || indirectMethod > Renderer::MethodResampledPT ||
-        indirectMethod == Renderer::MethodResampled)
-        return fail(RT_ERR_INVALID_ARGUMENT, "direct method must be 0..2 and indirect method 0, 1 or 3");
+    if (directMethod > Renderer::MethodResampled || indirectMethod > Renderer::MethodResampledPT)
+        return fail(RT_ERR_INVALID_ARGUMENT, "direct method must be 0..2 and indirect method 0..3");
     try {
         r->r->drawFrame(stream, cams, directMethod, indirectMethod);
         return RT_OK;

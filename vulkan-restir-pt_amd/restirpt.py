@@ -97,6 +97,10 @@ class PostSettings(C.Structure):
                 ("noIndirect", C.c_uint32)]
 
 
+class GIReservoirs(C.Structure):
+    _fields_ = [("reservoir", C.c_void_p), ("reservoirPrev", C.c_void_p)]
+
+
 class SceneArrays(C.Structure):
     _fields_ = [("materials", C.c_void_p), ("numMaterials", C.c_uint32),
                 ("materialIndices", C.c_void_p), ("numMaterialIndices", C.c_uint32),
@@ -181,6 +185,8 @@ CORE_SYMBOLS = [
     "rt_batch_destroy", "rt_batch_frame", "rt_batch_set_tuning", "rt_batch_get_tuning", "rt_batch_autotune",
     "rt_di_reservoir_bytes", "rt_di_reservoir_reset", "rt_di_reservoir_download", "rt_di_reservoir_upload",
     "rt_di_path_gen", "rt_di_temporal", "rt_di_spatial", "rt_di_render", "rt_post_process",
+    "rt_gi_reservoir_bytes", "rt_gi_reservoir_reset", "rt_gi_reservoir_download", "rt_gi_reservoir_upload",
+    "rt_gi_temporal",
 ]
 HOST_SYMBOLS = [
     "rt_scenegen", "rt_host_scene_destroy", "rt_host_scene_arrays", "rt_host_scene_camera", "rt_light_table_build",
@@ -243,6 +249,10 @@ def _declare_core(L):
         "rt_di_reservoir_download": [P, u32, u32, P, P],
         "rt_di_reservoir_upload": [P, u32, u32, P, P],
         "rt_post_process": [C.POINTER(Frame), C.POINTER(PostSettings), P, P],
+        "rt_gi_reservoir_reset": [P, u32, u32, P],
+        "rt_gi_reservoir_download": [P, u32, u32, P, P],
+        "rt_gi_reservoir_upload": [P, u32, u32, P, P],
+        "rt_gi_temporal": [P, C.POINTER(Camera), C.POINTER(Frame), C.POINTER(GIReservoirs), P],
     }
     di_pass = [P, C.POINTER(Camera), C.POINTER(Frame), C.POINTER(DIReservoirs), C.POINTER(DISettings), P]
     for name in ("rt_di_path_gen", "rt_di_temporal", "rt_di_spatial", "rt_di_render"):
@@ -251,7 +261,7 @@ def _declare_core(L):
         f = getattr(L, name)
         f.argtypes = args
         f.restype = i32
-    for name in ("rt_gris_reservoir_bytes", "rt_di_reservoir_bytes"):
+    for name in ("rt_gris_reservoir_bytes", "rt_di_reservoir_bytes", "rt_gi_reservoir_bytes"):
         getattr(L, name).argtypes = [u32, u32]
         getattr(L, name).restype = C.c_size_t
 
