@@ -33,6 +33,16 @@ typedef struct rt_renderer rt_renderer;
  *      triangles, GGX roughness in [0.02, 1] (~4.8M triangles)
  * Everything is drawn from PCG32 streams seeded by `seed`. */
 rt_status rt_scenegen(uint32_t config, uint32_t seed, rt_host_scene** out);
+/* Scene::load (src/Scene.cpp:104-303): a scene file in the reference's XML
+ * format (integrator film size, camera, model instances of Wavefront OBJ
+ * meshes with transforms, material overrides and emitters of given power).
+ * Textures are not supported. The camera is the file's, still, with a seeded
+ * path of 1024 frames for rt_host_scene_camera. Errors (unreadable or
+ * malformed files, textures, zero-power lights) return
+ * RT_ERR_INVALID_ARGUMENT with the message in rt_last_error(). */
+rt_status rt_scene_load(const char* path, rt_host_scene** out);
+/* The film size of a loaded scene file (<integrator><size>); 0 x 0 for procedural scenes. */
+rt_status rt_host_scene_film_size(const rt_host_scene* scene, uint32_t* width, uint32_t* height);
 rt_status rt_host_scene_destroy(rt_host_scene* scene);
 /* Borrowed views of the scene arrays (valid until the scene is destroyed). */
 rt_status rt_host_scene_arrays(const rt_host_scene* scene, rt_scene_arrays* out);

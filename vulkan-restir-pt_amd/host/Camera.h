@@ -23,6 +23,8 @@ public:
     void setAngle(Vec3 angle);
     void setFilmSize(uint32_t w, uint32_t h);
     void setPlanes(float nearZ, float farZ);
+    // setLensRadius / setFocalDist (src/Camera.h:35-38)
+    void setLens(float lensRadius, float focalDist) { mLensRadius = lensRadius; mFocalDist = focalDist; }
 
     Vec3 pos() const { return mPos; }
     Vec3 angle() const { return mAngle; }

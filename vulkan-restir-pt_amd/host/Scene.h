@@ -43,6 +43,8 @@ public:
     struct PathKey { Vec3 pos; Vec3 angle; uint32_t seed; };
     std::vector<PathKey> cameraPath;
     float fov = 60.0f;
+    float lensRadius = 0.0f, focalDist = 1.0f;
+    uint32_t filmSize[2] = {0, 0};  // the scene file's film size; 0 for procedural scenes
 };
 
 // Fills transform / transformInv / transformInvT from a column-major affine matrix.
@@ -65,6 +67,8 @@ private:
 };
 
 std::unique_ptr<Scene> generateScene(uint32_t config, uint32_t seed);
+// Scene::load (src/Scene.cpp:104-303): a scene file in the reference's XML format.
+std::unique_ptr<Scene> loadSceneFile(const std::string& path);
 
 inline void rtThrow(rt_status st, const char* what) {
     if (st != RT_OK) throw std::runtime_error(std::string(what) + ": " + rt_last_error());

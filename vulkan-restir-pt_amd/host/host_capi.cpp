@@ -11,9 +11,12 @@ struct rt_renderer {
     std::unique_ptr<LearnedTarget> learned;
 };
 
+namespace rti {
+void setError(const std::string& msg);  // the calling thread's rt_last_error() message (librestirpt.so)
+}
+
 namespace {
-thread_local std::string hostError;
-rt_status fail(rt_status st, const std::string& m) { hostError = m; return st; }
+rt_status fail(rt_status st, const std::string& m) { rti::setError(m); return st; }
 }  // namespace
 
 extern "C" {
@@ -28,6 +31,23 @@ rt_status rt_scenegen(uint32_t config, uint32_t seed, rt_host_scene** out) {
     } catch (const std::exception& e) {
         return fail(RT_ERR_INVALID_ARGUMENT, e.what());
     }
+}
+
+rt_status rt_scene_load(const char* path, rt_host_scene** out) {
+    if (!path || !out) return fail(RT_ERR_INVALID_ARGUMENT, "null argument");
+    try {
+        *out = new rt_host_scene{loadSceneFile(path)};
+        return RT_OK;
+    } catch (const std::exception& e) {
+        return fail(RT_ERR_INVALID_ARGUMENT, e.what());
+    }
+}
+
+rt_status rt_host_scene_film_size(const rt_host_scene* s, uint32_t* width, uint32_t* height) {
+    if (!s || !width || !height) return fail(RT_ERR_INVALID_ARGUMENT, "null argument");
+    *width = s->scene->filmSize[0];
+    *height = s->scene->filmSize[1];
+    return RT_OK;
 }
 
 rt_status rt_host_scene_destroy(rt_host_scene* s) {
@@ -52,6 +72,7 @@ static rt_camera deviceCamera(const Scene& S, uint32_t f, uint32_t width, uint32
         c.setFOV(S.fov);
         c.setPos(S.cameraPath[k].pos);
         c.setAngle(S.cameraPath[k].angle);
+        c.setLens(S.lensRadius, S.focalDist);
         return c;
     };
     Camera last = camAt(f > 0 ? f - 1 : 0);

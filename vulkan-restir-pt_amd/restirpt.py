@@ -189,7 +189,7 @@ CORE_SYMBOLS = [
     "rt_gi_temporal",
 ]
 HOST_SYMBOLS = [
-    "rt_scenegen", "rt_host_scene_destroy", "rt_host_scene_arrays", "rt_host_scene_camera", "rt_light_table_build",
+    "rt_scenegen", "rt_scene_load", "rt_host_scene_film_size", "rt_host_scene_destroy", "rt_host_scene_arrays", "rt_host_scene_camera", "rt_light_table_build",
     "rt_renderer_create", "rt_renderer_destroy", "rt_renderer_set_settings", "rt_renderer_frame",
     "rt_renderer_bindings", "rt_renderer_flip", "rt_renderer_read_output", "rt_renderer_reset_reservoirs",
     "rt_renderer_enable_learned", "rt_renderer_frame_methods", "rt_renderer_set_di_settings",
@@ -270,6 +270,8 @@ def _declare_host(L):
     P, u32, i32 = C.c_void_p, C.c_uint32, C.c_int32
     sigs = {
         "rt_scenegen": [u32, u32, C.POINTER(P)],
+        "rt_scene_load": [C.c_char_p, C.POINTER(P)],
+        "rt_host_scene_film_size": [P, C.POINTER(u32), C.POINTER(u32)],
         "rt_host_scene_destroy": [P],
         "rt_host_scene_arrays": [P, C.POINTER(SceneArrays)],
         "rt_host_scene_camera": [P, u32, u32, u32, C.POINTER(Camera)],
@@ -338,15 +340,31 @@ def _view(ptr, count, ctype):
 
 
 class HostScene:
-    """Procedural scene from rt_scenegen (config 0 box room, 1 terrain, 2 floor + light)."""
+    """Host scene: procedural (rt_scenegen: config 0 box room, 1 terrain, 2 floor + light, ...) or
+    loaded from a scene file in the reference's XML format (HostScene.load, Scene::load)."""
 
-    def __init__(self, config: int, seed: int = 0):
+    def __init__(self, config: int, seed: int = 0, _handle=None):
         H = host()
         h = C.c_void_p()
-        check(H.rt_scenegen(config, seed, C.byref(h)), "rt_scenegen")
+        if _handle is None:
+            check(H.rt_scenegen(config, seed, C.byref(h)), "rt_scenegen")
+        else:
+            h = _handle
         self.handle = h
         self.arrays = SceneArrays()
         check(H.rt_host_scene_arrays(h, C.byref(self.arrays)), "rt_host_scene_arrays")
+
+    @classmethod
+    def load(cls, path):
+        """rt_scene_load: the scene file at `path` (XML + OBJ meshes beside it)."""
+        h = C.c_void_p()
+        check(host().rt_scene_load(str(path).encode(), C.byref(h)), "rt_scene_load")
+        return cls(0, 0, _handle=h)
+
+    def film_size(self):
+        w, h = C.c_uint32(), C.c_uint32()
+        check(host().rt_host_scene_film_size(self.handle, C.byref(w), C.byref(h)), "rt_host_scene_film_size")
+        return w.value, h.value
 
     def camera(self, frame: int, width: int, height: int):
         cams = (Camera * 2)()
