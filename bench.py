@@ -63,6 +63,7 @@ def parse():
     ap.add_argument("--learned-kernel-loop", action="store_true",
                     help="time learned-target frames only (for profilers)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-view leg")
+    ap.add_argument("--no-4k", action="store_true", help="skip the 4K full-form leg")
     ap.add_argument("--batched-views", type=int, default=64)
     ap.add_argument("--lvp-frames", type=int, default=64, help="frames of the plain form in the equal-time run")
     ap.add_argument("--ref-frames", type=int, default=4096, help="frames of the converged reference image")
@@ -480,6 +481,88 @@ def batched_views(rt, args, hs1, ds1):
                         "round robin over 4 procedural scenes"}
 
 
+def full_form_4k(rt, args):
+    """The full-form workload at 3840x2160 on the ~4.8M-triangle scene (1024x1024-cell fBm heightfield
+    + 2000 instanced icospheres, 64 emissive triangles, GGX roughness U[0.02, 1]), depth 8, hybrid
+    shift, temporal + spatial: per frame the 4K view with the learned target (refresh after path
+    tracing, one training step after spatial reuse) plus 16 extra 512x512 views of the same scene in
+    one batched launch sequence (luminance target: the batched passes take no network). The learned
+    vMF-mixture proposal of this workload is not built; paths are sampled from the BSDFs."""
+    import numpy as np
+    import torch
+
+    hs = rt.HostScene(5, 4)
+    ds = rt.DeviceScene(hs.arrays)
+    W, H = 3840, 2160
+    fr = TorchFrames(rt, ds, W, H)
+    s4k = rt.settings(shift=2, temporal=True, spatial=True, cap=20, max_depth=8, neighbors=3, radius=30.0,
+                      learned=True)
+    info = ds.info()
+    lo, hi = [float(v) for v in info.sceneLo], [float(v) for v in info.sceneHi]
+    ext = [max(h - l, 1e-3) for l, h in zip(lo, hi)]
+    mlp = rt.Mlp(rt.mlp_config([l - 0.1 * e for l, e in zip(lo, ext)], [h + 0.1 * e for h, e in zip(hi, ext)],
+                               log2_table=19, seed=4))
+    n_extra, size = 16, 512
+    extra = [TorchFrames(rt, ds, size, size) for _ in range(n_extra)]
+    s_extra = rt.settings(shift=2, temporal=True, spatial=True, cap=20, max_depth=8, neighbors=3, radius=30.0)
+    offsets = np.random.default_rng(4).integers(0, 64, n_extra)
+    L = rt.core()
+    stream = fr.stream
+
+    def views_for(k):
+        vs = (rt.View * n_extra)()
+        for i in range(n_extra):
+            cams = hs.camera(int(offsets[i] + k) % 64, size, size)
+            v = vs[i]
+            v.scene = ds.handle.value
+            C.memmove(C.addressof(v.cams), C.addressof(cams), C.sizeof(rt.Camera) * 2)
+            v.frame = extra[i].frame_struct()
+            v.settings = s_extra
+        return vs
+
+    b = C.c_void_p()
+    rt.check(L.rt_batch_create(n_extra, C.byref(b)), "batch")
+    try:
+        best, tune_ms = rt.BatchTuning(), C.c_float()
+        rt.check(L.rt_batch_autotune(b, views_for(0), n_extra, 2, stream, C.byref(best), C.byref(tune_ms)), "tune")
+        for e in extra:
+            e.reset()
+
+        def frame(k, ev=None):
+            fr.frame(hs.camera(k % 64, W, H), s4k, mlp, ev)
+            if ev is not None:
+                ev[2].record()
+            rt.check(L.rt_batch_frame(b, views_for(k), n_extra, stream), "batch frame")
+            if ev is not None:
+                ev[3].record()
+            for e in extra:
+                e.cur ^= 1
+
+        for k in range(2):
+            frame(k)
+        torch.cuda.synchronize()
+        steps = 8
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
+        t0 = time.perf_counter()
+        for k in range(steps):
+            frame(2 + k, ev[k])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    finally:
+        L.rt_batch_destroy(b)
+    pixels = W * H + n_extra * size * size
+    view_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / steps
+    batch_ms = sum(e[2].elapsed_time(e[3]) for e in ev) / steps
+    return {"ms_per_frame": round(wall * 1e3 / steps, 3), "mpaths_per_s": round(pixels * steps / wall / 1e6, 2),
+            "main_view_learned_ms": round(view_ms, 3), "extra_views_batched_ms": round(batch_ms, 3),
+            "pixels_per_frame": pixels, "triangles": int(info.numPrimitives), "emissive": int(hs.arrays.numLights),
+            "tuning": {"persistent_groups": int(best.persistentGroups), "xcd_swizzle": int(best.xcdSwizzle)},
+            "workload": "3840x2160 view, learned target (hash grid 16x2x2^19 + MLP 2x64, trained each frame), "
+                        "+ 16 x 512x512 views batched (luminance target), ~4.8M-triangle scene, 64 emissive, "
+                        "depth 8, hybrid shift, temporal + spatial (3 neighbours, 30 px, M-cap 20); "
+                        "the vMF-mixture proposal is not built"}
+
+
 def restir_di(rt, hs, ds, args, kernel_loop=False):
     """ReSTIR DI (the reference's TestReSTIR: path generation, temporal reuse, spatial reuse with
     10 neighbours in 20 px and M-cap 40) at the bench resolution on the same terrain and camera walk,
@@ -617,6 +700,11 @@ def restir_gi(rt, hs, ds, args):
                         f"same terrain and camera walk"}
 
 
+def progress(msg):
+    """One line per finished step on stderr (the JSON result goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def read_traffic(kernel):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_1080p_terrain_traffic.md")
@@ -699,15 +787,24 @@ def main():
                             "launch_ms": round(pass_ms[dom], 3)}
         line["work_per_frame"] = {p: {"rays": int(work[p][2]) // args.steps, "nodes": int(work[p][0]) // args.steps,
                                       "tris": int(work[p][1]) // args.steps} for p in PASSES}
+    progress(f"headline: {line['ms_per_step']} ms/frame")
     if not args.no_di:
         line["restir_di"] = restir_di(rt, hs, ds, args)
+        progress("restir_di done")
         line["restir_gi"] = restir_gi(rt, hs, ds, args)
+        progress("restir_gi done")
     if not args.no_batched:
         line["batched"] = batched_views(rt, args, hs, ds)
+        progress("batched done")
+    if not args.no_4k:
+        line["full_form_4k"] = full_form_4k(rt, args)
+        progress("full_form_4k done")
     if not args.no_learned:
         line["learned_vs_plain"] = learned_vs_plain(rt, hs, ds, args)
+        progress("learned_vs_plain done")
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(rt, hs, args)
+        progress("cpu_baseline done")
     print(json.dumps(line))
 
 
