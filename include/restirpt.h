@@ -133,6 +133,13 @@ typedef struct rt_gris_settings {
      * contribution weights; M-cap applied once per pass; at most 8 spatial
      * neighbours). No counterpart in the reference. */
     uint32_t unbiasedReuse;
+    /* Learned directional proposal at the primary vertex (no counterpart in the
+     * reference): 1 = sample the primary vertex's direction from the frame's
+     * 8-lobe vMF mixture with probability 1/2, else from the BSDF, with the
+     * one-sample MIS density (the frame's guideLobes, guideLobesPrev and
+     * guideSamples must be bound; rt_gris_guide_refresh before the path-trace
+     * pass). 0 = the reference's BSDF sampling. */
+    uint32_t guidedProposal;
 } rt_gris_settings;
 
 /* Descriptor set 2 (RayImageDescSet) of the reference: layouts.glsl:180-195,
@@ -149,6 +156,14 @@ typedef struct rt_frame {
     void* grisReservoir;            /* binding 12 (this frame) */
     void* grisReservoirPrev;        /* binding 13 */
     void* grisReservoirTemp;        /* binding 14 */
+    /* Learned vMF-mixture proposal (no reference counterpart; read only when
+     * rt_gris_settings.guidedProposal = 1): the per-pixel lobe planes of this
+     * frame (written by rt_gris_guide_refresh) and of the previous frame,
+     * rt_guide_lobe_bytes(width, height) each, and the per-pixel training
+     * record the path-trace pass writes (16 B/px). Null otherwise. */
+    void* guideLobes;
+    void* guideLobesPrev;
+    void* guideSamples;
 } rt_frame;
 
 typedef struct rt_scene_info {
@@ -394,6 +409,33 @@ rt_status rt_mlp_batch_download(const rt_mlp* mlp, float* pos, float* target, ui
  * fixed stride of pixels. */
 rt_status rt_gris_learned_refresh(const rt_device_scene* scene, rt_mlp* mlp, const rt_camera cams[2],
                                   const rt_frame* frame, hipStream_t stream);
+
+/* ---- Learned directional proposal -----------------------------------------
+ * No counterpart in the reference. A second network of the same encoding and
+ * hidden layers whose 32 outputs are an 8-lobe von Mises-Fisher mixture per
+ * position (lobe k: weight logit, log concentration clamped to [-4, 8], polar
+ * and azimuth angles through sigmoids), trained online on the direction the
+ * path tracer sampled at each primary vertex weighted by the path's luminance
+ * (negative log-likelihood weighted by the sample weight over the batch mean),
+ * and sampled at the primary vertex by settings.guidedProposal = 1. All
+ * rt_mlp_* functions apply: rt_mlp_infer returns the 32 raw outputs per
+ * sample, training targets are 4 floats (unit direction, weight; weight <= 0
+ * masks the sample), weights are W1 [32][64], W2 [64][64], W3 [64][32]. */
+rt_status rt_guide_create(const rt_mlp_config* config, hipStream_t stream, rt_mlp** out);
+/* Bytes of one lobe-plane buffer: 16 float4 planes per pixel (256 B/px). */
+size_t rt_guide_lobe_bytes(uint32_t width, uint32_t height);
+/* After the G-buffer pass: evaluates the proposal network at every pixel's
+ * primary vertex and writes the mixture into frame->guideLobes (lobe k: plane
+ * 2k = (mu, kappa), plane 2k+1 = (pi kappa / (2 pi (1 - e^{-2 kappa})),
+ * e^{-2 kappa}, cumulative pi, pi)); background pixels are not written. */
+rt_status rt_gris_guide_refresh(const rt_device_scene* scene, const rt_mlp* guide, const rt_camera cams[2],
+                                const rt_frame* frame, hipStream_t stream);
+/* After the frame's passes: gathers the training batch (primary position,
+ * sampled direction, path luminance) from frame->guideSamples at a fixed
+ * stride of pixels (offset by the camera seed) and takes one training step;
+ * loss (device, may be null) receives the batch's mean loss before the step. */
+rt_status rt_gris_guide_train(const rt_device_scene* scene, rt_mlp* guide, const rt_camera cams[2],
+                              const rt_frame* frame, float* loss, hipStream_t stream);
 
 /* ---- Frame graphs -------------------------------------------------------
  * The per-frame command recording of Renderer::recordRenderCommand

@@ -4,6 +4,7 @@
 // C entry points over the scalar restatement, so that Python tests can drive it
 // with ctypes. Frames are split into row bands over std::thread workers.
 #include "orc_passes.h"
+#include "orc_guide.h"
 #include <atomic>
 #include <thread>
 
@@ -13,6 +14,10 @@ namespace {
 
 struct OScene {
     SceneArrays S;
+    // learned-proposal bindings of the GRIS passes (orc_gris_bind_guide)
+    const float* guideLobes = nullptr;
+    const float* guideLobesPrev = nullptr;
+    float* guideSamples = nullptr;
     // private copies so the caller's arrays may be freed
     std::vector<Material> materials;
     std::vector<int32_t> materialIndices;
@@ -256,6 +261,30 @@ void orc_di_spatial(void* h, const Camera* cams, const float* depthNormal, const
 }
 
 // ---- GRIS passes ----------------------------------------------------------------------------
+// Learned-proposal planes for the following GRIS calls with set->guidedProposal = 1 (float4
+// planes of W*H pixels as the kernels lay them out: 16 lobe planes for this and the previous
+// frame, one record plane written by the path tracer); null unbinds.
+void orc_gris_bind_guide(void* h, const float* lobes, const float* lobesPrev, float* samples) {
+    OScene* s = (OScene*)h;
+    s->guideLobes = lobes; s->guideLobesPrev = lobesPrev; s->guideSamples = samples;
+}
+
+// Mixture pdf and samples of one pixel's lobes (lobes: 16 float4 of that pixel, plane-major as
+// 16 x 4 floats): pdf at n directions, and n samples from (uLobe, u1, u2) triples.
+void orc_guide_mixture(const float* lobes16x4, const float* dirs, uint32_t n, float* pdf, const float* u3, uint32_t m,
+                       float* out) {
+    GuidePix g = guideAt(lobes16x4, 1, 0);
+    for (uint32_t i = 0; i < n; i++) pdf[i] = mixturePdf(g, V3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+    for (uint32_t i = 0; i < m; i++) {
+        vec3 w = mixtureSample(g, u3[3 * i], u3[3 * i + 1], u3[3 * i + 2]);
+        out[3 * i] = w.x; out[3 * i + 1] = w.y; out[3 * i + 2] = w.z;
+    }
+}
+
+// The oracle's exp and log on an array (mode 0 exp, 1 log).
+void orc_explog(const float* x, uint32_t n, int mode, float* y) {
+    for (uint32_t i = 0; i < n; i++) y[i] = mode ? rt_log(x[i]) : rt_exp(x[i]);
+}
 // cams[0] = uCamera, cams[1] = uPrevCamera
 void orc_gris_path_trace(void* h, const Camera* cams, const float* depthNormal, const uint32_t* albedoMatId,
                          const GRISSettings* set, GRISReservoir* resvThis, uint64_t* stats3, int nthreads) {
@@ -265,7 +294,7 @@ void orc_gris_path_trace(void* h, const Camera* cams, const float* depthNormal, 
     GBufView g{depthNormal, albedoMatId, nullptr, W, H};
     std::vector<TraceStats> st(nthreads > 1 ? nthreads : 1);
     parallelRows(H, nthreads, [&](uint32_t y, int t) {
-        Ctx c{&s->S, &s->bvh, &st[t]};
+        Ctx c{&s->S, &s->bvh, &st[t], s->guideLobes, s->guideLobesPrev, s->guideSamples};
         for (uint32_t x = 0; x < W; x++) {
             GRISReservoir r; bool written;
             tracePath(c, cam, g, *set, x, y, r, written);
@@ -289,7 +318,7 @@ void orc_gris_temporal(void* h, const Camera* cams, const float* depthNormal, co
     GBufView gp{depthNormalPrev, albedoMatIdPrev, nullptr, W, H};
     std::vector<TraceStats> st(nthreads > 1 ? nthreads : 1);
     parallelRows(H, nthreads, [&](uint32_t y, int t) {
-        Ctx c{&s->S, &s->bvh, &st[t]};
+        Ctx c{&s->S, &s->bvh, &st[t], s->guideLobes, s->guideLobesPrev, s->guideSamples};
         for (uint32_t x = 0; x < W; x++) {
             GRISReservoir r;
             LearnedData l{};
@@ -312,7 +341,7 @@ void orc_gris_spatial(void* h, const Camera* cams, const float* depthNormal, con
     GBufView g{depthNormal, albedoMatId, nullptr, W, H};
     std::vector<TraceStats> st(nthreads > 1 ? nthreads : 1);
     parallelRows(H, nthreads, [&](uint32_t y, int t) {
-        Ctx c{&s->S, &s->bvh, &st[t]};
+        Ctx c{&s->S, &s->bvh, &st[t], s->guideLobes, s->guideLobesPrev, s->guideSamples};
         for (uint32_t x = 0; x < W; x++) {
             size_t p = (size_t)y * W + x;
             GRISReservoir r; bool written;

@@ -105,6 +105,45 @@ static inline float rt_cos(float x) {
 }
 static inline float rt_tan(float x) { return rt_sin(x) / rt_cos(x); }
 
+// exp and log of the learned proposal (no counterpart in the reference), the definitions the
+// kernels use: exp by Cody-Waite reduction to |r| <= ln(2)/2 and the Taylor series through r^7;
+// log by x = m 2^e, m in [sqrt(1/2), sqrt(2)), and log(m) = 2 atanh((m-1)/(m+1)) through s^11.
+static inline float rt_exp(float x) {
+    if (x != x) return x;
+    if (x > 88.0f) return INFINITY;
+    if (!(x > -87.0f)) return 0.0f;
+    const float k = std::nearbyint(x * 1.44269504088896341f);
+    float r = x - k * 0.693145751953125f;
+    r = r - k * 1.42860682030941723212e-6f;
+    float p = 1.984126984126984e-4f;
+    p = p * r + 1.388888888888889e-3f;
+    p = p * r + 8.333333333333333e-3f;
+    p = p * r + 4.166666666666667e-2f;
+    p = p * r + 1.666666666666667e-1f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    return std::ldexp(p, (int)k);
+}
+static inline float rt_log(float x) {
+    if (x != x || x < 0.0f) return NAN;
+    if (x == 0.0f) return -INFINITY;
+    if (x == INFINITY) return x;
+    int e;
+    float m = std::frexp(x, &e);
+    if (m < 0.70710678118654752f) { m = m * 2.0f; e = e - 1; }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float z = s * s;
+    float p = 0.18181818181818182f;
+    p = p * z + 0.22222222222222222f;
+    p = p * z + 0.28571428571428571f;
+    p = p * z + 0.4f;
+    p = p * z + 0.66666666666666667f;
+    p = p * z + 2.0f;
+    const float fe = (float)e;
+    return fe * 0.693145751953125f + (fe * 1.42860682030941723212e-6f + s * p);
+}
+
 // ---- math.glsl ---------------------------------------------------------------
 static const float Pi = 3.14159265358979323846f;
 static const float PiInv = 1.0f / 3.14159265358979323846f;

@@ -1,6 +1,7 @@
 // ORACLE — test infrastructure only. Nothing in the product links or calls this.
 // Transliteration of the per-pixel GLSL passes; see orc_passes.h for sources.
 #include "orc_passes.h"
+#include "orc_guide.h"
 #include <vector>
 
 namespace orc {
@@ -243,6 +244,9 @@ vec3 tracePath(const Ctx& c, const Camera& cam, const GBufView& g, const GRISSet
     // StreamSampler (gris_path_trace.glsl:10-33)
     GRISPathSample streamSample; std::memset(&streamSample, 0, sizeof(streamSample));
     float streamWeight = 0.0f, streamSum = 0.0f;
+    const GuidePix gpx = (set.guidedProposal && c.guideLobes) ? guideAt(c.guideLobes, g.W * g.H, index1D(g.W, x, y))
+                                                              : noGuide();
+    vec3 dir0 = V3s(0.0f);
     auto addStream = [&](const GRISPathSample& ps, float weight, float r) {
         streamSum += weight;
         if (r * streamSum < weight) { streamWeight = weight; streamSample = ps; }
@@ -354,7 +358,16 @@ vec3 tracePath(const Ctx& c, const Camera& cam, const GBufView& g, const GRISSet
             rcThroughput = rcThroughput / (1.0f - pdfTerminate);
         }
 
-        if (!sampleBSDF(mat, surf.albedo, surf.norm, wo, sample3f(rng), s) || s.pdf < 1e-6f) break;
+        bool sampled;
+        if (bounce == 0 && gpx.L) {  // learned proposal at the primary vertex
+            const vec4 gu = sample4f(rng);
+            const vec3 rb = sample3f(rng);
+            sampled = guidedSampleBSDF(gpx, mat, surf.albedo, surf.norm, wo, gu, rb, s);
+            dir0 = s.wi;
+        } else {
+            sampled = sampleBSDF(mat, surf.albedo, surf.norm, wo, sample3f(rng), s);
+        }
+        if (!sampled || s.pdf < 1e-6f) break;
         float cosTheta = isSampleTypeDelta(s.type) ? 1.0f : absDot(surf.norm, s.wi);
         vec3 scatterTerm = s.bsdf * cosTheta / s.pdf;
         throughput = throughput * scatterTerm;
@@ -390,6 +403,10 @@ vec3 tracePath(const Ctx& c, const Camera& cam, const GBufView& g, const GRISSet
     resv.sampleCount = 1.0f;
     out = resv;
     written = true;
+    if (set.guidedProposal && c.guideSamples) {
+        float* rec = c.guideSamples + 4 * (size_t)index1D(g.W, x, y);
+        rec[0] = dir0.x; rec[1] = dir0.y; rec[2] = dir0.z; rec[3] = streamSum;
+    }
     if (!pathSampleIsValid(pathSample)) return V3s(0.0f);
     return radiance;
 }
@@ -399,7 +416,7 @@ vec3 tracePath(const Ctx& c, const Camera& cam, const GBufView& g, const GRISSet
 struct RcData { Intersection rcPrevIsec; vec3 rcPrevWo; vec3 rcPrevThroughput; };
 
 static void traceReplayPathForHybridShift(const Ctx& c, const GRISSettings& set, Intersection isec, SurfaceInfo surf,
-                                          Ray ray, uint32_t targetFlags, uint32_t rng, RcData& rc) {
+                                          Ray ray, uint32_t targetFlags, uint32_t rng, RcData& rc, const GuidePix& gpx) {
     const SceneArrays& S = *c.S;
     vec3 throughput = V3s(1.0f);
     vec3 wo = -ray.dir;
@@ -437,7 +454,15 @@ static void traceReplayPathForHybridShift(const Ctx& c, const GRISSettings& set,
             if (sample1f(rng) < pdfTerminate) break;
             throughput = throughput / (1.0f - pdfTerminate);
         }
-        if (!sampleBSDF(mat, surf.albedo, surf.norm, wo, sample3f(rng), s) || s.pdf < 1e-6f) break;
+        bool sampled;
+        if (bounce == 0 && gpx.L) {  // the destination's learned proposal, same random numbers
+            const vec4 gu = sample4f(rng);
+            const vec3 rb = sample3f(rng);
+            sampled = guidedSampleBSDF(gpx, mat, surf.albedo, surf.norm, wo, gu, rb, s);
+        } else {
+            sampled = sampleBSDF(mat, surf.albedo, surf.norm, wo, sample3f(rng), s);
+        }
+        if (!sampled || s.pdf < 1e-6f) break;
         float cosTheta = isSampleTypeDelta(s.type) ? 1.0f : absDot(surf.norm, s.wi);
         throughput = throughput * (s.bsdf * cosTheta / s.pdf);
         wo = -s.wi;
@@ -482,7 +507,8 @@ static vec3 targetRcLi(const GRISPathSample& ps, const LearnedData& L) {
 // values and the picked sample's learned data moves with it.
 static void reuseAndMerge(const Ctx& c, const GRISSettings& set, GRISReservoir& dstResv, LearnedData* dstL,
                           const SurfaceInfo& dstPrimarySurf, const Intersection& dstPrimaryIsec, const Ray& primaryRay,
-                          GRISReservoir srcResv, LearnedData srcL, bool learned, uint32_t& rng, bool unbiased = false) {
+                          GRISReservoir srcResv, LearnedData srcL, bool learned, uint32_t& rng, bool unbiased,
+                          const GuidePix& gpx) {
     const SceneArrays& S = *c.S;
     GRISPathSample srcSample = srcResv.pathSample;
     RcData dstRc;
@@ -495,7 +521,7 @@ static void reuseAndMerge(const Ctx& c, const GRISSettings& set, GRISReservoir& 
 
     if (pathSampleIsValid(srcSample)) {
         traceReplayPathForHybridShift(c, set, dstPrimaryIsec, dstPrimarySurf, primaryRay, srcSample.flags,
-                                      srcSample.primaryRng, dstRc);
+                                      srcSample.primaryRng, dstRc, gpx);
         if (dstRc.rcPrevIsec.instanceIdx != InvalidHitIndex) {
             if (dstRc.rcPrevIsec.instanceIdx == SpecialHitIndex) rcPrevSurf = dstPrimarySurf;
             else loadSurfaceInfo(S, dstRc.rcPrevIsec, rcPrevSurf);
@@ -528,7 +554,8 @@ static void reuseAndMerge(const Ctx& c, const GRISSettings& set, GRISReservoir& 
                 float sumPower = S.lightTable[0].prob;
                 dstSamplePdf = luminance(rcSurf.albedo) / sumPower / dstJacobian;
             } else {
-                dstSamplePdf = evalPdf(rcPrevMat, rcPrevSurf.norm, dstRc.rcPrevWo, wi);
+                dstSamplePdf = guidedPdf(dstRc.rcPrevIsec.instanceIdx == SpecialHitIndex ? gpx : noGuide(), rcPrevMat,
+                                         rcPrevSurf.norm, dstRc.rcPrevWo, wi);
             }
         }
         float srcPHat = learned ? srcL.pRef : luminance(L3(srcSample.F));
@@ -562,11 +589,12 @@ static void reuseAndMerge(const Ctx& c, const GRISSettings& set, GRISReservoir& 
 // sample's own contribution there (with its pdf at the reconnection predecessor)
 // has a positive target value.
 static bool shiftSupports(const Ctx& c, const GRISSettings& set, const GRISPathSample& y, const LearnedData& yL,
-                          bool learned, const SurfaceInfo& primSurf, const Intersection& primIsec, const Ray& primRay) {
+                          bool learned, const SurfaceInfo& primSurf, const Intersection& primIsec, const Ray& primRay,
+                          const GuidePix& gpx) {
     const SceneArrays& S = *c.S;
     if (!pathSampleIsValid(y)) return false;
     RcData rc;
-    traceReplayPathForHybridShift(c, set, primIsec, primSurf, primRay, y.flags, y.primaryRng, rc);
+    traceReplayPathForHybridShift(c, set, primIsec, primSurf, primRay, y.flags, y.primaryRng, rc, gpx);
     if (rc.rcPrevIsec.instanceIdx == InvalidHitIndex) return false;
     SurfaceInfo rcPrevSurf{}, rcSurf{};
     if (rc.rcPrevIsec.instanceIdx == SpecialHitIndex) rcPrevSurf = primSurf;
@@ -584,7 +612,8 @@ static bool shiftSupports(const Ctx& c, const GRISSettings& set, const GRISPathS
     GRISPathSample x = y;
     x.rcPrevSamplePdf = flagsRcVertexType(y.flags) == RcVertexTypeLightSampled
                             ? luminance(rcSurf.albedo) / S.lightTable[0].prob / jac
-                            : evalPdf(rcPrevMat, rcPrevSurf.norm, rc.rcPrevWo, wi);
+                            : guidedPdf(rc.rcPrevIsec.instanceIdx == SpecialHitIndex ? gpx : noGuide(), rcPrevMat,
+                                        rcPrevSurf.norm, rc.rcPrevWo, wi);
     if (std::isnan(x.rcPrevSamplePdf) || !(x.rcPrevSamplePdf > 1e-6f)) return false;
     const float misW = rcLightMIS(S, flagsRcVertexType(y.flags), rcPrevSurf, rc.rcPrevWo, rcSurf, wi, jac);
     vec3 Li = reconnectLi(x, L3(x.rcLi) * misW, rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, rc);
@@ -690,7 +719,9 @@ bool temporalReuse(const Ctx& c, const Camera& cam, const Camera& prevCam, const
             findPreviousReservoir(cam, prevCam, gPrev, resvPrev, uv + motion, pos, norm, matMeshId, temporalResv, pidx)) {
             if (reservoirIsValid(temporalResv)) {
                 reuseAndMerge(c, set, resv, learned ? &L : nullptr, dstPrimarySurf, dstPrimaryIsec, ray, temporalResv,
-                              learnedAt(learned ? learnPrev : nullptr, pidx), learned, resvRng, unbiased);
+                              learnedAt(learned ? learnPrev : nullptr, pidx), learned, resvRng, unbiased,
+                              (set.guidedProposal && c.guideLobes) ? guideAt(c.guideLobes, g.W * g.H, index1D(g.W, x, y))
+                                                                   : noGuide());
                 mPrev = temporalResv.sampleCount;
                 vec2 puv = uv + motion;
                 prevX = (int)(puv.x * (float)cam.filmSize[0]);
@@ -702,8 +733,11 @@ bool temporalReuse(const Ctx& c, const Camera& cam, const Camera& prevCam, const
         if (mPrev > 0.0f && reservoirIsValid(resv) && pathSampleIsValid(resv.pathSample)) {
             float z = mCanon;
             SurfaceInfo ps{}; Intersection pi{}; Ray pr{};
+            const GuidePix prevGp = (set.guidedProposal && c.guideLobesPrev)
+                                        ? guideAt(c.guideLobesPrev, g.W * g.H, (uint32_t)prevY * g.W + (uint32_t)prevX)
+                                        : noGuide();
             if (primaryOf(prevCam, gPrev, prevX, prevY, ps, pi, pr) &&
-                shiftSupports(c, set, resv.pathSample, L, learned, ps, pi, pr))
+                shiftSupports(c, set, resv.pathSample, L, learned, ps, pi, pr, prevGp))
                 z += mPrev;
             normaliseZ(resv, z);
         }
@@ -756,6 +790,8 @@ vec3 spatialReuse(const Ctx& c, const Camera& cam, const GBufView& g, const GRIS
     SurfaceInfo dstPrimarySurf{pos, norm, albedo, (uint32_t)matId, false};
     const bool unbiased = set.unbiasedReuse != 0u;
     const float mCanon = resv.sampleCount;
+    const bool guided = set.guidedProposal && c.guideLobes;
+    const GuidePix gpx = guided ? guideAt(c.guideLobes, g.W * g.H, index1D(g.W, x, y)) : noGuide();
     std::vector<std::pair<int, int>> nbPix;
     std::vector<float> nbM;
     if (set.spatialReuse) {
@@ -769,7 +805,7 @@ vec3 spatialReuse(const Ctx& c, const Camera& cam, const GBufView& g, const GRIS
             if (findNeighborReservoir(cam, g, resvTemp, neighbor, pos, norm, neighborResv, nidx)) {
                 if (reservoirIsValid(neighborResv)) {
                     reuseAndMerge(c, set, resv, learned ? &L : nullptr, dstPrimarySurf, dstPrimaryIsec, ray, neighborResv,
-                                  learnedAt(learned ? learnTemp : nullptr, nidx), learned, rng, unbiased);
+                                  learnedAt(learned ? learnTemp : nullptr, nidx), learned, rng, unbiased, gpx);
                     if (unbiased && nbPix.size() < UnbiasedMaxNeighbors) {
                         nbPix.emplace_back((int)(neighbor.x * (float)cam.filmSize[0]),
                                            (int)(neighbor.y * (float)cam.filmSize[1]));
@@ -785,8 +821,11 @@ vec3 spatialReuse(const Ctx& c, const Camera& cam, const GBufView& g, const GRIS
             for (size_t j = 0; j < nbPix.size(); j++) {
                 if (!(nbM[j] > 0.0f)) continue;
                 SurfaceInfo ns{}; Intersection ni{}; Ray nr{};
+                const GuidePix ngp = guided ? guideAt(c.guideLobes, g.W * g.H,
+                                                      (uint32_t)nbPix[j].second * g.W + (uint32_t)nbPix[j].first)
+                                            : noGuide();
                 if (primaryOf(cam, g, nbPix[j].first, nbPix[j].second, ns, ni, nr) &&
-                    shiftSupports(c, set, resv.pathSample, L, learned, ns, ni, nr))
+                    shiftSupports(c, set, resv.pathSample, L, learned, ns, ni, nr, ngp))
                     z += nbM[j];
             }
             normaliseZ(resv, z);
@@ -801,7 +840,7 @@ vec3 spatialReuse(const Ctx& c, const Camera& cam, const GBufView& g, const GRIS
         GRISPathSample ps = resv.pathSample;
         if (pathSampleIsValid(ps)) {
             RcData rc;
-            traceReplayPathForHybridShift(c, set, dstPrimaryIsec, dstPrimarySurf, ray, ps.flags, ps.primaryRng, rc);
+            traceReplayPathForHybridShift(c, set, dstPrimaryIsec, dstPrimarySurf, ray, ps.flags, ps.primaryRng, rc, gpx);
             if (rc.rcPrevIsec.instanceIdx != InvalidHitIndex) {
                 SurfaceInfo rcPrevSurf{}, rcSurf{};
                 if (rc.rcPrevIsec.instanceIdx == SpecialHitIndex) rcPrevSurf = dstPrimarySurf;

@@ -27,6 +27,7 @@ struct GRISSettings {
     float spatialRadius;        // ResampleRadius (gris_resample_spatial.glsl:66, 20)
     uint32_t targetFunction;    // 0 luminance (GRISToScalar), 1 learned
     uint32_t unbiasedReuse;     // 0 the reference's 1/M weights, 1 the "1/Z" contribution weights
+    uint32_t guidedProposal;    // 1: learned vMF-mixture proposal at the primary vertex (orc_guide.h)
 };
 static const uint32_t UnbiasedMaxNeighbors = 8;
 // Learned-target data of a reservoir: predicted reconnection radiance and the
@@ -49,6 +50,12 @@ struct Ctx {
     const SceneArrays* S;
     const BVH* B;
     TraceStats* st;
+    // learned proposal (settings.guidedProposal): lobe planes of this and the previous
+    // frame (float4 planes of W*H pixels) and the per-pixel training record written by
+    // tracePath (direction sampled at the primary vertex, stream-RIS weight sum)
+    const float* guideLobes = nullptr;
+    const float* guideLobesPrev = nullptr;
+    float* guideSamples = nullptr;
 };
 
 static inline uint32_t index1D(uint32_t W, uint32_t x, uint32_t y) { return W * y + x; }

@@ -63,6 +63,15 @@ def lib():
         L.orc_mlp_loss.argtypes = [u32, u32, u32, P, P, P, P, P, P, u32, P, P]
         L.orc_mlp_loss.restype = C.c_double
         L.orc_mlp_levels.argtypes = [u32, u32, u32, P]
+        L.orc_gris_bind_guide.argtypes = [P, P, P, P]
+        L.orc_guide_mixture.argtypes = [P, P, u32, P, P, u32, P]
+        L.orc_explog.argtypes = [P, u32, C.c_int, P]
+        L.orc_guide_forward.argtypes = [u32, u32, u32, P, P, P, P, P, u32, P, C.c_int]
+        L.orc_guide_lobes.argtypes = [P, u32, P]
+        L.orc_guide_gradients.argtypes = [u32, u32, u32, P, P, P, P, P, P, u32, P, P, C.c_int]
+        L.orc_guide_gradients.restype = C.c_double
+        L.orc_guide_loss.argtypes = [u32, u32, u32, P, P, P, P, P, P, u32, C.c_double]
+        L.orc_guide_loss.restype = C.c_double
         L.orc_mlp_levels.restype = u32
         L.orc_hash2.argtypes = [u32]
         L.orc_hash2.restype = u32
@@ -197,6 +206,12 @@ class OracleScene:
                               p(indirect), p(st), nthreads())
         return st
 
+    def bind_guide(self, lobes=None, lobes_prev=None, samples=None):
+        """Learned-proposal planes for the following GRIS calls (arrays kept alive by the caller):
+        lobes / lobes_prev float32 [16, H*W, 4], samples float32 [H*W, 4] (written by the path tracer)."""
+        self._guide = (lobes, lobes_prev, samples)
+        lib().orc_gris_bind_guide(self.h, p(lobes), p(lobes_prev), p(samples))
+
     def learned_refresh(self, W, H, dn, resv, lhat4, stride, offset):
         """Refreshes resv in place from the predictions lhat4 (H*W x 4); returns (learned planes, batch pos,
         batch target) with the batch over pixels p % stride == offset."""
@@ -289,6 +304,58 @@ class OracleMlp:
         target = np.ascontiguousarray(target, np.float32)
         return lib().orc_mlp_loss(self.log2, self.base, self.max, p(self.lo), p(self.hi), p(grid), p(weights), p(pos),
                                   p(target), len(pos), p(den_in), p(den_out))
+
+
+class OracleGuide(OracleMlp):
+    """The learned proposal's network: 32 raw outputs per position, the vMF-mixture loss."""
+
+    def forward(self, grid, weights, pos, mode=1):
+        pos = np.ascontiguousarray(pos, np.float32)
+        out = np.zeros((len(pos), 32), np.float32)
+        lib().orc_guide_forward(self.log2, self.base, self.max, p(self.lo), p(self.hi), p(grid), p(weights), p(pos),
+                                len(pos), p(out), mode)
+        return out
+
+    def gradients(self, grid, weights, pos, target, mode=1):
+        pos = np.ascontiguousarray(pos, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        gg = np.zeros(len(grid), np.float64)
+        gw = np.zeros(len(weights), np.float64)
+        loss = lib().orc_guide_gradients(self.log2, self.base, self.max, p(self.lo), p(self.hi), p(grid), p(weights),
+                                         p(pos), p(target), len(pos), p(gg), p(gw), mode)
+        return loss, gg, gw
+
+    def loss(self, grid, weights, pos, target, wmean=0.0):
+        pos = np.ascontiguousarray(pos, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        return lib().orc_guide_loss(self.log2, self.base, self.max, p(self.lo), p(self.hi), p(grid), p(weights),
+                                    p(pos), p(target), len(pos), wmean)
+
+
+def guide_lobes(raw):
+    """Lobe planes of raw proposal outputs: [n, 16, 4] (per sample the kernels' 16 float4)."""
+    raw = np.ascontiguousarray(raw, np.float32)
+    out = np.zeros((len(raw), 16, 4), np.float32)
+    lib().orc_guide_lobes(p(raw), len(raw), p(out))
+    return out
+
+
+def mixture(lobes16x4, dirs, u3):
+    """One pixel's mixture: pdf at dirs [n, 3] and samples for u3 [m, 3] (lobe choice, u1, u2)."""
+    lobes = np.ascontiguousarray(lobes16x4, np.float32)
+    dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    u3 = np.ascontiguousarray(u3, np.float32).reshape(-1, 3)
+    pdf = np.zeros(len(dirs), np.float32)
+    out = np.zeros((len(u3), 3), np.float32)
+    lib().orc_guide_mixture(p(lobes), p(dirs), len(dirs), p(pdf), p(u3), len(u3), p(out))
+    return pdf, out
+
+
+def explog(x, log=False):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros_like(x)
+    lib().orc_explog(p(x), x.size, int(log), p(y))
+    return y
 
 
 def reset_reservoirs(n):

@@ -58,6 +58,7 @@ rt_status checkView(const rt_view& v) {
         return invalid("invalid view settings");
     if (s.targetFunction != 0u) return invalid("batched views use the luminance target");
     if (s.unbiasedReuse != 0u) return invalid("batched views use the reference's resampling weights");
+    if (s.guidedProposal != 0u) return invalid("batched views sample with the BSDFs (no learned proposal)");
     return RT_OK;
 }
 

@@ -2,6 +2,7 @@
 // include/restirpt.h). Validates arguments on the host, owns the device scene
 // and launches the kernels of rt_bvh.hip / rt_passes.hip.
 #include "rt_internal.h"
+#include "rt_guide.h"
 #include <cstring>
 #include <mutex>
 
@@ -32,6 +33,10 @@ hipError_t launchGrisSpatial(const rt_device_scene*, const rt_camera*, const rt_
                              hipStream_t);
 hipError_t launchResvReset(void*, uint32_t, uint32_t, hipStream_t);
 hipError_t launchLearnedRefresh(const rt_device_scene*, rt_mlp*, const rt_camera*, const rt_frame&, hipStream_t);
+hipError_t launchGuideRefresh(const rt_mlp*, const rt_camera*, const rt_frame&, hipStream_t);
+hipError_t launchGuideGather(rt_mlp*, const rt_camera*, const rt_frame&, hipStream_t);
+hipError_t mlpGradients(rt_mlp* m, const float* pos, const float* target, uint32_t n, float* loss, hipStream_t st);
+hipError_t mlpAdam(rt_mlp* m, hipStream_t st);
 hipError_t launchDiPathGen(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_di_reservoirs&,
                            const rt_di_settings&, hipStream_t);
 hipError_t launchDiTemporal(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_di_reservoirs&,
@@ -132,6 +137,14 @@ rt_status checkSettings(const rt_gris_settings* st) {
     if (st->unbiasedReuse && st->spatialNeighbors > 8)
         return invalid("unbiasedReuse supports at most 8 spatialNeighbors");
     if (st->targetFunction > 1) return invalid("targetFunction must be 0 (luminance) or 1 (learned)");
+    if (st->guidedProposal > 1) return invalid("guidedProposal must be 0 or 1");
+    return RT_OK;
+}
+
+// The learned proposal's frame bindings, when the settings turn it on.
+rt_status checkGuide(const rt_frame* f, const rt_gris_settings* st) {
+    if (st->guidedProposal && (!f->guideLobes || !f->guideLobesPrev || !f->guideSamples))
+        return invalid("guidedProposal needs guideLobes, guideLobesPrev and guideSamples");
     return RT_OK;
 }
 
@@ -273,6 +286,7 @@ rt_status rt_gris_path_trace(const rt_device_scene* s, const rt_camera cams[2], 
                              const rt_gris_settings* set, hipStream_t stream) {
     rt_status st = checkFrame(s, cams, f);
     if (st == RT_OK) st = checkSettings(set);
+    if (st == RT_OK) st = checkGuide(f, set);
     if (st != RT_OK) return st;
     if (!f->grisReservoir) return invalid("grisReservoir is null");
     if (s->numLights == 0) return invalid("the scene has no lights");
@@ -284,6 +298,7 @@ rt_status rt_gris_temporal(const rt_device_scene* s, const rt_camera cams[2], co
                            const rt_gris_settings* set, hipStream_t stream) {
     rt_status st = checkFrame(s, cams, f);
     if (st == RT_OK) st = checkSettings(set);
+    if (st == RT_OK) st = checkGuide(f, set);
     if (st != RT_OK) return st;
     if (!f->grisReservoir || !f->grisReservoirPrev || !f->grisReservoirTemp) return invalid("reservoir buffer is null");
     if (!f->depthNormalPrev || !f->albedoMatIdPrev || !f->motionVector) return invalid("previous G-buffer is null");
@@ -296,6 +311,7 @@ rt_status rt_gris_spatial(const rt_device_scene* s, const rt_camera cams[2], con
                           const rt_gris_settings* set, hipStream_t stream) {
     rt_status st = checkFrame(s, cams, f);
     if (st == RT_OK) st = checkSettings(set);
+    if (st == RT_OK) st = checkGuide(f, set);
     if (st != RT_OK) return st;
     if (!f->grisReservoir || !f->grisReservoirTemp) return invalid("reservoir buffer is null");
     if (!f->indirectOutput) return invalid("indirectOutput is null");
@@ -504,6 +520,33 @@ rt_status rt_gris_learned_refresh(const rt_device_scene* s, rt_mlp* mlp, const r
     if (!mlp) return invalid("mlp is null");
     if (!f->grisReservoir) return invalid("frame reservoir is null");
     RT_LAUNCH(rti::launchLearnedRefresh(s, mlp, cams, *f, stream));
+    return RT_OK;
+}
+
+size_t rt_guide_lobe_bytes(uint32_t w, uint32_t h) { return (size_t)w * h * rtg::kPlanes * sizeof(float4); }
+
+rt_status rt_gris_guide_refresh(const rt_device_scene* s, const rt_mlp* guide, const rt_camera cams[2], const rt_frame* f,
+                                hipStream_t stream) {
+    rt_status st = checkFrame(s, cams, f);
+    if (st != RT_OK) return st;
+    if (!guide) return invalid("guide is null");
+    if (guide->outputs != (uint32_t)rtm::kGuideOut) return invalid("guide is not a proposal network (rt_guide_create)");
+    if (!f->guideLobes) return invalid("frame guideLobes is null");
+    RT_LAUNCH(rti::launchGuideRefresh(guide, cams, *f, stream));
+    return RT_OK;
+}
+
+rt_status rt_gris_guide_train(const rt_device_scene* s, rt_mlp* guide, const rt_camera cams[2], const rt_frame* f,
+                              float* loss, hipStream_t stream) {
+    rt_status st = checkFrame(s, cams, f);
+    if (st != RT_OK) return st;
+    if (!guide) return invalid("guide is null");
+    if (guide->outputs != (uint32_t)rtm::kGuideOut) return invalid("guide is not a proposal network (rt_guide_create)");
+    if (!f->guideSamples) return invalid("frame guideSamples is null");
+    RT_LAUNCH(rti::launchGuideGather(guide, cams, *f, stream));
+    if (guide->batchSize == 0) return RT_OK;
+    RT_LAUNCH(rti::mlpGradients(guide, guide->batchPos, guide->batchTarget, guide->batchSize, loss, stream));
+    RT_LAUNCH(rti::mlpAdam(guide, stream));
     return RT_OK;
 }
 

@@ -94,6 +94,48 @@ RT_DEV float rt_cos(float x) {
 }
 RT_DEV float rt_tan(float x) { return rt_sin(x) / rt_cos(x); }
 
+// ---- exp / log -----------------------------------------------------------------------
+// exp(x) = 2^k e^r with k = rint(x / ln 2) and r = x - k ln 2 taken in two parts (Cody and
+// Waite's additive range reduction); e^r by its Taylor series through r^7 (|r| <= ln(2)/2:
+// truncation below 2^-27 relative). Results below FLT_MIN are returned as 0.
+RT_DEV float rt_exp(float x) {
+    if (x != x) return x;
+    if (x > 88.0f) return INFINITY;
+    if (!(x > -87.0f)) return 0.0f;
+    const float k = rintf(x * 1.44269504088896341f);
+    float r = x - k * 0.693145751953125f;
+    r = r - k * 1.42860682030941723212e-6f;
+    float p = 1.984126984126984e-4f;
+    p = p * r + 1.388888888888889e-3f;
+    p = p * r + 8.333333333333333e-3f;
+    p = p * r + 4.166666666666667e-2f;
+    p = p * r + 1.666666666666667e-1f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    return ldexpf(p, (int)k);
+}
+// log(x) = e ln 2 + log(m) with x = m 2^e, m in [sqrt(1/2), sqrt(2)); log(m) = 2 atanh(s),
+// s = (m - 1) / (m + 1), |s| <= 0.1716, by the odd series 2 (s + s^3/3 + ... + s^11/11).
+RT_DEV float rt_log(float x) {
+    if (x != x || x < 0.0f) return NAN;
+    if (x == 0.0f) return -INFINITY;
+    if (x == INFINITY) return x;
+    int e;
+    float m = frexpf(x, &e);
+    if (m < 0.70710678118654752f) { m = m * 2.0f; e = e - 1; }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float z = s * s;
+    float p = 0.18181818181818182f;
+    p = p * z + 0.22222222222222222f;
+    p = p * z + 0.28571428571428571f;
+    p = p * z + 0.4f;
+    p = p * z + 0.66666666666666667f;
+    p = p * z + 2.0f;
+    const float fe = (float)e;
+    return fe * 0.693145751953125f + (fe * 1.42860682030941723212e-6f + s * p);
+}
+
 // ---- math.glsl -----------------------------------------------------------------------
 #define RT_PI 3.14159265358979323846f
 #define RT_PI_INV (1.0f / 3.14159265358979323846f)

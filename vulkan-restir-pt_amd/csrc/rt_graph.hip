@@ -110,6 +110,7 @@ rt_status rt_frame_graph_create(const rt_device_scene* scene, const rt_frame fra
         settings->spatialNeighbors > 64 || settings->targetFunction > 1)
         return invalid("invalid settings");
     if (settings->targetFunction == 1u && !mlp) return invalid("the learned target needs a network");
+    if (settings->guidedProposal != 0u) return invalid("frame graphs do not record the learned proposal");
     for (int j = 0; j < 2; j++) {
         const rt_frame& f = frames[j];
         if (f.width == 0 || f.height == 0 || f.width >= 32768 || f.height >= 32768 || f.width != frames[0].width ||

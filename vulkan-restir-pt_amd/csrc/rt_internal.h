@@ -42,6 +42,11 @@ struct rt_device_scene {
 // training batch, all on the device.
 struct rt_mlp {
     rt_mlp_config cfg{};
+    // 3: the radiance network of the learned target; 32: the vMF-mixture proposal network
+    uint32_t outputs = rtm::kOut;
+    uint32_t weights = rtm::kWeights;   // FP16/FP32 weight count for `outputs`
+    uint32_t targetStride = 3;          // floats per training target (RGB, or direction + weight)
+    float* wmean = nullptr;             // proposal network: mean sample weight of the last batch
     rtm::MlpView view{};          // kernels' view of the FP16 copies
     uint64_t gridParams = 0;      // levels * T * 2
     float* gridF32 = nullptr;     // FP32 master grid

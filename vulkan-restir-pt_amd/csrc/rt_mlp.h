@@ -21,9 +21,37 @@ constexpr int kFeatures = 2;
 constexpr int kEnc = kLevels * kFeatures;  // 32
 constexpr int kHidden = 64;
 constexpr int kOut = 3;
-// FP16 weight block, input-major: W1 [32][64], W2 [64][64], W3 [64][3]
+// FP16 weight block, input-major: W1 [32][64], W2 [64][64], W3 [64][outputs]
 constexpr int kW1 = 0, kW2 = kEnc * kHidden, kW3 = kW2 + kHidden * kHidden;
-constexpr int kWeights = kW3 + kHidden * kOut;  // 6336
+constexpr int weightsFor(int outputs) { return kW3 + kHidden * outputs; }
+constexpr int kWeights = weightsFor(kOut);  // 6336
+// The learned proposal's network: the same encoding and hidden layers with 32
+// outputs, four per lobe of an 8-lobe von Mises-Fisher mixture (weight logit,
+// log concentration, polar and azimuth parameters), rows 4k..4k+3 for lobe k.
+constexpr int kGuideOut = 32;
+constexpr int kGuideLobes = 8;
+constexpr int kGuideWeights = weightsFor(kGuideOut);  // 8192
+// Raw output -> lobe parameter, per lobe k: (a + 0, b + ln 4, c + cBias_k, d + dBias_k) with the
+// offsets placing the untrained network's lobes (outputs near 0: the network has no biases) at the
+// eight octant directions (+-1, +-1, +-1) / sqrt 3 with concentration ~4 (bit 0/1/2 of k: negative
+// x/y/z). pi = softmax(a'), log kappa = 2 + 6 tanh((b' - 2) / 6) in (-4, 8),
+// theta = pi sigmoid(c'), phi = 2 pi sigmoid(d'),  mu = (sin theta cos phi, sin theta sin phi, cos theta).
+constexpr float kGuideKappaBias = 1.3862943611198906f;          // ln 4
+constexpr float kGuideThetaBias = 0.8278732f;                   // logit(acos(1/sqrt 3) / pi), negated for +z
+constexpr float kGuidePhiBias[4] = {-1.9459101f, -0.5108256f,   // logit(phi / 2 pi), phi = pi/4, 3pi/4,
+                                    1.9459101f, 0.5108256f};     // 7pi/4, 5pi/4 (index: x sign + 2 y sign)
+RT_DEV float guideBias(int k, int e) {
+    if (e == 1) return kGuideKappaBias;
+    if (e == 2) return (k & 4) ? kGuideThetaBias : -kGuideThetaBias;
+    if (e == 3) return kGuidePhiBias[k & 3];
+    return 0.0f;
+}
+// log kappa of the biased raw output b' and its derivative d log kappa / d b'
+RT_DEV float guideLogKappa(float b, float& dlk) {
+    const float t = tanhf((b - 2.0f) * (1.0f / 6.0f));
+    dlk = 1.0f - t * t;
+    return 2.0f + 6.0f * t;
+}
 constexpr int kTileSamples = 32;
 constexpr uint32_t kBatchCap = 1u << 16;  // training samples gathered per frame
 constexpr float kLossEps = 0.01f;    // relative L2: (y^ - y)^2 / (sg(y^)^2 + eps)
@@ -144,9 +172,11 @@ RT_DEV h4 reluHalf(const f16v& acc, int g) {
 
 // Forward pass of one 32-sample tile. W points at the FP16 weight block.
 // Outputs: layer accumulators (pre-activation z1, z2 per 32-row block) and
-// z3, whose rows 0-2 sit in registers 0-2 of lanes 0-31.
+// z3, whose output row 8g + 4h + e sits in register 4g + e of the lanes with
+// lane / 32 = h (rows 0-2 of the radiance network: registers 0-2 of lanes 0-31).
 struct Fwd { f16v z1[2], z2[2], z3; };
-RT_DEV void forwardTile(const _Float16* W, uint32_t lane, const h4 enc[4], Fwd& F) {
+template <int kO>
+RT_DEV void forwardTileT(const _Float16* W, uint32_t lane, const h4 enc[4], Fwd& F) {
     const int j = (int)(lane & 31u), h = (int)(lane >> 5);
 #pragma unroll
     for (int t = 0; t < 2; t++) {
@@ -166,9 +196,10 @@ RT_DEV void forwardTile(const _Float16* W, uint32_t lane, const h4 enc[4], Fwd& 
     f16v acc = zero16();
 #pragma unroll
     for (int s = 0; s < 8; s++)
-        acc = mfma(fwdA(W + kW3, kOut, j, 8 * s + 4 * h, kOut), reluHalf(F.z2[s >> 2], s & 3), acc);
+        acc = mfma(fwdA(W + kW3, kO, j, 8 * s + 4 * h, kO), reluHalf(F.z2[s >> 2], s & 3), acc);
     F.z3 = acc;
 }
+RT_DEV void forwardTile(const _Float16* W, uint32_t lane, const h4 enc[4], Fwd& F) { forwardTileT<kOut>(W, lane, enc, F); }
 
 RT_DEV float outAct(float z) { return expf(fminf(fmaxf(z, -kLogClamp), kLogClamp)); }
 

@@ -21,7 +21,9 @@ using rti::launch;
 namespace {
 
 constexpr int kStats = 2;                 // loss sum, valid sample count (after the weight gradients)
-constexpr int kPartialStride = kWeights + kStats;
+constexpr int partialStride(int outputs) { return weightsFor(outputs) + kStats; }
+constexpr int kPartialStride = partialStride(kOut);
+constexpr int kMaxPartialStride = partialStride(kGuideOut);
 constexpr uint32_t kMaxTrainWaves = 512;
 constexpr int kInferWaves = 4;
 constexpr float kFix = 4294967296.0f;        // 2^32: grid-gradient fixed-point scale
@@ -45,13 +47,14 @@ __global__ void k_mlp_init_grid(float* g32, _Float16* g16, uint64_t n, uint32_t 
     g32[i] = v;
     g16[i] = (_Float16)v;
 }
+template <int kO>
 __global__ void k_mlp_init_weights(float* w32, _Float16* w16, uint32_t seed) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= kWeights) return;
+    if (i >= weightsFor(kO)) return;
     float lim;  // Glorot uniform per layer
     if (i < kW2) lim = sqrtf(6.0f / (float)(kEnc + kHidden));
     else if (i < kW3) lim = sqrtf(6.0f / (float)(kHidden + kHidden));
-    else lim = sqrtf(6.0f / (float)(kHidden + kOut));
+    else lim = sqrtf(6.0f / (float)(kHidden + kO));
     float v = uniformPM((uint64_t)seed ^ 0xA5A5A5A5ull, (uint64_t)i) * lim;
     w32[i] = v;
     w16[i] = (_Float16)v;
@@ -69,16 +72,19 @@ __global__ void k_to_f16(const float* src, _Float16* dst, uint64_t n) {
     if (i < n) dst[i] = (_Float16)src[i];
 }
 
+template <int kO>
 __device__ __forceinline__ void loadWeightsLDS(_Float16* sW, const _Float16* w) {
-    for (int i = threadIdx.x; i < kWeights; i += blockDim.x) sW[i] = w[i];
+    for (int i = threadIdx.x; i < weightsFor(kO); i += blockDim.x) sW[i] = w[i];
     __syncthreads();
 }
 
 // ---- inference ------------------------------------------------------------------------------
+// Radiance network: exp-activated RGB per sample. Proposal network: its 32 raw outputs per sample.
+template <int kO>
 __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(MlpView M, const float* __restrict__ pos, uint32_t n,
-                                                                float* __restrict__ rgb) {
-    __shared__ _Float16 sW[kWeights];
-    loadWeightsLDS(sW, M.w);
+                                                                float* __restrict__ out) {
+    __shared__ _Float16 sW[weightsFor(kO)];
+    loadWeightsLDS<kO>(sW, M.w);
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t s = (blockIdx.x * kInferWaves + wave) * kTileSamples + (lane & 31u);
     float u = 0.0f, v = 0.0f, w = 0.0f;
@@ -86,79 +92,161 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(MlpView M, const
     h4 enc[4];
     encodeLane(M, lane >> 5, u, v, w, enc);
     Fwd F;
-    forwardTile(sW, lane, enc, F);
-    if (lane < 32u && s < n) {
-        rgb[3 * (size_t)s] = outAct(F.z3[0]);
-        rgb[3 * (size_t)s + 1] = outAct(F.z3[1]);
-        rgb[3 * (size_t)s + 2] = outAct(F.z3[2]);
+    forwardTileT<kO>(sW, lane, enc, F);
+    if (s >= n) return;
+    if constexpr (kO == kOut) {
+        if (lane < 32u) {
+            out[3 * (size_t)s] = outAct(F.z3[0]);
+            out[3 * (size_t)s + 1] = outAct(F.z3[1]);
+            out[3 * (size_t)s + 2] = outAct(F.z3[2]);
+        }
+    } else {
+        const int h = (int)(lane >> 5);
+        for (int g = 0; g < 4; g++)
+            for (int e = 0; e < 4; e++) out[(size_t)kO * s + 8 * g + 4 * h + e] = F.z3[4 * g + e];
     }
 }
 
 // ---- fused forward + backward -----------------------------------------------------------------
+template <int kO>
 struct TrainLDS {
-    _Float16 w[kWeights];
+    _Float16 w[weightsFor(kO)];
     _Float16 enc[kTileSamples][kEnc];     // encoding, [sample][feature]
     _Float16 h1[kTileSamples][kHidden];   // relu(z1)
     _Float16 h2[kTileSamples][kHidden];   // relu(z2)
     _Float16 d1[kTileSamples][kHidden];   // dL/dz1
     _Float16 d2[kTileSamples][kHidden];   // dL/dz2
-    _Float16 d3[kTileSamples][4];         // dL/dz3
+    _Float16 d3[kTileSamples][kO < 4 ? 4 : kO];  // dL/dz3
 };
 
 // Row block g of an accumulator tile t of this lane: neuron 32t + 8g + 4h + e.
 __device__ __forceinline__ int neuron(int t, int g, int h, int e) { return 32 * t + 8 * g + 4 * h + e; }
 
+__device__ __forceinline__ float clipGrad(float g) { return fminf(fmaxf(g, -kGradClip), kGradClip); }
+
+// Relative L2 output gradient of the radiance network: rows 0-2 of lanes 0-31 (k-step 0).
+__device__ __forceinline__ void radianceLoss(const Fwd& F, int h, const float y[3], bool valid, float& lossSum,
+                                             float& count, h4 d3b[1]) {
+    for (int e = 0; e < 4; e++) d3b[0][e] = (_Float16)0.0f;
+    if (h != 0) return;
+    for (int c = 0; c < 3; c++) {
+        float z = F.z3[c];
+        float yh = outAct(z);
+        float den = yh * yh + kLossEps;
+        float diff = yh - y[c];
+        float g = 0.0f;
+        if (valid) {
+            lossSum += diff * diff / den;
+            // d/dz of (yh - y)^2 / sg(yh^2 + eps) with yh = exp(z)
+            g = (fabsf(z) < kLogClamp) ? 2.0f * diff / den * yh : 0.0f;
+            g = clipGrad(g);
+        }
+        d3b[0][c] = (_Float16)g;
+    }
+    if (valid) count += 1.0f;
+}
+
+// Weighted negative log-likelihood of the 8-lobe vMF mixture at direction wd, weight wv
+// (the sample's weight over the batch mean). This lane holds lobes k = 2g + h in registers
+// 4g..4g+3, mapped to (pi, kappa, mu) as guideBias / guideLogKappa state;
+// v(w) = kappa / (2 pi (1 - e^{-2 kappa})) exp(kappa (mu.w - 1)); loss = -wv log sum pi v.
+// Gradients with the responsibilities r = pi v / q: da = wv (pi - r);
+// db = -wv r (1 - 2 kappa e^{-2 kappa} / (1 - e^{-2 kappa}) + kappa (mu.w - 1)) dlogkappa/db;
+// dmu = -wv r kappa w, through dmu/dtheta and dmu/dphi to dc and dd.
+__device__ __forceinline__ void vmfLoss(const Fwd& F, int h, const float wd[3], float wv, bool valid, float& lossSum,
+                                        float& count, h4 d3b[4]) {
+    float amax = fmaxf(fmaxf(F.z3[0], F.z3[4]), fmaxf(F.z3[8], F.z3[12]));
+    amax = fmaxf(amax, __shfl_xor(amax, 32));
+    float ea[4], esum = 0.0f;
+    for (int g = 0; g < 4; g++) { ea[g] = expf(F.z3[4 * g] - amax); esum += ea[g]; }
+    esum += __shfl_xor(esum, 32);
+    float pi[4], kap[4], dlk[4], e2[4], dt[4], pv[4], sc[4], sd[4], st[4], ct[4], sp[4], cp[4];
+    float q = 0.0f;
+    for (int g = 0; g < 4; g++) {
+        const int k = 2 * g + h;
+        pi[g] = ea[g] / esum;
+        kap[g] = expf(guideLogKappa(F.z3[4 * g + 1] + guideBias(k, 1), dlk[g]));
+        sc[g] = 1.0f / (1.0f + expf(-(F.z3[4 * g + 2] + guideBias(k, 2))));
+        sd[g] = 1.0f / (1.0f + expf(-(F.z3[4 * g + 3] + guideBias(k, 3))));
+        const float th = RT_PI * sc[g], ph = 2.0f * RT_PI * sd[g];
+        st[g] = sinf(th); ct[g] = cosf(th); sp[g] = sinf(ph); cp[g] = cosf(ph);
+        dt[g] = st[g] * cp[g] * wd[0] + st[g] * sp[g] * wd[1] + ct[g] * wd[2];
+        e2[g] = expf(-2.0f * kap[g]);
+        const float v = kap[g] / (2.0f * RT_PI * (1.0f - e2[g])) * expf(kap[g] * (dt[g] - 1.0f));
+        pv[g] = pi[g] * v;
+        q += pv[g];
+    }
+    q += __shfl_xor(q, 32);
+    q = fmaxf(q, 1e-20f);
+    if (valid && h == 0) { lossSum += -wv * logf(q); count += 1.0f; }
+    for (int g = 0; g < 4; g++) {
+        float da = 0.0f, db = 0.0f, dc = 0.0f, dd = 0.0f;
+        if (valid) {
+            const float r = pv[g] / q;
+            da = wv * (pi[g] - r);
+            db = -wv * r * (1.0f - 2.0f * kap[g] * e2[g] / (1.0f - e2[g]) + kap[g] * (dt[g] - 1.0f)) * dlk[g];
+            const float m = -wv * r * kap[g];  // dL/dmu = m * wd
+            const float dth = m * (ct[g] * cp[g] * wd[0] + ct[g] * sp[g] * wd[1] - st[g] * wd[2]);
+            const float dph = m * (-st[g] * sp[g] * wd[0] + st[g] * cp[g] * wd[1]);
+            dc = dth * RT_PI * sc[g] * (1.0f - sc[g]);
+            dd = dph * 2.0f * RT_PI * sd[g] * (1.0f - sd[g]);
+        }
+        d3b[g][0] = (_Float16)clipGrad(da);
+        d3b[g][1] = (_Float16)clipGrad(db);
+        d3b[g][2] = (_Float16)clipGrad(dc);
+        d3b[g][3] = (_Float16)clipGrad(dd);
+    }
+}
+
+// Targets: 3 floats per sample (radiance network: RGB, negative = masked) or 4 (proposal
+// network: unit direction and weight; weight <= 0 or non-finite = masked). wmean: the mean
+// weight of the valid samples (proposal network only).
+template <int kO>
 __global__ __launch_bounds__(64) void k_mlp_train(MlpView M, const float* __restrict__ pos,
                                                   const float* __restrict__ target, uint32_t n,
+                                                  const float* __restrict__ wmean,
                                                   long long* __restrict__ gridGrad, float* __restrict__ partial) {
-    __shared__ TrainLDS L;
-    loadWeightsLDS(L.w, M.w);
+    constexpr int kSteps = (kO + 7) / 8;  // k-steps over the outputs
+    constexpr int kTS = kO == kOut ? 3 : 4;  // target stride
+    __shared__ TrainLDS<kO> L;
+    loadWeightsLDS<kO>(L.w, M.w);
     const uint32_t lane = threadIdx.x;
     const int j = (int)(lane & 31u), h = (int)(lane >> 5);
     const uint32_t tiles = (n + kTileSamples - 1) / kTileSamples;
     f16v gW1[2], gW2[2][2], gW3[2];
     for (int t = 0; t < 2; t++) { gW1[t] = zero16(); gW3[t] = zero16(); gW2[t][0] = zero16(); gW2[t][1] = zero16(); }
     float lossSum = 0.0f, count = 0.0f;
+    const float invMean = (kO == kOut || !wmean || !(*wmean > 0.0f)) ? 0.0f : 1.0f / *wmean;
 
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const uint32_t s = tile * kTileSamples + (uint32_t)j;
         float u = 0.0f, v = 0.0f, w = 0.0f;
-        float y[3] = {-1.0f, 0.0f, 0.0f};
+        float y[4] = {-1.0f, 0.0f, 0.0f, -1.0f};
         if (s < n) {
             normalizePos(M, pos[3 * (size_t)s], pos[3 * (size_t)s + 1], pos[3 * (size_t)s + 2], u, v, w);
-            y[0] = target[3 * (size_t)s]; y[1] = target[3 * (size_t)s + 1]; y[2] = target[3 * (size_t)s + 2];
+            for (int c = 0; c < kTS; c++) y[c] = target[kTS * (size_t)s + c];
         }
-        const bool valid = s < n && y[0] >= 0.0f && y[1] >= 0.0f && y[2] >= 0.0f && isfinite(y[0]) &&
-                           isfinite(y[1]) && isfinite(y[2]);
+        bool valid;
+        if constexpr (kO == kOut)
+            valid = s < n && y[0] >= 0.0f && y[1] >= 0.0f && y[2] >= 0.0f && isfinite(y[0]) && isfinite(y[1]) &&
+                    isfinite(y[2]);
+        else
+            valid = s < n && isfinite(y[3]) && y[3] > 0.0f && isfinite(y[0]) && isfinite(y[1]) && isfinite(y[2]) &&
+                    invMean > 0.0f;
         h4 enc[4];
         encodeLane(M, (uint32_t)h, u, v, w, enc);
         Fwd F;
-        forwardTile(L.w, lane, enc, F);
+        forwardTileT<kO>(L.w, lane, enc, F);
 
-        // output gradient (rows 0-2 live in registers 0-2 of lanes 0-31)
-        h4 d3b;
-        for (int e = 0; e < 4; e++) d3b[e] = (_Float16)0.0f;
-        if (h == 0) {
-            for (int c = 0; c < 3; c++) {
-                float z = F.z3[c];
-                float yh = outAct(z);
-                float den = yh * yh + kLossEps;
-                float diff = yh - y[c];
-                float g = 0.0f;
-                if (valid) {
-                    lossSum += diff * diff / den;
-                    // d/dz of (yh - y)^2 / sg(yh^2 + eps) with yh = exp(z)
-                    g = (fabsf(z) < kLogClamp) ? 2.0f * diff / den * yh : 0.0f;
-                    g = fminf(fmaxf(g, -kGradClip), kGradClip);
-                }
-                d3b[c] = (_Float16)g;
-            }
-            if (valid) count += 1.0f;
-        }
-        // dh2 = W3 dz3 (one k-step: outputs 0-7, of which 0-2 are real)
+        // output gradient: k-step g holds output rows 8g + 4h + (0..3) of this lane's sample
+        h4 d3b[kSteps];
+        if constexpr (kO == kOut) radianceLoss(F, h, y, valid, lossSum, count, d3b);
+        else vmfLoss(F, h, y, y[3] * invMean, valid, lossSum, count, d3b);
+        // dh2 = W3 dz3
         f16v dz2[2];
         for (int t = 0; t < 2; t++) {
-            f16v acc = mfma(bwdA(L.w + kW3, kOut, 32 * t + j, 4 * h, kOut), d3b, zero16());
+            f16v acc = zero16();
+            for (int g = 0; g < kSteps; g++) acc = mfma(bwdA(L.w + kW3, kO, 32 * t + j, 8 * g + 4 * h, kO), d3b[g], acc);
             for (int r = 0; r < 16; r++) acc[r] = F.z2[t][r] > 0.0f ? acc[r] : 0.0f;
             dz2[t] = acc;
         }
@@ -210,8 +298,13 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpView M, const float* __rest
                     L.d1[j][nn] = (_Float16)dz1[t][4 * g + e];
                     L.d2[j][nn] = (_Float16)dz2[t][4 * g + e];
                 }
-        if (h == 0)
-            for (int e = 0; e < 4; e++) L.d3[j][e] = d3b[e];
+        if constexpr (kO == kOut) {
+            if (h == 0)
+                for (int e = 0; e < 4; e++) L.d3[j][e] = d3b[0][e];
+        } else {
+            for (int g = 0; g < kSteps; g++)
+                for (int e = 0; e < 4; e++) L.d3[j][8 * g + 4 * h + e] = d3b[g][e];
+        }
         __syncthreads();
         // dW^T[o][i] += sum over samples dz[o][j] * act[i][j]; k-step s4 covers samples 8 s4 + 4h + e
         for (int s4 = 0; s4 < 4; s4++) {
@@ -224,7 +317,7 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpView M, const float* __rest
                     bH1[t][e] = L.h1[jb + e][32 * t + j];
                     bH2[t][e] = L.h2[jb + e][32 * t + j];
                 }
-                aD3[e] = (j < kOut) ? L.d3[jb + e][j] : (_Float16)0.0f;
+                aD3[e] = (j < kO) ? L.d3[jb + e][j] : (_Float16)0.0f;
                 bEnc[e] = L.enc[jb + e][j];
             }
             for (int t = 0; t < 2; t++) {
@@ -237,7 +330,7 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpView M, const float* __rest
     }
 
     // per-wave partials in the weight layout (input-major): C row = output neuron, column = input
-    float* P = partial + (size_t)blockIdx.x * kPartialStride;
+    float* P = partial + (size_t)blockIdx.x * partialStride(kO);
     for (int t = 0; t < 2; t++)
         for (int g = 0; g < 4; g++)
             for (int e = 0; e < 4; e++) {
@@ -246,56 +339,78 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpView M, const float* __rest
                 for (int u2 = 0; u2 < 2; u2++) P[kW2 + (32 * u2 + j) * kHidden + o] = gW2[t][u2][r];
             }
     for (int t = 0; t < 2; t++)
-        if (h == 0)
-            for (int e = 0; e < kOut; e++) P[kW3 + (32 * t + j) * kOut + e] = gW3[t][e];
+        for (int g = 0; g < 4; g++)
+            for (int e = 0; e < 4; e++) {
+                const int o = 8 * g + 4 * h + e;
+                if (o < kO) P[kW3 + (32 * t + j) * kO + o] = gW3[t][4 * g + e];
+            }
     // loss and count: sum over lanes 0-31
     for (int off = 16; off > 0; off >>= 1) {
         lossSum += __shfl_xor(lossSum, off);
         count += __shfl_xor(count, off);
     }
-    if (lane == 0) { P[kWeights] = lossSum; P[kWeights + 1] = count; }
+    if (lane == 0) { P[weightsFor(kO)] = lossSum; P[weightsFor(kO) + 1] = count; }
+}
+
+// Mean weight of the valid samples of a proposal batch (one workgroup, fixed-order sums).
+__global__ __launch_bounds__(256) void k_guide_wmean(const float* __restrict__ target, uint32_t n, float* __restrict__ out) {
+    __shared__ float sw[256], sc[256];
+    float w = 0.0f, c = 0.0f;
+    for (uint32_t i = threadIdx.x; i < n; i += 256u) {
+        const float* y = target + 4 * (size_t)i;
+        if (isfinite(y[3]) && y[3] > 0.0f && isfinite(y[0]) && isfinite(y[1]) && isfinite(y[2])) { w += y[3]; c += 1.0f; }
+    }
+    sw[threadIdx.x] = w; sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) { sw[threadIdx.x] += sw[threadIdx.x + off]; sc[threadIdx.x] += sc[threadIdx.x + off]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sc[0] > 0.0f ? sw[0] / sc[0] : 0.0f;
 }
 
 // Weight gradients and stats: fixed-order sum over the waves' partials.
 // Sum of column k over the waves' partials in a fixed order (eight
 // interleaved accumulators for memory-level parallelism, combined pairwise).
-__device__ __forceinline__ float columnSum(const float* __restrict__ partial, uint32_t waves, int k) {
+__device__ __forceinline__ float columnSum(const float* __restrict__ partial, uint32_t waves, int stride, int k) {
     float a[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t w = 0;
     for (; w + 8u <= waves; w += 8u)
 #pragma unroll
-        for (int i = 0; i < 8; i++) a[i] += partial[(size_t)(w + i) * kPartialStride + k];
-    for (; w < waves; w++) a[0] += partial[(size_t)w * kPartialStride + k];
+        for (int i = 0; i < 8; i++) a[i] += partial[(size_t)(w + i) * stride + k];
+    for (; w < waves; w++) a[0] += partial[(size_t)w * stride + k];
     return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
 }
 
-__global__ void k_mlp_reduce(const float* __restrict__ partial, uint32_t waves, float* __restrict__ out,
-                             float* __restrict__ lossOut) {
+// stride: partial stride of the network; norm: outputs per sample the mean loss averages over
+// (3 for the radiance network, 1 for the proposal network).
+__global__ void k_mlp_reduce(const float* __restrict__ partial, uint32_t waves, int stride, float norm,
+                             float* __restrict__ out, float* __restrict__ lossOut) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= kPartialStride) return;
-    const float acc = columnSum(partial, waves, k);
+    if (k >= stride) return;
+    const float acc = columnSum(partial, waves, stride, k);
     out[k] = acc;
-    if (k == kWeights && lossOut) {
-        const float cnt = columnSum(partial, waves, kWeights + 1);
-        *lossOut = cnt > 0.0f ? acc / (3.0f * cnt) : 0.0f;
+    if (k == stride - kStats && lossOut) {
+        const float cnt = columnSum(partial, waves, stride, stride - kStats + 1);
+        *lossOut = cnt > 0.0f ? acc / (norm * cnt) : 0.0f;
     }
 }
 
 __device__ __forceinline__ float gradValue(float g) { return g; }
 __device__ __forceinline__ float gradValue(long long g) { return (float)((double)g * kInvFix); }
 
-// Adam on one parameter array; the gradient of the mean loss is grad / (3 count).
+// Adam on one parameter array; the gradient of the mean loss is grad / (norm count).
 // Grid entries no sample touched (gradient exactly zero) are left unchanged.
 template <class G>
 __global__ void k_mlp_adam(float* __restrict__ p32, _Float16* __restrict__ p16, const G* __restrict__ grad,
                            float* __restrict__ m, float* __restrict__ v, uint64_t n, const float* __restrict__ stats,
-                           float lr, float b1, float b2, float eps, float c1, float c2, int skipZero) {
+                           float norm, float lr, float b1, float b2, float eps, float c1, float c2, int skipZero) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (skipZero && grad[i] == G(0)) return;
     float gr = gradValue(grad[i]);
     float cnt = stats[1];
-    float g = cnt > 0.0f ? gr / (3.0f * cnt) : 0.0f;
+    float g = cnt > 0.0f ? gr / (norm * cnt) : 0.0f;
     float mi = b1 * m[i] + (1.0f - b1) * g;
     float vi = b2 * v[i] + (1.0f - b2) * g * g;
     m[i] = mi; v[i] = vi;
@@ -312,15 +427,16 @@ void freeMlp(rt_mlp* m) {
     (void)hipFree(m->gridF32); (void)hipFree(m->gridF16); (void)hipFree(m->gridGrad); (void)hipFree(m->gridM);
     (void)hipFree(m->gridV); (void)hipFree(m->wF32); (void)hipFree(m->wF16); (void)hipFree(m->wGrad);
     (void)hipFree(m->wM); (void)hipFree(m->wV); (void)hipFree(m->partial); (void)hipFree(m->batchPos);
-    (void)hipFree(m->batchTarget);
+    (void)hipFree(m->batchTarget); (void)hipFree(m->wmean);
     delete m;
 }
 
 hipError_t zeroAdam(rt_mlp* m, hipStream_t st) {
+    const uint64_t nw = (uint64_t)m->weights;
     k_fill_f32<<<blocks(m->gridParams, 256), 256, 0, st>>>(m->gridM, m->gridParams, 0.0f);
     k_fill_f32<<<blocks(m->gridParams, 256), 256, 0, st>>>(m->gridV, m->gridParams, 0.0f);
-    k_fill_f32<<<blocks(kWeights, 256), 256, 0, st>>>(m->wM, kWeights, 0.0f);
-    k_fill_f32<<<blocks(kWeights, 256), 256, 0, st>>>(m->wV, kWeights, 0.0f);
+    k_fill_f32<<<blocks(nw, 256), 256, 0, st>>>(m->wM, nw, 0.0f);
+    k_fill_f32<<<blocks(nw, 256), 256, 0, st>>>(m->wV, nw, 0.0f);
     m->step = 0;
     return hipGetLastError();
 }
@@ -330,13 +446,23 @@ hipError_t zeroAdam(rt_mlp* m, hipStream_t st) {
 namespace rti {
 // Gradients of a batch into m->gridGrad / m->wGrad (used by the ABI and the frame trainer).
 hipError_t mlpGradients(rt_mlp* m, const float* pos, const float* target, uint32_t n, float* loss, hipStream_t st) {
+    const bool guide = m->outputs == (uint32_t)kGuideOut;
+    const int stride = guide ? partialStride(kGuideOut) : kPartialStride;
     (void)launch(k_fill_i64, dim3(blocks(m->gridParams, 256)), dim3(256), st, m->gridGrad, m->gridParams);
     const uint32_t tiles = (n + kTileSamples - 1) / kTileSamples;
     const uint32_t waves = std::max(1u, std::min(tiles, m->partialWaves));
-    if (n) (void)launch(k_mlp_train, dim3(waves), dim3(64), st, m->view, pos, target, n, m->gridGrad, m->partial);
-    else (void)launch(k_fill_f32, dim3(blocks(kPartialStride, 256)), dim3(256), st, m->partial, (uint64_t)kPartialStride, 0.0f);
-    (void)launch(k_mlp_reduce, dim3(blocks(kPartialStride, 256)), dim3(256), st, (const float*)m->partial,
-                 n ? waves : 1u, m->wGrad, loss);
+    if (n && guide) {
+        (void)launch(k_guide_wmean, dim3(1), dim3(256), st, target, n, m->wmean);
+        (void)launch(k_mlp_train<kGuideOut>, dim3(waves), dim3(64), st, m->view, pos, target, n, (const float*)m->wmean,
+                     m->gridGrad, m->partial);
+    } else if (n) {
+        (void)launch(k_mlp_train<kOut>, dim3(waves), dim3(64), st, m->view, pos, target, n, (const float*)nullptr,
+                     m->gridGrad, m->partial);
+    } else {
+        (void)launch(k_fill_f32, dim3(blocks(stride, 256)), dim3(256), st, m->partial, (uint64_t)stride, 0.0f);
+    }
+    (void)launch(k_mlp_reduce, dim3(blocks(stride, 256)), dim3(256), st, (const float*)m->partial, n ? waves : 1u,
+                 stride, guide ? 1.0f : 3.0f, m->wGrad, loss);
     return hipGetLastError();
 }
 hipError_t mlpAdam(rt_mlp* m, hipStream_t st) {
@@ -344,20 +470,20 @@ hipError_t mlpAdam(rt_mlp* m, hipStream_t st) {
     const rt_mlp_config& c = m->cfg;
     const float c1 = 1.0f - (float)std::pow((double)c.beta1, (double)m->step);
     const float c2 = 1.0f - (float)std::pow((double)c.beta2, (double)m->step);
-    const float* stats = m->wGrad + kWeights;
+    const float* stats = m->wGrad + m->weights;
+    const float norm = m->outputs == (uint32_t)kGuideOut ? 1.0f : 3.0f;
     (void)launch(k_mlp_adam<long long>, dim3(blocks(m->gridParams, 256)), dim3(256), st, m->gridF32, m->gridF16,
-                 (const long long*)m->gridGrad, m->gridM, m->gridV, m->gridParams, stats, c.learningRate, c.beta1,
+                 (const long long*)m->gridGrad, m->gridM, m->gridV, m->gridParams, stats, norm, c.learningRate, c.beta1,
                  c.beta2, c.epsilon, c1, c2, 1);
-    (void)launch(k_mlp_adam<float>, dim3(blocks(kWeights, 256)), dim3(256), st, m->wF32, m->wF16,
-                 (const float*)m->wGrad, m->wM, m->wV, (uint64_t)kWeights, stats, c.learningRate, c.beta1, c.beta2,
-                 c.epsilon, c1, c2, 0);
+    (void)launch(k_mlp_adam<float>, dim3(blocks(m->weights, 256)), dim3(256), st, m->wF32, m->wF16,
+                 (const float*)m->wGrad, m->wM, m->wV, (uint64_t)m->weights, stats, norm, c.learningRate, c.beta1,
+                 c.beta2, c.epsilon, c1, c2, 0);
     return hipGetLastError();
 }
 }  // namespace rti
 
-extern "C" {
-
-rt_status rt_mlp_create(const rt_mlp_config* c, hipStream_t stream, rt_mlp** out) {
+namespace {
+rt_status createNet(const rt_mlp_config* c, uint32_t outputs, hipStream_t stream, rt_mlp** out) {
     if (!out) return invalid("out is null");
     *out = nullptr;
     if (!c) return invalid("config is null");
@@ -376,6 +502,11 @@ rt_status rt_mlp_create(const rt_mlp_config* c, hipStream_t stream, rt_mlp** out
         return invalid("invalid optimiser settings");
     rt_mlp* m = new rt_mlp();
     m->cfg = *c;
+    m->outputs = outputs;
+    const bool guide = outputs == (uint32_t)kGuideOut;
+    m->weights = (uint32_t)weightsFor((int)outputs);
+    m->targetStride = guide ? 4u : 3u;
+    const int stride = partialStride((int)outputs);
     const uint64_t T = 1ull << c->log2TableSize;
     m->gridParams = (uint64_t)kLevels * T * kFeatures;
     MlpView& V = m->view;
@@ -389,33 +520,47 @@ rt_status rt_mlp_create(const rt_mlp_config* c, hipStream_t stream, rt_mlp** out
     }
     for (int a = 0; a < 3; a++) { V.lo[a] = c->boundsLo[a]; V.invExt[a] = 1.0f / (c->boundsHi[a] - c->boundsLo[a]); }
     m->partialWaves = kMaxTrainWaves;
-    const size_t G = m->gridParams;
+    const size_t G = m->gridParams, NW = m->weights;
     bool ok = rti::hipCheck(hipMalloc(&m->gridF32, 4 * G), "hipMalloc") &&
               rti::hipCheck(hipMalloc(&m->gridF16, 2 * G), "hipMalloc") &&
               rti::hipCheck(hipMalloc(&m->gridGrad, 8 * G), "hipMalloc") &&
               rti::hipCheck(hipMalloc(&m->gridM, 4 * G), "hipMalloc") &&
               rti::hipCheck(hipMalloc(&m->gridV, 4 * G), "hipMalloc") &&
-              rti::hipCheck(hipMalloc(&m->wF32, 4 * kWeights), "hipMalloc") &&
-              rti::hipCheck(hipMalloc(&m->wF16, 2 * kWeights), "hipMalloc") &&
-              rti::hipCheck(hipMalloc(&m->wGrad, 4 * kPartialStride), "hipMalloc") &&
-              rti::hipCheck(hipMalloc(&m->wM, 4 * kWeights), "hipMalloc") &&
-              rti::hipCheck(hipMalloc(&m->wV, 4 * kWeights), "hipMalloc") &&
-              rti::hipCheck(hipMalloc(&m->partial, 4 * (size_t)kPartialStride * kMaxTrainWaves), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->wF32, 4 * NW), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->wF16, 2 * NW), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->wGrad, 4 * (size_t)stride), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->wM, 4 * NW), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->wV, 4 * NW), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->partial, 4 * (size_t)stride * kMaxTrainWaves), "hipMalloc") &&
               rti::hipCheck(hipMalloc(&m->batchPos, sizeof(float) * 3 * kBatchCap), "hipMalloc") &&
-              rti::hipCheck(hipMalloc(&m->batchTarget, sizeof(float) * 3 * kBatchCap), "hipMalloc");
+              rti::hipCheck(hipMalloc(&m->batchTarget, sizeof(float) * m->targetStride * kBatchCap), "hipMalloc") &&
+              rti::hipCheck(hipMalloc(&m->wmean, sizeof(float)), "hipMalloc");
     if (ok) {
         V.grid = reinterpret_cast<const uint32_t*>(m->gridF16);
         V.w = m->wF16;
         k_mlp_init_grid<<<blocks(G, 256), 256, 0, stream>>>(m->gridF32, m->gridF16, G, c->seed);
-        k_mlp_init_weights<<<blocks(kWeights, 256), 256, 0, stream>>>(m->wF32, m->wF16, c->seed);
+        if (guide) k_mlp_init_weights<kGuideOut><<<blocks(NW, 256), 256, 0, stream>>>(m->wF32, m->wF16, c->seed);
+        else k_mlp_init_weights<kOut><<<blocks(NW, 256), 256, 0, stream>>>(m->wF32, m->wF16, c->seed);
         k_fill_i64<<<blocks(G, 256), 256, 0, stream>>>(m->gridGrad, G);
-        k_fill_f32<<<blocks(kPartialStride, 256), 256, 0, stream>>>(m->wGrad, kPartialStride, 0.0f);
+        k_fill_f32<<<blocks(stride, 256), 256, 0, stream>>>(m->wGrad, (uint64_t)stride, 0.0f);
+        k_fill_f32<<<1, 1, 0, stream>>>(m->wmean, 1, 0.0f);
         ok = rti::hipCheck(zeroAdam(m, stream), "init kernels") &&
              rti::hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
     }
     if (!ok) { freeMlp(m); return RT_ERR_HIP; }
     *out = m;
     return RT_OK;
+}
+}  // namespace
+
+extern "C" {
+
+rt_status rt_mlp_create(const rt_mlp_config* c, hipStream_t stream, rt_mlp** out) {
+    return createNet(c, (uint32_t)kOut, stream, out);
+}
+
+rt_status rt_guide_create(const rt_mlp_config* c, hipStream_t stream, rt_mlp** out) {
+    return createNet(c, (uint32_t)kGuideOut, stream, out);
 }
 
 rt_status rt_mlp_destroy(rt_mlp* m) {
@@ -427,14 +572,14 @@ rt_status rt_mlp_destroy(rt_mlp* m) {
 rt_status rt_mlp_param_counts(const rt_mlp* m, uint64_t* gridParams, uint64_t* weightParams) {
     if (!m) return invalid("mlp is null");
     if (gridParams) *gridParams = m->gridParams;
-    if (weightParams) *weightParams = kWeights;
+    if (weightParams) *weightParams = m->weights;
     return RT_OK;
 }
 
 rt_status rt_mlp_get_params(const rt_mlp* m, float* grid, float* weights) {
     if (!m) return invalid("mlp is null");
     if (grid) RT_HIP(hipMemcpy(grid, m->gridF32, 4 * m->gridParams, hipMemcpyDeviceToHost));
-    if (weights) RT_HIP(hipMemcpy(weights, m->wF32, 4 * kWeights, hipMemcpyDeviceToHost));
+    if (weights) RT_HIP(hipMemcpy(weights, m->wF32, 4 * (size_t)m->weights, hipMemcpyDeviceToHost));
     return RT_OK;
 }
 
@@ -445,8 +590,8 @@ rt_status rt_mlp_set_params(rt_mlp* m, const float* grid, const float* weights) 
         k_to_f16<<<blocks(m->gridParams, 256), 256>>>(m->gridF32, m->gridF16, m->gridParams);
     }
     if (weights) {
-        RT_HIP(hipMemcpy(m->wF32, weights, 4 * kWeights, hipMemcpyHostToDevice));
-        k_to_f16<<<blocks(kWeights, 256), 256>>>(m->wF32, m->wF16, kWeights);
+        RT_HIP(hipMemcpy(m->wF32, weights, 4 * (size_t)m->weights, hipMemcpyHostToDevice));
+        k_to_f16<<<blocks(m->weights, 256), 256>>>(m->wF32, m->wF16, (uint64_t)m->weights);
     }
     RT_HIP(zeroAdam(m, nullptr));
     RT_HIP(hipDeviceSynchronize());
@@ -458,7 +603,9 @@ rt_status rt_mlp_infer(const rt_mlp* m, const float* pos, uint32_t n, float* rgb
     if (n == 0) return RT_OK;
     if (!pos || !rgb) return invalid("null array");
     const uint32_t tiles = (n + kTileSamples - 1) / kTileSamples;
-    k_mlp_infer<<<(tiles + kInferWaves - 1) / kInferWaves, 64 * kInferWaves, 0, stream>>>(m->view, pos, n, rgb);
+    const unsigned grid = (tiles + kInferWaves - 1) / kInferWaves;
+    if (m->outputs == (uint32_t)kGuideOut) k_mlp_infer<kGuideOut><<<grid, 64 * kInferWaves, 0, stream>>>(m->view, pos, n, rgb);
+    else k_mlp_infer<kOut><<<grid, 64 * kInferWaves, 0, stream>>>(m->view, pos, n, rgb);
     RT_HIP(hipGetLastError());
     return RT_OK;
 }
@@ -474,12 +621,14 @@ rt_status rt_mlp_gradients(rt_mlp* m, const float* pos, const float* target, uin
 rt_status rt_mlp_read_gradients(const rt_mlp* m, float* gridGrad, float* weightGrad) {
     if (!m) return invalid("mlp is null");
     RT_HIP(hipDeviceSynchronize());
-    std::vector<float> w(kPartialStride);
-    RT_HIP(hipMemcpy(w.data(), m->wGrad, 4 * kPartialStride, hipMemcpyDeviceToHost));
-    const float cnt = w[kWeights + 1];
-    const float scale = cnt > 0.0f ? 1.0f / (3.0f * cnt) : 0.0f;
+    const int NW = (int)m->weights;
+    std::vector<float> w(NW + kStats);
+    RT_HIP(hipMemcpy(w.data(), m->wGrad, 4 * (size_t)(NW + kStats), hipMemcpyDeviceToHost));
+    const float cnt = w[NW + 1];
+    const float norm = m->outputs == (uint32_t)kGuideOut ? 1.0f : 3.0f;
+    const float scale = cnt > 0.0f ? 1.0f / (norm * cnt) : 0.0f;
     if (weightGrad)
-        for (int i = 0; i < kWeights; i++) weightGrad[i] = w[i] * scale;
+        for (int i = 0; i < NW; i++) weightGrad[i] = w[i] * scale;
     if (gridGrad) {
         std::vector<long long> g(m->gridParams);
         RT_HIP(hipMemcpy(g.data(), m->gridGrad, 8 * m->gridParams, hipMemcpyDeviceToHost));
@@ -507,13 +656,13 @@ rt_status rt_mlp_batch_download(const rt_mlp* m, float* pos, float* target, uint
     if (count) *count = n;
     RT_HIP(hipDeviceSynchronize());
     if (n && pos) RT_HIP(hipMemcpy(pos, m->batchPos, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
-    if (n && target) RT_HIP(hipMemcpy(target, m->batchTarget, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
+    if (n && target) RT_HIP(hipMemcpy(target, m->batchTarget, sizeof(float) * m->targetStride * n, hipMemcpyDeviceToHost));
     return RT_OK;
 }
 
 rt_status rt_mlp_train_frame(rt_mlp* m, float* loss, hipStream_t stream) {
     if (!m) return invalid("mlp is null");
-    if (m->batchSize == 0) return invalid("no training batch: call rt_gris_learned_refresh first");
+    if (m->batchSize == 0) return invalid("no training batch: call rt_gris_learned_refresh (rt_gris_guide_train for a proposal network) first");
     return rt_mlp_train_step(m, m->batchPos, m->batchTarget, m->batchSize, loss, stream);
 }
 

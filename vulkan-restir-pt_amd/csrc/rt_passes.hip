@@ -13,6 +13,7 @@
 // Arithmetic follows the reference GLSL expression by expression (see the
 // citations at each function) under the FP contract of rt_dmath.h.
 #include "rt_dpixel.h"
+#include "rt_guide.h"
 #include <algorithm>
 #include <cstring>
 
@@ -351,6 +352,7 @@ struct PTPath {
     Isec isec;
     Resv ps;  // the path sample
     float streamWeight, streamSum;
+    f3 dir0;  // direction sampled at the primary vertex (training record of the learned proposal)
 };
 
 // The stream-RIS pick of a lane lives in LDS (18 words, stride = lanes per
@@ -405,6 +407,7 @@ RT_DEV bool ptBegin(const DevScene& S, const rt_camera& cam, const GB& g, uint32
     P.ps.primaryRng = P.rng;
     P.picked = false;
     P.streamWeight = 0.0f; P.streamSum = 0.0f;
+    P.dir0 = s3(0.0f);
     return true;
 }
 
@@ -412,7 +415,8 @@ RT_DEV bool ptBegin(const DevScene& S, const rt_camera& cam, const GB& g, uint32
 // where the reference leaves the loop; otherwise the next ray is in rori/rdir.
 template <bool kCount>
 RT_DEV bool ptVertex(const DevScene& S, TStack st, float* slot, const rt_gris_settings& set, PTPath& P, const Surf& surf,
-                     const Mat& mat, const uint32_t bounce, TraceCounters* ctr) {
+                     const Mat& mat, const uint32_t bounce, TraceCounters* ctr,
+                     const rtg::GuidePix& gpx = rtg::noGuide()) {
     uint32_t& rng = P.rng;
     uint32_t& sampleState = P.sampleState;
     uint32_t& lastSampleState = P.lastSampleState;
@@ -528,7 +532,16 @@ RT_DEV bool ptVertex(const DevScene& S, TStack st, float* slot, const rt_gris_se
             rcThroughput = rcThroughput / (1.0f - pdfTerminate);
         }
 
-        if (!sampleBSDF(mat, surf.albedo, surf.norm, wo, sample3f(rng), s) || s.pdf < 1e-6f) return false;
+        bool sampled;
+        if (bounce == 0u && gpx.L) {  // learned proposal at the primary vertex
+            const f4 gu = sample4f(rng);
+            const f3 rb = sample3f(rng);
+            sampled = rtg::guidedSampleBSDF(gpx, mat, surf.albedo, surf.norm, wo, gu, rb, s);
+            P.dir0 = s.wi;
+        } else {
+            sampled = sampleBSDF(mat, surf.albedo, surf.norm, wo, sample3f(rng), s);
+        }
+        if (!sampled || s.pdf < 1e-6f) return false;
         float cosTheta = isSampleTypeDelta(s.type) ? 1.0f : absDot(surf.norm, s.wi);
         f3 scatterTerm = s.bsdf * cosTheta / s.pdf;
         throughput = throughput * scatterTerm;
@@ -585,7 +598,7 @@ RT_DEV Resv ptFinish(PTPath& P, const float* slot) {
 // when every lane of the wave is out of work.
 template <bool kCount>
 RT_DEV void ptRun(const DevScene& S, const rt_camera& cam, const GB& g, const rt_gris_settings& set, const ResvPlanes& outR,
-                  WorkQueue* q, TStack st, float* slot, TraceCounters* ctr) {
+                  WorkQueue* q, TStack st, float* slot, TraceCounters* ctr, const rtg::GuideFrame& gf) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t tilesX = (g.W + kTileX - 1) / kTileX;
     const uint32_t total = tilesX * ((g.H + kTileY - 1) / kTileY) * kTileThreads;
@@ -611,8 +624,12 @@ RT_DEV void ptRun(const DevScene& S, const rt_camera& cam, const GB& g, const rt
             Surf surf; Mat mat;
             if (x >= g.W || y >= g.H || !ptBegin(S, cam, g, x, y, P, surf, mat)) continue;
             pix = y * g.W + x;
-            live = ptVertex<kCount>(S, st, slot, set, P, surf, mat, 0u, ctr) && ++P.bounce < set.maxDepth;
-            if (!live) storeResv(outR, pix, ptFinish(P, slot));
+            const rtg::GuidePix gpx = gf.lobes ? rtg::guideAt(gf.lobes, g.W * g.H, pix) : rtg::noGuide();
+            live = ptVertex<kCount>(S, st, slot, set, P, surf, mat, 0u, ctr, gpx) && ++P.bounce < set.maxDepth;
+            if (!live) {
+                storeResv(outR, pix, ptFinish(P, slot));
+                if (gf.samples) gf.samples[pix] = make_float4(P.dir0.x, P.dir0.y, P.dir0.z, P.streamSum);
+            }
         }
         if (__ballot(live) == 0ull) break;
         if (live) {
@@ -624,7 +641,10 @@ RT_DEV void ptRun(const DevScene& S, const rt_camera& cam, const GB& g, const rt
                 Mat mat = loadMat(S, surf.matIndex);
                 live = ptVertex<kCount>(S, st, slot, set, P, surf, mat, P.bounce, ctr) && ++P.bounce < set.maxDepth;
             }
-            if (!live) storeResv(outR, pix, ptFinish(P, slot));
+            if (!live) {
+                storeResv(outR, pix, ptFinish(P, slot));
+                if (gf.samples) gf.samples[pix] = make_float4(P.dir0.x, P.dir0.y, P.dir0.z, P.streamSum);
+            }
         }
     }
 }
@@ -640,12 +660,15 @@ RT_DEV void queueDone(WorkQueue* q) {
     }
 }
 
-template <bool kCount>
+// kGuided: the learned proposal's variant (the other variants compile without its code).
+template <bool kCount, bool kGuided>
 __global__ __launch_bounds__(kTileThreads, RT_WAVES_PATH_TRACE) void k_gris_path_trace(
-    DevScene S, rt_camera cam, GB g, rt_gris_settings set, ResvPlanes outR, WorkQueue* q, TraceCounters* ctr) {
+    DevScene S, rt_camera cam, GB g, rt_gris_settings set, ResvPlanes outR, WorkQueue* q, rtg::GuideFrame gf,
+    TraceCounters* ctr) {
     __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
     __shared__ float picks[kStreamWords * kTileThreads];
-    ptRun<kCount>(S, cam, g, set, outR, q, TStack{stk + threadIdx.x, kTileThreads}, picks + threadIdx.x, ctr);
+    ptRun<kCount>(S, cam, g, set, outR, q, TStack{stk + threadIdx.x, kTileThreads}, picks + threadIdx.x, ctr,
+                  kGuided ? gf : rtg::GuideFrame{nullptr, nullptr, nullptr});
     queueDone(q);
 }
 
@@ -654,7 +677,7 @@ struct RcData { Isec rcPrevIsec; f3 rcPrevWo; f3 rcPrevThroughput; };
 
 template <bool kCount>
 RT_DEV void replayHybrid(const DevScene& S, TStack st, const rt_gris_settings& set, Isec isec, Surf surf, f3 rdir, f3 rori,
-                         uint32_t targetFlags, uint32_t rng, RcData& rc, TraceCounters* ctr) {
+                         uint32_t targetFlags, uint32_t rng, RcData& rc, TraceCounters* ctr, const rtg::GuidePix& gpx) {
     f3 throughput = s3(1.0f);
     f3 wo = -rdir;
     Mat mat = loadMat(S, surf.matIndex);
@@ -686,7 +709,15 @@ RT_DEV void replayHybrid(const DevScene& S, TStack st, const rt_gris_settings& s
             if (sample1f(rng) < pdfTerminate) break;
             throughput = throughput / (1.0f - pdfTerminate);
         }
-        if (!sampleBSDF(mat, surf.albedo, surf.norm, wo, sample3f(rng), s) || s.pdf < 1e-6f) break;
+        bool sampled;
+        if (bounce == 0u && gpx.L) {  // the destination's learned proposal, same random numbers
+            const f4 gu = sample4f(rng);
+            const f3 rb = sample3f(rng);
+            sampled = rtg::guidedSampleBSDF(gpx, mat, surf.albedo, surf.norm, wo, gu, rb, s);
+        } else {
+            sampled = sampleBSDF(mat, surf.albedo, surf.norm, wo, sample3f(rng), s);
+        }
+        if (!sampled || s.pdf < 1e-6f) break;
         float cosTheta = isSampleTypeDelta(s.type) ? 1.0f : absDot(surf.norm, s.wi);
         throughput = throughput * (s.bsdf * cosTheta / s.pdf);
         wo = -s.wi;
@@ -740,14 +771,16 @@ RT_DEV float rcLightMIS(const DevScene& S, uint32_t rcType, const Surf& rcPrevSu
 // is left to the caller (applied once per pass after the 1/Z normalisation).
 template <bool kCount, bool kLearned = false, bool kUnbiased = false>
 RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& set, Resv& dst, const Surf& dstPrimarySurf,
-                          const Isec& dstPrimaryIsec, f3 primDir, f3 primOri, Resv src, uint32_t& rng, TraceCounters* ctr) {
+                          const Isec& dstPrimaryIsec, f3 primDir, f3 primOri, Resv src, uint32_t& rng, TraceCounters* ctr,
+                          const rtg::GuidePix& gpx) {
     RcData rc;
     Surf rcPrevSurf, rcSurf;
     f3 wi = s3(0.0f), Li = s3(0.0f);
     bool srcSampleValid = false;
     float dstJacobian = 0.0f, jacobian = 0.0f, dstPHat = 0.0f, dstSamplePdf = 0.0f;
     if (src.rcIsec.inst != RT_INVALID_HIT) {
-        replayHybrid<kCount>(S, st, set, dstPrimaryIsec, dstPrimarySurf, primDir, primOri, src.flags, src.primaryRng, rc, ctr);
+        replayHybrid<kCount>(S, st, set, dstPrimaryIsec, dstPrimarySurf, primDir, primOri, src.flags, src.primaryRng, rc, ctr,
+                             gpx);
         if (rc.rcPrevIsec.inst != RT_INVALID_HIT) {
             if (rc.rcPrevIsec.inst == RT_SPECIAL_HIT) rcPrevSurf = dstPrimarySurf;
             else loadSurfaceInfo(S, rc.rcPrevIsec, rcPrevSurf);
@@ -778,7 +811,9 @@ RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& 
                 dstSamplePdf = luminance(rcSurf.albedo) / sumPower / dstJacobian;
             } else {
                 Mat rcPrevMat = loadMat(S, rcPrevSurf.matIndex);
-                dstSamplePdf = evalPdf(rcPrevMat, rcPrevSurf.norm, rc.rcPrevWo, wi);
+                // a primary-vertex predecessor samples with the learned proposal when it is on
+                dstSamplePdf = rtg::guidedPdf(rc.rcPrevIsec.inst == RT_SPECIAL_HIT ? gpx : rtg::noGuide(), rcPrevMat,
+                                              rcPrevSurf.norm, rc.rcPrevWo, wi);
             }
         }
         float srcPHat = kLearned ? src.pRef : luminance(src.F);
@@ -809,10 +844,10 @@ RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& 
 // that predecessor -- has a positive target value.
 template <bool kCount, bool kLearned>
 RT_DEV bool shiftSupports(const DevScene& S, TStack st, const rt_gris_settings& set, const Resv& y, const Surf& primSurf,
-                          const Isec& primIsec, f3 primDir, f3 primOri, TraceCounters* ctr) {
+                          const Isec& primIsec, f3 primDir, f3 primOri, TraceCounters* ctr, const rtg::GuidePix& gpx) {
     if (y.rcIsec.inst == RT_INVALID_HIT) return false;
     RcData rc;
-    replayHybrid<kCount>(S, st, set, primIsec, primSurf, primDir, primOri, y.flags, y.primaryRng, rc, ctr);
+    replayHybrid<kCount>(S, st, set, primIsec, primSurf, primDir, primOri, y.flags, y.primaryRng, rc, ctr, gpx);
     if (rc.rcPrevIsec.inst == RT_INVALID_HIT) return false;
     Surf rcPrevSurf, rcSurf;
     if (rc.rcPrevIsec.inst == RT_SPECIAL_HIT) rcPrevSurf = primSurf;
@@ -829,7 +864,8 @@ RT_DEV bool shiftSupports(const DevScene& S, TStack st, const rt_gris_settings& 
     Resv x = y;
     x.rcPrevSamplePdf = flagsRcVertexType(y.flags) == RC_LIGHT_SAMPLED
                             ? luminance(rcSurf.albedo) / S.lightTable[0].prob / jac
-                            : evalPdf(rcPrevMat, rcPrevSurf.norm, rc.rcPrevWo, wi);
+                            : rtg::guidedPdf(rc.rcPrevIsec.inst == RT_SPECIAL_HIT ? gpx : rtg::noGuide(), rcPrevMat,
+                                             rcPrevSurf.norm, rc.rcPrevWo, wi);
     if (isnan(x.rcPrevSamplePdf) || !(x.rcPrevSamplePdf > 1e-6f)) return false;
     const float misW = rcLightMIS(S, flagsRcVertexType(y.flags), rcPrevSurf, rc.rcPrevWo, rcSurf, wi, jac);
     const f3 Li = reconnectLi(S, x, x.rcLi * misW, rcPrevSurf, rcSurf, wi, rc);
@@ -870,7 +906,8 @@ RT_DEV Resv resvAt(const ResvPlanes& R, uint32_t W, uint32_t H, int x, int y) {
 template <bool kCount, bool kLearned, bool kUnbiased = false>
 RT_DEV void temporalPixel(const DevScene& S, const rt_camera& cam, const rt_camera& prevCam, const GB& g, const GB& gPrev,
                           const rt_gris_settings& set, const ResvPlanes& thisR, const ResvPlanes& prevR,
-                          const ResvPlanes& tempR, uint32_t x, uint32_t y, TStack st, TraceCounters* ctr) {
+                          const ResvPlanes& tempR, uint32_t x, uint32_t y, TStack st, TraceCounters* ctr,
+                          const rtg::GuideFrame& gf) {
     if (x >= g.W || y >= g.H) return;
     f2 uv = pixelUV(x, y, g.W, g.H);
     GPix gp;
@@ -905,8 +942,9 @@ RT_DEV void temporalPixel(const DevScene& S, const rt_camera& cam, const rt_came
                     if (!(pp.matMeshId != gp.matMeshId || dot(pp.norm, gp.norm) < 0.95f || distance(pos, posPrev) > 0.5f)) {
                         Resv tr = resvAt<kLearned>(prevR, cam.filmSize[0], cam.filmSize[1], px, py);
                         if (resvIsValid(tr)) {
-                            reuseAndMerge<kCount, kLearned, kUnbiased>(S, st, set, resv, dstPrimarySurf, dstPrimaryIsec,
-                                                                        ray.dir, ray.ori, tr, resvRng, ctr);
+                            reuseAndMerge<kCount, kLearned, kUnbiased>(
+                                S, st, set, resv, dstPrimarySurf, dstPrimaryIsec, ray.dir, ray.ori, tr, resvRng, ctr,
+                                gf.lobes ? rtg::guideAt(gf.lobes, g.W * g.H, (uint32_t)p) : rtg::noGuide());
                             mPrev = tr.sampleCount; prevX = px; prevY = py;
                         }
                     }
@@ -918,8 +956,11 @@ RT_DEV void temporalPixel(const DevScene& S, const rt_camera& cam, const rt_came
         if (mPrev > 0.0f && resvIsValid(resv) && resv.rcIsec.inst != RT_INVALID_HIT) {
             float z = mCanon;
             Surf ps; Isec pi; CamRay pr;
+            const rtg::GuidePix prevGp = gf.lobesPrev
+                                             ? rtg::guideAt(gf.lobesPrev, g.W * g.H, (uint32_t)prevY * g.W + (uint32_t)prevX)
+                                             : rtg::noGuide();
             if (primaryOf(prevCam, gPrev, prevX, prevY, ps, pi, pr) &&
-                shiftSupports<kCount, kLearned>(S, st, set, resv, ps, pi, pr.dir, pr.ori, ctr))
+                shiftSupports<kCount, kLearned>(S, st, set, resv, ps, pi, pr.dir, pr.ori, ctr, prevGp))
                 z += mPrev;
             normaliseZ(resv, z);
         }
@@ -928,23 +969,26 @@ RT_DEV void temporalPixel(const DevScene& S, const rt_camera& cam, const rt_came
     if (!resvIsValid(resv)) resvReset(resv);
     storeResv<kLearned>(tempR, p, resv);
 }
-template <bool kCount, bool kLearned, bool kUnbiased>
+template <bool kCount, bool kLearned, bool kUnbiased, bool kGuided>
 __global__ __launch_bounds__(kTileThreads, RT_WAVES_TEMPORAL) void k_gris_temporal(DevScene S, rt_camera cam, rt_camera prevCam, GB g,
                                                                 GB gPrev, rt_gris_settings set, ResvPlanes thisR,
-                                                                ResvPlanes prevR, ResvPlanes tempR, TraceCounters* ctr) {
+                                                                ResvPlanes prevR, ResvPlanes tempR, rtg::GuideFrame gf,
+                                                                TraceCounters* ctr) {
     __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
     PIXEL_INDEX
     temporalPixel<kCount, kLearned, kUnbiased>(S, cam, prevCam, g, gPrev, set, thisR, prevR, tempR, x, y,
-                                    TStack{stk + threadIdx.x, kTileThreads}, ctr);
+                                    TStack{stk + threadIdx.x, kTileThreads}, ctr,
+                                    kGuided ? gf : rtg::GuideFrame{nullptr, nullptr, nullptr});
 }
 
 // gris_resample_spatial.glsl:11-135 and gris_resample_spatial.comp accumulation
 template <bool kCount, bool kLearned, bool kUnbiased = false>
 RT_DEV void spatialPixel(const DevScene& S, const rt_camera& cam, const GB& g, const rt_gris_settings& set,
                          const ResvPlanes& tempR, const ResvPlanes& thisR, float* indirectOut, uint32_t x, uint32_t y,
-                         TStack st, TraceCounters* ctr) {
+                         TStack st, TraceCounters* ctr, const rtg::GuideFrame& gf) {
     if (x >= g.W || y >= g.H) return;
     size_t p = (size_t)y * g.W + x;
+    const rtg::GuidePix gpx = gf.lobes ? rtg::guideAt(gf.lobes, g.W * g.H, (uint32_t)p) : rtg::noGuide();
     f2 uv = pixelUV(x, y, g.W, g.H);
     GPix gp;
     f3 radiance = s3(0.0f);
@@ -980,7 +1024,7 @@ RT_DEV void spatialPixel(const DevScene& S, const rt_camera& cam, const GB& g, c
                 Resv nres = resvAt<kLearned>(tempR, cam.filmSize[0], cam.filmSize[1], px, py);
                 if (resvIsValid(nres)) {
                     reuseAndMerge<kCount, kLearned, kUnbiased>(S, st, set, resv, dstPrimarySurf, dstPrimaryIsec, ray.dir,
-                                                                ray.ori, nres, rng, ctr);
+                                                                ray.ori, nres, rng, ctr, gpx);
                     if (kUnbiased && nbCount < 8u) {
                         nbPix[nbCount] = ((uint32_t)px << 16) | (uint32_t)py;
                         nbM[nbCount] = nres.sampleCount;
@@ -995,8 +1039,10 @@ RT_DEV void spatialPixel(const DevScene& S, const rt_camera& cam, const GB& g, c
                 for (uint32_t j = 0; j < nbCount; j++) {
                     if (!(nbM[j] > 0.0f)) continue;
                     Surf ns; Isec ni; CamRay nr;
-                    if (primaryOf(cam, g, (int)(nbPix[j] >> 16), (int)(nbPix[j] & 0xffffu), ns, ni, nr) &&
-                        shiftSupports<kCount, kLearned>(S, st, set, resv, ns, ni, nr.dir, nr.ori, ctr))
+                    const uint32_t nx = nbPix[j] >> 16, ny = nbPix[j] & 0xffffu;
+                    const rtg::GuidePix ngp = gf.lobes ? rtg::guideAt(gf.lobes, g.W * g.H, ny * g.W + nx) : rtg::noGuide();
+                    if (primaryOf(cam, g, (int)nx, (int)ny, ns, ni, nr) &&
+                        shiftSupports<kCount, kLearned>(S, st, set, resv, ns, ni, nr.dir, nr.ori, ctr, ngp))
                         z += nbM[j];
                 }
                 normaliseZ(resv, z);
@@ -1008,7 +1054,7 @@ RT_DEV void spatialPixel(const DevScene& S, const rt_camera& cam, const GB& g, c
         if (resvIsValid(resv) && resv.sampleCount > 0.0f && resv.rcIsec.inst != RT_INVALID_HIT) {
             RcData rc;
             replayHybrid<kCount>(S, st, set, dstPrimaryIsec, dstPrimarySurf, ray.dir, ray.ori, resv.flags, resv.primaryRng,
-                                 rc, ctr);
+                                 rc, ctr, gpx);
             if (rc.rcPrevIsec.inst != RT_INVALID_HIT) {
                 Surf rcPrevSurf, rcSurf;
                 if (rc.rcPrevIsec.inst == RT_SPECIAL_HIT) rcPrevSurf = dstPrimarySurf;
@@ -1031,15 +1077,15 @@ RT_DEV void spatialPixel(const DevScene& S, const rt_camera& cam, const GB& g, c
     }
     accumulate(indirectOut, p, radiance, cam.frameIndex);
 }
-template <bool kCount, bool kLearned, bool kUnbiased>
+template <bool kCount, bool kLearned, bool kUnbiased, bool kGuided>
 __global__ __launch_bounds__(kTileThreads, RT_WAVES_SPATIAL) void k_gris_spatial(DevScene S, rt_camera cam, GB g, rt_gris_settings set,
                                                                ResvPlanes tempR, ResvPlanes thisR, float* indirectOut,
-                                                               TraceCounters* ctr) {
+                                                               rtg::GuideFrame gf, TraceCounters* ctr) {
     __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
     PIXEL_INDEX
     spatialPixel<kCount, kLearned, kUnbiased>(S, cam, g, set, tempR, thisR, indirectOut, x, y,
-                                              TStack{stk + threadIdx.x, kTileThreads},
-                                   ctr);
+                                              TStack{stk + threadIdx.x, kTileThreads}, ctr,
+                                              kGuided ? gf : rtg::GuideFrame{nullptr, nullptr, nullptr});
 }
 
 // ---- Batched views ---------------------------------------------------------------------------
@@ -1113,7 +1159,7 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_PATH_TRACE) void k_gris_path
         const uint32_t v = __builtin_amdgcn_readfirstlane((blockIdx.x + i) % numViews);
         const ViewDesc& D = V[v];
         ptRun<kCount>(D.S, D.cam, D.g, D.set, D.cur, D.queue, TStack{stk + threadIdx.x, kTileThreads},
-                      picks + threadIdx.x, ctr);
+                      picks + threadIdx.x, ctr, rtg::GuideFrame{nullptr, nullptr, nullptr});
         queueDone(D.queue);
     }
 }
@@ -1128,7 +1174,7 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_TEMPORAL) void k_gris_tempor
     if (!batchPixel(V, numViews, totalTiles, swz, v, x, y)) return;
     const ViewDesc& D = V[v];
     temporalPixel<kCount, false>(D.S, D.cam, D.prevCam, D.g, D.gPrev, D.set, D.cur, D.prev, D.temp, x, y,
-                                 TStack{stk + threadIdx.x, kTileThreads}, ctr);
+                                 TStack{stk + threadIdx.x, kTileThreads}, ctr, rtg::GuideFrame{nullptr, nullptr, nullptr});
 }
 
 template <bool kCount>
@@ -1140,7 +1186,7 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_SPATIAL) void k_gris_spatial
     if (!batchPixel(V, numViews, totalTiles, swz, v, x, y)) return;
     const ViewDesc& D = V[v];
     spatialPixel<kCount, false>(D.S, D.cam, D.g, D.set, D.temp, D.cur, D.indirectOut, x, y,
-                                TStack{stk + threadIdx.x, kTileThreads}, ctr);
+                                TStack{stk + threadIdx.x, kTileThreads}, ctr, rtg::GuideFrame{nullptr, nullptr, nullptr});
 }
 
 __global__ void k_resv_reset(ResvPlanes R) {
@@ -1269,26 +1315,27 @@ namespace rti {
         else (void)launch(KERNEL<false>, GRID, BLOCK, STREAM, __VA_ARGS__, nullptr);                        \
     } while (0)
 
-// Same, for kernels that also take the target function and the resampling-weight mode.
-#define RT_DISPATCH_LU(KERNEL, LEARNED, UNBIASED, GRID, BLOCK, STREAM, ...)                                   \
+// Same, for kernels that also take the target function, the resampling-weight mode and the
+// learned proposal (template arguments <counting, learned, unbiased, guided>).
+#define RT_LAUNCH_LUG_(KERNEL, C, G, LEARNED, UNBIASED, GRID, BLOCK, STREAM, ...)                           \
+    do {                                                                                                  \
+        if (LEARNED) {                                                                                    \
+            if (UNBIASED) (void)launch(KERNEL<C, true, true, G>, GRID, BLOCK, STREAM, __VA_ARGS__);       \
+            else (void)launch(KERNEL<C, true, false, G>, GRID, BLOCK, STREAM, __VA_ARGS__);               \
+        } else {                                                                                          \
+            if (UNBIASED) (void)launch(KERNEL<C, false, true, G>, GRID, BLOCK, STREAM, __VA_ARGS__);      \
+            else (void)launch(KERNEL<C, false, false, G>, GRID, BLOCK, STREAM, __VA_ARGS__);              \
+        }                                                                                                 \
+    } while (0)
+#define RT_DISPATCH_LUG(KERNEL, LEARNED, UNBIASED, GUIDED, GRID, BLOCK, STREAM, ...)                           \
     do {                                                                                                      \
         TraceCounters* ctr_ = gCountersEnabled ? counterBuffer() : nullptr;                                     \
         if (gCountersEnabled) {                                                                               \
-            if (LEARNED) {                                                                                    \
-                if (UNBIASED) (void)launch(KERNEL<true, true, true>, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_);   \
-                else (void)launch(KERNEL<true, true, false>, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_);           \
-            } else {                                                                                          \
-                if (UNBIASED) (void)launch(KERNEL<true, false, true>, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_);  \
-                else (void)launch(KERNEL<true, false, false>, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_);          \
-            }                                                                                                 \
+            if (GUIDED) RT_LAUNCH_LUG_(KERNEL, true, true, LEARNED, UNBIASED, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_);  \
+            else RT_LAUNCH_LUG_(KERNEL, true, false, LEARNED, UNBIASED, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_);        \
         } else {                                                                                              \
-            if (LEARNED) {                                                                                    \
-                if (UNBIASED) (void)launch(KERNEL<false, true, true>, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_);  \
-                else (void)launch(KERNEL<false, true, false>, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_);          \
-            } else {                                                                                          \
-                if (UNBIASED) (void)launch(KERNEL<false, false, true>, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_); \
-                else (void)launch(KERNEL<false, false, false>, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_);         \
-            }                                                                                                 \
+            if (GUIDED) RT_LAUNCH_LUG_(KERNEL, false, true, LEARNED, UNBIASED, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_); \
+            else RT_LAUNCH_LUG_(KERNEL, false, false, LEARNED, UNBIASED, GRID, BLOCK, STREAM, __VA_ARGS__, ctr_);       \
         }                                                                                                     \
     } while (0)
 
@@ -1331,16 +1378,36 @@ static hipError_t persistentGrid(Kernel kernel, uint32_t W, uint32_t H, uint32_t
     return hipSuccess;
 }
 
+// The learned proposal's bindings of a frame (all null unless settings.guidedProposal).
+rtg::GuideFrame guideFrame(const rt_frame& f, const rt_gris_settings& set) {
+    if (!set.guidedProposal) return rtg::GuideFrame{nullptr, nullptr, nullptr};
+    return rtg::GuideFrame{reinterpret_cast<const float4*>(f.guideLobes), reinterpret_cast<const float4*>(f.guideLobesPrev),
+                           reinterpret_cast<float4*>(f.guideSamples)};
+}
+
 // queue: the pixel queue of this launch (null: the scene's next round-robin slot).
 hipError_t launchGrisPathTrace(const rt_device_scene* s, const rt_camera* cams, const rt_frame& f,
                                const rt_gris_settings& set, hipStream_t stream, WorkQueue* queue) {
     GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
     uint32_t grid = 0;
-    hipError_t e = gCountersEnabled ? persistentGrid(k_gris_path_trace<true>, f.width, f.height, grid)
-                                    : persistentGrid(k_gris_path_trace<false>, f.width, f.height, grid);
+    const bool guided = set.guidedProposal == 1u;
+    hipError_t e = gCountersEnabled
+                       ? (guided ? persistentGrid(k_gris_path_trace<true, true>, f.width, f.height, grid)
+                                 : persistentGrid(k_gris_path_trace<true, false>, f.width, f.height, grid))
+                       : (guided ? persistentGrid(k_gris_path_trace<false, true>, f.width, f.height, grid)
+                                 : persistentGrid(k_gris_path_trace<false, false>, f.width, f.height, grid));
     if (e != hipSuccess) return e;
-    RT_DISPATCH(k_gris_path_trace, dim3(grid), kTileThreads, stream, s->dev, cams[0], g, set,
-                planes(f.grisReservoir, f.width, f.height), queue ? queue : s->nextQueue());
+    TraceCounters* ctr = gCountersEnabled ? counterBuffer() : nullptr;
+    const ResvPlanes R = planes(f.grisReservoir, f.width, f.height);
+    WorkQueue* q = queue ? queue : s->nextQueue();
+    const rtg::GuideFrame gf = guideFrame(f, set);
+    if (gCountersEnabled) {
+        if (guided) (void)launch(k_gris_path_trace<true, true>, dim3(grid), kTileThreads, stream, s->dev, cams[0], g, set, R, q, gf, ctr);
+        else (void)launch(k_gris_path_trace<true, false>, dim3(grid), kTileThreads, stream, s->dev, cams[0], g, set, R, q, gf, ctr);
+    } else {
+        if (guided) (void)launch(k_gris_path_trace<false, true>, dim3(grid), kTileThreads, stream, s->dev, cams[0], g, set, R, q, gf, ctr);
+        else (void)launch(k_gris_path_trace<false, false>, dim3(grid), kTileThreads, stream, s->dev, cams[0], g, set, R, q, gf, ctr);
+    }
     return hipGetLastError();
 }
 
@@ -1348,20 +1415,23 @@ hipError_t launchGrisTemporal(const rt_device_scene* s, const rt_camera* cams, c
                               const rt_gris_settings& set, hipStream_t stream) {
     GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
     GB gp = makeGB(f.depthNormalPrev, f.albedoMatIdPrev, nullptr, f.width, f.height);
-    RT_DISPATCH_LU(k_gris_temporal, set.targetFunction == 1u, set.unbiasedReuse == 1u, tileGrid(f.width, f.height),
+    RT_DISPATCH_LUG(k_gris_temporal, set.targetFunction == 1u, set.unbiasedReuse == 1u, set.guidedProposal == 1u,
+                    tileGrid(f.width, f.height),
                    kTileThreads, stream, s->dev,
                   cams[0], cams[1], g, gp, set, planes(f.grisReservoir, f.width, f.height),
-                  planes(f.grisReservoirPrev, f.width, f.height), planes(f.grisReservoirTemp, f.width, f.height));
+                  planes(f.grisReservoirPrev, f.width, f.height), planes(f.grisReservoirTemp, f.width, f.height),
+                  guideFrame(f, set));
     return hipGetLastError();
 }
 
 hipError_t launchGrisSpatial(const rt_device_scene* s, const rt_camera* cams, const rt_frame& f,
                              const rt_gris_settings& set, hipStream_t stream) {
     GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
-    RT_DISPATCH_LU(k_gris_spatial, set.targetFunction == 1u, set.unbiasedReuse == 1u, tileGrid(f.width, f.height),
+    RT_DISPATCH_LUG(k_gris_spatial, set.targetFunction == 1u, set.unbiasedReuse == 1u, set.guidedProposal == 1u,
+                    tileGrid(f.width, f.height),
                    kTileThreads, stream, s->dev,
                   cams[0], g, set, planes(f.grisReservoirTemp, f.width, f.height),
-                  planes(f.grisReservoir, f.width, f.height), f.indirectOutput);
+                  planes(f.grisReservoir, f.width, f.height), f.indirectOutput, guideFrame(f, set));
     return hipGetLastError();
 }
 

@@ -115,7 +115,7 @@ class GrisSettings(C.Structure):
     _fields_ = [("shiftType", C.c_uint32), ("rrScale", C.c_float), ("temporalReuse", C.c_uint32),
                 ("spatialReuse", C.c_uint32), ("cap", C.c_uint32), ("maxDepth", C.c_uint32),
                 ("spatialNeighbors", C.c_uint32), ("spatialRadius", C.c_float), ("targetFunction", C.c_uint32),
-                ("unbiasedReuse", C.c_uint32)]
+                ("unbiasedReuse", C.c_uint32), ("guidedProposal", C.c_uint32)]
 
 
 class MlpConfig(C.Structure):
@@ -130,7 +130,8 @@ class Frame(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("directOutput", C.c_void_p),
                 ("indirectOutput", C.c_void_p), ("depthNormal", C.c_void_p), ("depthNormalPrev", C.c_void_p),
                 ("albedoMatId", C.c_void_p), ("albedoMatIdPrev", C.c_void_p), ("motionVector", C.c_void_p),
-                ("grisReservoir", C.c_void_p), ("grisReservoirPrev", C.c_void_p), ("grisReservoirTemp", C.c_void_p)]
+                ("grisReservoir", C.c_void_p), ("grisReservoirPrev", C.c_void_p), ("grisReservoirTemp", C.c_void_p),
+                ("guideLobes", C.c_void_p), ("guideLobesPrev", C.c_void_p), ("guideSamples", C.c_void_p)]
 
 
 class View(C.Structure):
@@ -153,10 +154,10 @@ assert C.sizeof(DIReservoir) == 64 and C.sizeof(DISettings) == 16
 
 
 def settings(shift=2, rr_scale=1.0, temporal=False, spatial=True, cap=20, max_depth=15, neighbors=3, radius=20.0,
-             learned=False, unbiased=False):
+             learned=False, unbiased=False, guided=False):
     """GRISReSTIR::Settings (src/GRISReSTIR.h:29 defaults) + kernel constants."""
     return GrisSettings(shift, rr_scale, int(temporal), int(spatial), cap, max_depth, neighbors, radius, int(learned),
-                        int(unbiased))
+                        int(unbiased), int(guided))
 
 
 def di_settings(shift=0, sample=0, temporal=False, spatial=True):
@@ -181,6 +182,7 @@ CORE_SYMBOLS = [
     "rt_counters_read", "rt_mlp_create", "rt_mlp_destroy", "rt_mlp_param_counts", "rt_mlp_get_params",
     "rt_mlp_set_params", "rt_mlp_infer", "rt_mlp_gradients", "rt_mlp_read_gradients", "rt_mlp_adam_step",
     "rt_mlp_train_step", "rt_mlp_train_frame", "rt_mlp_batch_download", "rt_gris_learned_refresh",
+    "rt_guide_create", "rt_guide_lobe_bytes", "rt_gris_guide_refresh", "rt_gris_guide_train",
     "rt_frame_graph_create", "rt_frame_graph_launch", "rt_frame_graph_destroy", "rt_batch_create",
     "rt_batch_destroy", "rt_batch_frame", "rt_batch_set_tuning", "rt_batch_get_tuning", "rt_batch_autotune",
     "rt_di_reservoir_bytes", "rt_di_reservoir_reset", "rt_di_reservoir_download", "rt_di_reservoir_upload",
@@ -236,6 +238,9 @@ def _declare_core(L):
         "rt_mlp_train_frame": [P, P, P],
         "rt_mlp_batch_download": [P, P, P, C.POINTER(C.c_uint32)],
         "rt_gris_learned_refresh": [P, P, C.POINTER(Camera), C.POINTER(Frame), P],
+        "rt_guide_create": [C.POINTER(MlpConfig), P, C.POINTER(P)],
+        "rt_gris_guide_refresh": [P, P, C.POINTER(Camera), C.POINTER(Frame), P],
+        "rt_gris_guide_train": [P, P, C.POINTER(Camera), C.POINTER(Frame), P, P],
         "rt_frame_graph_create": [P, C.POINTER(Frame), C.POINTER(GrisSettings), P, C.POINTER(P)],
         "rt_frame_graph_launch": [P, u32, C.POINTER(Camera), P],
         "rt_frame_graph_destroy": [P],
@@ -261,7 +266,7 @@ def _declare_core(L):
         f = getattr(L, name)
         f.argtypes = args
         f.restype = i32
-    for name in ("rt_gris_reservoir_bytes", "rt_di_reservoir_bytes", "rt_gi_reservoir_bytes"):
+    for name in ("rt_gris_reservoir_bytes", "rt_di_reservoir_bytes", "rt_gi_reservoir_bytes", "rt_guide_lobe_bytes"):
         getattr(L, name).argtypes = [u32, u32]
         getattr(L, name).restype = C.c_size_t
 
@@ -415,12 +420,17 @@ class DeviceScene:
 
 
 class Mlp:
-    """rt_mlp: the learned target function (hash grid + MLP) on the current device."""
+    """rt_mlp: the learned target function (hash grid + MLP) on the current device; with
+    guide=True the learned proposal's network (rt_guide_create: 32 outputs, an 8-lobe vMF mixture)."""
 
-    def __init__(self, config: MlpConfig, stream=None):
+    def __init__(self, config: MlpConfig, stream=None, guide=False):
         L = core()
         h = C.c_void_p()
-        check(L.rt_mlp_create(C.byref(config), stream, C.byref(h)), "rt_mlp_create")
+        if guide:
+            check(L.rt_guide_create(C.byref(config), stream, C.byref(h)), "rt_guide_create")
+        else:
+            check(L.rt_mlp_create(C.byref(config), stream, C.byref(h)), "rt_mlp_create")
+        self.guide = guide
         self.handle = h
         self.config = config
         g, w = C.c_uint64(), C.c_uint64()
@@ -445,9 +455,10 @@ class Mlp:
         return grid, w
 
     def batch(self):
-        """Training batch gathered by the last rt_gris_learned_refresh: (pos [n,3], target [n,3])."""
+        """Training batch gathered by the last rt_gris_learned_refresh (rt_gris_guide_train):
+        (pos [n,3], target [n,3] or [n,4])."""
         pos = np.zeros((1 << 16, 3), np.float32)
-        tgt = np.zeros((1 << 16, 3), np.float32)
+        tgt = np.zeros((1 << 16, 4 if self.guide else 3), np.float32)
         n = C.c_uint32()
         check(core().rt_mlp_batch_download(self.handle, pos.ctypes.data, tgt.ctypes.data, C.byref(n)), "batch")
         return pos[: n.value], tgt[: n.value]
