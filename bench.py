@@ -220,6 +220,8 @@ class TorchFrames:
         self.resv, self.temp = [z(rb), z(rb)], z(rb)
         self.direct = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
         self.out = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+        lb = rt.core().rt_guide_lobe_bytes(W, H)
+        self.lobes, self.samples = [z(lb), z(lb)], z(16 * n)
         self.stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
         self.cur = 0
         self.reset()
@@ -239,15 +241,20 @@ class TorchFrames:
         f.motionVector = self.mv.data_ptr()
         f.grisReservoir, f.grisReservoirPrev = self.resv[c].data_ptr(), self.resv[l].data_ptr()
         f.grisReservoirTemp = self.temp.data_ptr()
+        f.guideLobes, f.guideLobesPrev = self.lobes[c].data_ptr(), self.lobes[l].data_ptr()
+        f.guideSamples = self.samples.data_ptr()
         return f
 
-    def frame(self, cams, s, mlp=None, events=None):
-        """G-buffer, GRIS path trace, [learned refresh], temporal, spatial, [one training step]."""
+    def frame(self, cams, s, mlp=None, events=None, guide=None):
+        """G-buffer, [proposal refresh], GRIS path trace, [learned refresh], temporal, spatial, [one training
+        step of each network]."""
         rt, L, st = self.rt, self.rt.core(), self.stream
         f = self.frame_struct()
         if events: events[0].record()
         rt.check(L.rt_gbuffer_trace(self.ds.handle, C.byref(cams[0]), f.depthNormal, f.albedoMatId, f.motionVector,
                                     st), "gbuffer")
+        if guide is not None:
+            rt.check(L.rt_gris_guide_refresh(self.ds.handle, guide.handle, cams, C.byref(f), st), "guide refresh")
         rt.check(L.rt_gris_path_trace(self.ds.handle, cams, C.byref(f), C.byref(s), st), "path trace")
         if mlp is not None:
             rt.check(L.rt_gris_learned_refresh(self.ds.handle, mlp.handle, cams, C.byref(f), st), "refresh")
@@ -255,6 +262,8 @@ class TorchFrames:
         rt.check(L.rt_gris_spatial(self.ds.handle, cams, C.byref(f), C.byref(s), st), "spatial")
         if mlp is not None:
             rt.check(L.rt_mlp_train_frame(mlp.handle, None, st), "train")
+        if guide is not None:
+            rt.check(L.rt_gris_guide_train(self.ds.handle, guide.handle, cams, C.byref(f), None, st), "guide train")
         if events: events[1].record()
         self.cur ^= 1
 
@@ -333,12 +342,15 @@ def learned_vs_plain(rt, hs, ds, args):
     def new_mlp():
         return rt.Mlp(rt.mlp_config(bounds[0], bounds[1], log2_table=19, seed=3))
 
-    def run(s, frames, seed, mlp=None):
+    def new_guide():
+        return rt.Mlp(rt.mlp_config(bounds[0], bounds[1], log2_table=19, seed=4), guide=True)
+
+    def run(s, frames, seed, mlp=None, guide=None):
         fr.reset()
         acc = torch.zeros((H, W, 3), dtype=torch.float64, device="cuda")
         ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(frames)]
         for i in range(frames):
-            fr.frame(static_cams(rt, base, i, seed), s, mlp, ev[i])
+            fr.frame(static_cams(rt, base, i, seed), s, mlp, ev[i], guide)
             acc += fr.out[..., :3].double()
         torch.cuda.synchronize()
         ms = sum(e[0].elapsed_time(e[1]) for e in ev) / frames
@@ -359,9 +371,34 @@ def learned_vs_plain(rt, hs, ds, args):
         return float((((img - ref) ** 2) / (ref ** 2 + 1e-2)).sum(-1).mean())
 
     rp, rl = relmse(img_p), relmse(img_l)
+    # learned vMF-mixture proposal, equal time, with the reference's weights and in the unbiased mode
+    guided = {}
+    for mode, unb, reuse in (("no_reuse", False, False), ("reference_weights", False, True), ("unbiased", True, True)):
+        kw = dict(shift=2, temporal=reuse, spatial=reuse, cap=20, max_depth=6, neighbors=3, radius=radius,
+                  unbiased=unb)
+        s_pl, s_gu = rt.settings(**kw), rt.settings(guided=True, **kw)
+        _, pl_ms = run(s_pl, 16, 11)
+        _, gu_ms = run(s_gu, 16, 13, None, new_guide())
+        n_pl = args.lvp_frames
+        n_gu = max(1, int(n_pl * pl_ms // gu_ms))
+        img_pl, pl_ms = run(s_pl, n_pl, 17)
+        img_gu, gu_ms = run(s_gu, n_gu, 17, None, new_guide())
+        # warm start: the network trained on 128 earlier frames of the view (other seeds), still
+        # training during the measured frames
+        warm = new_guide()
+        run(s_gu, 128, 29, None, warm)
+        img_gw, _ = run(s_gu, n_gu, 17, None, warm)
+        r_pl, r_gu, r_gw = relmse(img_pl), relmse(img_gu), relmse(img_gw)
+        guided[mode] = {"relmse_plain": round(r_pl, 6), "relmse_guided": round(r_gu, 6),
+                        "relmse_guided_warm": round(r_gw, 6),
+                        "plain_over_guided": round(r_pl / r_gu, 4) if r_gu > 0 else None,
+                        "plain_over_guided_warm": round(r_pl / r_gw, 4) if r_gw > 0 else None,
+                        "frames": {"plain": n_pl, "guided": n_gu},
+                        "ms_per_frame": {"plain": round(pl_ms, 3), "guided": round(gu_ms, 3)}}
     mlp_perf = mlp_utilisation(rt, ds, fr, s_learn, base, new_mlp())
     return {"relmse_plain": round(rp, 6), "relmse_learned": round(rl, 6),
             "plain_over_learned": round(rp / rl, 4) if rl > 0 else None,
+            "guided_proposal": guided,
             "frames": {"plain": n_plain, "learned": n_learn}, "ms_per_frame": {"plain": round(plain_ms, 3),
                                                                                 "learned": round(learn_ms, 3)},
             "equal_time_ms": round(budget, 1), "mlp": mlp_perf,

@@ -413,8 +413,9 @@ rt_status rt_gris_learned_refresh(const rt_device_scene* scene, rt_mlp* mlp, con
 /* ---- Learned directional proposal -----------------------------------------
  * No counterpart in the reference. A second network of the same encoding and
  * hidden layers whose 32 outputs are an 8-lobe von Mises-Fisher mixture per
- * position (lobe k: weight logit, log concentration clamped to [-4, 8], polar
- * and azimuth angles through sigmoids), trained online on the direction the
+ * position (lobe k: weight logit, log concentration squashed into (-4, 5), polar
+ * and azimuth angles through sigmoids; fixed offsets put the untrained lobes at
+ * the eight octant directions), trained online on the direction the
  * path tracer sampled at each primary vertex weighted by the path's luminance
  * (negative log-likelihood weighted by the sample weight over the batch mean),
  * and sampled at the primary vertex by settings.guidedProposal = 1. All

@@ -25,7 +25,9 @@ constexpr int kW1 = 0, kW2 = kEnc * kH, kW3 = kW2 + kH * kH, kWeights = kW3 + kH
 constexpr int kGuideWeights = kW3 + kH * kGuideOut;
 // Raw output -> lobe parameter offsets (the untrained network's lobes sit at the octant
 // directions (+-1, +-1, +-1)/sqrt 3, bit 0/1/2 of k = negative x/y/z, with concentration ~4):
-// b + ln 4, c + logit(theta_k / pi), d + logit(phi_k / 2 pi); log kappa = 2 + 6 tanh((b' - 2) / 6).
+// b + ln 4, c + logit(theta_k / pi), d + logit(phi_k / 2 pi); log kappa = 0.5 + 4.5 tanh((b' - 0.5) /
+// 4.5), in (-4, 5).
+constexpr double kLogKappaMid = 0.5, kLogKappaHalf = 4.5;
 double guideBias(int k, int e) {
     if (e == 1) return std::log(4.0);
     const double sx = (k & 1) ? -1.0 : 1.0, sy = (k & 2) ? -1.0 : 1.0, sz = (k & 4) ? -1.0 : 1.0;
@@ -145,7 +147,7 @@ void forward(const Net& N, const float* p, Act& A, int nOut = kOut) {
 
 // ---- proposal network: 8-lobe vMF mixture ------------------------------------------------
 // Lobe k of the raw outputs z[4k..4k+3] = (a, b, c, d), offset by guideBias: pi = softmax(a),
-// log kappa = 2 + 6 tanh((b' - 2) / 6), theta = pi sigmoid(c'), phi = 2 pi sigmoid(d'),
+// log kappa = 0.5 + 4.5 tanh((b' - 0.5) / 4.5), theta = pi sigmoid(c'), phi = 2 pi sigmoid(d'),
 // mu = (sin theta cos phi, sin theta sin phi, cos theta); v(w) = kappa / (2 pi (1 - e^{-2 kappa}))
 // e^{kappa (mu.w - 1)}.
 struct Mixture {
@@ -158,8 +160,8 @@ void mixtureOf(const double* z, Mixture& M) {
     for (int k = 0; k < kLobes; k++) { M.pi[k] = std::exp(z[4 * k] - amax); es += M.pi[k]; }
     for (int k = 0; k < kLobes; k++) {
         M.pi[k] /= es;
-        const double t = std::tanh((z[4 * k + 1] + guideBias(k, 1) - 2.0) / 6.0);
-        M.kap[k] = std::exp(2.0 + 6.0 * t);
+        const double t = std::tanh((z[4 * k + 1] + guideBias(k, 1) - kLogKappaMid) / kLogKappaHalf);
+        M.kap[k] = std::exp(kLogKappaMid + kLogKappaHalf * t);
         M.dlk[k] = 1.0 - t * t;
         M.sc[k] = 1.0 / (1.0 + std::exp(-(z[4 * k + 2] + guideBias(k, 2))));
         M.sd[k] = 1.0 / (1.0 + std::exp(-(z[4 * k + 3] + guideBias(k, 3))));

@@ -50,9 +50,11 @@ class View:
         self.run = gpuhelp.GrisRun(self.ds, W, H)
         self.lum = torch.tensor([0.299, 0.587, 0.114], dtype=torch.float64, device="cuda")
 
-    def frame(self, cams, s, mlp=None, train=False):
+    def frame(self, cams, s, mlp=None, train=False, guide=None, train_guide=False):
         r = self.run
         r.gbuffer(cams[0])
+        if guide is not None:
+            r.guide_refresh(guide, cams)
         r.path_trace(cams, s)
         if mlp is not None:
             r.learned_refresh(mlp, cams)
@@ -60,9 +62,11 @@ class View:
         r.spatial(cams, s)
         if train:
             rt.check(rt.core().rt_mlp_train_frame(mlp.handle, None, gpuhelp.stream()), "train")
+        if train_guide:
+            r.guide_train(guide, cams)
         r.flip()
 
-    def image_means(self, s, mlp=None, frames=2, label=""):
+    def image_means(self, s, mlp=None, frames=2, label="", guide=None):
         print(f"[gpu] bias {label}: {SEEDS} seeds x {frames} frames", flush=True)
         L, r = rt.core(), self.run
         out = []
@@ -71,7 +75,7 @@ class View:
                 rt.check(L.rt_gris_reservoir_reset(b.data_ptr(), self.W, self.H, gpuhelp.stream()), "reset")
             r.cur = 0
             for i in range(frames):
-                self.frame(still(self.base, i, seed), s, mlp)
+                self.frame(still(self.base, i, seed), s, mlp, guide=guide)
             img = r.indirect.view(torch.float32).view(self.H, self.W, 4)[..., :3].double() @ self.lum
             out.append(img.mean())
         return torch.stack(out).cpu().numpy()
@@ -121,3 +125,25 @@ def test_reference_weights_bias_is_detected_and_removed():
           f"unbiased {rel_unb:+.4f} (z {z_unb:+.2f})", flush=True)
     assert z_ref < -4.0
     assert abs(rel_unb) < 0.5 * abs(rel_ref)
+
+
+def test_guided_unbiased_reuse_shows_no_bias():
+    """The learned proposal (trained on the view, then fixed) with unbiased temporal + spatial reuse
+    against the GRIS estimator without reuse or proposal. The two consume different random numbers,
+    so the comparison is unpaired: z = difference of means over its standard error."""
+    from test_gpu_guide import GuidedRun
+
+    v = View(1, 128, 72)
+    v.run = GuidedRun(v.ds, v.W, v.H)
+    lo, hi = bounds(v.ds)
+    guide = rt.Mlp(rt.mlp_config(lo, hi, log2_table=16, seed=5), guide=True)
+    kw = dict(shift=2, max_depth=6, cap=20, radius=7.5)
+    s_gu = rt.settings(temporal=True, spatial=True, unbiased=True, guided=True, **kw)
+    for i in range(32):
+        v.frame(still(v.base, i, 77777), s_gu, guide=guide, train_guide=True)
+    none = v.image_means(rt.settings(temporal=False, spatial=False, **kw), label="no reuse, BSDF sampling")
+    gu = v.image_means(s_gu, label="guided, unbiased reuse", guide=guide)
+    d = gu.mean() - none.mean()
+    z = d / np.sqrt(gu.var(ddof=1) / len(gu) + none.var(ddof=1) / len(none))
+    print(f"[gpu] guided unbiased reuse vs none: bias {d / none.mean():+.4f}, z {z:+.2f}", flush=True)
+    assert abs(z) < 4.0 and abs(d / none.mean()) < 0.02

@@ -51,7 +51,7 @@ def numpy_mixture(z):
     ph0 = np.mod(np.arctan2(o[:, 1], o[:, 0]), 2 * np.pi)
     a = np.exp(z[:, 0] - z[:, 0].max())
     pi = a / a.sum()
-    kap = np.exp(2 + 6 * np.tanh((z[:, 1] + np.log(4) - 2) / 6))
+    kap = np.exp(0.5 + 4.5 * np.tanh((z[:, 1] + np.log(4) - 0.5) / 4.5))
     th = np.pi / (1 + np.exp(-(z[:, 2] + np.log(th0 / (np.pi - th0)))))
     ph = 2 * np.pi / (1 + np.exp(-(z[:, 3] + np.log(ph0 / (2 * np.pi - ph0)))))
     mu = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], 1)
@@ -77,14 +77,14 @@ def raw_outputs(seed, log_kappa):
     """Raw outputs whose lobes have the given log concentrations (inverting the soft clamp)."""
     g = np.random.default_rng(seed)
     z = g.normal(0, 1.2, (8, 4))
-    z[:, 1] = 2 + 6 * np.arctanh((np.asarray(log_kappa) - 2) / 6) - np.log(4)
+    z[:, 1] = 0.5 + 4.5 * np.arctanh((np.asarray(log_kappa) - 0.5) / 4.5) - np.log(4)
     return z.reshape(-1).astype(np.float32)
 
 
 def test_zero_outputs_give_octant_lobes():
     lobes = orc.guide_lobes(np.zeros((1, 32), np.float32))[0]
     assert np.allclose(lobes[0::2, :3], octants(), atol=1e-6)
-    assert np.allclose(lobes[0::2, 3], 4.0, rtol=1e-2) and np.allclose(lobes[1::2, 3], 1 / 8)
+    assert np.allclose(lobes[0::2, 3], 4.0, rtol=2e-2) and np.allclose(lobes[1::2, 3], 1 / 8)
 
 
 @pytest.mark.parametrize("log_kappa", [np.linspace(-3.5, 3, 8), np.linspace(0.5, 3.5, 8)])
@@ -106,7 +106,7 @@ def test_lobe_conversion_invariants():
     mu, kap = L[:, 0::2, :3], L[:, 0::2, 3]
     c, e2, cdf, pi = L[:, 1::2, 0], L[:, 1::2, 1], L[:, 1::2, 2], L[:, 1::2, 3]
     assert np.allclose(np.linalg.norm(mu, axis=-1), 1, atol=1e-6)
-    assert (kap >= np.exp(-4) * (1 - 1e-6)).all() and (kap <= np.exp(8) * (1 + 1e-6)).all() and kap.std() > 1
+    assert (kap >= np.exp(-4) * (1 - 1e-6)).all() and (kap <= np.exp(5) * (1 + 1e-6)).all() and kap.std() > 1
     assert np.allclose(pi.sum(1), 1, atol=1e-6) and (np.diff(cdf, axis=1) >= 0).all() and (cdf[:, -1] == 1).all()
     assert np.allclose(cdf, np.cumsum(pi, 1), atol=1e-6)
     # e^{-2 kappa} of the unrounded kappa: relative error up to 2 kappa x float rounding of kappa
@@ -114,7 +114,7 @@ def test_lobe_conversion_invariants():
     assert np.allclose(c, pi * kap / (2 * np.pi * (1 - np.exp(-2 * kap))), rtol=1e-5)
 
 
-@pytest.mark.parametrize("log_kappa", [np.linspace(-3, 2, 8), np.linspace(3, 7, 8)], ids=["broad", "sharp"])
+@pytest.mark.parametrize("log_kappa", [np.linspace(-3, 2, 8), np.linspace(3, 4.9, 8)], ids=["broad", "sharp"])
 def test_mixture_sampler_matches_density(log_kappa):
     z = raw_outputs(7, log_kappa)
     lobes = orc.guide_lobes(z[None])[0]

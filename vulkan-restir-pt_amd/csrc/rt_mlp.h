@@ -34,7 +34,7 @@ constexpr int kGuideWeights = weightsFor(kGuideOut);  // 8192
 // Raw output -> lobe parameter, per lobe k: (a + 0, b + ln 4, c + cBias_k, d + dBias_k) with the
 // offsets placing the untrained network's lobes (outputs near 0: the network has no biases) at the
 // eight octant directions (+-1, +-1, +-1) / sqrt 3 with concentration ~4 (bit 0/1/2 of k: negative
-// x/y/z). pi = softmax(a'), log kappa = 2 + 6 tanh((b' - 2) / 6) in (-4, 8),
+// x/y/z). pi = softmax(a'), log kappa = 0.5 + 4.5 tanh((b' - 0.5) / 4.5) in (-4, 5),
 // theta = pi sigmoid(c'), phi = 2 pi sigmoid(d'),  mu = (sin theta cos phi, sin theta sin phi, cos theta).
 constexpr float kGuideKappaBias = 1.3862943611198906f;          // ln 4
 constexpr float kGuideThetaBias = 0.8278732f;                   // logit(acos(1/sqrt 3) / pi), negated for +z
@@ -47,10 +47,11 @@ RT_DEV float guideBias(int k, int e) {
     return 0.0f;
 }
 // log kappa of the biased raw output b' and its derivative d log kappa / d b'
+constexpr float kGuideLogKappaMid = 0.5f, kGuideLogKappaHalf = 4.5f;  // log kappa in (-4, 5)
 RT_DEV float guideLogKappa(float b, float& dlk) {
-    const float t = tanhf((b - 2.0f) * (1.0f / 6.0f));
+    const float t = tanhf((b - kGuideLogKappaMid) * (1.0f / kGuideLogKappaHalf));
     dlk = 1.0f - t * t;
-    return 2.0f + 6.0f * t;
+    return kGuideLogKappaMid + kGuideLogKappaHalf * t;
 }
 constexpr int kTileSamples = 32;
 constexpr uint32_t kBatchCap = 1u << 16;  // training samples gathered per frame
