@@ -64,6 +64,8 @@ def parse():
                     help="time learned-target frames only (for profilers)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-view leg")
     ap.add_argument("--no-4k", action="store_true", help="skip the 4K full-form leg")
+    ap.add_argument("--guided-kernel-loop", action="store_true",
+                    help="time frames with the learned proposal only (for profilers)")
     ap.add_argument("--batched-views", type=int, default=64)
     ap.add_argument("--lvp-frames", type=int, default=64, help="frames of the plain form in the equal-time run")
     ap.add_argument("--ref-frames", type=int, default=4096, help="frames of the converged reference image")
@@ -435,6 +437,34 @@ def learned_kernel_loop(rt, hs, ds, args):
             "mpaths_per_s": round(W * H * args.steps / wall / 1e6, 3)}
 
 
+def guided_kernel_loop(rt, hs, ds, args):
+    """Frames with the learned proposal on the benchmark camera walk (G-buffer, proposal refresh, path
+    trace, temporal, spatial, one proposal training step), unbiased reuse, with HIP events around the
+    proposal's own work, for profiling."""
+    import torch
+
+    W, H = args.width, args.height
+    fr = TorchFrames(rt, ds, W, H)
+    s = rt.settings(shift=2, temporal=True, spatial=True, cap=20, max_depth=6, neighbors=3,
+                    radius=30.0 * W / 1920.0, unbiased=True, guided=True)
+    info = ds.info()
+    lo, hi = [float(v) for v in info.sceneLo], [float(v) for v in info.sceneHi]
+    ext = [max(h - l, 1e-3) for l, h in zip(lo, hi)]
+    guide = rt.Mlp(rt.mlp_config([l - 0.1 * e for l, e in zip(lo, ext)], [h + 0.1 * e for h, e in zip(hi, ext)],
+                                 log2_table=19, seed=4), guide=True)
+    for k in range(args.warmup):
+        fr.frame(hs.camera(k % 16, W, H), s, None, None, guide)
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        fr.frame(hs.camera((args.warmup + k) % 16, W, H), s, None, ev[k], guide)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return {"guided_frames": args.steps, "ms_per_frame": round(wall * 1e3 / args.steps, 3),
+            "gpu_ms_per_frame": round(sum(e[0].elapsed_time(e[1]) for e in ev) / args.steps, 3)}
+
+
 def batched_views(rt, args, hs1, ds1):
     """Batched plain ReSTIR PT: args.batched_views views over four scenes (~20k, ~225k, ~1.1M and ~3.0M
     triangles), square resolutions log-uniform in [128, 1024] (multiples of 8), max depth uniform in
@@ -780,6 +810,9 @@ def main():
         return
     if args.di_kernel_loop:
         print(json.dumps(restir_di(rt, hs, ds, args, kernel_loop=True)))
+        return
+    if args.guided_kernel_loop:
+        print(json.dumps(guided_kernel_loop(rt, hs, ds, args)))
         return
     loop = FrameLoop(rt, hs, ds, W, H, s)
     for k in range(args.warmup):

@@ -353,16 +353,18 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpView M, const float* __rest
 }
 
 // Mean weight of the valid samples of a proposal batch (one workgroup, fixed-order sums).
-__global__ __launch_bounds__(256) void k_guide_wmean(const float* __restrict__ target, uint32_t n, float* __restrict__ out) {
-    __shared__ float sw[256], sc[256];
+constexpr uint32_t kMeanThreads = 1024;
+__global__ __launch_bounds__(kMeanThreads) void k_guide_wmean(const float* __restrict__ target, uint32_t n,
+                                                              float* __restrict__ out) {
+    __shared__ float sw[kMeanThreads], sc[kMeanThreads];
     float w = 0.0f, c = 0.0f;
-    for (uint32_t i = threadIdx.x; i < n; i += 256u) {
+    for (uint32_t i = threadIdx.x; i < n; i += kMeanThreads) {
         const float* y = target + 4 * (size_t)i;
         if (isfinite(y[3]) && y[3] > 0.0f && isfinite(y[0]) && isfinite(y[1]) && isfinite(y[2])) { w += y[3]; c += 1.0f; }
     }
     sw[threadIdx.x] = w; sc[threadIdx.x] = c;
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
+    for (int off = (int)kMeanThreads / 2; off > 0; off >>= 1) {
         if ((int)threadIdx.x < off) { sw[threadIdx.x] += sw[threadIdx.x + off]; sc[threadIdx.x] += sc[threadIdx.x + off]; }
         __syncthreads();
     }
@@ -452,7 +454,7 @@ hipError_t mlpGradients(rt_mlp* m, const float* pos, const float* target, uint32
     const uint32_t tiles = (n + kTileSamples - 1) / kTileSamples;
     const uint32_t waves = std::max(1u, std::min(tiles, m->partialWaves));
     if (n && guide) {
-        (void)launch(k_guide_wmean, dim3(1), dim3(256), st, target, n, m->wmean);
+        (void)launch(k_guide_wmean, dim3(1), dim3(kMeanThreads), st, target, n, m->wmean);
         (void)launch(k_mlp_train<kGuideOut>, dim3(waves), dim3(64), st, m->view, pos, target, n, (const float*)m->wmean,
                      m->gridGrad, m->partial);
     } else if (n) {
