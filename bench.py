@@ -552,9 +552,10 @@ def full_form_4k(rt, args):
     """The full-form workload at 3840x2160 on the ~4.8M-triangle scene (1024x1024-cell fBm heightfield
     + 2000 instanced icospheres, 64 emissive triangles, GGX roughness U[0.02, 1]), depth 8, hybrid
     shift, temporal + spatial: per frame the 4K view with the learned target (refresh after path
-    tracing, one training step after spatial reuse) plus 16 extra 512x512 views of the same scene in
-    one batched launch sequence (luminance target: the batched passes take no network). The learned
-    vMF-mixture proposal of this workload is not built; paths are sampled from the BSDFs."""
+    tracing, one training step after spatial reuse) and the learned vMF-mixture proposal (refresh
+    after the G-buffer, one training step after spatial reuse), unbiased reuse, plus 16 extra 512x512
+    views of the same scene in one batched launch sequence (luminance target and BSDF sampling: the
+    batched passes take no network)."""
     import numpy as np
     import torch
 
@@ -563,12 +564,13 @@ def full_form_4k(rt, args):
     W, H = 3840, 2160
     fr = TorchFrames(rt, ds, W, H)
     s4k = rt.settings(shift=2, temporal=True, spatial=True, cap=20, max_depth=8, neighbors=3, radius=30.0,
-                      learned=True)
+                      learned=True, unbiased=True, guided=True)
     info = ds.info()
     lo, hi = [float(v) for v in info.sceneLo], [float(v) for v in info.sceneHi]
     ext = [max(h - l, 1e-3) for l, h in zip(lo, hi)]
-    mlp = rt.Mlp(rt.mlp_config([l - 0.1 * e for l, e in zip(lo, ext)], [h + 0.1 * e for h, e in zip(hi, ext)],
-                               log2_table=19, seed=4))
+    box = ([l - 0.1 * e for l, e in zip(lo, ext)], [h + 0.1 * e for h, e in zip(hi, ext)])
+    mlp = rt.Mlp(rt.mlp_config(box[0], box[1], log2_table=19, seed=4))
+    guide = rt.Mlp(rt.mlp_config(box[0], box[1], log2_table=19, seed=5), guide=True)
     n_extra, size = 16, 512
     extra = [TorchFrames(rt, ds, size, size) for _ in range(n_extra)]
     s_extra = rt.settings(shift=2, temporal=True, spatial=True, cap=20, max_depth=8, neighbors=3, radius=30.0)
@@ -596,7 +598,7 @@ def full_form_4k(rt, args):
             e.reset()
 
         def frame(k, ev=None):
-            fr.frame(hs.camera(k % 64, W, H), s4k, mlp, ev)
+            fr.frame(hs.camera(k % 64, W, H), s4k, mlp, ev, guide)
             if ev is not None:
                 ev[2].record()
             rt.check(L.rt_batch_frame(b, views_for(k), n_extra, stream), "batch frame")
@@ -618,16 +620,17 @@ def full_form_4k(rt, args):
     finally:
         L.rt_batch_destroy(b)
     pixels = W * H + n_extra * size * size
-    view_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / steps
+    view_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / steps  # G-buffer .. both training steps
     batch_ms = sum(e[2].elapsed_time(e[3]) for e in ev) / steps
     return {"ms_per_frame": round(wall * 1e3 / steps, 3), "mpaths_per_s": round(pixels * steps / wall / 1e6, 2),
-            "main_view_learned_ms": round(view_ms, 3), "extra_views_batched_ms": round(batch_ms, 3),
+            "main_view_learned_guided_ms": round(view_ms, 3), "extra_views_batched_ms": round(batch_ms, 3),
             "pixels_per_frame": pixels, "triangles": int(info.numPrimitives), "emissive": int(hs.arrays.numLights),
             "tuning": {"persistent_groups": int(best.persistentGroups), "xcd_swizzle": int(best.xcdSwizzle)},
-            "workload": "3840x2160 view, learned target (hash grid 16x2x2^19 + MLP 2x64, trained each frame), "
-                        "+ 16 x 512x512 views batched (luminance target), ~4.8M-triangle scene, 64 emissive, "
-                        "depth 8, hybrid shift, temporal + spatial (3 neighbours, 30 px, M-cap 20); "
-                        "the vMF-mixture proposal is not built"}
+            "workload": "3840x2160 view with the learned target (hash grid 16x2x2^19 + MLP 2x64) and the learned "
+                        "8-lobe vMF-mixture proposal (second network, 32 outputs), both trained each frame, unbiased "
+                        "reuse; + 16 x 512x512 views batched (luminance target, BSDF sampling), ~4.8M-triangle "
+                        "scene, 64 emissive, depth 8, hybrid shift, temporal + spatial (3 neighbours, 30 px, "
+                        "M-cap 20)"}
 
 
 def restir_di(rt, hs, ds, args, kernel_loop=False):

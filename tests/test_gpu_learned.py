@@ -30,10 +30,14 @@ def bits_equal(a, b):
     return np.array_equal(np.ascontiguousarray(a).view(np.uint32), np.ascontiguousarray(b).view(np.uint32))
 
 
-@pytest.mark.parametrize("config,W,H,unbiased", [(0, 96, 64, False), (1, 160, 96, False), (1, 160, 96, True)],
-                         ids=["boxroom", "terrain", "terrain-unbiased"])
-def test_learned_gris_parity(config, W, H, unbiased):
-    print(f"[gpu] learned parity config {config} {W}x{H} unbiased {unbiased}", flush=True)
+@pytest.mark.parametrize("config,W,H,unbiased,guided",
+                         [(0, 96, 64, False, False), (1, 160, 96, False, False), (1, 160, 96, True, False),
+                          (1, 160, 96, True, True)],
+                         ids=["boxroom", "terrain", "terrain-unbiased", "terrain-unbiased-proposal"])
+def test_learned_gris_parity(config, W, H, unbiased, guided):
+    """With guided: the learned vMF-mixture proposal on as well (the full form of the learned
+    estimator), its lobe planes taken from the GPU, its network trained between frames."""
+    print(f"[gpu] learned parity config {config} {W}x{H} unbiased {unbiased} proposal {guided}", flush=True)
     hs = rt.HostScene(config, 1)
     ds = rt.DeviceScene(hs.arrays)
     osc = orc.OracleScene(hs.arrays)
@@ -41,9 +45,16 @@ def test_learned_gris_parity(config, W, H, unbiased):
     mlp = rt.Mlp(rt.mlp_config(lo, hi, log2_table=16, seed=3))
     omlp = orc.OracleMlp(16, 16, 2048, lo, hi)
     s = rt.settings(shift=2, temporal=True, spatial=True, max_depth=6, cap=20, radius=20.0, learned=True,
-                    unbiased=unbiased)
-    run = gpuhelp.GrisRun(ds, W, H)
+                    unbiased=unbiased, guided=guided)
+    if guided:
+        from test_gpu_guide import GuidedRun
+        run = GuidedRun(ds, W, H)
+        guide = rt.Mlp(rt.mlp_config(lo, hi, log2_table=16, seed=4), guide=True)
+    else:
+        run = gpuhelp.GrisRun(ds, W, H)
     n = W * H
+    o_lobes = [np.zeros((16, n, 4), np.float32), np.zeros((16, n, 4), np.float32)]
+    o_samples = np.zeros((n, 4), np.float32)
     o_resv = [orc.reset_reservoirs(n), orc.reset_reservoirs(n)]
     o_learn = [np.zeros((n, 4), np.float32), np.zeros((n, 4), np.float32)]
     o_temp = orc.reset_reservoirs(n)
@@ -62,6 +73,10 @@ def test_learned_gris_parity(config, W, H, unbiased):
         prev = cur ^ 1
         pdn = o_dn[prev] if o_dn[prev] is not None else np.zeros_like(dn)
         pam = o_am[prev] if o_am[prev] is not None else np.zeros_like(am)
+        if guided:
+            run.guide_refresh(guide, cams)
+            o_lobes[cur] = run.lobes_host(run.cur)
+            osc.bind_guide(o_lobes[cur], o_lobes[prev], o_samples)
         run.path_trace(cams, s)
         osc.gris_path_trace(cams, dn, am, s, o_resv[cur])
         assert not gpuhelp.compare_reservoirs(run.reservoirs(run.resv[run.cur]), o_resv[cur]).any()
@@ -115,5 +130,11 @@ def test_learned_gris_parity(config, W, H, unbiased):
         rt.check(rt.core().rt_mlp_train_frame(mlp.handle, loss.data_ptr(), gpuhelp.stream()), "train frame")
         torch.cuda.synchronize()
         print(f"[gpu] frame {frame} training loss {loss.item():.4f}", flush=True)
+        if guided:
+            fg = dn.reshape(n, 4)[:, 0] > 0
+            assert bits_equal(run.samples_host()[fg], o_samples[fg]), "training records differ"
+            run.guide_train(guide, cams)
         run.flip()
         cur ^= 1
+    if guided:
+        osc.bind_guide()
