@@ -90,6 +90,15 @@ rt_status rt_renderer_reset_reservoirs(rt_renderer* r, hipStream_t stream);
  * *mlp (may be null) receives the network, owned by the renderer. */
 rt_status rt_renderer_enable_learned(rt_renderer* r, uint32_t seed, uint32_t log2TableSize, hipStream_t stream,
                                      rt_mlp** mlp);
+/* Creates the renderer's learned vMF-mixture proposal (rt_guide_create, same
+ * network shape and bounds as above), allocates its frame buffers (two lobe
+ * planes of 256 B/px, 16 B/px of training records) and binds it to the GRIS
+ * passes; frames with settings.guidedProposal = 1 then run the proposal's
+ * refresh before the path trace and one training step after spatial reuse.
+ * No counterpart in the reference. *mlp (may be null) receives the network,
+ * owned by the renderer. */
+rt_status rt_renderer_enable_proposal(rt_renderer* r, uint32_t seed, uint32_t log2TableSize, hipStream_t stream,
+                                      rt_mlp** mlp);
 
 /* Writes an 8-bit RGBA image (host memory, row 0 at the top) as a PNG file:
  * the stbi_write_png call of Renderer::writeScreenshot (src/Renderer.cpp:787-788).

@@ -289,3 +289,40 @@ def test_guided_requires_bindings():
     f.guideLobes = run.direct.data_ptr()
     assert rt.core().rt_gris_guide_refresh(ds.handle, radiance.handle, cams, C.byref(f), gpuhelp.stream()) != 0
     assert b"proposal network" in rt.core().rt_last_error()
+
+
+def test_renderer_with_learned_proposal():
+    """The host frame driver with the proposal enabled: guided frames of the unbiased estimator run,
+    train the network and give finite, positive images; the same camera path without the proposal
+    gives an image of the same mean within the noise."""
+    hs = rt.HostScene(0, 0)
+    ds = rt.DeviceScene(hs.arrays)
+    W, H = 64, 48
+    Hh = rt.host()
+    means = {}
+    for guided in (False, True):
+        r = C.c_void_p()
+        rt.check(Hh.rt_renderer_create(ds.handle, W, H, C.byref(r)), "renderer")
+        try:
+            if guided:
+                g = C.c_void_p()
+                rt.check(Hh.rt_renderer_enable_proposal(r, 4, 14, gpuhelp.stream(), C.byref(g)), "enable proposal")
+            s = rt.settings(shift=2, temporal=True, spatial=True, max_depth=4, unbiased=True, guided=guided)
+            rt.check(Hh.rt_renderer_set_settings(r, C.byref(s)), "settings")
+            acc = np.zeros((H, W, 3))
+            for frame in range(48):
+                rt.check(Hh.rt_renderer_frame(r, hs.camera(frame, W, H), 0, gpuhelp.stream()), "frame")
+                host = np.zeros((H, W, 4), np.float32)
+                rt.check(Hh.rt_renderer_read_output(r, 1, host.ctypes.data, gpuhelp.stream()), "read")
+                assert np.isfinite(host).all() and (host[..., :3] >= 0).all()
+                acc += host[..., :3]
+            means[guided] = acc.mean() / 48
+            if guided:
+                pos = np.zeros((1 << 16, 3), np.float32)
+                n = C.c_uint32()
+                rt.check(rt.core().rt_mlp_batch_download(g, pos.ctypes.data, None, C.byref(n)), "batch")
+                assert n.value == W * H
+        finally:
+            Hh.rt_renderer_destroy(r)
+    print(f"[gpu] renderer mean radiance: BSDF sampling {means[False]:.4f}, proposal {means[True]:.4f}", flush=True)
+    assert means[True] > 0 and abs(means[True] - means[False]) < 0.1 * means[False]

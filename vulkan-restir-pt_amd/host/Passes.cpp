@@ -53,13 +53,17 @@ void GRISReSTIR::createPipeline() {
 // Passes are stream ordered, which stands in for the reference's pipeline barriers.
 void GRISReSTIR::render(hipStream_t stream, const DeviceScene& scene, const rt_camera cams[2], const rt_frame& frame) {
     const bool learnedTarget = settings.targetFunction == 1u;
+    const bool guided = settings.guidedProposal == 1u;
     if (learnedTarget && !learned) throw std::runtime_error("GRISReSTIR: the learned target needs a network");
+    if (guided && !proposal) throw std::runtime_error("GRISReSTIR: the learned proposal needs a network");
+    if (guided) rtThrow(rt_gris_guide_refresh(scene.handle(), proposal, cams, &frame, stream), "rt_gris_guide_refresh");
     mPathTracePass.execute(stream, scene, cams, frame, &settings);
     if (learnedTarget)
         rtThrow(rt_gris_learned_refresh(scene.handle(), learned, cams, &frame, stream), "rt_gris_learned_refresh");
     mTemporalReusePass.execute(stream, scene, cams, frame, &settings);
     mSpatialReusePass.execute(stream, scene, cams, frame, &settings);
     if (learnedTarget) rtThrow(rt_mlp_train_frame(learned, nullptr, stream), "rt_mlp_train_frame");
+    if (guided) rtThrow(rt_gris_guide_train(scene.handle(), proposal, cams, &frame, nullptr, stream), "rt_gris_guide_train");
 }
 
 // ---- TestReSTIR: src/TestReSTIR.cpp:9-46 ----
@@ -82,7 +86,8 @@ void TestReSTIR::render(hipStream_t stream, const DeviceScene& scene, const rt_c
     mSpatialReusePass.execute(stream, scene, cams, frame, &settings);
 }
 
-LearnedTarget::LearnedTarget(const DeviceScene& scene, uint32_t seed, hipStream_t stream, uint32_t log2TableSize) {
+LearnedTarget::LearnedTarget(const DeviceScene& scene, uint32_t seed, hipStream_t stream, uint32_t log2TableSize,
+                             bool proposal) {
     rt_scene_info info{};
     rtThrow(rt_device_scene_info(scene.handle(), &info), "rt_device_scene_info");
     rt_mlp_config c{};
@@ -94,7 +99,8 @@ LearnedTarget::LearnedTarget(const DeviceScene& scene, uint32_t seed, hipStream_
         c.boundsLo[a] = info.sceneLo[a] - 0.1f * ext;
         c.boundsHi[a] = info.sceneHi[a] + 0.1f * ext;
     }
-    rtThrow(rt_mlp_create(&c, stream, &mlp), "rt_mlp_create");
+    if (proposal) rtThrow(rt_guide_create(&c, stream, &mlp), "rt_guide_create");
+    else rtThrow(rt_mlp_create(&c, stream, &mlp), "rt_mlp_create");
 }
 LearnedTarget::~LearnedTarget() {
     if (mlp) rt_mlp_destroy(mlp);
@@ -146,6 +152,13 @@ Renderer::~Renderer() {
     (void)hipFree(grisReservoir[0]); (void)hipFree(grisReservoir[1]); (void)hipFree(grisReservoirTemp);
     (void)hipFree(diReservoir[0]); (void)hipFree(diReservoir[1]); (void)hipFree(diReservoirTemp);
     (void)hipFree(giReservoir[0]); (void)hipFree(giReservoir[1]);
+    (void)hipFree(guideLobes[0]); (void)hipFree(guideLobes[1]); (void)hipFree(guideSamples);
+}
+void Renderer::allocateProposal() {
+    if (guideSamples) return;
+    const size_t lobeBytes = rt_guide_lobe_bytes(width, height);
+    for (int j = 0; j < 2; j++) guideLobes[j] = deviceAlloc<uint8_t>(lobeBytes);
+    guideSamples = deviceAlloc<uint8_t>((size_t)width * height * 16);
 }
 void Renderer::screenshot(hipStream_t stream, const rt_post_settings& post, const std::string& path) {
     const size_t bytes = (size_t)width * height * 4;
@@ -187,6 +200,9 @@ rt_frame Renderer::bindings() const {
     f.grisReservoir = grisReservoir[thisFrame];
     f.grisReservoirPrev = grisReservoir[lastFrame];
     f.grisReservoirTemp = grisReservoirTemp;
+    f.guideLobes = guideLobes[thisFrame];
+    f.guideLobesPrev = guideLobes[lastFrame];
+    f.guideSamples = guideSamples;
     return f;
 }
 void Renderer::drawFrame(hipStream_t stream, const rt_camera cams[2], Method method) {

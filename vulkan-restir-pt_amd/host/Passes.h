@@ -53,13 +53,16 @@ public:
 
     void createPipeline();
     // With settings.targetFunction = 1 the frame is: path trace, learned refresh,
-    // temporal reuse, spatial reuse, one training step of the network.
+    // temporal reuse, spatial reuse, one training step of the network. With
+    // settings.guidedProposal = 1 the proposal's refresh precedes the path trace and
+    // its training step follows the spatial reuse (the frame binds its lobe planes).
     void render(hipStream_t stream, const DeviceScene& scene, const rt_camera cams[2], const rt_frame& frame);
 
     // src/GRISReSTIR.h:29 {Hybrid, 1, false, true, 20}; then the kernel constants
     // MaxTracingDepth = 15, ResampleNum = 3, ResampleRadius = 20; luminance target; the reference's 1/M weights
     Settings settings = {Hybrid, 1.0f, 0u, 1u, 20u, 15u, 3u, 20.0f, 0u, 0u};
-    rt_mlp* learned = nullptr;  // borrowed; required when settings.targetFunction = 1
+    rt_mlp* learned = nullptr;   // borrowed; required when settings.targetFunction = 1
+    rt_mlp* proposal = nullptr;  // borrowed (rt_guide_create); required when settings.guidedProposal = 1
 
 private:
     RayTracing mPathTracePass, mTemporalReusePass, mSpatialReusePass;
@@ -83,11 +86,13 @@ private:
     rt_di_reservoirs mBound{};
 };
 
-// Owns the learned target function of a scene: network bounds are the scene
+// Owns a learned network of a scene: the learned target function, or with
+// proposal = true the learned vMF-mixture proposal; network bounds are the scene
 // box padded by 10%.
 class LearnedTarget {
 public:
-    LearnedTarget(const DeviceScene& scene, uint32_t seed, hipStream_t stream, uint32_t log2TableSize = 19);
+    LearnedTarget(const DeviceScene& scene, uint32_t seed, hipStream_t stream, uint32_t log2TableSize = 19,
+                  bool proposal = false);
     ~LearnedTarget();
     LearnedTarget(const LearnedTarget&) = delete;
     LearnedTarget& operator=(const LearnedTarget&) = delete;
@@ -119,6 +124,9 @@ public:
     // PostProcessFrag + Renderer::writeScreenshot (src/Renderer.cpp:758-793): the current outputs
     // post-processed and written to path as a PNG.
     void screenshot(hipStream_t stream, const rt_post_settings& post, const std::string& path);
+    // Allocates the learned proposal's frame buffers (two lobe-plane parities, the training
+    // records), bound by bindings() from then on.
+    void allocateProposal();
 
     GRISReSTIR gris;
     TestReSTIR restirDI;
@@ -130,6 +138,8 @@ public:
     void* diReservoir[2] = {nullptr, nullptr};
     void* diReservoirTemp = nullptr;
     void* giReservoir[2] = {nullptr, nullptr};
+    void* guideLobes[2] = {nullptr, nullptr};
+    void* guideSamples = nullptr;
     uint32_t width, height;
 
 private:

@@ -9,6 +9,7 @@ struct rt_renderer {
     std::unique_ptr<DeviceScene> scene;
     std::unique_ptr<Renderer> r;
     std::unique_ptr<LearnedTarget> learned;
+    std::unique_ptr<LearnedTarget> proposal;
 };
 
 namespace rti {
@@ -215,6 +216,21 @@ rt_status rt_renderer_enable_learned(rt_renderer* r, uint32_t seed, uint32_t log
         r->learned = std::make_unique<LearnedTarget>(*r->scene, seed, stream, log2TableSize);
         r->r->gris.learned = r->learned->mlp;
         if (mlp) *mlp = r->learned->mlp;
+        return RT_OK;
+    } catch (const std::exception& e) {
+        return fail(RT_ERR_HIP, e.what());
+    }
+}
+
+rt_status rt_renderer_enable_proposal(rt_renderer* r, uint32_t seed, uint32_t log2TableSize, hipStream_t stream,
+                                      rt_mlp** mlp) {
+    if (!r || log2TableSize < 10 || log2TableSize > 24) return fail(RT_ERR_INVALID_ARGUMENT, "invalid argument");
+    try {
+        r->r->gris.proposal = nullptr;
+        r->r->allocateProposal();
+        r->proposal = std::make_unique<LearnedTarget>(*r->scene, seed, stream, log2TableSize, true);
+        r->r->gris.proposal = r->proposal->mlp;
+        if (mlp) *mlp = r->proposal->mlp;
         return RT_OK;
     } catch (const std::exception& e) {
         return fail(RT_ERR_HIP, e.what());

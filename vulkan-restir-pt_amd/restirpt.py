@@ -194,7 +194,8 @@ HOST_SYMBOLS = [
     "rt_scenegen", "rt_scene_load", "rt_host_scene_film_size", "rt_host_scene_destroy", "rt_host_scene_arrays", "rt_host_scene_camera", "rt_light_table_build",
     "rt_renderer_create", "rt_renderer_destroy", "rt_renderer_set_settings", "rt_renderer_frame",
     "rt_renderer_bindings", "rt_renderer_flip", "rt_renderer_read_output", "rt_renderer_reset_reservoirs",
-    "rt_renderer_enable_learned", "rt_renderer_frame_methods", "rt_renderer_set_di_settings",
+    "rt_renderer_enable_learned", "rt_renderer_enable_proposal", "rt_renderer_frame_methods",
+    "rt_renderer_set_di_settings",
     "rt_renderer_di_bindings", "rt_write_png", "rt_renderer_screenshot",
 ]
 
@@ -290,6 +291,7 @@ def _declare_host(L):
         "rt_renderer_read_output": [P, u32, P, P],
         "rt_renderer_reset_reservoirs": [P, P],
         "rt_renderer_enable_learned": [P, u32, u32, P, C.POINTER(P)],
+        "rt_renderer_enable_proposal": [P, u32, u32, P, C.POINTER(P)],
         "rt_renderer_frame_methods": [P, C.POINTER(Camera), u32, u32, P],
         "rt_renderer_set_di_settings": [P, C.POINTER(DISettings)],
         "rt_renderer_di_bindings": [P, C.POINTER(DIReservoirs)],
