@@ -423,17 +423,21 @@ rt_status rt_gris_learned_refresh(const rt_device_scene* scene, rt_mlp* mlp, con
  * sample, training targets are 4 floats (unit direction, weight; weight <= 0
  * masks the sample), weights are W1 [32][64], W2 [64][64], W3 [64][32]. */
 rt_status rt_guide_create(const rt_mlp_config* config, hipStream_t stream, rt_mlp** out);
-/* Bytes of one lobe-plane buffer: 16 float4 planes per pixel (256 B/px). */
+/* Bytes of one lobe-plane buffer: one mixture per 2x2 pixel block, 16 float4
+ * planes of ceil(width/2) * ceil(height/2) blocks (256 B per block). */
 size_t rt_guide_lobe_bytes(uint32_t width, uint32_t height);
-/* After the G-buffer pass: evaluates the proposal network at every pixel's
- * primary vertex and writes the mixture into frame->guideLobes (lobe k: plane
- * 2k = (mu, kappa), plane 2k+1 = (pi kappa / (2 pi (1 - e^{-2 kappa})),
- * e^{-2 kappa}, cumulative pi, pi)); background pixels are not written. */
+/* After the G-buffer pass: evaluates the proposal network once per 2x2 pixel
+ * block, at the primary vertex of the block's first non-background pixel in
+ * raster order, and writes the mixture into frame->guideLobes at block index
+ * (y/2) * ceil(width/2) + x/2 (lobe k: plane 2k = (mu, kappa), plane 2k+1 =
+ * (pi kappa / (2 pi (1 - e^{-2 kappa})), e^{-2 kappa}, cumulative pi, pi));
+ * every pixel of the block samples that mixture; all-background blocks are
+ * not written. */
 rt_status rt_gris_guide_refresh(const rt_device_scene* scene, const rt_mlp* guide, const rt_camera cams[2],
                                 const rt_frame* frame, hipStream_t stream);
 /* After the frame's passes: gathers the training batch (primary position,
- * sampled direction, path luminance) from frame->guideSamples at a fixed
- * stride of pixels (offset by the camera seed) and takes one training step;
+ * sampled direction, path luminance) of about 2^14 pixels from
+ * frame->guideSamples at a fixed stride (offset by the camera seed) and takes one training step;
  * loss (device, may be null) receives the batch's mean loss before the step. */
 rt_status rt_gris_guide_train(const rt_device_scene* scene, rt_mlp* guide, const rt_camera cams[2],
                               const rt_frame* frame, float* loss, hipStream_t stream);

@@ -92,7 +92,7 @@ rt_status rt_renderer_enable_learned(rt_renderer* r, uint32_t seed, uint32_t log
                                      rt_mlp** mlp);
 /* Creates the renderer's learned vMF-mixture proposal (rt_guide_create, same
  * network shape and bounds as above), allocates its frame buffers (two lobe
- * planes of 256 B/px, 16 B/px of training records) and binds it to the GRIS
+ * planes of 256 B per 2x2 pixel block, 16 B/px of training records) and binds it to the GRIS
  * passes; frames with settings.guidedProposal = 1 then run the proposal's
  * refresh before the path trace and one training step after spatial reuse.
  * No counterpart in the reference. *mlp (may be null) receives the network,

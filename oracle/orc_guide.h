@@ -2,9 +2,9 @@
 //
 // The learned directional proposal at the primary vertex (no counterpart in the
 // reference), restated for the oracle's path tracer: an 8-lobe von Mises-Fisher
-// mixture per pixel read from lobe planes (the kernels' layout: float4 plane
+// mixture per 2x2 pixel block read from lobe planes (the kernels' layout: float4 plane
 // 2k = (mu_k, kappa_k), plane 2k+1 = (pi_k kappa_k / (2 pi (1 - e^{-2 kappa_k})),
-// e^{-2 kappa_k}, cdf_k, pi_k), plane stride = pixels), sampled with probability
+// e^{-2 kappa_k}, cdf_k, pi_k), plane stride = blocks), sampled with probability
 // 1/2 and otherwise the BSDF, under the one-sample MIS density
 //   p(w) = q(w) / 2 + p_bsdf(w) / 2.
 // Lobe directions follow Jakob's inversion for the vMF distribution on S^2
@@ -24,6 +24,13 @@ struct GuidePix {
 };
 static inline GuidePix noGuide() { return GuidePix{nullptr, 0, 0}; }
 static inline GuidePix guideAt(const float* L, uint32_t n, uint32_t pix) { return GuidePix{L, n, pix}; }
+// One mixture per 2x2 pixel block: plane stride = blocks, pixel (x, y) reads its block's.
+static inline uint32_t guideBlocksX(uint32_t W) { return (W + 1u) >> 1; }
+static inline uint32_t guideBlocks(uint32_t W, uint32_t H) { return guideBlocksX(W) * ((H + 1u) >> 1); }
+static inline GuidePix guidePixel(const float* L, uint32_t W, uint32_t H, uint32_t x, uint32_t y) {
+    if (!L) return noGuide();
+    return guideAt(L, guideBlocks(W, H), (y >> 1) * guideBlocksX(W) + (x >> 1));
+}
 static inline const float* guidePlane(const GuidePix& g, int plane) {
     return g.L + 4 * ((size_t)plane * g.n + g.pix);
 }

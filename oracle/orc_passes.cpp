@@ -244,8 +244,7 @@ vec3 tracePath(const Ctx& c, const Camera& cam, const GBufView& g, const GRISSet
     // StreamSampler (gris_path_trace.glsl:10-33)
     GRISPathSample streamSample; std::memset(&streamSample, 0, sizeof(streamSample));
     float streamWeight = 0.0f, streamSum = 0.0f;
-    const GuidePix gpx = (set.guidedProposal && c.guideLobes) ? guideAt(c.guideLobes, g.W * g.H, index1D(g.W, x, y))
-                                                              : noGuide();
+    const GuidePix gpx = set.guidedProposal ? guidePixel(c.guideLobes, g.W, g.H, x, y) : noGuide();
     vec3 dir0 = V3s(0.0f);
     auto addStream = [&](const GRISPathSample& ps, float weight, float r) {
         streamSum += weight;
@@ -720,8 +719,7 @@ bool temporalReuse(const Ctx& c, const Camera& cam, const Camera& prevCam, const
             if (reservoirIsValid(temporalResv)) {
                 reuseAndMerge(c, set, resv, learned ? &L : nullptr, dstPrimarySurf, dstPrimaryIsec, ray, temporalResv,
                               learnedAt(learned ? learnPrev : nullptr, pidx), learned, resvRng, unbiased,
-                              (set.guidedProposal && c.guideLobes) ? guideAt(c.guideLobes, g.W * g.H, index1D(g.W, x, y))
-                                                                   : noGuide());
+                              set.guidedProposal ? guidePixel(c.guideLobes, g.W, g.H, x, y) : noGuide());
                 mPrev = temporalResv.sampleCount;
                 vec2 puv = uv + motion;
                 prevX = (int)(puv.x * (float)cam.filmSize[0]);
@@ -733,8 +731,8 @@ bool temporalReuse(const Ctx& c, const Camera& cam, const Camera& prevCam, const
         if (mPrev > 0.0f && reservoirIsValid(resv) && pathSampleIsValid(resv.pathSample)) {
             float z = mCanon;
             SurfaceInfo ps{}; Intersection pi{}; Ray pr{};
-            const GuidePix prevGp = (set.guidedProposal && c.guideLobesPrev)
-                                        ? guideAt(c.guideLobesPrev, g.W * g.H, (uint32_t)prevY * g.W + (uint32_t)prevX)
+            const GuidePix prevGp = set.guidedProposal
+                                        ? guidePixel(c.guideLobesPrev, g.W, g.H, (uint32_t)prevX, (uint32_t)prevY)
                                         : noGuide();
             if (primaryOf(prevCam, gPrev, prevX, prevY, ps, pi, pr) &&
                 shiftSupports(c, set, resv.pathSample, L, learned, ps, pi, pr, prevGp))
@@ -791,7 +789,7 @@ vec3 spatialReuse(const Ctx& c, const Camera& cam, const GBufView& g, const GRIS
     const bool unbiased = set.unbiasedReuse != 0u;
     const float mCanon = resv.sampleCount;
     const bool guided = set.guidedProposal && c.guideLobes;
-    const GuidePix gpx = guided ? guideAt(c.guideLobes, g.W * g.H, index1D(g.W, x, y)) : noGuide();
+    const GuidePix gpx = guided ? guidePixel(c.guideLobes, g.W, g.H, x, y) : noGuide();
     std::vector<std::pair<int, int>> nbPix;
     std::vector<float> nbM;
     if (set.spatialReuse) {
@@ -821,8 +819,8 @@ vec3 spatialReuse(const Ctx& c, const Camera& cam, const GBufView& g, const GRIS
             for (size_t j = 0; j < nbPix.size(); j++) {
                 if (!(nbM[j] > 0.0f)) continue;
                 SurfaceInfo ns{}; Intersection ni{}; Ray nr{};
-                const GuidePix ngp = guided ? guideAt(c.guideLobes, g.W * g.H,
-                                                      (uint32_t)nbPix[j].second * g.W + (uint32_t)nbPix[j].first)
+                const GuidePix ngp = guided ? guidePixel(c.guideLobes, g.W, g.H, (uint32_t)nbPix[j].first,
+                                                         (uint32_t)nbPix[j].second)
                                             : noGuide();
                 if (primaryOf(cam, g, nbPix[j].first, nbPix[j].second, ns, ni, nr) &&
                     shiftSupports(c, set, resv.pathSample, L, learned, ns, ni, nr, ngp))

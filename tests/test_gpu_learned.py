@@ -53,7 +53,8 @@ def test_learned_gris_parity(config, W, H, unbiased, guided):
     else:
         run = gpuhelp.GrisRun(ds, W, H)
     n = W * H
-    o_lobes = [np.zeros((16, n, 4), np.float32), np.zeros((16, n, 4), np.float32)]
+    nb = ((W + 1) // 2) * ((H + 1) // 2)  # one mixture per 2x2 pixel block
+    o_lobes = [np.zeros((16, nb, 4), np.float32), np.zeros((16, nb, 4), np.float32)]
     o_samples = np.zeros((n, 4), np.float32)
     o_resv = [orc.reset_reservoirs(n), orc.reset_reservoirs(n)]
     o_learn = [np.zeros((n, 4), np.float32), np.zeros((n, 4), np.float32)]

@@ -523,7 +523,9 @@ rt_status rt_gris_learned_refresh(const rt_device_scene* s, rt_mlp* mlp, const r
     return RT_OK;
 }
 
-size_t rt_guide_lobe_bytes(uint32_t w, uint32_t h) { return (size_t)w * h * rtg::kPlanes * sizeof(float4); }
+size_t rt_guide_lobe_bytes(uint32_t w, uint32_t h) {
+    return (size_t)((w + 1u) >> 1) * ((h + 1u) >> 1) * rtg::kPlanes * sizeof(float4);
+}
 
 rt_status rt_gris_guide_refresh(const rt_device_scene* s, const rt_mlp* guide, const rt_camera cams[2], const rt_frame* f,
                                 hipStream_t stream) {

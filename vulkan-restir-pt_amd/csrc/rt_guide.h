@@ -8,7 +8,9 @@
 // Every function here is deterministic FP32 (own exp / log / sin / cos), so a
 // path sampled on the GPU is reproduced bit for bit wherever it is replayed.
 //
-// Lobe planes (float4, plane stride = pixels): plane 2k = (mu_k, kappa_k),
+// The mixture is evaluated once per 2x2 pixel block (at the block's first
+// non-background pixel in raster order) and shared by the block's pixels.
+// Lobe planes (float4, plane stride = blocks): plane 2k = (mu_k, kappa_k),
 // plane 2k + 1 = (pi_k kappa_k / (2 pi (1 - e^{-2 kappa_k})), e^{-2 kappa_k},
 // cdf_k = pi_0 + ... + pi_k (cdf_7 = 1), pi_k).
 #pragma once
@@ -29,6 +31,13 @@ struct GuidePix {
 };
 RT_DEV GuidePix noGuide() { GuidePix g; g.L = nullptr; g.n = 0; g.pix = 0; return g; }
 RT_DEV GuidePix guideAt(const float4* L, uint32_t n, uint32_t pix) { GuidePix g; g.L = L; g.n = n; g.pix = pix; return g; }
+RT_DEV uint32_t guideBlocksX(uint32_t W) { return (W + 1u) >> 1; }
+RT_DEV uint32_t guideBlocks(uint32_t W, uint32_t H) { return guideBlocksX(W) * ((H + 1u) >> 1); }
+// The mixture of pixel (x, y) of a W x H frame (null planes: no proposal).
+RT_DEV GuidePix guidePixel(const float4* L, uint32_t W, uint32_t H, uint32_t x, uint32_t y) {
+    if (!L) return noGuide();
+    return guideAt(L, guideBlocks(W, H), (y >> 1) * guideBlocksX(W) + (x >> 1));
+}
 // The frame's proposal bindings: lobe planes of this and the previous frame, and the
 // per-pixel training record (primary direction, path luminance) the path tracer writes.
 struct GuideFrame {

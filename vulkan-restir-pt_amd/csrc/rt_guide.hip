@@ -1,6 +1,6 @@
 // The learned directional proposal's per-frame kernels: the proposal network
-// evaluated at every pixel's primary vertex into the frame's lobe planes
-// (matrix-core forward of rt_mlp.h, 32 pixels per wave), and the gather of the
+// evaluated once per 2x2 pixel block into the frame's lobe planes
+// (matrix-core forward of rt_mlp.h, 32 blocks per wave), and the gather of the
 // frame's training batch from the path tracer's per-pixel records.
 #include "rt_dpixel.h"
 #include "rt_guide.h"
@@ -13,6 +13,7 @@ using rti::launch;
 namespace {
 
 constexpr int kRefreshWaves = 4;
+constexpr uint32_t kGuideBatch = 1u << 14;  // training samples gathered per frame
 
 // Primary vertex of pixel p as the pass kernels rebuild it; false for background.
 __device__ __forceinline__ bool primaryPos(const rt_camera& cam, const GB& g, uint32_t p, f3& pos) {
@@ -25,16 +26,24 @@ __device__ __forceinline__ bool primaryPos(const rt_camera& cam, const GB& g, ui
     return true;
 }
 
+// One sample per 2x2 pixel block b: the block's first non-background pixel in raster order.
 __global__ __launch_bounds__(64 * kRefreshWaves) void k_gris_guide_refresh(MlpView M, rt_camera cam, GB g,
                                                                           float4* __restrict__ lobes) {
     __shared__ _Float16 sW[kGuideWeights];
     for (int i = threadIdx.x; i < kGuideWeights; i += blockDim.x) sW[i] = M.w[i];
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t n = g.W * g.H;
+    const uint32_t bw = rtg::guideBlocksX(g.W), n = rtg::guideBlocks(g.W, g.H);
     const uint32_t p = (blockIdx.x * kRefreshWaves + wave) * kTileSamples + (lane & 31u);
     f3 pos = s3(0.0f);
-    const bool valid = p < n && primaryPos(cam, g, p, pos);
+    bool valid = false;
+    if (p < n) {
+        const uint32_t bx = 2u * (p % bw), by = 2u * (p / bw);
+        for (uint32_t c = 0; c < 4u && !valid; c++) {
+            const uint32_t x = bx + (c & 1u), y = by + (c >> 1);
+            if (x < g.W && y < g.H) valid = primaryPos(cam, g, y * g.W + x, pos);
+        }
+    }
     float u = 0.0f, v = 0.0f, w = 0.0f;
     normalizePos(M, pos.x, pos.y, pos.z, u, v, w);
     h4 enc[4];
@@ -104,7 +113,7 @@ namespace rti {
 
 hipError_t launchGuideRefresh(const rt_mlp* m, const rt_camera* cams, const rt_frame& f, hipStream_t stream) {
     const GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
-    const uint32_t n = f.width * f.height;
+    const uint32_t n = ((f.width + 1u) >> 1) * ((f.height + 1u) >> 1);
     const uint32_t tiles = (n + kTileSamples - 1) / kTileSamples;
     (void)launch(k_gris_guide_refresh, dim3((tiles + kRefreshWaves - 1) / kRefreshWaves), dim3(64 * kRefreshWaves),
                  stream, m->view, cams[0], g, reinterpret_cast<float4*>(f.guideLobes));
@@ -114,7 +123,7 @@ hipError_t launchGuideRefresh(const rt_mlp* m, const rt_camera* cams, const rt_f
 hipError_t launchGuideGather(rt_mlp* m, const rt_camera* cams, const rt_frame& f, hipStream_t stream) {
     const GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
     const uint32_t n = f.width * f.height;
-    const uint32_t stride = std::max(1u, (n + kBatchCap - 1) / kBatchCap);
+    const uint32_t stride = std::max(1u, (n + kGuideBatch - 1) / kGuideBatch);
     const uint32_t offset = cams[0].seed % stride;
     m->batchSize = offset < n ? (n - offset + stride - 1) / stride : 0u;
     if (m->batchSize)

@@ -624,7 +624,7 @@ RT_DEV void ptRun(const DevScene& S, const rt_camera& cam, const GB& g, const rt
             Surf surf; Mat mat;
             if (x >= g.W || y >= g.H || !ptBegin(S, cam, g, x, y, P, surf, mat)) continue;
             pix = y * g.W + x;
-            const rtg::GuidePix gpx = gf.lobes ? rtg::guideAt(gf.lobes, g.W * g.H, pix) : rtg::noGuide();
+            const rtg::GuidePix gpx = rtg::guidePixel(gf.lobes, g.W, g.H, x, y);
             live = ptVertex<kCount>(S, st, slot, set, P, surf, mat, 0u, ctr, gpx) && ++P.bounce < set.maxDepth;
             if (!live) {
                 storeResv(outR, pix, ptFinish(P, slot));
@@ -944,7 +944,7 @@ RT_DEV void temporalPixel(const DevScene& S, const rt_camera& cam, const rt_came
                         if (resvIsValid(tr)) {
                             reuseAndMerge<kCount, kLearned, kUnbiased>(
                                 S, st, set, resv, dstPrimarySurf, dstPrimaryIsec, ray.dir, ray.ori, tr, resvRng, ctr,
-                                gf.lobes ? rtg::guideAt(gf.lobes, g.W * g.H, (uint32_t)p) : rtg::noGuide());
+                                rtg::guidePixel(gf.lobes, g.W, g.H, x, y));
                             mPrev = tr.sampleCount; prevX = px; prevY = py;
                         }
                     }
@@ -956,9 +956,7 @@ RT_DEV void temporalPixel(const DevScene& S, const rt_camera& cam, const rt_came
         if (mPrev > 0.0f && resvIsValid(resv) && resv.rcIsec.inst != RT_INVALID_HIT) {
             float z = mCanon;
             Surf ps; Isec pi; CamRay pr;
-            const rtg::GuidePix prevGp = gf.lobesPrev
-                                             ? rtg::guideAt(gf.lobesPrev, g.W * g.H, (uint32_t)prevY * g.W + (uint32_t)prevX)
-                                             : rtg::noGuide();
+            const rtg::GuidePix prevGp = rtg::guidePixel(gf.lobesPrev, g.W, g.H, (uint32_t)prevX, (uint32_t)prevY);
             if (primaryOf(prevCam, gPrev, prevX, prevY, ps, pi, pr) &&
                 shiftSupports<kCount, kLearned>(S, st, set, resv, ps, pi, pr.dir, pr.ori, ctr, prevGp))
                 z += mPrev;
@@ -988,7 +986,7 @@ RT_DEV void spatialPixel(const DevScene& S, const rt_camera& cam, const GB& g, c
                          TStack st, TraceCounters* ctr, const rtg::GuideFrame& gf) {
     if (x >= g.W || y >= g.H) return;
     size_t p = (size_t)y * g.W + x;
-    const rtg::GuidePix gpx = gf.lobes ? rtg::guideAt(gf.lobes, g.W * g.H, (uint32_t)p) : rtg::noGuide();
+    const rtg::GuidePix gpx = rtg::guidePixel(gf.lobes, g.W, g.H, x, y);
     f2 uv = pixelUV(x, y, g.W, g.H);
     GPix gp;
     f3 radiance = s3(0.0f);
@@ -1040,7 +1038,7 @@ RT_DEV void spatialPixel(const DevScene& S, const rt_camera& cam, const GB& g, c
                     if (!(nbM[j] > 0.0f)) continue;
                     Surf ns; Isec ni; CamRay nr;
                     const uint32_t nx = nbPix[j] >> 16, ny = nbPix[j] & 0xffffu;
-                    const rtg::GuidePix ngp = gf.lobes ? rtg::guideAt(gf.lobes, g.W * g.H, ny * g.W + nx) : rtg::noGuide();
+                    const rtg::GuidePix ngp = rtg::guidePixel(gf.lobes, g.W, g.H, nx, ny);
                     if (primaryOf(cam, g, (int)nx, (int)ny, ns, ni, nr) &&
                         shiftSupports<kCount, kLearned>(S, st, set, resv, ns, ni, nr.dir, nr.ori, ctr, ngp))
                         z += nbM[j];
