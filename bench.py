@@ -561,7 +561,7 @@ def batched_views(rt, args, hs1, ds1):
     return {"views": len(specs), "pixels_per_frame": pixels, "scene_triangles": tris,
             "ms_per_frame": round(batched_ms, 3), "mpaths_per_s": round(pixels / batched_ms / 1e3, 2),
             "view_by_view_ms_per_frame": round(single_ms, 3),
-            "tuning": {"persistent_groups": int(best.persistentGroups), "xcd_swizzle": int(best.xcdSwizzle),
+            "tuning": {"persistent_groups": int(best.persistentGroups), "xcd_chunk_tiles": int(best.xcdSwizzle),
                        "tuned_ms": round(float(tune_ms.value), 3)},
             "workload": "batched plain ReSTIR PT (hybrid shift, temporal + spatial, 3 neighbours in 20 px, "
                         "M-cap 20), square views log-uniform in [128, 1024] px, depth U{3..8}, scenes "
@@ -571,11 +571,12 @@ def batched_views(rt, args, hs1, ds1):
 def full_form_4k(rt, args):
     """The full-form workload at 3840x2160 on the ~4.8M-triangle scene (1024x1024-cell fBm heightfield
     + 2000 instanced icospheres, 64 emissive triangles, GGX roughness U[0.02, 1]), depth 8, hybrid
-    shift, temporal + spatial: per frame the 4K view with the learned target (refresh after path
-    tracing, one training step after spatial reuse) and the learned vMF-mixture proposal (refresh
-    after the G-buffer, one training step after spatial reuse), unbiased reuse, plus 16 extra 512x512
-    views of the same scene in one batched launch sequence (luminance target and BSDF sampling: the
-    batched passes take no network)."""
+    shift, temporal + spatial: per frame one batched frame (rt_batch_frame) of the 4K view with the
+    learned target (refresh after path tracing, one training step after spatial reuse), the learned
+    vMF-mixture proposal (refresh after the G-buffer, one training step after spatial reuse) and
+    unbiased reuse, alongside 16 extra 512x512 views of the same scene (luminance target, BSDF
+    sampling): one persistent path-trace launch over all 17 views, G-buffer and reuse launches per
+    form."""
     import numpy as np
     import torch
 
@@ -597,41 +598,46 @@ def full_form_4k(rt, args):
     offsets = np.random.default_rng(4).integers(0, 64, n_extra)
     L = rt.core()
     stream = fr.stream
+    n_views = 1 + n_extra
 
     def views_for(k):
-        vs = (rt.View * n_extra)()
-        for i in range(n_extra):
-            cams = hs.camera(int(offsets[i] + k) % 64, size, size)
+        vs = (rt.View * n_views)()
+        for i in range(n_views):
             v = vs[i]
             v.scene = ds.handle.value
+            if i == 0:
+                cams = hs.camera(k % 64, W, H)
+                v.frame, v.settings = fr.frame_struct(), s4k
+                v.target, v.guide = mlp.handle.value, guide.handle.value
+            else:
+                cams = hs.camera(int(offsets[i - 1] + k) % 64, size, size)
+                v.frame, v.settings = extra[i - 1].frame_struct(), s_extra
             C.memmove(C.addressof(v.cams), C.addressof(cams), C.sizeof(rt.Camera) * 2)
-            v.frame = extra[i].frame_struct()
-            v.settings = s_extra
         return vs
 
     b = C.c_void_p()
-    rt.check(L.rt_batch_create(n_extra, C.byref(b)), "batch")
+    rt.check(L.rt_batch_create(n_views, C.byref(b)), "batch")
     try:
+        # tuned on these views (the tuning frames also train the 4K view's networks: warm-up)
         best, tune_ms = rt.BatchTuning(), C.c_float()
-        rt.check(L.rt_batch_autotune(b, views_for(0), n_extra, 2, stream, C.byref(best), C.byref(tune_ms)), "tune")
-        for e in extra:
+        rt.check(L.rt_batch_autotune(b, views_for(0), n_views, 2, stream, C.byref(best), C.byref(tune_ms)), "tune")
+        for e in [fr] + extra:
             e.reset()
 
         def frame(k, ev=None):
-            fr.frame(hs.camera(k % 64, W, H), s4k, mlp, ev, guide)
             if ev is not None:
-                ev[2].record()
-            rt.check(L.rt_batch_frame(b, views_for(k), n_extra, stream), "batch frame")
+                ev[0].record()
+            rt.check(L.rt_batch_frame(b, views_for(k), n_views, stream), "batch frame")
             if ev is not None:
-                ev[3].record()
-            for e in extra:
+                ev[1].record()
+            for e in [fr] + extra:
                 e.cur ^= 1
 
         for k in range(2):
             frame(k)
         torch.cuda.synchronize()
         steps = 8
-        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(steps)]
         t0 = time.perf_counter()
         for k in range(steps):
             frame(2 + k, ev[k])
@@ -640,17 +646,16 @@ def full_form_4k(rt, args):
     finally:
         L.rt_batch_destroy(b)
     pixels = W * H + n_extra * size * size
-    view_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / steps  # G-buffer .. both training steps
-    batch_ms = sum(e[2].elapsed_time(e[3]) for e in ev) / steps
+    batch_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / steps
     return {"ms_per_frame": round(wall * 1e3 / steps, 3), "mpaths_per_s": round(pixels * steps / wall / 1e6, 2),
-            "main_view_learned_guided_ms": round(view_ms, 3), "extra_views_batched_ms": round(batch_ms, 3),
+            "batched_frame_gpu_ms": round(batch_ms, 3),
             "pixels_per_frame": pixels, "triangles": int(info.numPrimitives), "emissive": int(hs.arrays.numLights),
-            "tuning": {"persistent_groups": int(best.persistentGroups), "xcd_swizzle": int(best.xcdSwizzle)},
-            "workload": "3840x2160 view with the learned target (hash grid 16x2x2^19 + MLP 2x64) and the learned "
-                        "8-lobe vMF-mixture proposal (second network, 32 outputs), both trained each frame, unbiased "
-                        "reuse; + 16 x 512x512 views batched (luminance target, BSDF sampling), ~4.8M-triangle "
-                        "scene, 64 emissive, depth 8, hybrid shift, temporal + spatial (3 neighbours, 30 px, "
-                        "M-cap 20)"}
+            "tuning": {"persistent_groups": int(best.persistentGroups), "xcd_chunk_tiles": int(best.xcdSwizzle)},
+            "workload": "one rt_batch_frame of 17 views: a 3840x2160 view with the learned target (hash grid "
+                        "16x2x2^19 + MLP 2x64) and the learned 8-lobe vMF-mixture proposal (second network, 32 "
+                        "outputs), both trained each frame, unbiased reuse; + 16 x 512x512 views (luminance "
+                        "target, BSDF sampling); ~4.8M-triangle scene, 64 emissive, depth 8, hybrid shift, "
+                        "temporal + spatial (3 neighbours, 30 px, M-cap 20)"}
 
 
 def restir_di(rt, hs, ds, args, kernel_loop=False):

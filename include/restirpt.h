@@ -459,22 +459,29 @@ rt_status rt_frame_graph_destroy(rt_frame_graph* graph);
 /* ---- Batched views --------------------------------------------------------
  * Many independent views (each its own scene, cameras, frame bindings and
  * settings) rendered together, the batched form of Renderer::drawFrame
- * (src/Renderer.cpp:539-569) for many views: per frame one G-buffer launch
- * over all views' 8x8 tiles, one persistent GRIS path-trace launch whose
- * waves drain the views' pixel queues in turn, and one temporal and one
- * spatial launch over all tiles. Luminance target only. The caller flips each
- * view's frame parity between frames; calls of one batch are ordered on one
- * stream. */
+ * (src/Renderer.cpp:539-569) for many views: one persistent GRIS path-trace
+ * launch whose waves drain all views' pixel queues in turn, and per form of
+ * the views (target function, resampling weights, learned proposal) one
+ * G-buffer, one temporal and one spatial launch over those views' 8x8 tiles.
+ * A view with the learned target or proposal brings its own network (one
+ * view per network); the batch runs that network's steps where a single
+ * view's frame runs them (proposal refresh before the path trace, target
+ * refresh after it, one training step each after spatial reuse), so every
+ * view renders exactly as it would alone. The caller flips each view's frame
+ * parity between frames; calls of one batch are ordered on one stream. */
 typedef struct rt_view {
     const rt_device_scene* scene;
     rt_camera cams[2];            /* uCamera, uPrevCamera */
     rt_frame frame;               /* the bindings of the parity being rendered */
-    rt_gris_settings settings;    /* targetFunction must be 0 */
+    rt_gris_settings settings;
+    rt_mlp* target;               /* radiance network when settings.targetFunction = 1, else ignored */
+    rt_mlp* guide;                /* proposal network when settings.guidedProposal = 1, else ignored */
 } rt_view;
 
 typedef struct rt_batch_tuning {
     uint32_t persistentGroups;    /* workgroups of the persistent path trace (0: CUs x resident) */
-    uint32_t xcdSwizzle;          /* per-pixel passes take XCD-contiguous tile ranges */
+    uint32_t xcdSwizzle;          /* per-pixel passes: each XCD takes runs of this many consecutive
+                                     8x8 tiles, every 8th run (0: dispatch order; at most 4096) */
 } rt_batch_tuning;
 
 typedef struct rt_batch rt_batch;
@@ -484,10 +491,11 @@ rt_status rt_batch_frame(rt_batch* batch, const rt_view* views, uint32_t numView
 rt_status rt_batch_set_tuning(rt_batch* batch, const rt_batch_tuning* tuning);
 rt_status rt_batch_get_tuning(const rt_batch* batch, rt_batch_tuning* tuning);
 /* Autotuner: times each candidate tuning (persistent workgroups at 1/2, 3/4
- * and 1 x the resident limit, with and without the XCD swizzle) over `frames`
+ * and 1 x the resident limit; XCD chunks of 0, 16 and 128 tiles) over `frames`
  * frames of these views (after one warm-up frame each), keeps the fastest and
  * returns it with its mean frame time. The views' reservoirs are left in an
- * arbitrary state: reset them before rendering for real. */
+ * arbitrary state (and their networks trained on those frames): reset them
+ * before rendering for real. */
 rt_status rt_batch_autotune(rt_batch* batch, const rt_view* views, uint32_t numViews, uint32_t frames,
                             hipStream_t stream, rt_batch_tuning* best, float* msPerFrame);
 

@@ -41,8 +41,8 @@ def test_struct_sizes_match_reference_layouts():
     # 20-B push constant + 3 kernel constants + target function + resampling-weight mode + proposal
     assert C.sizeof(rt.GrisSettings) == 44
     assert C.sizeof(rt.MlpConfig) == 72
-    # rt_view: scene pointer + 2 cameras + rt_frame (112 B) + settings (44 B)
-    assert C.sizeof(rt.Frame) == 112 and C.sizeof(rt.View) == 8 + 2 * 352 + 112 + 44 + 4  # tail padding to 8
+    # rt_view: scene pointer + 2 cameras + rt_frame (112 B) + settings (44 B) + 4 B padding + 2 networks
+    assert C.sizeof(rt.Frame) == 112 and C.sizeof(rt.View) == 8 + 2 * 352 + 112 + 44 + 4 + 2 * 8
     assert C.sizeof(rt.BatchTuning) == 8
     assert C.sizeof(rt.DIReservoir) == 64      # layouts.glsl DIReservoir (std430)
     assert C.sizeof(rt.DISettings) == 16       # TestReSTIR::Settings push constant

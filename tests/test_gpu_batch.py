@@ -68,6 +68,106 @@ def test_batched_views_match_single_views():
         ms = C.c_float()
         rt.check(L.rt_batch_autotune(b, views, len(VIEWS), 2, gpuhelp.stream(), C.byref(best), C.byref(ms)), "tune")
         print(f"[gpu] autotune: groups {best.persistentGroups} swizzle {best.xcdSwizzle} {ms.value:.3f} ms", flush=True)
-        assert best.persistentGroups > 0 and best.xcdSwizzle in (0, 1) and 0 < ms.value < 1e4
+        assert best.persistentGroups > 0 and best.xcdSwizzle in (0, 16, 128) and 0 < ms.value < 1e4
+    finally:
+        L.rt_batch_destroy(b)
+
+
+def _handle(h):
+    return h.value if hasattr(h, "value") else h
+
+
+# (scene config, width, height, max depth, learned target, unbiased reuse, learned proposal)
+FORM_VIEWS = [(1, 96, 64, 6, True, True, True), (0, 40, 24, 3, False, False, False), (1, 64, 48, 5, True, False, False),
+              (0, 48, 40, 4, False, True, True), (1, 33, 17, 6, False, False, True)]
+
+
+def test_batched_learned_forms_match_single_views():
+    """Views of every form in one batch (learned target, unbiased reuse and learned proposal in
+    several combinations, next to a plain view), each with its own networks: G-buffers,
+    reservoirs with their learned planes, lobe planes, training records, outputs and the
+    networks' parameters after three frames equal those of the same views rendered alone."""
+    from test_gpu_guide import GuidedRun
+    from test_gpu_learned import scene_bounds
+
+    hs = {c: rt.HostScene(c, 1) for c in (0, 1)}
+    ds = {c: rt.DeviceScene(h.arrays) for c, h in hs.items()}
+    bounds = {c: scene_bounds(d) for c, d in ds.items()}
+
+    def nets(i, c, learned, guided):
+        lo, hi = bounds[c]
+        t = rt.Mlp(rt.mlp_config(lo, hi, log2_table=15, seed=10 + i)) if learned else None
+        g = rt.Mlp(rt.mlp_config(lo, hi, log2_table=15, seed=40 + i), guide=True) if guided else None
+        return t, g
+
+    single = [GuidedRun(ds[v[0]], v[1], v[2]) for v in FORM_VIEWS]
+    batched = [GuidedRun(ds[v[0]], v[1], v[2]) for v in FORM_VIEWS]
+    s_nets = [nets(i, v[0], v[4], v[6]) for i, v in enumerate(FORM_VIEWS)]
+    b_nets = [nets(i, v[0], v[4], v[6]) for i, v in enumerate(FORM_VIEWS)]
+    sets = [rt.settings(shift=2, temporal=True, spatial=True, max_depth=d, cap=20, radius=10.0, learned=lt,
+                        unbiased=ub, guided=gd) for _, _, _, d, lt, ub, gd in FORM_VIEWS]
+    b = C.c_void_p()
+    L = rt.core()
+    rt.check(L.rt_batch_create(len(FORM_VIEWS), C.byref(b)), "batch create")
+    tuning = rt.BatchTuning()
+    tuning.persistentGroups, tuning.xcdSwizzle = 0, 3  # runs of 3 tiles per XCD: partial last runs
+    rt.check(L.rt_batch_set_tuning(b, C.byref(tuning)), "tuning")
+    try:
+        for frame in range(3):
+            print(f"[gpu] batched forms frame {frame}", flush=True)
+            views = (rt.View * len(FORM_VIEWS))()
+            for i, (c, W, H, _, lt, ub, gd) in enumerate(FORM_VIEWS):
+                cams = hs[c].camera(frame, W, H)
+                r, (t, g) = single[i], s_nets[i]
+                r.gbuffer(cams[0])
+                if gd:
+                    r.guide_refresh(g, cams)
+                r.path_trace(cams, sets[i])
+                if lt:
+                    r.learned_refresh(t, cams)
+                r.temporal(cams, sets[i])
+                r.spatial(cams, sets[i])
+                if lt:
+                    rt.check(L.rt_mlp_train_frame(t.handle, None, gpuhelp.stream()), "train frame")
+                if gd:
+                    r.guide_train(g, cams)
+                v = make_view(batched[i], ds[c], cams, sets[i])
+                bt, bg = b_nets[i]
+                v.target = _handle(bt.handle) if bt else None
+                v.guide = _handle(bg.handle) if bg else None
+                views[i] = v
+            rt.check(L.rt_batch_frame(b, views, len(FORM_VIEWS), gpuhelp.stream()), "batch frame")
+            torch.cuda.synchronize()
+            for i, (c, W, H, _, lt, ub, gd) in enumerate(FORM_VIEWS):
+                a, z = single[i], batched[i]
+                for ga, gz in zip(a.gbuffer_host(), z.gbuffer_host()):
+                    assert np.array_equal(ga.view(np.uint8), gz.view(np.uint8)), f"view {i}: G-buffer differs"
+                assert not gpuhelp.compare_reservoirs(a.reservoirs(a.resv[a.cur]), z.reservoirs(z.resv[z.cur])).any()
+                assert not gpuhelp.compare_reservoirs(a.reservoirs(a.temp), z.reservoirs(z.temp)).any()
+                if lt:
+                    assert np.array_equal(a.learned(a.resv[a.cur]).view(np.uint32),
+                                          z.learned(z.resv[z.cur]).view(np.uint32)), f"view {i}: learned planes"
+                if gd:
+                    assert np.array_equal(a.lobes_host(a.cur).view(np.uint32),
+                                          z.lobes_host(z.cur).view(np.uint32)), f"view {i}: lobe planes"
+                    assert np.array_equal(a.samples_host().view(np.uint32),
+                                          z.samples_host().view(np.uint32)), f"view {i}: training records"
+                assert np.array_equal(a.output(1).view(np.uint32), z.output(1).view(np.uint32)), f"view {i}: output"
+                for na, nz in zip(s_nets[i], b_nets[i]):
+                    if na is not None:
+                        for pa, pz in zip(na.params(), nz.params()):
+                            assert np.array_equal(pa.view(np.uint32), pz.view(np.uint32)), f"view {i}: network"
+                a.flip()
+                z.flip()
+        # a network bound to two views of one frame is rejected before anything runs
+        views = (rt.View * 2)()
+        for k in range(2):
+            c, W, H = FORM_VIEWS[0][:3]
+            v = make_view(batched[0], ds[c], hs[c].camera(3, W, H), sets[0])
+            v.target = _handle(b_nets[0][0].handle)
+            v.guide = _handle(b_nets[0][1].handle)
+            views[k] = v
+        assert L.rt_batch_frame(b, views, 2, gpuhelp.stream()) != 0
+        assert b"more than one view" in rt.core().rt_last_error()
     finally:
         L.rt_batch_destroy(b)

@@ -1100,8 +1100,9 @@ struct ViewDesc {
     uint2* amOut;
     uint32_t* mvOut;
     rt_gris_settings set;
-    uint32_t tileBase, tilesX, tiles, pad;
+    uint32_t tileBase, tilesX, tiles, pad;  // tileBase: first tile of the view in its launch
     WorkQueue* queue;
+    rtg::GuideFrame gf;  // the learned proposal's bindings (null unless the view samples with it)
 };
 
 // View of a linear tile index: the last view whose tileBase <= tile.
@@ -1115,16 +1116,17 @@ RT_DEV uint32_t viewOfTile(const ViewDesc* __restrict__ V, uint32_t numViews, ui
     return lo;
 }
 // Workgroup -> tile. Consecutive workgroups are dealt round robin to the 8
-// XCDs; with the swizzle, XCD k walks the k-th contiguous eighth of the tile
-// list, so the rays of neighbouring tiles share that XCD's L2.
-RT_DEV uint32_t batchTile(uint32_t b, uint32_t totalTiles, uint32_t swz) {
-    if (!swz) return b;
-    const uint32_t per = (totalTiles + 7u) / 8u;
-    return (b % 8u) * per + b / 8u;
+// XCDs; with a chunk of C tiles, XCD k takes runs of C consecutive tiles
+// (chunks k, k + 8, k + 16, ...), so neighbouring tiles share that XCD's L2
+// while every XCD still gets work from all over the views. C = 0: dispatch order.
+RT_DEV uint32_t batchTile(uint32_t b, uint32_t totalTiles, uint32_t chunk) {
+    if (!chunk) return b;
+    const uint32_t xcd = b & 7u, i = b >> 3;
+    return ((i / chunk) * 8u + xcd) * chunk + i % chunk;
 }
 RT_DEV bool batchPixel(const ViewDesc* __restrict__ V, uint32_t numViews, uint32_t totalTiles, uint32_t swz,
                        uint32_t& v, uint32_t& x, uint32_t& y) {
-    const uint32_t tile = batchTile(blockIdx.x, totalTiles, swz);
+    const uint32_t tile = batchTile(blockIdx.x, totalTiles, swz);  // swz: tiles per XCD chunk
     if (tile >= totalTiles) return false;
     v = __builtin_amdgcn_readfirstlane(viewOfTile(V, numViews, tile));
     const uint32_t t = tile - V[v].tileBase, tx = V[v].tilesX;
@@ -1146,8 +1148,9 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_GBUFFER) void k_gbuffer_batc
 
 // All views' GRIS path tracing in one persistent launch: workgroup b starts at
 // view b % numViews and drains the views' pixel queues in turn, so waves move
-// to the views that still have pixels.
-template <bool kCount>
+// to the views that still have pixels. kGuided: some views sample with the
+// learned proposal (a view whose gf is null samples with its BSDFs).
+template <bool kCount, bool kGuided>
 __global__ __launch_bounds__(kTileThreads, RT_WAVES_PATH_TRACE) void k_gris_path_trace_batch(const ViewDesc* __restrict__ V,
                                                                                             uint32_t numViews,
                                                                                             TraceCounters* ctr) {
@@ -1157,12 +1160,14 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_PATH_TRACE) void k_gris_path
         const uint32_t v = __builtin_amdgcn_readfirstlane((blockIdx.x + i) % numViews);
         const ViewDesc& D = V[v];
         ptRun<kCount>(D.S, D.cam, D.g, D.set, D.cur, D.queue, TStack{stk + threadIdx.x, kTileThreads},
-                      picks + threadIdx.x, ctr, rtg::GuideFrame{nullptr, nullptr, nullptr});
+                      picks + threadIdx.x, ctr, kGuided ? D.gf : rtg::GuideFrame{nullptr, nullptr, nullptr});
         queueDone(D.queue);
     }
 }
 
-template <bool kCount>
+// The reuse passes of one launch run one form (target function, resampling
+// weights, proposal) for all of its views.
+template <bool kCount, bool kLearned, bool kUnbiased, bool kGuided>
 __global__ __launch_bounds__(kTileThreads, RT_WAVES_TEMPORAL) void k_gris_temporal_batch(const ViewDesc* __restrict__ V,
                                                                                         uint32_t numViews,
                                                                                         uint32_t totalTiles, uint32_t swz,
@@ -1171,11 +1176,12 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_TEMPORAL) void k_gris_tempor
     uint32_t v, x, y;
     if (!batchPixel(V, numViews, totalTiles, swz, v, x, y)) return;
     const ViewDesc& D = V[v];
-    temporalPixel<kCount, false>(D.S, D.cam, D.prevCam, D.g, D.gPrev, D.set, D.cur, D.prev, D.temp, x, y,
-                                 TStack{stk + threadIdx.x, kTileThreads}, ctr, rtg::GuideFrame{nullptr, nullptr, nullptr});
+    temporalPixel<kCount, kLearned, kUnbiased>(D.S, D.cam, D.prevCam, D.g, D.gPrev, D.set, D.cur, D.prev, D.temp, x, y,
+                                               TStack{stk + threadIdx.x, kTileThreads}, ctr,
+                                               kGuided ? D.gf : rtg::GuideFrame{nullptr, nullptr, nullptr});
 }
 
-template <bool kCount>
+template <bool kCount, bool kLearned, bool kUnbiased, bool kGuided>
 __global__ __launch_bounds__(kTileThreads, RT_WAVES_SPATIAL) void k_gris_spatial_batch(const ViewDesc* __restrict__ V,
                                                                                       uint32_t numViews, uint32_t totalTiles,
                                                                                       uint32_t swz, TraceCounters* ctr) {
@@ -1183,8 +1189,9 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_SPATIAL) void k_gris_spatial
     uint32_t v, x, y;
     if (!batchPixel(V, numViews, totalTiles, swz, v, x, y)) return;
     const ViewDesc& D = V[v];
-    spatialPixel<kCount, false>(D.S, D.cam, D.g, D.set, D.temp, D.cur, D.indirectOut, x, y,
-                                TStack{stk + threadIdx.x, kTileThreads}, ctr, rtg::GuideFrame{nullptr, nullptr, nullptr});
+    spatialPixel<kCount, kLearned, kUnbiased>(D.S, D.cam, D.g, D.set, D.temp, D.cur, D.indirectOut, x, y,
+                                              TStack{stk + threadIdx.x, kTileThreads}, ctr,
+                                              kGuided ? D.gf : rtg::GuideFrame{nullptr, nullptr, nullptr});
 }
 
 __global__ void k_resv_reset(ResvPlanes R) {
@@ -1476,33 +1483,72 @@ void batchFillDesc(void* dst, const rt_device_scene* scene, const rt_camera* cam
     d.tilesX = (f.width + kTileX - 1) / kTileX;
     d.tiles = batchViewTiles(f);
     d.queue = queue;
+    d.gf = guideFrame(f, set);
     std::memcpy(dst, &d, sizeof(d));
 }
 
-// Resident workgroups of the batched persistent path trace per CU, and the CU count.
-hipError_t batchOccupancy(uint32_t& perCU, uint32_t& cus) {
-    int dev = 0, c = 0, o = 0;
+// Resident workgroups of the batched persistent path trace per CU (the variant
+// without and with the learned proposal), and the CU count.
+hipError_t batchOccupancy(uint32_t& perCU, uint32_t& perCUGuided, uint32_t& cus) {
+    int dev = 0, c = 0, o = 0, og = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev);
     if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, reinterpret_cast<const void*>(k_gris_path_trace_batch<false>),
-                                                         kTileThreads, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &o, reinterpret_cast<const void*>(k_gris_path_trace_batch<false, false>), kTileThreads, 0);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &og, reinterpret_cast<const void*>(k_gris_path_trace_batch<false, true>), kTileThreads, 0);
     perCU = (uint32_t)std::max(1, o);
+    perCUGuided = (uint32_t)std::max(1, og);
     cus = (uint32_t)std::max(1, c);
     return e;
 }
 
-// One frame of all views: G-buffer, path trace (persistent, across views),
-// temporal, spatial. descs = device array of numViews descriptors.
-hipError_t launchBatchFrame(const void* descs, uint32_t numViews, uint32_t totalTiles, uint32_t persistentGroups,
-                            uint32_t xcdSwizzle, hipStream_t stream) {
+// The passes of one frame over device arrays of view descriptors. The
+// G-buffer and reuse launches cover the numViews descriptors at descs, whose
+// tile bases count from 0 over totalTiles tiles; the persistent path trace
+// covers every view of the frame.
+// Grid of a tile launch: with chunks of C tiles, every tile below totalTiles is
+// reached when the grid is a multiple of 8 C (the caller bounds C and totalTiles).
+static dim3 batchTileGrid(uint32_t totalTiles, uint32_t chunk) {
+    if (!chunk) return dim3(totalTiles);
+    const uint64_t span = 8ull * chunk;
+    return dim3((uint32_t)((totalTiles + span - 1) / span * span));
+}
+
+hipError_t launchBatchGBuffer(const void* descs, uint32_t numViews, uint32_t totalTiles, uint32_t xcdSwizzle,
+                              hipStream_t stream) {
     const ViewDesc* V = reinterpret_cast<const ViewDesc*>(descs);
-    const uint32_t swz = xcdSwizzle ? 1u : 0u;
-    const dim3 tileGridDim(swz ? 8u * ((totalTiles + 7u) / 8u) : totalTiles);
-    RT_DISPATCH(k_gbuffer_batch, tileGridDim, kTileThreads, stream, V, numViews, totalTiles, swz);
-    RT_DISPATCH(k_gris_path_trace_batch, dim3(std::max(1u, persistentGroups)), kTileThreads, stream, V, numViews);
-    RT_DISPATCH(k_gris_temporal_batch, tileGridDim, kTileThreads, stream, V, numViews, totalTiles, swz);
-    RT_DISPATCH(k_gris_spatial_batch, tileGridDim, kTileThreads, stream, V, numViews, totalTiles, swz);
+    const uint32_t swz = xcdSwizzle;
+    RT_DISPATCH(k_gbuffer_batch, batchTileGrid(totalTiles, swz), kTileThreads, stream, V, numViews, totalTiles, swz);
+    return hipGetLastError();
+}
+
+hipError_t launchBatchPathTrace(const void* descs, uint32_t numViews, uint32_t persistentGroups, bool guided,
+                                hipStream_t stream) {
+    const ViewDesc* V = reinterpret_cast<const ViewDesc*>(descs);
+    const dim3 grid(std::max(1u, persistentGroups));
+    TraceCounters* ctr = gCountersEnabled ? counterBuffer() : nullptr;
+    if (gCountersEnabled) {
+        if (guided) (void)launch(k_gris_path_trace_batch<true, true>, grid, kTileThreads, stream, V, numViews, ctr);
+        else (void)launch(k_gris_path_trace_batch<true, false>, grid, kTileThreads, stream, V, numViews, ctr);
+    } else {
+        if (guided) (void)launch(k_gris_path_trace_batch<false, true>, grid, kTileThreads, stream, V, numViews, ctr);
+        else (void)launch(k_gris_path_trace_batch<false, false>, grid, kTileThreads, stream, V, numViews, ctr);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchBatchReuse(const void* descs, uint32_t numViews, uint32_t totalTiles, uint32_t xcdSwizzle,
+                            bool learned, bool unbiased, bool guided, hipStream_t stream) {
+    const ViewDesc* V = reinterpret_cast<const ViewDesc*>(descs);
+    const uint32_t swz = xcdSwizzle;
+    const dim3 grid = batchTileGrid(totalTiles, swz);
+    RT_DISPATCH_LUG(k_gris_temporal_batch, learned, unbiased, guided, grid, kTileThreads, stream, V, numViews,
+                    totalTiles, swz);
+    RT_DISPATCH_LUG(k_gris_spatial_batch, learned, unbiased, guided, grid, kTileThreads, stream, V, numViews,
+                    totalTiles, swz);
     return hipGetLastError();
 }
 

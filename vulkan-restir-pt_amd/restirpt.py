@@ -135,7 +135,8 @@ class Frame(C.Structure):
 
 
 class View(C.Structure):
-    _fields_ = [("scene", C.c_void_p), ("cams", Camera * 2), ("frame", Frame), ("settings", GrisSettings)]
+    _fields_ = [("scene", C.c_void_p), ("cams", Camera * 2), ("frame", Frame), ("settings", GrisSettings),
+                ("target", C.c_void_p), ("guide", C.c_void_p)]
 
 
 class BatchTuning(C.Structure):
