@@ -45,6 +45,14 @@ namespace {
 #ifndef RT_WAVES_SPATIAL
 #define RT_WAVES_SPATIAL 5
 #endif
+// The reuse passes with the unbiased weights (a shift check per merged pixel
+// on top of the merges) hold more state; their variants have their own count.
+#ifndef RT_WAVES_TEMPORAL_UB
+#define RT_WAVES_TEMPORAL_UB RT_WAVES_TEMPORAL
+#endif
+#ifndef RT_WAVES_SPATIAL_UB
+#define RT_WAVES_SPATIAL_UB RT_WAVES_SPATIAL
+#endif
 
 // ================================================================================================
 // Ray query test entry
@@ -968,7 +976,7 @@ RT_DEV void temporalPixel(const DevScene& S, const rt_camera& cam, const rt_came
     storeResv<kLearned>(tempR, p, resv);
 }
 template <bool kCount, bool kLearned, bool kUnbiased, bool kGuided>
-__global__ __launch_bounds__(kTileThreads, RT_WAVES_TEMPORAL) void k_gris_temporal(DevScene S, rt_camera cam, rt_camera prevCam, GB g,
+__global__ __launch_bounds__(kTileThreads, kUnbiased ? RT_WAVES_TEMPORAL_UB : RT_WAVES_TEMPORAL) void k_gris_temporal(DevScene S, rt_camera cam, rt_camera prevCam, GB g,
                                                                 GB gPrev, rt_gris_settings set, ResvPlanes thisR,
                                                                 ResvPlanes prevR, ResvPlanes tempR, rtg::GuideFrame gf,
                                                                 TraceCounters* ctr) {
@@ -1076,7 +1084,7 @@ RT_DEV void spatialPixel(const DevScene& S, const rt_camera& cam, const GB& g, c
     accumulate(indirectOut, p, radiance, cam.frameIndex);
 }
 template <bool kCount, bool kLearned, bool kUnbiased, bool kGuided>
-__global__ __launch_bounds__(kTileThreads, RT_WAVES_SPATIAL) void k_gris_spatial(DevScene S, rt_camera cam, GB g, rt_gris_settings set,
+__global__ __launch_bounds__(kTileThreads, kUnbiased ? RT_WAVES_SPATIAL_UB : RT_WAVES_SPATIAL) void k_gris_spatial(DevScene S, rt_camera cam, GB g, rt_gris_settings set,
                                                                ResvPlanes tempR, ResvPlanes thisR, float* indirectOut,
                                                                rtg::GuideFrame gf, TraceCounters* ctr) {
     __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
@@ -1168,7 +1176,7 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_PATH_TRACE) void k_gris_path
 // The reuse passes of one launch run one form (target function, resampling
 // weights, proposal) for all of its views.
 template <bool kCount, bool kLearned, bool kUnbiased, bool kGuided>
-__global__ __launch_bounds__(kTileThreads, RT_WAVES_TEMPORAL) void k_gris_temporal_batch(const ViewDesc* __restrict__ V,
+__global__ __launch_bounds__(kTileThreads, kUnbiased ? RT_WAVES_TEMPORAL_UB : RT_WAVES_TEMPORAL) void k_gris_temporal_batch(const ViewDesc* __restrict__ V,
                                                                                         uint32_t numViews,
                                                                                         uint32_t totalTiles, uint32_t swz,
                                                                                         TraceCounters* ctr) {
@@ -1182,7 +1190,7 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_TEMPORAL) void k_gris_tempor
 }
 
 template <bool kCount, bool kLearned, bool kUnbiased, bool kGuided>
-__global__ __launch_bounds__(kTileThreads, RT_WAVES_SPATIAL) void k_gris_spatial_batch(const ViewDesc* __restrict__ V,
+__global__ __launch_bounds__(kTileThreads, kUnbiased ? RT_WAVES_SPATIAL_UB : RT_WAVES_SPATIAL) void k_gris_spatial_batch(const ViewDesc* __restrict__ V,
                                                                                       uint32_t numViews, uint32_t totalTiles,
                                                                                       uint32_t swz, TraceCounters* ctr) {
     __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
