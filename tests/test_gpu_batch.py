@@ -110,6 +110,8 @@ def test_batched_learned_forms_match_single_views():
     L = rt.core()
     rt.check(L.rt_batch_create(len(FORM_VIEWS), C.byref(b)), "batch create")
     tuning = rt.BatchTuning()
+    tuning.persistentGroups, tuning.xcdSwizzle = 0, 5000
+    assert L.rt_batch_set_tuning(b, C.byref(tuning)) != 0  # more than 4096 tiles per XCD chunk
     tuning.persistentGroups, tuning.xcdSwizzle = 0, 3  # runs of 3 tiles per XCD: partial last runs
     rt.check(L.rt_batch_set_tuning(b, C.byref(tuning)), "tuning")
     try:
