@@ -64,6 +64,7 @@ def parse():
                     help="time learned-target frames only (for profilers)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-view leg")
     ap.add_argument("--no-4k", action="store_true", help="skip the 4K full-form leg")
+    ap.add_argument("--only-4k", action="store_true", help="run the 4K full-form leg alone (for profilers)")
     ap.add_argument("--guided-kernel-loop", action="store_true",
                     help="time frames with the learned proposal only (for profilers)")
     ap.add_argument("--batched-views", type=int, default=64)
@@ -827,6 +828,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.only_4k:
+        print(json.dumps(full_form_4k(rt, args)))
         return
     W, H = args.width, args.height
     hs = rt.HostScene(1, args.seed)
