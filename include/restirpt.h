@@ -445,14 +445,18 @@ rt_status rt_gris_guide_train(const rt_device_scene* scene, rt_mlp* guide, const
 /* ---- Frame graphs -------------------------------------------------------
  * The per-frame command recording of Renderer::recordRenderCommand
  * (src/Renderer.cpp:400-503, one command buffer per frame) as HIP graphs: one
- * GRIS frame of a view (G-buffer, path trace, [learned refresh], temporal,
- * spatial, [one training step]) per frame parity, built on the first launch
+ * GRIS frame of a view (G-buffer, [proposal refresh], path trace, [learned
+ * refresh], temporal, spatial, [training steps]) per frame parity, built on the first launch
  * of that parity; later launches update the kernel arguments (cameras, Adam
  * step) and replay. frames[j] are the bindings of parity j (same size); the
- * graph owns its pixel queue. mlp is required when settings->targetFunction = 1. */
+ * graph owns its pixel queue. mlp (the radiance network) is required when
+ * settings->targetFunction = 1; guide (the proposal network) when
+ * settings->guidedProposal = 1, whose refresh after the G-buffer and training
+ * step (gather + step, as rt_gris_guide_train) the graph records too. */
 typedef struct rt_frame_graph rt_frame_graph;
 rt_status rt_frame_graph_create(const rt_device_scene* scene, const rt_frame frames[2],
-                                const rt_gris_settings* settings, rt_mlp* mlp, rt_frame_graph** out);
+                                const rt_gris_settings* settings, rt_mlp* mlp, rt_mlp* guide,
+                                rt_frame_graph** out);
 rt_status rt_frame_graph_launch(rt_frame_graph* graph, uint32_t parity, const rt_camera cams[2], hipStream_t stream);
 rt_status rt_frame_graph_destroy(rt_frame_graph* graph);
 

@@ -34,21 +34,27 @@ def scene_bounds(ds):
     return tuple(lo - 0.1 * ext), tuple(hi + 0.1 * ext)
 
 
-@pytest.mark.parametrize("learned", [False, True], ids=["luminance", "learned"])
-def test_frame_graph_matches_eager(learned):
-    print(f"[gpu] frame graph learned={learned}", flush=True)
+@pytest.mark.parametrize("learned,unbiased,guided", [(False, False, False), (True, False, False), (True, True, True)],
+                         ids=["luminance", "learned", "learned-unbiased-proposal"])
+def test_frame_graph_matches_eager(learned, unbiased, guided):
+    """With guided: the proposal's refresh, gather and training step are graph nodes too."""
+    from test_gpu_guide import GuidedRun
+
+    print(f"[gpu] frame graph learned={learned} unbiased={unbiased} proposal={guided}", flush=True)
     hs = rt.HostScene(0, 2)
     ds = rt.DeviceScene(hs.arrays)
     W, H = 96, 64
-    s = rt.settings(shift=2, temporal=True, spatial=True, max_depth=6, cap=20, radius=20.0, learned=learned)
-    eager, graphed = gpuhelp.GrisRun(ds, W, H), gpuhelp.GrisRun(ds, W, H)
-    mlps = [None, None]
-    if learned:
-        lo, hi = scene_bounds(ds)
-        mlps = [rt.Mlp(rt.mlp_config(lo, hi, log2_table=16, seed=4)) for _ in range(2)]
+    s = rt.settings(shift=2, temporal=True, spatial=True, max_depth=6, cap=20, radius=20.0, learned=learned,
+                    unbiased=unbiased, guided=guided)
+    eager, graphed = GuidedRun(ds, W, H), GuidedRun(ds, W, H)
+    lo, hi = scene_bounds(ds)
+    mlps = [rt.Mlp(rt.mlp_config(lo, hi, log2_table=16, seed=4)) for _ in range(2)] if learned else [None, None]
+    guides = ([rt.Mlp(rt.mlp_config(lo, hi, log2_table=16, seed=6), guide=True) for _ in range(2)] if guided
+              else [None, None])
     g = C.c_void_p()
     rt.check(rt.core().rt_frame_graph_create(ds.handle, frames_of(graphed), C.byref(s),
-                                             mlps[1].handle if learned else None, C.byref(g)), "graph create")
+                                             mlps[1].handle if learned else None,
+                                             guides[1].handle if guided else None, C.byref(g)), "graph create")
     L = rt.core()
     try:
         for frame in range(6):
@@ -56,6 +62,8 @@ def test_frame_graph_matches_eager(learned):
             cams = hs.camera(frame, W, H)
             # eager
             eager.gbuffer(cams[0])
+            if guided:
+                eager.guide_refresh(guides[0], cams)
             eager.path_trace(cams, s)
             if learned:
                 eager.learned_refresh(mlps[0], cams)
@@ -63,6 +71,8 @@ def test_frame_graph_matches_eager(learned):
             eager.spatial(cams, s)
             if learned:
                 rt.check(L.rt_mlp_train_frame(mlps[0].handle, None, gpuhelp.stream()), "train")
+            if guided:
+                eager.guide_train(guides[0], cams)
             # the program's own work between replays
             junk = torch.randn(1 << 20, device="cuda").sum()
             torch.cuda.synchronize()
@@ -78,9 +88,15 @@ def test_frame_graph_matches_eager(learned):
                     assert np.array_equal(eager.learned(a).view(np.uint32), graphed.learned(b).view(np.uint32))
             oa, ob = eager.output(1), graphed.output(1)
             assert np.array_equal(oa.view(np.uint32), ob.view(np.uint32)), f"frame {frame}: outputs differ"
-            if learned:
-                pa, pb = mlps[0].params(), mlps[1].params()
-                assert np.array_equal(pa[0], pb[0]) and np.array_equal(pa[1], pb[1]), "network parameters differ"
+            for nets in (mlps, guides):
+                if nets[0] is not None:
+                    pa, pb = nets[0].params(), nets[1].params()
+                    assert np.array_equal(pa[0], pb[0]) and np.array_equal(pa[1], pb[1]), "network parameters differ"
+            if guided:
+                assert np.array_equal(eager.lobes_host(eager.cur).view(np.uint32),
+                                      graphed.lobes_host(graphed.cur).view(np.uint32)), f"frame {frame}: lobes"
+                assert np.array_equal(eager.samples_host().view(np.uint32),
+                                      graphed.samples_host().view(np.uint32)), f"frame {frame}: records"
             eager.flip()
             graphed.flip()
     finally:

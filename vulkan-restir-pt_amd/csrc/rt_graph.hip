@@ -21,6 +21,8 @@ hipError_t launchGrisSpatial(const rt_device_scene*, const rt_camera*, const rt_
 hipError_t launchLearnedRefresh(const rt_device_scene*, rt_mlp*, const rt_camera*, const rt_frame&, hipStream_t);
 hipError_t mlpGradients(rt_mlp* m, const float* pos, const float* target, uint32_t n, float* loss, hipStream_t st);
 hipError_t mlpAdam(rt_mlp* m, hipStream_t st);
+hipError_t launchGuideRefresh(const rt_mlp* m, const rt_camera* cams, const rt_frame& f, hipStream_t stream);
+hipError_t launchGuideGather(rt_mlp* m, const rt_camera* cams, const rt_frame& f, hipStream_t stream);
 }  // namespace rti
 
 struct rt_frame_graph {
@@ -28,6 +30,7 @@ struct rt_frame_graph {
     rt_frame frames[2]{};
     rt_gris_settings settings{};
     rt_mlp* mlp = nullptr;
+    rt_mlp* guide = nullptr;          // the learned proposal's network (settings.guidedProposal)
     rtd::WorkQueue* queue = nullptr;  // this graph's own pixel queue
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -45,9 +48,11 @@ hipError_t recordFrame(rt_frame_graph* g, uint32_t parity, const rt_camera* cams
     rti::tlsRecorder = &out;
     const rt_frame& f = g->frames[parity];
     const bool learned = g->settings.targetFunction == 1u;
+    const bool guided = g->settings.guidedProposal == 1u;
     hipError_t e = rti::launchGBuffer(g->scene, cams[0], const_cast<float*>(f.depthNormal),
                                       const_cast<uint32_t*>(f.albedoMatId), const_cast<uint16_t*>(f.motionVector),
                                       nullptr);
+    if (e == hipSuccess && guided) e = rti::launchGuideRefresh(g->guide, cams, f, nullptr);
     if (e == hipSuccess) e = rti::launchGrisPathTrace(g->scene, cams, f, g->settings, nullptr, g->queue);
     if (e == hipSuccess && learned) e = rti::launchLearnedRefresh(g->scene, g->mlp, cams, f, nullptr);
     if (e == hipSuccess) e = rti::launchGrisTemporal(g->scene, cams, f, g->settings, nullptr);
@@ -55,6 +60,14 @@ hipError_t recordFrame(rt_frame_graph* g, uint32_t parity, const rt_camera* cams
     if (e == hipSuccess && learned) {
         e = rti::mlpGradients(g->mlp, g->mlp->batchPos, g->mlp->batchTarget, g->mlp->batchSize, nullptr, nullptr);
         if (e == hipSuccess) e = rti::mlpAdam(g->mlp, nullptr);
+    }
+    if (e == hipSuccess && guided) {  // rt_gris_guide_train: gather, then one step when the batch is not empty
+        e = rti::launchGuideGather(g->guide, cams, f, nullptr);
+        if (e == hipSuccess && g->guide->batchSize) {
+            e = rti::mlpGradients(g->guide, g->guide->batchPos, g->guide->batchTarget, g->guide->batchSize, nullptr,
+                                  nullptr);
+            if (e == hipSuccess) e = rti::mlpAdam(g->guide, nullptr);
+        }
     }
     rti::tlsRecorder = nullptr;
     return e;
@@ -102,15 +115,26 @@ hipError_t buildGraph(rt_frame_graph* g, uint32_t j, std::vector<rti::KernelLaun
 extern "C" {
 
 rt_status rt_frame_graph_create(const rt_device_scene* scene, const rt_frame frames[2], const rt_gris_settings* settings,
-                                rt_mlp* mlp, rt_frame_graph** out) {
+                                rt_mlp* mlp, rt_mlp* guide, rt_frame_graph** out) {
     if (!out) return invalid("out is null");
     *out = nullptr;
     if (!scene || !frames || !settings) return invalid("null argument");
     if (settings->maxDepth == 0 || settings->maxDepth > 255 || settings->shiftType > 2 ||
-        settings->spatialNeighbors > 64 || settings->targetFunction > 1)
+        settings->spatialNeighbors > 64 || settings->targetFunction > 1 || settings->unbiasedReuse > 1 ||
+        settings->guidedProposal > 1)
         return invalid("invalid settings");
-    if (settings->targetFunction == 1u && !mlp) return invalid("the learned target needs a network");
-    if (settings->guidedProposal != 0u) return invalid("frame graphs do not record the learned proposal");
+    if (settings->unbiasedReuse && settings->spatialNeighbors > 8)
+        return invalid("unbiasedReuse supports at most 8 spatialNeighbors");
+    if (settings->targetFunction == 1u && (!mlp || mlp->outputs != (uint32_t)rtm::kOut))
+        return invalid("the learned target needs its radiance network (rt_mlp_create)");
+    if (settings->guidedProposal == 1u) {
+        if (!guide || guide->outputs != (uint32_t)rtm::kGuideOut)
+            return invalid("the learned proposal needs its proposal network (rt_guide_create)");
+        if (mlp && mlp == guide) return invalid("the target and proposal networks must differ");
+        for (int j = 0; j < 2; j++)
+            if (!frames[j].guideLobes || !frames[j].guideLobesPrev || !frames[j].guideSamples)
+                return invalid("guidedProposal needs guideLobes, guideLobesPrev and guideSamples");
+    }
     for (int j = 0; j < 2; j++) {
         const rt_frame& f = frames[j];
         if (f.width == 0 || f.height == 0 || f.width >= 32768 || f.height >= 32768 || f.width != frames[0].width ||
@@ -126,6 +150,7 @@ rt_status rt_frame_graph_create(const rt_device_scene* scene, const rt_frame fra
     g->frames[1] = frames[1];
     g->settings = *settings;
     g->mlp = mlp;
+    g->guide = settings->guidedProposal == 1u ? guide : nullptr;
     if (!rti::hipCheck(hipMalloc(&g->queue, sizeof(rtd::WorkQueue)), "hipMalloc") ||
         !rti::hipCheck(hipMemset(g->queue, 0, sizeof(rtd::WorkQueue)), "hipMemset")) {
         (void)hipFree(g->queue);

@@ -243,7 +243,7 @@ def _declare_core(L):
         "rt_guide_create": [C.POINTER(MlpConfig), P, C.POINTER(P)],
         "rt_gris_guide_refresh": [P, P, C.POINTER(Camera), C.POINTER(Frame), P],
         "rt_gris_guide_train": [P, P, C.POINTER(Camera), C.POINTER(Frame), P, P],
-        "rt_frame_graph_create": [P, C.POINTER(Frame), C.POINTER(GrisSettings), P, C.POINTER(P)],
+        "rt_frame_graph_create": [P, C.POINTER(Frame), C.POINTER(GrisSettings), P, P, C.POINTER(P)],
         "rt_frame_graph_launch": [P, u32, C.POINTER(Camera), P],
         "rt_frame_graph_destroy": [P],
         "rt_batch_create": [u32, C.POINTER(P)],
