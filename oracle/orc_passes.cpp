@@ -414,8 +414,13 @@ vec3 tracePath(const Ctx& c, const Camera& cam, const GBufView& g, const GRISSet
 // gris_retrace.glsl:42-136
 struct RcData { Intersection rcPrevIsec; vec3 rcPrevWo; vec3 rcPrevThroughput; };
 
+// canonical (unbiased reuse, no reference counterpart): the replayed prefix must
+// classify the reconnection vertex as the destination's path tracer would --
+// its sampleState over the replayed vertices is exactly 1 at the predecessor,
+// i.e. no earlier pair of consecutive connectible vertices.
 static void traceReplayPathForHybridShift(const Ctx& c, const GRISSettings& set, Intersection isec, SurfaceInfo surf,
-                                          Ray ray, uint32_t targetFlags, uint32_t rng, RcData& rc, const GuidePix& gpx) {
+                                          Ray ray, uint32_t targetFlags, uint32_t rng, RcData& rc, const GuidePix& gpx,
+                                          bool canonical = false) {
     const SceneArrays& S = *c.S;
     vec3 throughput = V3s(1.0f);
     vec3 wo = -ray.dir;
@@ -430,6 +435,7 @@ static void traceReplayPathForHybridShift(const Ctx& c, const GRISSettings& set,
         rc.rcPrevIsec = isec; rc.rcPrevWo = wo; rc.rcPrevThroughput = throughput;
         return;
     }
+    uint32_t sampleState = 0;
     for (uint32_t bounce = 0; bounce < set.maxDepth; bounce++) {
         if (bounce > 0) {
             isec = traceClosestHit(c, ray.ori, MinRayDistance, ray.dir, MaxRayDistance);
@@ -438,10 +444,11 @@ static void traceReplayPathForHybridShift(const Ctx& c, const GRISSettings& set,
             mat = S.materials[surf.matIndex];
         }
         bool isThisVertexConnectible = isBSDFConnectible(mat);
+        sampleState = (sampleState == 2) ? 2u : (!isThisVertexConnectible ? 0u : sampleState + 1u);
         (void)sample1f(rng);
         if (surf.isLight) break;
         if (bounce == targetId - 1u) {
-            if (isThisVertexConnectible) {
+            if (isThisVertexConnectible && (!canonical || sampleState == 1u)) {
                 rc.rcPrevIsec = isec; rc.rcPrevWo = wo; rc.rcPrevThroughput = throughput;
             }
             break;
@@ -520,7 +527,7 @@ static void reuseAndMerge(const Ctx& c, const GRISSettings& set, GRISReservoir& 
 
     if (pathSampleIsValid(srcSample)) {
         traceReplayPathForHybridShift(c, set, dstPrimaryIsec, dstPrimarySurf, primaryRay, srcSample.flags,
-                                      srcSample.primaryRng, dstRc, gpx);
+                                      srcSample.primaryRng, dstRc, gpx, unbiased);
         if (dstRc.rcPrevIsec.instanceIdx != InvalidHitIndex) {
             if (dstRc.rcPrevIsec.instanceIdx == SpecialHitIndex) rcPrevSurf = dstPrimarySurf;
             else loadSurfaceInfo(S, dstRc.rcPrevIsec, rcPrevSurf);
@@ -593,7 +600,7 @@ static bool shiftSupports(const Ctx& c, const GRISSettings& set, const GRISPathS
     const SceneArrays& S = *c.S;
     if (!pathSampleIsValid(y)) return false;
     RcData rc;
-    traceReplayPathForHybridShift(c, set, primIsec, primSurf, primRay, y.flags, y.primaryRng, rc, gpx);
+    traceReplayPathForHybridShift(c, set, primIsec, primSurf, primRay, y.flags, y.primaryRng, rc, gpx, true);
     if (rc.rcPrevIsec.instanceIdx == InvalidHitIndex) return false;
     SurfaceInfo rcPrevSurf{}, rcSurf{};
     if (rc.rcPrevIsec.instanceIdx == SpecialHitIndex) rcPrevSurf = primSurf;

@@ -683,7 +683,13 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_PATH_TRACE) void k_gris_path
 // ---- gris_retrace.glsl:42-136 --------------------------------------------------------------------
 struct RcData { Isec rcPrevIsec; f3 rcPrevWo; f3 rcPrevThroughput; };
 
-template <bool kCount>
+// kCanon (unbiased reuse): the shifted path must also be one the destination's
+// path tracer classifies the same way -- its reconnection vertex is the first
+// vertex whose predecessor is connectible after a run of one connectible vertex
+// (the path tracer's sampleState machine) -- so the replayed prefix may not
+// hold an earlier pair of consecutive connectible vertices. Without this the
+// hybrid shift is not a bijection between the domains.
+template <bool kCount, bool kCanon = false>
 RT_DEV void replayHybrid(const DevScene& S, TStack st, const rt_gris_settings& set, Isec isec, Surf surf, f3 rdir, f3 rori,
                          uint32_t targetFlags, uint32_t rng, RcData& rc, TraceCounters* ctr, const rtg::GuidePix& gpx) {
     f3 throughput = s3(1.0f);
@@ -696,6 +702,7 @@ RT_DEV void replayHybrid(const DevScene& S, TStack st, const rt_gris_settings& s
     rc.rcPrevIsec.inst = RT_INVALID_HIT;
     uint32_t targetId = flagsRcVertexId(targetFlags);
     if (targetId == 1) { rc.rcPrevIsec = isec; rc.rcPrevWo = wo; rc.rcPrevThroughput = throughput; return; }
+    uint32_t state = 0;  // the path tracer's sampleState over the replayed vertices
     for (uint32_t bounce = 0; bounce < set.maxDepth; bounce++) {
         if (bounce > 0) {
             isec = traceClosestHit<kCount>(S, st, rori, RT_MIN_RAY, rdir, RT_MAX_RAY, ctr);
@@ -704,10 +711,13 @@ RT_DEV void replayHybrid(const DevScene& S, TStack st, const rt_gris_settings& s
             mat = loadMat(S, surf.matIndex);
         }
         bool conn = isBSDFConnectible(mat);
+        if (kCanon) state = state == 2u ? 2u : (conn ? state + 1u : 0u);
         (void)sample1f(rng);
         if (surf.isLight) break;
         if (bounce == targetId - 1u) {
-            if (conn) { rc.rcPrevIsec = isec; rc.rcPrevWo = wo; rc.rcPrevThroughput = throughput; }
+            if (conn && (!kCanon || state == 1u)) {
+                rc.rcPrevIsec = isec; rc.rcPrevWo = wo; rc.rcPrevThroughput = throughput;
+            }
             break;
         }
         (void)sample4f(rng);
@@ -787,8 +797,8 @@ RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& 
     bool srcSampleValid = false;
     float dstJacobian = 0.0f, jacobian = 0.0f, dstPHat = 0.0f, dstSamplePdf = 0.0f;
     if (src.rcIsec.inst != RT_INVALID_HIT) {
-        replayHybrid<kCount>(S, st, set, dstPrimaryIsec, dstPrimarySurf, primDir, primOri, src.flags, src.primaryRng, rc, ctr,
-                             gpx);
+        replayHybrid<kCount, kUnbiased>(S, st, set, dstPrimaryIsec, dstPrimarySurf, primDir, primOri, src.flags,
+                                        src.primaryRng, rc, ctr, gpx);
         if (rc.rcPrevIsec.inst != RT_INVALID_HIT) {
             if (rc.rcPrevIsec.inst == RT_SPECIAL_HIT) rcPrevSurf = dstPrimarySurf;
             else loadSurfaceInfo(S, rc.rcPrevIsec, rcPrevSurf);
@@ -855,7 +865,7 @@ RT_DEV bool shiftSupports(const DevScene& S, TStack st, const rt_gris_settings& 
                           const Isec& primIsec, f3 primDir, f3 primOri, TraceCounters* ctr, const rtg::GuidePix& gpx) {
     if (y.rcIsec.inst == RT_INVALID_HIT) return false;
     RcData rc;
-    replayHybrid<kCount>(S, st, set, primIsec, primSurf, primDir, primOri, y.flags, y.primaryRng, rc, ctr, gpx);
+    replayHybrid<kCount, true>(S, st, set, primIsec, primSurf, primDir, primOri, y.flags, y.primaryRng, rc, ctr, gpx);
     if (rc.rcPrevIsec.inst == RT_INVALID_HIT) return false;
     Surf rcPrevSurf, rcSurf;
     if (rc.rcPrevIsec.inst == RT_SPECIAL_HIT) rcPrevSurf = primSurf;
