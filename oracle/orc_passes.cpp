@@ -539,7 +539,11 @@ static void reuseAndMerge(const Ctx& c, const GRISSettings& set, GRISReservoir& 
             float cosTheta = -dot(rcSurf.norm, wi);
             dstJacobian = std::fabs(cosTheta) / square(dist);
             jacobian = dstJacobian / srcSample.rcJacobian;
-            if (dist > GRISDistanceThreshold && cosTheta > 0.0f && !std::isnan(jacobian) && srcSample.rcJacobian > 0.0f &&
+            // unbiased reuse: a surface rc vertex may face away from its predecessor, as generated
+            const bool facing = (unbiased && flagsRcVertexType(srcSample.flags) == RcVertexTypeSurface)
+                                    ? cosTheta != 0.0f
+                                    : cosTheta > 0.0f;
+            if (dist > GRISDistanceThreshold && facing && !std::isnan(jacobian) && srcSample.rcJacobian > 0.0f &&
                 isBSDFConnectible(rcPrevMat)) {
                 if (traceVisibility(c, rcPrevSurf.pos, rcSurf.pos)) srcSampleValid = true;
             }
@@ -612,7 +616,8 @@ static bool shiftSupports(const Ctx& c, const GRISSettings& set, const GRISPathS
     vec3 wi = normalize(rcSurf.pos - rcPrevSurf.pos);
     float cosTheta = -dot(rcSurf.norm, wi);
     float jac = std::fabs(cosTheta) / square(dist);
-    if (!(dist > GRISDistanceThreshold && cosTheta > 0.0f && jac > 0.0f && !std::isnan(jac) && isBSDFConnectible(rcPrevMat)))
+    const bool facing = flagsRcVertexType(y.flags) == RcVertexTypeSurface ? cosTheta != 0.0f : cosTheta > 0.0f;
+    if (!(dist > GRISDistanceThreshold && facing && jac > 0.0f && !std::isnan(jac) && isBSDFConnectible(rcPrevMat)))
         return false;
     if (!traceVisibility(c, rcPrevSurf.pos, rcSurf.pos)) return false;
     GRISPathSample x = y;

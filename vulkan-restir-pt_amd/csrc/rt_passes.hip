@@ -809,7 +809,11 @@ RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& 
             float cosTheta = -dot(rcSurf.norm, wi);
             dstJacobian = fabsf(cosTheta) / square(dist);
             jacobian = dstJacobian / src.rcJacobian;
-            if (dist > GRIS_DIST_THRESHOLD && cosTheta > 0.0f && !isnan(jacobian) && src.rcJacobian > 0.0f &&
+            // unbiased reuse: a surface rc vertex may face away from its predecessor (interpolated
+            // shading normal), as the path tracer generates it; light vertices must face it
+            const bool facing = (kUnbiased && flagsRcVertexType(src.flags) == RC_SURFACE) ? cosTheta != 0.0f
+                                                                                         : cosTheta > 0.0f;
+            if (dist > GRIS_DIST_THRESHOLD && facing && !isnan(jacobian) && src.rcJacobian > 0.0f &&
                 isBSDFConnectible(rcPrevMat)) {
                 if (traceVisibility<kCount>(S, st, rcPrevSurf.pos, rcSurf.pos, ctr)) srcSampleValid = true;
             }
@@ -876,7 +880,8 @@ RT_DEV bool shiftSupports(const DevScene& S, TStack st, const rt_gris_settings& 
     const f3 wi = normalize(rcSurf.pos - rcPrevSurf.pos);
     const float cosTheta = -dot(rcSurf.norm, wi);
     const float jac = fabsf(cosTheta) / square(dist);
-    if (!(dist > GRIS_DIST_THRESHOLD && cosTheta > 0.0f && jac > 0.0f && !isnan(jac) && isBSDFConnectible(rcPrevMat)))
+    const bool facing = flagsRcVertexType(y.flags) == RC_SURFACE ? cosTheta != 0.0f : cosTheta > 0.0f;
+    if (!(dist > GRIS_DIST_THRESHOLD && facing && jac > 0.0f && !isnan(jac) && isBSDFConnectible(rcPrevMat)))
         return false;
     if (!traceVisibility<kCount>(S, st, rcPrevSurf.pos, rcSurf.pos, ctr)) return false;
     Resv x = y;
