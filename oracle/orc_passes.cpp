@@ -480,15 +480,23 @@ static void traceReplayPathForHybridShift(const Ctx& c, const GRISSettings& set,
 // gris_retrace.glsl:138-236
 // Reconnection shading of gris_retrace.glsl:188-203 with the reconnection
 // radiance rcLi given (the sample's own, or the learned prediction).
+// generated (unbiased reuse, no reference counterpart): the path tracer's own
+// cosines -- |cos| for a direction sampled from a metallic BSDF, clamped for a
+// light sample and for a Lambert vertex (upper-hemisphere sampling only).
 static vec3 reconnectLi(const GRISPathSample& srcSample, vec3 rcLi, const Material& rcMat, const SurfaceInfo& rcSurf,
-                        const Material& rcPrevMat, const SurfaceInfo& rcPrevSurf, vec3 wi, const RcData& rc) {
+                        const Material& rcPrevMat, const SurfaceInfo& rcPrevSurf, vec3 wi, const RcData& rc,
+                        bool generated = false) {
     vec3 Li = rcLi;
     uint32_t rcType = flagsRcVertexType(srcSample.flags);
     if (rcType == RcVertexTypeSurface && length(L3(srcSample.rcWi)) > 0.5f) {
-        Li = Li * (evalBSDF(rcMat, rcSurf.albedo, rcSurf.norm, -wi, L3(srcSample.rcWi)) *
-                   satDot(rcSurf.norm, L3(srcSample.rcWi)));
+        const float cr = (generated && rcMat.type != MatLambert) ? absDot(rcSurf.norm, L3(srcSample.rcWi))
+                                                                 : satDot(rcSurf.norm, L3(srcSample.rcWi));
+        Li = Li * (evalBSDF(rcMat, rcSurf.albedo, rcSurf.norm, -wi, L3(srcSample.rcWi)) * cr);
     }
-    Li = Li * (evalBSDF(rcPrevMat, rcPrevSurf.albedo, rcPrevSurf.norm, rc.rcPrevWo, wi) * satDot(rcPrevSurf.norm, wi));
+    const float cp = (generated && rcType != RcVertexTypeLightSampled && rcPrevMat.type != MatLambert)
+                         ? absDot(rcPrevSurf.norm, wi)
+                         : satDot(rcPrevSurf.norm, wi);
+    Li = Li * (evalBSDF(rcPrevMat, rcPrevSurf.albedo, rcPrevSurf.norm, rc.rcPrevWo, wi) * cp);
     Li = Li * rc.rcPrevThroughput;
     Li = Li / srcSample.rcPrevSamplePdf;
     return Li;
@@ -555,10 +563,10 @@ static void reuseAndMerge(const Ctx& c, const GRISSettings& set, GRISReservoir& 
         if (!std::isnan(srcSample.rcPrevSamplePdf) && srcSample.rcPrevSamplePdf > 1e-6f) {
             const float misW = unbiased ? rcLightMIS(S, rcType, rcPrevSurf, dstRc.rcPrevWo, rcSurf, wi, dstJacobian) : 1.0f;
             Li = reconnectLi(srcSample, unbiased ? L3(srcSample.rcLi) * misW : L3(srcSample.rcLi), rcMat, rcSurf, rcPrevMat,
-                             rcPrevSurf, wi, dstRc);
+                             rcPrevSurf, wi, dstRc, unbiased);
             if (learned)
                 LiL = reconnectLi(srcSample, unbiased ? targetRcLi(srcSample, srcL) * misW : targetRcLi(srcSample, srcL),
-                                  rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, dstRc);
+                                  rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, dstRc, unbiased);
             if (!isBlack(Li) && !hasNan(Li)) dstPHat = luminance((learned ? LiL : Li) * jacobian);
             if (rcType == RcVertexTypeLightSampled) {
                 float sumPower = S.lightTable[0].prob;
@@ -627,11 +635,12 @@ static bool shiftSupports(const Ctx& c, const GRISSettings& set, const GRISPathS
                                         rcPrevSurf.norm, rc.rcPrevWo, wi);
     if (std::isnan(x.rcPrevSamplePdf) || !(x.rcPrevSamplePdf > 1e-6f)) return false;
     const float misW = rcLightMIS(S, flagsRcVertexType(y.flags), rcPrevSurf, rc.rcPrevWo, rcSurf, wi, jac);
-    vec3 Li = reconnectLi(x, L3(x.rcLi) * misW, rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, rc);
+    vec3 Li = reconnectLi(x, L3(x.rcLi) * misW, rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, rc, true);
     // x is a candidate of that domain iff its resampling weight luminance(F) is positive
     if (hasNan(Li) || !(luminance(Li) > 0.0f)) return false;
     if (learned)
-        return luminance(reconnectLi(x, targetRcLi(x, yL) * misW, rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, rc)) > 0.0f;
+        return luminance(reconnectLi(x, targetRcLi(x, yL) * misW, rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, rc, true)) >
+               0.0f;
     return true;
 }
 
@@ -865,9 +874,9 @@ vec3 spatialReuse(const Ctx& c, const Camera& cam, const GBufView& g, const GRIS
                     misW = rcLightMIS(S, flagsRcVertexType(ps.flags), rcPrevSurf, rc.rcPrevWo, rcSurf, wi, gj);
                 }
                 vec3 Li = reconnectLi(ps, unbiased ? L3(ps.rcLi) * misW : L3(ps.rcLi), rcMat, rcSurf, rcPrevMat, rcPrevSurf,
-                                      wi, rc);
+                                      wi, rc, unbiased);
                 float pHat = learned ? luminance(reconnectLi(ps, unbiased ? targetRcLi(ps, L) * misW : targetRcLi(ps, L),
-                                                             rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, rc))
+                                                             rcMat, rcSurf, rcPrevMat, rcPrevSurf, wi, rc, unbiased))
                                      : luminance(Li);
                 if (!isBlack(Li) && !hasNan(Li)) {
                     radiance = Li / pHat * resv.resampleWeight / resv.sampleCount;

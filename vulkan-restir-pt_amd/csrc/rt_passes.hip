@@ -748,15 +748,26 @@ RT_DEV void replayHybrid(const DevScene& S, TStack st, const rt_gris_settings& s
 // GRISReservoirReuseAndMerge (gris_retrace.glsl:188-213) and the final
 // shading of spatialReuse (gris_resample_spatial.glsl:114-125). rcLi is the
 // sample's reconnection radiance (or, for the learned target, its prediction).
+// generated (unbiased reuse): the cosines the path tracer applies when it
+// generates the sample -- |cos| for a direction sampled from a metallic BSDF
+// (nonzero with both directions below the shading normal), the clamped cosine
+// for a light sample and for a Lambert vertex (which samples only its upper
+// hemisphere) -- instead of the reference's clamped cosines, so that a shift
+// is valid exactly where the domain generates.
 RT_DEV f3 reconnectLi(const DevScene& S, const Resv& src, f3 rcLi, const Surf& rcPrevSurf, const Surf& rcSurf, f3 wi,
-                      const RcData& rc) {
+                      const RcData& rc, bool generated = false) {
     Mat rcMat = loadMat(S, rcSurf.matIndex);
     Mat rcPrevMat = loadMat(S, rcPrevSurf.matIndex);
     f3 Li = rcLi;
     uint32_t rcType = flagsRcVertexType(src.flags);
-    if (rcType == RC_SURFACE && length(src.rcWi) > 0.5f)
-        Li = Li * (evalBSDF(rcMat, rcSurf.albedo, rcSurf.norm, -wi, src.rcWi) * satDot(rcSurf.norm, src.rcWi));
-    Li = Li * (evalBSDF(rcPrevMat, rcPrevSurf.albedo, rcPrevSurf.norm, rc.rcPrevWo, wi) * satDot(rcPrevSurf.norm, wi));
+    if (rcType == RC_SURFACE && length(src.rcWi) > 0.5f) {
+        const float c = (generated && rcMat.type != kLambert) ? absDot(rcSurf.norm, src.rcWi)
+                                                              : satDot(rcSurf.norm, src.rcWi);
+        Li = Li * (evalBSDF(rcMat, rcSurf.albedo, rcSurf.norm, -wi, src.rcWi) * c);
+    }
+    const float cp = (generated && rcType != RC_LIGHT_SAMPLED && rcPrevMat.type != kLambert) ? absDot(rcPrevSurf.norm, wi)
+                                                                                             : satDot(rcPrevSurf.norm, wi);
+    Li = Li * (evalBSDF(rcPrevMat, rcPrevSurf.albedo, rcPrevSurf.norm, rc.rcPrevWo, wi) * cp);
     Li = Li * rc.rcPrevThroughput;
     Li = Li / src.rcPrevSamplePdf;
     return Li;
@@ -824,9 +835,10 @@ RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& 
         f3 LiL = s3(0.0f);
         if (!isnan(src.rcPrevSamplePdf) && src.rcPrevSamplePdf > 1e-6f) {
             const float misW = kUnbiased ? rcLightMIS(S, rcType, rcPrevSurf, rc.rcPrevWo, rcSurf, wi, dstJacobian) : 1.0f;
-            Li = reconnectLi(S, src, kUnbiased ? src.rcLi * misW : src.rcLi, rcPrevSurf, rcSurf, wi, rc);
+            Li = reconnectLi(S, src, kUnbiased ? src.rcLi * misW : src.rcLi, rcPrevSurf, rcSurf, wi, rc, kUnbiased);
             if (kLearned)
-                LiL = reconnectLi(S, src, kUnbiased ? targetRcLi(src) * misW : targetRcLi(src), rcPrevSurf, rcSurf, wi, rc);
+                LiL = reconnectLi(S, src, kUnbiased ? targetRcLi(src) * misW : targetRcLi(src), rcPrevSurf, rcSurf, wi, rc,
+                                  kUnbiased);
             if (!isBlack(Li) && !hasNan(Li)) dstPHat = luminance((kLearned ? LiL : Li) * jacobian);
             if (rcType == RC_LIGHT_SAMPLED) {
                 float sumPower = S.lightTable[0].prob;
@@ -891,10 +903,10 @@ RT_DEV bool shiftSupports(const DevScene& S, TStack st, const rt_gris_settings& 
                                              rcPrevSurf.norm, rc.rcPrevWo, wi);
     if (isnan(x.rcPrevSamplePdf) || !(x.rcPrevSamplePdf > 1e-6f)) return false;
     const float misW = rcLightMIS(S, flagsRcVertexType(y.flags), rcPrevSurf, rc.rcPrevWo, rcSurf, wi, jac);
-    const f3 Li = reconnectLi(S, x, x.rcLi * misW, rcPrevSurf, rcSurf, wi, rc);
+    const f3 Li = reconnectLi(S, x, x.rcLi * misW, rcPrevSurf, rcSurf, wi, rc, true);
     // x is a candidate of that domain iff its resampling weight luminance(F) is positive
     if (hasNan(Li) || !(luminance(Li) > 0.0f)) return false;
-    if (kLearned) return luminance(reconnectLi(S, x, targetRcLi(x) * misW, rcPrevSurf, rcSurf, wi, rc)) > 0.0f;
+    if (kLearned) return luminance(reconnectLi(S, x, targetRcLi(x) * misW, rcPrevSurf, rcSurf, wi, rc, true)) > 0.0f;
     return true;
 }
 
@@ -1087,9 +1099,10 @@ RT_DEV void spatialPixel(const DevScene& S, const rt_camera& cam, const GB& g, c
                     const float gj = fabsf(dot(rcSurf.norm, wi)) / square(distance(rcPrevSurf.pos, rcSurf.pos));
                     misW = rcLightMIS(S, flagsRcVertexType(resv.flags), rcPrevSurf, rc.rcPrevWo, rcSurf, wi, gj);
                 }
-                f3 Li = reconnectLi(S, resv, kUnbiased ? resv.rcLi * misW : resv.rcLi, rcPrevSurf, rcSurf, wi, rc);
+                f3 Li = reconnectLi(S, resv, kUnbiased ? resv.rcLi * misW : resv.rcLi, rcPrevSurf, rcSurf, wi, rc,
+                                    kUnbiased);
                 float pHat = kLearned ? luminance(reconnectLi(S, resv, kUnbiased ? targetRcLi(resv) * misW : targetRcLi(resv),
-                                                              rcPrevSurf, rcSurf, wi, rc))
+                                                              rcPrevSurf, rcSurf, wi, rc, kUnbiased))
                                       : luminance(Li);
                 if (!isBlack(Li) && !hasNan(Li)) radiance = Li / pHat * resv.resampleWeight / resv.sampleCount;
             }
