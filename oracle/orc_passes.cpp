@@ -304,7 +304,9 @@ vec3 tracePath(const Ctx& c, const Camera& cam, const GBufView& g, const GRISSet
             pathSample.rcIsec = isec;
             pathSample.rcRng = rng;
             pathSample.rcPrevSamplePdf = s.pdf;
-            pathSample.rcJacobian = geometryJacobian;
+            // unbiased reuse: the Jacobian with the triangle's geometric normal
+            pathSample.rcJacobian = set.unbiasedReuse ? std::fabs(dot(ray.dir, geomNormal(S, isec))) / square(distToPrev)
+                                                      : geometryJacobian;
             flagsSetRcVertexId(pathSample.flags, bounce);
             flagsSetRcVertexType(pathSample.flags, RcVertexTypeSurface);
             rcThroughput = V3s(1.0f);
@@ -545,7 +547,8 @@ static void reuseAndMerge(const Ctx& c, const GRISSettings& set, GRISReservoir& 
             dist = distance(rcPrevSurf.pos, rcSurf.pos);
             wi = normalize(rcSurf.pos - rcPrevSurf.pos);
             float cosTheta = -dot(rcSurf.norm, wi);
-            dstJacobian = std::fabs(cosTheta) / square(dist);
+            const bool geoJ = unbiased && flagsRcVertexType(srcSample.flags) == RcVertexTypeSurface;
+            dstJacobian = std::fabs(geoJ ? dot(geomNormal(S, srcSample.rcIsec), wi) : cosTheta) / square(dist);
             jacobian = dstJacobian / srcSample.rcJacobian;
             // unbiased reuse: a surface rc vertex may face away from its predecessor, as generated
             const bool facing = (unbiased && flagsRcVertexType(srcSample.flags) == RcVertexTypeSurface)
@@ -623,7 +626,9 @@ static bool shiftSupports(const Ctx& c, const GRISSettings& set, const GRISPathS
     float dist = distance(rcPrevSurf.pos, rcSurf.pos);
     vec3 wi = normalize(rcSurf.pos - rcPrevSurf.pos);
     float cosTheta = -dot(rcSurf.norm, wi);
-    float jac = std::fabs(cosTheta) / square(dist);
+    float jac =
+        std::fabs(flagsRcVertexType(y.flags) == RcVertexTypeSurface ? dot(geomNormal(S, y.rcIsec), wi) : cosTheta) /
+        square(dist);
     const bool facing = flagsRcVertexType(y.flags) == RcVertexTypeSurface ? cosTheta != 0.0f : cosTheta > 0.0f;
     if (!(dist > GRISDistanceThreshold && facing && jac > 0.0f && !std::isnan(jac) && isBSDFConnectible(rcPrevMat)))
         return false;

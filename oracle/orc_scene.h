@@ -307,6 +307,20 @@ static inline void loadObjectSurfaceInfo(const SceneArrays& S, uint32_t inst, ui
     // Textured albedo is not part of this build's scenes (textureIdx must be invalid).
     info.albedo = L3(m.baseColor);
 }
+// Geometric normal of a hit triangle in world space (sign arbitrary; callers
+// take |cos|): the area measure of a reconnection vertex's Jacobian.
+static inline vec3 geomNormal(const SceneArrays& S, const Intersection& isec) {
+    if (isec.instanceIdx == 0) {
+        const TriangleLight& L = S.lights[isec.triangleIdx];
+        return V3(L.nx, L.ny, L.nz);
+    }
+    const ObjectInstance& I = S.instances[isec.instanceIdx - 1];
+    const uint32_t io = I.indexOffset, tri = isec.triangleIdx;
+    const vec3 p0 = mat4MulPoint(I.transform, L3(S.vertices[S.indices[io + tri * 3 + 0]].pos), 1.0f);
+    const vec3 p1 = mat4MulPoint(I.transform, L3(S.vertices[S.indices[io + tri * 3 + 1]].pos), 1.0f);
+    const vec3 p2 = mat4MulPoint(I.transform, L3(S.vertices[S.indices[io + tri * 3 + 2]].pos), 1.0f);
+    return normalize(cross(p1 - p0, p2 - p0));
+}
 // ray_layouts.glsl:90-104
 static inline void loadSurfaceInfo(const SceneArrays& S, const Intersection& isec, SurfaceInfo& info) {
     vec3 b = V3(1.0f - isec.bary[0] - isec.bary[1], isec.bary[0], isec.bary[1]);

@@ -124,7 +124,9 @@ def test_reference_weights_bias_is_detected_and_removed():
     print(f"[gpu] box room spatial reuse: reference {rel_ref:+.4f} (z {z_ref:+.2f}), "
           f"unbiased {rel_unb:+.4f} (z {z_unb:+.2f})", flush=True)
     assert z_ref < -4.0
-    assert abs(rel_unb) < 0.5 * abs(rel_ref)
+    # no detectable bias: the reconnection Jacobian with the geometric normal, surface vertices on
+    # either side of their shading normal and the path tracer's own cosines and classification
+    assert abs(z_unb) < 4.0 and abs(rel_unb) < 0.1 * abs(rel_ref)
 
 
 def test_guided_unbiased_reuse_shows_no_bias():

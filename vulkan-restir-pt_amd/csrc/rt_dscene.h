@@ -274,6 +274,20 @@ RT_DEV void loadSurfaceInfo(const DevScene& S, const Isec& is, Surf& info) {
     info.isLight = false;
 }
 
+// Geometric normal of a hit triangle in world space (its sign is arbitrary:
+// callers take |cos|). The area-to-solid-angle factor of a reconnection
+// vertex is |n_g . w| / d^2; the interpolated shading normal is not the
+// surface's measure.
+RT_DEV f3 geomNormal(const DevScene& S, const Isec& is) {
+    if (is.inst == 0) { const rt_triangle_light& L = S.lights[is.prim]; return mk3(L.nx, L.ny, L.nz); }
+    const DevInstance& I = S.instances[is.inst - 1];
+    const uint32_t io = I.indexOffset;
+    const f3 p0 = colMul4(I.m, ld3(S.vertices[S.indices[io + is.prim * 3 + 0]].pos), 1.0f);
+    const f3 p1 = colMul4(I.m, ld3(S.vertices[S.indices[io + is.prim * 3 + 1]].pos), 1.0f);
+    const f3 p2 = colMul4(I.m, ld3(S.vertices[S.indices[io + is.prim * 3 + 2]].pos), 1.0f);
+    return normalize(cross(p1 - p0, p2 - p0));
+}
+
 // ---- light_sampling.glsl:6-53 -----------------------------------------------------------------
 RT_DEV f3 sampleLightByPower(const DevScene& S, f3 ref, f3& wi, float& dist, float& pdf, float& jacobian, f2& bary,
                              uint32_t& id, f4 r) {

@@ -487,7 +487,9 @@ RT_DEV bool ptVertex(const DevScene& S, TStack st, float* slot, const rt_gris_se
             ps.rcIsec = isec;
             ps.rcRng = rng;
             ps.rcPrevSamplePdf = s.pdf;
-            ps.rcJacobian = geometryJacobian;
+            // unbiased reuse: the Jacobian with the triangle's geometric normal
+            ps.rcJacobian = set.unbiasedReuse ? fabsf(dot(rdir, geomNormal(S, isec))) / square(distToPrev)
+                                              : geometryJacobian;
             flagsSetRcVertexId(ps.flags, bounce);
             flagsSetRcVertexType(ps.flags, RC_SURFACE);
             rcThroughput = s3(1.0f);
@@ -818,7 +820,9 @@ RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& 
             float dist = distance(rcPrevSurf.pos, rcSurf.pos);
             wi = normalize(rcSurf.pos - rcPrevSurf.pos);
             float cosTheta = -dot(rcSurf.norm, wi);
-            dstJacobian = fabsf(cosTheta) / square(dist);
+            // unbiased reuse: a surface vertex's Jacobian uses its geometric normal
+            const bool geoJ = kUnbiased && flagsRcVertexType(src.flags) == RC_SURFACE;
+            dstJacobian = fabsf(geoJ ? dot(geomNormal(S, src.rcIsec), wi) : cosTheta) / square(dist);
             jacobian = dstJacobian / src.rcJacobian;
             // unbiased reuse: a surface rc vertex may face away from its predecessor (interpolated
             // shading normal), as the path tracer generates it; light vertices must face it
@@ -891,7 +895,8 @@ RT_DEV bool shiftSupports(const DevScene& S, TStack st, const rt_gris_settings& 
     const float dist = distance(rcPrevSurf.pos, rcSurf.pos);
     const f3 wi = normalize(rcSurf.pos - rcPrevSurf.pos);
     const float cosTheta = -dot(rcSurf.norm, wi);
-    const float jac = fabsf(cosTheta) / square(dist);
+    const float jac =
+        fabsf(flagsRcVertexType(y.flags) == RC_SURFACE ? dot(geomNormal(S, y.rcIsec), wi) : cosTheta) / square(dist);
     const bool facing = flagsRcVertexType(y.flags) == RC_SURFACE ? cosTheta != 0.0f : cosTheta > 0.0f;
     if (!(dist > GRIS_DIST_THRESHOLD && facing && jac > 0.0f && !isnan(jac) && isBSDFConnectible(rcPrevMat)))
         return false;
