@@ -131,7 +131,11 @@ typedef struct rt_gris_settings {
      * (the total M is replaced by the sum of the M of the domains whose inverse
      * shift of the picked sample is valid with a positive target -- the "1/Z"
      * contribution weights; M-cap applied once per pass; at most 8 spatial
-     * neighbours). No counterpart in the reference. */
+     * neighbours), with shifts held to what each domain's path tracer
+     * generates: reconnection Jacobians with the triangle's geometric normal,
+     * surface reconnection vertices on either side of their shading normal,
+     * the path tracer's own cosines and reconnection-vertex classification.
+     * No counterpart in the reference. */
     uint32_t unbiasedReuse;
     /* Learned directional proposal at the primary vertex (no counterpart in the
      * reference): 1 = sample the primary vertex's direction from the frame's
