@@ -1,8 +1,8 @@
 """GPU parity: every kernel of the hot path against the oracle, through the C ABI.
 
 Integer/index work (LBVH codes, order, topology, hit ids) must be bit-exact;
-floating-point per-pixel results are compared at 1e-4 relative (north-star
-tolerance for the plain form) and are expected to be identical because both
+floating-point per-pixel results are compared at 1e-4 relative (the workload's
+stated tolerance for the plain form) and are expected to be identical because both
 sides follow one FP contract (no contraction, correctly rounded div/sqrt,
 fixed sin/cos definition).
 """
