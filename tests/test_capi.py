@@ -93,6 +93,20 @@ def test_argument_checks_fail_loudly():
     assert b"sampleType" in L.rt_last_error()
     assert L.rt_di_path_gen(fake, cams, C.byref(f), None, C.byref(rt.di_settings()), None) == 1
     assert b"reservoirs is null" in L.rt_last_error()
+    # frame graphs: every form's requirements are checked before anything is allocated
+    fs = (rt.Frame * 2)()
+    for j in range(2):
+        fs[j].width, fs[j].height = 4, 4
+    g = C.c_void_p()
+    guided = rt.settings(guided=True)
+    assert L.rt_frame_graph_create(fake, fs, C.byref(guided), None, None, C.byref(g)) == 1
+    assert b"proposal network" in L.rt_last_error()
+    wide = rt.settings(unbiased=True, neighbors=9)
+    assert L.rt_frame_graph_create(fake, fs, C.byref(wide), None, None, C.byref(g)) == 1
+    assert b"at most 8" in L.rt_last_error()
+    learned = rt.settings(learned=True)
+    assert L.rt_frame_graph_create(fake, fs, C.byref(learned), None, None, C.byref(g)) == 1
+    assert b"radiance network" in L.rt_last_error()
 
 
 def test_missing_library_is_an_error(monkeypatch):
