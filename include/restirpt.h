@@ -225,14 +225,22 @@ rt_status rt_gbuffer_trace(const rt_device_scene* scene, const rt_camera* cam, f
 rt_status rt_pt_plain(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
                       uint32_t maxDepth, uint32_t which, hipStream_t stream);
 
-/* The three passes of GRISReSTIR::render (src/GRISReSTIR.cpp:9-53). */
+/* The three passes of GRISReSTIR::render (src/GRISReSTIR.cpp:9-53).
+ * rt_gris_path_trace (also inside rt_gris_render) is a persistent kernel that
+ * takes its pixel queue from 64 slots per scene, handed out round robin: at
+ * most 64 such launches on one scene may be in flight at once across streams
+ * (launches on one stream are ordered and never collide). */
 rt_status rt_gris_path_trace(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
                              const rt_gris_settings* settings, hipStream_t stream);
 rt_status rt_gris_temporal(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
                            const rt_gris_settings* settings, hipStream_t stream);
 rt_status rt_gris_spatial(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
                           const rt_gris_settings* settings, hipStream_t stream);
-/* GRISReSTIR::render: path trace, temporal reuse, spatial reuse, in order. */
+/* GRISReSTIR::render: path trace, temporal reuse, spatial reuse, in order.
+ * Luminance target and BSDF proposal only: targetFunction = 1 and
+ * guidedProposal = 1 are rejected, since their refresh and training steps run
+ * between the passes (rt_gris_learned_refresh, rt_gris_guide_refresh/train,
+ * or the frame graph, which records them). */
 rt_status rt_gris_render(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
                          const rt_gris_settings* settings, hipStream_t stream);
 

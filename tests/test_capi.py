@@ -107,6 +107,32 @@ def test_argument_checks_fail_loudly():
     learned = rt.settings(learned=True)
     assert L.rt_frame_graph_create(fake, fs, C.byref(learned), None, None, C.byref(g)) == 1
     assert b"radiance network" in L.rt_last_error()
+    # the graph always records temporal reuse: both parities need the previous G-buffer
+    plain = rt.settings()
+    for j in range(2):
+        for name in ("depthNormal", "albedoMatId", "motionVector", "grisReservoir", "grisReservoirPrev",
+                     "grisReservoirTemp", "indirectOutput"):
+            setattr(fs[j], name, 8)
+    assert L.rt_frame_graph_create(fake, fs, C.byref(plain), None, None, C.byref(g)) == 1
+    assert b"previous G-buffer" in L.rt_last_error()
+    for j in range(2):
+        fs[j].depthNormalPrev = fs[j].albedoMatIdPrev = 8
+    fs[1].albedoMatIdPrev = None
+    assert L.rt_frame_graph_create(fake, fs, C.byref(plain), None, None, C.byref(g)) == 1
+    assert b"previous G-buffer" in L.rt_last_error()
+    fs[1].albedoMatIdPrev = 8
+    # a scene without lights is rejected as the eager passes reject it (a zero-filled handle has none)
+    unlit = C.create_string_buffer(1 << 16)
+    assert L.rt_frame_graph_create(unlit, fs, C.byref(plain), None, None, C.byref(g)) == 1
+    assert b"no lights" in L.rt_last_error()
+    # eager entry points: the learned refresh takes a radiance network only, and the one-call
+    # frame refuses the learned proposal, whose refresh and training it does not run
+    f.grisReservoir = 8
+    not_radiance = C.create_string_buffer(1 << 16)  # a zero-filled network handle has 0 outputs
+    assert L.rt_gris_learned_refresh(fake, not_radiance, cams, C.byref(f), None) == 1
+    assert b"not a radiance network" in L.rt_last_error()
+    assert L.rt_gris_render(fake, cams, C.byref(f), C.byref(guided), None) == 1
+    assert b"rt_gris_guide_refresh" in L.rt_last_error()
 
 
 def test_missing_library_is_an_error(monkeypatch):

@@ -325,6 +325,9 @@ rt_status rt_gris_render(const rt_device_scene* s, const rt_camera cams[2], cons
     if (set && set->targetFunction == 1u)
         return invalid("rt_gris_render runs the luminance target only; the learned target needs "
                        "rt_gris_learned_refresh between the path-trace and temporal passes");
+    if (set && set->guidedProposal == 1u)
+        return invalid("rt_gris_render runs the BSDF proposal only; the learned proposal needs "
+                       "rt_gris_guide_refresh before the path trace and rt_gris_guide_train after the frame");
     rt_status st = rt_gris_path_trace(s, cams, f, set, stream);
     if (st == RT_OK) st = rt_gris_temporal(s, cams, f, set, stream);
     if (st == RT_OK) st = rt_gris_spatial(s, cams, f, set, stream);
@@ -518,6 +521,7 @@ rt_status rt_gris_learned_refresh(const rt_device_scene* s, rt_mlp* mlp, const r
     rt_status st = checkFrame(s, cams, f);
     if (st != RT_OK) return st;
     if (!mlp) return invalid("mlp is null");
+    if (mlp->outputs != (uint32_t)rtm::kOut) return invalid("mlp is not a radiance network (rt_mlp_create)");
     if (!f->grisReservoir) return invalid("frame reservoir is null");
     RT_LAUNCH(rti::launchLearnedRefresh(s, mlp, cams, *f, stream));
     return RT_OK;

@@ -143,7 +143,10 @@ rt_status rt_frame_graph_create(const rt_device_scene* scene, const rt_frame fra
         if (!f.depthNormal || !f.albedoMatId || !f.motionVector || !f.grisReservoir || !f.grisReservoirPrev ||
             !f.grisReservoirTemp || !f.indirectOutput)
             return invalid("frame bindings are incomplete");
+        // the graph always records temporal reuse, which reads the previous G-buffer
+        if (!f.depthNormalPrev || !f.albedoMatIdPrev) return invalid("previous G-buffer is null");
     }
+    if (scene->numLights == 0) return invalid("the scene has no lights");
     rt_frame_graph* g = new rt_frame_graph();
     g->scene = scene;
     g->frames[0] = frames[0];
