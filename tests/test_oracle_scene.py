@@ -1,5 +1,6 @@
 """Pins the light table (AliasTable.h), light sampling (light_sampling.glsl), the
-camera ray (camera.glsl) and the procedural workloads."""
+camera ray (camera.glsl), the G-buffer's motion vectors (GBuffer.frag) and the procedural
+workloads."""
 import ctypes as C
 
 import numpy as np
@@ -99,6 +100,52 @@ def test_camera_rays_closed_form():
     p = np.array(list(cam.pos)) + 3.0 * F
     c = P @ np.append(p, 1.0)
     assert np.allclose(c[:2] / c[3], 0.0, atol=1e-5)
+
+
+def pixel_rays(cam, W, H):
+    fov = np.radians(cam.FOV) / 2
+    y, x = np.mgrid[0:H, 0:W]
+    px = (2 * (x + 0.5) / W - 1) * (W / H) * np.tan(fov)
+    py = (2 * (1 - (y + 0.5) / H) - 1) * np.tan(fov)
+    R, U, F = (np.array(list(getattr(cam, k)), np.float64) for k in ("right", "up", "front"))
+    d = px[..., None] * R + py[..., None] * U + F
+    return d / np.linalg.norm(d, axis=-1, keepdims=True)
+
+
+@pytest.mark.parametrize("frame", [0, 5, 11])
+def test_motion_vectors_reproject_through_last_proj_view(frame):
+    """GBuffer.frag:40-53: motion = (lastProjView p).xy / w * 0.5 + 0.5 - gl_FragCoord.xy / filmSize
+    for the surface point p. Here p is rebuilt from the G-buffer alone (camera position + unit ray
+    direction * the stored Euclidean depth) and projected in float64 through the camera walk's
+    lastProjView; with lastProjView = projView the motion is zero (the projection and the pixel
+    centres agree). Tolerance: the half-float store (2^-11 relative) plus float32 depth rounding."""
+    hs = rt.HostScene(1, 1)
+    W, H = 64, 36
+    osc = orc.OracleScene(hs.arrays)
+    cams = hs.camera(frame, W, H)
+    cam = cams[0]
+    for still in (True, False):
+        c = rt.Camera.from_buffer_copy(bytes(cam))
+        if still:
+            c.lastProjView[:] = list(cam.projView)
+        dn, _, mv, _ = osc.gbuffer(c)
+        fg = dn[..., 0] > 0
+        assert fg.mean() > 0.5
+        p = np.array(list(c.pos), np.float64) + pixel_rays(c, W, H) * dn[..., :1].astype(np.float64)
+        LPV = np.array(list(c.lastProjView), np.float64).reshape(4, 4).T
+        q = np.concatenate([p, np.ones((H, W, 1))], -1) @ LPV.T
+        last = q[..., :2] / q[..., 3:4] * 0.5 + 0.5
+        y, x = np.mgrid[0:H, 0:W]
+        this = np.stack([(x + 0.5) / W, (y + 0.5) / H], -1)
+        ref = last - this
+        got = mv.view(np.float16).astype(np.float64)
+        err = np.abs(got - ref)[fg]
+        assert (err <= 1e-3 * np.abs(ref[fg]) + 2e-5).all(), err.max()
+        assert (got[~fg] == 0).all()
+        if still:
+            assert np.abs(got[fg]).max() < 1e-4
+        elif frame > 0:
+            assert np.abs(ref[fg]).max() > 1e-3   # the camera walk moves
 
 
 def test_procedural_workloads():
