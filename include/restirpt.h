@@ -179,6 +179,8 @@ typedef struct rt_scene_info {
     size_t deviceBytes;
     uint32_t bvh4Nodes;        /* nodes of the collapsed 4-wide tree that is traversed */
     uint32_t bvh4Depth;
+    float bvhBuildMs;          /* GPU time of the LBVH build: world triangles to the 4-wide tree */
+    float bvhBinaryMs;         /* of which the binary LBVH (bounds, Morton codes, sort, topology, refit) */
 } rt_scene_info;
 
 typedef struct rt_device_scene rt_device_scene;

@@ -25,6 +25,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kItems = 16;
 constexpr int kTile = kBlock * kItems;
+constexpr uint32_t kNoParent = 0xffffffffu;
 
 __device__ __forceinline__ uint64_t expandBits21(uint64_t v) {
     v &= 0x1fffffull;
@@ -140,6 +141,30 @@ __global__ void k_morton(const float4* __restrict__ primTris, uint32_t n, const 
     vals[p] = p;
 }
 
+// Block-wide exclusive scan of one value per thread (blockDim.x a multiple of
+// 64, at most 1024); *total receives the block's sum. Holds two barriers.
+__device__ __forceinline__ uint32_t blockExclusiveScan(uint32_t v, uint32_t* total) {
+    __shared__ uint32_t waveSum[16];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(inc, o, 64);
+        if (lane >= (uint32_t)o) inc += t;
+    }
+    if (lane == 63u) waveSum[wave] = inc;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+    for (uint32_t w = 0; w < waves; w++) {
+        uint32_t x = waveSum[w];
+        if (w < wave) base += x;
+        tot += x;
+    }
+    __syncthreads();
+    *total = tot;
+    return base + inc - v;
+}
+
 // 4. radix sort
 __global__ void k_radix_hist(const unsigned long long* __restrict__ keys, uint32_t n, int shift, uint32_t nBlocks,
                              uint32_t* __restrict__ blockHist) {
@@ -155,32 +180,36 @@ __global__ void k_radix_hist(const unsigned long long* __restrict__ keys, uint32
     blockHist[threadIdx.x * nBlocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// exclusive scan of a uint32 array in one block of 1024 threads
-__global__ void k_scan_single(uint32_t* __restrict__ data, uint32_t n) {
-    __shared__ uint32_t part[1024];
-    uint32_t per = (n + 1023) / 1024;
-    uint32_t b = threadIdx.x * per, e = min(n, b + per);
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; i++) s += data[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t v = (threadIdx.x >= (uint32_t)off) ? part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
+// One workgroup per digit: exclusive scan of the digit's per-block counts in
+// place, digit total to digitTotal[digit].
+__global__ void k_radix_digit_scan(uint32_t* __restrict__ blockHist, uint32_t nBlocks, uint32_t* __restrict__ digitTotal) {
+    uint32_t* h = blockHist + (size_t)blockIdx.x * nBlocks;
+    uint32_t carry = 0;
+    for (uint32_t c = 0; c < nBlocks; c += kBlock) {
+        const uint32_t i = c + threadIdx.x;
+        const uint32_t v = i < nBlocks ? h[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = blockExclusiveScan(v, &tot);
+        if (i < nBlocks) h[i] = carry + ex;
+        carry += tot;
     }
-    uint32_t run = part[threadIdx.x] - s;  // exclusive prefix of this chunk
-    for (uint32_t i = b; i < e; i++) { uint32_t v = data[i]; data[i] = run; run += v; }
+    if (threadIdx.x == 0) digitTotal[blockIdx.x] = carry;
 }
 
+// Stable scatter: a key's destination is the exclusive count of smaller digits
+// (scan of the digit totals), plus the count of its digit in earlier blocks,
+// plus its rank among equal digits earlier in the block (8 wave ballots give
+// the equal-digit lane mask; per-wave digit counts go through LDS).
 __global__ void k_radix_scatter(const unsigned long long* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
                                 uint32_t n, int shift, uint32_t nBlocks, const uint32_t* __restrict__ blockOffset,
-                                unsigned long long* __restrict__ keysOut, uint32_t* __restrict__ valsOut) {
+                                const uint32_t* __restrict__ digitTotal, unsigned long long* __restrict__ keysOut,
+                                uint32_t* __restrict__ valsOut) {
     __shared__ uint32_t running[256];
     __shared__ uint32_t waveCnt[kBlock / 64][256];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    running[tid] = blockOffset[tid * nBlocks + blockIdx.x];
+    uint32_t unused;
+    const uint32_t digitBase = blockExclusiveScan(digitTotal[tid], &unused);
+    running[tid] = digitBase + blockOffset[tid * nBlocks + blockIdx.x];
     for (int w = 0; w < kBlock / 64; w++) waveCnt[w][tid] = 0;
     __syncthreads();
     const unsigned long long ltMask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
@@ -214,7 +243,48 @@ __global__ void k_radix_scatter(const unsigned long long* __restrict__ keysIn, c
     }
 }
 
-// 5. Karras topology
+// Exclusive scan of n uint32 (in may equal out): per-chunk sums, one workgroup
+// scanning the chunk sums, then each chunk rescanned with its base.
+constexpr uint32_t kScanChunk = kBlock * kItems;
+
+__global__ void k_scan_reduce(const uint32_t* __restrict__ in, uint32_t n, uint32_t* __restrict__ sums) {
+    const uint32_t b = blockIdx.x * kScanChunk + threadIdx.x * kItems;
+    uint32_t s = 0;
+    for (int k = 0; k < kItems; k++)
+        if (b + k < n) s += in[b + k];
+    uint32_t tot;
+    (void)blockExclusiveScan(s, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_sums(uint32_t* __restrict__ sums, uint32_t m) {
+    uint32_t carry = 0;
+    for (uint32_t c = 0; c < m; c += blockDim.x) {
+        const uint32_t i = c + threadIdx.x;
+        const uint32_t v = i < m ? sums[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = blockExclusiveScan(v, &tot);
+        if (i < m) sums[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+__global__ void k_scan_apply(const uint32_t* in, uint32_t n, const uint32_t* __restrict__ sums, uint32_t* out) {
+    const uint32_t b = blockIdx.x * kScanChunk + threadIdx.x * kItems;
+    uint32_t v[kItems];
+    uint32_t s = 0;
+    for (int k = 0; k < kItems; k++) {
+        v[k] = b + k < n ? in[b + k] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = sums[blockIdx.x] + blockExclusiveScan(s, &tot);
+    for (int k = 0; k < kItems; k++)
+        if (b + k < n) { out[b + k] = run; run += v[k]; }
+}
+
+// 5. Karras topology. Also records each node's leaf range and clears the
+// counters of the refit's second phase.
 __device__ __forceinline__ int kdelta(const unsigned long long* c, int n, int i, int j) {
     if (j < 0 || j >= n) return -1;
     unsigned long long a = c[i], b = c[j];
@@ -227,7 +297,8 @@ __device__ __forceinline__ int kdelta(const unsigned long long* c, int n, int i,
 }
 
 __global__ void k_karras(const unsigned long long* __restrict__ codes, int n, float4* __restrict__ nodes,
-                         uint32_t* __restrict__ parentI, uint32_t* __restrict__ parentL) {
+                         uint32_t* __restrict__ parentI, uint32_t* __restrict__ parentL, uint2* __restrict__ range,
+                         uint32_t* __restrict__ counters, uint32_t* __restrict__ numStops) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     int d = (kdelta(codes, n, i, i + 1) - kdelta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -250,7 +321,9 @@ __global__ void k_karras(const unsigned long long* __restrict__ codes, int n, fl
     nodes[4 * (size_t)i + 3] = make_float4(__uint_as_float(left), __uint_as_float(right), 0.0f, 0.0f);
     if (left & RT_LEAF) parentL[left & ~RT_LEAF] = (uint32_t)i; else parentI[left] = (uint32_t)i;
     if (right & RT_LEAF) parentL[right & ~RT_LEAF] = (uint32_t)i; else parentI[right] = (uint32_t)i;
-    if (i == 0) parentI[0] = 0xffffffffu;
+    range[i] = make_uint2((uint32_t)min(i, j), (uint32_t)max(i, j));
+    counters[i] = 0u;
+    if (i == 0) { parentI[0] = kNoParent; *numStops = 0u; }
 }
 
 // leaves in Morton order
@@ -270,45 +343,111 @@ __device__ __forceinline__ void writeSlot(float* nf, bool right, const float* lo
     nf[o + 3] = hi[0]; nf[o + 4] = hi[1]; nf[o + 5] = hi[2];
 }
 
-// 6. refit
-__global__ void k_refit(const float4* __restrict__ leafTris, uint32_t n, float4* nodes,
-                        const uint32_t* __restrict__ parentI, const uint32_t* __restrict__ parentL, uint32_t* counters) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4* t = leafTris + 3 * (size_t)i;
+// union of the node's two child boxes; returns the node's height (a leaf has
+// height 0) from its children's
+__device__ __forceinline__ uint32_t mergeNode(const float* nf, const uint32_t* height, float* lo, float* hi) {
+    for (int a = 0; a < 3; a++) {
+        lo[a] = fminf(nf[a], nf[6 + a]);
+        hi[a] = fmaxf(nf[3 + a], nf[9 + a]);
+    }
+    const uint32_t l = __float_as_uint(nf[12]), r = __float_as_uint(nf[13]);
+    const uint32_t hl = (l & RT_LEAF) ? 0u : height[l], hr = (r & RT_LEAF) ? 0u : height[r];
+    return 1u + max(hl, hr);
+}
+
+// 6. refit, phase 1. Workgroup b owns leaves [b*S, b*S + S) and every internal
+// node whose leaf range lies inside them (a Karras node is one end of its own
+// range, so those nodes are numbered inside the same interval). One thread per
+// leaf climbs: it writes its box into its slot of the parent; the second
+// arrival at a node (counted in LDS) merges both slots and climbs on. Every
+// arrival is ordered by workgroup-scope fences only, which on this part need no
+// cache maintenance. A climb that reaches a node with leaves outside the
+// workgroup leaves its slot written and records the node for phase 2.
+constexpr uint32_t kRefitThreads = 1024;
+constexpr uint32_t kRefitLeaves = 4096;
+
+__global__ __launch_bounds__(kRefitThreads) void k_refit_block(const float4* __restrict__ tris,
+                                                               const uint32_t* __restrict__ order, uint32_t n,
+                                                               float4* nodes, const uint32_t* __restrict__ parentI,
+                                                               const uint32_t* __restrict__ parentL,
+                                                               const uint2* __restrict__ range, uint32_t* height,
+                                                               uint32_t* __restrict__ stops, uint32_t* numStops) {
+    __shared__ uint32_t arrivals[kRefitLeaves];
+    const uint32_t L0 = blockIdx.x * kRefitLeaves, L1 = min(n, L0 + kRefitLeaves);
+    for (uint32_t k = threadIdx.x; k < kRefitLeaves; k += kRefitThreads) arrivals[k] = 0u;
+    __syncthreads();
+    for (uint32_t i = L0 + threadIdx.x; i < L1; i += kRefitThreads) {
+        const float4* t = tris + 3 * (size_t)(order ? order[i] : i);
+        float lo[3] = {fminf(fminf(t[0].x, t[1].x), t[2].x), fminf(fminf(t[0].y, t[1].y), t[2].y),
+                       fminf(fminf(t[0].z, t[1].z), t[2].z)};
+        float hi[3] = {fmaxf(fmaxf(t[0].x, t[1].x), t[2].x), fmaxf(fmaxf(t[0].y, t[1].y), t[2].y),
+                       fmaxf(fmaxf(t[0].z, t[1].z), t[2].z)};
+        uint32_t ref = RT_LEAF | i;
+        uint32_t p = parentL[i];
+        while (true) {
+            float* nf = reinterpret_cast<float*>(nodes + 4 * (size_t)p);
+            writeSlot(nf, __float_as_uint(nf[12]) != ref, lo, hi);
+            const uint2 r = range[p];
+            if (r.x < L0 || r.y >= L1) {
+                stops[atomicAdd(numStops, 1u)] = p;
+                break;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            const uint32_t old = atomicAdd(&arrivals[p - L0], 1u);
+            if (old == 0u) break;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            height[p] = mergeNode(nf, height, lo, hi);
+            ref = p;
+            p = parentI[p];
+            if (p == kNoParent) break;
+        }
+    }
+}
+
+// Refit, phase 2: the nodes whose leaves span several phase-1 workgroups (a
+// few per workgroup boundary) in one workgroup, so that the same
+// workgroup-scope ordering suffices; each recorded stop is one arrival.
+__global__ __launch_bounds__(kRefitThreads) void k_refit_top(float4* nodes, const uint32_t* __restrict__ parentI,
+                                                             uint32_t* counters, uint32_t* height,
+                                                             const uint32_t* __restrict__ stops,
+                                                             const uint32_t* __restrict__ numStops) {
+    const uint32_t ns = *numStops;
+    for (uint32_t e = threadIdx.x; e < ns; e += blockDim.x) {
+        uint32_t p = stops[e];
+        while (true) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            const uint32_t old = __hip_atomic_fetch_add(&counters[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0u) break;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const float* nf = reinterpret_cast<const float*>(nodes + 4 * (size_t)p);
+            float lo[3], hi[3];
+            height[p] = mergeNode(nf, height, lo, hi);
+            const uint32_t q = parentI[p];
+            if (q == kNoParent) break;
+            float* qf = reinterpret_cast<float*>(nodes + 4 * (size_t)q);
+            writeSlot(qf, __float_as_uint(qf[12]) != p, lo, hi);
+            p = q;
+        }
+    }
+}
+
+// a one-primitive scene: one node holding the leaf in its left slot
+__global__ void k_single_leaf(const float4* __restrict__ tris, float4* __restrict__ nodes, uint32_t* __restrict__ height) {
+    const float4* t = tris;
     float lo[3] = {fminf(fminf(t[0].x, t[1].x), t[2].x), fminf(fminf(t[0].y, t[1].y), t[2].y),
                    fminf(fminf(t[0].z, t[1].z), t[2].z)};
     float hi[3] = {fmaxf(fmaxf(t[0].x, t[1].x), t[2].x), fmaxf(fmaxf(t[0].y, t[1].y), t[2].y),
                    fmaxf(fmaxf(t[0].z, t[1].z), t[2].z)};
-    uint32_t ref = RT_LEAF | i;
-    uint32_t p = parentL[i];
-    while (p != 0xffffffffu) {
-        float* nf = reinterpret_cast<float*>(nodes + 4 * (size_t)p);
-        uint32_t leftRef = __float_as_uint(nf[12]);
-        writeSlot(nf, leftRef != ref, lo, hi);
-        __threadfence();
-        uint32_t old = atomicAdd(&counters[p], 1u);
-        if (old == 0) return;
-        __threadfence();
-        for (int a = 0; a < 3; a++) {
-            lo[a] = fminf(__hip_atomic_load(&nf[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                          __hip_atomic_load(&nf[6 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            hi[a] = fmaxf(__hip_atomic_load(&nf[3 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                          __hip_atomic_load(&nf[9 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        }
-        ref = p;
-        p = parentI[p];
-    }
+    nodes[0] = make_float4(lo[0], lo[1], lo[2], hi[0]);
+    nodes[1] = make_float4(hi[1], hi[2], INFINITY, INFINITY);
+    nodes[2] = make_float4(INFINITY, -INFINITY, -INFINITY, -INFINITY);
+    nodes[3] = make_float4(__uint_as_float(RT_LEAF), __uint_as_float(RT_LEAF), 0.0f, 0.0f);
+    height[0] = 1u;
 }
 
-// depth: each leaf climbs to the root counting internal nodes
-__global__ void k_depth(const uint32_t* __restrict__ parentI, const uint32_t* __restrict__ parentL, uint32_t n,
-                        uint32_t* maxDepth) {
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t d = 0, p = parentL[i];
-    while (p != 0xffffffffu && d < 4096u) { d++; p = parentI[p]; }
-    atomicMax(maxDepth, d);
+    if (i < n) p[i] = v;
 }
 
 // 7. collapse to a 4-wide tree. A binary node's candidate slots: its two
@@ -346,104 +485,233 @@ __device__ int collapseSlots(const float4* nodes, uint32_t n, Slot4 s[4]) {
     return cnt;
 }
 
-// frontier expansion: marks the internal slots of every frontier node
-__global__ void k_collapse_mark(const float4* __restrict__ nodes, const uint32_t* __restrict__ frontier, uint32_t nf,
-                                uint32_t* __restrict__ keep, uint32_t* __restrict__ next, uint32_t* __restrict__ nNext) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nf) return;
-    Slot4 s[4];
-    int cnt = collapseSlots(nodes, frontier[i], s);
-    for (int k = 0; k < cnt; k++) {
-        if (s[k].ref & RT_LEAF) continue;
-        keep[s[k].ref] = 1u;
-        next[atomicAdd(nNext, 1u)] = s[k].ref;
+// Frontier expansion of one level. Every kept node is listed once in `kept`,
+// level after level: level l occupies kept[start_l, start_l + cnt[l]) with
+// start_l the sum of the earlier levels' counts (all final before this launch),
+// and its nodes append their internal slots behind it as level l + 1 (one
+// atomic per wave for the wave's appends). The marks in `keep` give the 4-wide
+// node numbering in binary order.
+__device__ __forceinline__ uint32_t levelStart(const uint32_t* cnt, uint32_t level) {
+    uint32_t s = 0;
+    for (uint32_t j = 0; j < level; j++) s += cnt[j];
+    return s;
+}
+
+__global__ void k_collapse_mark(const float4* __restrict__ nodes, uint32_t* __restrict__ kept, uint32_t* cnt,
+                                uint32_t level, uint32_t* __restrict__ keep) {
+    const uint32_t start = levelStart(cnt, level);
+    const uint32_t nf = cnt[level];
+    uint32_t* next = kept + start + nf;
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < nf; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        Slot4 s[4];
+        int n = 0;
+        uint32_t refs[4] = {0u, 0u, 0u, 0u};
+        if (i < nf) {
+            const int c = collapseSlots(nodes, kept[start + i], s);
+            for (int k = 0; k < c; k++)
+                if (!(s[k].ref & RT_LEAF)) refs[n++] = s[k].ref;
+        }
+        // exclusive prefix of the appends over the wave, one atomic per wave
+        uint32_t inc = (uint32_t)n;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(inc, o, 64);
+            if (lane >= (uint32_t)o) inc += t;
+        }
+        const uint32_t total = __shfl(inc, 63, 64);
+        uint32_t waveBase = 0;
+        if (lane == 63u && total) waveBase = atomicAdd(&cnt[level + 1], total);
+        waveBase = __shfl(waveBase, 63, 64);
+        const uint32_t at = waveBase + inc - (uint32_t)n;
+        for (int k = 0; k < n; k++) {
+            keep[refs[k]] = 1u;
+            next[at + k] = refs[k];
+        }
     }
 }
 
-__global__ void k_collapse_emit(const float4* __restrict__ nodes, uint32_t nb, const uint32_t* __restrict__ keep,
-                                const uint32_t* __restrict__ newId, float4* __restrict__ nodes4) {
-    uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nb || !keep[n]) return;
-    Slot4 s[4];
-    int cnt = collapseSlots(nodes, n, s);
-    float lo[3][4], hi[3][4];
-    uint32_t ch[4];
-    for (int k = 0; k < 4; k++) {
-        if (k < cnt) {
-            ch[k] = (s[k].ref & RT_LEAF) ? s[k].ref : newId[s[k].ref];
-            for (int a = 0; a < 3; a++) { lo[a][k] = s[k].lo[a]; hi[a][k] = s[k].hi[a]; }
-        } else {
-            ch[k] = 0xffffffffu;
-            for (int a = 0; a < 3; a++) { lo[a][k] = INFINITY; hi[a][k] = -INFINITY; }
+// One thread per kept node (grid-stride over all levels' lists).
+__global__ void k_collapse_emit(const float4* __restrict__ nodes, const uint32_t* __restrict__ kept,
+                                const uint32_t* __restrict__ cnt, uint32_t levels, const uint32_t* __restrict__ newId,
+                                float4* __restrict__ nodes4) {
+    const uint32_t total = levelStart(cnt, levels + 1);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const uint32_t n = kept[i];
+        Slot4 s[4];
+        int cnt4 = collapseSlots(nodes, n, s);
+        float lo[3][4], hi[3][4];
+        uint32_t ch[4];
+        for (int k = 0; k < 4; k++) {
+            if (k < cnt4) {
+                ch[k] = (s[k].ref & RT_LEAF) ? s[k].ref : newId[s[k].ref];
+                for (int a = 0; a < 3; a++) { lo[a][k] = s[k].lo[a]; hi[a][k] = s[k].hi[a]; }
+            } else {
+                ch[k] = 0xffffffffu;
+                for (int a = 0; a < 3; a++) { lo[a][k] = INFINITY; hi[a][k] = -INFINITY; }
+            }
         }
+        float4* o = nodes4 + 8 * (size_t)newId[n];
+        for (int a = 0; a < 3; a++) {
+            o[a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+            o[3 + a] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+        }
+        o[6] = make_float4(__uint_as_float(ch[0]), __uint_as_float(ch[1]), __uint_as_float(ch[2]), __uint_as_float(ch[3]));
+        o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    float4* o = nodes4 + 8 * (size_t)newId[n];
-    for (int a = 0; a < 3; a++) {
-        o[a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
-        o[3 + a] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
-    }
-    o[6] = make_float4(__uint_as_float(ch[0]), __uint_as_float(ch[1]), __uint_as_float(ch[2]), __uint_as_float(ch[3]));
-    o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// Exclusive scan of n uint32; sums holds cdiv(n, kScanChunk) entries.
+void scanExclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* sums, hipStream_t stream) {
+    const uint32_t chunks = cdiv(n, kScanChunk);
+    k_scan_reduce<<<chunks, kBlock, 0, stream>>>(in, n, sums);
+    k_scan_sums<<<1, 1024, 0, stream>>>(sums, chunks);
+    k_scan_apply<<<chunks, kBlock, 0, stream>>>(in, n, sums, out);
+}
 
 }  // namespace
 
 namespace rti {
 
-// Collapses the binary LBVH (s->nodes) into the 4-wide tree s->nodes4.
-static bool collapseBVH4(rt_device_scene* s, uint32_t nb, hipStream_t stream) {
-    uint32_t *keep = nullptr, *newId = nullptr, *fa = nullptr, *fb = nullptr, *cnt = nullptr;
-    bool ok = hipCheck(hipMalloc(&keep, sizeof(uint32_t) * nb), "hipMalloc") &&
-              hipCheck(hipMalloc(&newId, sizeof(uint32_t) * nb), "hipMalloc") &&
-              hipCheck(hipMalloc(&fa, sizeof(uint32_t) * nb), "hipMalloc") &&
-              hipCheck(hipMalloc(&fb, sizeof(uint32_t) * nb), "hipMalloc") &&
-              hipCheck(hipMalloc(&cnt, sizeof(uint32_t)), "hipMalloc");
+size_t carveLbvhScratch(uint32_t n, char* base, LbvhScratch* out) {
+    const size_t nTiles = cdiv(n, kTile), nb = std::min<uint32_t>(1024u, cdiv(n, kBlock));
+    const size_t nNodes = n > 1 ? n - 1 : 1;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char* p = base ? base + off : nullptr;
+        off += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    LbvhScratch s;
+    s.keysB = reinterpret_cast<unsigned long long*>(take(8 * (size_t)n));
+    s.valsB = reinterpret_cast<uint32_t*>(take(4 * (size_t)n));
+    s.blockHist = reinterpret_cast<uint32_t*>(take(4 * 256 * nTiles));
+    s.digitTotal = reinterpret_cast<uint32_t*>(take(4 * 256));
+    s.partial = reinterpret_cast<float*>(take(4 * 6 * nb));
+    s.box = reinterpret_cast<float*>(take(4 * 6));
+    s.parentI = reinterpret_cast<uint32_t*>(take(4 * nNodes));
+    s.parentL = reinterpret_cast<uint32_t*>(take(4 * (size_t)n));
+    s.range = reinterpret_cast<uint2*>(take(8 * nNodes));
+    s.counters = reinterpret_cast<uint32_t*>(take(4 * nNodes));
+    s.height = reinterpret_cast<uint32_t*>(take(4 * nNodes));
+    s.stops = reinterpret_cast<uint32_t*>(take(8 * nNodes));
+    s.numStops = reinterpret_cast<uint32_t*>(take(4));
+    if (out) *out = s;
+    return off;
+}
+
+// Binary LBVH of n world-space triangles (3 float4 per primitive): centroid
+// bounds, Morton codes, radix sort, Karras topology and refit. Launches only:
+// no allocation, no host synchronisation. Outputs codes, order (n entries),
+// nodes (max(n - 1, 1) internal nodes); with leafTris non-null the triangles
+// are also gathered into Morton order and the refit reads them there. The
+// scene's box lands in sc.box, the tree's height in sc.height[0].
+hipError_t launchBinaryBuild(const float4* primTris, uint32_t n, unsigned long long* codes, uint32_t* order,
+                             float4* nodes, float4* leafTris, const LbvhScratch& sc, hipStream_t stream) {
+    const uint32_t nTiles = cdiv(n, kTile);
+    const uint32_t nbBounds = std::min<uint32_t>(1024u, cdiv(n, kBlock));
+    k_bounds_partial<<<nbBounds, kBlock, 0, stream>>>(primTris, n, sc.partial);
+    k_bounds_final<<<1, kBlock, 0, stream>>>(sc.partial, nbBounds, sc.box);
+    k_morton<<<cdiv(n, kBlock), kBlock, 0, stream>>>(primTris, n, sc.box, codes, order);
+    unsigned long long *kin = codes, *kout = sc.keysB;
+    uint32_t *vin = order, *vout = sc.valsB;
+    for (int pass = 0; pass < 8; pass++) {
+        const int shift = pass * 8;
+        k_radix_hist<<<nTiles, kBlock, 0, stream>>>(kin, n, shift, nTiles, sc.blockHist);
+        k_radix_digit_scan<<<256, kBlock, 0, stream>>>(sc.blockHist, nTiles, sc.digitTotal);
+        k_radix_scatter<<<nTiles, kBlock, 0, stream>>>(kin, vin, n, shift, nTiles, sc.blockHist, sc.digitTotal, kout,
+                                                       vout);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    // 8 passes: the sorted data is back in codes / order
+    if (n == 1) {
+        k_single_leaf<<<1, 1, 0, stream>>>(primTris, nodes, sc.height);
+        if (leafTris) k_gather_leaves<<<1, kBlock, 0, stream>>>(primTris, order, n, leafTris);
+        return hipGetLastError();
+    }
+    k_karras<<<cdiv(n - 1, kBlock), kBlock, 0, stream>>>(codes, (int)n, nodes, sc.parentI, sc.parentL, sc.range,
+                                                         sc.counters, sc.numStops);
+    const float4* tris = primTris;
+    const uint32_t* refitOrder = order;
+    if (leafTris) {
+        k_gather_leaves<<<cdiv(n, kBlock), kBlock, 0, stream>>>(primTris, order, n, leafTris);
+        tris = leafTris;
+        refitOrder = nullptr;
+    }
+    k_refit_block<<<cdiv(n, kRefitLeaves), kRefitThreads, 0, stream>>>(tris, refitOrder, n, nodes, sc.parentI,
+                                                                       sc.parentL, sc.range, sc.height, sc.stops,
+                                                                       sc.numStops);
+    k_refit_top<<<1, kRefitThreads, 0, stream>>>(nodes, sc.parentI, sc.counters, sc.height, sc.stops, sc.numStops);
+    return hipGetLastError();
+}
+
+// Scratch of the 4-wide collapse, allocated before the build starts.
+struct CollapseScratch {
+    static constexpr uint32_t kMaxLevels = 4096;
+    uint32_t *keep = nullptr, *newId = nullptr, *kept = nullptr, *cnt = nullptr, *sums = nullptr;
+    bool alloc(uint32_t nb) {
+        return hipCheck(hipMalloc(&keep, sizeof(uint32_t) * nb), "hipMalloc") &&
+               hipCheck(hipMalloc(&newId, sizeof(uint32_t) * nb), "hipMalloc") &&
+               hipCheck(hipMalloc(&kept, sizeof(uint32_t) * nb), "hipMalloc") &&
+               hipCheck(hipMalloc(&cnt, sizeof(uint32_t) * (kMaxLevels + 2)), "hipMalloc") &&
+               hipCheck(hipMalloc(&sums, sizeof(uint32_t) * cdiv(nb, kScanChunk)), "hipMalloc");
+    }
+    ~CollapseScratch() {
+        (void)hipFree(keep); (void)hipFree(newId); (void)hipFree(kept); (void)hipFree(cnt); (void)hipFree(sums);
+    }
+};
+
+// Collapses the binary LBVH (s->nodes) into the 4-wide tree s->nodes4 (allocated
+// for nb nodes: the kept nodes are a subset of the binary internal nodes).
+// maxLevels bounds the 4-wide depth (the binary height does: every 4-wide
+// level consumes at least one binary level).
+static bool collapseBVH4(rt_device_scene* s, uint32_t nb, uint32_t maxLevels, CollapseScratch& cs, hipEvent_t done,
+                         hipStream_t stream) {
+    if (maxLevels > CollapseScratch::kMaxLevels) {
+        setError("binary LBVH height " + std::to_string(maxLevels) + " exceeds the collapse's level limit");
+        return false;
+    }
+    uint32_t *keep = cs.keep, *newId = cs.newId, *kept = cs.kept, *cnt = cs.cnt, *sums = cs.sums;
+    const uint32_t levels = maxLevels + 1;
+    bool ok = true;
     uint32_t depth4 = 0, total = 0;
     if (ok) {
-        uint32_t one = 1u, zero = 0u;
-        ok = hipCheck(hipMemsetAsync(keep, 0, sizeof(uint32_t) * nb, stream), "memset") &&
-             hipCheck(hipMemcpyAsync(keep, &one, 4, hipMemcpyHostToDevice, stream), "copy") &&
-             hipCheck(hipMemcpyAsync(fa, &zero, 4, hipMemcpyHostToDevice, stream), "copy");
-        uint32_t nf = 1;
-        while (ok && nf > 0) {
-            depth4++;
-            ok = hipCheck(hipMemsetAsync(cnt, 0, 4, stream), "memset");
-            if (!ok) break;
-            k_collapse_mark<<<cdiv(nf, kBlock), kBlock, 0, stream>>>(s->nodes, fa, nf, keep, fb, cnt);
-            ok = hipCheck(hipGetLastError(), "collapse mark") &&
-                 hipCheck(hipMemcpyAsync(&nf, cnt, 4, hipMemcpyDeviceToHost, stream), "copy") &&
-                 hipCheck(hipStreamSynchronize(stream), "sync");
-            std::swap(fa, fb);
-        }
-    }
-    if (ok) {
-        ok = hipCheck(hipMemcpyAsync(newId, keep, sizeof(uint32_t) * nb, hipMemcpyDeviceToDevice, stream), "copy");
-        k_scan_single<<<1, 1024, 0, stream>>>(newId, nb);
+        // keep = {1, 0, 0, ...}, level 0 = {root}, counts {1, 0, 0, ...}
+        k_fill_u32<<<cdiv(nb, kBlock), kBlock, 0, stream>>>(keep, nb, 0u);
+        k_fill_u32<<<1, 1, 0, stream>>>(keep, 1, 1u);
+        k_fill_u32<<<1, 1, 0, stream>>>(kept, 1, 0u);
+        k_fill_u32<<<cdiv(levels + 1, kBlock), kBlock, 0, stream>>>(cnt, levels + 1, 0u);
+        k_fill_u32<<<1, 1, 0, stream>>>(cnt, 1, 1u);
+        // grid-stride over the frontier: an empty level costs one small launch
+        const uint32_t grid = std::max(1u, std::min(512u, cdiv(nb, kBlock)));
+        for (uint32_t l = 0; l < levels; l++)
+            k_collapse_mark<<<grid, kBlock, 0, stream>>>(s->nodes, kept, cnt, l, keep);
+        scanExclusive(keep, newId, nb, sums, stream);
+        k_collapse_emit<<<std::max(1u, std::min(2048u, cdiv(nb, kBlock))), kBlock, 0, stream>>>(s->nodes, kept, cnt,
+                                                                                               levels, newId, s->nodes4);
+        std::vector<uint32_t> h(levels + 1, 0u);
         uint32_t lastKeep = 0, lastId = 0;
-        ok = ok && hipCheck(hipMemcpyAsync(&lastKeep, keep + nb - 1, 4, hipMemcpyDeviceToHost, stream), "copy") &&
+        ok = hipCheck(hipGetLastError(), "collapse launch") && hipCheck(hipEventRecord(done, stream), "event") &&
+             hipCheck(hipMemcpyAsync(h.data(), cnt, sizeof(uint32_t) * (levels + 1), hipMemcpyDeviceToHost, stream), "copy") &&
+             hipCheck(hipMemcpyAsync(&lastKeep, keep + nb - 1, 4, hipMemcpyDeviceToHost, stream), "copy") &&
              hipCheck(hipMemcpyAsync(&lastId, newId + nb - 1, 4, hipMemcpyDeviceToHost, stream), "copy") &&
              hipCheck(hipStreamSynchronize(stream), "sync");
+        for (uint32_t l = 0; l <= levels; l++)
+            if (h[l]) depth4 = l + 1;
+        if (ok && h[levels] != 0) {
+            setError("4-wide collapse did not finish within the binary tree's height");
+            ok = false;
+        }
         total = lastId + lastKeep;
-    }
-    if (ok) {
-        ok = hipCheck(hipMalloc(&s->nodes4, sizeof(float4) * 8 * (size_t)total), "hipMalloc");
-        if (ok) s->deviceBytes += sizeof(float4) * 8 * (size_t)total;
-    }
-    if (ok) {
-        k_collapse_emit<<<cdiv(nb, kBlock), kBlock, 0, stream>>>(s->nodes, nb, keep, newId, s->nodes4);
-        ok = hipCheck(hipGetLastError(), "collapse emit") && hipCheck(hipStreamSynchronize(stream), "sync");
     }
     s->numNodes4 = total;
     s->bvh4Depth = depth4;
-    (void)hipFree(keep); (void)hipFree(newId); (void)hipFree(fa); (void)hipFree(fb); (void)hipFree(cnt);
     return ok;
 }
-
-}  // namespace rti
-
-namespace rti {
 
 bool buildLBVH(rt_device_scene* s, const std::vector<uint32_t>& instPrimOffset, hipStream_t stream) {
     const uint32_t n = s->numPrims;
@@ -459,88 +727,46 @@ bool buildLBVH(rt_device_scene* s, const std::vector<uint32_t>& instPrimOffset, 
     if (!alloc((void**)&s->order, sizeof(uint32_t) * n)) return false;
     const uint32_t nNodes = n > 1 ? n - 1 : 1;
     if (!alloc((void**)&s->nodes, sizeof(float4) * 4 * (size_t)nNodes)) return false;
+    if (!alloc((void**)&s->nodes4, sizeof(float4) * 8 * (size_t)nNodes)) return false;
 
-    // scratch
-    uint32_t* dOffsets = nullptr; unsigned long long* keysB = nullptr; uint32_t* valsB = nullptr;
-    uint32_t* blockHist = nullptr; float* partial = nullptr; float* box = nullptr;
-    uint32_t *parentI = nullptr, *parentL = nullptr, *counters = nullptr, *dDepth = nullptr;
-    const uint32_t nTiles = cdiv(n, kTile);
-    const uint32_t nbBounds = std::min<uint32_t>(1024u, cdiv(n, kBlock));
-    bool ok = true;
-    ok = ok && hipCheck(hipMalloc(&dOffsets, sizeof(uint32_t) * std::max<size_t>(1, instPrimOffset.size())), "hipMalloc");
-    ok = ok && hipCheck(hipMalloc(&keysB, sizeof(unsigned long long) * n), "hipMalloc");
-    ok = ok && hipCheck(hipMalloc(&valsB, sizeof(uint32_t) * n), "hipMalloc");
-    ok = ok && hipCheck(hipMalloc(&blockHist, sizeof(uint32_t) * 256 * (size_t)nTiles), "hipMalloc");
-    ok = ok && hipCheck(hipMalloc(&partial, sizeof(float) * 6 * nbBounds), "hipMalloc");
-    ok = ok && hipCheck(hipMalloc(&box, sizeof(float) * 6), "hipMalloc");
-    ok = ok && hipCheck(hipMalloc(&parentI, sizeof(uint32_t) * nNodes), "hipMalloc");
-    ok = ok && hipCheck(hipMalloc(&parentL, sizeof(uint32_t) * n), "hipMalloc");
-    ok = ok && hipCheck(hipMalloc(&counters, sizeof(uint32_t) * nNodes), "hipMalloc");
-    ok = ok && hipCheck(hipMalloc(&dDepth, sizeof(uint32_t)), "hipMalloc");
+    uint32_t* dOffsets = nullptr;
+    char* scratch = nullptr;
+    LbvhScratch sc;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    bool ok = hipCheck(hipMalloc(&dOffsets, sizeof(uint32_t) * std::max<size_t>(1, instPrimOffset.size())), "hipMalloc") &&
+              hipCheck(hipMalloc(&scratch, carveLbvhScratch(n, nullptr, nullptr)), "hipMalloc");
+    CollapseScratch cs;
+    ok = ok && cs.alloc(nNodes);
+    for (int k = 0; ok && k < 3; k++) ok = hipCheck(hipEventCreate(&ev[k]), "hipEventCreate");
+    if (ok) carveLbvhScratch(n, scratch, &sc);
     if (ok && !instPrimOffset.empty())
         ok = hipCheck(hipMemcpyAsync(dOffsets, instPrimOffset.data(), sizeof(uint32_t) * instPrimOffset.size(),
                                      hipMemcpyHostToDevice, stream), "hipMemcpyAsync");
     if (ok) {
+        ok = hipCheck(hipEventRecord(ev[0], stream), "event");
         k_world_tris<<<cdiv(n, kBlock), kBlock, 0, stream>>>(s->dev, dOffsets, s->numInstances, s->primTris);
-        k_bounds_partial<<<nbBounds, kBlock, 0, stream>>>(s->primTris, n, partial);
-        k_bounds_final<<<1, kBlock, 0, stream>>>(partial, nbBounds, box);
-        k_morton<<<cdiv(n, kBlock), kBlock, 0, stream>>>(s->primTris, n, box, s->codes, s->order);
-        unsigned long long *kin = s->codes, *kout = keysB;
-        uint32_t *vin = s->order, *vout = valsB;
-        for (int pass = 0; pass < 8; pass++) {
-            int shift = pass * 8;
-            k_radix_hist<<<nTiles, kBlock, 0, stream>>>(kin, n, shift, nTiles, blockHist);
-            k_scan_single<<<1, 1024, 0, stream>>>(blockHist, 256 * nTiles);
-            k_radix_scatter<<<nTiles, kBlock, 0, stream>>>(kin, vin, n, shift, nTiles, blockHist, kout, vout);
-            std::swap(kin, kout); std::swap(vin, vout);
-        }
-        // 8 passes: sorted data is back in s->codes / s->order
-        ok = hipCheck(hipGetLastError(), "radix sort launch");
-    }
-    if (ok) {
-        if (n == 1) {
-            float4 h[8];
-            ok = hipCheck(hipMemcpyAsync(h, s->primTris, sizeof(float4) * 3, hipMemcpyDeviceToHost, stream), "copy") &&
-                 hipCheck(hipStreamSynchronize(stream), "sync");
-            if (ok) {
-                float lo[3] = {std::min(std::min(h[0].x, h[1].x), h[2].x), std::min(std::min(h[0].y, h[1].y), h[2].y),
-                               std::min(std::min(h[0].z, h[1].z), h[2].z)};
-                float hi[3] = {std::max(std::max(h[0].x, h[1].x), h[2].x), std::max(std::max(h[0].y, h[1].y), h[2].y),
-                               std::max(std::max(h[0].z, h[1].z), h[2].z)};
-                float nf[16] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], INFINITY, INFINITY, INFINITY,
-                                -INFINITY, -INFINITY, -INFINITY, 0, 0, 0, 0};
-                uint32_t leaf = RT_LEAF;
-                std::memcpy(&nf[12], &leaf, 4); std::memcpy(&nf[13], &leaf, 4);
-                ok = hipCheck(hipMemcpyAsync(s->nodes, nf, sizeof(nf), hipMemcpyHostToDevice, stream), "copy") &&
-                     hipCheck(hipMemcpyAsync(s->leafTris, s->primTris, sizeof(float4) * 3, hipMemcpyDeviceToDevice, stream), "copy") &&
-                     hipCheck(hipStreamSynchronize(stream), "sync");
-                s->bvhDepth = 1;
-            }
-        } else {
-            ok = hipCheck(hipMemsetAsync(counters, 0, sizeof(uint32_t) * nNodes, stream), "memset") &&
-                 hipCheck(hipMemsetAsync(dDepth, 0, sizeof(uint32_t), stream), "memset");
-            if (ok) {
-                k_karras<<<cdiv(n - 1, kBlock), kBlock, 0, stream>>>(s->codes, (int)n, s->nodes, parentI, parentL);
-                k_gather_leaves<<<cdiv(n, kBlock), kBlock, 0, stream>>>(s->primTris, s->order, n, s->leafTris);
-                k_refit<<<cdiv(n, kBlock), kBlock, 0, stream>>>(s->leafTris, n, s->nodes, parentI, parentL, counters);
-                k_depth<<<cdiv(n, kBlock), kBlock, 0, stream>>>(parentI, parentL, n, dDepth);
-                ok = hipCheck(hipGetLastError(), "lbvh launch");
-            }
-            uint32_t depth = 0;
-            if (ok) ok = hipCheck(hipMemcpyAsync(&depth, dDepth, 4, hipMemcpyDeviceToHost, stream), "copy") &&
-                         hipCheck(hipStreamSynchronize(stream), "sync");
-            s->bvhDepth = depth;
-        }
+        ok = ok && hipCheck(launchBinaryBuild(s->primTris, n, s->codes, s->order, s->nodes, s->leafTris, sc, stream),
+                            "lbvh launch") &&
+             hipCheck(hipEventRecord(ev[1], stream), "event");
     }
     if (ok) {
         float hb[6] = {0, 0, 0, 0, 0, 0};
-        ok = hipCheck(hipMemcpyAsync(hb, box, sizeof(hb), hipMemcpyDeviceToHost, stream), "copy") &&
+        uint32_t depth = 0;
+        ok = hipCheck(hipMemcpyAsync(hb, sc.box, sizeof(hb), hipMemcpyDeviceToHost, stream), "copy") &&
+             hipCheck(hipMemcpyAsync(&depth, sc.height, 4, hipMemcpyDeviceToHost, stream), "copy") &&
              hipCheck(hipStreamSynchronize(stream), "sync");
         for (int a = 0; a < 3; a++) { s->sceneLo[a] = hb[a]; s->sceneHi[a] = hb[3 + a]; }
+        s->bvhDepth = depth;
     }
-    (void)hipFree(dOffsets); (void)hipFree(keysB); (void)hipFree(valsB); (void)hipFree(blockHist); (void)hipFree(partial); (void)hipFree(box);
-    (void)hipFree(parentI); (void)hipFree(parentL); (void)hipFree(counters); (void)hipFree(dDepth);
-    if (ok) ok = collapseBVH4(s, nNodes, stream);
+    (void)hipFree(dOffsets);
+    (void)hipFree(scratch);
+    if (ok) ok = collapseBVH4(s, nNodes, s->bvhDepth, cs, ev[2], stream);
+    if (ok) {
+        ok = hipCheck(hipEventElapsedTime(&s->bvhBinaryMs, ev[0], ev[1]), "event time") &&
+             hipCheck(hipEventElapsedTime(&s->bvhBuildMs, ev[0], ev[2]), "event time");
+    }
+    for (int k = 0; k < 3; k++)
+        if (ev[k]) (void)hipEventDestroy(ev[k]);
     // a 4-wide node pushes at most three children: the stack never holds more
     // than 3 (depth - 1) entries
     if (ok && 3 * (uint64_t)s->bvh4Depth > RT_LDS_STACK + RT_SPILL_STACK) {

@@ -224,6 +224,8 @@ rt_status rt_device_scene_info(const rt_device_scene* s, rt_scene_info* out) {
     out->bvh4Depth = s->bvh4Depth;
     for (int a = 0; a < 3; a++) { out->sceneLo[a] = s->sceneLo[a]; out->sceneHi[a] = s->sceneHi[a]; }
     out->deviceBytes = s->deviceBytes;
+    out->bvhBuildMs = s->bvhBuildMs;
+    out->bvhBinaryMs = s->bvhBinaryMs;
     return RT_OK;
 }
 

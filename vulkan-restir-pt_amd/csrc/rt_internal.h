@@ -28,6 +28,7 @@ struct rt_device_scene {
     rt_light_table_entry* lightTable = nullptr;
     uint32_t numPrims = 0, numLights = 0, numInstances = 0;
     uint32_t bvhDepth = 0;
+    float bvhBinaryMs = 0.0f, bvhBuildMs = 0.0f;  // GPU time of the binary build / of the whole build
     float sceneLo[3] = {0, 0, 0}, sceneHi[3] = {0, 0, 0};
     size_t deviceBytes = 0;
     // Pixel queues of the persistent passes, handed out round robin: up to
@@ -74,6 +75,34 @@ bool hipCheck(hipError_t e, const char* what);
 
 // LBVH build over dev scene arrays already uploaded; fills nodes/leafTris/codes/order.
 bool buildLBVH(rt_device_scene* s, const std::vector<uint32_t>& instPrimOffset, hipStream_t stream);
+
+// Scratch of one binary build, carved from one buffer (256-B aligned pieces).
+struct LbvhScratch {
+    unsigned long long* keysB = nullptr;
+    uint32_t* valsB = nullptr;
+    uint32_t* blockHist = nullptr;   // 256 x tiles
+    uint32_t* digitTotal = nullptr;  // 256
+    float* partial = nullptr;        // 6 x bounds blocks
+    float* box = nullptr;            // 6
+    uint32_t* parentI = nullptr;     // internal nodes
+    uint32_t* parentL = nullptr;     // leaves
+    uint2* range = nullptr;          // internal nodes
+    uint32_t* counters = nullptr;    // internal nodes
+    uint32_t* height = nullptr;      // internal nodes
+    uint32_t* stops = nullptr;       // 2 x internal nodes
+    uint32_t* numStops = nullptr;    // 1
+};
+
+// Bytes of the scratch of an n-primitive binary build; with base non-null also
+// carves it into *out.
+size_t carveLbvhScratch(uint32_t n, char* base, LbvhScratch* out);
+// Binary LBVH of n world-space triangles (3 float4 per primitive) into codes,
+// order (n) and nodes (max(n - 1, 1) x 4 float4): kernel launches only, no
+// allocation and no host synchronisation. With leafTris non-null the
+// triangles are also gathered into Morton order. The centroid box lands in
+// sc.box, the tree's height in sc.height[0].
+hipError_t launchBinaryBuild(const float4* primTris, uint32_t n, unsigned long long* codes, uint32_t* order,
+                             float4* nodes, float4* leafTris, const LbvhScratch& sc, hipStream_t stream);
 
 // Counters of the counting kernel builds.
 extern bool gCountersEnabled;

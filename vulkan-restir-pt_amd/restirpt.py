@@ -146,7 +146,8 @@ class BatchTuning(C.Structure):
 class SceneInfo(C.Structure):
     _fields_ = [("numPrimitives", C.c_uint32), ("numLights", C.c_uint32), ("bvhNodes", C.c_uint32),
                 ("bvhDepth", C.c_uint32), ("sceneLo", C.c_float * 3), ("sceneHi", C.c_float * 3),
-                ("deviceBytes", C.c_size_t), ("bvh4Nodes", C.c_uint32), ("bvh4Depth", C.c_uint32)]
+                ("deviceBytes", C.c_size_t), ("bvh4Nodes", C.c_uint32), ("bvh4Depth", C.c_uint32),
+                ("bvhBuildMs", C.c_float), ("bvhBinaryMs", C.c_float)]
 
 
 assert C.sizeof(Material) == 32 and C.sizeof(MeshVertex) == 32 and C.sizeof(ObjectInstance) == 224
