@@ -173,3 +173,135 @@ def test_batched_learned_forms_match_single_views():
         assert b"more than one view" in rt.core().rt_last_error()
     finally:
         L.rt_batch_destroy(b)
+
+
+def same_bytes(a, b):
+    """Byte equality of two device buffers, on the device."""
+    return bool(torch.equal(a, b))
+
+
+def assert_views_equal(i, a, z, guided=False):
+    c = a.cur
+    for name, x, y in (("depthNormal", a.dn[c], z.dn[c]), ("albedoMatId", a.am[c], z.am[c]), ("motion", a.mv, z.mv),
+                       ("reservoirs", a.resv[c], z.resv[c]), ("temp reservoirs", a.temp, z.temp),
+                       ("indirect output", a.indirect, z.indirect)):
+        assert same_bytes(x, y), f"view {i}: {name} differ"
+    if guided:
+        assert same_bytes(a.lobes[c], z.lobes[c]), f"view {i}: lobe planes differ"
+        assert same_bytes(a.samples, z.samples), f"view {i}: training records differ"
+
+
+def config2_specs():
+    """The batched workload's views (as bench.py draws them): 64 square views, sizes log-uniform in
+    [128, 1024] (multiples of 8), depth U{3..8}, scenes round robin, a camera-walk offset each."""
+    g = np.random.default_rng(2)
+    specs = []
+    for i in range(64):
+        size = int(np.exp(g.uniform(np.log(128), np.log(1024)))) // 8 * 8
+        specs.append((i % 4, size, int(g.integers(3, 9)), int(g.integers(0, 64))))
+    return specs
+
+
+def test_config2_batch_matches_single_views():
+    """The batched workload at full size: 64 views (16.0M pixels) over the four scenes of ~20k, ~225k,
+    ~1.1M and ~3.0M triangles in one rt_batch_frame per frame, against the same views rendered one by
+    one: G-buffers, reservoirs and outputs byte-identical over two frames."""
+    hosts = [rt.HostScene(0, 2), rt.HostScene(3, 2), rt.HostScene(1, 1), rt.HostScene(4, 2)]
+    devs = [rt.DeviceScene(h.arrays) for h in hosts]
+    specs = config2_specs()
+    assert sum(n * n for _, n, _, _ in specs) > 15_000_000
+    single = [gpuhelp.GrisRun(devs[si], n, n) for si, n, _, _ in specs]
+    batched = [gpuhelp.GrisRun(devs[si], n, n) for si, n, _, _ in specs]
+    sets = [rt.settings(shift=2, temporal=True, spatial=True, cap=20, max_depth=d, neighbors=3, radius=20.0)
+            for _, _, d, _ in specs]
+    L = rt.core()
+    b = C.c_void_p()
+    rt.check(L.rt_batch_create(len(specs), C.byref(b)), "batch create")
+    try:
+        for frame in range(2):
+            print(f"[gpu] config-2 batch frame {frame}", flush=True)
+            views = (rt.View * len(specs))()
+            for i, (si, n, _, f0) in enumerate(specs):
+                cams = hosts[si].camera((f0 + frame) % 64, n, n)
+                r = single[i]
+                r.gbuffer(cams[0])
+                r.path_trace(cams, sets[i])
+                r.temporal(cams, sets[i])
+                r.spatial(cams, sets[i])
+                views[i] = make_view(batched[i], devs[si], cams, sets[i])
+            rt.check(L.rt_batch_frame(b, views, len(specs), gpuhelp.stream()), "batch frame")
+            torch.cuda.synchronize()
+            for i in range(len(specs)):
+                assert_views_equal(i, single[i], batched[i])
+                single[i].flip()
+                batched[i].flip()
+    finally:
+        L.rt_batch_destroy(b)
+
+
+def test_full_form_4k_batch_matches_single_views():
+    """The full-form workload at full size: a 3840x2160 view of the ~4.8M-triangle scene at depth 8
+    with the learned target, the learned proposal and unbiased reuse (each network trained every
+    frame) and 16 plain 512x512 views in one rt_batch_frame, against the same views rendered alone:
+    G-buffers, reservoirs, lobe planes, training records, outputs and both networks' parameters
+    byte-identical over two frames; the 4K image is finite and non-negative and its reservoirs
+    respect the M-cap."""
+    from test_gpu_guide import GuidedRun
+    from test_gpu_learned import scene_bounds
+
+    hs = rt.HostScene(5, 4)
+    ds = rt.DeviceScene(hs.arrays)
+    assert ds.info().numPrimitives > 4_500_000
+    W, H = 3840, 2160
+    lo, hi = scene_bounds(ds)
+    nets = [(rt.Mlp(rt.mlp_config(lo, hi, log2_table=19, seed=4)),
+             rt.Mlp(rt.mlp_config(lo, hi, log2_table=19, seed=5), guide=True)) for _ in range(2)]
+    s4k = rt.settings(shift=2, temporal=True, spatial=True, cap=20, max_depth=8, neighbors=3, radius=30.0,
+                      learned=True, unbiased=True, guided=True)
+    s512 = rt.settings(shift=2, temporal=True, spatial=True, cap=20, max_depth=8, neighbors=3, radius=30.0)
+    offsets = np.random.default_rng(4).integers(0, 64, 16)
+    single = [GuidedRun(ds, W, H)] + [gpuhelp.GrisRun(ds, 512, 512) for _ in range(16)]
+    batched = [GuidedRun(ds, W, H)] + [gpuhelp.GrisRun(ds, 512, 512) for _ in range(16)]
+    L = rt.core()
+    b = C.c_void_p()
+    rt.check(L.rt_batch_create(17, C.byref(b)), "batch create")
+    try:
+        for frame in range(2):
+            print(f"[gpu] 4K full-form batch frame {frame}", flush=True)
+            views = (rt.View * 17)()
+            for i in range(17):
+                r = single[i]
+                if i == 0:
+                    cams = hs.camera(frame, W, H)
+                    mlp, guide = nets[0]
+                    r.gbuffer(cams[0])
+                    r.guide_refresh(guide, cams)
+                    r.path_trace(cams, s4k)
+                    r.learned_refresh(mlp, cams)
+                    r.temporal(cams, s4k)
+                    r.spatial(cams, s4k)
+                    rt.check(L.rt_mlp_train_frame(mlp.handle, None, gpuhelp.stream()), "train frame")
+                    r.guide_train(guide, cams)
+                    v = make_view(batched[0], ds, cams, s4k)
+                    v.target, v.guide = _handle(nets[1][0].handle), _handle(nets[1][1].handle)
+                else:
+                    cams = hs.camera(int(offsets[i - 1] + frame) % 64, 512, 512)
+                    r.frame(cams, s512)
+                    v = make_view(batched[i], ds, cams, s512)
+                views[i] = v
+            rt.check(L.rt_batch_frame(b, views, 17, gpuhelp.stream()), "batch frame")
+            torch.cuda.synchronize()
+            for i in range(17):
+                assert_views_equal(i, single[i], batched[i], guided=(i == 0))
+            for na, nz in zip(nets[0], nets[1]):
+                for pa, pz in zip(na.params(), nz.params()):
+                    assert np.array_equal(pa.view(np.uint32), pz.view(np.uint32)), "network parameters differ"
+            out = batched[0].output(1)
+            assert np.isfinite(out).all() and (out[..., :3] >= 0).all()
+            assert out[..., :3].mean() > 0
+            m = batched[0].reservoirs(batched[0].resv[batched[0].cur])["sampleCount"]
+            assert m.max() <= 20
+            for r in single + batched:
+                r.flip()
+    finally:
+        L.rt_batch_destroy(b)

@@ -30,26 +30,34 @@ def bits_equal(a, b):
     return np.array_equal(np.ascontiguousarray(a).view(np.uint32), np.ascontiguousarray(b).view(np.uint32))
 
 
-@pytest.mark.parametrize("config,W,H,unbiased,guided",
-                         [(0, 96, 64, False, False), (1, 160, 96, False, False), (1, 160, 96, True, False),
-                          (1, 160, 96, True, True)],
-                         ids=["boxroom", "terrain", "terrain-unbiased", "terrain-unbiased-proposal"])
-def test_learned_gris_parity(config, W, H, unbiased, guided):
+@pytest.mark.parametrize("config,W,H,unbiased,guided,log2,depth",
+                         [(0, 96, 64, False, False, 16, 6), (1, 160, 96, False, False, 16, 6),
+                          (1, 160, 96, True, False, 16, 6), (1, 160, 96, True, True, 16, 6),
+                          (1, 640, 360, False, False, 19, 6), (5, 128, 72, True, True, 19, 8)],
+                         ids=["boxroom", "terrain", "terrain-unbiased", "terrain-unbiased-proposal",
+                              "terrain-640x360-table2^19", "scene4.8M-depth8-full-form"])
+def test_learned_gris_parity(config, W, H, unbiased, guided, log2, depth):
     """With guided: the learned vMF-mixture proposal on as well (the full form of the learned
-    estimator), its lobe planes taken from the GPU, its network trained between frames."""
-    print(f"[gpu] learned parity config {config} {W}x{H} unbiased {unbiased} proposal {guided}", flush=True)
+    estimator), its lobe planes taken from the GPU, its network trained between frames.
+    terrain-640x360-table2^19: the learned-target workload's network (2^19 entries per level) at a
+    view of more than 2^16 pixels, so the training batch takes every stride-th pixel (stride 4)
+    with a per-frame offset. scene4.8M-depth8-full-form: the full-form workload's scene (~4.8M
+    triangles, 64 emissive, roughness U[0.02, 1]) at depth 8 with the learned target, the learned
+    proposal and unbiased reuse, at a reduced view."""
+    print(f"[gpu] learned parity config {config} {W}x{H} unbiased {unbiased} proposal {guided} table 2^{log2} "
+          f"depth {depth}", flush=True)
     hs = rt.HostScene(config, 1)
     ds = rt.DeviceScene(hs.arrays)
     osc = orc.OracleScene(hs.arrays)
     lo, hi = scene_bounds(ds)
-    mlp = rt.Mlp(rt.mlp_config(lo, hi, log2_table=16, seed=3))
-    omlp = orc.OracleMlp(16, 16, 2048, lo, hi)
-    s = rt.settings(shift=2, temporal=True, spatial=True, max_depth=6, cap=20, radius=20.0, learned=True,
+    mlp = rt.Mlp(rt.mlp_config(lo, hi, log2_table=log2, seed=3))
+    omlp = orc.OracleMlp(log2, 16, 2048, lo, hi)
+    s = rt.settings(shift=2, temporal=True, spatial=True, max_depth=depth, cap=20, radius=20.0, learned=True,
                     unbiased=unbiased, guided=guided)
     if guided:
         from test_gpu_guide import GuidedRun
         run = GuidedRun(ds, W, H)
-        guide = rt.Mlp(rt.mlp_config(lo, hi, log2_table=16, seed=4), guide=True)
+        guide = rt.Mlp(rt.mlp_config(lo, hi, log2_table=log2, seed=4), guide=True)
     else:
         run = gpuhelp.GrisRun(ds, W, H)
     n = W * H
@@ -65,6 +73,8 @@ def test_learned_gris_parity(config, W, H, unbiased, guided):
     o_am = [None, None]
     cur = 0
     stride = max(1, (n + (1 << 16) - 1) >> 16)
+    if W * H > (1 << 16):
+        assert stride > 1
     for frame in range(3):
         print(f"[gpu] frame {frame}", flush=True)
         cams = hs.camera(frame, W, H)

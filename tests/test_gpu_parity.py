@@ -27,7 +27,7 @@ def close(a, b):
     return (d <= tol) | (np.isnan(a) & np.isnan(b))
 
 
-@pytest.fixture(scope="module", params=[0, 2], ids=["boxroom", "floor"])
+@pytest.fixture(scope="module", params=[0, 2, 3, 4], ids=["boxroom", "floor", "scene225k", "scene3.0M"])
 def scene(request):
     print(f"[gpu] scene config {request.param}", flush=True)
     hs = rt.HostScene(request.param, 0)
