@@ -146,11 +146,17 @@ RT_DEV bool traverse(const DevScene& S, const RayT& r, bool skipLights, TStack s
         cswap(k1, r1, k2, r2); cswap(k0, r0, k1, r1);
         cswap(k2, r2, k3, r3); cswap(k1, r1, k2, r2); cswap(k0, r0, k1, r1);
         uint32_t rs[4] = {r0, r1, r2, r3};
-        // leaves in order
+        // leaves in sorted order, one triangle test per loop trip: a wave runs
+        // as many trips as its lane with the most leaf hits (a leaf is tested
+        // by ~1 lane in 5 per node visit), not one test per slot position
+        uint32_t leafMask = 0;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            uint32_t ref = rs[q];
-            if (ref == RT_EMPTY || !(ref & RT_LEAF)) continue;
+        for (int q = 0; q < 4; q++)
+            if (rs[q] != RT_EMPTY && (rs[q] & RT_LEAF)) leafMask |= 1u << q;
+        while (leafMask) {
+            const uint32_t q = (uint32_t)__builtin_ctz(leafMask);
+            leafMask &= leafMask - 1u;
+            const uint32_t ref = q == 0 ? r0 : (q == 1 ? r1 : (q == 2 ? r2 : r3));
             uint32_t leaf = ref & ~RT_LEAF;
             const float4* tp = S.leafTris + 3 * (size_t)leaf;
             float4 a = tp[0], b = tp[1];
