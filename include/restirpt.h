@@ -206,6 +206,22 @@ rt_status rt_bvh_export(const rt_device_scene* scene, uint64_t* codes, uint32_t*
  * gives the count. */
 rt_status rt_bvh4_export(const rt_device_scene* scene, uint32_t* nodes28);
 
+/* Allocation-free variant of the binary LBVH build that rt_device_scene_create
+ * runs (and rt_bvh_export reads back). No counterpart in the reference: it is
+ * for callers that cannot allocate -- code under HIP graph capture, serving
+ * loops that rebuild dynamic geometry each frame. primTris: n world-space
+ * triangles on the device, 3 float4 each ({v0, instance custom index as float
+ * bits}, {v1, primitive index as float bits}, {v2, 0}). Writes the sorted
+ * Morton codes (n), the leaf -> primitive order (n) and the internal nodes
+ * (max(n - 1, 1) x 16 floats: the 14 words of rt_bvh_export, then 2 pad words)
+ * to caller-owned device buffers, bit-identical to the scene build of the
+ * same triangles. Stream ordered, no host synchronisation, no hipMalloc:
+ * its scratch is a static 8 MiB device array per GPU, so n is limited to
+ * [1, 131072] (RT_ERR_CAPACITY beyond) and calls that overlap on one GPU race.
+ * The device of the stream must be the current device. */
+rt_status rt_bvh_build_noalloc(const float* primTris, uint32_t n, uint64_t* codes, uint32_t* order, float* nodes,
+                               hipStream_t stream);
+
 /* Ray queries of src/shader/ray_query.glsl:6-70 (traceClosestHit /
  * traceShadow) over device arrays: rays8 = {ox, oy, oz, dx, dy, dz, tmin, tmax}.
  * mode 0 = closest hit (t, bary, instance custom index, primitive index;

@@ -178,6 +178,7 @@ def mlp_config(lo, hi, log2_table=19, base_res=16, max_res=2048, seed=3, lr=1e-2
 
 CORE_SYMBOLS = [
     "rt_last_error", "rt_device_scene_create", "rt_device_scene_destroy", "rt_device_scene_info", "rt_bvh_export", "rt_bvh4_export",
+    "rt_bvh_build_noalloc",
     "rt_trace_rays", "rt_gbuffer_trace", "rt_pt_plain", "rt_gris_path_trace", "rt_gris_temporal", "rt_gris_spatial",
     "rt_gris_render", "rt_gris_reservoir_bytes", "rt_gris_reservoir_reset", "rt_gris_reservoir_download",
     "rt_gris_reservoir_upload", "rt_gris_learned_download", "rt_gris_learned_upload", "rt_counters_enable",
@@ -214,6 +215,7 @@ def _declare_core(L):
         "rt_device_scene_info": [P, C.POINTER(SceneInfo)],
         "rt_bvh_export": [P, P, P, P],
         "rt_bvh4_export": [P, P],
+        "rt_bvh_build_noalloc": [P, u32, P, P, P, P],
         "rt_trace_rays": [P, P, u32, C.c_int, C.c_int, P, P, P, P, P],
         "rt_gbuffer_trace": [P, C.POINTER(Camera), P, P, P, P],
         "rt_pt_plain": [P, C.POINTER(Camera), C.POINTER(Frame), u32, u32, P],
