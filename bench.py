@@ -71,6 +71,13 @@ def parse():
     ap.add_argument("--lvp-frames", type=int, default=64, help="frames of the plain form in the equal-time run")
     ap.add_argument("--ref-frames", type=int, default=4096, help="frames of the converged reference image")
     ap.add_argument("--no-di", action="store_true", help="skip the ReSTIR DI leg")
+    ap.add_argument("--no-lbvh", action="store_true", help="skip the LBVH build leg")
+    ap.add_argument("--only-lbvh", action="store_true", help="run the LBVH build leg alone (for profilers)")
+    ap.add_argument("--only-plain", action="store_true", help="run the 256x256 plain path tracing leg alone")
+    ap.add_argument("--learned-study", action="store_true",
+                    help="run the paired equal-time study of every learned form alone (long)")
+    ap.add_argument("--study-sets", type=int, default=16)
+    ap.add_argument("--no-plain", action="store_true", help="skip the 256x256 plain path tracing leg")
     ap.add_argument("--di-kernel-loop", action="store_true", help="time ReSTIR DI frames only (for profilers)")
     return ap.parse_args()
 
@@ -430,6 +437,94 @@ def learned_vs_plain(rt, hs, ds, args):
                     f"+ MLP 2x64 FP16 (MFMA), trained online each frame on 2^16 reconnection samples, Adam lr 1e-2"}
 
 
+def learned_study(rt, hs, ds, args):
+    """Equal-time relMSE of every learned form against its plain counterpart, paired over
+    args.study_sets independent seed sets on the still view of learned_vs_plain (reference: the
+    GRIS estimator without reuse, args.ref_frames frames, FP64 mean). For each mode (no reuse, the
+    reference's resampling weights, unbiased reuse) the plain form renders args.lvp_frames frames
+    and each learned form (learned target, learned proposal, both) as many frames as fit the same
+    GPU time, its networks warm (trained on 128 earlier frames of the view with other seeds) and
+    training on. Per form: the geometric mean over sets of relMSE(plain) / relMSE(learned) with a
+    95 % t-interval on the mean log ratio, the median ratio, and the sets where the learned form is
+    better. A set's relMSE is dominated by a few fireflies, which is why the comparison is paired
+    and on the log scale."""
+    import math
+
+    import torch
+
+    W, H = args.width, args.height
+    base = hs.camera(0, W, H)
+    fr = TorchFrames(rt, ds, W, H)
+    radius = 30.0 * W / 1920.0
+    info = ds.info()
+    lo, hi = [float(v) for v in info.sceneLo], [float(v) for v in info.sceneHi]
+    ext = [max(h - l, 1e-3) for l, h in zip(lo, hi)]
+    bounds = ([l - 0.1 * e for l, e in zip(lo, ext)], [h + 0.1 * e for h, e in zip(hi, ext)])
+
+    def run(s, frames, seed, mlp=None, guide=None):
+        fr.reset()
+        acc = torch.zeros((H, W, 3), dtype=torch.float64, device="cuda")
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(frames)]
+        for i in range(frames):
+            fr.frame(static_cams(rt, base, i, seed), s, mlp, ev[i], guide)
+            acc += fr.out[..., :3].double()
+        torch.cuda.synchronize()
+        return acc / frames, sum(e[0].elapsed_time(e[1]) for e in ev) / frames
+
+    s_ref = rt.settings(shift=2, temporal=False, spatial=False, cap=20, max_depth=6, neighbors=3, radius=radius)
+    ref, _ = run(s_ref, args.ref_frames, 7)
+
+    def relmse(img):
+        return float((((img - ref) ** 2) / (ref ** 2 + 1e-2)).sum(-1).mean())
+
+    # two-sided 95 % Student t quantiles by degrees of freedom
+    tq = {5: 2.571, 7: 2.365, 9: 2.262, 11: 2.201, 15: 2.131, 19: 2.093, 23: 2.069, 31: 2.040}
+    sets = [1000 + 37 * k for k in range(args.study_sets)]
+    t95 = tq.get(len(sets) - 1, 2.0)
+    out = {"sets": len(sets), "frames_plain": args.lvp_frames}
+    for mode, unb, reuse in (("no_reuse", False, False), ("reference_weights", False, True), ("unbiased", True, True)):
+        kw = dict(shift=2, temporal=reuse, spatial=reuse, cap=20, max_depth=6, neighbors=3, radius=radius, unbiased=unb)
+        forms = {"plain": rt.settings(**kw), "target": rt.settings(learned=True, **kw),
+                 "proposal": rt.settings(guided=True, **kw), "both": rt.settings(learned=True, guided=True, **kw)}
+        if not reuse:
+            del forms["target"], forms["both"]   # the learned target only enters the reuse passes
+        nets = {}
+        ms = {}
+        for name, s in forms.items():
+            mlp = rt.Mlp(rt.mlp_config(bounds[0], bounds[1], log2_table=19, seed=3)) if s.targetFunction else None
+            guide = (rt.Mlp(rt.mlp_config(bounds[0], bounds[1], log2_table=19, seed=4), guide=True)
+                     if s.guidedProposal else None)
+            if mlp or guide:
+                run(s, 128, 29, mlp, guide)
+            _, ms[name] = run(s, 16, 13, mlp, guide)
+            nets[name] = (mlp, guide)
+        budget = args.lvp_frames * ms["plain"]
+        frames = {k: max(1, int(budget // v)) if k != "plain" else args.lvp_frames for k, v in ms.items()}
+        err = {k: [] for k in forms}
+        for seed in sets:
+            for name, s in forms.items():
+                img, _ = run(s, frames[name], seed, *nets[name])
+                err[name].append(relmse(img))
+        res = {"ms_per_frame": {k: round(v, 3) for k, v in ms.items()}, "frames": frames,
+               "relmse_median": {k: round(sorted(v)[len(v) // 2], 5) for k, v in err.items()},
+               "per_set": {k: [round(x, 5) for x in v] for k, v in err.items()}}
+        for name in forms:
+            if name == "plain":
+                continue
+            lr = [math.log(p / q) for p, q in zip(err["plain"], err[name]) if p > 0 and q > 0]
+            m = sum(lr) / len(lr)
+            sd = math.sqrt(sum((x - m) ** 2 for x in lr) / (len(lr) - 1)) if len(lr) > 1 else float("nan")
+            half = t95 * sd / math.sqrt(len(lr))
+            ratios = sorted(math.exp(x) for x in lr)
+            res[f"plain_over_{name}"] = {"geomean": round(math.exp(m), 4),
+                                         "ci95": [round(math.exp(m - half), 4), round(math.exp(m + half), 4)],
+                                         "median": round(ratios[len(ratios) // 2], 4),
+                                         "better_in": f"{sum(1 for x in lr if x > 0)} of {len(lr)}"}
+        out[mode] = res
+        progress(f"learned study: {mode} done")
+    return out
+
+
 def learned_kernel_loop(rt, hs, ds, args):
     """Learned-target frames of the benchmark camera walk (path trace, refresh, temporal,
     spatial, one training step) with per-pass HIP events, for profiling."""
@@ -659,6 +754,112 @@ def full_form_4k(rt, args):
                         "temporal + spatial (3 neighbours, 30 px, M-cap 20)"}
 
 
+def lbvh_bytes(n):
+    """Algorithmic HBM bytes of one binary LBVH build over n triangles: the triangles read once
+    (48 B) and the (Morton code, primitive) pairs written once (12 B); each of the 8 LSD radix
+    passes reads and writes the pairs (24 B); topology and refit write and read the 2n - 1 node
+    slots (64 B per node: the (2N-1) x 64 B of the build's roofline)."""
+    return 48 * n + 12 * n + 8 * 24 * n + 64 * (2 * n - 1)
+
+
+def lbvh_build(rt, scenes, reps=3):
+    """GPU time of the LBVH build (rt_device_scene_create: world triangles, bounds, Morton codes, radix
+    sort, Karras topology, refit, 4-wide collapse) per scene, best of reps builds, and the roofline
+    of its binary part against 8 TB/s."""
+    out = {}
+    for label, hs in scenes:
+        best_bin, best_tot, n = None, None, 0
+        for _ in range(reps):
+            ds = rt.DeviceScene(hs.arrays)
+            info = ds.info()
+            n = int(info.numPrimitives)
+            b, t = float(info.bvhBinaryMs), float(info.bvhBuildMs)
+            best_bin = b if best_bin is None else min(best_bin, b)
+            best_tot = t if best_tot is None else min(best_tot, t)
+            depth, depth4 = int(info.bvhDepth), int(info.bvh4Depth)
+            del ds
+        byts = lbvh_bytes(n)
+        gbs = byts / (best_bin * 1e-3) / 1e9
+        out[label] = {"primitives": n, "binary_ms": round(best_bin, 3), "total_ms": round(best_tot, 3),
+                      "bvh_depth": depth, "bvh4_depth": depth4,
+                      "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                   "frac": round(gbs / PEAK_HBM_GBS, 4), "algorithmic_bytes": byts}}
+    return out
+
+
+def pt_plain_c0(rt, args):
+    """Plain path tracing with NEE+MIS (di_naive.comp + gi_naive.comp through rt_pt_plain) at 256x256,
+    1 spp, depth 4, on the box room (12 quads + 64 icospheres of 320 triangles, one ceiling quad
+    light): per frame a G-buffer and the plain path tracer (direct + indirect), following the
+    scene's camera path; Mpaths/s, Mrays/s from the counting builds, and the oracle on the host
+    cores over the same frames."""
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+
+    hs = rt.HostScene(0, 0)
+    ds = rt.DeviceScene(hs.arrays)
+    W = H = 256
+    depth = 4
+    fr = TorchFrames(rt, ds, W, H)
+    L = rt.core()
+
+    def frame(k):
+        cams = hs.camera(k % 16, W, H)
+        f = fr.frame_struct()
+        rt.check(L.rt_gbuffer_trace(ds.handle, C.byref(cams[0]), f.depthNormal, f.albedoMatId, f.motionVector,
+                                    fr.stream), "gbuffer")
+        rt.check(L.rt_pt_plain(ds.handle, cams, C.byref(f), depth, 3, fr.stream), "pt plain")
+        fr.cur ^= 1
+
+    for k in range(5):
+        frame(k)
+    torch.cuda.synchronize()
+    steps = 400
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for k in range(steps):
+        frame(5 + k)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = e0.elapsed_time(e1) / steps
+    buf = np.zeros(3, np.uint64)
+    rt.check(L.rt_counters_enable(1), "counters")
+    try:
+        rt.check(L.rt_counters_read(buf.ctypes.data, 1), "read")
+        for k in range(16):
+            frame(5 + k)
+        torch.cuda.synchronize()
+        rt.check(L.rt_counters_read(buf.ctypes.data, 1), "read")
+    finally:
+        L.rt_counters_enable(0)
+    rays_per_frame = int(buf[2]) / 16
+    # the oracle on the host cores, same frames, bounded to about 10 s
+    osc = orc.OracleScene(hs.arrays)
+    orc.nthreads = lambda: args.cpu_threads
+    t1 = time.perf_counter()
+    frames = 0
+    while frames < 200 and (frames < 4 or time.perf_counter() - t1 < 10.0):
+        cams = hs.camera((5 + frames) % 16, W, H)
+        dn, am, _, _ = osc.gbuffer(cams[0])
+        osc.pt_plain(cams[0], dn, am, depth)
+        frames += 1
+    cpu_s = time.perf_counter() - t1
+    return {"workload": "plain PT with NEE+MIS (di_naive + gi_naive), 256x256, 1 spp, depth 4, box room with 64 "
+                        "icospheres (%d triangles), G-buffer + plain PT per frame" % int(ds.info().numPrimitives),
+            "ms_per_frame": round(wall * 1e3 / steps, 4), "gpu_ms_per_frame": round(gpu_ms, 4),
+            "mpaths_per_s": round(W * H * steps / wall / 1e6, 2),
+            "mrays_per_s": round(rays_per_frame * steps / wall / 1e6, 1), "rays_per_frame": int(rays_per_frame),
+            "cpu_baseline": {"value": round(W * H * frames / cpu_s / 1e6, 4), "unit": "Mpaths/s",
+                             "cores": args.cpu_threads, "kind": "port",
+                             "sample": f"oracle G-buffer + pt_plain on {args.cpu_threads} host threads, {frames} "
+                                       f"frames of the same camera path at 256x256, {cpu_s:.1f} s"}}
+
+
 def restir_di(rt, hs, ds, args, kernel_loop=False):
     """ReSTIR DI (the reference's TestReSTIR: path generation, temporal reuse, spatial reuse with
     10 neighbours in 20 px and M-cap 40) at the bench resolution on the same terrain and camera walk,
@@ -801,18 +1002,40 @@ def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def kernel_sources_digest():
+    """SHA-256 over the kernel sources (csrc/*.hip, csrc/*.h, include/*.h, sorted by name): the
+    build a traffic measurement belongs to."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for d in (os.path.join(ROOT, "vulkan-restir-pt_amd", "csrc"), os.path.join(ROOT, "include")):
+        for name in sorted(os.listdir(d)):
+            if name.endswith((".hip", ".h")):
+                h.update(name.encode())
+                with open(os.path.join(d, name), "rb") as f:
+                    h.update(f.read())
+    return h.hexdigest()
+
+
 def read_traffic(kernel):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, with the kernel-source digest
+    it was measured at: (bytes, None) when the digest matches this tree, (None, reason) when the
+    summary is missing or was measured on other kernel sources (a stale figure is not reported)."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_1080p_terrain_traffic.md")
     if not os.path.exists(path):
-        return None
+        return None, "no traffic summary"
+    val, digest = None, None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
             try:
-                return float(line.split(":", 1)[1].strip())
+                val = float(line.split(":", 1)[1].strip())
             except ValueError:
-                return None
-    return None
+                val = None
+        elif line.startswith("kernel_sources_sha256:"):
+            digest = line.split(":", 1)[1].strip()
+    if digest != kernel_sources_digest():
+        return None, "the traffic summary was measured on other kernel sources (stale)"
+    return val, None
 
 
 def main():
@@ -832,6 +1055,13 @@ def main():
     if args.only_4k:
         print(json.dumps(full_form_4k(rt, args)))
         return
+    if args.only_lbvh:
+        print(json.dumps(lbvh_build(rt, [("terrain_1080p_scene", rt.HostScene(1, args.seed)),
+                                         ("full_form_4k_scene", rt.HostScene(5, 4))])))
+        return
+    if args.only_plain:
+        print(json.dumps(pt_plain_c0(rt, args)))
+        return
     W, H = args.width, args.height
     hs = rt.HostScene(1, args.seed)
     ds = rt.DeviceScene(hs.arrays)
@@ -845,6 +1075,9 @@ def main():
         return
     if args.guided_kernel_loop:
         print(json.dumps(guided_kernel_loop(rt, hs, ds, args)))
+        return
+    if args.learned_study:
+        print(json.dumps(learned_study(rt, hs, ds, args)))
         return
     loop = FrameLoop(rt, hs, ds, W, H, s)
     for k in range(args.warmup):
@@ -884,12 +1117,21 @@ def main():
         achieved = per_launch / (pass_ms[dom] * 1e-3) / 1e9
         line["roofline"] = {"bound": "hbm", "kernel": KERNELS[dom], "achieved": round(achieved, 1),
                             "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                            "traffic": read_traffic(KERNELS[dom]),
+                            "traffic": read_traffic(KERNELS[dom])[0],
                             "algorithmic_bytes_per_launch": int(per_launch),
                             "launch_ms": round(pass_ms[dom], 3)}
+        stale = read_traffic(KERNELS[dom])[1]
+        if stale:
+            line["roofline"]["traffic_note"] = stale
         line["work_per_frame"] = {p: {"rays": int(work[p][2]) // args.steps, "nodes": int(work[p][0]) // args.steps,
                                       "tris": int(work[p][1]) // args.steps} for p in PASSES}
     progress(f"headline: {line['ms_per_step']} ms/frame")
+    if not args.no_lbvh:
+        line["lbvh_build"] = lbvh_build(rt, [("terrain_1080p_scene", hs), ("full_form_4k_scene", rt.HostScene(5, 4))])
+        progress("lbvh_build done")
+    if not args.no_plain:
+        line["pt_plain_256"] = pt_plain_c0(rt, args)
+        progress("pt_plain_256 done")
     if not args.no_di:
         line["restir_di"] = restir_di(rt, hs, ds, args)
         progress("restir_di done")

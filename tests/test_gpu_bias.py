@@ -149,3 +149,68 @@ def test_guided_unbiased_reuse_shows_no_bias():
     z = d / np.sqrt(gu.var(ddof=1) / len(gu) + none.var(ddof=1) / len(none))
     print(f"[gpu] guided unbiased reuse vs none: bias {d / none.mean():+.4f}, z {z:+.2f}", flush=True)
     assert abs(z) < 4.0 and abs(d / none.mean()) < 0.02
+
+
+def walk(hs, W, H, i, seed):
+    """Frame i of the scene's camera walk (moving camera, lastProjView = the previous frame's), with
+    a per-seed random seed; frameIndex 0 so the output is that frame's own estimate (clear flag on
+    frame 0)."""
+    c = hs.camera(i, W, H)
+    h = (seed * 0x9E3779B1 + i * 0x85EBCA77 + 0x165667B1) & 0xFFFFFFFF
+    h ^= h >> 15
+    h = (h * 0x2C1B3C6D) & 0xFFFFFFFF
+    c[0].seed = c[1].seed = h ^ (h >> 12)
+    c[0].frameIndex = 0x80000000 if i == 0 else 0
+    return c
+
+
+def test_unbiased_reuse_under_camera_motion():
+    """Temporal reuse across the terrain's moving camera (reprojection by the G-buffer's motion
+    vectors, rejection by normal, distance and material) plus spatial reuse, unbiased weights,
+    against the estimator without reuse at the same camera: the fourth frame's image mean over 1024
+    seeds, paired by seed."""
+    v = View(1, 128, 72)
+    kw = dict(shift=2, max_depth=6, cap=20, radius=7.5)
+    L, r = rt.core(), v.run
+
+    def means(s, label):
+        print(f"[gpu] bias under motion, {label}: {SEEDS} seeds x 4 frames", flush=True)
+        out = []
+        for seed in range(SEEDS):
+            for b in r.resv + [r.temp]:
+                rt.check(L.rt_gris_reservoir_reset(b.data_ptr(), v.W, v.H, gpuhelp.stream()), "reset")
+            r.cur = 0
+            for i in range(4):
+                v.frame(walk(v.hs, v.W, v.H, i, seed), s)
+            img = r.indirect.view(torch.float32).view(v.H, v.W, 4)[..., :3].double() @ v.lum
+            out.append(img.mean())
+        return torch.stack(out).cpu().numpy()
+
+    none = means(rt.settings(temporal=False, spatial=False, **kw), "no reuse")
+    unb = means(rt.settings(temporal=True, spatial=True, unbiased=True, **kw), "unbiased reuse")
+    rel, z = paired(unb, none)
+    print(f"[gpu] moving camera: unbiased temporal+spatial reuse vs none: bias {rel:+.4f}, z {z:+.2f}", flush=True)
+    assert abs(z) < 4.0 and abs(rel) < 0.02
+
+
+def test_unbiased_reuse_on_glossy_metals():
+    """A metallic-heavy scene (tests/golden/scenes/ggx_room.xml: GGX walls, a rough metallic box, a
+    delta metal box, a glossy metal ball too smooth to reconnect on) with spatial reuse in the
+    unbiased mode against no reuse, 1024 paired seeds."""
+    from pathlib import Path
+
+    import rtpkg
+
+    hs = rt.HostScene.load(str(Path(rtpkg.ROOT) / "tests" / "golden" / "scenes" / "ggx_room.xml"))
+    v = View.__new__(View)
+    v.hs, v.ds, v.W, v.H = hs, rt.DeviceScene(hs.arrays), 96, 72
+    v.base = hs.camera(0, v.W, v.H)[0]
+    v.run = gpuhelp.GrisRun(v.ds, v.W, v.H)
+    v.lum = torch.tensor([0.299, 0.587, 0.114], dtype=torch.float64, device="cuda")
+    kw = dict(shift=2, max_depth=5, cap=20, radius=8.0)
+    none = v.image_means(rt.settings(temporal=False, spatial=False, **kw), label="glossy metals, no reuse")
+    unb = v.image_means(rt.settings(temporal=True, spatial=True, unbiased=True, **kw),
+                        label="glossy metals, unbiased reuse")
+    rel, z = paired(unb, none)
+    print(f"[gpu] glossy metals: unbiased temporal+spatial reuse vs none: bias {rel:+.4f}, z {z:+.2f}", flush=True)
+    assert abs(z) < 4.0 and abs(rel) < 0.02

@@ -267,33 +267,42 @@ def test_nonfinite_rays_miss_without_traversal():
 
 @pytest.mark.parametrize("method", ["plain", "gris"])
 def test_converged_accumulation_matches_oracle(method):
-    """Progressive accumulation over 4096 frames (4096 spp) of a still view (the renderer's
-    (acc * k + c) / (k + 1) with k = frameIndex): the converged GPU image equals the oracle's.
-    Stated bound: relMSE < 1e-10 (identical per-frame results give 0)."""
+    """Progressive accumulation over 4096 frames (4096 spp) of a still view on the GPU (the
+    renderer's FP32 (acc * k + c) / (k + 1) with k = frameIndex) against the oracle's
+    FP64-accumulated converged image (the mean of its 4096 per-frame estimates, summed in float64;
+    the same seeds, so every frame's estimate is the GPU's). Stated bound: relMSE < 1e-9, the FP32
+    accumulation's rounding."""
     hs = rt.HostScene(0, 4)
     ds = rt.DeviceScene(hs.arrays)
     osc = orc.OracleScene(hs.arrays)
-    W, H = 24, 16
+    W, H = 48, 32
     base = hs.camera(0, W, H)
     run = gpuhelp.GrisRun(ds, W, H)
     s = rt.settings(shift=2, temporal=True, spatial=True, max_depth=4, cap=20, radius=10.0)
     n = W * H
     o_direct = np.zeros((H, W, 4), np.float32)
     o_indirect = np.zeros((H, W, 4), np.float32)
+    acc_direct = np.zeros((H, W, 3), np.float64)
+    acc_indirect = np.zeros((H, W, 3), np.float64)
     o_resv = [orc.reset_reservoirs(n), orc.reset_reservoirs(n)]
     o_temp = orc.reset_reservoirs(n)
     o_dn = [None, None]
     o_am = [None, None]
     cur = 0
-    for i in range(4096):
+    frames = 4096
+    for i in range(frames):
         if i % 512 == 0:
             print(f"[gpu] converged {method}: frame {i}", flush=True)
         cams = gpuhelp.still_cams(base, i, 5)
+        ocams = gpuhelp.still_cams(base, i, 5)
+        if i > 0:   # the oracle writes each frame's own estimate (k = 0); the clear flag stays on frame 0
+            ocams[0].frameIndex = ocams[1].frameIndex = 0
         run.gbuffer(cams[0])
-        dn, am, mv, _ = osc.gbuffer(cams[0])
+        dn, am, mv, _ = osc.gbuffer(ocams[0])
         if method == "plain":
             run.plain(cams, 4)
-            osc.pt_plain(cams[0], dn, am, 4, o_direct, o_indirect)
+            osc.pt_plain(ocams[0], dn, am, 4, o_direct, o_indirect)
+            acc_direct += o_direct[..., :3]
         else:
             o_dn[cur], o_am[cur] = dn, am
             prev = cur ^ 1
@@ -302,14 +311,15 @@ def test_converged_accumulation_matches_oracle(method):
             run.path_trace(cams, s)
             run.temporal(cams, s)
             run.spatial(cams, s)
-            osc.gris_path_trace(cams, dn, am, s, o_resv[cur])
-            osc.gris_temporal(cams, dn, am, mv, pdn, pam, s, o_resv[cur], o_resv[prev], o_temp)
-            osc.gris_spatial(cams, dn, am, s, o_temp, o_resv[cur], o_indirect)
+            osc.gris_path_trace(ocams, dn, am, s, o_resv[cur])
+            osc.gris_temporal(ocams, dn, am, mv, pdn, pam, s, o_resv[cur], o_resv[prev], o_temp)
+            osc.gris_spatial(ocams, dn, am, s, o_temp, o_resv[cur], o_indirect)
             run.flip()
             cur ^= 1
-    outs = [(run.output(1), o_indirect)] + ([(run.output(0), o_direct)] if method == "plain" else [])
+        acc_indirect += o_indirect[..., :3]
+    outs = [(run.output(1), acc_indirect / frames)] + ([(run.output(0), acc_direct / frames)] if method == "plain" else [])
     for g, o in outs:
-        rel = (((g[..., :3].astype(np.float64) - o[..., :3]) ** 2) / (o[..., :3].astype(np.float64) ** 2 + 1e-2)).sum(-1).mean()
-        print(f"[gpu] {method}: converged relMSE GPU vs oracle {rel:.3e}", flush=True)
-        assert rel < 1e-10
-        assert o[..., :3].mean() > 0
+        rel = (((g[..., :3].astype(np.float64) - o) ** 2) / (o ** 2 + 1e-2)).sum(-1).mean()
+        print(f"[gpu] {method}: converged relMSE GPU (FP32 accumulation) vs FP64 oracle {rel:.3e}", flush=True)
+        assert rel < 1e-9
+        assert o.mean() > 0

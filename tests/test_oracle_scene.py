@@ -173,3 +173,19 @@ def test_procedural_workloads():
     assert list(c1[1].projView) == list(prev[0].projView)
     assert list(c1[0].lastProjView) == list(prev[0].projView)
     assert terr.camera(0, 32, 32)[0].frameIndex == 0x80000000
+    # the full-form scene: 1024 x 1024 cells, 2000 instanced displaced icospheres (8 shapes), 64 lights
+    full = rt.HostScene(5, 4)
+    assert full.num_triangles() - full.arrays.numLights == 1024 * 1024 * 4 + 2000 * 320
+    assert full.arrays.numLights == 64
+    a = full.numpy()
+    assert (a["materials"]["roughness"] >= 0.02).all()
+    verts = a["vertices"]
+    sphere = verts[(1025 * 1025 + 1024 * 1024):]
+    assert len(sphere) == 8 * 162
+    pos = np.array([list(v) for v in sphere["pos"]], np.float64)
+    nrm = np.array([list(v) for v in sphere["norm"]], np.float64)
+    r = np.linalg.norm(pos, axis=1)
+    assert 0.65 < r.min() < 0.95 and 1.05 < r.max() < 1.35, "spheres are displaced within +-30 %"
+    assert np.allclose(np.linalg.norm(nrm, axis=1), 1.0, atol=1e-5)
+    assert ((nrm * pos).sum(1) > 0).all(), "normals point outwards"
+    assert not np.allclose(r[:162], r[162:324]), "the 8 shapes differ"

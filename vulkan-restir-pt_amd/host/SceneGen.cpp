@@ -145,6 +145,56 @@ struct Noise {
     }
 };
 
+// Value noise on the integer lattice of 3-space (trilinear, smoothstep weights), in [-1, 1].
+float valueNoise3(uint32_t seed, float x, float y, float z) {
+    auto lat = [&](int i, int j, int k) {
+        uint32_t h = (uint32_t)i * 374761393u + (uint32_t)j * 668265263u + (uint32_t)k * 3266489917u + seed * 2246822519u;
+        h = (h ^ (h >> 13)) * 1274126177u;
+        h ^= h >> 16;
+        return (float)(h & 0xffffffu) / 16777215.0f * 2.0f - 1.0f;
+    };
+    int x0 = (int)std::floor(x), y0 = (int)std::floor(y), z0 = (int)std::floor(z);
+    float fx = x - x0, fy = y - y0, fz = z - z0;
+    float sx = fx * fx * (3.0f - 2.0f * fx), sy = fy * fy * (3.0f - 2.0f * fy), sz = fz * fz * (3.0f - 2.0f * fz);
+    auto lerp = [](float a, float b, float t) { return a + (b - a) * t; };
+    float c00 = lerp(lat(x0, y0, z0), lat(x0 + 1, y0, z0), sx), c10 = lerp(lat(x0, y0 + 1, z0), lat(x0 + 1, y0 + 1, z0), sx);
+    float c01 = lerp(lat(x0, y0, z0 + 1), lat(x0 + 1, y0, z0 + 1), sx);
+    float c11 = lerp(lat(x0, y0 + 1, z0 + 1), lat(x0 + 1, y0 + 1, z0 + 1), sx);
+    return lerp(lerp(c00, c10, sy), lerp(c01, c11, sy), sz);
+}
+
+// The icosphere with each vertex pushed along its direction by up to +-30 % (3-D value noise,
+// two octaves) and smooth normals rebuilt from the displaced faces (area-weighted sums). The
+// surface stays star-shaped around the centre, so it does not intersect itself.
+std::vector<rt_mesh_vertex> displacedIcosphere(const Mesh& ico, uint32_t seed) {
+    std::vector<Vec3> p(ico.pos.size());
+    for (size_t i = 0; i < p.size(); i++) {
+        const Vec3& d = ico.pos[i];
+        float n = 0.7f * valueNoise3(seed, 2.5f * d.x, 2.5f * d.y, 2.5f * d.z) +
+                  0.3f * valueNoise3(seed + 1u, 5.0f * d.x, 5.0f * d.y, 5.0f * d.z);
+        float r = 1.0f + 0.3f * n;
+        p[i] = {d.x * r, d.y * r, d.z * r};
+    }
+    std::vector<Vec3> nrm(p.size(), Vec3{0.0f, 0.0f, 0.0f});
+    for (size_t t = 0; t < ico.tri.size(); t += 3) {
+        const Vec3 &a = p[ico.tri[t]], &b = p[ico.tri[t + 1]], &c = p[ico.tri[t + 2]];
+        Vec3 e1 = {b.x - a.x, b.y - a.y, b.z - a.z}, e2 = {c.x - a.x, c.y - a.y, c.z - a.z};
+        Vec3 f = {e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x};
+        // outward: the face normal points away from the centre
+        if (f.x * (a.x + b.x + c.x) + f.y * (a.y + b.y + c.y) + f.z * (a.z + b.z + c.z) < 0.0f) f = {-f.x, -f.y, -f.z};
+        for (int k = 0; k < 3; k++) {
+            Vec3& q = nrm[ico.tri[t + k]];
+            q = {q.x + f.x, q.y + f.y, q.z + f.z};
+        }
+    }
+    std::vector<rt_mesh_vertex> v;
+    for (size_t i = 0; i < p.size(); i++) {
+        float l = std::sqrt(nrm[i].x * nrm[i].x + nrm[i].y * nrm[i].y + nrm[i].z * nrm[i].z);
+        v.push_back(vtx(p[i], {nrm[i].x / l, nrm[i].y / l, nrm[i].z / l}));
+    }
+    return v;
+}
+
 // ---- configs -----------------------------------------------------------------------
 std::unique_ptr<Scene> boxRoom(uint32_t seed) {
     auto S = std::make_unique<Scene>();
@@ -207,7 +257,7 @@ std::unique_ptr<Scene> boxRoom(uint32_t seed) {
 // instanced icospheres (8 unique models) and numLights emissive triangles;
 // GGX roughness uniform in [roughMin, 1], metallic with p = 0.3.
 std::unique_ptr<Scene> terrain(uint32_t seed, int N = 512, int numSpheres = 200, int numLights = 16,
-                               float roughMin = 0.05f) {
+                               float roughMin = 0.05f, bool displaced = false) {
     auto S = std::make_unique<Scene>();
     Pcg32 rng(seed, 1);
     const float extent = 24.0f, cell = extent / N, amp = 2.4f;
@@ -259,19 +309,25 @@ std::unique_ptr<Scene> terrain(uint32_t seed, int N = 512, int numSpheres = 200,
     float id[16]; identityM(id);
     addInstance(*S, ioff, (uint32_t)tris.size(), id);
 
-    // 200 instanced icospheres: 8 unique models (own material), 25 instances each
+    // instanced icospheres: 8 unique models (own material; with `displaced`, own displaced
+    // shape), instanced round robin
     Mesh ico = icosphere(2);
     std::vector<rt_mesh_vertex> iv;
     for (const Vec3& p : ico.pos) iv.push_back(vtx(p, p));
     uint32_t vbase = (uint32_t)S->vertices.size();
-    S->vertices.insert(S->vertices.end(), iv.begin(), iv.end());
+    if (!displaced) S->vertices.insert(S->vertices.end(), iv.begin(), iv.end());
     uint32_t modelOff[8];
     for (int k = 0; k < 8; k++) {
         int32_t mat = (int32_t)S->materials.size();
         float metallic = rng.uniform() < 0.3f ? 1.0f : 0.0f;
         S->materials.push_back(ggx(rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f),
                                    rng.uniform(roughMin, 1.0f), metallic));
-        modelOff[k] = addModelSharedVerts(*S, vbase, ico.tri, mat);
+        if (displaced) {
+            std::vector<int32_t> triMat(ico.tri.size() / 3, mat);
+            modelOff[k] = addModel(*S, displacedIcosphere(ico, seed * 31u + (uint32_t)k), ico.tri, triMat);
+        } else {
+            modelOff[k] = addModelSharedVerts(*S, vbase, ico.tri, mat);
+        }
     }
     for (int i = 0; i < numSpheres; i++) {
         float r = rng.uniform(0.12f, 0.6f);
@@ -337,7 +393,7 @@ std::unique_ptr<Scene> generateScene(uint32_t config, uint32_t seed) {
     case 2: return floorUnderLight(seed);
     case 3: return terrain(seed, 200);                        // ~225k triangles
     case 4: return terrain(seed, 860);                        // ~3.0M triangles
-    case 5: return terrain(seed, 1024, 2000, 64, 0.02f);      // ~4.8M triangles, 64 lights
+    case 5: return terrain(seed, 1024, 2000, 64, 0.02f, true); // ~4.8M triangles, 64 lights, displaced spheres
     default: return nullptr;
     }
 }
