@@ -133,11 +133,13 @@ RT_DEV bool traverse(const DevScene& S, const RayT& r, bool skipLights, TStack s
         if (kCount) cn++;
         uint32_t c0 = __float_as_uint(cr.x), c1 = __float_as_uint(cr.y), c2 = __float_as_uint(cr.z),
                  c3 = __float_as_uint(cr.w);
+        // all four slab tests unconditionally (branch-free, so the node's seven loads issue
+        // together); an empty slot is masked afterwards
         float t0, t1, t2, t3;
-        bool h0 = c0 != RT_EMPTY && boxHit(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, tBest, t0);
-        bool h1 = c1 != RT_EMPTY && boxHit(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, tBest, t1);
-        bool h2 = c2 != RT_EMPTY && boxHit(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, tBest, t2);
-        bool h3 = c3 != RT_EMPTY && boxHit(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, tBest, t3);
+        bool h0 = boxHit(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, tBest, t0) & (c0 != RT_EMPTY);
+        bool h1 = boxHit(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, tBest, t1) & (c1 != RT_EMPTY);
+        bool h2 = boxHit(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, tBest, t2) & (c2 != RT_EMPTY);
+        bool h3 = boxHit(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, tBest, t3) & (c3 != RT_EMPTY);
         // keys: (t, slot); strict < keeps slot order on equal t
         float k0 = h0 ? t0 : INFINITY, k1 = h1 ? t1 : INFINITY, k2 = h2 ? t2 : INFINITY, k3 = h3 ? t3 : INFINITY;
         uint32_t r0 = h0 ? c0 : RT_EMPTY, r1 = h1 ? c1 : RT_EMPTY, r2 = h2 ? c2 : RT_EMPTY, r3 = h3 ? c3 : RT_EMPTY;
