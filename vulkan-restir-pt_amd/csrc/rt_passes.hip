@@ -24,9 +24,11 @@ namespace {
 
 // Occupancy that each kernel's register allocation is held to, in waves per
 // SIMD (a lane then has at most 512 / waves VGPRs). Chosen by measurement on
-// the 1080p terrain frame: the GRIS kernels run fastest at 4-5 waves even with
-// some spilling, ~25% faster than at the 2-3 waves their unconstrained
-// allocation (150-170 VGPRs) gives. Each can be overridden at build time.
+// the 1080p terrain frame: the GRIS kernels run fastest at 4 waves even with
+// some spilling (path trace 2.63 ms at 4 waves, 2.72 at 5, 2.78 at 3; spatial
+// 1.93 ms at 4, 2.19 at 5, 2.35 at 6), faster than at the 2-3 waves their
+// unconstrained allocation (150-170 VGPRs) gives. Each can be overridden at
+// build time.
 #ifndef RT_WAVES_TRACE
 #define RT_WAVES_TRACE 4
 #endif
@@ -43,7 +45,7 @@ namespace {
 #define RT_WAVES_TEMPORAL 4
 #endif
 #ifndef RT_WAVES_SPATIAL
-#define RT_WAVES_SPATIAL 5
+#define RT_WAVES_SPATIAL 4
 #endif
 // The reuse passes with the unbiased weights (a shift check per merged pixel
 // on top of the merges) hold more state; their variants have their own count.
