@@ -43,7 +43,7 @@ enum {
 typedef struct rt_material {          /* layouts.glsl:6-25, 32 B */
     float baseColor[3];
     uint32_t type;                    /* 1 Lambert, 2 MetallicWorkflow, 3 Metal, 4 Dielectric, 6 Fake */
-    uint32_t textureIdx;              /* must be 0xffffffff: textures are not supported */
+    uint32_t textureIdx;              /* 0xffffffff: none; else an index into rt_scene_arrays.textures */
     float metallic, roughness, ior;
 } rt_material;
 
@@ -94,6 +94,20 @@ typedef struct rt_gris_reservoir {    /* layouts.glsl:132-156, 96 B (host view o
     float sampleCount, resampleWeight, contribWeight, pad0;
 } rt_gris_reservoir;
 
+/* An albedo texture (uTextures, layouts.glsl:178; loaded by Resource::addImage,
+ * src/Resource.cpp:20-35, as 4-channel 8-bit images and bound as
+ * R8G8B8A8_SRGB with a repeat sampler, src/Scene.cpp:428-445,
+ * zvk/core/Memory.cpp:75-92): width x height RGBA8 texels, row 0 first.
+ * Sampling converts sRGB to linear per texel, then filters: filter 0 =
+ * nearest, 1 = bilinear (weights with 8 fractional bits, rounded to nearest);
+ * repeat wrap; level 0 (the reference creates no mip data). */
+typedef struct rt_texture {
+    uint32_t width, height;
+    uint32_t filter;
+    uint32_t pad;
+    const uint8_t* rgba8;
+} rt_texture;
+
 /* Host arrays of a scene, as the reference binds them in descriptor set 1
  * (layouts.glsl:171-178; filled by Scene::load, src/Scene.cpp:104-322). */
 typedef struct rt_scene_arrays {
@@ -104,6 +118,7 @@ typedef struct rt_scene_arrays {
     const rt_object_instance* instances;      uint32_t numInstances;
     const rt_triangle_light* lights;          uint32_t numLights;
     const rt_light_table_entry* lightTable;   uint32_t numLightTable;
+    const rt_texture* textures;               uint32_t numTextures;
 } rt_scene_arrays;
 
 /* GRISReSTIR::Settings (src/GRISReSTIR.h:11-17, push constant

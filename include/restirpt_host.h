@@ -104,6 +104,11 @@ rt_status rt_renderer_enable_proposal(rt_renderer* r, uint32_t seed, uint32_t lo
  * the stbi_write_png call of Renderer::writeScreenshot (src/Renderer.cpp:787-788).
  * Image data is stored in uncompressed deflate blocks. */
 rt_status rt_write_png(const char* path, const uint8_t* rgba8, uint32_t width, uint32_t height);
+/* Reads an 8-bit PNG file (grey, grey + alpha, RGB, RGBA or palette; not
+ * interlaced) as RGBA8, row 0 at the top: the albedo-texture load of
+ * zvk::HostImage::createFromFile (zvk/core/HostImage.cpp:65-96, 4 channels).
+ * With rgba8 null only the size is returned. */
+rt_status rt_read_png(const char* path, uint32_t* width, uint32_t* height, uint8_t* rgba8);
 /* Renderer::writeScreenshot after PostProcessFrag: post-processes the current
  * outputs (the last frame drawn) and writes them to path as a PNG. */
 rt_status rt_renderer_screenshot(rt_renderer* r, const rt_post_settings* settings, const char* path, hipStream_t stream);

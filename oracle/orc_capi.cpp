@@ -26,6 +26,8 @@ struct OScene {
     std::vector<ObjectInstance> instances;
     std::vector<TriangleLight> lights;
     std::vector<LightSampleTableElement> table;
+    std::vector<Texture> textures;
+    std::vector<std::vector<uint8_t>> texels;
     BVH bvh;
 };
 
@@ -74,10 +76,19 @@ void* orc_scene_create(const SceneArrays* in) {
     s->instances.assign(in->instances, in->instances + in->numInstances);
     s->lights.assign(in->lights, in->lights + in->numLights);
     s->table.assign(in->lightTable, in->lightTable + in->numLightTable);
+    s->texels.resize(in->numTextures);
+    s->textures.resize(in->numTextures);
+    for (uint32_t i = 0; i < in->numTextures; i++) {
+        const Texture& t = in->textures[i];
+        s->texels[i].assign(t.rgba8, t.rgba8 + 4 * (size_t)t.width * t.height);
+        s->textures[i] = t;
+        s->textures[i].rgba8 = s->texels[i].data();
+    }
     s->S = {s->materials.data(), (uint32_t)s->materials.size(), s->materialIndices.data(),
             (uint32_t)s->materialIndices.size(), s->vertices.data(), (uint32_t)s->vertices.size(),
             s->indices.data(), (uint32_t)s->indices.size(), s->instances.data(), (uint32_t)s->instances.size(),
-            s->lights.data(), (uint32_t)s->lights.size(), s->table.data(), (uint32_t)s->table.size()};
+            s->lights.data(), (uint32_t)s->lights.size(), s->table.data(), (uint32_t)s->table.size(),
+            s->textures.data(), (uint32_t)s->textures.size()};
     buildWorldTris(s->S, s->bvh.tris);
     buildBVH(s->bvh);
     collapseBVH4(s->bvh, s->bvh.nodes4);
@@ -389,6 +400,14 @@ void orc_gris_learned_refresh(void* h, uint32_t W, uint32_t H, const float* dept
 // ---- component entry points for pinning tests ------------------------------------------------
 uint32_t orc_hash2(uint32_t s) { return hash2(s); }
 uint32_t orc_make_seed(uint32_t seed, uint32_t x, uint32_t y) { return makeSeedPix(seed, x, y); }
+// texture(uTextures[i], uv) for n coordinates (for the sampler's pins)
+void orc_sample_texture(const Texture* tex, const float* uv2, uint32_t n, float* rgb3) {
+    for (uint32_t i = 0; i < n; i++) {
+        vec3 c = sampleTexture(*tex, uv2[2 * i], uv2[2 * i + 1]);
+        rgb3[3 * i] = c.x; rgb3[3 * i + 1] = c.y; rgb3[3 * i + 2] = c.z;
+    }
+}
+void orc_srgb_table(float* out256) { std::memcpy(out256, srgbTable(), 256 * sizeof(float)); }
 // n draws of sample1f from state rng
 void orc_sample1f(uint32_t rng, uint32_t n, float* out) { for (uint32_t i = 0; i < n; i++) out[i] = sample1f(rng); }
 void orc_sincos(const float* x, uint32_t n, float* s, float* c) {

@@ -25,8 +25,10 @@ void gbufferPixel(const Ctx& c, const Camera& cam, uint32_t x, uint32_t y, float
     uint32_t tri = h.isec.triangleIdx;
     float b1 = h.isec.bary[0], b2 = h.isec.bary[1], b0 = 1.0f - b1 - b2;
     vec3 P[3], N[3];
+    float U[3], Vt[3];
     for (int k = 0; k < 3; k++) {
         const MeshVertex& v = S.vertices[S.indices[I.indexOffset + tri * 3 + k]];
+        U[k] = v.uvx; Vt[k] = v.uvy;                        // vsOut.uv
         P[k] = mat4MulPoint(I.transform, L3(v.pos), 1.0f);  // vsOut.pos
         const float* m = I.transformInvT;                    // mat3(transformInvT) * aNorm
         vec3 nn = V3(m[0], m[1], m[2]) * v.norm[0] + V3(m[4], m[5], m[6]) * v.norm[1] + V3(m[8], m[9], m[10]) * v.norm[2];
@@ -35,7 +37,9 @@ void gbufferPixel(const Ctx& c, const Camera& cam, uint32_t x, uint32_t y, float
     vec3 pos = P[0] * b0 + P[1] * b1 + P[2] * b2;
     vec3 nrm = normalize(N[0] * b0 + N[1] * b1 + N[2] * b2);
     uint32_t matIndex = (uint32_t)S.materialIndices[I.indexOffset / 3 + tri];
-    vec3 albedo = L3(S.materials[matIndex].baseColor);
+    // GBuffer.frag:33-37: the texture at the interpolated uv, or baseColor
+    vec3 albedo = materialAlbedo(S, S.materials[matIndex], U[0] * b0 + U[1] * b1 + U[2] * b2,
+                                 Vt[0] * b0 + Vt[1] * b1 + Vt[2] * b2);
     vec4 last = mat4MulVec4(cam.lastProjView, {pos.x, pos.y, pos.z, 1.0f});
     float w = last.w;
     last = {last.x / w, last.y / w, last.z / w, last.w / w};

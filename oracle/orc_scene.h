@@ -17,6 +17,10 @@ struct Material {           // layouts.glsl:6-25 (RESTIR_PT_MATERIAL = 1), 32 B
     uint32_t textureIdx;
     float metallic, roughness, ior;
 };
+struct Texture {            // an R8G8B8A8_SRGB albedo image (uTextures, layouts.glsl:178)
+    uint32_t width, height, filter, pad;   // filter 0 nearest, 1 linear (src/Scene.cpp:442)
+    const uint8_t* rgba8;
+};
 struct MeshVertex {         // layouts.glsl:53-58, 32 B
     float pos[3]; float uvx; float norm[3]; float uvy;
 };
@@ -64,6 +68,7 @@ struct SceneArrays {
     const ObjectInstance* instances; uint32_t numInstances;
     const TriangleLight* lights; uint32_t numLights;
     const LightSampleTableElement* lightTable; uint32_t numLightTable;
+    const Texture* textures; uint32_t numTextures;
 };
 
 static inline vec3 L3(const float* p) { return V3(p[0], p[1], p[2]); }
@@ -292,6 +297,57 @@ static inline void loadLightSurfaceInfo(const SceneArrays& S, uint32_t tri, vec3
     info.matIndex = 0;  // left unset by the reference; never read for lights
 }
 // ray_layouts.glsl:59-88
+// ---- texture(uTextures[i], uv) (ray_layouts.glsl:80-86, GBuffer.frag:33-37) ------
+// The sampler of zvk::Image::samplerCreateInfo (zvk/core/Memory.cpp:75-92): repeat wrap,
+// nearest or linear filter, level 0 (no mip data is generated). The image format is
+// R8G8B8A8_SRGB, so each texel is converted to linear before filtering (Vulkan spec,
+// "Texel Filtering": sRGB conversion precedes filtering). Definitions fixed here where the
+// API leaves the implementation free: the sRGB transfer function evaluated in double and
+// rounded to float (a 256-entry table); texel indices from floor in float32, clamped to
+// +-2^30 before the integer conversion; bilinear weights with 8 fractional bits
+// (subTexelPrecisionBits = 8), rounded to nearest even; the blend
+// (t00 (1-a) + t10 a) (1-b) + (t01 (1-a) + t11 a) b.
+static inline const float* srgbTable() {
+    static float t[256];
+    static bool init = false;
+    if (!init) {
+        for (int c = 0; c < 256; c++) {
+            double v = c / 255.0;
+            t[c] = (float)(v <= 0.04045 ? v / 12.92 : std::pow((v + 0.055) / 1.055, 2.4));
+        }
+        init = true;
+    }
+    return t;
+}
+static inline int texIndex(float x) { return (int)std::fmax(std::fmin(x, 1073741824.0f), -1073741824.0f); }
+static inline int wrapRepeat(int i, int n) { int r = i % n; return r < 0 ? r + n : r; }
+static inline vec3 texelLinear(const Texture& t, int x, int y) {
+    const uint8_t* p = t.rgba8 + 4 * ((size_t)y * t.width + (size_t)x);
+    const float* lut = srgbTable();
+    return V3(lut[p[0]], lut[p[1]], lut[p[2]]);
+}
+static inline vec3 sampleTexture(const Texture& t, float u, float v) {
+    const int W = (int)t.width, H = (int)t.height;
+    if (t.filter == 0) {
+        int ix = wrapRepeat(texIndex(std::floor(u * (float)W)), W), iy = wrapRepeat(texIndex(std::floor(v * (float)H)), H);
+        return texelLinear(t, ix, iy);
+    }
+    float fu = u * (float)W - 0.5f, fv = v * (float)H - 0.5f;
+    float i0f = std::floor(fu), j0f = std::floor(fv);
+    float a = std::nearbyint((fu - i0f) * 256.0f) * (1.0f / 256.0f);
+    float b = std::nearbyint((fv - j0f) * 256.0f) * (1.0f / 256.0f);
+    int i0 = texIndex(i0f), j0 = texIndex(j0f);
+    int x0 = wrapRepeat(i0, W), x1 = wrapRepeat(i0 + 1, W), y0 = wrapRepeat(j0, H), y1 = wrapRepeat(j0 + 1, H);
+    vec3 t00 = texelLinear(t, x0, y0), t10 = texelLinear(t, x1, y0), t01 = texelLinear(t, x0, y1),
+         t11 = texelLinear(t, x1, y1);
+    return (t00 * (1.0f - a) + t10 * a) * (1.0f - b) + (t01 * (1.0f - a) + t11 * a) * b;
+}
+// A material's albedo at texture coordinate (u, v): baseColor without a texture.
+static inline vec3 materialAlbedo(const SceneArrays& S, const Material& m, float u, float v) {
+    if (m.textureIdx == InvalidResourceIdx) return L3(m.baseColor);
+    return sampleTexture(S.textures[m.textureIdx], u, v);
+}
+
 static inline void loadObjectSurfaceInfo(const SceneArrays& S, uint32_t inst, uint32_t tri, vec3 bary, SurfaceInfo& info) {
     const ObjectInstance& I = S.instances[inst];
     info.matIndex = (uint32_t)S.materialIndices[I.indexOffset / 3 + tri];
@@ -303,9 +359,9 @@ static inline void loadObjectSurfaceInfo(const SceneArrays& S, uint32_t inst, ui
     vec3 norm = L3(v0.norm) * bary.x + L3(v1.norm) * bary.y + L3(v2.norm) * bary.z;
     info.pos = mat4MulPoint(I.transform, pos, 1.0f);
     info.norm = normalize(mat4MulPoint(I.transformInvT, norm, 1.0f));
-    const Material& m = S.materials[info.matIndex];
-    // Textured albedo is not part of this build's scenes (textureIdx must be invalid).
-    info.albedo = L3(m.baseColor);
+    float uvx = v0.uvx * bary.x + v1.uvx * bary.y + v2.uvx * bary.z;
+    float uvy = v0.uvy * bary.x + v1.uvy * bary.y + v2.uvy * bary.z;
+    info.albedo = materialAlbedo(S, S.materials[info.matIndex], uvx, uvy);
 }
 // Geometric normal of a hit triangle in world space (sign arbitrary; callers
 // take |cos|): the area measure of a reconnection vertex's Jacobian.

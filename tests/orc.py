@@ -40,6 +40,8 @@ def lib():
         L.orc_bvh4_num_nodes.argtypes = [P]
         L.orc_bvh4_num_nodes.restype = u32
         L.orc_bvh4_export.argtypes = [P, P]
+        L.orc_sample_texture.argtypes = [C.POINTER(rt.Texture), P, u32, P]
+        L.orc_srgb_table.argtypes = [P]
         L.orc_trace_rays.argtypes = [P, P, u32, C.c_int, C.c_int, P, P, P, P, P, C.c_int]
         L.orc_gbuffer.argtypes = [P, C.POINTER(rt.Camera), P, P, P, P, C.c_int]
         L.orc_pt_plain.argtypes = [P, C.POINTER(rt.Camera), P, P, u32, P, P, P, C.c_int]

@@ -73,12 +73,12 @@ def test_argument_checks_fail_loudly():
     h = C.c_void_p()
     assert L.rt_device_scene_create(C.byref(a), None, C.byref(h)) == 1
     assert b"light table" in L.rt_last_error()
-    # textured materials are not supported: rejected, not silently untextured
+    # a material naming a texture the scene does not hold: rejected, not silently untextured
     mat.textureIdx = 0
     a.numLights, a.numLightTable = 0, 1
     tab[0].prob, tab[0].failId = 0.0, 0
     assert L.rt_device_scene_create(C.byref(a), None, C.byref(h)) == 1
-    assert b"textured" in L.rt_last_error()
+    assert b"textureIdx out of range" in L.rt_last_error()
     bad = rt.settings(max_depth=0)
     fake = C.c_void_p(1)
     f.width, f.height = 4, 4

@@ -248,12 +248,12 @@ def test_transform_defaults_and_polygons(tmp_path):
     ("missing file", None, "cannot open"),
     ("not a scene", "<notascene/>", "failed to load"),
     ("unterminated", "<scene><modelInstances></scene>", "does not close"),
-    ("texture", '<scene><modelInstances><model path="cube.obj"><material type="lambertian">'
-                '<baseColor image="wood.png"/></material></model></modelInstances></scene>', "texture"),
+    ("missing texture", '<scene><modelInstances><model path="cube.obj"><material type="lambertian">'
+                        '<baseColor image="wood.png"/></material></model></modelInstances></scene>', "png: cannot open"),
     ("zero power", '<scene><modelInstances><model type="light" path="lamp.obj"><radiance value="0 0 0"/>'
                    '</model></modelInstances></scene>', "no power"),
     ("bad index", '<scene><modelInstances><model path="bad.obj"/></modelInstances></scene>', "out of range"),
-    ("map_Kd", '<scene><modelInstances><model path="tex.obj"/></modelInstances></scene>', "texture"),
+    ("missing map_Kd", '<scene><modelInstances><model path="tex.obj"/></modelInstances></scene>', "png: cannot open"),
     ("missing mesh", '<scene><modelInstances><model path="nothere.obj"/></modelInstances></scene>', "cannot open"),
     ("camera", '<scene><camera><position value="1 1 1"/><lookAt value="1 1 1"/></camera></scene>', "lookAt"),
 ])

@@ -2,6 +2,7 @@
 // include/restirpt.h). Validates arguments on the host, owns the device scene
 // and launches the kernels of rt_bvh.hip / rt_passes.hip.
 #include "rt_internal.h"
+#include <cmath>
 #include "rt_guide.h"
 #include <cstring>
 #include <mutex>
@@ -70,6 +71,7 @@ void freeScene(rt_device_scene* s) {
     (void)hipFree(s->nodes); (void)hipFree(s->nodes4); (void)hipFree(s->leafTris); (void)hipFree(s->primTris); (void)hipFree(s->codes); (void)hipFree(s->order);
     (void)hipFree(s->materials); (void)hipFree(s->materialIndices); (void)hipFree(s->vertices); (void)hipFree(s->indices);
     (void)hipFree(s->instances); (void)hipFree(s->lights); (void)hipFree(s->lightTable); (void)hipFree(s->queues);
+    (void)hipFree(s->texDesc); (void)hipFree(s->texels); (void)hipFree(s->srgbLut);
     delete s;
 }
 
@@ -79,8 +81,16 @@ rt_status checkScene(const rt_scene_arrays* a) {
     if (a->numLightTable != a->numLights + 1 || !a->lightTable) return invalid("light table must hold numLights + 1 entries");
     if (a->numLights && !a->lights) return invalid("lights is null");
     if (a->numInstances && !a->instances) return invalid("instances is null");
+    if (a->numTextures && !a->textures) return invalid("textures is null");
+    for (uint32_t t = 0; t < a->numTextures; t++) {
+        const rt_texture& T = a->textures[t];
+        if (T.width == 0 || T.height == 0 || T.width > 16384 || T.height > 16384 || !T.rgba8)
+            return invalid("texture " + std::to_string(t) + ": size must be in [1, 16384] with texels");
+        if (T.filter > 1) return invalid("texture filter must be 0 (nearest) or 1 (linear)");
+    }
     for (uint32_t m = 0; m < a->numMaterials; m++)
-        if (a->materials[m].textureIdx != 0xffffffffu) return invalid("textured materials are not supported");
+        if (a->materials[m].textureIdx != 0xffffffffu && a->materials[m].textureIdx >= a->numTextures)
+            return invalid("material " + std::to_string(m) + ": textureIdx out of range");
     if (a->lightTable[0].failId != a->numLights) return invalid("light table entry 0 must hold numLights");
     for (uint32_t i = 1; i < a->numLightTable; i++)
         if (a->lightTable[i].failId < 1 || a->lightTable[i].failId > a->numLights)
@@ -186,6 +196,28 @@ rt_status rt_device_scene_create(const rt_scene_arrays* a, hipStream_t stream, r
               upload(s, &s->instances, inst.data(), inst.size(), stream) &&
               upload(s, &s->lights, a->lights, a->numLights, stream) &&
               upload(s, &s->lightTable, a->lightTable, a->numLightTable, stream);
+    // textures: one texel array (RGBA8 words), a descriptor per texture {first texel, width,
+    // height, filter} and the sRGB-to-linear table of the 256 byte values (the transfer function
+    // evaluated in double, rounded to float)
+    std::vector<uint4> texDesc(a->numTextures);
+    std::vector<uint32_t> texels;
+    for (uint32_t t = 0; t < a->numTextures; t++) {
+        const rt_texture& T = a->textures[t];
+        const size_t n = (size_t)T.width * T.height;
+        texDesc[t] = make_uint4((uint32_t)texels.size(), T.width, T.height, T.filter);
+        const size_t at = texels.size();
+        texels.resize(at + n);
+        std::memcpy(texels.data() + at, T.rgba8, 4 * n);
+    }
+    float srgb[256];
+    for (int c = 0; c < 256; c++) {
+        const double v = c / 255.0;
+        srgb[c] = (float)(v <= 0.04045 ? v / 12.92 : std::pow((v + 0.055) / 1.055, 2.4));
+    }
+    if (ok && texels.size() >= 0xffffffffull) ok = (invalid("textures hold too many texels"), false);
+    ok = ok && upload(s, &s->texDesc, texDesc.data(), texDesc.size(), stream) &&
+         upload(s, &s->texels, texels.data(), texels.size(), stream) &&
+         upload(s, &s->srgbLut, srgb, 256, stream);
     if (ok) {
         const size_t qb = sizeof(rtd::WorkQueue) * rt_device_scene::kQueueSlots;
         ok = rti::hipCheck(hipMalloc(&s->queues, qb), "hipMalloc") &&
@@ -199,6 +231,7 @@ rt_status rt_device_scene_create(const rt_scene_arrays* a, hipStream_t stream, r
         d.numPrims = s->numPrims; d.numLights = s->numLights; d.numLightTable = a->numLightTable;
         d.numMaterials = a->numMaterials;
         d.numInstances = a->numInstances;
+        d.textures = s->texDesc; d.texels = s->texels; d.srgb = s->srgbLut; d.numTextures = a->numTextures;
         ok = rti::buildLBVH(s, offs, stream);
         d.nodes = s->nodes4; d.leafTris = s->leafTris;
     }

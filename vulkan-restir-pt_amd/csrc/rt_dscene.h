@@ -33,6 +33,10 @@ struct DevScene {
     const DevInstance* instances;
     const rt_triangle_light* lights;
     const rt_light_table_entry* lightTable;
+    const uint4* textures;        // per texture {first texel, width, height, filter}
+    const uint32_t* texels;       // RGBA8
+    const float* srgb;            // sRGB byte -> linear
+    uint32_t numTextures;
     uint32_t numPrims;
     uint32_t numLights;
     uint32_t numLightTable;
@@ -248,6 +252,40 @@ RT_DEV f3 colMul4(const float* m, f3 p, float w) {  // GLSL mat4 * vec4(p, w), x
            mk3(m[9], m[10], m[11]) * w;
 }
 
+// ---- texture(uTextures[i], uv) (ray_layouts.glsl:80-86, GBuffer.frag:33-37) ------------
+// R8G8B8A8_SRGB image, repeat sampler, level 0 (zvk/core/Memory.cpp:75-92,
+// src/Scene.cpp:428-445): texels converted to linear, then nearest or bilinear
+// filtering with 8-bit weights rounded to nearest even; texel indices from
+// floor in float32, clamped to +-2^30 before the integer conversion.
+RT_DEV int texIndex(float x) { return (int)fmaxf(fminf(x, 1073741824.0f), -1073741824.0f); }
+RT_DEV int wrapRepeat(int i, int n) { int r = i % n; return r < 0 ? r + n : r; }
+RT_DEV f3 texelLinear(const DevScene& S, const uint4& t, int x, int y) {
+    const uint32_t c = S.texels[t.x + (uint32_t)y * t.y + (uint32_t)x];
+    return mk3(S.srgb[c & 255u], S.srgb[(c >> 8) & 255u], S.srgb[(c >> 16) & 255u]);
+}
+RT_DEV f3 sampleTexture(const DevScene& S, uint32_t idx, float u, float v) {
+    const uint4 t = S.textures[idx];
+    const int W = (int)t.y, H = (int)t.z;
+    if (t.w == 0u) {
+        const int ix = wrapRepeat(texIndex(floorf(u * (float)W)), W), iy = wrapRepeat(texIndex(floorf(v * (float)H)), H);
+        return texelLinear(S, t, ix, iy);
+    }
+    const float fu = u * (float)W - 0.5f, fv = v * (float)H - 0.5f;
+    const float i0f = floorf(fu), j0f = floorf(fv);
+    const float a = rintf((fu - i0f) * 256.0f) * (1.0f / 256.0f);
+    const float b = rintf((fv - j0f) * 256.0f) * (1.0f / 256.0f);
+    const int i0 = texIndex(i0f), j0 = texIndex(j0f);
+    const int x0 = wrapRepeat(i0, W), x1 = wrapRepeat(i0 + 1, W), y0 = wrapRepeat(j0, H), y1 = wrapRepeat(j0 + 1, H);
+    const f3 t00 = texelLinear(S, t, x0, y0), t10 = texelLinear(S, t, x1, y0), t01 = texelLinear(S, t, x0, y1),
+             t11 = texelLinear(S, t, x1, y1);
+    return (t00 * (1.0f - a) + t10 * a) * (1.0f - b) + (t01 * (1.0f - a) + t11 * a) * b;
+}
+// A material's albedo at texture coordinate (u, v): baseColor without a texture.
+RT_DEV f3 materialAlbedo(const DevScene& S, const rt_material& m, float u, float v) {
+    if (m.textureIdx == 0xffffffffu) return ld3(m.baseColor);
+    return sampleTexture(S, m.textureIdx, u, v);
+}
+
 RT_DEV void loadSurfaceInfo(const DevScene& S, const Isec& is, Surf& info) {
     float b0 = 1.0f - is.b1 - is.b2;
     if (is.inst == 0) {
@@ -270,7 +308,13 @@ RT_DEV void loadSurfaceInfo(const DevScene& S, const Isec& is, Surf& info) {
     info.pos = colMul4(I.m, pos, 1.0f);
     info.norm = normalize(colMul4(I.n, nrm, 1.0f));
     const rt_material& m = S.materials[info.matIndex];
-    info.albedo = ld3(m.baseColor);
+    if (m.textureIdx == 0xffffffffu) {
+        info.albedo = ld3(m.baseColor);
+    } else {
+        const float uvx = v0.uvx * b0 + v1.uvx * is.b1 + v2.uvx * is.b2;
+        const float uvy = v0.uvy * b0 + v1.uvy * is.b1 + v2.uvy * is.b2;
+        info.albedo = sampleTexture(S, m.textureIdx, uvx, uvy);
+    }
     info.isLight = false;
 }
 

@@ -26,6 +26,9 @@ struct rt_device_scene {
     rtd::DevInstance* instances = nullptr;
     rt_triangle_light* lights = nullptr;
     rt_light_table_entry* lightTable = nullptr;
+    uint4* texDesc = nullptr;     // per texture {first texel, width, height, filter}
+    uint32_t* texels = nullptr;   // RGBA8 texels of all textures
+    float* srgbLut = nullptr;     // sRGB byte -> linear
     uint32_t numPrims = 0, numLights = 0, numInstances = 0;
     uint32_t bvhDepth = 0;
     float bvhBinaryMs = 0.0f, bvhBuildMs = 0.0f;  // GPU time of the binary build / of the whole build

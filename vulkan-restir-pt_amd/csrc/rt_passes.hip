@@ -102,8 +102,10 @@ RT_DEV void gbufferPixel(const DevScene& S, const rt_camera& cam, float4* __rest
     uint32_t tri = h.prim;
     float b1 = h.b1, b2 = h.b2, b0 = 1.0f - b1 - b2;
     f3 P[3], N[3];
+    float U[3], V[3];
     for (int k = 0; k < 3; k++) {
         const rt_mesh_vertex& v = S.vertices[S.indices[I.indexOffset + tri * 3 + k]];
+        U[k] = v.uvx; V[k] = v.uvy;
         P[k] = colMul4(I.m, ld3(v.pos), 1.0f);
         f3 nn = mk3(I.n[0], I.n[1], I.n[2]) * v.norm[0] + mk3(I.n[3], I.n[4], I.n[5]) * v.norm[1] +
                 mk3(I.n[6], I.n[7], I.n[8]) * v.norm[2];
@@ -120,8 +122,10 @@ RT_DEV void gbufferPixel(const DevScene& S, const rt_camera& cam, float4* __rest
     lx = lx / lw; ly = ly / lw;
     f2 lastC = mk2(lx * 0.5f + 0.5f, ly * 0.5f + 0.5f);
     dnOut[p] = make_float4(length(ld3(cam.pos) - pos), nrm.x, nrm.y, nrm.z);
-    uint32_t alb = packUnorm8(m.baseColor[0]) | (packUnorm8(m.baseColor[1]) << 8) |
-                   (packUnorm8(m.baseColor[2]) << 16) | (packUnorm8(1.0f) << 24);
+    // GBuffer.frag:33-37: the texture at the interpolated uv, or baseColor
+    const f3 albedo = materialAlbedo(S, m, U[0] * b0 + U[1] * b1 + U[2] * b2, V[0] * b0 + V[1] * b1 + V[2] * b2);
+    uint32_t alb = packUnorm8(albedo.x) | (packUnorm8(albedo.y) << 8) | (packUnorm8(albedo.z) << 16) |
+                   (packUnorm8(1.0f) << 24);
     amOut[p] = make_uint2(alb, (matIndex << 16) | meshIdx);
     uint32_t mx = __half_as_ushort(__float2half_rn(lastC.x - uv.x));
     uint32_t my = __half_as_ushort(__float2half_rn(lastC.y - uv.y));

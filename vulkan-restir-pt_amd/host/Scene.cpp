@@ -66,6 +66,14 @@ rt_scene_arrays Scene::arrays() const {
     a.lights = triangleLights.data(); a.numLights = (uint32_t)triangleLights.size();
     a.lightTable = lightSampleTable.binomDistribs.data();
     a.numLightTable = (uint32_t)lightSampleTable.binomDistribs.size();
+    textureViews.clear();
+    for (const Texture& t : textures) {
+        rt_texture v{};
+        v.width = t.width; v.height = t.height; v.filter = t.filter; v.rgba8 = t.rgba.data();
+        textureViews.push_back(v);
+    }
+    a.textures = textureViews.data();
+    a.numTextures = (uint32_t)textureViews.size();
     return a;
 }
 

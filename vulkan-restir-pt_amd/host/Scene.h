@@ -21,6 +21,14 @@ struct DiscreteSampler1D {
     uint32_t sample(float r1, float r2) const;
 };
 
+// An albedo texture: RGBA8 texels, row 0 first; filter 0 nearest, 1 linear.
+struct Texture {
+    uint32_t width = 0, height = 0, filter = 1;
+    std::vector<uint8_t> rgba;
+};
+// Reads an 8-bit PNG file (grey, grey + alpha, RGB, RGBA or palette, not interlaced) as RGBA8.
+Texture readPng(const std::string& path);
+
 class Scene {
 public:
     // Adds a light triangle with world-space vertices and per-triangle radiance;
@@ -38,6 +46,8 @@ public:
     std::vector<rt_object_instance> objectInstances;
     std::vector<rt_triangle_light> triangleLights;
     DiscreteSampler1D lightSampleTable;
+    std::vector<Texture> textures;              // Resource::imagePool (src/Resource.h)
+    mutable std::vector<rt_texture> textureViews;  // what arrays() hands out
 
     // camera path of the procedural workloads: per frame position, angle, seed
     struct PathKey { Vec3 pos; Vec3 angle; uint32_t seed; };

@@ -11,8 +11,8 @@
 //    content is ignored.
 //  * OBJ: v / vt / vn / f (any polygon, fan-triangulated; v, v/vt, v//vn,
 //    v/vt/vn; negative indices), o / g / usemtl start a new mesh, mtllib + MTL
-//    newmtl / Kd (baseColor); map_Kd textures are rejected (this build has no
-//    textures). Each face corner is its own vertex (no vertex joining), UVs are
+//    newmtl / Kd (baseColor) / map_Kd (an albedo texture with the linear filter,
+//    Resource.cpp:160-171). Each face corner is its own vertex (no vertex joining), UVs are
 //    flipped (v -> 1 - v), meshes without normals get smooth normals (the
 //    normalised sum of the unit normals of the faces sharing a position).
 //    Faces before any usemtl use a default material (grey 0.6).
@@ -211,7 +211,7 @@ Vec3 xformPoint(const M4& m, const float p[3]) {
 }
 
 // ---- OBJ / MTL ------------------------------------------------------------------------------
-struct ObjMaterial { std::string name; float kd[3] = {0.6f, 0.6f, 0.6f}; };
+struct ObjMaterial { std::string name; float kd[3] = {0.6f, 0.6f, 0.6f}; std::string map; };
 struct ObjMesh {
     uint32_t material = 0;   // into ObjModel::materials (0 = the default material)
     std::vector<rt_mesh_vertex> vertices;
@@ -223,6 +223,20 @@ struct ObjModel { std::vector<ObjMaterial> materials; std::vector<ObjMesh> meshe
 std::string dirOf(const std::string& path) {
     size_t k = path.find_last_of('/');
     return k == std::string::npos ? std::string() : path.substr(0, k + 1);
+}
+
+// Resource::addImage (src/Resource.cpp:20-35): one texture per file path, loaded once.
+// Only PNG files are read (readPng); a file that cannot be read is an error here (the
+// reference leaves such a material untextured).
+uint32_t addTexture(Scene& S, std::map<std::string, uint32_t>& index, const std::string& path, uint32_t filter) {
+    auto it = index.find(path);
+    if (it != index.end()) return it->second;
+    Texture t = readPng(path);
+    t.filter = filter;
+    S.textures.push_back(std::move(t));
+    const uint32_t k = (uint32_t)S.textures.size() - 1;
+    index[path] = k;
+    return k;
 }
 
 void readMtl(const std::string& path, std::vector<ObjMaterial>& mats) {
@@ -239,8 +253,10 @@ void readMtl(const std::string& path, std::vector<ObjMaterial>& mats) {
             mats.push_back(m);
         } else if (key == "Kd" && !mats.empty()) {
             ss >> mats.back().kd[0] >> mats.back().kd[1] >> mats.back().kd[2];
-        } else if (key == "map_Kd") {
-            throw std::runtime_error("scene: texture maps are not supported (" + path + ")");
+        } else if (key == "map_Kd" && !mats.empty()) {
+            std::string file;
+            ss >> file;
+            mats.back().map = file.empty() || file[0] == '/' ? file : dirOf(path) + file;
         }
     }
 }
@@ -393,6 +409,7 @@ std::unique_ptr<Scene> loadSceneFile(const std::string& path) {
     const XmlNode* sceneNode = doc.child("scene");
     if (!sceneNode) throw std::runtime_error("Scene: failed to load");
     auto S = std::make_unique<Scene>();
+    std::map<std::string, uint32_t> textureIndex;
     // Resource() starts with an empty magenta material (src/Resource.cpp:36-40)
     rt_material empty{};
     empty.baseColor[0] = 1.0f; empty.baseColor[1] = 0.0f; empty.baseColor[2] = 1.0f;
@@ -504,6 +521,7 @@ std::unique_ptr<Scene> loadSceneFile(const std::string& path) {
             for (const ObjMaterial& om : obj.materials) {
                 rt_material mat = empty;
                 for (int k = 0; k < 3; k++) mat.baseColor[k] = om.kd[k];
+                if (!om.map.empty()) mat.textureIdx = addTexture(*S, textureIndex, om.map, 1u);
                 S->materials.push_back(mat);
             }
             rt_object_instance inst{};
@@ -524,14 +542,20 @@ std::unique_ptr<Scene> loadSceneFile(const std::string& path) {
                 if (loadMaterial(*matNode, over)) {
                     bool overrideColor = false;
                     float bc[3] = {1, 1, 1};
+                    uint32_t texIdx = 0xffffffffu;
                     if (const XmlNode* b = matNode->child("baseColor")) {
-                        if (b->attr("image")) throw std::runtime_error("scene: texture images are not supported");
                         if (b->attr("value")) readFloats(b->attr("value"), bc, 3);
+                        if (const char* img = b->attr("image")) {
+                            const char* filt = b->attr("filter");
+                            const uint32_t filter = (filt && std::string(filt) == "nearest") ? 0u : 1u;
+                            texIdx = addTexture(*S, textureIndex, dir + img, filter);
+                        }
                         overrideColor = true;
                     }
                     for (const ObjMesh& mesh : obj.meshes) {
                         const uint32_t idx = materialOffset + mesh.material;
                         rt_material mm = over;
+                        mm.textureIdx = texIdx != 0xffffffffu ? texIdx : S->materials[idx].textureIdx;
                         for (int k = 0; k < 3; k++) mm.baseColor[k] = overrideColor ? bc[k] : S->materials[idx].baseColor[k];
                         S->materials[idx] = mm;
                     }

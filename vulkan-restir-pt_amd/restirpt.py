@@ -108,7 +108,13 @@ class SceneArrays(C.Structure):
                 ("indices", C.c_void_p), ("numIndices", C.c_uint32),
                 ("instances", C.c_void_p), ("numInstances", C.c_uint32),
                 ("lights", C.c_void_p), ("numLights", C.c_uint32),
-                ("lightTable", C.c_void_p), ("numLightTable", C.c_uint32)]
+                ("lightTable", C.c_void_p), ("numLightTable", C.c_uint32),
+                ("textures", C.c_void_p), ("numTextures", C.c_uint32)]
+
+
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("filter", C.c_uint32), ("pad", C.c_uint32),
+                ("rgba8", C.c_void_p)]
 
 
 class GrisSettings(C.Structure):
@@ -199,7 +205,7 @@ HOST_SYMBOLS = [
     "rt_renderer_bindings", "rt_renderer_flip", "rt_renderer_read_output", "rt_renderer_reset_reservoirs",
     "rt_renderer_enable_learned", "rt_renderer_enable_proposal", "rt_renderer_frame_methods",
     "rt_renderer_set_di_settings",
-    "rt_renderer_di_bindings", "rt_write_png", "rt_renderer_screenshot",
+    "rt_renderer_di_bindings", "rt_write_png", "rt_read_png", "rt_renderer_screenshot",
 ]
 
 _core = None
@@ -300,6 +306,7 @@ def _declare_host(L):
         "rt_renderer_set_di_settings": [P, C.POINTER(DISettings)],
         "rt_renderer_di_bindings": [P, C.POINTER(DIReservoirs)],
         "rt_write_png": [C.c_char_p, P, u32, u32],
+        "rt_read_png": [C.c_char_p, C.POINTER(u32), C.POINTER(u32), P],
         "rt_renderer_screenshot": [P, C.POINTER(PostSettings), C.c_char_p, P],
     }
     for name, args in sigs.items():
