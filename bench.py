@@ -632,16 +632,47 @@ def batched_views(rt, args, hs1, ds1):
             flip()
         torch.cuda.synchronize()
         steps = 8
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        host_s = 0.0
         t0 = time.perf_counter()
+        e0.record()
         for k in range(steps):
+            h0 = time.perf_counter()
             rt.check(L.rt_batch_frame(b, views_for(2 + k), len(specs), stream), "frame")
+            host_s += time.perf_counter() - h0
             flip()
+        e1.record()
         torch.cuda.synchronize()
         batched_ms = (time.perf_counter() - t0) * 1e3 / steps
+        gpu_ms = e0.elapsed_time(e1) / steps
+
+        def timed_gpu(tuning, first, n):
+            rt.check(L.rt_batch_set_tuning(b, C.byref(tuning)), "tuning")
+            rt.check(L.rt_batch_frame(b, views_for(first), len(specs), stream), "frame")  # warm-up
+            flip()
+            a, z = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for k in range(n):
+                rt.check(L.rt_batch_frame(b, views_for(first + 1 + k), len(specs), stream), "frame")
+                flip()
+            z.record()
+            torch.cuda.synchronize()
+            return a.elapsed_time(z) / n
+
+        # both path-trace variants at the tuned settings: the persistent kernel and the
+        # wavefront variant with material-sorted shading (the tuner keeps the faster)
+        cus = torch.cuda.get_device_properties(0).multi_processor_count
+        variant_ms = {}
+        for shading in (0, 1):
+            t = rt.BatchTuning.from_buffer_copy(bytes(best))
+            if t.shading != shading:
+                t.shading, t.persistentGroups = shading, (4 * cus if shading else 0)
+            variant_ms[("persistent", "wavefront_material_sorted")[shading]] = round(timed_gpu(t, 20, 4), 3)
+        rt.check(L.rt_batch_set_tuning(b, C.byref(best)), "tuning")
         # the same frames launched view by view (4 launches per view)
         t0 = time.perf_counter()
         for k in range(2):
-            vs = views_for(10 + k)
+            vs = views_for(40 + k)
             for i in range(len(specs)):
                 v = vs[i]
                 cams = v.cams
@@ -656,8 +687,12 @@ def batched_views(rt, args, hs1, ds1):
         L.rt_batch_destroy(b)
     return {"views": len(specs), "pixels_per_frame": pixels, "scene_triangles": tris,
             "ms_per_frame": round(batched_ms, 3), "mpaths_per_s": round(pixels / batched_ms / 1e3, 2),
+            "gpu_ms_per_frame": round(gpu_ms, 3), "host_ms_per_frame": round(host_s * 1e3 / steps, 3),
+            "path_trace_variants_gpu_ms_per_frame": variant_ms,
             "view_by_view_ms_per_frame": round(single_ms, 3),
             "tuning": {"persistent_groups": int(best.persistentGroups), "xcd_chunk_tiles": int(best.xcdSwizzle),
+                       "waves": int(best.waves), "queue_tile": ["8x8", "16x4", "64x1"][best.queueTile],
+                       "graph": int(best.graph), "shading": ["persistent", "wavefront_material_sorted"][best.shading],
                        "tuned_ms": round(float(tune_ms.value), 3)},
             "workload": "batched plain ReSTIR PT (hybrid shift, temporal + spatial, 3 neighbours in 20 px, "
                         "M-cap 20), square views log-uniform in [128, 1024] px, depth U{3..8}, scenes "
@@ -746,7 +781,10 @@ def full_form_4k(rt, args):
     return {"ms_per_frame": round(wall * 1e3 / steps, 3), "mpaths_per_s": round(pixels * steps / wall / 1e6, 2),
             "batched_frame_gpu_ms": round(batch_ms, 3),
             "pixels_per_frame": pixels, "triangles": int(info.numPrimitives), "emissive": int(hs.arrays.numLights),
-            "tuning": {"persistent_groups": int(best.persistentGroups), "xcd_chunk_tiles": int(best.xcdSwizzle)},
+            "tuning": {"persistent_groups": int(best.persistentGroups), "xcd_chunk_tiles": int(best.xcdSwizzle),
+                       "waves": int(best.waves), "queue_tile": ["8x8", "16x4", "64x1"][best.queueTile],
+                       "graph": int(best.graph), "shading": ["persistent", "wavefront_material_sorted"][best.shading],
+                       "tuned_ms": round(float(tune_ms.value), 3)},
             "workload": "one rt_batch_frame of 17 views: a 3840x2160 view with the learned target (hash grid "
                         "16x2x2^19 + MLP 2x64) and the learned 8-lobe vMF-mixture proposal (second network, 32 "
                         "outputs), both trained each frame, unbiased reuse; + 16 x 512x512 views (luminance "

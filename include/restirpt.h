@@ -531,6 +531,18 @@ typedef struct rt_batch_tuning {
     uint32_t persistentGroups;    /* workgroups of the persistent path trace (0: CUs x resident) */
     uint32_t xcdSwizzle;          /* per-pixel passes: each XCD takes runs of this many consecutive
                                      8x8 tiles, every 8th run (0: dispatch order; at most 4096) */
+    uint32_t waves;               /* register budget of the persistent path trace, in waves per
+                                     SIMD: 3, 4 or 5 (0: 4) */
+    uint32_t queueTile;           /* pixels of one path-trace queue item (64 lanes refilled together):
+                                     0 = an 8x8 tile, 1 = 16x4, 2 = 64x1 */
+    uint32_t graph;               /* 1: the frame's launches replay as one HIP graph whose kernel
+                                     nodes are updated per frame; 0: launched one by one */
+    uint32_t shading;             /* 0: the persistent path trace (one lane per path, lanes refilled
+                                     from the pixel queues); 1: the wavefront variant -- path state
+                                     in HBM, per bounce a closest-hit launch, a counting sort of
+                                     the live paths by the hit's material type and a shading
+                                     launch in that order (views without the learned proposal;
+                                     allocates 256 B per pixel of the batch on first use) */
 } rt_batch_tuning;
 
 typedef struct rt_batch rt_batch;
@@ -539,12 +551,20 @@ rt_status rt_batch_destroy(rt_batch* batch);
 rt_status rt_batch_frame(rt_batch* batch, const rt_view* views, uint32_t numViews, hipStream_t stream);
 rt_status rt_batch_set_tuning(rt_batch* batch, const rt_batch_tuning* tuning);
 rt_status rt_batch_get_tuning(const rt_batch* batch, rt_batch_tuning* tuning);
-/* Autotuner: times each candidate tuning (persistent workgroups at 1/2, 3/4
- * and 1 x the resident limit; XCD chunks of 0, 16 and 128 tiles) over `frames`
- * frames of these views (after one warm-up frame each), keeps the fastest and
- * returns it with its mean frame time. The views' reservoirs are left in an
- * arbitrary state (and their networks trained on those frames): reset them
- * before rendering for real. */
+/* Autotuner: coordinate descent over the tuning from {persistentGroups 0,
+ * xcdSwizzle 16, waves 4, queueTile 0, graph 0, shading 0}: the register
+ * budget (3, 4, 5), then the queue tile shape, the persistent workgroups (1/2,
+ * 3/4 and 1 x the resident limit of the chosen budget), the XCD chunk (0, 16,
+ * 128 tiles), graph replay and, when no view uses the learned proposal, the
+ * wavefront variant (with 2, 4 and 8 workgroups per CU), each searched with
+ * the others at their best so far. Each
+ * candidate is timed with HIP events over `frames` frames of these views after
+ * one warm-up frame; the frames alternate the views' parity (this frame's
+ * G-buffer, reservoirs and proposal lobes swapped with the previous frame's),
+ * as a caller's frames do, so temporal reuse works on the previous frame's
+ * output. Keeps the fastest and returns it with its mean frame time. The
+ * views' reservoirs are left in an arbitrary state (and their networks trained
+ * on those frames): reset them before rendering for real. */
 rt_status rt_batch_autotune(rt_batch* batch, const rt_view* views, uint32_t numViews, uint32_t frames,
                             hipStream_t stream, rt_batch_tuning* best, float* msPerFrame);
 

@@ -25,6 +25,7 @@
 #pragma once
 #include "orc_scene.h"
 #include <algorithm>
+#include <cstring>
 #include <numeric>
 #include <vector>
 
@@ -438,11 +439,26 @@ static inline bool traceAny(const BVH& B, const RayIn& r, TraceStats* st) {
 // 4-wide traversal. At each node the hit children are ordered by entry distance
 // (slot order on ties); leaves are tested in that order as they come, the nearest
 // internal child is visited next and the other internal children are pushed
-// farthest first.
+// farthest first. A closest-hit query keeps each pushed node's entry distance,
+// truncated to the float's top 16 bits (a value <= the distance; -inf for a
+// negative one), and skips a popped node whose stored distance exceeds the
+// closest hit so far: all of its children would miss their box tests, since a
+// node's box holds its children's boxes and the slab distances are monotone in
+// the bounds. Hits are those of the full walk; the work counters are lower.
+static inline uint16_t stackKey(float t) {
+    uint32_t b; std::memcpy(&b, &t, 4);
+    return (b >> 31) ? (uint16_t)0xff80u : (uint16_t)(b >> 16);
+}
+static inline float stackKeyValue(uint16_t k) {
+    const uint32_t b = (uint32_t)k << 16;
+    float t; std::memcpy(&t, &b, 4);
+    return t;
+}
+
 template <bool kAny>
 static inline bool traverse4(const BVH& B, const RayIn& r, bool skipLights, float& tBest, Hit& h, TraceStats* st) {
     if (B.nodes4.empty() || !rayFinite(r)) return false;
-    uint32_t stack[512]; int sp = 0;
+    uint32_t stack[512]; uint16_t stackT[512]; int sp = 0;
     uint32_t node = 0;
     bool found = false;
     for (;;) {
@@ -459,7 +475,7 @@ static inline bool traverse4(const BVH& B, const RayIn& r, bool skipLights, floa
             tk[j] = te; ord[j] = k;
         }
         uint32_t next = EmptyRef;
-        uint32_t pend[3]; int np = 0;
+        uint32_t pend[3]; uint16_t pendT[3]; int np = 0;
         for (int q = 0; q < nh; q++) {
             uint32_t ref = nd.child[ord[q]];
             if (ref & LeafFlag) {
@@ -476,14 +492,16 @@ static inline bool traverse4(const BVH& B, const RayIn& r, bool skipLights, floa
             } else if (next == EmptyRef) {
                 next = ref;
             } else {
+                pendT[np] = stackKey(tk[q]);
                 pend[np++] = ref;
             }
         }
-        while (np > 0) stack[sp++] = pend[--np];
-        if (next == EmptyRef) {
-            if (sp == 0) break;
-            next = stack[--sp];
+        while (np > 0) { np--; stack[sp] = pend[np]; stackT[sp] = pendT[np]; sp++; }
+        while (next == EmptyRef && sp > 0) {
+            sp--;
+            if (kAny || !(stackKeyValue(stackT[sp]) > tBest)) next = stack[sp];
         }
+        if (next == EmptyRef) break;
         node = next;
     }
     return found;

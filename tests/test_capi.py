@@ -43,7 +43,7 @@ def test_struct_sizes_match_reference_layouts():
     assert C.sizeof(rt.MlpConfig) == 72
     # rt_view: scene pointer + 2 cameras + rt_frame (112 B) + settings (44 B) + 4 B padding + 2 networks
     assert C.sizeof(rt.Frame) == 112 and C.sizeof(rt.View) == 8 + 2 * 352 + 112 + 44 + 4 + 2 * 8
-    assert C.sizeof(rt.BatchTuning) == 8
+    assert C.sizeof(rt.BatchTuning) == 24
     assert C.sizeof(rt.DIReservoir) == 64      # layouts.glsl DIReservoir (std430)
     assert C.sizeof(rt.DISettings) == 16       # TestReSTIR::Settings push constant
     assert C.sizeof(rt.DIReservoirs) == 24

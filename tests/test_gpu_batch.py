@@ -1,7 +1,9 @@
 """Batched views against per-view launches: views of two scenes with different sizes
 and depths rendered together (one G-buffer, one persistent path-trace, one temporal and
 one spatial launch per frame) must give the same G-buffers, reservoirs and outputs, bit
-for bit, as each view rendered on its own; and the autotuner returns a valid tuning."""
+for bit, as each view rendered on its own, under every tuning (register budget, queue
+tile shape, persistent grid, eager launches or graph replay); and the autotuner returns
+a valid tuning."""
 import ctypes as C
 
 import numpy as np
@@ -17,6 +19,20 @@ rt = gpuhelp.rt
 
 # (scene config, width, height, max depth)
 VIEWS = [(0, 96, 64, 4), (2, 40, 24, 3), (0, 130, 71, 8), (2, 64, 64, 6), (0, 8, 8, 5)]
+
+# per frame: (persistentGroups, xcdSwizzle, waves, queueTile, graph, shading); the third
+# and fourth frames replay the graph built by the second with updated kernel arguments;
+# the last four run the wavefront (material-sorted) variant, with 7 workgroups (many
+# trips of its grid-stride loops) and as a graph built and then replayed
+TUNINGS = [(0, 16, 0, 0, 0, 0), (0, 16, 3, 1, 1, 0), (7, 16, 3, 1, 1, 0), (0, 0, 5, 2, 1, 0), (0, 128, 4, 0, 0, 1),
+           (7, 16, 4, 0, 0, 1), (700, 16, 4, 0, 1, 1), (700, 16, 4, 0, 1, 1)]
+
+
+def tuning(persistent, swizzle, waves, tile, graph, shading=0):
+    t = rt.BatchTuning()
+    t.persistentGroups, t.xcdSwizzle, t.waves, t.queueTile, t.graph = persistent, swizzle, waves, tile, graph
+    t.shading = shading
+    return t
 
 
 def make_view(run, sc, cams, s):
@@ -38,8 +54,9 @@ def test_batched_views_match_single_views():
     L = rt.core()
     rt.check(L.rt_batch_create(len(VIEWS), C.byref(b)), "batch create")
     try:
-        for frame in range(3):
-            print(f"[gpu] batch frame {frame}", flush=True)
+        for frame in range(len(TUNINGS)):
+            print(f"[gpu] batch frame {frame} tuning {TUNINGS[frame]}", flush=True)
+            rt.check(L.rt_batch_set_tuning(b, C.byref(tuning(*TUNINGS[frame]))), "tuning")
             views = (rt.View * len(VIEWS))()
             for i, (c, W, H, _) in enumerate(VIEWS):
                 cams = hs[c].camera(frame, W, H)
@@ -63,12 +80,20 @@ def test_batched_views_match_single_views():
         # autotuner: a valid tuning and a finite time
         views = (rt.View * len(VIEWS))()
         for i, (c, W, H, _) in enumerate(VIEWS):
-            views[i] = make_view(batched[i], ds[c], hs[c].camera(3, W, H), sets[i])
+            views[i] = make_view(batched[i], ds[c], hs[c].camera(len(TUNINGS), W, H), sets[i])
         best = rt.BatchTuning()
         ms = C.c_float()
         rt.check(L.rt_batch_autotune(b, views, len(VIEWS), 2, gpuhelp.stream(), C.byref(best), C.byref(ms)), "tune")
-        print(f"[gpu] autotune: groups {best.persistentGroups} swizzle {best.xcdSwizzle} {ms.value:.3f} ms", flush=True)
-        assert best.persistentGroups > 0 and best.xcdSwizzle in (0, 16, 128) and 0 < ms.value < 1e4
+        print(f"[gpu] autotune: groups {best.persistentGroups} swizzle {best.xcdSwizzle} waves {best.waves} "
+              f"tile {best.queueTile} graph {best.graph} shading {best.shading} {ms.value:.3f} ms", flush=True)
+        assert best.xcdSwizzle in (0, 16, 128) and best.waves in (3, 4, 5) and best.queueTile in (0, 1, 2)
+        assert best.graph in (0, 1) and best.shading in (0, 1) and 0 < ms.value < 1e4
+        got = rt.BatchTuning()
+        rt.check(L.rt_batch_get_tuning(b, C.byref(got)), "get tuning")
+        assert bytes(got) == bytes(best)
+        # invalid tunings are rejected
+        for bad in ((0, 16, 6, 0, 0), (0, 16, 4, 3, 0), (0, 16, 4, 0, 2), (0, 16, 4, 0, 0, 2)):
+            assert L.rt_batch_set_tuning(b, C.byref(tuning(*bad))) != 0
     finally:
         L.rt_batch_destroy(b)
 
@@ -109,13 +134,11 @@ def test_batched_learned_forms_match_single_views():
     b = C.c_void_p()
     L = rt.core()
     rt.check(L.rt_batch_create(len(FORM_VIEWS), C.byref(b)), "batch create")
-    tuning = rt.BatchTuning()
-    tuning.persistentGroups, tuning.xcdSwizzle = 0, 5000
-    assert L.rt_batch_set_tuning(b, C.byref(tuning)) != 0  # more than 4096 tiles per XCD chunk
-    tuning.persistentGroups, tuning.xcdSwizzle = 0, 3  # runs of 3 tiles per XCD: partial last runs
-    rt.check(L.rt_batch_set_tuning(b, C.byref(tuning)), "tuning")
+    assert L.rt_batch_set_tuning(b, C.byref(tuning(0, 5000, 0, 0, 0))) != 0  # more than 4096 tiles per XCD chunk
     try:
         for frame in range(3):
+            # runs of 3 tiles per XCD (partial last runs); graph replay from the second frame on
+            rt.check(L.rt_batch_set_tuning(b, C.byref(tuning(0, 3, 4, frame, int(frame > 0)))), "tuning")
             print(f"[gpu] batched forms frame {frame}", flush=True)
             views = (rt.View * len(FORM_VIEWS))()
             for i, (c, W, H, _, lt, ub, gd) in enumerate(FORM_VIEWS):
@@ -171,6 +194,75 @@ def test_batched_learned_forms_match_single_views():
             views[k] = v
         assert L.rt_batch_frame(b, views, 2, gpuhelp.stream()) != 0
         assert b"more than one view" in rt.core().rt_last_error()
+        # the wavefront variant does not run the learned proposal: refused, not silently replaced
+        rt.check(L.rt_batch_set_tuning(b, C.byref(tuning(0, 16, 4, 0, 0, 1))), "tuning")
+        views = (rt.View * 1)()
+        c, W, H = FORM_VIEWS[0][:3]
+        v = make_view(batched[0], ds[c], hs[c].camera(3, W, H), sets[0])
+        v.target, v.guide = _handle(b_nets[0][0].handle), _handle(b_nets[0][1].handle)
+        views[0] = v
+        assert L.rt_batch_frame(b, views, 1, gpuhelp.stream()) != 0
+        assert b"learned proposal" in rt.core().rt_last_error()
+    finally:
+        L.rt_batch_destroy(b)
+
+
+# (scene config, width, height, max depth, learned target, unbiased reuse)
+WF_VIEWS = [(1, 72, 40, 6, True, False), (0, 40, 24, 5, False, True), (1, 33, 17, 7, True, True), (0, 24, 24, 3, False, False)]
+
+
+def test_wavefront_learned_and_unbiased_views():
+    """The wavefront variant beside views with the learned target and the unbiased weights, eager
+    and as a graph: every view equals the same view rendered alone, bit for bit."""
+    from test_gpu_learned import scene_bounds
+
+    hs = {c: rt.HostScene(c, 1) for c in (0, 1)}
+    ds = {c: rt.DeviceScene(h.arrays) for c, h in hs.items()}
+    single = [gpuhelp.GrisRun(ds[v[0]], v[1], v[2]) for v in WF_VIEWS]
+    batched = [gpuhelp.GrisRun(ds[v[0]], v[1], v[2]) for v in WF_VIEWS]
+
+    def net(i, c, learned):
+        lo, hi = scene_bounds(ds[c])
+        return rt.Mlp(rt.mlp_config(lo, hi, log2_table=15, seed=70 + i)) if learned else None
+
+    s_nets = [net(i, v[0], v[4]) for i, v in enumerate(WF_VIEWS)]
+    b_nets = [net(i, v[0], v[4]) for i, v in enumerate(WF_VIEWS)]
+    sets = [rt.settings(shift=2, temporal=True, spatial=True, max_depth=d, cap=20, radius=10.0, learned=lt, unbiased=ub)
+            for _, _, _, d, lt, ub in WF_VIEWS]
+    b = C.c_void_p()
+    L = rt.core()
+    rt.check(L.rt_batch_create(len(WF_VIEWS), C.byref(b)), "batch create")
+    try:
+        for frame in range(3):
+            rt.check(L.rt_batch_set_tuning(b, C.byref(tuning(64, 16, 4, 0, int(frame > 0), 1))), "tuning")
+            print(f"[gpu] wavefront forms frame {frame}", flush=True)
+            views = (rt.View * len(WF_VIEWS))()
+            for i, (c, W, H, _, lt, ub) in enumerate(WF_VIEWS):
+                cams = hs[c].camera(frame, W, H)
+                r, t = single[i], s_nets[i]
+                r.gbuffer(cams[0])
+                r.path_trace(cams, sets[i])
+                if lt:
+                    r.learned_refresh(t, cams)
+                r.temporal(cams, sets[i])
+                r.spatial(cams, sets[i])
+                if lt:
+                    rt.check(L.rt_mlp_train_frame(t.handle, None, gpuhelp.stream()), "train frame")
+                v = make_view(batched[i], ds[c], cams, sets[i])
+                v.target = _handle(b_nets[i].handle) if lt else None
+                views[i] = v
+            rt.check(L.rt_batch_frame(b, views, len(WF_VIEWS), gpuhelp.stream()), "batch frame")
+            torch.cuda.synchronize()
+            for i, (c, W, H, _, lt, ub) in enumerate(WF_VIEWS):
+                a, z = single[i], batched[i]
+                assert not gpuhelp.compare_reservoirs(a.reservoirs(a.resv[a.cur]), z.reservoirs(z.resv[z.cur])).any()
+                assert not gpuhelp.compare_reservoirs(a.reservoirs(a.temp), z.reservoirs(z.temp)).any()
+                if lt:
+                    assert np.array_equal(a.learned(a.resv[a.cur]).view(np.uint32),
+                                          z.learned(z.resv[z.cur]).view(np.uint32)), f"view {i}: learned planes"
+                assert np.array_equal(a.output(1).view(np.uint32), z.output(1).view(np.uint32)), f"view {i}: output"
+                a.flip()
+                z.flip()
     finally:
         L.rt_batch_destroy(b)
 

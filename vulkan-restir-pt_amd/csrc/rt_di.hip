@@ -260,7 +260,7 @@ template <bool kCount>
 __global__ __launch_bounds__(kTileThreads, RT_WAVES_DI) void k_di_path_gen(DevScene S, rt_camera cam, GB g,
                                                                           uint32_t sampleMode, DIPlanes thisR,
                                                                           TraceCounters* ctr) {
-    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    __shared__ StackLDS<kTileThreads> stk;
     PIXEL_INDEX
     if (x >= g.W || y >= g.H) return;
     DIPixel px;
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_DI) void k_di_path_gen(DevSc
     uint32_t rng = makeSeedPix(cam.seed, x, y);
     uint32_t resvRng = ~rng;
     DIResv resv = diZero();
-    (void)sampleLi<kCount>(S, TStack{stk + threadIdx.x, kTileThreads}, sampleMode, px.surf, loadMat(S, px.surf.matIndex),
+    (void)sampleLi<kCount>(S, laneStack(stk), sampleMode, px.surf, loadMat(S, px.surf.matIndex),
                            -px.ray.dir, rng, resvRng, resv, ctr);
     diStore(thisR, (size_t)y * g.W + x, resv);
 }
@@ -280,10 +280,10 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_DI) void k_di_temporal(DevSc
                                                                           GB g, GB gPrev, rt_di_settings set,
                                                                           DIPlanes thisR, DIPlanes prevR, DIPlanes tempR,
                                                                           TraceCounters* ctr) {
-    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    __shared__ StackLDS<kTileThreads> stk;
     PIXEL_INDEX
     if (x >= g.W || y >= g.H) return;
-    TStack st{stk + threadIdx.x, kTileThreads};
+    TStack st = laneStack(stk);
     DIPixel px;
     if (!diPixel(cam, g, x, y, px)) return;
     const f2 motion = fetchMotion(g, (int)x, (int)y);
@@ -326,10 +326,10 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_DI) void k_di_spatial(DevSce
                                                                          rt_di_settings set, DIPlanes tempR,
                                                                          DIPlanes thisR, float* directOut,
                                                                          TraceCounters* ctr) {
-    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    __shared__ StackLDS<kTileThreads> stk;
     PIXEL_INDEX
     if (x >= g.W || y >= g.H) return;
-    TStack st{stk + threadIdx.x, kTileThreads};
+    TStack st = laneStack(stk);
     const size_t p = (size_t)y * g.W + x;
     f3 radiance = s3(0.0f);
     DIPixel px;

@@ -97,13 +97,13 @@ RT_DEV bool unpackGB(f4 dn, uint2 am, GPix& p) {
 }
 
 // ---- traversal wrappers (ray_query.glsl:6-70) ------------------------------------------------
-template <bool kCount>
+template <bool kCount, bool kCull = true>
 RT_DEV Isec traceClosestHit(const DevScene& S, TStack st, f3 o, float tmin, f3 d, float tmax, TraceCounters* ctr,
                             bool skipLights = false, float* tOut = nullptr) {
     RayT r = makeRay(o, d, tmin, tmax);
     Isec h; h.b1 = 0.0f; h.b2 = 0.0f; h.inst = RT_INVALID_HIT; h.prim = 0;
     float tBest = tmax;
-    bool hit = traverse<false, kCount>(S, r, skipLights, st, tBest, h, ctr);
+    bool hit = traverse<false, kCount, kCull>(S, r, skipLights, st, tBest, h, ctr);
     if (!hit) { h.inst = RT_INVALID_HIT; h.b1 = 0.0f; h.b2 = 0.0f; h.prim = 0; }
     if (tOut) *tOut = hit ? tBest : 0.0f;
     return h;

@@ -11,6 +11,12 @@
 //
 // Per lane: a short stack in LDS (strided by the block size so a wave's pushes
 // and pops hit distinct banks) that spills to a private array for deep trees.
+// Closest-hit traversal keeps each pushed node's entry distance beside it (the
+// top 16 bits of the float, i.e. rounded toward zero) and drops a popped node
+// that lies beyond the closest hit found since it was pushed. That changes no
+// result: a node's box contains its children's boxes, and the slab distances are
+// monotone in the bounds under IEEE rounding, so every child of such a node
+// would fail its own box test against the same closest hit.
 // Leaves hold one triangle, tested with the watertight algorithm of Woop,
 // Benthin and Wald (JCGT 2013) including its double-precision fallback. Box
 // tests use (bound - o) * (1 / d) with IEEE minNum/maxNum so that 0 * inf slabs
@@ -19,7 +25,7 @@
 #include "rt_dscene.h"
 
 #ifndef RT_LDS_STACK
-#define RT_LDS_STACK 16
+#define RT_LDS_STACK 12
 #endif
 #define RT_SPILL_STACK 240
 #define RT_LEAF 0x80000000u
@@ -96,11 +102,33 @@ RT_DEV bool triHit(const RayT& r, f3 v0, f3 v1, f3 v2, float tBest, float& tOut,
     return true;
 }
 
-// LDS stack slice for this lane: stride = threads per block.
+// LDS stack slice for this lane: node references and their entry distances
+// (16-bit, see above), stride = threads per block.
 struct TStack {
     uint32_t* lds;
+    uint16_t* ldsT;
     uint32_t stride;
 };
+
+// The LDS of a block's traversal stacks.
+template <int kThreads>
+struct StackLDS {
+    uint32_t ref[RT_LDS_STACK * kThreads];
+    uint16_t t[RT_LDS_STACK * kThreads];
+};
+
+template <int kThreads>
+RT_DEV TStack laneStack(StackLDS<kThreads>& s) {
+    return TStack{s.ref + threadIdx.x, s.t + threadIdx.x, (uint32_t)kThreads};
+}
+
+// Entry distance as stored: the float's top 16 bits, which for t >= 0 is a
+// value <= t; a negative t is stored as -inf so that it never drops a node.
+RT_DEV uint16_t stackKey(float t) {
+    const uint32_t b = __float_as_uint(t);
+    return (b >> 31) ? (uint16_t)0xff80u : (uint16_t)(b >> 16);
+}
+RT_DEV float stackKeyValue(uint16_t k) { return __uint_as_float((uint32_t)k << 16); }
 
 // (t, slot) key order; a missed child carries t = +inf and sorts last.
 RT_DEV void cswap(float& ta, uint32_t& ra, float& tb, uint32_t& rb) {
@@ -110,10 +138,15 @@ RT_DEV void cswap(float& ta, uint32_t& ra, float& tb, uint32_t& rb) {
     ta = t; ra = r;
 }
 
-template <bool kAny, bool kCount>
+// kCull: closest-hit queries drop popped nodes beyond the closest hit (the
+// result is the same either way; the reuse passes' replays, which rarely pop
+// such nodes, leave it off to save registers).
+template <bool kAny, bool kCount, bool kCull = true>
 RT_DEV bool traverse(const DevScene& S, const RayT& r, bool skipLights, TStack st, float& tBest, Isec& hit,
                      TraceCounters* ctr) {
     uint32_t spill[RT_SPILL_STACK];
+    constexpr bool kKeep = kCull && !kAny;  // entry distances kept on the stack
+    uint16_t spillT[kKeep ? RT_SPILL_STACK : 1];
     int sp = 0;
     bool found = false;
     unsigned long long cn = 0, ct = 0;
@@ -148,6 +181,7 @@ RT_DEV bool traverse(const DevScene& S, const RayT& r, bool skipLights, TStack s
         cswap(k1, r1, k2, r2); cswap(k0, r0, k1, r1);
         cswap(k2, r2, k3, r3); cswap(k1, r1, k2, r2); cswap(k0, r0, k1, r1);
         uint32_t rs[4] = {r0, r1, r2, r3};
+        const float ks[4] = {k0, k1, k2, k3};
         // leaves in sorted order, one triangle test per loop trip: a wave runs
         // as many trips as its lane with the most leaf hits (a leaf is tested
         // by ~1 lane in 5 per node visit), not one test per slot position
@@ -189,16 +223,26 @@ RT_DEV bool traverse(const DevScene& S, const RayT& r, bool skipLights, TStack s
             if (ref == RT_EMPTY || (ref & RT_LEAF)) continue;
             if (q == first) { next = ref; continue; }
             if (sp < RT_LDS_STACK + RT_SPILL_STACK) {  // the build bounds the stack depth
-                if (sp < RT_LDS_STACK) st.lds[sp * st.stride] = ref;
-                else spill[sp - RT_LDS_STACK] = ref;
+                if (sp < RT_LDS_STACK) {
+                    st.lds[sp * st.stride] = ref;
+                    if (kKeep) st.ldsT[sp * st.stride] = stackKey(ks[q]);
+                } else {
+                    spill[sp - RT_LDS_STACK] = ref;
+                    if (kKeep) spillT[sp - RT_LDS_STACK] = stackKey(ks[q]);
+                }
                 sp++;
             }
         }
-        if (next == RT_EMPTY) {
-            if (sp == 0) break;
+        // pop; a shadow ray's tBest is fixed, so only closest-hit rays drop nodes
+        while (next == RT_EMPTY && sp > 0) {
             sp--;
-            next = (sp < RT_LDS_STACK) ? st.lds[sp * st.stride] : spill[sp - RT_LDS_STACK];
+            if (sp < RT_LDS_STACK) {
+                if (!kKeep || !(stackKeyValue(st.ldsT[sp * st.stride]) > tBest)) next = st.lds[sp * st.stride];
+            } else {
+                if (!kKeep || !(stackKeyValue(spillT[sp - RT_LDS_STACK]) > tBest)) next = spill[sp - RT_LDS_STACK];
+            }
         }
+        if (next == RT_EMPTY) break;
         node = next;
     }
     if (kCount) { atomicAdd(&ctr->nodes, cn); atomicAdd(&ctr->tris, ct); atomicAdd(&ctr->rays, 1ull); }

@@ -67,10 +67,10 @@ template <bool kCount>
 __global__ __launch_bounds__(kTileThreads, RT_WAVES_GI) void k_gi_temporal(DevScene S, rt_camera cam, rt_camera prevCam,
                                                                           GB g, GB gPrev, GIPlanes thisR, GIPlanes prevR,
                                                                           float* indirectOut, TraceCounters* ctr) {
-    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    __shared__ StackLDS<kTileThreads> stk;
     PIXEL_INDEX
     if (x >= g.W || y >= g.H) return;
-    TStack st{stk + threadIdx.x, kTileThreads};
+    TStack st = laneStack(stk);
     const size_t p = (size_t)y * g.W + x;
     const f2 uv = pixelUV(x, y, g.W, g.H);
     GPix gp;

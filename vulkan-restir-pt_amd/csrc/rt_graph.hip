@@ -4,10 +4,10 @@
 // the eager entry points, so the kernels and arguments are identical), turned
 // into a chain of kernel nodes on the first launch of a parity, and on later
 // launches re-recorded with the new cameras and written into the instantiated
-// graph with hipGraphExecKernelNodeSetParams before it is replayed. The graph
-// holds no memset or copy nodes: counters that a frame needs cleared are
-// cleared by kernels (the pixel queue re-arms itself, the gradient tables are
-// zeroed by fill kernels).
+// graph with hipGraphExecKernelNodeSetParams before it is replayed
+// (rti::KernelChain, also used by batched frames). The graph holds no memset or
+// copy nodes: counters that a frame needs cleared are cleared by kernels (the
+// pixel queue re-arms itself, the gradient tables are zeroed by fill kernels).
 #include "rt_internal.h"
 
 namespace rti {
@@ -32,10 +32,7 @@ struct rt_frame_graph {
     rt_mlp* mlp = nullptr;
     rt_mlp* guide = nullptr;          // the learned proposal's network (settings.guidedProposal)
     rtd::WorkQueue* queue = nullptr;  // this graph's own pixel queue
-    hipGraph_t graph[2] = {nullptr, nullptr};
-    hipGraphExec_t exec[2] = {nullptr, nullptr};
-    std::vector<hipGraphNode_t> nodes[2];
-    std::vector<const void*> funcs[2];
+    rti::KernelChain chain[2];
 };
 
 namespace {
@@ -70,43 +67,6 @@ hipError_t recordFrame(rt_frame_graph* g, uint32_t parity, const rt_camera* cams
         }
     }
     rti::tlsRecorder = nullptr;
-    return e;
-}
-
-hipKernelNodeParams nodeParams(rti::KernelLaunch& k) {
-    hipKernelNodeParams p{};
-    p.func = const_cast<void*>(k.func);
-    p.gridDim = k.grid;
-    p.blockDim = k.block;
-    p.sharedMemBytes = 0;
-    p.kernelParams = k.args.data();
-    p.extra = nullptr;
-    return p;
-}
-
-void dropGraph(rt_frame_graph* g, uint32_t j) {
-    if (g->exec[j]) (void)hipGraphExecDestroy(g->exec[j]);
-    if (g->graph[j]) (void)hipGraphDestroy(g->graph[j]);
-    g->exec[j] = nullptr;
-    g->graph[j] = nullptr;
-    g->nodes[j].clear();
-    g->funcs[j].clear();
-}
-
-hipError_t buildGraph(rt_frame_graph* g, uint32_t j, std::vector<rti::KernelLaunch>& L) {
-    dropGraph(g, j);
-    hipError_t e = hipGraphCreate(&g->graph[j], 0);
-    hipGraphNode_t prev = nullptr;
-    for (size_t i = 0; e == hipSuccess && i < L.size(); i++) {
-        hipKernelNodeParams p = nodeParams(L[i]);
-        hipGraphNode_t node = nullptr;
-        e = hipGraphAddKernelNode(&node, g->graph[j], prev ? &prev : nullptr, prev ? 1 : 0, &p);
-        g->nodes[j].push_back(node);
-        g->funcs[j].push_back(L[i].func);
-        prev = node;
-    }
-    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec[j], g->graph[j], nullptr, nullptr, 0);
-    if (e != hipSuccess) dropGraph(g, j);
     return e;
 }
 
@@ -171,27 +131,82 @@ rt_status rt_frame_graph_launch(rt_frame_graph* g, uint32_t parity, const rt_cam
         return invalid("camera filmSize differs from the frame size");
     std::vector<rti::KernelLaunch> L;
     RT_HIP(recordFrame(g, parity, cams, L));
-    bool same = g->exec[parity] && L.size() == g->nodes[parity].size();
-    for (size_t i = 0; same && i < L.size(); i++) same = L[i].func == g->funcs[parity][i];
-    if (same) {
-        for (size_t i = 0; i < L.size(); i++) {
-            hipKernelNodeParams p = nodeParams(L[i]);
-            RT_HIP(hipGraphExecKernelNodeSetParams(g->exec[parity], g->nodes[parity][i], &p));
-        }
-    } else {
-        RT_HIP(buildGraph(g, parity, L));
-    }
-    RT_HIP(hipGraphLaunch(g->exec[parity], stream));
+    RT_HIP(g->chain[parity].update(L));
+    RT_HIP(g->chain[parity].launch(stream));
     return RT_OK;
 }
 
 rt_status rt_frame_graph_destroy(rt_frame_graph* g) {
     if (!g) return invalid("graph is null");
-    dropGraph(g, 0);
-    dropGraph(g, 1);
+    g->chain[0].reset();
+    g->chain[1].reset();
     (void)hipFree(g->queue);
     delete g;
     return RT_OK;
 }
 
 }  // extern "C"
+
+namespace rti {
+
+namespace {
+hipKernelNodeParams nodeParams(KernelLaunch& k) {
+    hipKernelNodeParams p{};
+    p.func = const_cast<void*>(k.func);
+    p.gridDim = k.grid;
+    p.blockDim = k.block;
+    p.sharedMemBytes = 0;
+    p.kernelParams = k.args.data();
+    p.extra = nullptr;
+    return p;
+}
+bool sameDim(const dim3& a, const dim3& b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+}  // namespace
+
+void KernelChain::reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    nodes.clear();
+    funcs.clear();
+    grids.clear();
+    blocks.clear();
+}
+
+hipError_t KernelChain::update(std::vector<KernelLaunch>& L) {
+    bool same = exec && L.size() == nodes.size();
+    for (size_t i = 0; same && i < L.size(); i++)
+        same = L[i].func == funcs[i] && sameDim(L[i].grid, grids[i]) && sameDim(L[i].block, blocks[i]);
+    if (same) {
+        for (size_t i = 0; i < L.size(); i++) {
+            hipKernelNodeParams p = nodeParams(L[i]);
+            hipError_t e = hipGraphExecKernelNodeSetParams(exec, nodes[i], &p);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    reset();
+    hipError_t e = hipGraphCreate(&graph, 0);
+    hipGraphNode_t prev = nullptr;
+    for (size_t i = 0; e == hipSuccess && i < L.size(); i++) {
+        hipKernelNodeParams p = nodeParams(L[i]);
+        hipGraphNode_t node = nullptr;
+        e = hipGraphAddKernelNode(&node, graph, prev ? &prev : nullptr, prev ? 1 : 0, &p);
+        nodes.push_back(node);
+        funcs.push_back(L[i].func);
+        grids.push_back(L[i].grid);
+        blocks.push_back(L[i].block);
+        prev = node;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) reset();
+    return e;
+}
+
+hipError_t KernelChain::launch(hipStream_t stream) const {
+    if (!exec) return hipErrorInvalidValue;
+    return hipGraphLaunch(exec, stream);
+}
+
+}  // namespace rti

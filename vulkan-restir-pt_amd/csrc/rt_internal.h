@@ -143,6 +143,39 @@ hipError_t launch(void (*kernel)(P...), dim3 grid, dim3 block, hipStream_t strea
     return hipLaunchKernel(reinterpret_cast<const void*>(kernel), grid, block, ptrs, 0, stream);
 }
 
+// A chain of kernel nodes replayed as one HIP graph. update() takes a frame's
+// recorded launches: when the kernels and launch shapes are those of the
+// instantiated graph it writes the new arguments into its nodes
+// (hipGraphExecKernelNodeSetParams), otherwise it builds and instantiates the
+// graph anew. The graph holds kernel nodes only (no memset or copy nodes).
+struct KernelChain {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipGraphNode_t> nodes;
+    std::vector<const void*> funcs;
+    std::vector<dim3> grids, blocks;
+    hipError_t update(std::vector<KernelLaunch>& launches);
+    hipError_t launch(hipStream_t stream) const;
+    void reset();
+    ~KernelChain() { reset(); }
+};
+
+// Path state of the wavefront (material-sorted) variant of the batched path
+// trace, for up to `capacity` paths (one per pixel of the batch).
+struct WavefrontBuffers {
+    float* words = nullptr;       // per-path state planes
+    uint32_t* lists = nullptr;    // 2 x capacity live-path lists
+    uint32_t* sorted = nullptr;   // capacity: a bounce's live paths in bucket order
+    uint32_t* keys = nullptr;     // capacity: buckets
+    uint32_t* counters = nullptr; // list lengths, bucket counts and cursors
+    uint32_t capacity = 0;
+};
+
+// Register budgets (waves per SIMD) of the batched path-trace variants that a
+// batch's tuning chooses from.
+constexpr uint32_t kBatchWaveVariants = 3;
+constexpr int kBatchWaves[kBatchWaveVariants] = {3, 4, 5};
+
 }  // namespace rti
 
 #define RT_HIP(call) do { if (!rti::hipCheck((call), #call)) return RT_ERR_HIP; } while (0)

@@ -62,10 +62,10 @@ template <bool kCount>
 __global__ __launch_bounds__(kTileThreads, RT_WAVES_TRACE) void k_trace_rays(DevScene S, const float* __restrict__ rays8, uint32_t n,
                                                              int mode, int skipLights, float* t, float* bary2,
                                                              uint32_t* inst, uint32_t* prim, TraceCounters* ctr) {
-    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    __shared__ StackLDS<kTileThreads> stk;
     uint32_t i = blockIdx.x * kTileThreads + threadIdx.x;
     if (i >= n) return;
-    TStack st{stk + threadIdx.x, kTileThreads};
+    TStack st = laneStack(stk);
     const float* q = rays8 + 8 * (size_t)i;
     f3 o = mk3(q[0], q[1], q[2]), d = mk3(q[3], q[4], q[5]);
     if (mode == 1) {
@@ -135,9 +135,9 @@ template <bool kCount>
 __global__ __launch_bounds__(kTileThreads, RT_WAVES_GBUFFER) void k_gbuffer(DevScene S, rt_camera cam, float4* __restrict__ dnOut,
                                                           uint2* __restrict__ amOut, uint32_t* __restrict__ mvOut,
                                                           TraceCounters* ctr) {
-    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    __shared__ StackLDS<kTileThreads> stk;
     PIXEL_INDEX
-    gbufferPixel<kCount>(S, cam, dnOut, amOut, mvOut, x, y, TStack{stk + threadIdx.x, kTileThreads}, ctr);
+    gbufferPixel<kCount>(S, cam, dnOut, amOut, mvOut, x, y, laneStack(stk), ctr);
 }
 
 // ================================================================================================
@@ -265,10 +265,10 @@ template <bool kCount>
 __global__ __launch_bounds__(kTileThreads, RT_WAVES_PT_PLAIN) void k_pt_plain(DevScene S, rt_camera cam, GB g, uint32_t maxDepth,
                                                            uint32_t which, float* directOut, float* indirectOut,
                                                            TraceCounters* ctr) {
-    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    __shared__ StackLDS<kTileThreads> stk;
     PIXEL_INDEX
     if (x >= g.W || y >= g.H) return;
-    TStack st{stk + threadIdx.x, kTileThreads};
+    TStack st = laneStack(stk);
     size_t p = (size_t)y * g.W + x;
     if (which & 1u) accumulate(directOut, p, directIllumination2<kCount>(S, st, cam, g, x, y, ctr), cam.frameIndex);
     if (which & 2u) accumulate(indirectOut, p, indirectIllumination<kCount>(S, st, cam, g, x, y, maxDepth, ctr), cam.frameIndex);
@@ -369,22 +369,25 @@ struct PTPath {
     f3 dir0;  // direction sampled at the primary vertex (training record of the learned proposal)
 };
 
-// The stream-RIS pick of a lane lives in LDS (18 words, stride = lanes per
-// workgroup): it is written only when a candidate is picked and read once at
-// the end of the path, so keeping it out of the registers buys occupancy.
+// The stream-RIS pick of a path: 18 words at p[0], p[stride], ... -- in LDS for
+// the persistent kernels (stride = lanes per workgroup), in the path-state
+// planes for the wavefront variant. It is written only when a candidate is
+// picked and read once at the end of the path, so keeping it out of the
+// registers buys occupancy.
 constexpr int kStreamWords = 18;
-RT_DEV void storeStream(float* slot, const Resv& r) {
+struct StreamSlot { float* p; uint32_t stride; };
+RT_DEV void storeStream(StreamSlot slot, const Resv& r) {
     const float v[kStreamWords] = {r.rcIsec.b1, r.rcIsec.b2, __uint_as_float(r.rcIsec.inst), __uint_as_float(r.rcIsec.prim),
                                    r.rcLi.x, r.rcLi.y, r.rcLi.z, __uint_as_float(r.rcRng),
                                    r.rcWi.x, r.rcWi.y, r.rcWi.z, __uint_as_float(r.flags),
                                    r.rcPrevSamplePdf, r.rcJacobian, r.F.x, r.F.y, r.F.z, __uint_as_float(r.primaryRng)};
 #pragma unroll
-    for (int i = 0; i < kStreamWords; i++) slot[i * kTileThreads] = v[i];
+    for (int i = 0; i < kStreamWords; i++) slot.p[i * slot.stride] = v[i];
 }
-RT_DEV Resv loadStream(const float* slot) {
+RT_DEV Resv loadStream(StreamSlot slot) {
     float v[kStreamWords];
 #pragma unroll
-    for (int i = 0; i < kStreamWords; i++) v[i] = slot[i * kTileThreads];
+    for (int i = 0; i < kStreamWords; i++) v[i] = slot.p[i * slot.stride];
     Resv r = zeroResv();
     r.rcIsec.b1 = v[0]; r.rcIsec.b2 = v[1]; r.rcIsec.inst = __float_as_uint(v[2]); r.rcIsec.prim = __float_as_uint(v[3]);
     r.rcLi = mk3(v[4], v[5], v[6]); r.rcRng = __float_as_uint(v[7]);
@@ -428,7 +431,7 @@ RT_DEV bool ptBegin(const DevScene& S, const rt_camera& cam, const GB& g, uint32
 // One vertex of the bounce loop, after its closest-hit query. Returns false
 // where the reference leaves the loop; otherwise the next ray is in rori/rdir.
 template <bool kCount>
-RT_DEV bool ptVertex(const DevScene& S, TStack st, float* slot, const rt_gris_settings& set, PTPath& P, const Surf& surf,
+RT_DEV bool ptVertex(const DevScene& S, TStack st, StreamSlot slot, const rt_gris_settings& set, PTPath& P, const Surf& surf,
                      const Mat& mat, const uint32_t bounce, TraceCounters* ctr,
                      const rtg::GuidePix& gpx = rtg::noGuide()) {
     uint32_t& rng = P.rng;
@@ -579,7 +582,7 @@ RT_DEV bool ptVertex(const DevScene& S, TStack st, float* slot, const rt_gris_se
 
 // After the bounce loop (gris_path_trace.glsl:280-306): last stream-RIS
 // candidate and the reservoir written for the pixel.
-RT_DEV Resv ptFinish(PTPath& P, const float* slot) {
+RT_DEV Resv ptFinish(PTPath& P, StreamSlot slot) {
     Resv out;
     bool lastPicked = false;
     if (P.sampleState == 2 && P.lastSampleState == 2) {
@@ -607,17 +610,21 @@ RT_DEV Resv ptFinish(PTPath& P, const float* slot) {
     return out;
 }
 
-// Pixels are handed out in 8x8-tile order, so the lanes refilled together by
-// one wave take neighbouring pixels. Every lane leaves the loop once the queue
+// Pixels are handed out in tile order (64-pixel tiles of 2^qShift x 2^(6-qShift)
+// pixels, 8x8 for qShift 3), so the lanes refilled together by one wave take
+// neighbouring pixels. Every lane leaves the loop once the queue
 // is drained; the last wave to finish re-arms the queue for the next launch.
 // Drains one view's pixel queue (the body of the persistent kernels): returns
 // when every lane of the wave is out of work.
 template <bool kCount>
 RT_DEV void ptRun(const DevScene& S, const rt_camera& cam, const GB& g, const rt_gris_settings& set, const ResvPlanes& outR,
-                  WorkQueue* q, TStack st, float* slot, TraceCounters* ctr, const rtg::GuideFrame& gf) {
+                  WorkQueue* q, uint32_t qShift, TStack st, float* lds, TraceCounters* ctr,
+                  const rtg::GuideFrame& gf) {
+    const StreamSlot slot{lds, (uint32_t)kTileThreads};
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t tilesX = (g.W + kTileX - 1) / kTileX;
-    const uint32_t total = tilesX * ((g.H + kTileY - 1) / kTileY) * kTileThreads;
+    const uint32_t tw = 1u << qShift, th = 64u >> qShift;
+    const uint32_t tilesX = (g.W + tw - 1) / tw;
+    const uint32_t total = tilesX * ((g.H + th - 1) / th) * kTileThreads;
     PTPath P;
     uint32_t pix = 0;
     bool live = false, drained = false;
@@ -636,7 +643,7 @@ RT_DEV void ptRun(const DevScene& S, const rt_camera& cam, const GB& g, const rt
             const uint32_t w = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
             if (w >= total) { drained = true; continue; }
             const uint32_t tile = w / kTileThreads, t = w % kTileThreads;
-            const uint32_t x = (tile % tilesX) * kTileX + t % kTileX, y = (tile / tilesX) * kTileY + t / kTileX;
+            const uint32_t x = (tile % tilesX) * tw + (t & (tw - 1u)), y = (tile / tilesX) * th + (t >> qShift);
             Surf surf; Mat mat;
             if (x >= g.W || y >= g.H || !ptBegin(S, cam, g, x, y, P, surf, mat)) continue;
             pix = y * g.W + x;
@@ -681,9 +688,9 @@ template <bool kCount, bool kGuided>
 __global__ __launch_bounds__(kTileThreads, RT_WAVES_PATH_TRACE) void k_gris_path_trace(
     DevScene S, rt_camera cam, GB g, rt_gris_settings set, ResvPlanes outR, WorkQueue* q, rtg::GuideFrame gf,
     TraceCounters* ctr) {
-    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    __shared__ StackLDS<kTileThreads> stk;
     __shared__ float picks[kStreamWords * kTileThreads];
-    ptRun<kCount>(S, cam, g, set, outR, q, TStack{stk + threadIdx.x, kTileThreads}, picks + threadIdx.x, ctr,
+    ptRun<kCount>(S, cam, g, set, outR, q, 3u, laneStack(stk), picks + threadIdx.x, ctr,
                   kGuided ? gf : rtg::GuideFrame{nullptr, nullptr, nullptr});
     queueDone(q);
 }
@@ -713,7 +720,7 @@ RT_DEV void replayHybrid(const DevScene& S, TStack st, const rt_gris_settings& s
     uint32_t state = 0;  // the path tracer's sampleState over the replayed vertices
     for (uint32_t bounce = 0; bounce < set.maxDepth; bounce++) {
         if (bounce > 0) {
-            isec = traceClosestHit<kCount>(S, st, rori, RT_MIN_RAY, rdir, RT_MAX_RAY, ctr);
+            isec = traceClosestHit<kCount, false>(S, st, rori, RT_MIN_RAY, rdir, RT_MAX_RAY, ctr);
             if (isec.inst == RT_INVALID_HIT) break;
             loadSurfaceInfo(S, isec, surf);
             mat = loadMat(S, surf.matIndex);
@@ -1018,10 +1025,10 @@ __global__ __launch_bounds__(kTileThreads, kUnbiased ? RT_WAVES_TEMPORAL_UB : RT
                                                                 GB gPrev, rt_gris_settings set, ResvPlanes thisR,
                                                                 ResvPlanes prevR, ResvPlanes tempR, rtg::GuideFrame gf,
                                                                 TraceCounters* ctr) {
-    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    __shared__ StackLDS<kTileThreads> stk;
     PIXEL_INDEX
     temporalPixel<kCount, kLearned, kUnbiased>(S, cam, prevCam, g, gPrev, set, thisR, prevR, tempR, x, y,
-                                    TStack{stk + threadIdx.x, kTileThreads}, ctr,
+                                    laneStack(stk), ctr,
                                     kGuided ? gf : rtg::GuideFrame{nullptr, nullptr, nullptr});
 }
 
@@ -1126,10 +1133,10 @@ template <bool kCount, bool kLearned, bool kUnbiased, bool kGuided>
 __global__ __launch_bounds__(kTileThreads, kUnbiased ? RT_WAVES_SPATIAL_UB : RT_WAVES_SPATIAL) void k_gris_spatial(DevScene S, rt_camera cam, GB g, rt_gris_settings set,
                                                                ResvPlanes tempR, ResvPlanes thisR, float* indirectOut,
                                                                rtg::GuideFrame gf, TraceCounters* ctr) {
-    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    __shared__ StackLDS<kTileThreads> stk;
     PIXEL_INDEX
     spatialPixel<kCount, kLearned, kUnbiased>(S, cam, g, set, tempR, thisR, indirectOut, x, y,
-                                              TStack{stk + threadIdx.x, kTileThreads}, ctr,
+                                              laneStack(stk), ctr,
                                               kGuided ? gf : rtg::GuideFrame{nullptr, nullptr, nullptr});
 }
 
@@ -1147,7 +1154,9 @@ struct ViewDesc {
     uint2* amOut;
     uint32_t* mvOut;
     rt_gris_settings set;
-    uint32_t tileBase, tilesX, tiles, pad;  // tileBase: first tile of the view in its launch
+    uint32_t tileBase, tilesX, tiles;  // tileBase: first tile of the view in its launch
+    uint32_t queueShift;               // path-trace queue tiles of 2^queueShift x 2^(6-queueShift) pixels
+    uint32_t pixBase, pad[3];          // wavefront variant: first path (pixel) of the view in the batch
     WorkQueue* queue;
     rtg::GuideFrame gf;  // the learned proposal's bindings (null unless the view samples with it)
 };
@@ -1186,27 +1195,27 @@ template <bool kCount>
 __global__ __launch_bounds__(kTileThreads, RT_WAVES_GBUFFER) void k_gbuffer_batch(const ViewDesc* __restrict__ V,
                                                                                   uint32_t numViews, uint32_t totalTiles,
                                                                                   uint32_t swz, TraceCounters* ctr) {
-    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    __shared__ StackLDS<kTileThreads> stk;
     uint32_t v, x, y;
     if (!batchPixel(V, numViews, totalTiles, swz, v, x, y)) return;
     const ViewDesc& D = V[v];
-    gbufferPixel<kCount>(D.S, D.cam, D.dnOut, D.amOut, D.mvOut, x, y, TStack{stk + threadIdx.x, kTileThreads}, ctr);
+    gbufferPixel<kCount>(D.S, D.cam, D.dnOut, D.amOut, D.mvOut, x, y, laneStack(stk), ctr);
 }
 
 // All views' GRIS path tracing in one persistent launch: workgroup b starts at
 // view b % numViews and drains the views' pixel queues in turn, so waves move
 // to the views that still have pixels. kGuided: some views sample with the
-// learned proposal (a view whose gf is null samples with its BSDFs).
-template <bool kCount, bool kGuided>
-__global__ __launch_bounds__(kTileThreads, RT_WAVES_PATH_TRACE) void k_gris_path_trace_batch(const ViewDesc* __restrict__ V,
-                                                                                            uint32_t numViews,
-                                                                                            TraceCounters* ctr) {
-    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+// learned proposal (a view whose gf is null samples with its BSDFs). kWaves:
+// the register budget, in waves per SIMD (a tuning choice of the batch).
+template <bool kCount, bool kGuided, int kWaves = RT_WAVES_PATH_TRACE>
+__global__ __launch_bounds__(kTileThreads, kWaves) void k_gris_path_trace_batch(const ViewDesc* __restrict__ V,
+                                                                               uint32_t numViews, TraceCounters* ctr) {
+    __shared__ StackLDS<kTileThreads> stk;
     __shared__ float picks[kStreamWords * kTileThreads];
     for (uint32_t i = 0; i < numViews; i++) {
         const uint32_t v = __builtin_amdgcn_readfirstlane((blockIdx.x + i) % numViews);
         const ViewDesc& D = V[v];
-        ptRun<kCount>(D.S, D.cam, D.g, D.set, D.cur, D.queue, TStack{stk + threadIdx.x, kTileThreads},
+        ptRun<kCount>(D.S, D.cam, D.g, D.set, D.cur, D.queue, D.queueShift, laneStack(stk),
                       picks + threadIdx.x, ctr, kGuided ? D.gf : rtg::GuideFrame{nullptr, nullptr, nullptr});
         queueDone(D.queue);
     }
@@ -1219,12 +1228,12 @@ __global__ __launch_bounds__(kTileThreads, kUnbiased ? RT_WAVES_TEMPORAL_UB : RT
                                                                                         uint32_t numViews,
                                                                                         uint32_t totalTiles, uint32_t swz,
                                                                                         TraceCounters* ctr) {
-    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    __shared__ StackLDS<kTileThreads> stk;
     uint32_t v, x, y;
     if (!batchPixel(V, numViews, totalTiles, swz, v, x, y)) return;
     const ViewDesc& D = V[v];
     temporalPixel<kCount, kLearned, kUnbiased>(D.S, D.cam, D.prevCam, D.g, D.gPrev, D.set, D.cur, D.prev, D.temp, x, y,
-                                               TStack{stk + threadIdx.x, kTileThreads}, ctr,
+                                               laneStack(stk), ctr,
                                                kGuided ? D.gf : rtg::GuideFrame{nullptr, nullptr, nullptr});
 }
 
@@ -1232,13 +1241,250 @@ template <bool kCount, bool kLearned, bool kUnbiased, bool kGuided>
 __global__ __launch_bounds__(kTileThreads, kUnbiased ? RT_WAVES_SPATIAL_UB : RT_WAVES_SPATIAL) void k_gris_spatial_batch(const ViewDesc* __restrict__ V,
                                                                                       uint32_t numViews, uint32_t totalTiles,
                                                                                       uint32_t swz, TraceCounters* ctr) {
-    __shared__ uint32_t stk[RT_LDS_STACK * kTileThreads];
+    __shared__ StackLDS<kTileThreads> stk;
     uint32_t v, x, y;
     if (!batchPixel(V, numViews, totalTiles, swz, v, x, y)) return;
     const ViewDesc& D = V[v];
     spatialPixel<kCount, kLearned, kUnbiased>(D.S, D.cam, D.g, D.set, D.temp, D.cur, D.indirectOut, x, y,
-                                              TStack{stk + threadIdx.x, kTileThreads}, ctr,
+                                              laneStack(stk), ctr,
                                               kGuided ? D.gf : rtg::GuideFrame{nullptr, nullptr, nullptr});
+}
+
+// ---- Wavefront variant of the batched path trace: material-sorted shading ----
+// The persistent kernels keep each path in one lane from its primary vertex to
+// its end. This variant instead keeps every path's state in HBM (kWfWords
+// planes of one word per path; path = pixel of the batch) and runs each bounce
+// as four launches over a device-side list of live paths:
+//   extend  -- closest hit of each live path's next ray; its shading bucket
+//              (miss, light, or the hit material's type) and per-bucket counts,
+//              aggregated per wave with ballots;
+//   scan    -- bucket offsets (one wave), and the next list's length cleared;
+//   scatter -- the live paths reordered by bucket (wave-aggregated cursors);
+//   shade   -- the vertex in bucket order, so the lanes of a wave run the same
+//              BSDF code, with its light-sample shadow ray; a path that goes
+//              on is appended to the next list (ballot + one atomic per wave),
+//              a finished one writes its reservoir.
+// Every path computes exactly what the persistent kernel computes for it (the
+// same ptBegin / ptVertex / ptFinish on its own random numbers), so the
+// reservoirs are identical; only the order of work differs.
+constexpr uint32_t kWfWords = 60;
+constexpr uint32_t kWfBuckets = 8;  // 0 miss, 1 light, 2..5 material types 1..4, 6 other types
+constexpr uint32_t kWfSlot = 42;    // first plane of the stream-RIS pick
+
+struct WfState {
+    float* w;            // kWfWords x n words
+    uint32_t* lists;     // 2 x n: live paths of this bounce and of the next
+    uint32_t* sorted;    // n: this bounce's live paths in bucket order
+    uint32_t* keys;      // n: bucket of lists[cur][j]
+    uint32_t* ctr;       // [0], [1] list lengths; [2, 2 + K) bucket counts; [2 + K, 2 + 2K) cursors
+    uint32_t n;
+};
+
+RT_DEV StreamSlot wfSlot(const WfState& B, uint32_t i) { return StreamSlot{B.w + (size_t)kWfSlot * B.n + i, B.n}; }
+
+RT_DEV void wfStore(const WfState& B, uint32_t i, uint32_t view, uint32_t pix, const PTPath& P) {
+    float* w = B.w + i;
+    const size_t n = B.n;
+    auto put = [&](uint32_t k, float v) { w[k * n] = v; };
+    auto putU = [&](uint32_t k, uint32_t v) { w[k * n] = __uint_as_float(v); };
+    putU(0, view); putU(1, pix);
+    putU(2, P.bounce | (P.sampleState << 8) | (P.lastSampleState << 10) | ((P.isLastVertexConnectible ? 1u : 0u) << 12) |
+                ((P.picked ? 1u : 0u) << 13));
+    putU(3, P.rng);
+    put(4, P.throughput.x); put(5, P.throughput.y); put(6, P.throughput.z);
+    put(7, P.rcThroughput.x); put(8, P.rcThroughput.y); put(9, P.rcThroughput.z);
+    put(10, P.lastPos.x); put(11, P.lastPos.y); put(12, P.lastPos.z);
+    put(13, P.rdir.x); put(14, P.rdir.y); put(15, P.rdir.z);
+    put(16, P.s.pdf); putU(17, P.s.type);
+    storeStream(StreamSlot{w + 22 * n, (uint32_t)n}, P.ps);
+    put(40, P.streamWeight); put(41, P.streamSum);
+}
+
+// The path after its last closest-hit query: the state and that query's hit.
+RT_DEV void wfLoad(const WfState& B, uint32_t i, uint32_t& view, uint32_t& pix, PTPath& P) {
+    const float* w = B.w + i;
+    const size_t n = B.n;
+    auto get = [&](uint32_t k) { return w[k * n]; };
+    auto getU = [&](uint32_t k) { return __float_as_uint(w[k * n]); };
+    view = getU(0); pix = getU(1);
+    const uint32_t f = getU(2);
+    P.bounce = f & 0xffu; P.sampleState = (f >> 8) & 3u; P.lastSampleState = (f >> 10) & 3u;
+    P.isLastVertexConnectible = ((f >> 12) & 1u) != 0u; P.picked = ((f >> 13) & 1u) != 0u;
+    P.rng = getU(3);
+    P.throughput = mk3(get(4), get(5), get(6));
+    P.rcThroughput = mk3(get(7), get(8), get(9));
+    P.lastPos = mk3(get(10), get(11), get(12));
+    P.rdir = mk3(get(13), get(14), get(15));
+    P.rori = P.lastPos + P.rdir * 1e-4f;  // as ptVertex set it
+    P.s.pdf = get(16); P.s.type = getU(17); P.s.wi = s3(0.0f); P.s.bsdf = s3(0.0f);
+    P.isec.b1 = get(18); P.isec.b2 = get(19); P.isec.inst = getU(20); P.isec.prim = getU(21);
+    P.ps = loadStream(StreamSlot{const_cast<float*>(w) + 22 * n, (uint32_t)n});
+    P.streamWeight = get(40); P.streamSum = get(41);
+    P.dir0 = s3(0.0f);
+}
+
+// View of a batch pixel: the last view whose pixBase <= i.
+RT_DEV uint32_t viewOfPixel(const ViewDesc* __restrict__ V, uint32_t numViews, uint32_t i) {
+    uint32_t lo = 0, hi = numViews;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (V[mid].pixBase <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Appends the lanes with `take` to list L (length at *len): one atomic per wave.
+RT_DEV void wfAppend(uint32_t* L, uint32_t* len, bool take, uint32_t value) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const unsigned long long m = __ballot(take);
+    if (m == 0ull) return;
+    const int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(len, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, leader);
+    if (take) L[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = value;
+}
+
+__global__ void k_wf_clear(WfState B) {
+    if (threadIdx.x < 2u + 2u * kWfBuckets) B.ctr[threadIdx.x] = 0u;
+}
+
+// Primary vertex of every pixel of the batch (bounce 0: no closest-hit query).
+__global__ __launch_bounds__(kTileThreads, RT_WAVES_PATH_TRACE) void k_wf_begin(const ViewDesc* __restrict__ V,
+                                                                               uint32_t numViews, uint32_t total,
+                                                                               WfState B) {
+    __shared__ StackLDS<kTileThreads> stk;
+    const TStack st = laneStack(stk);
+    for (uint32_t base = blockIdx.x * kTileThreads; base < total; base += gridDim.x * kTileThreads) {
+        const uint32_t i = base + threadIdx.x;
+        bool live = false;
+        if (i < total) {
+            const uint32_t v = viewOfPixel(V, numViews, i);
+            const ViewDesc& D = V[v];
+            const uint32_t p = i - D.pixBase, x = p % D.g.W, y = p / D.g.W;
+            PTPath P;
+            Surf surf; Mat mat;
+            if (ptBegin(D.S, D.cam, D.g, x, y, P, surf, mat)) {
+                live = ptVertex<false>(D.S, st, wfSlot(B, i), D.set, P, surf, mat, 0u, nullptr) &&
+                       ++P.bounce < D.set.maxDepth;
+                if (live) wfStore(B, i, v, p, P);
+                else storeResv(D.cur, p, ptFinish(P, wfSlot(B, i)));
+            }
+        }
+        wfAppend(B.lists, B.ctr, live, i);
+    }
+}
+
+// Shading bucket of a hit: 0 miss, 1 light, 2 + (material type - 1) for types
+// 1..4, 6 for any other type.
+RT_DEV uint32_t wfBucket(const DevScene& S, const Isec& h) {
+    if (h.inst == RT_INVALID_HIT) return 0u;
+    if (h.inst == 0u) return 1u;
+    const DevInstance& I = S.instances[h.inst - 1];
+    const uint32_t type = S.materials[S.materialIndices[I.indexOffset / 3 + h.prim]].type;
+    return (type >= 1u && type <= 4u) ? type + 1u : 6u;
+}
+
+// Adds each lane's key to counts[key], one atomic per distinct key of the wave;
+// with cursors, also returns the lane's position (cursor base + rank in its key).
+RT_DEV uint32_t wfKeyAtomic(uint32_t* counts, bool valid, uint32_t key) {
+    const uint32_t lane = threadIdx.x & 63u;
+    unsigned long long todo = __ballot(valid);
+    uint32_t pos = 0;
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const uint32_t k = (uint32_t)__shfl((int)key, leader);
+        const unsigned long long same = __ballot(valid && key == k);
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(counts + k, (uint32_t)__popcll(same));
+        base = (uint32_t)__shfl((int)base, leader);
+        if (valid && key == k) pos = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        todo &= ~same;
+    }
+    return pos;
+}
+
+__global__ __launch_bounds__(kTileThreads, RT_WAVES_GBUFFER) void k_wf_extend(const ViewDesc* __restrict__ V,
+                                                                             WfState B, uint32_t cur) {
+    __shared__ StackLDS<kTileThreads> stk;
+    const TStack st = laneStack(stk);
+    const uint32_t n = B.ctr[cur];
+    const uint32_t* L = B.lists + (size_t)cur * B.n;
+    for (uint32_t base = blockIdx.x * kTileThreads; base < n; base += gridDim.x * kTileThreads) {
+        const uint32_t j = base + threadIdx.x;
+        const bool valid = j < n;
+        uint32_t key = 0;
+        if (valid) {
+            const uint32_t i = L[j];
+            float* w = B.w + i;
+            const size_t N = B.n;
+            const ViewDesc& D = V[__float_as_uint(w[0])];
+            const f3 lastPos = mk3(w[10 * N], w[11 * N], w[12 * N]);
+            const f3 rdir = mk3(w[13 * N], w[14 * N], w[15 * N]);
+            const Isec h = traceClosestHit<false>(D.S, st, lastPos + rdir * 1e-4f, RT_MIN_RAY, rdir, RT_MAX_RAY, nullptr);
+            w[18 * N] = h.b1; w[19 * N] = h.b2; w[20 * N] = __uint_as_float(h.inst); w[21 * N] = __uint_as_float(h.prim);
+            key = wfBucket(D.S, h);
+            B.keys[j] = key;
+        }
+        (void)wfKeyAtomic(B.ctr + 2, valid, key);
+    }
+}
+
+// Bucket offsets into the cursors, counts cleared for the next bounce, and the
+// next list emptied.
+__global__ void k_wf_scan(WfState B, uint32_t cur) {
+    if (threadIdx.x != 0u) return;
+    uint32_t off = 0;
+    for (uint32_t k = 0; k < kWfBuckets; k++) {
+        B.ctr[2 + kWfBuckets + k] = off;
+        off += B.ctr[2 + k];
+        B.ctr[2 + k] = 0u;
+    }
+    B.ctr[cur ^ 1u] = 0u;
+}
+
+__global__ __launch_bounds__(kTileThreads) void k_wf_scatter(WfState B, uint32_t cur) {
+    const uint32_t n = B.ctr[cur];
+    const uint32_t* L = B.lists + (size_t)cur * B.n;
+    for (uint32_t base = blockIdx.x * kTileThreads; base < n; base += gridDim.x * kTileThreads) {
+        const uint32_t j = base + threadIdx.x;
+        const bool valid = j < n;
+        const uint32_t key = valid ? B.keys[j] : 0u;
+        const uint32_t pos = wfKeyAtomic(B.ctr + 2 + kWfBuckets, valid, key);
+        if (valid) B.sorted[pos] = L[j];
+    }
+}
+
+__global__ __launch_bounds__(kTileThreads, RT_WAVES_PATH_TRACE) void k_wf_shade(const ViewDesc* __restrict__ V,
+                                                                               WfState B, uint32_t cur) {
+    __shared__ StackLDS<kTileThreads> stk;
+    const TStack st = laneStack(stk);
+    const uint32_t n = B.ctr[cur];
+    uint32_t* next = B.lists + (size_t)(cur ^ 1u) * B.n;
+    for (uint32_t base = blockIdx.x * kTileThreads; base < n; base += gridDim.x * kTileThreads) {
+        const uint32_t j = base + threadIdx.x;
+        bool live = false;
+        uint32_t i = 0;
+        if (j < n) {
+            i = B.sorted[j];
+            uint32_t v, p;
+            PTPath P;
+            wfLoad(B, i, v, p, P);
+            const ViewDesc& D = V[v];
+            live = P.isec.inst != RT_INVALID_HIT;
+            if (live) {
+                Surf surf;
+                loadSurfaceInfo(D.S, P.isec, surf);
+                const Mat mat = loadMat(D.S, surf.matIndex);
+                live = ptVertex<false>(D.S, st, wfSlot(B, i), D.set, P, surf, mat, P.bounce, nullptr) &&
+                       ++P.bounce < D.set.maxDepth;
+            }
+            if (live) wfStore(B, i, v, p, P);
+            else storeResv(D.cur, p, ptFinish(P, wfSlot(B, i)));
+        }
+        wfAppend(next, B.ctr + (cur ^ 1u), live, i);
+    }
 }
 
 __global__ void k_resv_reset(ResvPlanes R) {
@@ -1510,7 +1756,8 @@ uint32_t batchViewTiles(const rt_frame& f) {
 
 // Writes the descriptor of one view into host memory at dst.
 void batchFillDesc(void* dst, const rt_device_scene* scene, const rt_camera* cams, const rt_frame& f,
-                   const rt_gris_settings& set, uint32_t tileBase, WorkQueue* queue) {
+                   const rt_gris_settings& set, uint32_t tileBase, WorkQueue* queue, uint32_t queueShift,
+                   uint32_t pixBase) {
     ViewDesc d;
     std::memset(&d, 0, sizeof(d));
     d.S = scene->dev;
@@ -1530,24 +1777,41 @@ void batchFillDesc(void* dst, const rt_device_scene* scene, const rt_camera* cam
     d.tilesX = (f.width + kTileX - 1) / kTileX;
     d.tiles = batchViewTiles(f);
     d.queue = queue;
+    d.queueShift = queueShift;
+    d.pixBase = pixBase;
     d.gf = guideFrame(f, set);
     std::memcpy(dst, &d, sizeof(d));
 }
 
-// Resident workgroups of the batched persistent path trace per CU (the variant
-// without and with the learned proposal), and the CU count.
-hipError_t batchOccupancy(uint32_t& perCU, uint32_t& perCUGuided, uint32_t& cus) {
-    int dev = 0, c = 0, o = 0, og = 0;
+// The batched persistent path trace's kernel for a counting build, the
+// learned proposal and a register budget of kBatchWaves[w] waves per SIMD.
+static const void* batchPathTraceKernel(bool counting, bool guided, uint32_t w) {
+    if (counting) return guided ? reinterpret_cast<const void*>(k_gris_path_trace_batch<true, true>)
+                                : reinterpret_cast<const void*>(k_gris_path_trace_batch<true, false>);
+    static const void* const k[2][kBatchWaveVariants] = {
+        {reinterpret_cast<const void*>(k_gris_path_trace_batch<false, false, kBatchWaves[0]>),
+         reinterpret_cast<const void*>(k_gris_path_trace_batch<false, false, kBatchWaves[1]>),
+         reinterpret_cast<const void*>(k_gris_path_trace_batch<false, false, kBatchWaves[2]>)},
+        {reinterpret_cast<const void*>(k_gris_path_trace_batch<false, true, kBatchWaves[0]>),
+         reinterpret_cast<const void*>(k_gris_path_trace_batch<false, true, kBatchWaves[1]>),
+         reinterpret_cast<const void*>(k_gris_path_trace_batch<false, true, kBatchWaves[2]>)}};
+    return k[guided ? 1 : 0][w];
+}
+
+// Resident workgroups per CU of each batched persistent path-trace variant
+// (perCU[guided][w]), and the CU count.
+hipError_t batchOccupancy(uint32_t perCU[2][kBatchWaveVariants], uint32_t& cus) {
+    int dev = 0, c = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &o, reinterpret_cast<const void*>(k_gris_path_trace_batch<false, false>), kTileThreads, 0);
-    if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &og, reinterpret_cast<const void*>(k_gris_path_trace_batch<false, true>), kTileThreads, 0);
-    perCU = (uint32_t)std::max(1, o);
-    perCUGuided = (uint32_t)std::max(1, og);
+    for (int gd = 0; gd < 2; gd++)
+        for (uint32_t w = 0; w < kBatchWaveVariants; w++) {
+            int o = 0;
+            if (e == hipSuccess)
+                e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, batchPathTraceKernel(false, gd == 1, w),
+                                                                 kTileThreads, 0);
+            perCU[gd][w] = (uint32_t)std::max(1, o);
+        }
     cus = (uint32_t)std::max(1, c);
     return e;
 }
@@ -1573,16 +1837,53 @@ hipError_t launchBatchGBuffer(const void* descs, uint32_t numViews, uint32_t tot
 }
 
 hipError_t launchBatchPathTrace(const void* descs, uint32_t numViews, uint32_t persistentGroups, bool guided,
-                                hipStream_t stream) {
+                                uint32_t waveVariant, hipStream_t stream) {
     const ViewDesc* V = reinterpret_cast<const ViewDesc*>(descs);
     const dim3 grid(std::max(1u, persistentGroups));
     TraceCounters* ctr = gCountersEnabled ? counterBuffer() : nullptr;
-    if (gCountersEnabled) {
-        if (guided) (void)launch(k_gris_path_trace_batch<true, true>, grid, kTileThreads, stream, V, numViews, ctr);
-        else (void)launch(k_gris_path_trace_batch<true, false>, grid, kTileThreads, stream, V, numViews, ctr);
-    } else {
-        if (guided) (void)launch(k_gris_path_trace_batch<false, true>, grid, kTileThreads, stream, V, numViews, ctr);
-        else (void)launch(k_gris_path_trace_batch<false, false>, grid, kTileThreads, stream, V, numViews, ctr);
+    // every variant has the parameters (const ViewDesc*, uint32_t, TraceCounters*)
+    auto* kernel = reinterpret_cast<void (*)(const ViewDesc*, uint32_t, TraceCounters*)>(
+        const_cast<void*>(batchPathTraceKernel(gCountersEnabled, guided, waveVariant)));
+    (void)launch(kernel, grid, kTileThreads, stream, V, numViews, ctr);
+    return hipGetLastError();
+}
+
+// Path-state bytes of the wavefront variant for n paths, and their layout.
+size_t wavefrontBytes(uint32_t n) {
+    return ((size_t)kWfWords + 4u) * 4u * n + 4u * (2u + 2u * kWfBuckets);
+}
+void wavefrontCarve(void* base, uint32_t n, WavefrontBuffers& wb) {
+    char* p = static_cast<char*>(base);
+    wb.words = reinterpret_cast<float*>(p);
+    p += (size_t)kWfWords * 4u * n;
+    wb.lists = reinterpret_cast<uint32_t*>(p);
+    p += 8u * (size_t)n;
+    wb.sorted = reinterpret_cast<uint32_t*>(p);
+    p += 4u * (size_t)n;
+    wb.keys = reinterpret_cast<uint32_t*>(p);
+    p += 4u * (size_t)n;
+    wb.counters = reinterpret_cast<uint32_t*>(p);
+    wb.capacity = n;
+}
+
+// The wavefront variant of launchBatchPathTrace (non-guided views; the plain
+// kernels, not the counting builds): clear, primary vertices, then per bounce
+// extend, scan, scatter and shade, each a grid-stride launch of `groups`
+// one-wave workgroups over the device-side lists.
+hipError_t launchBatchWavefront(const void* descs, uint32_t numViews, uint32_t totalPixels, uint32_t maxDepth,
+                                const WavefrontBuffers& wb, uint32_t groups, hipStream_t stream) {
+    const ViewDesc* V = reinterpret_cast<const ViewDesc*>(descs);
+    const WfState B{wb.words, wb.lists, wb.sorted, wb.keys, wb.counters, wb.capacity};
+    const dim3 grid(std::max(1u, groups));
+    (void)launch(k_wf_clear, dim3(1), dim3(64), stream, B);
+    (void)launch(k_wf_begin, grid, dim3(kTileThreads), stream, V, numViews, totalPixels, B);
+    uint32_t cur = 0;
+    for (uint32_t b = 1; b < maxDepth; b++) {
+        (void)launch(k_wf_extend, grid, dim3(kTileThreads), stream, V, B, cur);
+        (void)launch(k_wf_scan, dim3(1), dim3(64), stream, B, cur);
+        (void)launch(k_wf_scatter, grid, dim3(kTileThreads), stream, B, cur);
+        (void)launch(k_wf_shade, grid, dim3(kTileThreads), stream, V, B, cur);
+        cur ^= 1u;
     }
     return hipGetLastError();
 }

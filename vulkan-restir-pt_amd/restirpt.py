@@ -146,7 +146,8 @@ class View(C.Structure):
 
 
 class BatchTuning(C.Structure):
-    _fields_ = [("persistentGroups", C.c_uint32), ("xcdSwizzle", C.c_uint32)]
+    _fields_ = [("persistentGroups", C.c_uint32), ("xcdSwizzle", C.c_uint32), ("waves", C.c_uint32),
+                ("queueTile", C.c_uint32), ("graph", C.c_uint32), ("shading", C.c_uint32)]
 
 
 class SceneInfo(C.Structure):
