@@ -371,29 +371,46 @@ __global__ __launch_bounds__(kMeanThreads) void k_guide_wmean(const float* __res
     if (threadIdx.x == 0) *out = sc[0] > 0.0f ? sw[0] / sc[0] : 0.0f;
 }
 
-// Weight gradients and stats: fixed-order sum over the waves' partials.
-// Sum of column k over the waves' partials in a fixed order (eight
+// Weight gradients and stats: a fixed-order sum over the waves' partials, in
+// two launches so that the reduction spreads over the whole GPU: each group of
+// kReduceGroup consecutive waves is summed into the first row of the group,
+// then the groups' rows are summed.
+constexpr uint32_t kReduceGroup = 16;
+
+// Sum of column k over `rows` rows rowStride apart, in a fixed order (eight
 // interleaved accumulators for memory-level parallelism, combined pairwise).
-__device__ __forceinline__ float columnSum(const float* __restrict__ partial, uint32_t waves, int stride, int k) {
+__device__ __forceinline__ float columnSum(const float* __restrict__ partial, uint32_t rows, size_t rowStride, int k) {
     float a[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t w = 0;
-    for (; w + 8u <= waves; w += 8u)
+    for (; w + 8u <= rows; w += 8u)
 #pragma unroll
-        for (int i = 0; i < 8; i++) a[i] += partial[(size_t)(w + i) * stride + k];
-    for (; w < waves; w++) a[0] += partial[(size_t)w * stride + k];
+        for (int i = 0; i < 8; i++) a[i] += partial[(size_t)(w + i) * rowStride + k];
+    for (; w < rows; w++) a[0] += partial[(size_t)w * rowStride + k];
     return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
 }
 
-// stride: partial stride of the network; norm: outputs per sample the mean loss averages over
-// (3 for the radiance network, 1 for the proposal network).
-__global__ void k_mlp_reduce(const float* __restrict__ partial, uint32_t waves, int stride, float norm,
+// Group g (blockIdx.y) of the waves: rows [16 g, 16 g + 16) summed into row 16 g.
+__global__ void k_mlp_reduce_groups(float* __restrict__ partial, uint32_t waves, int stride) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= stride) return;
+    const uint32_t w0 = blockIdx.y * kReduceGroup;
+    const uint32_t rows = min(kReduceGroup, waves - w0);
+    float* p = partial + (size_t)w0 * stride;
+    p[k] = columnSum(p, rows, (size_t)stride, k);
+}
+
+// The groups' sums (rows 16 g). stride: partial stride of the network; norm:
+// outputs per sample the mean loss averages over (3 for the radiance network,
+// 1 for the proposal network).
+__global__ void k_mlp_reduce(const float* __restrict__ partial, uint32_t groups, int stride, float norm,
                              float* __restrict__ out, float* __restrict__ lossOut) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= stride) return;
-    const float acc = columnSum(partial, waves, stride, k);
+    const size_t rowStride = (size_t)kReduceGroup * stride;
+    const float acc = columnSum(partial, groups, rowStride, k);
     out[k] = acc;
     if (k == stride - kStats && lossOut) {
-        const float cnt = columnSum(partial, waves, stride, stride - kStats + 1);
+        const float cnt = columnSum(partial, groups, rowStride, stride - kStats + 1);
         *lossOut = cnt > 0.0f ? acc / (norm * cnt) : 0.0f;
     }
 }
@@ -463,8 +480,11 @@ hipError_t mlpGradients(rt_mlp* m, const float* pos, const float* target, uint32
     } else {
         (void)launch(k_fill_f32, dim3(blocks(stride, 256)), dim3(256), st, m->partial, (uint64_t)stride, 0.0f);
     }
-    (void)launch(k_mlp_reduce, dim3(blocks(stride, 256)), dim3(256), st, (const float*)m->partial, n ? waves : 1u,
-                 stride, guide ? 1.0f : 3.0f, m->wGrad, loss);
+    const uint32_t rows = n ? waves : 1u;
+    const uint32_t groups = (rows + kReduceGroup - 1u) / kReduceGroup;
+    (void)launch(k_mlp_reduce_groups, dim3(blocks(stride, 256), groups), dim3(256), st, m->partial, rows, stride);
+    (void)launch(k_mlp_reduce, dim3(blocks(stride, 256)), dim3(256), st, (const float*)m->partial, groups, stride,
+                 guide ? 1.0f : 3.0f, m->wGrad, loss);
     return hipGetLastError();
 }
 hipError_t mlpAdam(rt_mlp* m, hipStream_t st) {

@@ -1495,8 +1495,9 @@ __global__ void k_resv_reset(ResvPlanes R) {
     storeResv<true>(R, i, r);
 }
 
-// Learned-target refresh after the path-trace pass. A one-wave workgroup walks
-// 64-pixel chunks; each lane reads its pixel's reservoir and writes the
+// Learned-target refresh after the path-trace pass. Each wave (four per
+// workgroup, sharing one LDS copy of the network's weights) walks 64-pixel
+// chunks; each lane reads its pixel's reservoir and writes the
 // training slot at once, while the pixels whose sample has a surface rc vertex
 // (the only ones the network serves, often ~10 %) are compacted into an LDS
 // queue; whenever 32 are queued the wave encodes them (lanes 0-31 and 32-63
@@ -1509,15 +1510,25 @@ __global__ void k_resv_reset(ResvPlanes R) {
 // target -1 if unusable).
 struct RefreshItem { uint32_t p; float y[3], F[3], rcLi[3], rw; };
 constexpr int kRefreshQueue = 96;  // 32 in flight + up to 63 more from one chunk
-__global__ __launch_bounds__(64) void k_gris_learned_refresh(DevScene S, rtm::MlpView M, GB g, ResvPlanes R,
-                                                             float* __restrict__ batchPos,
-                                                             float* __restrict__ batchTarget, uint32_t stride,
-                                                             uint32_t offset) {
+constexpr uint32_t kRefreshWaves = 4;
+
+// Orders this wave's LDS accesses before and after it (a wave's LDS operations
+// complete in issue order; this keeps the compiler from moving them across).
+RT_DEV void waveLdsSync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__global__ __launch_bounds__(64 * kRefreshWaves, 3) void k_gris_learned_refresh(DevScene S, rtm::MlpView M, GB g,
+                                                                             ResvPlanes R, float* __restrict__ batchPos,
+                                                                             float* __restrict__ batchTarget,
+                                                                             uint32_t stride, uint32_t offset) {
     __shared__ _Float16 sW[rtm::kWeights];
-    __shared__ RefreshItem q[kRefreshQueue];
+    __shared__ RefreshItem qs[kRefreshWaves][kRefreshQueue];
     for (int i = threadIdx.x; i < rtm::kWeights; i += blockDim.x) sW[i] = M.w[i];
     __syncthreads();
-    const uint32_t lane = threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    RefreshItem* q = qs[wave];
     const uint32_t n = g.W * g.H;
     const uint32_t chunks = (n + 63u) / 64u;
     uint32_t queued = 0;  // wave-uniform
@@ -1545,17 +1556,17 @@ __global__ __launch_bounds__(64) void k_gris_learned_refresh(DevScene S, rtm::Ml
             R.p[4 * R.n + it.p].w = rw;
             R.p[5 * R.n + it.p] = make_float4(L.x, L.y, L.z, pNew);
         }
-        __syncthreads();
+        waveLdsSync();
         // move the entries behind the drained ones to the front
         const uint32_t rest = queued - count;
         RefreshItem keep;
         if (lane < rest) keep = q[count + lane];
-        __syncthreads();
+        waveLdsSync();
         if (lane < rest) q[lane] = keep;
-        __syncthreads();
+        waveLdsSync();
         queued = rest;
     };
-    for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+    for (uint32_t c = blockIdx.x * kRefreshWaves + wave; c < chunks; c += gridDim.x * kRefreshWaves) {
         const uint32_t p = c * 64u + lane;
         bool need = false;
         Resv r = zeroResv();
@@ -1590,7 +1601,7 @@ __global__ __launch_bounds__(64) void k_gris_learned_refresh(DevScene S, rtm::Ml
             it.rw = r.resampleWeight;
         }
         queued += (uint32_t)__popcll(mask);
-        __syncthreads();
+        waveLdsSync();
         while (queued >= 32u) drain(32u);
     }
     if (queued > 0u) drain(queued);
@@ -1741,8 +1752,8 @@ hipError_t launchLearnedRefresh(const rt_device_scene* s, rt_mlp* m, const rt_ca
     m->batchSize = offset < n ? (n - offset + stride - 1) / stride : 0u;
     GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
     const uint32_t chunks = (n + 63u) / 64u;
-    const uint32_t blocks = std::min<uint32_t>(chunks, 2048u);
-    (void)launch(k_gris_learned_refresh, dim3(blocks), dim3(64), stream, s->dev, m->view, g,
+    const uint32_t blocks = std::min<uint32_t>((chunks + kRefreshWaves - 1u) / kRefreshWaves, 1280u);
+    (void)launch(k_gris_learned_refresh, dim3(blocks), dim3(64 * kRefreshWaves), stream, s->dev, m->view, g,
                  planes(f.grisReservoir, f.width, f.height), m->batchPos, m->batchTarget, stride, offset);
     return hipGetLastError();
 }
