@@ -60,6 +60,7 @@ constexpr float kGradClip = 64.0f;   // per-sample output gradient bound (FP16 r
 constexpr float kLogClamp = 20.0f;   // exp output activation input bound
 
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 
 struct MlpView {
@@ -71,7 +72,18 @@ struct MlpView {
     float lo[3], invExt[3];  // world -> unit cube
 };
 
+// C += A B on the matrix cores, 32 x 32 outputs. The operands of the kernels
+// are built as k-steps of 8 (a lane holds 4 halves of its row / column: lanes
+// 0-31 the step's k 0-3, lanes 32-63 k 4-7). mfma2 takes two such steps at
+// once on gfx950's K = 16 instruction: each lane's two 4-half pieces are
+// concatenated for A and for B alike, so every product of the two steps is
+// summed exactly once whatever k the hardware assigns to a slot. mfma is the
+// single-step form for an odd step count.
 RT_DEV f16v mfma(h4 a, h4 b, f16v c) { return __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, c, 0, 0, 0); }
+RT_DEV h8 cat8(h4 a, h4 b) { return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7); }
+RT_DEV f16v mfma2(h4 a0, h4 b0, h4 a1, h4 b1, f16v c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(cat8(a0, a1), cat8(b0, b1), c, 0, 0, 0);
+}
 RT_DEV f16v zero16() {
     f16v z;
 #pragma unroll
@@ -183,21 +195,26 @@ RT_DEV void forwardTileT(const _Float16* W, uint32_t lane, const h4 enc[4], Fwd&
     for (int t = 0; t < 2; t++) {
         f16v acc = zero16();
 #pragma unroll
-        for (int s = 0; s < 4; s++) acc = mfma(fwdA(W + kW1, kHidden, 32 * t + j, 8 * s + 4 * h, kHidden), enc[s], acc);
+        for (int s = 0; s < 4; s += 2)
+            acc = mfma2(fwdA(W + kW1, kHidden, 32 * t + j, 8 * s + 4 * h, kHidden), enc[s],
+                        fwdA(W + kW1, kHidden, 32 * t + j, 8 * (s + 1) + 4 * h, kHidden), enc[s + 1], acc);
         F.z1[t] = acc;
     }
 #pragma unroll
     for (int t = 0; t < 2; t++) {
         f16v acc = zero16();
 #pragma unroll
-        for (int s = 0; s < 8; s++)
-            acc = mfma(fwdA(W + kW2, kHidden, 32 * t + j, 8 * s + 4 * h, kHidden), reluHalf(F.z1[s >> 2], s & 3), acc);
+        for (int s = 0; s < 8; s += 2)
+            acc = mfma2(fwdA(W + kW2, kHidden, 32 * t + j, 8 * s + 4 * h, kHidden), reluHalf(F.z1[s >> 2], s & 3),
+                        fwdA(W + kW2, kHidden, 32 * t + j, 8 * (s + 1) + 4 * h, kHidden),
+                        reluHalf(F.z1[(s + 1) >> 2], (s + 1) & 3), acc);
         F.z2[t] = acc;
     }
     f16v acc = zero16();
 #pragma unroll
-    for (int s = 0; s < 8; s++)
-        acc = mfma(fwdA(W + kW3, kO, j, 8 * s + 4 * h, kO), reluHalf(F.z2[s >> 2], s & 3), acc);
+    for (int s = 0; s < 8; s += 2)
+        acc = mfma2(fwdA(W + kW3, kO, j, 8 * s + 4 * h, kO), reluHalf(F.z2[s >> 2], s & 3),
+                    fwdA(W + kW3, kO, j, 8 * (s + 1) + 4 * h, kO), reluHalf(F.z2[(s + 1) >> 2], (s + 1) & 3), acc);
     F.z3 = acc;
 }
 RT_DEV void forwardTile(const _Float16* W, uint32_t lane, const h4 enc[4], Fwd& F) { forwardTileT<kOut>(W, lane, enc, F); }

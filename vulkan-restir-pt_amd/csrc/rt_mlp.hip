@@ -246,7 +246,11 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpView M, const float* __rest
         f16v dz2[2];
         for (int t = 0; t < 2; t++) {
             f16v acc = zero16();
-            for (int g = 0; g < kSteps; g++) acc = mfma(bwdA(L.w + kW3, kO, 32 * t + j, 8 * g + 4 * h, kO), d3b[g], acc);
+            int g = 0;
+            for (; g + 1 < kSteps; g += 2)
+                acc = mfma2(bwdA(L.w + kW3, kO, 32 * t + j, 8 * g + 4 * h, kO), d3b[g],
+                            bwdA(L.w + kW3, kO, 32 * t + j, 8 * (g + 1) + 4 * h, kO), d3b[g + 1], acc);
+            if (g < kSteps) acc = mfma(bwdA(L.w + kW3, kO, 32 * t + j, 8 * g + 4 * h, kO), d3b[g], acc);
             for (int r = 0; r < 16; r++) acc[r] = F.z2[t][r] > 0.0f ? acc[r] : 0.0f;
             dz2[t] = acc;
         }
@@ -254,20 +258,28 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpView M, const float* __rest
         f16v dz1[2];
         for (int t = 0; t < 2; t++) {
             f16v acc = zero16();
-            for (int s8 = 0; s8 < 8; s8++) {
-                h4 b;
-                for (int e = 0; e < 4; e++) b[e] = (_Float16)dz2[s8 >> 2][4 * (s8 & 3) + e];
-                acc = mfma(bwdA(L.w + kW2, kHidden, 32 * t + j, 8 * s8 + 4 * h, kHidden), b, acc);
+            for (int s8 = 0; s8 < 8; s8 += 2) {
+                h4 b0, b1;
+                for (int e = 0; e < 4; e++) {
+                    b0[e] = (_Float16)dz2[s8 >> 2][4 * (s8 & 3) + e];
+                    b1[e] = (_Float16)dz2[(s8 + 1) >> 2][4 * ((s8 + 1) & 3) + e];
+                }
+                acc = mfma2(bwdA(L.w + kW2, kHidden, 32 * t + j, 8 * s8 + 4 * h, kHidden), b0,
+                            bwdA(L.w + kW2, kHidden, 32 * t + j, 8 * (s8 + 1) + 4 * h, kHidden), b1, acc);
             }
             for (int r = 0; r < 16; r++) acc[r] = F.z1[t][r] > 0.0f ? acc[r] : 0.0f;
             dz1[t] = acc;
         }
         // denc = W1 dz1: rows = the 32 encoding features, laid out like enc[]
         f16v denc = zero16();
-        for (int s8 = 0; s8 < 8; s8++) {
-            h4 b;
-            for (int e = 0; e < 4; e++) b[e] = (_Float16)dz1[s8 >> 2][4 * (s8 & 3) + e];
-            denc = mfma(bwdA(L.w + kW1, kHidden, j, 8 * s8 + 4 * h, kHidden), b, denc);
+        for (int s8 = 0; s8 < 8; s8 += 2) {
+            h4 b0, b1;
+            for (int e = 0; e < 4; e++) {
+                b0[e] = (_Float16)dz1[s8 >> 2][4 * (s8 & 3) + e];
+                b1[e] = (_Float16)dz1[(s8 + 1) >> 2][4 * ((s8 + 1) & 3) + e];
+            }
+            denc = mfma2(bwdA(L.w + kW1, kHidden, j, 8 * s8 + 4 * h, kHidden), b0,
+                         bwdA(L.w + kW1, kHidden, j, 8 * (s8 + 1) + 4 * h, kHidden), b1, denc);
         }
         // scatter the encoding gradient: register group g holds levels 4g + 2h (+1)
         if (s < n && valid) {
@@ -306,24 +318,28 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpView M, const float* __rest
                 for (int e = 0; e < 4; e++) L.d3[j][8 * g + 4 * h + e] = d3b[g][e];
         }
         __syncthreads();
-        // dW^T[o][i] += sum over samples dz[o][j] * act[i][j]; k-step s4 covers samples 8 s4 + 4h + e
-        for (int s4 = 0; s4 < 4; s4++) {
-            const int jb = 8 * s4 + 4 * h;
-            h4 aD1[2], aD2[2], aD3, bEnc, bH1[2], bH2[2];
-            for (int e = 0; e < 4; e++) {
-                for (int t = 0; t < 2; t++) {
-                    aD1[t][e] = L.d1[jb + e][32 * t + j];
-                    aD2[t][e] = L.d2[jb + e][32 * t + j];
-                    bH1[t][e] = L.h1[jb + e][32 * t + j];
-                    bH2[t][e] = L.h2[jb + e][32 * t + j];
+        // dW^T[o][i] += sum over samples dz[o][j] * act[i][j]; k-step s4 covers samples 8 s4 + 4h + e,
+        // two k-steps per instruction
+        for (int s4 = 0; s4 < 4; s4 += 2) {
+            h4 aD1[2][2], aD2[2][2], aD3[2], bEnc[2], bH1[2][2], bH2[2][2];
+            for (int q = 0; q < 2; q++) {
+                const int jb = 8 * (s4 + q) + 4 * h;
+                for (int e = 0; e < 4; e++) {
+                    for (int t = 0; t < 2; t++) {
+                        aD1[q][t][e] = L.d1[jb + e][32 * t + j];
+                        aD2[q][t][e] = L.d2[jb + e][32 * t + j];
+                        bH1[q][t][e] = L.h1[jb + e][32 * t + j];
+                        bH2[q][t][e] = L.h2[jb + e][32 * t + j];
+                    }
+                    aD3[q][e] = (j < kO) ? L.d3[jb + e][j] : (_Float16)0.0f;
+                    bEnc[q][e] = L.enc[jb + e][j];
                 }
-                aD3[e] = (j < kO) ? L.d3[jb + e][j] : (_Float16)0.0f;
-                bEnc[e] = L.enc[jb + e][j];
             }
             for (int t = 0; t < 2; t++) {
-                gW1[t] = mfma(aD1[t], bEnc, gW1[t]);
-                for (int u2 = 0; u2 < 2; u2++) gW2[t][u2] = mfma(aD2[t], bH1[u2], gW2[t][u2]);
-                gW3[t] = mfma(aD3, bH2[t], gW3[t]);
+                gW1[t] = mfma2(aD1[0][t], bEnc[0], aD1[1][t], bEnc[1], gW1[t]);
+                for (int u2 = 0; u2 < 2; u2++)
+                    gW2[t][u2] = mfma2(aD2[0][t], bH1[0][u2], aD2[1][t], bH1[1][u2], gW2[t][u2]);
+                gW3[t] = mfma2(aD3[0], bH2[0][t], aD3[1], bH2[1][t], gW3[t]);
             }
         }
         __syncthreads();

@@ -270,8 +270,11 @@ __global__ __launch_bounds__(kTileThreads, RT_WAVES_PT_PLAIN) void k_pt_plain(De
     if (x >= g.W || y >= g.H) return;
     TStack st = laneStack(stk);
     size_t p = (size_t)y * g.W + x;
-    if (which & 1u) accumulate(directOut, p, directIllumination2<kCount>(S, st, cam, g, x, y, ctr), cam.frameIndex);
-    if (which & 2u) accumulate(indirectOut, p, indirectIllumination<kCount>(S, st, cam, g, x, y, maxDepth, ctr), cam.frameIndex);
+    // both estimators: the grid's two z layers take one each (indirect first), so a pixel's two
+    // independent estimates run on two lanes instead of one after the other
+    const uint32_t part = which == 3u ? (blockIdx.z == 0u ? 2u : 1u) : which;
+    if (part & 1u) accumulate(directOut, p, directIllumination2<kCount>(S, st, cam, g, x, y, ctr), cam.frameIndex);
+    if (part & 2u) accumulate(indirectOut, p, indirectIllumination<kCount>(S, st, cam, g, x, y, maxDepth, ctr), cam.frameIndex);
 }
 
 // ================================================================================================
@@ -1667,8 +1670,10 @@ hipError_t launchGBuffer(const rt_device_scene* s, const rt_camera& cam, float* 
 hipError_t launchPtPlain(const rt_device_scene* s, const rt_camera* cams, const rt_frame& f, uint32_t maxDepth,
                          uint32_t which, hipStream_t stream) {
     GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
-    RT_DISPATCH(k_pt_plain, tileGrid(f.width, f.height), kTileThreads, stream, s->dev, cams[0], g, maxDepth, which,
-                f.directOutput, f.indirectOutput);
+    dim3 grid = tileGrid(f.width, f.height);
+    if (which == 3u) grid.z = 2;
+    RT_DISPATCH(k_pt_plain, grid, kTileThreads, stream, s->dev, cams[0], g, maxDepth, which, f.directOutput,
+                f.indirectOutput);
     return hipGetLastError();
 }
 
