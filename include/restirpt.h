@@ -260,9 +260,11 @@ rt_status rt_pt_plain(const rt_device_scene* scene, const rt_camera cams[2], con
 
 /* The three passes of GRISReSTIR::render (src/GRISReSTIR.cpp:9-53).
  * rt_gris_path_trace (also inside rt_gris_render) is a persistent kernel that
- * takes its pixel queue from 64 slots per scene, handed out round robin: at
- * most 64 such launches on one scene may be in flight at once across streams
- * (launches on one stream are ordered and never collide). */
+ * takes its pixel queue from 64 slots per scene, handed out round robin; a
+ * launch whose slot's previous launch may still run (on another stream) first
+ * waits for it on its own stream, so concurrent launches never share a queue.
+ * Under stream capture no such wait is recorded: at most 64 captured launches
+ * on one scene may then be in flight at once. */
 rt_status rt_gris_path_trace(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
                              const rt_gris_settings* settings, hipStream_t stream);
 rt_status rt_gris_temporal(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,

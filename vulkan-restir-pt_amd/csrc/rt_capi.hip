@@ -72,6 +72,8 @@ void freeScene(rt_device_scene* s) {
     (void)hipFree(s->materials); (void)hipFree(s->materialIndices); (void)hipFree(s->vertices); (void)hipFree(s->indices);
     (void)hipFree(s->instances); (void)hipFree(s->lights); (void)hipFree(s->lightTable); (void)hipFree(s->queues);
     (void)hipFree(s->texDesc); (void)hipFree(s->texels); (void)hipFree(s->srgbLut);
+    for (hipEvent_t& e : s->queueEvents)
+        if (e) (void)hipEventDestroy(e);
     delete s;
 }
 
@@ -223,6 +225,8 @@ rt_status rt_device_scene_create(const rt_scene_arrays* a, hipStream_t stream, r
         ok = rti::hipCheck(hipMalloc(&s->queues, qb), "hipMalloc") &&
              rti::hipCheck(hipMemsetAsync(s->queues, 0, qb, stream), "hipMemsetAsync");
         s->deviceBytes += qb;
+        for (uint32_t k = 0; ok && k < rt_device_scene::kQueueSlots; k++)
+            ok = rti::hipCheck(hipEventCreateWithFlags(&s->queueEvents[k], hipEventDisableTiming), "hipEventCreate");
     }
     if (ok) {
         rtd::DevScene& d = s->dev;

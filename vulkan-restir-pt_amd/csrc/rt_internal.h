@@ -34,12 +34,15 @@ struct rt_device_scene {
     float bvhBinaryMs = 0.0f, bvhBuildMs = 0.0f;  // GPU time of the binary build / of the whole build
     float sceneLo[3] = {0, 0, 0}, sceneHi[3] = {0, 0, 0};
     size_t deviceBytes = 0;
-    // Pixel queues of the persistent passes, handed out round robin: up to
-    // kQueueSlots such launches on this scene may be in flight at once.
+    // Pixel queues of the eager persistent path traces, handed out round robin.
+    // Each slot's event marks the end of its last launch; a launch that takes a
+    // slot whose previous launch may still run waits for that event on its own
+    // stream, so launches in flight on other streams never share a queue.
     static constexpr uint32_t kQueueSlots = 64;
     rtd::WorkQueue* queues = nullptr;
+    hipEvent_t queueEvents[kQueueSlots] = {};
     mutable std::atomic<uint32_t> queueTick{0};
-    rtd::WorkQueue* nextQueue() const { return queues + (queueTick.fetch_add(1u) % kQueueSlots); }
+    uint32_t nextQueueSlot() const { return queueTick.fetch_add(1u) % kQueueSlots; }
 };
 
 // Learned target function: parameters, optimiser state and the frame's

@@ -1713,7 +1713,24 @@ hipError_t launchGrisPathTrace(const rt_device_scene* s, const rt_camera* cams, 
     if (e != hipSuccess) return e;
     TraceCounters* ctr = gCountersEnabled ? counterBuffer() : nullptr;
     const ResvPlanes R = planes(f.grisReservoir, f.width, f.height);
-    WorkQueue* q = queue ? queue : s->nextQueue();
+    // the scene's next queue slot: wait (on this stream) for the slot's last launch if it may
+    // still run; a stream under capture is left alone (its recorded graph orders its own work)
+    hipEvent_t slotEvent = nullptr;
+    WorkQueue* q = queue;
+    if (!q) {
+        const uint32_t slot = s->nextQueueSlot();
+        q = s->queues + slot;
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        if (!tlsRecorder) e = hipStreamIsCapturing(stream, &cap);
+        if (e != hipSuccess) return e;
+        if (!tlsRecorder && cap == hipStreamCaptureStatusNone) {
+            slotEvent = s->queueEvents[slot];
+            const hipError_t qe = hipEventQuery(slotEvent);
+            if (qe == hipErrorNotReady) e = hipStreamWaitEvent(stream, slotEvent, 0);
+            else if (qe != hipSuccess) e = qe;
+            if (e != hipSuccess) return e;
+        }
+    }
     const rtg::GuideFrame gf = guideFrame(f, set);
     if (gCountersEnabled) {
         if (guided) (void)launch(k_gris_path_trace<true, true>, dim3(grid), kTileThreads, stream, s->dev, cams[0], g, set, R, q, gf, ctr);
@@ -1722,7 +1739,9 @@ hipError_t launchGrisPathTrace(const rt_device_scene* s, const rt_camera* cams, 
         if (guided) (void)launch(k_gris_path_trace<false, true>, dim3(grid), kTileThreads, stream, s->dev, cams[0], g, set, R, q, gf, ctr);
         else (void)launch(k_gris_path_trace<false, false>, dim3(grid), kTileThreads, stream, s->dev, cams[0], g, set, R, q, gf, ctr);
     }
-    return hipGetLastError();
+    e = hipGetLastError();
+    if (e == hipSuccess && slotEvent) e = hipEventRecord(slotEvent, stream);
+    return e;
 }
 
 hipError_t launchGrisTemporal(const rt_device_scene* s, const rt_camera* cams, const rt_frame& f,
