@@ -212,25 +212,42 @@ RT_DEV bool traverse(const DevScene& S, const RayT& r, bool skipLights, TStack s
             }
         }
         // nearest internal child next, the others pushed farthest first
-        uint32_t next = RT_EMPTY;
-        int first = -1;
+        uint32_t im = 0;  // internal children, by sorted position
 #pragma unroll
         for (int q = 0; q < 4; q++)
-            if (first < 0 && rs[q] != RT_EMPTY && !(rs[q] & RT_LEAF)) first = q;
+            if (rs[q] != RT_EMPTY && !(rs[q] & RT_LEAF)) im |= 1u << q;
+        uint32_t next = RT_EMPTY;
+        if (im) {
+            const uint32_t f = (uint32_t)__builtin_ctz(im);
+            next = f == 0u ? r0 : (f == 1u ? r1 : (f == 2u ? r2 : r3));
+        }
+        const uint32_t pm = im & (im - 1u);  // pushed: every internal child but the nearest
+        const int npush = __builtin_popcount(pm);
+        if (sp + npush <= RT_LDS_STACK) {
+            // all pushes fit in LDS: slot q lands at sp + the number of pushed slots farther than q
 #pragma unroll
-        for (int q = 3; q >= 0; q--) {
-            uint32_t ref = rs[q];
-            if (ref == RT_EMPTY || (ref & RT_LEAF)) continue;
-            if (q == first) { next = ref; continue; }
-            if (sp < RT_LDS_STACK + RT_SPILL_STACK) {  // the build bounds the stack depth
-                if (sp < RT_LDS_STACK) {
-                    st.lds[sp * st.stride] = ref;
-                    if (kKeep) st.ldsT[sp * st.stride] = stackKey(ks[q]);
-                } else {
-                    spill[sp - RT_LDS_STACK] = ref;
-                    if (kKeep) spillT[sp - RT_LDS_STACK] = stackKey(ks[q]);
+            for (int q = 3; q >= 0; q--) {
+                if ((pm >> q) & 1u) {
+                    const int pos = sp + __builtin_popcount(pm >> (q + 1));
+                    st.lds[pos * st.stride] = rs[q];
+                    if (kKeep) st.ldsT[pos * st.stride] = stackKey(ks[q]);
                 }
-                sp++;
+            }
+            sp += npush;
+        } else {
+#pragma unroll
+            for (int q = 3; q >= 0; q--) {
+                if (!((pm >> q) & 1u)) continue;
+                if (sp < RT_LDS_STACK + RT_SPILL_STACK) {  // the build bounds the stack depth
+                    if (sp < RT_LDS_STACK) {
+                        st.lds[sp * st.stride] = rs[q];
+                        if (kKeep) st.ldsT[sp * st.stride] = stackKey(ks[q]);
+                    } else {
+                        spill[sp - RT_LDS_STACK] = rs[q];
+                        if (kKeep) spillT[sp - RT_LDS_STACK] = stackKey(ks[q]);
+                    }
+                    sp++;
+                }
             }
         }
         // pop; a shadow ray's tBest is fixed, so only closest-hit rays drop nodes
