@@ -823,7 +823,7 @@ RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& 
     RcData rc;
     Surf rcPrevSurf, rcSurf;
     f3 wi = s3(0.0f), Li = s3(0.0f);
-    bool srcSampleValid = false;
+    bool shiftable = false;  // every condition of the shift but the visibility of the reconnection
     float dstJacobian = 0.0f, jacobian = 0.0f, dstPHat = 0.0f, dstSamplePdf = 0.0f;
     if (src.rcIsec.inst != RT_INVALID_HIT) {
         replayHybrid<kCount, kUnbiased>(S, st, set, dstPrimaryIsec, dstPrimarySurf, primDir, primOri, src.flags,
@@ -844,13 +844,15 @@ RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& 
             // shading normal), as the path tracer generates it; light vertices must face it
             const bool facing = (kUnbiased && flagsRcVertexType(src.flags) == RC_SURFACE) ? cosTheta != 0.0f
                                                                                          : cosTheta > 0.0f;
-            if (dist > GRIS_DIST_THRESHOLD && facing && !isnan(jacobian) && src.rcJacobian > 0.0f &&
-                isBSDFConnectible(rcPrevMat)) {
-                if (traceVisibility<kCount>(S, st, rcPrevSurf.pos, rcSurf.pos, ctr)) srcSampleValid = true;
-            }
+            shiftable = dist > GRIS_DIST_THRESHOLD && facing && !isnan(jacobian) && src.rcJacobian > 0.0f &&
+                        isBSDFConnectible(rcPrevMat);
         }
     }
-    if (srcSampleValid) {
+    // The shifted sample is formed before its reconnection's visibility is traced, so that only
+    // it and the ray stay live across the traversal; an occluded or failed shift merges with
+    // weight 0 (only its M counts, as a weight-0 candidate is never picked).
+    if (shiftable) {
+        const f3 visFrom = rcPrevSurf.pos, visTo = rcSurf.pos;
         uint32_t rcType = flagsRcVertexType(src.flags);
         f3 LiL = s3(0.0f);
         if (!isnan(src.rcPrevSamplePdf) && src.rcPrevSamplePdf > 1e-6f) {
@@ -883,6 +885,7 @@ RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& 
         src.F = Li;
         if (src.rcPrevSamplePdf < 1e-6f || isnan(src.rcPrevSamplePdf)) src.rcPrevSamplePdf = 0.0f;
         src.resampleWeight *= dstPHat / srcPHat;
+        if (!traceVisibility<kCount>(S, st, visFrom, visTo, ctr)) src.resampleWeight = 0.0f;
     } else {
         src.resampleWeight = 0.0f;
     }
