@@ -624,7 +624,9 @@ def batched_views(rt, args, hs1, ds1):
     rt.check(L.rt_batch_create(len(specs), C.byref(b)), "batch")
     try:
         best, tune_ms = rt.BatchTuning(), C.c_float()
-        rt.check(L.rt_batch_autotune(b, views_for(0), len(specs), 2, stream, C.byref(best), C.byref(tune_ms)), "tune")
+        # tuned on frame 1 of each view's walk: frame 0's camera carries the first-frame flag, under which
+        # temporal reuse does nothing, so frames tuned on it would not represent the timed ones
+        rt.check(L.rt_batch_autotune(b, views_for(1), len(specs), 2, stream, C.byref(best), C.byref(tune_ms)), "tune")
         for fr in frames:
             fr.reset()
         for k in range(2):
@@ -751,7 +753,8 @@ def full_form_4k(rt, args):
     try:
         # tuned on these views (the tuning frames also train the 4K view's networks: warm-up)
         best, tune_ms = rt.BatchTuning(), C.c_float()
-        rt.check(L.rt_batch_autotune(b, views_for(0), n_views, 2, stream, C.byref(best), C.byref(tune_ms)), "tune")
+        # frame 1 of the walk (frame 0's first-frame flag would switch temporal reuse off while tuning)
+        rt.check(L.rt_batch_autotune(b, views_for(1), n_views, 2, stream, C.byref(best), C.byref(tune_ms)), "tune")
         for e in [fr] + extra:
             e.reset()
 

@@ -564,7 +564,9 @@ rt_status rt_batch_get_tuning(const rt_batch* batch, rt_batch_tuning* tuning);
  * one warm-up frame; the frames alternate the views' parity (this frame's
  * G-buffer, reservoirs and proposal lobes swapped with the previous frame's),
  * as a caller's frames do, so temporal reuse works on the previous frame's
- * output. Keeps the fastest and returns it with its mean frame time. The
+ * output (the cameras are used as given: a camera flagged as a sequence's
+ * first frame switches temporal reuse off in every tuning frame, so tune on
+ * a later frame). Keeps the fastest and returns it with its mean frame time. The
  * views' reservoirs are left in an arbitrary state (and their networks trained
  * on those frames): reset them before rendering for real. */
 rt_status rt_batch_autotune(rt_batch* batch, const rt_view* views, uint32_t numViews, uint32_t frames,

@@ -198,11 +198,11 @@ RT_DEV void reconnection(const DevScene& S, TStack st, DIResv& dst, const Surf& 
     const float cosTheta = -dot(rc.norm, wi);
     const float dstJacobian = fabsf(cosTheta) / square(dist);
     const float jacobian = dstJacobian / ss.jacobian;
-    bool valid = false;
     float dstPHat = 0.0f, dstSamplePdf = 0.0f;
-    if (diSampleValid(ss) && dist > 1e-4f && cosTheta > 0.0f && !isnan(jacobian) && ss.jacobian > 0.0f &&
-        traceVisibility<kCount>(S, st, dstSurf.pos, rc.pos, ctr)) {
-        valid = true;
+    // The shifted sample is formed before the visibility of its reconnection is traced, so that
+    // only it and the ray stay live across the traversal; an occluded shift merges with weight 0
+    // (only its M counts: a weight-0 candidate is never picked).
+    if (diSampleValid(ss) && dist > 1e-4f && cosTheta > 0.0f && !isnan(jacobian) && ss.jacobian > 0.0f) {
         const Mat dstMat = loadMat(S, dstSurf.matIndex);
         if (!isnan(ss.samplePdf) && ss.samplePdf > 1e-6f) {
             const f3 Li = src.s.Li * evalBSDF(dstMat, dstSurf.albedo, dstSurf.norm, wo, wi) * satDot(dstSurf.norm, wi) /
@@ -211,14 +211,13 @@ RT_DEV void reconnection(const DevScene& S, TStack st, DIResv& dst, const Surf& 
         }
         if (ss.isLight) dstSamplePdf = luminance(rc.albedo) / S.lightTable[0].prob / dstJacobian;
         else dstSamplePdf = evalPdf(dstMat, dstSurf.norm, wo, wi);
-    }
-    if (valid) {
         ss.jacobian = dstJacobian;
         ss.samplePdf = dstSamplePdf;
         if (ss.samplePdf < 1e-6f || isnan(ss.samplePdf)) ss.samplePdf = 0.0f;
         src.s = ss;
         src.resampleWeight *= dstPHat / src.weight;
         if (isnan(src.resampleWeight)) src.resampleWeight = 0.0f;
+        if (!traceVisibility<kCount>(S, st, dstSurf.pos, rc.pos, ctr)) src.resampleWeight = 0.0f;
     } else {
         src.resampleWeight = 0.0f;
     }

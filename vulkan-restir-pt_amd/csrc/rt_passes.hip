@@ -919,7 +919,6 @@ RT_DEV bool shiftSupports(const DevScene& S, TStack st, const rt_gris_settings& 
     const bool facing = flagsRcVertexType(y.flags) == RC_SURFACE ? cosTheta != 0.0f : cosTheta > 0.0f;
     if (!(dist > GRIS_DIST_THRESHOLD && facing && jac > 0.0f && !isnan(jac) && isBSDFConnectible(rcPrevMat)))
         return false;
-    if (!traceVisibility<kCount>(S, st, rcPrevSurf.pos, rcSurf.pos, ctr)) return false;
     Resv x = y;
     x.rcPrevSamplePdf = flagsRcVertexType(y.flags) == RC_LIGHT_SAMPLED
                             ? luminance(rcSurf.albedo) / S.lightTable[0].prob / jac
@@ -930,8 +929,10 @@ RT_DEV bool shiftSupports(const DevScene& S, TStack st, const rt_gris_settings& 
     const f3 Li = reconnectLi(S, x, x.rcLi * misW, rcPrevSurf, rcSurf, wi, rc, true);
     // x is a candidate of that domain iff its resampling weight luminance(F) is positive
     if (hasNan(Li) || !(luminance(Li) > 0.0f)) return false;
-    if (kLearned) return luminance(reconnectLi(S, x, targetRcLi(x) * misW, rcPrevSurf, rcSurf, wi, rc, true)) > 0.0f;
-    return true;
+    if (kLearned && !(luminance(reconnectLi(S, x, targetRcLi(x) * misW, rcPrevSurf, rcSurf, wi, rc, true)) > 0.0f))
+        return false;
+    // the visibility of the reconnection last: a shift that fails the other conditions traces no ray
+    return traceVisibility<kCount>(S, st, rcPrevSurf.pos, rcSurf.pos, ctr);
 }
 
 // Primary vertex of pixel (x, y) as its own passes rebuilt it (texel-centre
