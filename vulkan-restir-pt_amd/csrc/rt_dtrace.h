@@ -36,10 +36,16 @@ namespace rtd {
 // Work counters of the counting kernel builds (nodes, triangles, rays).
 struct TraceCounters { unsigned long long nodes, tris, rays; };
 
-// Pixel queue of a persistent pass: `next` is the next unclaimed work item,
-// `done` counts finished workgroups. Both are zero between launches (the
-// last workgroup of a launch clears them). One 128-B line per queue.
-struct WorkQueue { uint32_t next, done, pad[30]; };
+// Pixel queue of a persistent pass. The work items (pixels in tile order) are
+// split into kQueueParts contiguous bands of tiles, each with its own counter
+// of claimed items in its own 128-B line, so that the waves' claims spread over
+// kQueueParts atomics instead of one; `done` counts finished workgroups. All are
+// zero between launches (the last workgroup of a launch clears them).
+constexpr uint32_t kQueueParts = 8;
+struct WorkQueue {
+    struct Part { uint32_t next, pad[31]; } part[kQueueParts];
+    uint32_t done, pad[31];
+};
 
 struct RayT {
     f3 o, d, invd;

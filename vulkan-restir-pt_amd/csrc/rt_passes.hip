@@ -631,6 +631,10 @@ RT_DEV void ptRun(const DevScene& S, const rt_camera& cam, const GB& g, const rt
     PTPath P;
     uint32_t pix = 0;
     bool live = false, drained = false;
+    // the queue's band this wave claims from (workgroup b starts at band b mod kQueueParts, so
+    // the waves of one XCD start in one band) and the bands it has found exhausted
+    const uint32_t tiles = total / kTileThreads;
+    uint32_t part = blockIdx.x % kQueueParts, exhausted = 0;
     while (true) {
         // lanes without a path take the next pixels; a background pixel or a
         // path that ends at its primary vertex sends the lane back for more
@@ -638,13 +642,24 @@ RT_DEV void ptRun(const DevScene& S, const rt_camera& cam, const GB& g, const rt
             const bool want = !live && !drained;
             const unsigned long long mask = __ballot(want);
             if (mask == 0ull) break;
+            if (exhausted == kQueueParts) {  // every band is claimed: the lanes are done
+                drained = true;
+                continue;
+            }
+            const uint32_t lo = (uint32_t)((uint64_t)tiles * part / kQueueParts) * kTileThreads;
+            const uint32_t hi = (uint32_t)((uint64_t)tiles * (part + 1u) / kQueueParts) * kTileThreads;
             const int leader = __ffsll((long long)mask) - 1;
             uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(&q->next, (uint32_t)__popcll(mask));
+            if ((int)lane == leader) base = atomicAdd(&q->part[part].next, (uint32_t)__popcll(mask));
             base = (uint32_t)__shfl((int)base, leader);
+            if (lo + base >= hi) {  // this band is claimed: move to the next one
+                part = part + 1u == kQueueParts ? 0u : part + 1u;
+                exhausted++;
+                continue;
+            }
             if (!want) continue;
-            const uint32_t w = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-            if (w >= total) { drained = true; continue; }
+            const uint32_t w = lo + base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+            if (w >= hi) continue;  // past the band's end: the lane claims again
             const uint32_t tile = w / kTileThreads, t = w % kTileThreads;
             const uint32_t x = (tile % tilesX) * tw + (t & (tw - 1u)), y = (tile / tilesX) * th + (t >> qShift);
             Surf surf; Mat mat;
@@ -680,7 +695,7 @@ RT_DEV void queueDone(WorkQueue* q) {
     if ((threadIdx.x & 63u) == 0u) {
         __threadfence();
         if (atomicAdd(&q->done, 1u) == gridDim.x - 1u) {
-            atomicExch(&q->next, 0u);
+            for (uint32_t k = 0; k < kQueueParts; k++) atomicExch(&q->part[k].next, 0u);
             atomicExch(&q->done, 0u);
         }
     }
