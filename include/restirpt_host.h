@@ -19,6 +19,7 @@ extern "C" {
 
 typedef struct rt_host_scene rt_host_scene;
 typedef struct rt_renderer rt_renderer;
+typedef struct rt_pipeline rt_pipeline;
 
 /* Procedural scenes (the workloads this build is measured on):
  *   0  closed box room of 12 quads + 64 icospheres (subdivision 2), Lambert,
@@ -99,6 +100,39 @@ rt_status rt_renderer_enable_learned(rt_renderer* r, uint32_t seed, uint32_t log
  * owned by the renderer. */
 rt_status rt_renderer_enable_proposal(rt_renderer* r, uint32_t seed, uint32_t log2TableSize, hipStream_t stream,
                                       rt_mlp** mlp);
+
+/* GRIS frames overlapped on two streams: no counterpart in the reference,
+ * whose Renderer::drawFrame (src/Renderer.cpp:539-569) records each frame
+ * after the last. Frame k + 1's G-buffer and path trace need only its camera,
+ * so they run on traceStream beside frame k's temporal and spatial reuse on
+ * reuseStream (events order the two). Each frame computes exactly what
+ * rt_renderer_frame (method 0) computes for it with the same settings and
+ * cameras: the same reservoirs and output, bit for bit. The pipeline owns
+ * three G-buffers, two motion-vector images, two path-trace and two final
+ * reservoir buffers, one Temp buffer and the indirect output.
+ * Settings: the luminance target with BSDF sampling (targetFunction 0,
+ * guidedProposal 0), either resampling weights; rt_pipeline_set_settings
+ * rejects the others with RT_ERR_INVALID_ARGUMENT. The two streams may be the
+ * same stream (then the frames run one after the other); the gain needs two
+ * streams on different hardware queues. */
+rt_status rt_pipeline_create(rt_device_scene* scene, uint32_t width, uint32_t height, rt_pipeline** out);
+rt_status rt_pipeline_destroy(rt_pipeline* p);
+rt_status rt_pipeline_set_settings(rt_pipeline* p, const rt_gris_settings* settings);
+/* Enqueues one frame; cams[0].filmSize must equal the pipeline's size. The
+ * frame's output and final reservoirs are complete once reuseStream has run
+ * up to this call. */
+rt_status rt_pipeline_frame(rt_pipeline* p, const rt_camera cams[2], hipStream_t traceStream, hipStream_t reuseStream);
+/* The bindings after the last enqueued frame, as rt_renderer_bindings gives
+ * them before the flip: that frame's G-buffer and motion vectors, its final
+ * reservoirs (grisReservoir), the frame before's (grisReservoirPrev), the
+ * indirect output. */
+rt_status rt_pipeline_bindings(const rt_pipeline* p, rt_frame* out);
+/* Copies the RGBA32F indirect output to host memory on stream (pass the
+ * reuse stream: the copy follows the last frame's spatial reuse). */
+rt_status rt_pipeline_read_output(const rt_pipeline* p, float* host, hipStream_t stream);
+/* Waits for the device, resets every reservoir buffer on stream, waits for
+ * the reset, and restarts the frame count. */
+rt_status rt_pipeline_reset_reservoirs(rt_pipeline* p, hipStream_t stream);
 
 /* Writes an 8-bit RGBA image (host memory, row 0 at the top) as a PNG file:
  * the stbi_write_png call of Renderer::writeScreenshot (src/Renderer.cpp:787-788).

@@ -1,0 +1,107 @@
+"""rt_pipeline (GRIS frames overlapped on two streams) against rt_renderer (one frame after the
+other): over a camera walk both must leave the same final reservoirs for the last two frames and
+the same indirect output, bit for bit, with biased and unbiased resampling weights; a reset
+restarts the walk; settings the pipeline does not take are refused."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import gpuhelp  # noqa: E402
+import orc  # noqa: E402
+
+rt = gpuhelp.rt
+W, H = 160, 96
+FRAMES = 7
+INVALID_ARGUMENT = 1  # RT_ERR_INVALID_ARGUMENT
+
+
+def download(ptr):
+    out = np.zeros(W * H, orc.RESV_DTYPE)
+    torch.cuda.synchronize()
+    rt.check(rt.core().rt_gris_reservoir_download(ptr, W, H, out.ctypes.data, gpuhelp.stream()), "download")
+    return out
+
+
+def sequential(hs, ds, s, frames):
+    Hh = rt.host()
+    r = C.c_void_p()
+    rt.check(Hh.rt_renderer_create(ds.handle, W, H, C.byref(r)), "renderer")
+    try:
+        rt.check(Hh.rt_renderer_set_settings(r, C.byref(s)), "settings")
+        for k in range(frames):
+            rt.check(Hh.rt_renderer_frame(r, hs.camera(k, W, H), 0, gpuhelp.stream()), "frame")
+        f = rt.Frame()
+        rt.check(Hh.rt_renderer_bindings(r, C.byref(f)), "bindings")
+        out = np.zeros((H, W, 4), np.float32)
+        rt.check(Hh.rt_renderer_read_output(r, 1, out.ctypes.data, gpuhelp.stream()), "read")
+        # after the flip: grisReservoirPrev holds the last frame's reservoirs
+        return download(f.grisReservoirPrev), download(f.grisReservoir), out
+    finally:
+        Hh.rt_renderer_destroy(r)
+
+
+def pipelined(Hh, p, hs, frames, ta, tb):
+    for k in range(frames):
+        rt.check(Hh.rt_pipeline_frame(p, hs.camera(k, W, H), ta, tb), "pipeline frame")
+    f = rt.Frame()
+    rt.check(Hh.rt_pipeline_bindings(p, C.byref(f)), "bindings")
+    out = np.zeros((H, W, 4), np.float32)
+    rt.check(Hh.rt_pipeline_read_output(p, out.ctypes.data, tb), "read")
+    return download(f.grisReservoir), download(f.grisReservoirPrev), out
+
+
+@pytest.mark.parametrize("unbiased", [False, True])
+def test_pipeline_matches_sequential_frames(unbiased):
+    hs = rt.HostScene(1, 1)
+    ds = rt.DeviceScene(hs.arrays)
+    s = rt.settings(shift=2, temporal=True, spatial=True, max_depth=6, cap=20, neighbors=3, radius=20.0,
+                    unbiased=unbiased)
+    print(f"[gpu] pipeline vs sequential, unbiased={unbiased}", flush=True)
+    want_last, want_prev, want_out = sequential(hs, ds, s, FRAMES)
+    assert (want_last["F"] != 0).any()
+    Hh = rt.host()
+    p = C.c_void_p()
+    rt.check(Hh.rt_pipeline_create(ds.handle, W, H, C.byref(p)), "pipeline")
+    try:
+        rt.check(Hh.rt_pipeline_set_settings(p, C.byref(s)), "settings")
+        streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+        ta, tb = (C.c_void_p(x.cuda_stream) for x in streams)
+        torch.cuda.synchronize()
+        for run in range(2):  # the second walk after a reset must repeat the first
+            last, prev, out = pipelined(Hh, p, hs, FRAMES, ta, tb)
+            assert not gpuhelp.compare_reservoirs(want_last, last).any(), f"walk {run}: last frame differs"
+            assert not gpuhelp.compare_reservoirs(want_prev, prev).any(), f"walk {run}: frame before differs"
+            assert gpuhelp.same_float(want_out, out).all(), f"walk {run}: output differs"
+            rt.check(Hh.rt_pipeline_reset_reservoirs(p, gpuhelp.stream()), "reset")
+        # one stream for both: the frames run one after the other and still agree
+        last, prev, out = pipelined(Hh, p, hs, FRAMES, ta, ta)
+        assert not gpuhelp.compare_reservoirs(want_last, last).any()
+        assert gpuhelp.same_float(want_out, out).all()
+    finally:
+        torch.cuda.synchronize()
+        Hh.rt_pipeline_destroy(p)
+
+
+def test_pipeline_refuses_learned_forms_and_wrong_film_size():
+    hs = rt.HostScene(1, 1)
+    ds = rt.DeviceScene(hs.arrays)
+    Hh = rt.host()
+    p = C.c_void_p()
+    rt.check(Hh.rt_pipeline_create(ds.handle, W, H, C.byref(p)), "pipeline")
+    try:
+        st = gpuhelp.stream()
+        for kw in ({"learned": True}, {"guided": True}):
+            s = rt.settings(shift=2, temporal=True, spatial=True, max_depth=4, **kw)
+            assert Hh.rt_pipeline_set_settings(p, C.byref(s)) == INVALID_ARGUMENT
+        s = rt.settings(shift=2, temporal=True, spatial=True, max_depth=4)
+        rt.check(Hh.rt_pipeline_set_settings(p, C.byref(s)), "settings")
+        assert Hh.rt_pipeline_frame(p, hs.camera(0, W + 8, H), st, st) == INVALID_ARGUMENT
+        rt.check(Hh.rt_pipeline_frame(p, hs.camera(0, W, H), st, st), "frame")
+    finally:
+        torch.cuda.synchronize()
+        Hh.rt_pipeline_destroy(p)

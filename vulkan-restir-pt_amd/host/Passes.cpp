@@ -230,3 +230,99 @@ void Renderer::drawFrame(hipStream_t stream, const rt_camera cams[2], uint32_t d
     } else if (indirectMethod == MethodResampledPT) gris.render(stream, mScene, cams, f);
     flip();
 }
+
+// ---- FramePipeline ----
+FramePipeline::FramePipeline(const DeviceScene& scene, uint32_t w, uint32_t h) : width(w), height(h), mScene(scene) {
+    const size_t n = (size_t)w * h;
+    for (int j = 0; j < 3; j++) {
+        mDepthNormal[j] = deviceAlloc<float>(n * GBufferPass::DepthNormalBytes);
+        mAlbedoMatId[j] = deviceAlloc<uint32_t>(n * GBufferPass::AlbedoMatIdBytes);
+    }
+    for (int j = 0; j < 2; j++) mMotion[j] = deviceAlloc<uint16_t>(n * GBufferPass::MotionVectorBytes);
+    const size_t rb = rt_gris_reservoir_bytes(w, h);
+    for (int j = 0; j < 2; j++) {
+        mTraced[j] = deviceAlloc<uint8_t>(rb);
+        mFinal[j] = deviceAlloc<uint8_t>(rb);
+    }
+    mTemp = deviceAlloc<uint8_t>(rb);
+    indirectOutput = deviceAlloc<float>(n * 16);
+    for (int j = 0; j < 3; j++) {
+        if (hipEventCreateWithFlags(&mTracedEv[j], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&mTemporalEv[j], hipEventDisableTiming) != hipSuccess)
+            throw std::runtime_error("hipEventCreate failed");
+    }
+    resetReservoirs(0);
+    if (hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("hipDeviceSynchronize failed");
+}
+FramePipeline::~FramePipeline() {
+    (void)hipDeviceSynchronize();
+    for (int j = 0; j < 3; j++) {
+        (void)hipFree(mDepthNormal[j]); (void)hipFree(mAlbedoMatId[j]);
+        if (mTracedEv[j]) (void)hipEventDestroy(mTracedEv[j]);
+        if (mTemporalEv[j]) (void)hipEventDestroy(mTemporalEv[j]);
+    }
+    for (int j = 0; j < 2; j++) {
+        (void)hipFree(mMotion[j]); (void)hipFree(mTraced[j]); (void)hipFree(mFinal[j]);
+    }
+    (void)hipFree(mTemp); (void)hipFree(indirectOutput);
+}
+void FramePipeline::resetReservoirs(hipStream_t stream) {
+    // frames still in flight on the caller's two streams read and write these buffers
+    if (mFrame && hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("hipDeviceSynchronize failed");
+    for (int j = 0; j < 2; j++) {
+        rtThrow(rt_gris_reservoir_reset(mTraced[j], width, height, stream), "reset");
+        rtThrow(rt_gris_reservoir_reset(mFinal[j], width, height, stream), "reset");
+    }
+    rtThrow(rt_gris_reservoir_reset(mTemp, width, height, stream), "reset");
+    // the next frames run on other streams
+    if (hipStreamSynchronize(stream) != hipSuccess) throw std::runtime_error("hipStreamSynchronize failed");
+    mFrame = 0;
+}
+// Bindings of frame k: its G-buffer and the previous one, its motion vectors, the path trace's
+// reservoirs as grisReservoir and the previous frame's final reservoirs as grisReservoirPrev.
+rt_frame FramePipeline::frameOf(uint64_t k) const {
+    rt_frame f{};
+    const uint32_t g = (uint32_t)(k % 3u), gp = (uint32_t)((k + 2u) % 3u), p = (uint32_t)(k & 1u);
+    f.width = width; f.height = height;
+    f.indirectOutput = indirectOutput;
+    f.depthNormal = mDepthNormal[g]; f.depthNormalPrev = mDepthNormal[gp];
+    f.albedoMatId = mAlbedoMatId[g]; f.albedoMatIdPrev = mAlbedoMatId[gp];
+    f.motionVector = mMotion[p];
+    f.grisReservoir = mTraced[p];
+    f.grisReservoirPrev = mFinal[p ^ 1u];
+    f.grisReservoirTemp = mTemp;
+    return f;
+}
+void FramePipeline::drawFrame(hipStream_t traceStream, hipStream_t reuseStream, const rt_camera cams[2]) {
+    if (settings.targetFunction != 0u || settings.guidedProposal != 0u)
+        throw std::invalid_argument("FramePipeline: the luminance target and BSDF sampling only");
+    if (cams[0].filmSize[0] != width || cams[0].filmSize[1] != height)
+        throw std::invalid_argument("FramePipeline: camera film size differs from the pipeline's");
+    const uint64_t k = mFrame;
+    const uint32_t e = (uint32_t)(k % 3u);
+    // frame k overwrites what frame k - 2's temporal reuse read (the G-buffer three frames
+    // back as its previous one, the path-trace reservoirs and motion vectors of its parity)
+    if (k >= 2 && hipStreamWaitEvent(traceStream, mTemporalEv[(k - 2) % 3u], 0) != hipSuccess)
+        throw std::runtime_error("hipStreamWaitEvent failed");
+    rt_frame f = frameOf(k);
+    rtThrow(rt_gbuffer_trace(mScene.handle(), &cams[0], const_cast<float*>(f.depthNormal),
+                             const_cast<uint32_t*>(f.albedoMatId), const_cast<uint16_t*>(f.motionVector), traceStream),
+            "rt_gbuffer_trace");
+    rtThrow(rt_gris_path_trace(mScene.handle(), cams, &f, &settings, traceStream), "rt_gris_path_trace");
+    if (hipEventRecord(mTracedEv[e], traceStream) != hipSuccess ||
+        hipStreamWaitEvent(reuseStream, mTracedEv[e], 0) != hipSuccess)
+        throw std::runtime_error("hipEventRecord failed");
+    rtThrow(rt_gris_temporal(mScene.handle(), cams, &f, &settings, reuseStream), "rt_gris_temporal");
+    if (hipEventRecord(mTemporalEv[e], reuseStream) != hipSuccess) throw std::runtime_error("hipEventRecord failed");
+    // spatial reuse reads Temp and writes the frame's final reservoirs
+    f.grisReservoir = mFinal[k & 1u];
+    rtThrow(rt_gris_spatial(mScene.handle(), cams, &f, &settings, reuseStream), "rt_gris_spatial");
+    mFrame++;
+}
+rt_frame FramePipeline::bindings() const {
+    const uint64_t k = mFrame ? mFrame - 1u : 0u;
+    rt_frame f = frameOf(k);
+    f.grisReservoir = mFinal[k & 1u];
+    f.grisReservoirPrev = mFinal[(k & 1u) ^ 1u];
+    return f;
+}

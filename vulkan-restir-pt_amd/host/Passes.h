@@ -148,3 +148,48 @@ private:
     RayTracing mNaiveDI, mNaiveGI;
     uint32_t mCurFrame = 0;
 };
+
+// GRIS frames with consecutive frames overlapped on two streams (no reference
+// counterpart; the reference records one frame after the other). Frame k + 1's
+// G-buffer and path trace depend only on its camera, so they run on the trace
+// stream beside frame k's temporal and spatial reuse on the reuse stream: the
+// path trace's latency-bound tail and the reuse passes fill each other's idle
+// CUs. Buffers: three G-buffers (a frame reads its own and the previous one
+// while the next is written), two motion-vector images, the path trace's
+// reservoirs and the final reservoirs apart (two each), one Temp. Frame k:
+//   trace stream:  wait for frame k - 2's temporal reuse; G-buffer; path trace
+//   reuse stream:  wait for frame k's path trace; temporal reuse; spatial reuse
+// Every frame computes exactly what Renderer::drawFrame computes for it. The
+// luminance target only (the learned forms' networks are read by one frame's
+// refresh while the previous frame trains them); either resampling weights.
+class FramePipeline {
+public:
+    FramePipeline(const DeviceScene& scene, uint32_t width, uint32_t height);
+    ~FramePipeline();
+    FramePipeline(const FramePipeline&) = delete;
+    FramePipeline& operator=(const FramePipeline&) = delete;
+
+    void drawFrame(hipStream_t traceStream, hipStream_t reuseStream, const rt_camera cams[2]);
+    // The bindings a sequential renderer presents after the last enqueued frame: its G-buffer,
+    // its final reservoirs (grisReservoir), the previous frame's (grisReservoirPrev), the outputs.
+    rt_frame bindings() const;
+    // Resets every reservoir buffer and restarts the frame count (the next frame is frame 0).
+    void resetReservoirs(hipStream_t stream);
+
+    GRISReSTIR::Settings settings = {GRISReSTIR::Hybrid, 1.0f, 0u, 1u, 20u, 15u, 3u, 20.0f, 0u, 0u, 0u};
+    float* indirectOutput = nullptr;
+    uint32_t width, height;
+
+private:
+    rt_frame frameOf(uint64_t k) const;
+    const DeviceScene& mScene;
+    float* mDepthNormal[3] = {nullptr, nullptr, nullptr};
+    uint32_t* mAlbedoMatId[3] = {nullptr, nullptr, nullptr};
+    uint16_t* mMotion[2] = {nullptr, nullptr};
+    void* mTraced[2] = {nullptr, nullptr};  // path-trace reservoirs
+    void* mFinal[2] = {nullptr, nullptr};   // reservoirs after spatial reuse
+    void* mTemp = nullptr;
+    hipEvent_t mTracedEv[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t mTemporalEv[3] = {nullptr, nullptr, nullptr};
+    uint64_t mFrame = 0;
+};

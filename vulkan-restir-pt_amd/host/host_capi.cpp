@@ -5,6 +5,10 @@
 #include <exception>
 
 struct rt_host_scene { std::unique_ptr<Scene> scene; };
+struct rt_pipeline {
+    std::unique_ptr<DeviceScene> scene;
+    std::unique_ptr<FramePipeline> p;
+};
 struct rt_renderer {
     std::unique_ptr<DeviceScene> scene;
     std::unique_ptr<Renderer> r;
@@ -202,6 +206,70 @@ rt_status rt_renderer_reset_reservoirs(rt_renderer* r, hipStream_t stream) {
     if (!r) return RT_ERR_INVALID_ARGUMENT;
     try {
         r->r->resetReservoirs(stream);
+        return RT_OK;
+    } catch (const std::exception& e) {
+        return fail(RT_ERR_HIP, e.what());
+    }
+}
+
+rt_status rt_pipeline_create(rt_device_scene* scene, uint32_t width, uint32_t height, rt_pipeline** out) {
+    if (!scene || !out || width == 0 || height == 0) return RT_ERR_INVALID_ARGUMENT;
+    try {
+        auto p = std::make_unique<rt_pipeline>();
+        p->scene = std::make_unique<DeviceScene>(scene);  // borrowed: the caller keeps ownership
+        p->p = std::make_unique<FramePipeline>(*p->scene, width, height);
+        *out = p.release();
+        return RT_OK;
+    } catch (const std::exception& e) {
+        return fail(RT_ERR_HIP, e.what());
+    }
+}
+
+rt_status rt_pipeline_destroy(rt_pipeline* p) {
+    if (!p) return RT_ERR_INVALID_ARGUMENT;
+    p->p.reset();
+    delete p;
+    return RT_OK;
+}
+
+rt_status rt_pipeline_set_settings(rt_pipeline* p, const rt_gris_settings* s) {
+    if (!p || !s) return RT_ERR_INVALID_ARGUMENT;
+    if (s->targetFunction != 0u || s->guidedProposal != 0u)
+        return fail(RT_ERR_INVALID_ARGUMENT, "rt_pipeline: the luminance target and BSDF sampling only");
+    p->p->settings = *s;
+    return RT_OK;
+}
+
+rt_status rt_pipeline_frame(rt_pipeline* p, const rt_camera cams[2], hipStream_t traceStream, hipStream_t reuseStream) {
+    if (!p || !cams) return RT_ERR_INVALID_ARGUMENT;
+    try {
+        p->p->drawFrame(traceStream, reuseStream, cams);
+        return RT_OK;
+    } catch (const std::invalid_argument& e) {
+        return fail(RT_ERR_INVALID_ARGUMENT, e.what());
+    } catch (const std::exception& e) {
+        return fail(RT_ERR_HIP, e.what());
+    }
+}
+
+rt_status rt_pipeline_bindings(const rt_pipeline* p, rt_frame* out) {
+    if (!p || !out) return RT_ERR_INVALID_ARGUMENT;
+    *out = p->p->bindings();
+    return RT_OK;
+}
+
+rt_status rt_pipeline_read_output(const rt_pipeline* p, float* host, hipStream_t stream) {
+    if (!p || !host) return RT_ERR_INVALID_ARGUMENT;
+    const size_t bytes = (size_t)p->p->width * p->p->height * 16;
+    if (hipMemcpyAsync(host, p->p->indirectOutput, bytes, hipMemcpyDeviceToHost, stream) != hipSuccess) return RT_ERR_HIP;
+    if (hipStreamSynchronize(stream) != hipSuccess) return RT_ERR_HIP;
+    return RT_OK;
+}
+
+rt_status rt_pipeline_reset_reservoirs(rt_pipeline* p, hipStream_t stream) {
+    if (!p) return RT_ERR_INVALID_ARGUMENT;
+    try {
+        p->p->resetReservoirs(stream);
         return RT_OK;
     } catch (const std::exception& e) {
         return fail(RT_ERR_HIP, e.what());

@@ -207,6 +207,8 @@ HOST_SYMBOLS = [
     "rt_renderer_enable_learned", "rt_renderer_enable_proposal", "rt_renderer_frame_methods",
     "rt_renderer_set_di_settings",
     "rt_renderer_di_bindings", "rt_write_png", "rt_read_png", "rt_renderer_screenshot",
+    "rt_pipeline_create", "rt_pipeline_destroy", "rt_pipeline_set_settings", "rt_pipeline_frame",
+    "rt_pipeline_bindings", "rt_pipeline_read_output", "rt_pipeline_reset_reservoirs",
 ]
 
 _core = None
@@ -293,6 +295,13 @@ def _declare_host(L):
         "rt_host_scene_arrays": [P, C.POINTER(SceneArrays)],
         "rt_host_scene_camera": [P, u32, u32, u32, C.POINTER(Camera)],
         "rt_light_table_build": [P, u32, P],
+        "rt_pipeline_create": [P, u32, u32, C.POINTER(P)],
+        "rt_pipeline_destroy": [P],
+        "rt_pipeline_set_settings": [P, C.POINTER(GrisSettings)],
+        "rt_pipeline_frame": [P, C.POINTER(Camera), P, P],
+        "rt_pipeline_bindings": [P, C.POINTER(Frame)],
+        "rt_pipeline_read_output": [P, P, P],
+        "rt_pipeline_reset_reservoirs": [P, P],
         "rt_renderer_create": [P, u32, u32, C.POINTER(P)],
         "rt_renderer_destroy": [P],
         "rt_renderer_set_settings": [P, C.POINTER(GrisSettings)],
