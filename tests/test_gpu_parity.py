@@ -45,7 +45,15 @@ def test_lbvh_bitexact(scene):
     assert np.array_equal(order, oorder), "radix sort order differs"
     assert np.array_equal(nodes, onodes), "LBVH nodes differ"
     assert np.array_equal(np.array(info.sceneLo, np.float32), obox[:3])
-    assert np.array_equal(gpuhelp.gpu_bvh4(ds), osc.bvh4()), "4-wide nodes differ"
+    nodes4 = gpuhelp.gpu_bvh4(ds)
+    assert np.array_equal(nodes4, osc.bvh4()), "4-wide nodes differ"
+    # the reported 4-wide depth is the number of levels below the root, walked on the oracle's tree
+    levels, cur = 0, np.array([0])
+    while cur.size:
+        levels += 1
+        ch = nodes4[cur, 24:28].reshape(-1)
+        cur = ch[(ch != 0xFFFFFFFF) & (ch & 0x80000000 == 0)].astype(np.int64)
+    assert info.bvh4Depth == levels
     assert 3 * info.bvh4Depth <= 256
 
 

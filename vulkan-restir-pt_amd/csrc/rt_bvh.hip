@@ -269,7 +269,10 @@ __global__ void k_scan_sums(uint32_t* __restrict__ sums, uint32_t m) {
     }
 }
 
-__global__ void k_scan_apply(const uint32_t* in, uint32_t n, const uint32_t* __restrict__ sums, uint32_t* out) {
+// With compact non-null (and in holding 0/1 marks) also writes the marked
+// indices, in order, to compact[out[i]].
+__global__ void k_scan_apply(const uint32_t* in, uint32_t n, const uint32_t* __restrict__ sums, uint32_t* out,
+                             uint32_t* __restrict__ compact) {
     const uint32_t b = blockIdx.x * kScanChunk + threadIdx.x * kItems;
     uint32_t v[kItems];
     uint32_t s = 0;
@@ -280,7 +283,11 @@ __global__ void k_scan_apply(const uint32_t* in, uint32_t n, const uint32_t* __r
     uint32_t tot;
     uint32_t run = sums[blockIdx.x] + blockExclusiveScan(s, &tot);
     for (int k = 0; k < kItems; k++)
-        if (b + k < n) { out[b + k] = run; run += v[k]; }
+        if (b + k < n) {
+            out[b + k] = run;
+            if (compact && v[k]) compact[run] = b + k;
+            run += v[k];
+        }
 }
 
 // 5. Karras topology. Also records each node's leaf range and clears the
@@ -485,60 +492,147 @@ __device__ int collapseSlots(const float4* nodes, uint32_t n, Slot4 s[4]) {
     return cnt;
 }
 
-// Frontier expansion of one level. Every kept node is listed once in `kept`,
-// level after level: level l occupies kept[start_l, start_l + cnt[l]) with
-// start_l the sum of the earlier levels' counts (all final before this launch),
-// and its nodes append their internal slots behind it as level l + 1 (one
-// atomic per wave for the wave's appends). The marks in `keep` give the 4-wide
-// node numbering in binary order.
-__device__ __forceinline__ uint32_t levelStart(const uint32_t* cnt, uint32_t level) {
-    uint32_t s = 0;
-    for (uint32_t j = 0; j < level; j++) s += cnt[j];
-    return s;
+// Keep marks, top-down in two launches. A kept node keeps the internal nodes
+// among its collapsed slots; every other internal node is absorbed into the
+// 4-wide node of its nearest kept ancestor. The nodes whose leaves span several
+// kCollapseLeaves-leaf blocks (the top part, a few per block boundary) are
+// expanded by one workgroup, level after level; a kept node it reaches inside a
+// block is a seed of that block. Then one workgroup per block expands from its
+// seeds: a block's nodes are numbered inside its leaf interval and its subtrees
+// stay inside it, so the workgroup first copies each of its nodes' two slot
+// references and slot box areas into LDS and then runs every level from LDS.
+// Each node's 4-wide level (root 0) travels with it; depth4 receives the
+// largest level + 1.
+constexpr uint32_t kCollapseLeaves = kRefitLeaves;
+constexpr uint32_t kCollapseThreads = 1024;
+
+__device__ __forceinline__ uint32_t blockOf(uint32_t leaf) { return leaf / kCollapseLeaves; }
+
+// list: nb entries (the top part's kept nodes, level after level); seeds: nb
+// entries of (node, block << 8 | level); ctr[0] = seed count, ctr[1] = depth4
+// (both zero). A 4-wide level is below the binary height (< 256).
+__global__ __launch_bounds__(kCollapseThreads) void k_collapse_top(const float4* __restrict__ nodes,
+                                                                   const uint2* __restrict__ range, uint32_t* keep,
+                                                                   uint32_t* list, uint2* __restrict__ seeds,
+                                                                   uint32_t* ctr) {
+    __shared__ uint32_t tail, deepest;
+    if (threadIdx.x == 0) {
+        list[0] = 0u;  // the root; kept by the caller's fill
+        tail = 1u;
+        deepest = 0u;
+    }
+    __syncthreads();
+    uint32_t a = 0, b = 1, level = 0;
+    while (a < b) {
+        for (uint32_t i = a + threadIdx.x; i < b; i += blockDim.x) {
+            Slot4 sl[4];
+            const int c = collapseSlots(nodes, list[i], sl);
+            for (int k = 0; k < c; k++) {
+                const uint32_t r = sl[k].ref;
+                if (r & RT_LEAF) continue;
+                keep[r] = 1u;
+                const uint2 lr = range[r];
+                if (blockOf(lr.x) != blockOf(lr.y)) list[atomicAdd(&tail, 1u)] = r;
+                else seeds[atomicAdd(&ctr[0], 1u)] = make_uint2(r, blockOf(lr.x) << 8 | (level + 1u));
+            }
+        }
+        if (threadIdx.x == 0) deepest = level;
+        __syncthreads();
+        a = b;
+        b = tail;
+        level++;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(&ctr[1], deepest + 1u);
 }
 
-__global__ void k_collapse_mark(const float4* __restrict__ nodes, uint32_t* __restrict__ kept, uint32_t* cnt,
-                                uint32_t level, uint32_t* __restrict__ keep) {
-    const uint32_t start = levelStart(cnt, level);
-    const uint32_t nf = cnt[level];
-    uint32_t* next = kept + start + nf;
-    const uint32_t lane = threadIdx.x & 63u;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < nf; base += gridDim.x * blockDim.x) {
-        const uint32_t i = base + threadIdx.x;
-        Slot4 s[4];
-        int n = 0;
-        uint32_t refs[4] = {0u, 0u, 0u, 0u};
-        if (i < nf) {
-            const int c = collapseSlots(nodes, kept[start + i], s);
-            for (int k = 0; k < c; k++)
-                if (!(s[k].ref & RT_LEAF)) refs[n++] = s[k].ref;
-        }
-        // exclusive prefix of the appends over the wave, one atomic per wave
-        uint32_t inc = (uint32_t)n;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t t = __shfl_up(inc, o, 64);
-            if (lane >= (uint32_t)o) inc += t;
-        }
-        const uint32_t total = __shfl(inc, 63, 64);
-        uint32_t waveBase = 0;
-        if (lane == 63u && total) waveBase = atomicAdd(&cnt[level + 1], total);
-        waveBase = __shfl(waveBase, 63, 64);
-        const uint32_t at = waveBase + inc - (uint32_t)n;
-        for (int k = 0; k < n; k++) {
-            keep[refs[k]] = 1u;
-            next[at + k] = refs[k];
+// One workgroup per block of kCollapseLeaves leaves (n >= 2); nb = n - 1.
+__global__ __launch_bounds__(kCollapseThreads) void k_collapse_block(const float4* __restrict__ nodes, uint32_t n,
+                                                                     uint32_t nb, uint32_t* keep,
+                                                                     const uint2* __restrict__ seeds,
+                                                                     const uint32_t* ctr, uint32_t* depth4) {
+    // the block's node ids [L0, min(L1, nb)): slot references and slot box areas
+    __shared__ uint2 slotRef[kCollapseLeaves];
+    __shared__ float2 slotArea[kCollapseLeaves];
+    // a block holds at most kCollapseLeaves - 1 internal nodes, each in one frontier once
+    __shared__ uint32_t front[2][kCollapseLeaves];
+    __shared__ uint8_t lev[2][kCollapseLeaves];
+    __shared__ uint32_t size[2], deepest;
+    const uint32_t L0 = blockIdx.x * kCollapseLeaves, I1 = min(nb, min(n, L0 + kCollapseLeaves));
+    if (threadIdx.x == 0) { size[0] = 0u; size[1] = 0u; deepest = 0u; }
+    for (uint32_t i = L0 + threadIdx.x; i < I1; i += blockDim.x) {
+        const float4* q = nodes + 4 * (size_t)i;
+        const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+        const float v[12] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+        slotRef[i - L0] = make_uint2(__float_as_uint(q3.x), __float_as_uint(q3.y));
+        slotArea[i - L0] = make_float2(boxArea(v, v + 3), boxArea(v + 6, v + 9));
+    }
+    __syncthreads();
+    const uint32_t ns = ctr[0];
+    for (uint32_t e = threadIdx.x; e < ns; e += blockDim.x) {
+        const uint2 sd = seeds[e];
+        if ((sd.y >> 8) == blockIdx.x) {
+            const uint32_t at = atomicAdd(&size[0], 1u);
+            front[0][at] = sd.x;
+            lev[0][at] = (uint8_t)(sd.y & 0xffu);
         }
     }
+    __syncthreads();
+    uint32_t cur = 0, myDeepest = 0;
+    while (size[cur] != 0u) {
+        const uint32_t nf = size[cur];
+        for (uint32_t i = threadIdx.x; i < nf; i += blockDim.x) {
+            const uint32_t level = lev[cur][i];
+            myDeepest = max(myDeepest, level);
+            // collapseSlots on the LDS copy: the same slots, areas and first-largest choice
+            const uint32_t x = front[cur][i] - L0;
+            uint32_t r[4] = {slotRef[x].x, slotRef[x].y, 0u, 0u};
+            float A[4] = {slotArea[x].x, slotArea[x].y, 0.0f, 0.0f};
+            int c = 2;
+            for (int step = 0; step < 2; step++) {
+                int best = -1; float bestA = -1.0f;
+                for (int k = 0; k < c; k++) {
+                    if (r[k] & RT_LEAF) continue;
+                    if (best < 0 || A[k] > bestA) { best = k; bestA = A[k]; }
+                }
+                if (best < 0) break;
+                const uint32_t y = r[best] - L0;
+                for (int k = c; k > best + 1; k--) { r[k] = r[k - 1]; A[k] = A[k - 1]; }
+                r[best] = slotRef[y].x; A[best] = slotArea[y].x;
+                r[best + 1] = slotRef[y].y; A[best + 1] = slotArea[y].y;
+                c++;
+            }
+            int m = 0;
+            for (int k = 0; k < c; k++) m += (r[k] & RT_LEAF) ? 0 : 1;
+            if (m) {
+                uint32_t at = atomicAdd(&size[cur ^ 1u], (uint32_t)m);
+                for (int k = 0; k < c; k++) {
+                    if (r[k] & RT_LEAF) continue;
+                    keep[r[k]] = 1u;
+                    front[cur ^ 1u][at] = r[k];
+                    lev[cur ^ 1u][at] = (uint8_t)(level + 1u);
+                    at++;
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) size[cur] = 0u;
+        cur ^= 1u;
+        __syncthreads();
+    }
+    if (myDeepest) atomicMax(&deepest, myDeepest);
+    __syncthreads();
+    if (threadIdx.x == 0 && deepest) atomicMax(depth4, deepest + 1u);
 }
 
-// One thread per kept node (grid-stride over all levels' lists).
-__global__ void k_collapse_emit(const float4* __restrict__ nodes, const uint32_t* __restrict__ kept,
-                                const uint32_t* __restrict__ cnt, uint32_t levels, const uint32_t* __restrict__ newId,
+// One thread per kept node: kept[j] is the binary node of 4-wide node j; the
+// count is the last node's exclusive-scan position plus its mark.
+__global__ void k_collapse_emit(const float4* __restrict__ nodes, const uint32_t* __restrict__ keep,
+                                const uint32_t* __restrict__ kept, uint32_t nb, const uint32_t* __restrict__ newId,
                                 float4* __restrict__ nodes4) {
-    const uint32_t total = levelStart(cnt, levels + 1);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
-        const uint32_t n = kept[i];
+    const uint32_t total = newId[nb - 1] + keep[nb - 1];
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x) {
+        const uint32_t n = kept[j];
         Slot4 s[4];
         int cnt4 = collapseSlots(nodes, n, s);
         float lo[3][4], hi[3][4];
@@ -552,7 +646,7 @@ __global__ void k_collapse_emit(const float4* __restrict__ nodes, const uint32_t
                 for (int a = 0; a < 3; a++) { lo[a][k] = INFINITY; hi[a][k] = -INFINITY; }
             }
         }
-        float4* o = nodes4 + 8 * (size_t)newId[n];
+        float4* o = nodes4 + 8 * (size_t)j;
         for (int a = 0; a < 3; a++) {
             o[a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
             o[3 + a] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
@@ -564,12 +658,14 @@ __global__ void k_collapse_emit(const float4* __restrict__ nodes, const uint32_t
 
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-// Exclusive scan of n uint32; sums holds cdiv(n, kScanChunk) entries.
-void scanExclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* sums, hipStream_t stream) {
+// Exclusive scan of n uint32 (see k_scan_apply for compact); sums holds
+// cdiv(n, kScanChunk) entries.
+void scanExclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* sums, uint32_t* compact,
+                   hipStream_t stream) {
     const uint32_t chunks = cdiv(n, kScanChunk);
     k_scan_reduce<<<chunks, kBlock, 0, stream>>>(in, n, sums);
     k_scan_sums<<<1, 1024, 0, stream>>>(sums, chunks);
-    k_scan_apply<<<chunks, kBlock, 0, stream>>>(in, n, sums, out);
+    k_scan_apply<<<chunks, kBlock, 0, stream>>>(in, n, sums, out, compact);
 }
 
 }  // namespace
@@ -651,66 +747,39 @@ hipError_t launchBinaryBuild(const float4* primTris, uint32_t n, unsigned long l
 
 // Scratch of the 4-wide collapse, allocated before the build starts.
 struct CollapseScratch {
-    static constexpr uint32_t kMaxLevels = 4096;
-    uint32_t *keep = nullptr, *newId = nullptr, *kept = nullptr, *cnt = nullptr, *sums = nullptr;
+    uint32_t *keep = nullptr, *newId = nullptr, *kept = nullptr, *ctr = nullptr, *sums = nullptr;
+    uint2* seeds = nullptr;
     bool alloc(uint32_t nb) {
         return hipCheck(hipMalloc(&keep, sizeof(uint32_t) * nb), "hipMalloc") &&
                hipCheck(hipMalloc(&newId, sizeof(uint32_t) * nb), "hipMalloc") &&
                hipCheck(hipMalloc(&kept, sizeof(uint32_t) * nb), "hipMalloc") &&
-               hipCheck(hipMalloc(&cnt, sizeof(uint32_t) * (kMaxLevels + 2)), "hipMalloc") &&
+               hipCheck(hipMalloc(&seeds, sizeof(uint2) * nb), "hipMalloc") &&
+               hipCheck(hipMalloc(&ctr, sizeof(uint32_t) * 2), "hipMalloc") &&
                hipCheck(hipMalloc(&sums, sizeof(uint32_t) * cdiv(nb, kScanChunk)), "hipMalloc");
     }
     ~CollapseScratch() {
-        (void)hipFree(keep); (void)hipFree(newId); (void)hipFree(kept); (void)hipFree(cnt); (void)hipFree(sums);
+        (void)hipFree(keep); (void)hipFree(newId); (void)hipFree(kept); (void)hipFree(seeds); (void)hipFree(ctr);
+        (void)hipFree(sums);
     }
 };
 
-// Collapses the binary LBVH (s->nodes) into the 4-wide tree s->nodes4 (allocated
-// for nb nodes: the kept nodes are a subset of the binary internal nodes).
-// maxLevels bounds the 4-wide depth (the binary height does: every 4-wide
-// level consumes at least one binary level).
-static bool collapseBVH4(rt_device_scene* s, uint32_t nb, uint32_t maxLevels, CollapseScratch& cs, hipEvent_t done,
-                         hipStream_t stream) {
-    if (maxLevels > CollapseScratch::kMaxLevels) {
-        setError("binary LBVH height " + std::to_string(maxLevels) + " exceeds the collapse's level limit");
-        return false;
-    }
-    uint32_t *keep = cs.keep, *newId = cs.newId, *kept = cs.kept, *cnt = cs.cnt, *sums = cs.sums;
-    const uint32_t levels = maxLevels + 1;
-    bool ok = true;
-    uint32_t depth4 = 0, total = 0;
-    if (ok) {
-        // keep = {1, 0, 0, ...}, level 0 = {root}, counts {1, 0, 0, ...}
-        k_fill_u32<<<cdiv(nb, kBlock), kBlock, 0, stream>>>(keep, nb, 0u);
-        k_fill_u32<<<1, 1, 0, stream>>>(keep, 1, 1u);
-        k_fill_u32<<<1, 1, 0, stream>>>(kept, 1, 0u);
-        k_fill_u32<<<cdiv(levels + 1, kBlock), kBlock, 0, stream>>>(cnt, levels + 1, 0u);
-        k_fill_u32<<<1, 1, 0, stream>>>(cnt, 1, 1u);
-        // grid-stride over the frontier: an empty level costs one small launch
-        const uint32_t grid = std::max(1u, std::min(512u, cdiv(nb, kBlock)));
-        for (uint32_t l = 0; l < levels; l++)
-            k_collapse_mark<<<grid, kBlock, 0, stream>>>(s->nodes, kept, cnt, l, keep);
-        scanExclusive(keep, newId, nb, sums, stream);
-        k_collapse_emit<<<std::max(1u, std::min(2048u, cdiv(nb, kBlock))), kBlock, 0, stream>>>(s->nodes, kept, cnt,
-                                                                                               levels, newId, s->nodes4);
-        std::vector<uint32_t> h(levels + 1, 0u);
-        uint32_t lastKeep = 0, lastId = 0;
-        ok = hipCheck(hipGetLastError(), "collapse launch") && hipCheck(hipEventRecord(done, stream), "event") &&
-             hipCheck(hipMemcpyAsync(h.data(), cnt, sizeof(uint32_t) * (levels + 1), hipMemcpyDeviceToHost, stream), "copy") &&
-             hipCheck(hipMemcpyAsync(&lastKeep, keep + nb - 1, 4, hipMemcpyDeviceToHost, stream), "copy") &&
-             hipCheck(hipMemcpyAsync(&lastId, newId + nb - 1, 4, hipMemcpyDeviceToHost, stream), "copy") &&
-             hipCheck(hipStreamSynchronize(stream), "sync");
-        for (uint32_t l = 0; l <= levels; l++)
-            if (h[l]) depth4 = l + 1;
-        if (ok && h[levels] != 0) {
-            setError("4-wide collapse did not finish within the binary tree's height");
-            ok = false;
-        }
-        total = lastId + lastKeep;
-    }
-    s->numNodes4 = total;
-    s->bvh4Depth = depth4;
-    return ok;
+// Collapses the binary LBVH of n primitives (s->nodes, nb internal nodes) into
+// the 4-wide tree s->nodes4 (allocated for nb nodes: the kept nodes are a
+// subset of the binary internal nodes). Launches only: keep marks (top part,
+// then per block), the exclusive scan of the marks that numbers the kept nodes
+// in binary order and lists them, then one thread per 4-wide node.
+static void launchCollapseBVH4(rt_device_scene* s, uint32_t n, uint32_t nb, const uint2* range, CollapseScratch& cs,
+                               hipStream_t stream) {
+    k_fill_u32<<<cdiv(nb, kBlock), kBlock, 0, stream>>>(cs.keep, nb, 0u);
+    k_fill_u32<<<1, 1, 0, stream>>>(cs.keep, 1, 1u);
+    k_fill_u32<<<1, 2, 0, stream>>>(cs.ctr, 2, 0u);
+    k_collapse_top<<<1, kCollapseThreads, 0, stream>>>(s->nodes, range, cs.keep, cs.kept, cs.seeds, cs.ctr);
+    if (n >= 2)
+        k_collapse_block<<<cdiv(n, kCollapseLeaves), kCollapseThreads, 0, stream>>>(s->nodes, n, nb, cs.keep, cs.seeds,
+                                                                                     cs.ctr, cs.ctr + 1);
+    scanExclusive(cs.keep, cs.newId, nb, cs.sums, cs.kept, stream);
+    k_collapse_emit<<<std::max(1u, std::min(2048u, cdiv(nb, kBlock))), kBlock, 0, stream>>>(s->nodes, cs.keep, cs.kept,
+                                                                                           nb, cs.newId, s->nodes4);
 }
 
 bool buildLBVH(rt_device_scene* s, const std::vector<uint32_t>& instPrimOffset, hipStream_t stream) {
@@ -750,17 +819,25 @@ bool buildLBVH(rt_device_scene* s, const std::vector<uint32_t>& instPrimOffset, 
              hipCheck(hipEventRecord(ev[1], stream), "event");
     }
     if (ok) {
+        launchCollapseBVH4(s, n, nNodes, sc.range, cs, stream);
+        ok = hipCheck(hipGetLastError(), "collapse launch") && hipCheck(hipEventRecord(ev[2], stream), "event");
+    }
+    if (ok) {
         float hb[6] = {0, 0, 0, 0, 0, 0};
-        uint32_t depth = 0;
+        uint32_t depth = 0, depth4 = 0, lastKeep = 0, lastId = 0;
         ok = hipCheck(hipMemcpyAsync(hb, sc.box, sizeof(hb), hipMemcpyDeviceToHost, stream), "copy") &&
              hipCheck(hipMemcpyAsync(&depth, sc.height, 4, hipMemcpyDeviceToHost, stream), "copy") &&
+             hipCheck(hipMemcpyAsync(&depth4, cs.ctr + 1, 4, hipMemcpyDeviceToHost, stream), "copy") &&
+             hipCheck(hipMemcpyAsync(&lastKeep, cs.keep + nNodes - 1, 4, hipMemcpyDeviceToHost, stream), "copy") &&
+             hipCheck(hipMemcpyAsync(&lastId, cs.newId + nNodes - 1, 4, hipMemcpyDeviceToHost, stream), "copy") &&
              hipCheck(hipStreamSynchronize(stream), "sync");
         for (int a = 0; a < 3; a++) { s->sceneLo[a] = hb[a]; s->sceneHi[a] = hb[3 + a]; }
         s->bvhDepth = depth;
+        s->bvh4Depth = depth4;
+        s->numNodes4 = lastId + lastKeep;
     }
     (void)hipFree(dOffsets);
     (void)hipFree(scratch);
-    if (ok) ok = collapseBVH4(s, nNodes, s->bvhDepth, cs, ev[2], stream);
     if (ok) {
         ok = hipCheck(hipEventElapsedTime(&s->bvhBinaryMs, ev[0], ev[1]), "event time") &&
              hipCheck(hipEventElapsedTime(&s->bvhBuildMs, ev[0], ev[2]), "event time");
