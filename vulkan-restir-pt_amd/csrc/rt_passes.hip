@@ -399,6 +399,17 @@ RT_DEV Resv loadStream(StreamSlot slot) {
     r.F = mk3(v[14], v[15], v[16]); r.primaryRng = __float_as_uint(v[17]);
     return r;
 }
+// A destination reservoir whose sample lives in an LDS stream slot (the count
+// and weight stay in registers): the same merge and cap as on a Resv.
+struct LdsResv { StreamSlot slot; float sampleCount, resampleWeight; };
+RT_DEV void resvMerge(LdsResv& r, const Resv& rhs, float rnd) {
+    r.sampleCount += rhs.sampleCount;
+    r.resampleWeight += rhs.resampleWeight;
+    if (rnd * r.resampleWeight < rhs.resampleWeight) storeStream(r.slot, rhs);
+}
+RT_DEV void resvCap(LdsResv& r, float cap) {
+    if (r.sampleCount > cap) { r.resampleWeight *= cap / r.sampleCount; r.sampleCount = cap; }
+}
 
 // Start of the invocation (gris_path_trace.glsl:45-100): primary vertex from
 // the G-buffer. Returns false for background pixels, whose reservoir is left
@@ -831,8 +842,8 @@ RT_DEV float rcLightMIS(const DevScene& S, uint32_t rcType, const Surf& rcPrevSu
 // pdf_src / pdf_dst, so the next shift's target ratio is exact, as it is for a
 // fresh sample), a shift to a zero-pdf reconnection is rejected, and the M-cap
 // is left to the caller (applied once per pass after the 1/Z normalisation).
-template <bool kCount, bool kLearned = false, bool kUnbiased = false>
-RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& set, Resv& dst, const Surf& dstPrimarySurf,
+template <bool kCount, bool kLearned = false, bool kUnbiased = false, class Dst = Resv>
+RT_DEV void reuseAndMerge(const DevScene& S, TStack st, const rt_gris_settings& set, Dst& dst, const Surf& dstPrimarySurf,
                           const Isec& dstPrimaryIsec, f3 primDir, f3 primOri, Resv src, uint32_t& rng, TraceCounters* ctr,
                           const rtg::GuidePix& gpx) {
     RcData rc;
@@ -1058,7 +1069,10 @@ __global__ __launch_bounds__(kTileThreads, kUnbiased ? RT_WAVES_TEMPORAL_UB : RT
 template <bool kCount, bool kLearned, bool kUnbiased = false>
 RT_DEV void spatialPixel(const DevScene& S, const rt_camera& cam, const GB& g, const rt_gris_settings& set,
                          const ResvPlanes& tempR, const ResvPlanes& thisR, float* indirectOut, uint32_t x, uint32_t y,
-                         TStack st, TraceCounters* ctr, const rtg::GuideFrame& gf) {
+                         TStack st, TraceCounters* ctr, const rtg::GuideFrame& gf, StreamSlot dstSlot) {
+    // the luminance target with the reference's weights keeps the merged sample in LDS between
+    // the neighbours (the forms with more per-sample state keep it in registers)
+    constexpr bool kDstInLds = !kLearned && !kUnbiased;
     if (x >= g.W || y >= g.H) return;
     size_t p = (size_t)y * g.W + x;
     const rtg::GuidePix gpx = rtg::guidePixel(gf.lobes, g.W, g.H, x, y);
@@ -1080,6 +1094,8 @@ RT_DEV void spatialPixel(const DevScene& S, const rt_camera& cam, const GB& g, c
         uint32_t nbPix[kUnbiased ? 8 : 1];
         float nbM[kUnbiased ? 8 : 1];
         uint32_t nbCount = 0;
+        LdsResv dstL{dstSlot, resv.sampleCount, resv.resampleWeight};
+        if constexpr (kDstInLds) storeStream(dstSlot, resv);
         if (set.spatialReuse) {
             f2 texel = mk2(1.0f / (float)g.W, 1.0f / (float)g.H);
             for (uint32_t i = 0; i < set.spatialNeighbors; i++) {
@@ -1096,8 +1112,12 @@ RT_DEV void spatialPixel(const DevScene& S, const rt_camera& cam, const GB& g, c
                 if (dot(np.norm, gp.norm) < 0.9f || distance(pos, posN) > 0.4f) continue;
                 Resv nres = resvAt<kLearned>(tempR, cam.filmSize[0], cam.filmSize[1], px, py);
                 if (resvIsValid(nres)) {
-                    reuseAndMerge<kCount, kLearned, kUnbiased>(S, st, set, resv, dstPrimarySurf, dstPrimaryIsec, ray.dir,
-                                                                ray.ori, nres, rng, ctr, gpx);
+                    if constexpr (kDstInLds)
+                        reuseAndMerge<kCount, kLearned, kUnbiased>(S, st, set, dstL, dstPrimarySurf, dstPrimaryIsec,
+                                                                    ray.dir, ray.ori, nres, rng, ctr, gpx);
+                    else
+                        reuseAndMerge<kCount, kLearned, kUnbiased>(S, st, set, resv, dstPrimarySurf, dstPrimaryIsec,
+                                                                    ray.dir, ray.ori, nres, rng, ctr, gpx);
                     if (kUnbiased && nbCount < 8u) {
                         nbPix[nbCount] = ((uint32_t)px << 16) | (uint32_t)py;
                         nbM[nbCount] = nres.sampleCount;
@@ -1105,6 +1125,11 @@ RT_DEV void spatialPixel(const DevScene& S, const rt_camera& cam, const GB& g, c
                     }
                 }
             }
+        }
+        if constexpr (kDstInLds) {
+            resv = loadStream(dstSlot);
+            resv.sampleCount = dstL.sampleCount;
+            resv.resampleWeight = dstL.resampleWeight;
         }
         if (kUnbiased) {
             if (nbCount > 0u && resvIsValid(resv) && resv.rcIsec.inst != RT_INVALID_HIT) {
@@ -1156,10 +1181,12 @@ __global__ __launch_bounds__(kTileThreads, kUnbiased ? RT_WAVES_SPATIAL_UB : RT_
                                                                ResvPlanes tempR, ResvPlanes thisR, float* indirectOut,
                                                                rtg::GuideFrame gf, TraceCounters* ctr) {
     __shared__ StackLDS<kTileThreads> stk;
+    __shared__ float dstWords[(!kLearned && !kUnbiased) ? kStreamWords * kTileThreads : 1];
     PIXEL_INDEX
     spatialPixel<kCount, kLearned, kUnbiased>(S, cam, g, set, tempR, thisR, indirectOut, x, y,
                                               laneStack(stk), ctr,
-                                              kGuided ? gf : rtg::GuideFrame{nullptr, nullptr, nullptr});
+                                              kGuided ? gf : rtg::GuideFrame{nullptr, nullptr, nullptr},
+                                              StreamSlot{dstWords + threadIdx.x, (uint32_t)kTileThreads});
 }
 
 // ---- Batched views ---------------------------------------------------------------------------
@@ -1264,12 +1291,14 @@ __global__ __launch_bounds__(kTileThreads, kUnbiased ? RT_WAVES_SPATIAL_UB : RT_
                                                                                       uint32_t numViews, uint32_t totalTiles,
                                                                                       uint32_t swz, TraceCounters* ctr) {
     __shared__ StackLDS<kTileThreads> stk;
+    __shared__ float dstWords[(!kLearned && !kUnbiased) ? kStreamWords * kTileThreads : 1];
     uint32_t v, x, y;
     if (!batchPixel(V, numViews, totalTiles, swz, v, x, y)) return;
     const ViewDesc& D = V[v];
     spatialPixel<kCount, kLearned, kUnbiased>(D.S, D.cam, D.g, D.set, D.temp, D.cur, D.indirectOut, x, y,
                                               laneStack(stk), ctr,
-                                              kGuided ? D.gf : rtg::GuideFrame{nullptr, nullptr, nullptr});
+                                              kGuided ? D.gf : rtg::GuideFrame{nullptr, nullptr, nullptr},
+                                              StreamSlot{dstWords + threadIdx.x, (uint32_t)kTileThreads});
 }
 
 // ---- Wavefront variant of the batched path trace: material-sorted shading ----
