@@ -145,8 +145,9 @@ def overlapped_frames(rt, hs, ds, W, H, s, args):
     """The headline frames again through rt_pipeline: frame k + 1's G-buffer and path trace on one
     stream beside frame k's temporal and spatial reuse on another (the same frames, computed bit for
     bit as one after the other). HIP gives streams hardware queues as they are created, so a short
-    calibration times a few stream pairs and keeps the fastest. Returns the figures and the last
-    frame's indirect output."""
+    calibration times a few stream pairs and keeps the fastest, then times the path trace's
+    persistent launch at a few sizes (fewer workgroups leave CUs to the reuse passes beside it) and
+    keeps the fastest. Returns the figures and the last frame's indirect output."""
     import numpy as np
     import torch
 
@@ -162,17 +163,27 @@ def overlapped_frames(rt, hs, ds, W, H, s, args):
             for k in range(first, first + frames):
                 rt.check(Hh.rt_pipeline_frame(p, hs.camera(k % 16, W, H), hs_[a], hs_[b]), "pipeline frame")
 
-        calib = {}
-        for a, b in ((0, 1), (0, 2), (1, 3), (2, 3)):
+        def timed(a, b):
             rt.check(Hh.rt_pipeline_reset_reservoirs(p, hs_[a]), "reset")
             run(a, b, 0, 3)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             run(a, b, 3, 12)
             torch.cuda.synchronize()
-            calib[f"{a}{b}"] = round((time.perf_counter() - t0) * 1e3 / 12, 3)
+            return round((time.perf_counter() - t0) * 1e3 / 12, 3)
+
+        calib = {f"{a}{b}": timed(a, b) for a, b in ((0, 1), (0, 2), (1, 3), (2, 3))}
         pair = min(calib, key=calib.get)
         a, b = int(pair[0]), int(pair[1])
+        # one-wave workgroups: 16 per CU is all the path trace's occupancy holds
+        resident = torch.cuda.get_device_properties(0).multi_processor_count * 16
+        group_calib = {}
+        for frac in (16, 12, 10, 8, 6, 4):
+            n = resident * frac // 16
+            rt.check(Hh.rt_pipeline_set_trace_groups(p, 0 if frac == 16 else n), "trace groups")
+            group_calib[n] = timed(a, b)
+        groups = min(group_calib, key=group_calib.get)
+        rt.check(Hh.rt_pipeline_set_trace_groups(p, 0 if groups == resident else groups), "trace groups")
         rt.check(Hh.rt_pipeline_reset_reservoirs(p, hs_[a]), "reset")
         run(a, b, 0, args.warmup)
         torch.cuda.synchronize()
@@ -186,7 +197,8 @@ def overlapped_frames(rt, hs, ds, W, H, s, args):
         torch.cuda.synchronize()
         Hh.rt_pipeline_destroy(p)
     return {"ms_per_step": round(wall * 1e3 / args.steps, 3), "value": round(W * H * args.steps / wall / 1e6, 3),
-            "stream_pair_calibration_ms": calib, "streams": pair}, out
+            "stream_pair_calibration_ms": calib, "streams": pair, "trace_workgroups": groups,
+            "trace_workgroups_calibration_ms": group_calib}, out
 
 
 def count_work(rt, hs, ds, W, H, s, warmup, steps):

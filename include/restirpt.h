@@ -267,6 +267,13 @@ rt_status rt_pt_plain(const rt_device_scene* scene, const rt_camera cams[2], con
  * on one scene may then be in flight at once. */
 rt_status rt_gris_path_trace(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
                              const rt_gris_settings* settings, hipStream_t stream);
+/* rt_gris_path_trace with at most `workgroups` one-wave workgroups in the
+ * persistent launch (0: as many as the device holds at once, which is what
+ * rt_gris_path_trace launches). No counterpart in the reference. The result
+ * does not depend on the count; a smaller launch leaves CUs to kernels running
+ * beside it on other streams (rt_pipeline's frame overlap). */
+rt_status rt_gris_path_trace_groups(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
+                                    const rt_gris_settings* settings, uint32_t workgroups, hipStream_t stream);
 rt_status rt_gris_temporal(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,
                            const rt_gris_settings* settings, hipStream_t stream);
 rt_status rt_gris_spatial(const rt_device_scene* scene, const rt_camera cams[2], const rt_frame* frame,

@@ -118,6 +118,10 @@ rt_status rt_renderer_enable_proposal(rt_renderer* r, uint32_t seed, uint32_t lo
 rt_status rt_pipeline_create(rt_device_scene* scene, uint32_t width, uint32_t height, rt_pipeline** out);
 rt_status rt_pipeline_destroy(rt_pipeline* p);
 rt_status rt_pipeline_set_settings(rt_pipeline* p, const rt_gris_settings* settings);
+/* Workgroups of each frame's persistent path trace (rt_gris_path_trace_groups;
+ * 0, the default: all the device holds). Fewer leave CUs to the reuse passes
+ * of the frame before, which run beside it; the results do not change. */
+rt_status rt_pipeline_set_trace_groups(rt_pipeline* p, uint32_t workgroups);
 /* Enqueues one frame; cams[0].filmSize must equal the pipeline's size. The
  * frame's output and final reservoirs are complete once reuseStream has run
  * up to this call. */

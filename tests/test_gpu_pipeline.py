@@ -1,7 +1,8 @@
 """rt_pipeline (GRIS frames overlapped on two streams) against rt_renderer (one frame after the
 other): over a camera walk both must leave the same final reservoirs for the last two frames and
-the same indirect output, bit for bit, with biased and unbiased resampling weights; a reset
-restarts the walk; settings the pipeline does not take are refused."""
+the same indirect output, bit for bit, with biased and unbiased resampling weights, with one stream
+for both and with a capped path-trace launch; a reset restarts the walk; settings the pipeline does
+not take are refused."""
 import ctypes as C
 
 import numpy as np
@@ -80,6 +81,12 @@ def test_pipeline_matches_sequential_frames(unbiased):
             rt.check(Hh.rt_pipeline_reset_reservoirs(p, gpuhelp.stream()), "reset")
         # one stream for both: the frames run one after the other and still agree
         last, prev, out = pipelined(Hh, p, hs, FRAMES, ta, ta)
+        assert not gpuhelp.compare_reservoirs(want_last, last).any()
+        assert gpuhelp.same_float(want_out, out).all()
+        # a smaller path-trace launch (a few workgroups) changes nothing either
+        rt.check(Hh.rt_pipeline_reset_reservoirs(p, gpuhelp.stream()), "reset")
+        rt.check(Hh.rt_pipeline_set_trace_groups(p, 37), "trace groups")
+        last, prev, out = pipelined(Hh, p, hs, FRAMES, ta, tb)
         assert not gpuhelp.compare_reservoirs(want_last, last).any()
         assert gpuhelp.same_float(want_out, out).all()
     finally:

@@ -41,3 +41,26 @@ def test_concurrent_path_traces_never_share_a_queue():
     assert (first["F"] != 0).any()  # the view has lit samples
     for k in range(1, launches):
         assert not gpuhelp.compare_reservoirs(first, run.reservoirs(outs[k])).any(), f"launch {k} differs"
+
+
+def test_path_trace_result_does_not_depend_on_workgroup_count():
+    W, H = 96, 64
+    hs = rt.HostScene(1, 1)
+    ds = rt.DeviceScene(hs.arrays)
+    run = gpuhelp.GrisRun(ds, W, H)
+    cams = hs.camera(0, W, H)
+    run.gbuffer(cams[0])
+    s = rt.settings(shift=2, max_depth=6)
+    L = rt.core()
+    rb = L.rt_gris_reservoir_bytes(W, H)
+    outs = {}
+    for groups in (0, 1, 7, 100, 1 << 20):
+        out = gpuhelp.dev_empty(rb)
+        f = run.frame_struct()
+        f.grisReservoir = out.data_ptr()
+        rt.check(L.rt_gris_path_trace_groups(ds.handle, cams, C.byref(f), C.byref(s), groups, gpuhelp.stream()),
+                 f"path trace, {groups} workgroups")
+        outs[groups] = run.reservoirs(out)
+    assert (outs[0]["F"] != 0).any()
+    for groups, r in outs.items():
+        assert not gpuhelp.compare_reservoirs(outs[0], r).any(), f"{groups} workgroups differ"

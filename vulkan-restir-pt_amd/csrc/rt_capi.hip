@@ -27,7 +27,7 @@ hipError_t launchTraceRays(const rt_device_scene*, const float*, uint32_t, int, 
 hipError_t launchGBuffer(const rt_device_scene*, const rt_camera&, float*, uint32_t*, uint16_t*, hipStream_t);
 hipError_t launchPtPlain(const rt_device_scene*, const rt_camera*, const rt_frame&, uint32_t, uint32_t, hipStream_t);
 hipError_t launchGrisPathTrace(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_gris_settings&,
-                               hipStream_t, rtd::WorkQueue* queue = nullptr);
+                               hipStream_t, rtd::WorkQueue* queue = nullptr, uint32_t maxGroups = 0);
 hipError_t launchGrisTemporal(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_gris_settings&,
                               hipStream_t);
 hipError_t launchGrisSpatial(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_gris_settings&,
@@ -321,16 +321,21 @@ rt_status rt_pt_plain(const rt_device_scene* s, const rt_camera cams[2], const r
     return RT_OK;
 }
 
-rt_status rt_gris_path_trace(const rt_device_scene* s, const rt_camera cams[2], const rt_frame* f,
-                             const rt_gris_settings* set, hipStream_t stream) {
+rt_status rt_gris_path_trace_groups(const rt_device_scene* s, const rt_camera cams[2], const rt_frame* f,
+                                    const rt_gris_settings* set, uint32_t workgroups, hipStream_t stream) {
     rt_status st = checkFrame(s, cams, f);
     if (st == RT_OK) st = checkSettings(set);
     if (st == RT_OK) st = checkGuide(f, set);
     if (st != RT_OK) return st;
     if (!f->grisReservoir) return invalid("grisReservoir is null");
     if (s->numLights == 0) return invalid("the scene has no lights");
-    RT_LAUNCH(rti::launchGrisPathTrace(s, cams, *f, *set, stream));
+    RT_LAUNCH(rti::launchGrisPathTrace(s, cams, *f, *set, stream, nullptr, workgroups));
     return RT_OK;
+}
+
+rt_status rt_gris_path_trace(const rt_device_scene* s, const rt_camera cams[2], const rt_frame* f,
+                             const rt_gris_settings* set, hipStream_t stream) {
+    return rt_gris_path_trace_groups(s, cams, f, set, 0u, stream);
 }
 
 rt_status rt_gris_temporal(const rt_device_scene* s, const rt_camera cams[2], const rt_frame* f,

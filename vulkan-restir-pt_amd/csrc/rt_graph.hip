@@ -13,7 +13,7 @@
 namespace rti {
 hipError_t launchGBuffer(const rt_device_scene*, const rt_camera&, float*, uint32_t*, uint16_t*, hipStream_t);
 hipError_t launchGrisPathTrace(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_gris_settings&,
-                               hipStream_t, rtd::WorkQueue* queue);
+                               hipStream_t, rtd::WorkQueue* queue, uint32_t maxGroups);
 hipError_t launchGrisTemporal(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_gris_settings&,
                               hipStream_t);
 hipError_t launchGrisSpatial(const rt_device_scene*, const rt_camera*, const rt_frame&, const rt_gris_settings&,
@@ -50,7 +50,7 @@ hipError_t recordFrame(rt_frame_graph* g, uint32_t parity, const rt_camera* cams
                                       const_cast<uint32_t*>(f.albedoMatId), const_cast<uint16_t*>(f.motionVector),
                                       nullptr);
     if (e == hipSuccess && guided) e = rti::launchGuideRefresh(g->guide, cams, f, nullptr);
-    if (e == hipSuccess) e = rti::launchGrisPathTrace(g->scene, cams, f, g->settings, nullptr, g->queue);
+    if (e == hipSuccess) e = rti::launchGrisPathTrace(g->scene, cams, f, g->settings, nullptr, g->queue, 0u);
     if (e == hipSuccess && learned) e = rti::launchLearnedRefresh(g->scene, g->mlp, cams, f, nullptr);
     if (e == hipSuccess) e = rti::launchGrisTemporal(g->scene, cams, f, g->settings, nullptr);
     if (e == hipSuccess) e = rti::launchGrisSpatial(g->scene, cams, f, g->settings, nullptr);

@@ -1748,8 +1748,9 @@ rtg::GuideFrame guideFrame(const rt_frame& f, const rt_gris_settings& set) {
 }
 
 // queue: the pixel queue of this launch (null: the scene's next round-robin slot).
+// maxGroups: at most this many workgroups (0: as many as the device holds at once).
 hipError_t launchGrisPathTrace(const rt_device_scene* s, const rt_camera* cams, const rt_frame& f,
-                               const rt_gris_settings& set, hipStream_t stream, WorkQueue* queue) {
+                               const rt_gris_settings& set, hipStream_t stream, WorkQueue* queue, uint32_t maxGroups) {
     GB g = makeGB(f.depthNormal, f.albedoMatId, f.motionVector, f.width, f.height);
     uint32_t grid = 0;
     const bool guided = set.guidedProposal == 1u;
@@ -1759,6 +1760,7 @@ hipError_t launchGrisPathTrace(const rt_device_scene* s, const rt_camera* cams, 
                        : (guided ? persistentGrid(k_gris_path_trace<false, true>, f.width, f.height, grid)
                                  : persistentGrid(k_gris_path_trace<false, false>, f.width, f.height, grid));
     if (e != hipSuccess) return e;
+    if (maxGroups) grid = std::min(grid, maxGroups);
     TraceCounters* ctr = gCountersEnabled ? counterBuffer() : nullptr;
     const ResvPlanes R = planes(f.grisReservoir, f.width, f.height);
     // the scene's next queue slot: wait (on this stream) for the slot's last launch if it may

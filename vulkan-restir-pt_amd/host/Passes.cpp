@@ -308,7 +308,8 @@ void FramePipeline::drawFrame(hipStream_t traceStream, hipStream_t reuseStream, 
     rtThrow(rt_gbuffer_trace(mScene.handle(), &cams[0], const_cast<float*>(f.depthNormal),
                              const_cast<uint32_t*>(f.albedoMatId), const_cast<uint16_t*>(f.motionVector), traceStream),
             "rt_gbuffer_trace");
-    rtThrow(rt_gris_path_trace(mScene.handle(), cams, &f, &settings, traceStream), "rt_gris_path_trace");
+    rtThrow(rt_gris_path_trace_groups(mScene.handle(), cams, &f, &settings, traceGroups, traceStream),
+            "rt_gris_path_trace_groups");
     if (hipEventRecord(mTracedEv[e], traceStream) != hipSuccess ||
         hipStreamWaitEvent(reuseStream, mTracedEv[e], 0) != hipSuccess)
         throw std::runtime_error("hipEventRecord failed");

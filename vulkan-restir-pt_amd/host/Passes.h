@@ -177,6 +177,8 @@ public:
     void resetReservoirs(hipStream_t stream);
 
     GRISReSTIR::Settings settings = {GRISReSTIR::Hybrid, 1.0f, 0u, 1u, 20u, 15u, 3u, 20.0f, 0u, 0u, 0u};
+    // workgroups of the persistent path trace (rt_gris_path_trace_groups; 0: all the device holds)
+    uint32_t traceGroups = 0;
     float* indirectOutput = nullptr;
     uint32_t width, height;
 

@@ -240,6 +240,12 @@ rt_status rt_pipeline_set_settings(rt_pipeline* p, const rt_gris_settings* s) {
     return RT_OK;
 }
 
+rt_status rt_pipeline_set_trace_groups(rt_pipeline* p, uint32_t workgroups) {
+    if (!p) return RT_ERR_INVALID_ARGUMENT;
+    p->p->traceGroups = workgroups;
+    return RT_OK;
+}
+
 rt_status rt_pipeline_frame(rt_pipeline* p, const rt_camera cams[2], hipStream_t traceStream, hipStream_t reuseStream) {
     if (!p || !cams) return RT_ERR_INVALID_ARGUMENT;
     try {

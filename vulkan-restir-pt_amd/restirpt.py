@@ -186,7 +186,8 @@ def mlp_config(lo, hi, log2_table=19, base_res=16, max_res=2048, seed=3, lr=1e-2
 CORE_SYMBOLS = [
     "rt_last_error", "rt_device_scene_create", "rt_device_scene_destroy", "rt_device_scene_info", "rt_bvh_export", "rt_bvh4_export",
     "rt_bvh_build_noalloc",
-    "rt_trace_rays", "rt_gbuffer_trace", "rt_pt_plain", "rt_gris_path_trace", "rt_gris_temporal", "rt_gris_spatial",
+    "rt_trace_rays", "rt_gbuffer_trace", "rt_pt_plain", "rt_gris_path_trace", "rt_gris_path_trace_groups",
+    "rt_gris_temporal", "rt_gris_spatial",
     "rt_gris_render", "rt_gris_reservoir_bytes", "rt_gris_reservoir_reset", "rt_gris_reservoir_download",
     "rt_gris_reservoir_upload", "rt_gris_learned_download", "rt_gris_learned_upload", "rt_counters_enable",
     "rt_counters_read", "rt_mlp_create", "rt_mlp_destroy", "rt_mlp_param_counts", "rt_mlp_get_params",
@@ -207,7 +208,8 @@ HOST_SYMBOLS = [
     "rt_renderer_enable_learned", "rt_renderer_enable_proposal", "rt_renderer_frame_methods",
     "rt_renderer_set_di_settings",
     "rt_renderer_di_bindings", "rt_write_png", "rt_read_png", "rt_renderer_screenshot",
-    "rt_pipeline_create", "rt_pipeline_destroy", "rt_pipeline_set_settings", "rt_pipeline_frame",
+    "rt_pipeline_create", "rt_pipeline_destroy", "rt_pipeline_set_settings", "rt_pipeline_set_trace_groups",
+    "rt_pipeline_frame",
     "rt_pipeline_bindings", "rt_pipeline_read_output", "rt_pipeline_reset_reservoirs",
 ]
 
@@ -229,6 +231,7 @@ def _declare_core(L):
         "rt_gbuffer_trace": [P, C.POINTER(Camera), P, P, P, P],
         "rt_pt_plain": [P, C.POINTER(Camera), C.POINTER(Frame), u32, u32, P],
         "rt_gris_path_trace": [P, C.POINTER(Camera), C.POINTER(Frame), C.POINTER(GrisSettings), P],
+        "rt_gris_path_trace_groups": [P, C.POINTER(Camera), C.POINTER(Frame), C.POINTER(GrisSettings), u32, P],
         "rt_gris_temporal": [P, C.POINTER(Camera), C.POINTER(Frame), C.POINTER(GrisSettings), P],
         "rt_gris_spatial": [P, C.POINTER(Camera), C.POINTER(Frame), C.POINTER(GrisSettings), P],
         "rt_gris_render": [P, C.POINTER(Camera), C.POINTER(Frame), C.POINTER(GrisSettings), P],
@@ -298,6 +301,7 @@ def _declare_host(L):
         "rt_pipeline_create": [P, u32, u32, C.POINTER(P)],
         "rt_pipeline_destroy": [P],
         "rt_pipeline_set_settings": [P, C.POINTER(GrisSettings)],
+        "rt_pipeline_set_trace_groups": [P, u32],
         "rt_pipeline_frame": [P, C.POINTER(Camera), P, P],
         "rt_pipeline_bindings": [P, C.POINTER(Frame)],
         "rt_pipeline_read_output": [P, P, P],
