@@ -133,6 +133,23 @@ def test_argument_checks_fail_loudly():
     assert b"not a radiance network" in L.rt_last_error()
     assert L.rt_gris_render(fake, cams, C.byref(f), C.byref(guided), None) == 1
     assert b"rt_gris_guide_refresh" in L.rt_last_error()
+    # the capped path trace checks its arguments as rt_gris_path_trace does
+    assert L.rt_gris_path_trace_groups(None, cams, C.byref(f), C.byref(s), 64, None) == 1
+    assert b"scene is null" in L.rt_last_error()
+    assert L.rt_gris_path_trace_groups(fake, cams, C.byref(f), C.byref(bad), 64, None) == 1
+    assert b"maxDepth" in L.rt_last_error()
+    # frame pipelines: null handles and arguments are refused before any device work
+    Hh = rt.host()
+    pipe = C.c_void_p()
+    assert Hh.rt_pipeline_create(None, 4, 4, C.byref(pipe)) == 1
+    assert Hh.rt_pipeline_create(fake, 0, 4, C.byref(pipe)) == 1
+    assert Hh.rt_pipeline_destroy(None) == 1
+    assert Hh.rt_pipeline_set_settings(None, C.byref(s)) == 1
+    assert Hh.rt_pipeline_set_trace_groups(None, 64) == 1
+    assert Hh.rt_pipeline_frame(None, cams, None, None) == 1
+    assert Hh.rt_pipeline_bindings(None, C.byref(f)) == 1
+    assert Hh.rt_pipeline_read_output(None, None, None) == 1
+    assert Hh.rt_pipeline_reset_reservoirs(None, None) == 1
 
 
 def test_missing_library_is_an_error(monkeypatch):
