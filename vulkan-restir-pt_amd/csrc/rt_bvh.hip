@@ -10,8 +10,9 @@
 //      (digit, block), then a scatter whose in-block rank comes from 8 wave
 //      ballots (equal-digit lane masks) plus per-wave digit counts in LDS
 //   5. Karras topology, one thread per internal node
-//   6. bottom-up refit: one thread per leaf climbs; the second arrival at a
-//      node (device-scope atomic counter) merges both child boxes
+//   6. bottom-up refit: per block of leaves, one thread per leaf climbs and
+//      the second arrival at a node merges both child boxes; then the nodes
+//      spanning several blocks, level by level from the deepest
 // Every step is integer or exact min/max work, so the result is bit-identical
 // to the CPU restatement of the same algorithm.
 #include "rt_internal.h"
@@ -290,8 +291,7 @@ __global__ void k_scan_apply(const uint32_t* in, uint32_t n, const uint32_t* __r
         }
 }
 
-// 5. Karras topology. Also records each node's leaf range and clears the
-// counters of the refit's second phase.
+// 5. Karras topology. Also records each node's leaf range.
 __device__ __forceinline__ int kdelta(const unsigned long long* c, int n, int i, int j) {
     if (j < 0 || j >= n) return -1;
     unsigned long long a = c[i], b = c[j];
@@ -304,8 +304,7 @@ __device__ __forceinline__ int kdelta(const unsigned long long* c, int n, int i,
 }
 
 __global__ void k_karras(const unsigned long long* __restrict__ codes, int n, float4* __restrict__ nodes,
-                         uint32_t* __restrict__ parentI, uint32_t* __restrict__ parentL, uint2* __restrict__ range,
-                         uint32_t* __restrict__ counters, uint32_t* __restrict__ numStops) {
+                         uint32_t* __restrict__ parentI, uint32_t* __restrict__ parentL, uint2* __restrict__ range) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     int d = (kdelta(codes, n, i, i + 1) - kdelta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -329,8 +328,7 @@ __global__ void k_karras(const unsigned long long* __restrict__ codes, int n, fl
     if (left & RT_LEAF) parentL[left & ~RT_LEAF] = (uint32_t)i; else parentI[left] = (uint32_t)i;
     if (right & RT_LEAF) parentL[right & ~RT_LEAF] = (uint32_t)i; else parentI[right] = (uint32_t)i;
     range[i] = make_uint2((uint32_t)min(i, j), (uint32_t)max(i, j));
-    counters[i] = 0u;
-    if (i == 0) { parentI[0] = kNoParent; *numStops = 0u; }
+    if (i == 0) parentI[0] = kNoParent;
 }
 
 // leaves in Morton order
@@ -369,7 +367,7 @@ __device__ __forceinline__ uint32_t mergeNode(const float* nf, const uint32_t* h
 // arrival at a node (counted in LDS) merges both slots and climbs on. Every
 // arrival is ordered by workgroup-scope fences only, which on this part need no
 // cache maintenance. A climb that reaches a node with leaves outside the
-// workgroup leaves its slot written and records the node for phase 2.
+// workgroup leaves its slot written and stops there (phase 2 takes over).
 constexpr uint32_t kRefitThreads = 1024;
 constexpr uint32_t kRefitLeaves = 4096;
 
@@ -377,8 +375,7 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_block(const float4* __r
                                                                const uint32_t* __restrict__ order, uint32_t n,
                                                                float4* nodes, const uint32_t* __restrict__ parentI,
                                                                const uint32_t* __restrict__ parentL,
-                                                               const uint2* __restrict__ range, uint32_t* height,
-                                                               uint32_t* __restrict__ stops, uint32_t* numStops) {
+                                                               const uint2* __restrict__ range, uint32_t* height) {
     __shared__ uint32_t arrivals[kRefitLeaves];
     const uint32_t L0 = blockIdx.x * kRefitLeaves, L1 = min(n, L0 + kRefitLeaves);
     for (uint32_t k = threadIdx.x; k < kRefitLeaves; k += kRefitThreads) arrivals[k] = 0u;
@@ -395,10 +392,7 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_block(const float4* __r
             float* nf = reinterpret_cast<float*>(nodes + 4 * (size_t)p);
             writeSlot(nf, __float_as_uint(nf[12]) != ref, lo, hi);
             const uint2 r = range[p];
-            if (r.x < L0 || r.y >= L1) {
-                stops[atomicAdd(numStops, 1u)] = p;
-                break;
-            }
+            if (r.x < L0 || r.y >= L1) break;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             const uint32_t old = atomicAdd(&arrivals[p - L0], 1u);
             if (old == 0u) break;
@@ -411,30 +405,59 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_block(const float4* __r
     }
 }
 
-// Refit, phase 2: the nodes whose leaves span several phase-1 workgroups (a
-// few per workgroup boundary) in one workgroup, so that the same
-// workgroup-scope ordering suffices; each recorded stop is one arrival.
+// Refit, phase 2: the nodes whose leaves span several phase-1 workgroups (the
+// top part, a few per workgroup boundary), in one workgroup. They are listed
+// level by level from the root (a child belongs to the top part when its leaf
+// range spans blocks), then merged from the deepest level up: a node's slots
+// then hold both children's boxes (phase 1 wrote the block subtrees' roots,
+// the level below the top part's children), and its union goes into its
+// parent's slot. Workgroup barriers order the levels. The binary tree is at
+// most 98 levels high (one Karras split per common-prefix length of the 64-bit
+// codes and 32-bit indices), so kMaxTopLevels bounds the level list.
+constexpr uint32_t kMaxTopLevels = 128;
+
 __global__ __launch_bounds__(kRefitThreads) void k_refit_top(float4* nodes, const uint32_t* __restrict__ parentI,
-                                                             uint32_t* counters, uint32_t* height,
-                                                             const uint32_t* __restrict__ stops,
-                                                             const uint32_t* __restrict__ numStops) {
-    const uint32_t ns = *numStops;
-    for (uint32_t e = threadIdx.x; e < ns; e += blockDim.x) {
-        uint32_t p = stops[e];
-        while (true) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            const uint32_t old = __hip_atomic_fetch_add(&counters[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (old == 0u) break;
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            const float* nf = reinterpret_cast<const float*>(nodes + 4 * (size_t)p);
+                                                             const uint2* __restrict__ range, uint32_t n,
+                                                             uint32_t* height, uint32_t* list) {
+    __shared__ uint32_t levelEnd[kMaxTopLevels];
+    __shared__ uint32_t tail, levels;
+    if (n <= kRefitLeaves) return;  // one block: phase 1 merged every node up to the root
+    if (threadIdx.x == 0) {
+        list[0] = 0u;  // the root
+        tail = 1u;
+        levels = 0u;
+    }
+    __syncthreads();
+    uint32_t a = 0, b = 1;
+    while (a < b) {
+        if (threadIdx.x == 0) levelEnd[min(levels++, kMaxTopLevels - 1u)] = b;
+        for (uint32_t i = a + threadIdx.x; i < b; i += blockDim.x) {
+            const float* nf = reinterpret_cast<const float*>(nodes + 4 * (size_t)list[i]);
+            for (int k = 0; k < 2; k++) {
+                const uint32_t c = __float_as_uint(nf[12 + k]);
+                if (c & RT_LEAF) continue;
+                const uint2 r = range[c];
+                if (r.x / kRefitLeaves != r.y / kRefitLeaves) list[atomicAdd(&tail, 1u)] = c;
+            }
+        }
+        __syncthreads();
+        a = b;
+        b = tail;
+        __syncthreads();
+    }
+    const int top = (int)min(levels, kMaxTopLevels);
+    for (int L = top - 1; L >= 0; L--) {
+        const uint32_t s0 = L ? levelEnd[L - 1] : 0u, s1 = levelEnd[L];
+        for (uint32_t i = s0 + threadIdx.x; i < s1; i += blockDim.x) {
+            const uint32_t p = list[i];
             float lo[3], hi[3];
-            height[p] = mergeNode(nf, height, lo, hi);
+            height[p] = mergeNode(reinterpret_cast<const float*>(nodes + 4 * (size_t)p), height, lo, hi);
             const uint32_t q = parentI[p];
-            if (q == kNoParent) break;
+            if (q == kNoParent) continue;
             float* qf = reinterpret_cast<float*>(nodes + 4 * (size_t)q);
             writeSlot(qf, __float_as_uint(qf[12]) != p, lo, hi);
-            p = q;
         }
+        __syncthreads();
     }
 }
 
@@ -691,10 +714,8 @@ size_t carveLbvhScratch(uint32_t n, char* base, LbvhScratch* out) {
     s.parentI = reinterpret_cast<uint32_t*>(take(4 * nNodes));
     s.parentL = reinterpret_cast<uint32_t*>(take(4 * (size_t)n));
     s.range = reinterpret_cast<uint2*>(take(8 * nNodes));
-    s.counters = reinterpret_cast<uint32_t*>(take(4 * nNodes));
     s.height = reinterpret_cast<uint32_t*>(take(4 * nNodes));
-    s.stops = reinterpret_cast<uint32_t*>(take(8 * nNodes));
-    s.numStops = reinterpret_cast<uint32_t*>(take(4));
+    s.topList = reinterpret_cast<uint32_t*>(take(4 * nNodes));
     if (out) *out = s;
     return off;
 }
@@ -729,8 +750,7 @@ hipError_t launchBinaryBuild(const float4* primTris, uint32_t n, unsigned long l
         if (leafTris) k_gather_leaves<<<1, kBlock, 0, stream>>>(primTris, order, n, leafTris);
         return hipGetLastError();
     }
-    k_karras<<<cdiv(n - 1, kBlock), kBlock, 0, stream>>>(codes, (int)n, nodes, sc.parentI, sc.parentL, sc.range,
-                                                         sc.counters, sc.numStops);
+    k_karras<<<cdiv(n - 1, kBlock), kBlock, 0, stream>>>(codes, (int)n, nodes, sc.parentI, sc.parentL, sc.range);
     const float4* tris = primTris;
     const uint32_t* refitOrder = order;
     if (leafTris) {
@@ -739,9 +759,8 @@ hipError_t launchBinaryBuild(const float4* primTris, uint32_t n, unsigned long l
         refitOrder = nullptr;
     }
     k_refit_block<<<cdiv(n, kRefitLeaves), kRefitThreads, 0, stream>>>(tris, refitOrder, n, nodes, sc.parentI,
-                                                                       sc.parentL, sc.range, sc.height, sc.stops,
-                                                                       sc.numStops);
-    k_refit_top<<<1, kRefitThreads, 0, stream>>>(nodes, sc.parentI, sc.counters, sc.height, sc.stops, sc.numStops);
+                                                                       sc.parentL, sc.range, sc.height);
+    k_refit_top<<<1, kRefitThreads, 0, stream>>>(nodes, sc.parentI, sc.range, n, sc.height, sc.topList);
     return hipGetLastError();
 }
 

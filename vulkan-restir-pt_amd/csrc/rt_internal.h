@@ -93,10 +93,8 @@ struct LbvhScratch {
     uint32_t* parentI = nullptr;     // internal nodes
     uint32_t* parentL = nullptr;     // leaves
     uint2* range = nullptr;          // internal nodes
-    uint32_t* counters = nullptr;    // internal nodes
     uint32_t* height = nullptr;      // internal nodes
-    uint32_t* stops = nullptr;       // 2 x internal nodes
-    uint32_t* numStops = nullptr;    // 1
+    uint32_t* topList = nullptr;     // internal nodes
 };
 
 // Bytes of the scratch of an n-primitive binary build; with base non-null also
