@@ -192,7 +192,7 @@ def overlapped_frames(rt, hs, ds, W, H, s, args):
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         out = np.zeros((H, W, 4), np.float32)
-        rt.check(Hh.rt_pipeline_read_output(p, out.ctypes.data, hs_[b]), "read output")
+        rt.check(Hh.rt_pipeline_read_output(p, 1, out.ctypes.data, hs_[b]), "read output")
     finally:
         torch.cuda.synchronize()
         Hh.rt_pipeline_destroy(p)
@@ -1105,17 +1105,22 @@ def progress(msg):
 
 
 def kernel_sources_digest():
-    """SHA-256 over the kernel sources (csrc/*.hip, csrc/*.h, include/*.h, sorted by name): the
-    build a traffic measurement belongs to."""
+    """SHA-256 over the sources the kernels are built from (csrc/*.hip, csrc/*.h and
+    include/restirpt.h, sorted by name): the build a traffic measurement belongs to. The host
+    layer's header and sources are not part of it."""
     import hashlib
 
     h = hashlib.sha256()
-    for d in (os.path.join(ROOT, "vulkan-restir-pt_amd", "csrc"), os.path.join(ROOT, "include")):
-        for name in sorted(os.listdir(d)):
-            if name.endswith((".hip", ".h")):
-                h.update(name.encode())
-                with open(os.path.join(d, name), "rb") as f:
-                    h.update(f.read())
+    files = []
+    csrc = os.path.join(ROOT, "vulkan-restir-pt_amd", "csrc")
+    for name in sorted(os.listdir(csrc)):
+        if name.endswith((".hip", ".h")):
+            files.append((name, os.path.join(csrc, name)))
+    files.append(("restirpt.h", os.path.join(ROOT, "include", "restirpt.h")))
+    for name, path in files:
+        h.update(name.encode())
+        with open(path, "rb") as f:
+            h.update(f.read())
     return h.hexdigest()
 
 

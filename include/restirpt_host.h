@@ -131,9 +131,15 @@ rt_status rt_pipeline_frame(rt_pipeline* p, const rt_camera cams[2], hipStream_t
  * reservoirs (grisReservoir), the frame before's (grisReservoirPrev), the
  * indirect output. */
 rt_status rt_pipeline_bindings(const rt_pipeline* p, rt_frame* out);
-/* Copies the RGBA32F indirect output to host memory on stream (pass the
- * reuse stream: the copy follows the last frame's spatial reuse). */
-rt_status rt_pipeline_read_output(const rt_pipeline* p, float* host, hipStream_t stream);
+/* The direct light of each frame (Renderer::Settings directMethod): 0 None
+ * (the default), 1 Naive (rt_pt_plain's direct part, depth settings.maxDepth,
+ * on traceStream after the G-buffer, as rt_renderer_frame_methods(r, cams, 1,
+ * 3) computes it). ReSTIR DI (2) is refused: it runs in rt_renderer. */
+rt_status rt_pipeline_set_direct_method(rt_pipeline* p, uint32_t directMethod);
+/* Copies an RGBA32F output (0 direct, 1 indirect) to host memory on stream:
+ * pass the trace stream for the direct output and the reuse stream for the
+ * indirect one (the copy then follows the last frame's pass that wrote it). */
+rt_status rt_pipeline_read_output(const rt_pipeline* p, uint32_t which, float* host, hipStream_t stream);
 /* Waits for the device, resets every reservoir buffer on stream, waits for
  * the reset, and restarts the frame count. */
 rt_status rt_pipeline_reset_reservoirs(rt_pipeline* p, hipStream_t stream);

@@ -1,8 +1,8 @@
 """rt_pipeline (GRIS frames overlapped on two streams) against rt_renderer (one frame after the
 other): over a camera walk both must leave the same final reservoirs for the last two frames and
 the same indirect output, bit for bit, with biased and unbiased resampling weights, with one stream
-for both and with a capped path-trace launch; a reset restarts the walk; settings the pipeline does
-not take are refused."""
+for both, with a capped path-trace launch and with the naive direct light (and the same direct
+output); a reset restarts the walk; settings the pipeline does not take are refused."""
 import ctypes as C
 
 import numpy as np
@@ -28,20 +28,24 @@ def download(ptr):
     return out
 
 
-def sequential(hs, ds, s, frames):
+def sequential(hs, ds, s, frames, direct=0):
+    """rt_renderer frames (direct method `direct`, ReSTIR PT indirect): the last frame's and the
+    frame before's final reservoirs, the indirect and the direct output."""
     Hh = rt.host()
     r = C.c_void_p()
     rt.check(Hh.rt_renderer_create(ds.handle, W, H, C.byref(r)), "renderer")
     try:
         rt.check(Hh.rt_renderer_set_settings(r, C.byref(s)), "settings")
         for k in range(frames):
-            rt.check(Hh.rt_renderer_frame(r, hs.camera(k, W, H), 0, gpuhelp.stream()), "frame")
+            rt.check(Hh.rt_renderer_frame_methods(r, hs.camera(k, W, H), direct, 3, gpuhelp.stream()), "frame")
         f = rt.Frame()
         rt.check(Hh.rt_renderer_bindings(r, C.byref(f)), "bindings")
         out = np.zeros((H, W, 4), np.float32)
         rt.check(Hh.rt_renderer_read_output(r, 1, out.ctypes.data, gpuhelp.stream()), "read")
+        dout = np.zeros((H, W, 4), np.float32)
+        rt.check(Hh.rt_renderer_read_output(r, 0, dout.ctypes.data, gpuhelp.stream()), "read")
         # after the flip: grisReservoirPrev holds the last frame's reservoirs
-        return download(f.grisReservoirPrev), download(f.grisReservoir), out
+        return download(f.grisReservoirPrev), download(f.grisReservoir), out, dout
     finally:
         Hh.rt_renderer_destroy(r)
 
@@ -52,7 +56,7 @@ def pipelined(Hh, p, hs, frames, ta, tb):
     f = rt.Frame()
     rt.check(Hh.rt_pipeline_bindings(p, C.byref(f)), "bindings")
     out = np.zeros((H, W, 4), np.float32)
-    rt.check(Hh.rt_pipeline_read_output(p, out.ctypes.data, tb), "read")
+    rt.check(Hh.rt_pipeline_read_output(p, 1, out.ctypes.data, tb), "read")
     return download(f.grisReservoir), download(f.grisReservoirPrev), out
 
 
@@ -63,7 +67,7 @@ def test_pipeline_matches_sequential_frames(unbiased):
     s = rt.settings(shift=2, temporal=True, spatial=True, max_depth=6, cap=20, neighbors=3, radius=20.0,
                     unbiased=unbiased)
     print(f"[gpu] pipeline vs sequential, unbiased={unbiased}", flush=True)
-    want_last, want_prev, want_out = sequential(hs, ds, s, FRAMES)
+    want_last, want_prev, want_out, _ = sequential(hs, ds, s, FRAMES)
     assert (want_last["F"] != 0).any()
     Hh = rt.host()
     p = C.c_void_p()
@@ -94,6 +98,33 @@ def test_pipeline_matches_sequential_frames(unbiased):
         Hh.rt_pipeline_destroy(p)
 
 
+def test_pipeline_naive_direct_light_matches_sequential_frames():
+    hs = rt.HostScene(1, 1)
+    ds = rt.DeviceScene(hs.arrays)
+    s = rt.settings(shift=2, temporal=True, spatial=True, max_depth=6, cap=20, neighbors=3, radius=20.0)
+    print("[gpu] pipeline with naive direct light vs sequential", flush=True)
+    want_last, _, want_out, want_direct = sequential(hs, ds, s, FRAMES, direct=1)
+    assert (want_direct[..., :3] > 0).any()
+    Hh = rt.host()
+    p = C.c_void_p()
+    rt.check(Hh.rt_pipeline_create(ds.handle, W, H, C.byref(p)), "pipeline")
+    try:
+        rt.check(Hh.rt_pipeline_set_settings(p, C.byref(s)), "settings")
+        rt.check(Hh.rt_pipeline_set_direct_method(p, 1), "direct method")
+        streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+        ta, tb = (C.c_void_p(x.cuda_stream) for x in streams)
+        torch.cuda.synchronize()
+        last, _, out = pipelined(Hh, p, hs, FRAMES, ta, tb)
+        direct = np.zeros((H, W, 4), np.float32)
+        rt.check(Hh.rt_pipeline_read_output(p, 0, direct.ctypes.data, ta), "read direct")
+        assert not gpuhelp.compare_reservoirs(want_last, last).any()
+        assert gpuhelp.same_float(want_out, out).all()
+        assert gpuhelp.same_float(want_direct, direct).all()
+    finally:
+        torch.cuda.synchronize()
+        Hh.rt_pipeline_destroy(p)
+
+
 def test_pipeline_refuses_learned_forms_and_wrong_film_size():
     hs = rt.HostScene(1, 1)
     ds = rt.DeviceScene(hs.arrays)
@@ -108,6 +139,7 @@ def test_pipeline_refuses_learned_forms_and_wrong_film_size():
         s = rt.settings(shift=2, temporal=True, spatial=True, max_depth=4)
         rt.check(Hh.rt_pipeline_set_settings(p, C.byref(s)), "settings")
         assert Hh.rt_pipeline_frame(p, hs.camera(0, W + 8, H), st, st) == INVALID_ARGUMENT
+        assert Hh.rt_pipeline_set_direct_method(p, 2) == INVALID_ARGUMENT  # ReSTIR DI: rt_renderer
         rt.check(Hh.rt_pipeline_frame(p, hs.camera(0, W, H), st, st), "frame")
     finally:
         torch.cuda.synchronize()

@@ -245,6 +245,7 @@ FramePipeline::FramePipeline(const DeviceScene& scene, uint32_t w, uint32_t h) :
         mFinal[j] = deviceAlloc<uint8_t>(rb);
     }
     mTemp = deviceAlloc<uint8_t>(rb);
+    directOutput = deviceAlloc<float>(n * 16);
     indirectOutput = deviceAlloc<float>(n * 16);
     for (int j = 0; j < 3; j++) {
         if (hipEventCreateWithFlags(&mTracedEv[j], hipEventDisableTiming) != hipSuccess ||
@@ -264,7 +265,7 @@ FramePipeline::~FramePipeline() {
     for (int j = 0; j < 2; j++) {
         (void)hipFree(mMotion[j]); (void)hipFree(mTraced[j]); (void)hipFree(mFinal[j]);
     }
-    (void)hipFree(mTemp); (void)hipFree(indirectOutput);
+    (void)hipFree(mTemp); (void)hipFree(directOutput); (void)hipFree(indirectOutput);
 }
 void FramePipeline::resetReservoirs(hipStream_t stream) {
     // frames still in flight on the caller's two streams read and write these buffers
@@ -284,6 +285,7 @@ rt_frame FramePipeline::frameOf(uint64_t k) const {
     rt_frame f{};
     const uint32_t g = (uint32_t)(k % 3u), gp = (uint32_t)((k + 2u) % 3u), p = (uint32_t)(k & 1u);
     f.width = width; f.height = height;
+    f.directOutput = directOutput;
     f.indirectOutput = indirectOutput;
     f.depthNormal = mDepthNormal[g]; f.depthNormalPrev = mDepthNormal[gp];
     f.albedoMatId = mAlbedoMatId[g]; f.albedoMatIdPrev = mAlbedoMatId[gp];
@@ -298,6 +300,8 @@ void FramePipeline::drawFrame(hipStream_t traceStream, hipStream_t reuseStream, 
         throw std::invalid_argument("FramePipeline: the luminance target and BSDF sampling only");
     if (cams[0].filmSize[0] != width || cams[0].filmSize[1] != height)
         throw std::invalid_argument("FramePipeline: camera film size differs from the pipeline's");
+    if (directMethod > Renderer::MethodNaive)
+        throw std::invalid_argument("FramePipeline: direct light None or Naive (ReSTIR DI runs in Renderer)");
     const uint64_t k = mFrame;
     const uint32_t e = (uint32_t)(k % 3u);
     // frame k overwrites what frame k - 2's temporal reuse read (the G-buffer three frames
@@ -308,6 +312,9 @@ void FramePipeline::drawFrame(hipStream_t traceStream, hipStream_t reuseStream, 
     rtThrow(rt_gbuffer_trace(mScene.handle(), &cams[0], const_cast<float*>(f.depthNormal),
                              const_cast<uint32_t*>(f.albedoMatId), const_cast<uint16_t*>(f.motionVector), traceStream),
             "rt_gbuffer_trace");
+    // Renderer's naive direct light (rt_pt_plain, direct part, its maxDepthNaive = settings.maxDepth)
+    if (directMethod == Renderer::MethodNaive)
+        rtThrow(rt_pt_plain(mScene.handle(), cams, &f, settings.maxDepth, 1u, traceStream), "rt_pt_plain");
     rtThrow(rt_gris_path_trace_groups(mScene.handle(), cams, &f, &settings, traceGroups, traceStream),
             "rt_gris_path_trace_groups");
     if (hipEventRecord(mTracedEv[e], traceStream) != hipSuccess ||

@@ -159,9 +159,12 @@ private:
 // reservoirs and the final reservoirs apart (two each), one Temp. Frame k:
 //   trace stream:  wait for frame k - 2's temporal reuse; G-buffer; path trace
 //   reuse stream:  wait for frame k's path trace; temporal reuse; spatial reuse
-// Every frame computes exactly what Renderer::drawFrame computes for it. The
-// luminance target only (the learned forms' networks are read by one frame's
-// refresh while the previous frame trains them); either resampling weights.
+// Every frame computes exactly what Renderer::drawFrame computes for it (with
+// directMethod = MethodNaive, the naive direct light of the same frame runs on
+// the trace stream after the G-buffer, as drawFrame(direct, ResampledPT) does
+// before the GRIS passes). The luminance target only (the learned forms'
+// networks are read by one frame's refresh while the previous frame trains
+// them); either resampling weights.
 class FramePipeline {
 public:
     FramePipeline(const DeviceScene& scene, uint32_t width, uint32_t height);
@@ -179,6 +182,9 @@ public:
     GRISReSTIR::Settings settings = {GRISReSTIR::Hybrid, 1.0f, 0u, 1u, 20u, 15u, 3u, 20.0f, 0u, 0u, 0u};
     // workgroups of the persistent path trace (rt_gris_path_trace_groups; 0: all the device holds)
     uint32_t traceGroups = 0;
+    // Renderer::RayTracingMethod of the direct light: MethodNone or MethodNaive
+    uint32_t directMethod = 0;
+    float* directOutput = nullptr;
     float* indirectOutput = nullptr;
     uint32_t width, height;
 

@@ -264,10 +264,19 @@ rt_status rt_pipeline_bindings(const rt_pipeline* p, rt_frame* out) {
     return RT_OK;
 }
 
-rt_status rt_pipeline_read_output(const rt_pipeline* p, float* host, hipStream_t stream) {
-    if (!p || !host) return RT_ERR_INVALID_ARGUMENT;
+rt_status rt_pipeline_set_direct_method(rt_pipeline* p, uint32_t directMethod) {
+    if (!p) return RT_ERR_INVALID_ARGUMENT;
+    if (directMethod > Renderer::MethodNaive)
+        return fail(RT_ERR_INVALID_ARGUMENT, "rt_pipeline: direct light 0 (None) or 1 (Naive); ReSTIR DI runs in rt_renderer");
+    p->p->directMethod = directMethod;
+    return RT_OK;
+}
+
+rt_status rt_pipeline_read_output(const rt_pipeline* p, uint32_t which, float* host, hipStream_t stream) {
+    if (!p || !host || which > 1) return RT_ERR_INVALID_ARGUMENT;
     const size_t bytes = (size_t)p->p->width * p->p->height * 16;
-    if (hipMemcpyAsync(host, p->p->indirectOutput, bytes, hipMemcpyDeviceToHost, stream) != hipSuccess) return RT_ERR_HIP;
+    const float* src = which == 0 ? p->p->directOutput : p->p->indirectOutput;
+    if (hipMemcpyAsync(host, src, bytes, hipMemcpyDeviceToHost, stream) != hipSuccess) return RT_ERR_HIP;
     if (hipStreamSynchronize(stream) != hipSuccess) return RT_ERR_HIP;
     return RT_OK;
 }
