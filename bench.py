@@ -1048,6 +1048,72 @@ def restir_di(rt, hs, ds, args, kernel_loop=False):
     return res
 
 
+def full_image(rt, hs, ds, args):
+    """The whole image as the reference composes it with both resampled methods: ReSTIR DI for the
+    direct light (the reference's defaults) and ReSTIR PT for the indirect light, on the headline's
+    view, settings and camera walk. Frames one after the other (rt_renderer_frame_methods(2, 3)) and
+    overlapped on two streams (rt_pipeline with the direct method 2); the last frame's direct and
+    indirect outputs must agree bit for bit."""
+    import numpy as np
+    import torch
+
+    W, H = args.width, args.height
+    Hh = rt.host()
+    s = settings(rt, W)
+    di = rt.di_settings()
+    frames = args.warmup + args.steps
+    outs = {}
+
+    def read(fn, handle, which, stream):
+        o = np.zeros((H, W, 4), np.float32)
+        rt.check(fn(handle, which, o.ctypes.data, stream), "read output")
+        return o
+
+    r = C.c_void_p()
+    rt.check(Hh.rt_renderer_create(ds.handle, W, H, C.byref(r)), "rt_renderer_create")
+    try:
+        rt.check(Hh.rt_renderer_set_settings(r, C.byref(s)), "settings")
+        rt.check(Hh.rt_renderer_set_di_settings(r, C.byref(di)), "di settings")
+        st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        for k in range(args.warmup):
+            rt.check(Hh.rt_renderer_frame_methods(r, hs.camera(k % 16, W, H), 2, 3, st), "frame")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.warmup, frames):
+            rt.check(Hh.rt_renderer_frame_methods(r, hs.camera(k % 16, W, H), 2, 3, st), "frame")
+        torch.cuda.synchronize()
+        seq_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        outs["seq"] = (read(Hh.rt_renderer_read_output, r, 0, st), read(Hh.rt_renderer_read_output, r, 1, st))
+    finally:
+        Hh.rt_renderer_destroy(r)
+    p = C.c_void_p()
+    rt.check(Hh.rt_pipeline_create(ds.handle, W, H, C.byref(p)), "rt_pipeline_create")
+    try:
+        rt.check(Hh.rt_pipeline_set_settings(p, C.byref(s)), "settings")
+        rt.check(Hh.rt_pipeline_set_di_settings(p, C.byref(di)), "di settings")
+        rt.check(Hh.rt_pipeline_set_direct_method(p, 2), "direct method")
+        streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+        ta, tb = (C.c_void_p(x.cuda_stream) for x in streams)
+        for k in range(args.warmup):
+            rt.check(Hh.rt_pipeline_frame(p, hs.camera(k % 16, W, H), ta, tb), "frame")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.warmup, frames):
+            rt.check(Hh.rt_pipeline_frame(p, hs.camera(k % 16, W, H), ta, tb), "frame")
+        torch.cuda.synchronize()
+        ov_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        outs["ov"] = (read(Hh.rt_pipeline_read_output, p, 0, tb), read(Hh.rt_pipeline_read_output, p, 1, tb))
+    finally:
+        torch.cuda.synchronize()
+        Hh.rt_pipeline_destroy(p)
+    same = all(((a == b) | (np.isnan(a) & np.isnan(b))).all() for a, b in zip(outs["seq"], outs["ov"]))
+    return {"ms_per_frame_sequential": round(seq_ms, 3), "ms_per_frame_overlapped": round(ov_ms, 3),
+            "mpixels_per_s_overlapped": round(W * H / ov_ms / 1e3, 2), "identical_to_sequential": bool(same),
+            "workload": f"ReSTIR DI direct (reference defaults) + ReSTIR PT indirect (headline settings), "
+                        f"{W}x{H}, terrain camera walk; overlapped: rt_pipeline on two streams, full path-trace "
+                        f"launch"}
+
+
 def restir_gi(rt, hs, ds, args):
     """ReSTIR GI temporal reuse (the reference's indirect method 2: one pass per frame tracing up to
     15 bounces from the primary vertex, merging into the reprojected previous reservoir, M-cap 40)
@@ -1258,6 +1324,8 @@ def main():
         progress("restir_di done")
         line["restir_gi"] = restir_gi(rt, hs, ds, args)
         progress("restir_gi done")
+        line["full_image"] = full_image(rt, hs, ds, args)
+        progress("full_image done")
     if not args.no_batched:
         line["batched"] = batched_views(rt, args, hs, ds)
         progress("batched done")

@@ -131,11 +131,20 @@ rt_status rt_pipeline_frame(rt_pipeline* p, const rt_camera cams[2], hipStream_t
  * reservoirs (grisReservoir), the frame before's (grisReservoirPrev), the
  * indirect output. */
 rt_status rt_pipeline_bindings(const rt_pipeline* p, rt_frame* out);
-/* The direct light of each frame (Renderer::Settings directMethod): 0 None
- * (the default), 1 Naive (rt_pt_plain's direct part, depth settings.maxDepth,
- * on traceStream after the G-buffer, as rt_renderer_frame_methods(r, cams, 1,
- * 3) computes it). ReSTIR DI (2) is refused: it runs in rt_renderer. */
+/* The direct light of each frame (Renderer::Settings directMethod), as
+ * rt_renderer_frame_methods(r, cams, directMethod, 3) computes it: 0 None (the
+ * default); 1 Naive (rt_pt_plain's direct part, depth settings.maxDepth, on
+ * traceStream after the G-buffer); 2 ReSTIR DI with rt_pipeline_set_di_settings
+ * (path generation on traceStream, temporal and spatial reuse on reuseStream
+ * beside the path trace; the first call with 2 allocates five DI reservoir
+ * buffers and waits for the device). */
 rt_status rt_pipeline_set_direct_method(rt_pipeline* p, uint32_t directMethod);
+/* TestReSTIR::Settings of the pipeline's ReSTIR DI (default {0, 0, 0, 1}). */
+rt_status rt_pipeline_set_di_settings(rt_pipeline* p, const rt_di_settings* settings);
+/* ReSTIR DI's final reservoirs after the last enqueued frame (reservoir), the
+ * frame before's (reservoirPrev) and Temp, as rt_renderer_di_bindings gives
+ * them before the flip. */
+rt_status rt_pipeline_di_bindings(const rt_pipeline* p, rt_di_reservoirs* out);
 /* Copies an RGBA32F output (0 direct, 1 indirect) to host memory on stream:
  * pass the trace stream for the direct output and the reuse stream for the
  * indirect one (the copy then follows the last frame's pass that wrote it). */

@@ -150,6 +150,8 @@ def test_argument_checks_fail_loudly():
     assert Hh.rt_pipeline_bindings(None, C.byref(f)) == 1
     assert Hh.rt_pipeline_read_output(None, 1, None, None) == 1
     assert Hh.rt_pipeline_set_direct_method(None, 1) == 1
+    assert Hh.rt_pipeline_set_di_settings(None, C.byref(rt.di_settings())) == 1
+    assert Hh.rt_pipeline_di_bindings(None, C.byref(rt.DIReservoirs())) == 1
     assert Hh.rt_pipeline_reset_reservoirs(None, None) == 1
 
 

@@ -249,7 +249,8 @@ FramePipeline::FramePipeline(const DeviceScene& scene, uint32_t w, uint32_t h) :
     indirectOutput = deviceAlloc<float>(n * 16);
     for (int j = 0; j < 3; j++) {
         if (hipEventCreateWithFlags(&mTracedEv[j], hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&mTemporalEv[j], hipEventDisableTiming) != hipSuccess)
+            hipEventCreateWithFlags(&mTemporalEv[j], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&mDiGenEv[j], hipEventDisableTiming) != hipSuccess)
             throw std::runtime_error("hipEventCreate failed");
     }
     resetReservoirs(0);
@@ -261,10 +262,13 @@ FramePipeline::~FramePipeline() {
         (void)hipFree(mDepthNormal[j]); (void)hipFree(mAlbedoMatId[j]);
         if (mTracedEv[j]) (void)hipEventDestroy(mTracedEv[j]);
         if (mTemporalEv[j]) (void)hipEventDestroy(mTemporalEv[j]);
+        if (mDiGenEv[j]) (void)hipEventDestroy(mDiGenEv[j]);
     }
     for (int j = 0; j < 2; j++) {
         (void)hipFree(mMotion[j]); (void)hipFree(mTraced[j]); (void)hipFree(mFinal[j]);
+        (void)hipFree(mDiTraced[j]); (void)hipFree(mDiFinal[j]);
     }
+    (void)hipFree(mDiTemp);
     (void)hipFree(mTemp); (void)hipFree(directOutput); (void)hipFree(indirectOutput);
 }
 void FramePipeline::resetReservoirs(hipStream_t stream) {
@@ -275,6 +279,13 @@ void FramePipeline::resetReservoirs(hipStream_t stream) {
         rtThrow(rt_gris_reservoir_reset(mFinal[j], width, height, stream), "reset");
     }
     rtThrow(rt_gris_reservoir_reset(mTemp, width, height, stream), "reset");
+    if (mDiTemp) {
+        for (int j = 0; j < 2; j++) {
+            rtThrow(rt_di_reservoir_reset(mDiTraced[j], width, height, stream), "reset");
+            rtThrow(rt_di_reservoir_reset(mDiFinal[j], width, height, stream), "reset");
+        }
+        rtThrow(rt_di_reservoir_reset(mDiTemp, width, height, stream), "reset");
+    }
     // the next frames run on other streams
     if (hipStreamSynchronize(stream) != hipSuccess) throw std::runtime_error("hipStreamSynchronize failed");
     mFrame = 0;
@@ -295,13 +306,38 @@ rt_frame FramePipeline::frameOf(uint64_t k) const {
     f.grisReservoirTemp = mTemp;
     return f;
 }
+void FramePipeline::setDirectMethod(uint32_t method) {
+    if (method > Renderer::MethodResampled)
+        throw std::invalid_argument("FramePipeline: direct light None, Naive or ReSTIR DI");
+    if (method == Renderer::MethodResampled && !mDiTemp) {
+        if (hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("hipDeviceSynchronize failed");
+        const size_t db = rt_di_reservoir_bytes(width, height);
+        for (int j = 0; j < 2; j++) {
+            mDiTraced[j] = deviceAlloc<uint8_t>(db);
+            mDiFinal[j] = deviceAlloc<uint8_t>(db);
+            rtThrow(rt_di_reservoir_reset(mDiTraced[j], width, height, 0), "rt_di_reservoir_reset");
+            rtThrow(rt_di_reservoir_reset(mDiFinal[j], width, height, 0), "rt_di_reservoir_reset");
+        }
+        mDiTemp = deviceAlloc<uint8_t>(db);
+        rtThrow(rt_di_reservoir_reset(mDiTemp, width, height, 0), "rt_di_reservoir_reset");
+        if (hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("hipDeviceSynchronize failed");
+    }
+    mDirect = method;
+}
+rt_di_reservoirs FramePipeline::diBindings() const {
+    const uint64_t k = mFrame ? mFrame - 1u : 0u;
+    rt_di_reservoirs r{};
+    r.reservoir = mDiFinal[k & 1u];
+    r.reservoirPrev = mDiFinal[(k & 1u) ^ 1u];
+    r.reservoirTemp = mDiTemp;
+    return r;
+}
 void FramePipeline::drawFrame(hipStream_t traceStream, hipStream_t reuseStream, const rt_camera cams[2]) {
     if (settings.targetFunction != 0u || settings.guidedProposal != 0u)
         throw std::invalid_argument("FramePipeline: the luminance target and BSDF sampling only");
     if (cams[0].filmSize[0] != width || cams[0].filmSize[1] != height)
         throw std::invalid_argument("FramePipeline: camera film size differs from the pipeline's");
-    if (directMethod > Renderer::MethodNaive)
-        throw std::invalid_argument("FramePipeline: direct light None or Naive (ReSTIR DI runs in Renderer)");
+
     const uint64_t k = mFrame;
     const uint32_t e = (uint32_t)(k % 3u);
     // frame k overwrites what frame k - 2's temporal reuse read (the G-buffer three frames
@@ -312,9 +348,25 @@ void FramePipeline::drawFrame(hipStream_t traceStream, hipStream_t reuseStream, 
     rtThrow(rt_gbuffer_trace(mScene.handle(), &cams[0], const_cast<float*>(f.depthNormal),
                              const_cast<uint32_t*>(f.albedoMatId), const_cast<uint16_t*>(f.motionVector), traceStream),
             "rt_gbuffer_trace");
-    // Renderer's naive direct light (rt_pt_plain, direct part, its maxDepthNaive = settings.maxDepth)
-    if (directMethod == Renderer::MethodNaive)
+    // the direct light: Renderer's naive pass (rt_pt_plain, direct part, its maxDepthNaive =
+    // settings.maxDepth), or ReSTIR DI's path generation, whose reuse the reuse stream starts
+    // as soon as it is done (frame k - 2's DI temporal reuse, the last reader of these
+    // reservoirs, ran before frame k - 2's GRIS temporal reuse)
+    rt_di_reservoirs di{};
+    if (mDirect == Renderer::MethodNaive)
         rtThrow(rt_pt_plain(mScene.handle(), cams, &f, settings.maxDepth, 1u, traceStream), "rt_pt_plain");
+    if (mDirect == Renderer::MethodResampled) {
+        di.reservoir = mDiTraced[k & 1u];
+        di.reservoirPrev = mDiFinal[(k & 1u) ^ 1u];
+        di.reservoirTemp = mDiTemp;
+        rtThrow(rt_di_path_gen(mScene.handle(), cams, &f, &di, &diSettings, traceStream), "rt_di_path_gen");
+        if (hipEventRecord(mDiGenEv[e], traceStream) != hipSuccess ||
+            hipStreamWaitEvent(reuseStream, mDiGenEv[e], 0) != hipSuccess)
+            throw std::runtime_error("hipEventRecord failed");
+        rtThrow(rt_di_temporal(mScene.handle(), cams, &f, &di, &diSettings, reuseStream), "rt_di_temporal");
+        di.reservoir = mDiFinal[k & 1u];  // spatial reuse reads Temp and writes the final reservoirs
+        rtThrow(rt_di_spatial(mScene.handle(), cams, &f, &di, &diSettings, reuseStream), "rt_di_spatial");
+    }
     rtThrow(rt_gris_path_trace_groups(mScene.handle(), cams, &f, &settings, traceGroups, traceStream),
             "rt_gris_path_trace_groups");
     if (hipEventRecord(mTracedEv[e], traceStream) != hipSuccess ||

@@ -159,12 +159,14 @@ private:
 // reservoirs and the final reservoirs apart (two each), one Temp. Frame k:
 //   trace stream:  wait for frame k - 2's temporal reuse; G-buffer; path trace
 //   reuse stream:  wait for frame k's path trace; temporal reuse; spatial reuse
-// Every frame computes exactly what Renderer::drawFrame computes for it (with
-// directMethod = MethodNaive, the naive direct light of the same frame runs on
-// the trace stream after the G-buffer, as drawFrame(direct, ResampledPT) does
-// before the GRIS passes). The luminance target only (the learned forms'
-// networks are read by one frame's refresh while the previous frame trains
-// them); either resampling weights.
+// Every frame computes exactly what Renderer::drawFrame(directMethod,
+// ResampledPT) computes for it. The direct light: MethodNaive runs on the trace
+// stream after the G-buffer; MethodResampled (ReSTIR DI) runs its path
+// generation there and its temporal and spatial reuse on the reuse stream as
+// soon as the generation is done (beside the path trace), with its reservoirs
+// buffered like the GRIS ones (allocated when the method is chosen). The
+// luminance target only (the learned forms' networks are read by one frame's
+// refresh while the previous frame trains them); either resampling weights.
 class FramePipeline {
 public:
     FramePipeline(const DeviceScene& scene, uint32_t width, uint32_t height);
@@ -182,8 +184,14 @@ public:
     GRISReSTIR::Settings settings = {GRISReSTIR::Hybrid, 1.0f, 0u, 1u, 20u, 15u, 3u, 20.0f, 0u, 0u, 0u};
     // workgroups of the persistent path trace (rt_gris_path_trace_groups; 0: all the device holds)
     uint32_t traceGroups = 0;
-    // Renderer::RayTracingMethod of the direct light: MethodNone or MethodNaive
-    uint32_t directMethod = 0;
+    // Renderer::RayTracingMethod of the direct light: MethodNone, MethodNaive or MethodResampled
+    // (allocates the ReSTIR DI reservoirs the first time)
+    void setDirectMethod(uint32_t method);
+    uint32_t directMethod() const { return mDirect; }
+    // ReSTIR DI's final reservoirs after the last enqueued frame (reservoir) and the frame before
+    // (reservoirPrev), as Renderer::diBindings gives them before the flip
+    rt_di_reservoirs diBindings() const;
+    TestReSTIR::Settings diSettings = {TestReSTIR::Reconnection, TestReSTIR::Light, 0u, 1u};
     float* directOutput = nullptr;
     float* indirectOutput = nullptr;
     uint32_t width, height;
@@ -199,5 +207,10 @@ private:
     void* mTemp = nullptr;
     hipEvent_t mTracedEv[3] = {nullptr, nullptr, nullptr};
     hipEvent_t mTemporalEv[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t mDiGenEv[3] = {nullptr, nullptr, nullptr};
+    void* mDiTraced[2] = {nullptr, nullptr};  // ReSTIR DI: path generation's reservoirs
+    void* mDiFinal[2] = {nullptr, nullptr};   // after spatial reuse
+    void* mDiTemp = nullptr;
+    uint32_t mDirect = 0;
     uint64_t mFrame = 0;
 };

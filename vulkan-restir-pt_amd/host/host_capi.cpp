@@ -266,9 +266,25 @@ rt_status rt_pipeline_bindings(const rt_pipeline* p, rt_frame* out) {
 
 rt_status rt_pipeline_set_direct_method(rt_pipeline* p, uint32_t directMethod) {
     if (!p) return RT_ERR_INVALID_ARGUMENT;
-    if (directMethod > Renderer::MethodNaive)
-        return fail(RT_ERR_INVALID_ARGUMENT, "rt_pipeline: direct light 0 (None) or 1 (Naive); ReSTIR DI runs in rt_renderer");
-    p->p->directMethod = directMethod;
+    if (directMethod > Renderer::MethodResampled)
+        return fail(RT_ERR_INVALID_ARGUMENT, "rt_pipeline: direct light 0 (None), 1 (Naive) or 2 (ReSTIR DI)");
+    try {
+        p->p->setDirectMethod(directMethod);
+        return RT_OK;
+    } catch (const std::exception& e) {
+        return fail(RT_ERR_HIP, e.what());
+    }
+}
+
+rt_status rt_pipeline_set_di_settings(rt_pipeline* p, const rt_di_settings* s) {
+    if (!p || !s || s->shiftType > 2 || s->sampleType > 2) return RT_ERR_INVALID_ARGUMENT;
+    p->p->diSettings = *s;
+    return RT_OK;
+}
+
+rt_status rt_pipeline_di_bindings(const rt_pipeline* p, rt_di_reservoirs* out) {
+    if (!p || !out) return RT_ERR_INVALID_ARGUMENT;
+    *out = p->p->diBindings();
     return RT_OK;
 }
 
