@@ -1092,8 +1092,28 @@ def full_image(rt, hs, ds, args):
         rt.check(Hh.rt_pipeline_set_settings(p, C.byref(s)), "settings")
         rt.check(Hh.rt_pipeline_set_di_settings(p, C.byref(di)), "di settings")
         rt.check(Hh.rt_pipeline_set_direct_method(p, 2), "direct method")
-        streams = [torch.cuda.Stream(), torch.cuda.Stream()]
-        ta, tb = (C.c_void_p(x.cuda_stream) for x in streams)
+        streams = [torch.cuda.Stream() for _ in range(3)]
+        ta, tb, tc = (C.c_void_p(x.cuda_stream) for x in streams)
+        # ReSTIR DI's reuse on a third stream, beside the path trace and the GRIS reuse
+        rt.check(Hh.rt_pipeline_set_direct_stream(p, tc), "direct stream")
+        # the path trace's launch size, timed over a few frames at each cap as for the headline
+        resident = torch.cuda.get_device_properties(0).multi_processor_count * 16
+        cap_ms = {}
+        for frac in (16, 12, 10, 8):
+            n = resident * frac // 16
+            rt.check(Hh.rt_pipeline_set_trace_groups(p, 0 if frac == 16 else n), "trace groups")
+            rt.check(Hh.rt_pipeline_reset_reservoirs(p, ta), "reset")
+            for k in range(3):
+                rt.check(Hh.rt_pipeline_frame(p, hs.camera(k % 16, W, H), ta, tb), "frame")
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for k in range(3, 11):
+                rt.check(Hh.rt_pipeline_frame(p, hs.camera(k % 16, W, H), ta, tb), "frame")
+            torch.cuda.synchronize()
+            cap_ms[n] = round((time.perf_counter() - t0) * 1e3 / 8, 3)
+        groups = min(cap_ms, key=cap_ms.get)
+        rt.check(Hh.rt_pipeline_set_trace_groups(p, 0 if groups == resident else groups), "trace groups")
+        rt.check(Hh.rt_pipeline_reset_reservoirs(p, ta), "reset")
         for k in range(args.warmup):
             rt.check(Hh.rt_pipeline_frame(p, hs.camera(k % 16, W, H), ta, tb), "frame")
         torch.cuda.synchronize()
@@ -1102,16 +1122,17 @@ def full_image(rt, hs, ds, args):
             rt.check(Hh.rt_pipeline_frame(p, hs.camera(k % 16, W, H), ta, tb), "frame")
         torch.cuda.synchronize()
         ov_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-        outs["ov"] = (read(Hh.rt_pipeline_read_output, p, 0, tb), read(Hh.rt_pipeline_read_output, p, 1, tb))
+        outs["ov"] = (read(Hh.rt_pipeline_read_output, p, 0, tc), read(Hh.rt_pipeline_read_output, p, 1, tb))
     finally:
         torch.cuda.synchronize()
         Hh.rt_pipeline_destroy(p)
     same = all(((a == b) | (np.isnan(a) & np.isnan(b))).all() for a, b in zip(outs["seq"], outs["ov"]))
     return {"ms_per_frame_sequential": round(seq_ms, 3), "ms_per_frame_overlapped": round(ov_ms, 3),
             "mpixels_per_s_overlapped": round(W * H / ov_ms / 1e3, 2), "identical_to_sequential": bool(same),
+            "trace_workgroups": groups, "trace_workgroups_calibration_ms": cap_ms,
             "workload": f"ReSTIR DI direct (reference defaults) + ReSTIR PT indirect (headline settings), "
-                        f"{W}x{H}, terrain camera walk; overlapped: rt_pipeline on two streams, full path-trace "
-                        f"launch"}
+                        f"{W}x{H}, terrain camera walk; overlapped: rt_pipeline on three streams (trace, "
+                        f"GRIS reuse, DI reuse)"}
 
 
 def restir_gi(rt, hs, ds, args):

@@ -139,6 +139,11 @@ rt_status rt_pipeline_bindings(const rt_pipeline* p, rt_frame* out);
  * beside the path trace; the first call with 2 allocates five DI reservoir
  * buffers and waits for the device). */
 rt_status rt_pipeline_set_direct_method(rt_pipeline* p, uint32_t directMethod);
+/* The stream of ReSTIR DI's temporal and spatial reuse (null, the default: the
+ * reuse stream). A third stream lets them run beside both the path trace and
+ * the GRIS reuse; the ReSTIR DI output is then complete once this stream has
+ * run up to the frame. The stream is used from the next rt_pipeline_frame on. */
+rt_status rt_pipeline_set_direct_stream(rt_pipeline* p, hipStream_t stream);
 /* TestReSTIR::Settings of the pipeline's ReSTIR DI (default {0, 0, 0, 1}). */
 rt_status rt_pipeline_set_di_settings(rt_pipeline* p, const rt_di_settings* settings);
 /* ReSTIR DI's final reservoirs after the last enqueued frame (reservoir), the

@@ -152,6 +152,7 @@ def test_argument_checks_fail_loudly():
     assert Hh.rt_pipeline_set_direct_method(None, 1) == 1
     assert Hh.rt_pipeline_set_di_settings(None, C.byref(rt.di_settings())) == 1
     assert Hh.rt_pipeline_di_bindings(None, C.byref(rt.DIReservoirs())) == 1
+    assert Hh.rt_pipeline_set_direct_stream(None, None) == 1
     assert Hh.rt_pipeline_reset_reservoirs(None, None) == 1
 
 

@@ -114,8 +114,9 @@ def test_pipeline_matches_sequential_frames(unbiased):
         Hh.rt_pipeline_destroy(p)
 
 
-@pytest.mark.parametrize("direct", [1, 2], ids=["naive", "restir_di"])
-def test_pipeline_direct_light_matches_sequential_frames(direct):
+@pytest.mark.parametrize("direct,third", [(1, False), (2, False), (2, True)],
+                         ids=["naive", "restir_di", "restir_di_own_stream"])
+def test_pipeline_direct_light_matches_sequential_frames(direct, third):
     hs = rt.HostScene(1, 1)
     ds = rt.DeviceScene(hs.arrays)
     s = rt.settings(shift=2, temporal=True, spatial=True, max_depth=6, cap=20, neighbors=3, radius=20.0)
@@ -130,13 +131,16 @@ def test_pipeline_direct_light_matches_sequential_frames(direct):
         rt.check(Hh.rt_pipeline_set_settings(p, C.byref(s)), "settings")
         rt.check(Hh.rt_pipeline_set_di_settings(p, C.byref(di)), "di settings")
         rt.check(Hh.rt_pipeline_set_direct_method(p, direct), "direct method")
-        streams = [torch.cuda.Stream(), torch.cuda.Stream()]
-        ta, tb = (C.c_void_p(x.cuda_stream) for x in streams)
+        streams = [torch.cuda.Stream(), torch.cuda.Stream(), torch.cuda.Stream()]
+        ta, tb, tc = (C.c_void_p(x.cuda_stream) for x in streams)
+        if third:
+            rt.check(Hh.rt_pipeline_set_direct_stream(p, tc), "direct stream")
         torch.cuda.synchronize()
         last, _, out = pipelined(Hh, p, hs, FRAMES, ta, tb)
         direct_out = np.zeros((H, W, 4), np.float32)
-        # the naive direct light is written on the trace stream, ReSTIR DI's on the reuse stream
-        rt.check(Hh.rt_pipeline_read_output(p, 0, direct_out.ctypes.data, ta if direct == 1 else tb), "read direct")
+        # the naive direct light is written on the trace stream, ReSTIR DI's on its own stream or the reuse one
+        rt.check(Hh.rt_pipeline_read_output(p, 0, direct_out.ctypes.data, ta if direct == 1 else (tc if third else tb)),
+                 "read direct")
         assert not gpuhelp.compare_reservoirs(want_last, last).any()
         assert gpuhelp.same_float(want_out, out).all()
         assert gpuhelp.same_float(want_direct, direct_out).all()

@@ -250,7 +250,8 @@ FramePipeline::FramePipeline(const DeviceScene& scene, uint32_t w, uint32_t h) :
     for (int j = 0; j < 3; j++) {
         if (hipEventCreateWithFlags(&mTracedEv[j], hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&mTemporalEv[j], hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&mDiGenEv[j], hipEventDisableTiming) != hipSuccess)
+            hipEventCreateWithFlags(&mDiGenEv[j], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&mDiDoneEv[j], hipEventDisableTiming) != hipSuccess)
             throw std::runtime_error("hipEventCreate failed");
     }
     resetReservoirs(0);
@@ -263,6 +264,7 @@ FramePipeline::~FramePipeline() {
         if (mTracedEv[j]) (void)hipEventDestroy(mTracedEv[j]);
         if (mTemporalEv[j]) (void)hipEventDestroy(mTemporalEv[j]);
         if (mDiGenEv[j]) (void)hipEventDestroy(mDiGenEv[j]);
+        if (mDiDoneEv[j]) (void)hipEventDestroy(mDiDoneEv[j]);
     }
     for (int j = 0; j < 2; j++) {
         (void)hipFree(mMotion[j]); (void)hipFree(mTraced[j]); (void)hipFree(mFinal[j]);
@@ -341,17 +343,19 @@ void FramePipeline::drawFrame(hipStream_t traceStream, hipStream_t reuseStream, 
     const uint64_t k = mFrame;
     const uint32_t e = (uint32_t)(k % 3u);
     // frame k overwrites what frame k - 2's temporal reuse read (the G-buffer three frames
-    // back as its previous one, the path-trace reservoirs and motion vectors of its parity)
-    if (k >= 2 && hipStreamWaitEvent(traceStream, mTemporalEv[(k - 2) % 3u], 0) != hipSuccess)
+    // back as its previous one, the path-trace reservoirs and motion vectors of its parity),
+    // the DI reuse of frame k - 2 included when it runs on a stream of its own
+    if (k >= 2 && (hipStreamWaitEvent(traceStream, mTemporalEv[(k - 2) % 3u], 0) != hipSuccess ||
+                   hipStreamWaitEvent(traceStream, mDiDoneEv[(k - 2) % 3u], 0) != hipSuccess))
         throw std::runtime_error("hipStreamWaitEvent failed");
+    hipStream_t diStream = directStream ? directStream : reuseStream;
     rt_frame f = frameOf(k);
     rtThrow(rt_gbuffer_trace(mScene.handle(), &cams[0], const_cast<float*>(f.depthNormal),
                              const_cast<uint32_t*>(f.albedoMatId), const_cast<uint16_t*>(f.motionVector), traceStream),
             "rt_gbuffer_trace");
     // the direct light: Renderer's naive pass (rt_pt_plain, direct part, its maxDepthNaive =
-    // settings.maxDepth), or ReSTIR DI's path generation, whose reuse the reuse stream starts
-    // as soon as it is done (frame k - 2's DI temporal reuse, the last reader of these
-    // reservoirs, ran before frame k - 2's GRIS temporal reuse)
+    // settings.maxDepth), or ReSTIR DI's path generation, whose reuse the DI stream (the
+    // direct stream, else the reuse stream) starts as soon as it is done
     rt_di_reservoirs di{};
     if (mDirect == Renderer::MethodNaive)
         rtThrow(rt_pt_plain(mScene.handle(), cams, &f, settings.maxDepth, 1u, traceStream), "rt_pt_plain");
@@ -360,12 +364,16 @@ void FramePipeline::drawFrame(hipStream_t traceStream, hipStream_t reuseStream, 
         di.reservoirPrev = mDiFinal[(k & 1u) ^ 1u];
         di.reservoirTemp = mDiTemp;
         rtThrow(rt_di_path_gen(mScene.handle(), cams, &f, &di, &diSettings, traceStream), "rt_di_path_gen");
+        // after this frame's generation and the frame before's DI reuse (which may have run on
+        // another stream if the direct stream changed)
         if (hipEventRecord(mDiGenEv[e], traceStream) != hipSuccess ||
-            hipStreamWaitEvent(reuseStream, mDiGenEv[e], 0) != hipSuccess)
+            hipStreamWaitEvent(diStream, mDiGenEv[e], 0) != hipSuccess ||
+            (k >= 1 && hipStreamWaitEvent(diStream, mDiDoneEv[(k - 1) % 3u], 0) != hipSuccess))
             throw std::runtime_error("hipEventRecord failed");
-        rtThrow(rt_di_temporal(mScene.handle(), cams, &f, &di, &diSettings, reuseStream), "rt_di_temporal");
+        rtThrow(rt_di_temporal(mScene.handle(), cams, &f, &di, &diSettings, diStream), "rt_di_temporal");
         di.reservoir = mDiFinal[k & 1u];  // spatial reuse reads Temp and writes the final reservoirs
-        rtThrow(rt_di_spatial(mScene.handle(), cams, &f, &di, &diSettings, reuseStream), "rt_di_spatial");
+        rtThrow(rt_di_spatial(mScene.handle(), cams, &f, &di, &diSettings, diStream), "rt_di_spatial");
+        if (hipEventRecord(mDiDoneEv[e], diStream) != hipSuccess) throw std::runtime_error("hipEventRecord failed");
     }
     rtThrow(rt_gris_path_trace_groups(mScene.handle(), cams, &f, &settings, traceGroups, traceStream),
             "rt_gris_path_trace_groups");

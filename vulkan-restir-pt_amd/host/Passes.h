@@ -192,6 +192,8 @@ public:
     // (reservoirPrev), as Renderer::diBindings gives them before the flip
     rt_di_reservoirs diBindings() const;
     TestReSTIR::Settings diSettings = {TestReSTIR::Reconnection, TestReSTIR::Light, 0u, 1u};
+    // the stream of ReSTIR DI's temporal and spatial reuse (null: the reuse stream)
+    hipStream_t directStream = nullptr;
     float* directOutput = nullptr;
     float* indirectOutput = nullptr;
     uint32_t width, height;
@@ -208,6 +210,7 @@ private:
     hipEvent_t mTracedEv[3] = {nullptr, nullptr, nullptr};
     hipEvent_t mTemporalEv[3] = {nullptr, nullptr, nullptr};
     hipEvent_t mDiGenEv[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t mDiDoneEv[3] = {nullptr, nullptr, nullptr};
     void* mDiTraced[2] = {nullptr, nullptr};  // ReSTIR DI: path generation's reservoirs
     void* mDiFinal[2] = {nullptr, nullptr};   // after spatial reuse
     void* mDiTemp = nullptr;

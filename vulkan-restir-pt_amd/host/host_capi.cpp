@@ -282,6 +282,12 @@ rt_status rt_pipeline_set_di_settings(rt_pipeline* p, const rt_di_settings* s) {
     return RT_OK;
 }
 
+rt_status rt_pipeline_set_direct_stream(rt_pipeline* p, hipStream_t stream) {
+    if (!p) return RT_ERR_INVALID_ARGUMENT;
+    p->p->directStream = stream;
+    return RT_OK;
+}
+
 rt_status rt_pipeline_di_bindings(const rt_pipeline* p, rt_di_reservoirs* out) {
     if (!p || !out) return RT_ERR_INVALID_ARGUMENT;
     *out = p->p->diBindings();

@@ -210,6 +210,7 @@ HOST_SYMBOLS = [
     "rt_renderer_di_bindings", "rt_write_png", "rt_read_png", "rt_renderer_screenshot",
     "rt_pipeline_create", "rt_pipeline_destroy", "rt_pipeline_set_settings", "rt_pipeline_set_trace_groups",
     "rt_pipeline_set_direct_method", "rt_pipeline_set_di_settings", "rt_pipeline_di_bindings",
+    "rt_pipeline_set_direct_stream",
     "rt_pipeline_frame",
     "rt_pipeline_bindings", "rt_pipeline_read_output", "rt_pipeline_reset_reservoirs",
 ]
@@ -308,6 +309,7 @@ def _declare_host(L):
         "rt_pipeline_set_direct_method": [P, u32],
         "rt_pipeline_set_di_settings": [P, C.POINTER(DISettings)],
         "rt_pipeline_di_bindings": [P, C.POINTER(DIReservoirs)],
+        "rt_pipeline_set_direct_stream": [P, P],
         "rt_pipeline_read_output": [P, u32, P, P],
         "rt_pipeline_reset_reservoirs": [P, P],
         "rt_renderer_create": [P, u32, u32, C.POINTER(P)],
