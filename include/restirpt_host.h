@@ -137,7 +137,7 @@ rt_status rt_pipeline_bindings(const rt_pipeline* p, rt_frame* out);
  * traceStream after the G-buffer); 2 ReSTIR DI with rt_pipeline_set_di_settings
  * (path generation on traceStream, temporal and spatial reuse on reuseStream
  * beside the path trace; the first call with 2 allocates five DI reservoir
- * buffers and waits for the device). */
+ * buffers). A change of method waits for the device first. */
 rt_status rt_pipeline_set_direct_method(rt_pipeline* p, uint32_t directMethod);
 /* The stream of ReSTIR DI's temporal and spatial reuse (null, the default: the
  * reuse stream). A third stream lets them run beside both the path trace and

@@ -274,7 +274,7 @@ FramePipeline::~FramePipeline() {
     (void)hipFree(mTemp); (void)hipFree(directOutput); (void)hipFree(indirectOutput);
 }
 void FramePipeline::resetReservoirs(hipStream_t stream) {
-    // frames still in flight on the caller's two streams read and write these buffers
+    // frames still in flight on the caller's streams read and write these buffers
     if (mFrame && hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("hipDeviceSynchronize failed");
     for (int j = 0; j < 2; j++) {
         rtThrow(rt_gris_reservoir_reset(mTraced[j], width, height, stream), "reset");
@@ -311,8 +311,11 @@ rt_frame FramePipeline::frameOf(uint64_t k) const {
 void FramePipeline::setDirectMethod(uint32_t method) {
     if (method > Renderer::MethodResampled)
         throw std::invalid_argument("FramePipeline: direct light None, Naive or ReSTIR DI");
+    if (method == mDirect) return;
+    // the two methods write the direct output from different streams: let the frames in
+    // flight finish before the next one switches
+    if (hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("hipDeviceSynchronize failed");
     if (method == Renderer::MethodResampled && !mDiTemp) {
-        if (hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("hipDeviceSynchronize failed");
         const size_t db = rt_di_reservoir_bytes(width, height);
         for (int j = 0; j < 2; j++) {
             mDiTraced[j] = deviceAlloc<uint8_t>(db);
