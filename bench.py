@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-counters", action="store_true")
     ap.add_argument("--only-kernel-loop", action="store_true", help="time frames only (for profilers)")
+    ap.add_argument("--only-overlapped-loop", action="store_true",
+                    help="time the headline frames overlapped on two streams only (for profilers)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1-GiB device copy (calibrates HBM byte counters)")
     ap.add_argument("--no-learned", action="store_true", help="skip the equal-time relMSE comparison")
@@ -1272,6 +1274,11 @@ def main():
         return
     if args.learned_study:
         print(json.dumps(learned_study(rt, hs, ds, args)))
+        return
+    if args.only_overlapped_loop:
+        ov, _ = overlapped_frames(rt, hs, ds, W, H, s, args)
+        print(json.dumps({"metric": METRIC, "value": ov["value"], "unit": "Mpaths/s", "ms_per_step": ov["ms_per_step"],
+                          "frames_overlapped": ov}))
         return
     loop = FrameLoop(rt, hs, ds, W, H, s)
     for k in range(args.warmup):
